@@ -1,0 +1,138 @@
+/*
+ * rheocell.h -- C ABI of the MI355X (gfx950) RheoCell per-step hot path.
+ *
+ * The library owns device buffers behind an opaque handle.  Host callers
+ * (the C++ FD_HIPBackend in rheocell_amd/host, or ctypes) keep their own
+ * BaseField-like host copies and refresh them through the sync calls.
+ *
+ * Reference interfaces replaced (files of deppinto/RheoCell):
+ *   rc_create            SimBackend::init + WetPolarModel::init/set_box
+ *                        (src/Backends/SimBackend.cpp:171-218,
+ *                         src/Interactions/WetPolarModel.cpp:49-57,95-106)
+ *   rc_upload_cells      SimBackend::read_next_configuration per-cell state
+ *                        (src/Backends/SimBackend.cpp:268-299) and
+ *                        SimBackend::apply_changes_to_simulation_data
+ *                        (src/Backends/SimBackend.h:148)
+ *   rc_init_forces       FD_CPUBackend::init force evaluation
+ *                        (src/Backends/FD_CPUBackend.cpp:19-25)
+ *   rc_apply_changes_after_equilibration
+ *                        WetPolarModel::apply_changes_after_equilibration
+ *                        (src/Interactions/WetPolarModel.cpp:84-93)
+ *   rc_step              SimBackend::sim_step = FD_CPUBackend::sim_step
+ *                        (src/Backends/SimBackend.h:133,
+ *                         src/Backends/FD_CPUBackend.cpp:134-160)
+ *   rc_download_cells    SimBackend::apply_simulation_data_changes
+ *                        (src/Backends/SimBackend.h:138) for Configuration
+ *                        (src/Observables/Configuration.cpp:46-55)
+ *   rc_get_rows          WetPolarModel::get_velocity_x/y and the force rows
+ *                        (src/Interactions/WetPolarModel.h:101-106)
+ *   rc_get_site_array    BaseField public site vectors (src/Fields/BaseField.h:188-195)
+ *   rc_cg_stats          ConjugateGradient::iterations()/error() of the wet solve
+ *                        (src/Interactions/WetPolarModel.cpp:220-233)
+ *   rc_set_warm_start    warm-start rule of the wet solve
+ *                        (src/Interactions/WetPolarModel.cpp:152-157,223-232)
+ *   rc_last_error        RCexception message (src/Utilities/RCexception.h)
+ *   rc_destroy           SimBackend::~SimBackend (src/Backends/SimBackend.cpp:37-40)
+ *
+ * Every call returns RC_OK (0) or an RC_ERR_* status; rc_last_error() gives
+ * the message.  A handle is not re-entrant: one host thread at a time.
+ */
+#ifndef RHEOCELL_H_
+#define RHEOCELL_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RC_OK 0
+#define RC_ERR_ARG 1          /* invalid argument / unsupported option */
+#define RC_ERR_HIP 2          /* HIP runtime error */
+#define RC_ERR_OVERLAP 3      /* > 20 patches cover one box site (WetPolarModel.cpp:250-255) */
+#define RC_ERR_GROWTH 4       /* a patch must grow (MultiPhaseField.cpp:248-310) */
+#define RC_ERR_CG 5           /* CG hit its iteration cap */
+#define RC_ERR_STATE 6        /* call out of order (e.g. step before upload) */
+#define RC_ERR_TILE 7         /* too many patches intersect one box tile */
+
+/* Model parameters of one box, as read from the input file: the *_active
+ * values of WetPolarModel::get_settings (WetPolarModel.cpp:28-46) plus the
+ * backend keys J0 (FD_CPUBackend.cpp:15) and dt (FDBackend.cpp:34). */
+typedef struct rc_model_params {
+	double gamma, lambda, omega, mu, kappa;
+	int R;
+	int anchoring;
+	double zetaQ_self, zetaQ_inter, J_Q;
+	double friction, friction_cell, CGtolerance, wall_slip, passive_alpha, beta;
+	double J0, dt;
+	int cg_max_iter;      /* <= 0: Eigen's default of 2 * rows */
+} rc_model_params;
+
+/* One periodic square box (SquareBox, src/Boxes/SquareBox.cpp:21-29) with
+ * its own cells and parameters.  Several boxes = independent replicas
+ * advanced together. */
+typedef struct rc_box_desc {
+	int L;                /* box side */
+	int ncells;
+	rc_model_params params;
+} rc_box_desc;
+
+/* Per-cell scalar state (public members of BaseField, src/Fields/BaseField.h:122-225). */
+typedef struct rc_cell {
+	int LsubX, LsubY;
+	int offset[2];
+	int corner, corner_old;          /* sub_corner_bottom_left(_old) */
+	int x_sub_left, y_sub_bottom;
+	double CoM[2], CoM_old[2];
+	double unrap[2];                 /* unrap_sub_corner_bottom_left_x/y */
+	double Q00, Q01, thetaQ, thetaQ_old;
+	double area, area_old, sumF, sumF_old, perimeter, S00, S01;
+	double total_force_repulsion[2];
+} rc_cell;
+
+typedef struct rc_cg_stats_t {
+	long long solves;                /* one per component */
+	long long iters_total;           /* SpMV iterations summed over solves */
+	int last_iters[2];               /* x, y of the last solve */
+	double last_relres[2];
+	int last_cold;
+	double last_solve_ms;            /* device time of the last x+y solve */
+	double total_solve_ms;
+} rc_cg_stats_t;
+
+#define RC_WARM_REFERENCE 0   /* previous solution, cold restart every 11th solve (reference rule) */
+#define RC_WARM_PREVIOUS 1    /* previous solution, never a periodic cold restart */
+
+typedef struct rc_handle rc_handle;
+
+int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out);
+/* cells: sum(ncells) entries, box-major; phi: the patches concatenated in
+ * cell order, storage order inside a patch (rows = field_start_index order). */
+int rc_upload_cells(rc_handle *h, const rc_cell *cells, const double *phi);
+int rc_init_forces(rc_handle *h);
+int rc_apply_changes_after_equilibration(rc_handle *h);
+/* gauss: nsteps * total_cells standard normal draws, step-major, cell order
+ * (the Utils::gaussian() calls of the predictor, WetPolarModel.cpp:365).
+ * NULL means zero noise. */
+int rc_step(rc_handle *h, int nsteps, const double *gauss);
+int rc_download_cells(rc_handle *h, rc_cell *cells, double *phi);
+/* which: 0 vx, 1 vy, 2 fx, 3 fy -- rows in field_start_index order */
+int rc_get_rows(rc_handle *h, int which, double *out);
+/* which: 0 phi, 1 freeEnergy, 2 fieldDX, 3 fieldDY, 4 laplacianPhi -- all cells concatenated */
+int rc_get_site_array(rc_handle *h, int which, double *out);
+int rc_total_rows(rc_handle *h, long long *rows);
+int rc_cg_stats(rc_handle *h, rc_cg_stats_t *out);
+int rc_set_warm_start(rc_handle *h, int mode);
+/* Device time (ms) of the stencil/field kernels over the last rc_step call,
+ * measured with HIP events on the library's stream; kind: 0 box gather,
+ * 1 per-cell stage kernels, 2 CG kernels, 3 whole rc_step. launches gets the
+ * number of launches of that kind. */
+int rc_timing(rc_handle *h, int kind, double *ms, long long *launches);
+int rc_enable_timing(rc_handle *h, int on);
+const char *rc_last_error(rc_handle *h);
+void rc_destroy(rc_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

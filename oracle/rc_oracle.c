@@ -1,0 +1,830 @@
+/*
+ * rc_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY) for the RheoCell
+ * per-step hot path.  A plain-C restatement of the reference algorithm:
+ *
+ *   FD_CPUBackend::init / first_step / compute_forces / sim_step
+ *       (src/Backends/FD_CPUBackend.cpp:19-160)
+ *   WetPolarModel::begin_energy_computation, computeGlobalSums,
+ *       f_interaction, calc_internal_forces, updateDirectedActiveForces
+ *       (src/Interactions/WetPolarModel.cpp:84-430)
+ *   BaseInteraction::derivX/derivY/Laplacian/updateFieldProperties
+ *       (src/Interactions/BaseInteraction.h:196-263, BaseInteraction.cpp:62-69)
+ *   MultiPhaseField ring buffer / recentering / growth
+ *       (src/Fields/MultiPhaseField.cpp:27-359)
+ *   SquareBox periodic map (src/Boxes/SquareBox.cpp:21-82, BaseBox.cpp:29-50)
+ *   Utils::gaussian (src/Utilities/Utils.h:210-234) on POSIX drand48
+ *   Eigen::ConjugateGradient<SparseMatrix, Upper> with the default Jacobi
+ *       preconditioner (Eigen 3, un-vendored; restated from its published
+ *       algorithm: r=b-Ax, stop when |r|^2 < max(tol^2 |b|^2, DBL_MIN),
+ *       z = invdiag.*r, p = z + (rz_new/rz_old) p).
+ *
+ * Expressions keep the reference's operand order so that, compiled with
+ * -ffp-contract=off, every per-site value is the IEEE evaluation of the
+ * reference source.  The float PI of src/defs.h:4 is kept as float.
+ *
+ * Parity pins (tests/): trig tables, ring-buffer map, recentering and growth
+ * against the reference's own MultiPhaseField/SquareBox compiled from
+ * source (oracle/build_ref.sh); drand48 against glibc; formulas against
+ * closed forms; the wet operator against explicit triplet assembly; the CG
+ * against dense numpy solves.  Eigen's exact CG iterate sequence is parity
+ * unpinned (Eigen is absent from this image).
+ */
+#include "rc_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI_F 3.141592653589793238462643f   /* src/defs.h:4, a float literal */
+#define STORE_MAX 20                          /* WetPolarModel.cpp:51 */
+#define BORDER 4                              /* BaseField.cpp:5 */
+
+typedef struct ocell {
+	int index;
+	int LX, LY, S;
+	int offset[2];
+	int corner, corner_old;
+	double unrap[2];
+	int x_sub_left, y_sub_bottom;
+	double CoM[2], CoM_old[2];
+	double Q00, Q01, thetaQ, thetaQ_old;
+	double area, area_old, sumF, sumF_old, perimeter, S00, S01;
+	double tf[2], tr[2];
+	int run_clock;
+	double *phi, *phi_old, *dphi_old, *mu, *dx, *dy, *lap, *rep, *pressure;
+	int *nbr;                  /* 9*S, periodic in the patch */
+	int *map;                  /* map_sub_to_box */
+	double *cosx, *sinx, *cosy, *siny;
+} ocell;
+
+struct orc_sim {
+	int L, N;
+	orc_params act;            /* values as read from the input */
+	/* currently applied values (WetPolarModel members) */
+	double zetaQ_self, zetaQ_inter, J_Q, friction, friction_cell, beta;
+	double a0, p0, tolerance;
+	ocell *c;
+	int *bnbr;                 /* 9*L*L box neighbours */
+	double *phi2, *sum_phi;
+	int *store_size, *store_idx;
+	int *start;
+	int size_rows, size_rows_old, restart_solver;
+	int rows_cap;
+	double *vx, *vy, *fx, *fy;
+	orc_rng rng;
+	int warm_mode;
+	orc_cg_stats cg;
+	double U;
+	char err[512];
+};
+
+/* ------------------------------------------------------------------ */
+static int set_err(orc_sim *s, const char *fmt, ...) {
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(s->err, sizeof(s->err), fmt, ap);
+	va_end(ap);
+	return -1;
+}
+
+const char *orc_last_error(orc_sim *s) { return s->err; }
+
+/* ---------------- RNG: POSIX drand48 + Utils::gaussian -------------- */
+void orc_rng_seed(orc_rng *r, long seed) {
+	/* srand48: high 32 bits = seed, low 16 bits = 0x330E */
+	r->x = (((unsigned long long) (unsigned int) seed) << 16) | 0x330Eull;
+	r->has_next = 0;
+	r->next = -1e6;
+}
+
+double orc_rng_drand48(orc_rng *r) {
+	r->x = (0x5DEECE66Dull * r->x + 0xBull) & 0xFFFFFFFFFFFFull;
+	return ldexp((double) r->x, -48);
+}
+
+/* Marsaglia polar method with the cached second variate (Utils.h:210-234) */
+double orc_rng_gaussian(orc_rng *r) {
+	double u, v, w, toRet;
+	if(r->has_next) {
+		r->has_next = 0;
+		return r->next;
+	}
+	w = 2.;
+	while(w >= 1.0) {
+		u = 2. * orc_rng_drand48(r) - 1.0;
+		v = 2. * orc_rng_drand48(r) - 1.0;
+		w = u * u + v * v;
+	}
+	w = sqrt((-2. * log(w)) / w);
+	toRet = u * w;
+	r->next = v * w;
+	r->has_next = 1;
+	return toRet;
+}
+
+void orc_trig_table(int L, int which, double *out) {
+	/* MultiPhaseField.cpp:58-65: cos(2*PI*i/LsubX), argument in float */
+	for(int i = 0; i < L; i++) {
+		float a = 2 * PI_F * i / L;
+		out[i] = which == 0 ? cos(a) : sin(a);
+	}
+}
+
+/* ---------------- SquareBox (SquareBox.cpp:31-45, 71-76) ------------ */
+static int box_x(int side, int site, int distX) {
+	int x = (site - ((site / side) * side)) + distX;
+	while(x < 0) x += side;
+	while(x >= side) x -= side;
+	return x;
+}
+static int box_y(int side, int site, int distY) {
+	int y = (site / side) + distY;
+	while(y < 0) y += side;
+	while(y >= side) y -= side;
+	return y;
+}
+static int box_elem(int side, int site, int dX, int dY) {
+	return box_x(side, site, dX) + box_y(side, site, dY) * side;
+}
+static void min_image(int side, const double *v1, const double *v2, double *out) {
+	out[0] = v2[0] - v1[0] - rint((v2[0] - v1[0]) / side) * side;
+	out[1] = v2[1] - v1[1] - rint((v2[1] - v1[1]) / side) * side;
+}
+
+/* BaseBox::setNeighborsPeriodic (BaseBox.cpp:29-50) and
+ * MultiPhaseField::setNeighborsSubSquarePeriodic (MultiPhaseField.cpp:188-208) */
+static void periodic_nbrs(int Lx, int Ly, int *nb) {
+	for(int i = 0; i < Lx * Ly; i++) {
+		int y = i / Lx, x = i - (i / Lx) * Lx, ss = 0;
+		for(int j = -1; j <= 1; j++)
+			for(int k = -1; k <= 1; k++) {
+				int xx = x + k, yy = y + j;
+				if(x == 0 && k == -1) xx = Lx - 1;
+				else if(x == Lx - 1 && k == 1) xx = 0;
+				if(y == 0 && j == -1) yy = Ly - 1;
+				else if(y == Ly - 1 && j == 1) yy = 0;
+				nb[ss + i * 9] = xx + yy * Lx;
+				ss++;
+			}
+	}
+}
+
+/* ---------------- MultiPhaseField ------------------------------------ */
+static void *xrealloc_keep(void *p, size_t old_n, size_t new_n, size_t sz) {
+	/* std::vector::resize: keep the first min(old,new) elements, zero the rest */
+	void *q = realloc(p, new_n * sz ? new_n * sz : sz);
+	if(new_n > old_n) memset((char *) q + old_n * sz, 0, (new_n - old_n) * sz);
+	return q;
+}
+
+static void cell_resizing(ocell *p, int oldS, int newS) {
+	p->phi = xrealloc_keep(p->phi, oldS, newS, sizeof(double));
+	p->phi_old = xrealloc_keep(p->phi_old, oldS, newS, sizeof(double));
+	p->dphi_old = xrealloc_keep(p->dphi_old, oldS, newS, sizeof(double));
+	p->mu = xrealloc_keep(p->mu, oldS, newS, sizeof(double));
+	p->dx = xrealloc_keep(p->dx, oldS, newS, sizeof(double));
+	p->dy = xrealloc_keep(p->dy, oldS, newS, sizeof(double));
+	p->lap = xrealloc_keep(p->lap, oldS, newS, sizeof(double));
+	p->rep = xrealloc_keep(p->rep, oldS, newS, sizeof(double));
+	p->pressure = xrealloc_keep(p->pressure, oldS, newS, sizeof(double));
+	p->map = xrealloc_keep(p->map, oldS, newS, sizeof(int));
+	free(p->nbr);
+	p->nbr = malloc(sizeof(int) * 9 * (size_t) newS);
+	free(p->cosx); free(p->sinx); free(p->cosy); free(p->siny);
+	p->cosx = malloc(sizeof(double) * p->LX);
+	p->sinx = malloc(sizeof(double) * p->LX);
+	p->cosy = malloc(sizeof(double) * p->LY);
+	p->siny = malloc(sizeof(double) * p->LY);
+	orc_trig_table(p->LX, 0, p->cosx);
+	orc_trig_table(p->LX, 1, p->sinx);
+	orc_trig_table(p->LY, 0, p->cosy);
+	orc_trig_table(p->LY, 1, p->siny);
+}
+
+static void cell_free(ocell *p) {
+	free(p->phi); free(p->phi_old); free(p->dphi_old); free(p->mu);
+	free(p->dx); free(p->dy); free(p->lap); free(p->rep); free(p->pressure);
+	free(p->map); free(p->nbr);
+	free(p->cosx); free(p->sinx); free(p->cosy); free(p->siny);
+}
+
+/* GetSubIndex (MultiPhaseField.cpp:348-350) */
+static int sub_index(const orc_sim *s, const ocell *p, int q) {
+	return box_elem(s->L, p->corner, ((q - (q / p->LX) * p->LX) + p->offset[0]) % p->LX,
+			((q / p->LX) + p->offset[1]) % p->LY);
+}
+
+static void set_properties_to_zero(ocell *p) {
+	p->S00 = 0; p->S01 = 0;
+	p->x_sub_left = p->LX;
+	p->y_sub_bottom = p->LY;
+	p->area = 0; p->perimeter = 0; p->sumF = 0;
+	p->CoM[0] = 0.; p->CoM[1] = 0.;
+}
+
+static void check_borders(ocell *p, int q) {
+	int y = q / p->LX;
+	int x = q - y * p->LX;
+	x = (x + p->offset[0]) % p->LX;
+	y = (y + p->offset[1]) % p->LY;
+	if(x < p->x_sub_left) p->x_sub_left = x;
+	if(y < p->y_sub_bottom) p->y_sub_bottom = y;
+}
+
+/* MultiPhaseField::set_positions(BaseBox*) (MultiPhaseField.cpp:223-310) */
+static int set_positions(orc_sim *s, ocell *p) {
+	const int side = s->L;
+	if(p->x_sub_left >= BORDER && p->y_sub_bottom >= BORDER) {
+		int site = box_elem(side, p->corner, (int) p->CoM[0], (int) p->CoM[1]);
+		p->CoM[1] = (int) (site / side) + (p->CoM[1] - (int) p->CoM[1]);
+		p->CoM[0] = (int) (site - (int) (site / side) * side) + (p->CoM[0] - (int) p->CoM[0]);
+		double corner_old[2] = { (double) box_x(side, p->corner, 0), (double) box_y(side, p->corner, 0) };
+		p->corner_old = p->corner;
+		p->corner = box_elem(side, site, (int) -p->LX / 2, (int) -p->LY / 2);
+		double corner_new[2] = { (double) box_x(side, p->corner, 0), (double) box_y(side, p->corner, 0) };
+		double d[2];
+		min_image(side, corner_old, corner_new, d);
+		p->unrap[0] += d[0];
+		p->unrap[1] += d[1];
+		p->offset[0] = (int) (p->offset[0] + p->LX - d[0]) % p->LX;
+		p->offset[1] = (int) (p->offset[1] + p->LY - d[1]) % p->LY;
+		return 0;
+	}
+	/* patch growth / reshaping */
+	int LX = p->LX, LY = p->LY;
+	int nLX = LX - (2 * p->x_sub_left) + (2 * BORDER);
+	int nLY = LY - (2 * p->y_sub_bottom) + (2 * BORDER);
+	int nS = nLX * nLY;
+	int ncorner = box_elem(side, p->corner, -(-(2 * p->x_sub_left) + (2 * BORDER)) / 2,
+			-(-(2 * p->y_sub_bottom) + (2 * BORDER)) / 2);
+	double *nf = calloc((size_t) nS, sizeof(double));
+	p->area = 0;
+	int sy, sx, dimy, dimx;
+	if(nLY < LY) { sy = (LY - nLY) / 2; dimy = nLY; } else { sy = 0; dimy = LY; }
+	if(nLX < LX) { sx = (LX - nLX) / 2; dimx = nLX; } else { sx = 0; dimx = LX; }
+	for(int i = sy; i < dimy; i++) {
+		int yy = box_y(side, p->corner, i);
+		int off_y = i - p->offset[1];
+		if(off_y < 0) off_y += LY;
+		for(int j = sx; j < dimx; j++) {
+			int xx = box_x(side, p->corner, j);
+			int dx = (ncorner - (ncorner / side) * side) - xx;
+			if(dx > side / 2) dx -= side;
+			else if(dx < -side / 2) dx += side;
+			int dy = (ncorner / side) - yy;
+			if(dy > side / 2) dy -= side;
+			else if(dy < -side / 2) dy += side;
+			int off_x = j - p->offset[0];
+			if(off_x < 0) off_x += LX;
+			int dst = abs(dx) + abs(dy) * nLX;
+			if(dst < 0 || dst >= nS) {
+				free(nf);
+				return set_err(s, "patch growth of cell %d writes outside the new patch", p->index);
+			}
+			double v = p->phi[off_x + off_y * LX];
+			nf[dst] = v;
+			p->area += v * v;
+		}
+	}
+	double c0[2] = { (double) box_x(side, p->corner, 0), (double) box_y(side, p->corner, 0) };
+	double c1[2] = { (double) box_x(side, ncorner, 0), (double) box_y(side, ncorner, 0) };
+	double d[2];
+	min_image(side, c0, c1, d);
+	p->unrap[0] += d[0];
+	p->unrap[1] += d[1];
+	int oldS = p->S;
+	p->LX = nLX; p->LY = nLY;
+	p->corner_old = p->corner;
+	p->corner = ncorner;
+	p->offset[0] = 0; p->offset[1] = 0;
+	p->S = nLX * nLY;
+	cell_resizing(p, oldS, p->S);
+	memcpy(p->phi, nf, sizeof(double) * (size_t) nS);
+	free(nf);
+	periodic_nbrs(p->LX, p->LY, p->nbr);
+	int site = box_elem(side, p->corner, p->LX / 2, p->LY / 2);
+	p->CoM[1] = (int) (site / side);
+	p->CoM[0] = (int) (site - (int) (site / side) * side);
+	return 0;
+}
+
+/* ---------------- WetPolarModel -------------------------------------- */
+orc_sim *orc_create(int L, int N, const orc_params *prm, long seed) {
+	orc_sim *s = calloc(1, sizeof(orc_sim));
+	if(!(prm->wall_slip > 0)) {
+		/* getWalls()==0 in a periodic box; wall_slip<=0 would mask every stencil */
+		snprintf(s->err, sizeof(s->err), "wall_slip must be > 0 for a periodic square box");
+	}
+	s->L = L;
+	s->N = N;
+	s->act = *prm;
+	/* constructor values before apply_changes_after_equilibration
+	 * (WetPolarModel.cpp:4-21) */
+	s->zetaQ_self = 0; s->zetaQ_inter = 0; s->J_Q = 0;
+	s->friction = 1.; s->friction_cell = 0.; s->beta = 0.;
+	s->p0 = 12.;
+	s->a0 = PI_F * prm->R * prm->R;           /* float arithmetic, WetPolarModel.cpp:50 */
+	s->tolerance = prm->CGtolerance > 0 ? prm->CGtolerance : DBL_EPSILON;
+	s->c = calloc((size_t) N, sizeof(ocell));
+	s->start = calloc((size_t) N, sizeof(int));
+	orc_rng_seed(&s->rng, seed);
+	for(int i = 0; i < N; i++) {             /* read_topology, WetPolarModel.cpp:69-75 */
+		s->c[i].index = i;
+		s->c[i].run_clock = (int) (2 * orc_rng_drand48(&s->rng) * prm->J_Q);
+	}
+	s->bnbr = malloc(sizeof(int) * 9 * (size_t) L * L);
+	periodic_nbrs(L, L, s->bnbr);
+	s->phi2 = calloc((size_t) L * L, sizeof(double));
+	s->sum_phi = calloc((size_t) L * L, sizeof(double));
+	s->store_size = calloc((size_t) L * L, sizeof(int));
+	s->store_idx = calloc((size_t) L * L * STORE_MAX, sizeof(int));
+	return s;
+}
+
+void orc_destroy(orc_sim *s) {
+	if(!s) return;
+	for(int i = 0; i < s->N; i++) cell_free(&s->c[i]);
+	free(s->c); free(s->start); free(s->bnbr); free(s->phi2); free(s->sum_phi);
+	free(s->store_size); free(s->store_idx);
+	free(s->vx); free(s->vy); free(s->fx); free(s->fy);
+	free(s);
+}
+
+int orc_set_cell(orc_sim *s, int i, int LX, int LY, double comx, double comy, int offx, int offy,
+		int corner, double unrapx, double unrapy, double Q00, double Q01, const double *phi) {
+	if(i < 0 || i >= s->N) return set_err(s, "cell index %d out of range", i);
+	ocell *p = &s->c[i];
+	/* MultiPhaseField::init(Lx,Ly) (MultiPhaseField.cpp:68-107) */
+	int oldS = p->S;
+	p->LX = LX; p->LY = LY; p->S = LX * LY;
+	p->x_sub_left = LX; p->y_sub_bottom = LY;
+	cell_resizing(p, oldS, p->S);
+	p->S00 = 0.; p->S01 = 0.;
+	p->thetaQ = p->thetaQ_old = PI_F * (1 - 2 * orc_rng_drand48(&s->rng));
+	p->area = 0; p->perimeter = 0; p->sumF = 0;
+	periodic_nbrs(LX, LY, p->nbr);
+	/* SimBackend.cpp:274-296 */
+	p->CoM[0] = comx; p->CoM[1] = comy;
+	p->CoM_old[0] = comx; p->CoM_old[1] = comy;
+	p->unrap[0] = unrapx; p->unrap[1] = unrapy;
+	p->corner = corner; p->corner_old = corner;
+	p->offset[0] = offx; p->offset[1] = offy;
+	p->Q00 = Q00; p->Q01 = Q01;
+	p->thetaQ = 0.5 * atan2(2 * p->Q01, 2 * p->Q00);
+	p->thetaQ_old = p->thetaQ;
+	for(int k = 0; k < p->S; k++) {
+		p->map[k] = sub_index(s, p, k);
+		p->phi[k] = phi[k];
+		p->area += phi[k] * phi[k];
+		p->sumF += phi[k];
+	}
+	return 0;
+}
+
+static void resize_rows(orc_sim *s, int rows) {
+	if(rows > s->rows_cap) {
+		s->vx = realloc(s->vx, sizeof(double) * rows);
+		s->vy = realloc(s->vy, sizeof(double) * rows);
+		s->fx = realloc(s->fx, sizeof(double) * rows);
+		s->fy = realloc(s->fy, sizeof(double) * rows);
+		s->rows_cap = rows;
+	}
+}
+
+/* BaseInteraction::derivX/derivY/Laplacian in a periodic box (walls = 0) */
+static inline double derivX(const ocell *p, int q) {
+	return .5 * (p->phi[p->nbr[5 + q * 9]] - p->phi[p->nbr[3 + q * 9]]);
+}
+static inline double derivY(const ocell *p, int q) {
+	return .5 * (p->phi[p->nbr[7 + q * 9]] - p->phi[p->nbr[1 + q * 9]]);
+}
+static inline double laplacian(const ocell *p, int q) {
+	return p->phi[p->nbr[7 + q * 9]] + p->phi[p->nbr[1 + q * 9]] + p->phi[p->nbr[5 + q * 9]]
+			+ p->phi[p->nbr[3 + q * 9]] - 4 * p->phi[q];
+}
+
+/* begin_energy_computation() without arguments (WetPolarModel.cpp:113-147) */
+static void init_field_properties(orc_sim *s) {
+	for(int i = 0; i < s->N; i++) {
+		ocell *p = &s->c[i];
+		for(int q = 0; q < p->S; q++) {
+			p->tf[0] = p->tf[1] = p->tr[0] = p->tr[1] = 0.;
+			p->mu[q] = 0.;
+			double dx = derivX(p, q), dy = derivY(p, q);
+			p->dx[q] = dx;
+			p->dy[q] = dy;
+			p->S00 += -0.5 * (dx * dx - dy * dy);
+			p->S01 += -dx * dy;
+		}
+	}
+}
+
+static double f_interaction(orc_sim *s, ocell *p, int q) {
+	int k = p->map[q];
+	const int *bn = s->bnbr + 9 * k;
+	double *phi2 = s->phi2;
+	double laplacianSquare = phi2[bn[5]] + phi2[bn[7]] + phi2[bn[3]] + phi2[bn[1]] - 4. * phi2[k];
+	double ph = p->phi[q];
+	double gamma = s->act.gamma, lambda = s->act.lambda, mu = s->act.mu, kappa = s->act.kappa,
+			omega = s->act.omega, a0 = s->a0;
+	double CH = gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / lambda - 2 * lambda * p->lap[q]);
+	double A = -4 * (mu / a0) * (1 - p->area / a0) * ph;
+	double Rep = 4 * (kappa / lambda) * ph * (phi2[k] - ph * ph);
+	double lsquare = 2 * ph * p->lap[q] + 2 * (p->dx[q] * p->dx[q] + p->dy[q] * p->dy[q]);
+	double suppress = (laplacianSquare - lsquare) / sqrt(1 + (laplacianSquare - lsquare) * (laplacianSquare - lsquare));
+	double Adh = -4 * lambda * omega * suppress * ph;
+	double P = 4 * (s->beta / s->p0) * (1 - p->perimeter / s->p0) * p->lap[q];
+	double V = CH + A + Rep + Adh + P;
+	p->mu[q] += V;
+	p->pressure[q] = Rep - CH - A;
+	p->rep[q] += Rep;
+	return V;
+}
+
+static void calc_internal_forces(orc_sim *s, ocell *p, int q) {
+	const int *nb = p->nbr + 9 * q;
+	double f_passive_x = (-1) * 0.5 * (p->mu[nb[5]] - p->mu[nb[3]]);
+	double f_passive_y = (-1) * 0.5 * (p->mu[nb[7]] - p->mu[nb[1]]);
+	p->tr[0] += (-1) * 0.5 * (p->rep[nb[5]] - p->rep[nb[3]]) * p->phi[q];
+	p->tr[1] += (-1) * 0.5 * (p->rep[nb[7]] - p->rep[nb[1]]) * p->phi[q];
+	double lambda = s->act.lambda;
+	double Theta = (-p->dx[q] * p->Q00) < 0 ? 0.0 : 1.0;
+	double fQx = lambda * (-p->dx[q] * p->Q00) * Theta * p->Q00;
+	Theta = (-p->dy[q] * p->Q01) < 0 ? 0.0 : 1.0;
+	double fQy = lambda * (-p->dy[q] * p->Q01) * Theta * p->Q01;
+	int row = q + s->start[p->index];
+	s->fx[row] = f_passive_x * s->act.passive_alpha + fQx * s->zetaQ_self;
+	s->fy[row] = f_passive_y * s->act.passive_alpha + fQy * s->zetaQ_self;
+	p->tf[0] += s->fx[row] * p->phi[q];
+	p->tf[1] += s->fy[row] * p->phi[q];
+}
+
+/* matrix-free form of the assembled operator (see orc_assemble_dense):
+ * (A v)_i = D_i v_i - eta * sum_{j in 1,3,5,7} sum_{l at nbr_j(k_i)} v_l */
+static void compute_diag(orc_sim *s, double *diag) {
+	for(int i = 0; i < s->N; i++) {
+		ocell *p = &s->c[i];
+		for(int q = 0; q < p->S; q++) {
+			int row = q + s->start[i];
+			int k = p->map[q];
+			double d = s->friction;
+			static const int nv[4] = { 1, 3, 5, 7 };
+			for(int a = 0; a < 4; a++) {
+				int o = s->bnbr[nv[a] + k * 9];
+				for(int l = 0; l < s->store_size[o]; l++) d += s->friction_cell * 1.0;
+			}
+			diag[row] = d;
+		}
+	}
+}
+
+static void apply_A_mf(orc_sim *s, const double *diag, const double *x, double *y) {
+	static const int nv[4] = { 1, 3, 5, 7 };
+	for(int i = 0; i < s->N; i++) {
+		ocell *p = &s->c[i];
+		for(int q = 0; q < p->S; q++) {
+			int row = q + s->start[i];
+			int k = p->map[q];
+			double acc = diag[row] * x[row];
+			for(int a = 0; a < 4; a++) {
+				int o = s->bnbr[nv[a] + k * 9];
+				for(int l = 0; l < s->store_size[o]; l++)
+					acc += (-s->friction_cell * 1.0) * x[s->store_idx[l + o * STORE_MAX]];
+			}
+			y[row] = acc;
+		}
+	}
+}
+
+static double dotv(const double *a, const double *b, int n) {
+	double acc = 0.;
+	for(int i = 0; i < n; i++) acc += a[i] * b[i];
+	return acc;
+}
+
+/* Eigen::ConjugateGradient (Jacobi preconditioner) on x = guess */
+static int cg_solve(orc_sim *s, const double *diag, const double *b, double *x, int *iters_out, double *relres) {
+	int n = s->size_rows;
+	double *r = malloc(sizeof(double) * n), *pp = malloc(sizeof(double) * n),
+			*z = malloc(sizeof(double) * n), *t = malloc(sizeof(double) * n), *inv = malloc(sizeof(double) * n);
+	for(int i = 0; i < n; i++) inv[i] = 1. / diag[i];
+	apply_A_mf(s, diag, x, t);
+	for(int i = 0; i < n; i++) r[i] = b[i] - t[i];
+	double rhsNorm2 = dotv(b, b, n);
+	int iters = 0;
+	if(rhsNorm2 == 0) {
+		for(int i = 0; i < n; i++) x[i] = 0.;
+		*relres = 0.;
+		goto done;
+	}
+	double tol = s->tolerance;
+	double threshold = tol * tol * rhsNorm2;
+	if(threshold < DBL_MIN) threshold = DBL_MIN;
+	double rn2 = dotv(r, r, n);
+	if(rn2 < threshold) {
+		*relres = sqrt(rn2 / rhsNorm2);
+		goto done;
+	}
+	for(int i = 0; i < n; i++) pp[i] = inv[i] * r[i];
+	double absNew = dotv(r, pp, n);
+	int maxIters = s->act.cg_max_iter > 0 ? s->act.cg_max_iter : 2 * n;
+	int i = 0;
+	while(i < maxIters) {
+		apply_A_mf(s, diag, pp, t);
+		double alpha = absNew / dotv(pp, t, n);
+		for(int j = 0; j < n; j++) x[j] += alpha * pp[j];
+		for(int j = 0; j < n; j++) r[j] -= alpha * t[j];
+		rn2 = dotv(r, r, n);
+		iters++;
+		if(rn2 < threshold) break;
+		for(int j = 0; j < n; j++) z[j] = inv[j] * r[j];
+		double absOld = absNew;
+		absNew = dotv(r, z, n);
+		double beta = absNew / absOld;
+		for(int j = 0; j < n; j++) pp[j] = z[j] + beta * pp[j];
+		i++;
+	}
+	*relres = sqrt(rn2 / rhsNorm2);
+done:
+	*iters_out = iters;
+	free(r); free(pp); free(z); free(t); free(inv);
+	return 0;
+}
+
+/* WetPolarModel::begin_energy_computation(fields) (WetPolarModel.cpp:149-234) */
+static int compute_forces(orc_sim *s) {
+	int LL = s->L * s->L;
+	memset(s->store_size, 0, sizeof(int) * LL);
+	if(s->restart_solver < 10 || s->warm_mode == 1) s->size_rows_old = s->size_rows;
+	else {
+		s->size_rows_old = 0;
+		s->restart_solver = 0;
+	}
+	s->size_rows = 0;
+	for(int i = 0; i < s->N; i++) {
+		ocell *p = &s->c[i];
+		s->start[i] = s->size_rows;
+		s->size_rows += p->S;
+		for(int q = 0; q < p->S; q++) {     /* computeGlobalSums */
+			int k = sub_index(s, p, q);
+			s->phi2[k] += p->phi[q] * p->phi[q];
+			s->sum_phi[k] += p->phi[q];
+			p->map[q] = k;
+			p->dx[q] = derivX(p, q);
+			p->dy[q] = derivY(p, q);
+			p->lap[q] = laplacian(p, q);
+			p->perimeter += p->dx[q] * p->dx[q] + p->dy[q] * p->dy[q];
+			p->tf[0] = p->tf[1] = p->tr[0] = p->tr[1] = 0.;
+			p->mu[q] = 0.;
+			if(s->store_size[k] >= STORE_MAX)
+				return set_err(s, "Too many field patches overlap: %d, %d, %d, %d", p->index, k, q, p->corner);
+			s->store_idx[s->store_size[k] + k * STORE_MAX] = q + s->start[i];
+			s->store_size[k]++;
+		}
+	}
+	int cold = s->size_rows != s->size_rows_old;
+	if(cold) {
+		/* Eigen VectorXd::resize loses the contents */
+		resize_rows(s, s->size_rows);
+		for(int r = 0; r < s->size_rows; r++) s->vx[r] = s->vy[r] = 0.;
+	}
+	s->U = 0;
+	for(int i = 0; i < s->N; i++) {
+		ocell *p = &s->c[i];
+		for(int q = 0; q < p->S; q++) s->U += f_interaction(s, p, q);
+	}
+	for(int i = 0; i < s->N; i++) {
+		ocell *p = &s->c[i];
+		for(int q = 0; q < p->S; q++) {
+			calc_internal_forces(s, p, q);
+			if(s->friction_cell == 0) {
+				int row = q + s->start[i];
+				s->vx[row] = s->fx[row] / s->friction;
+				s->vy[row] = s->fy[row] / s->friction;
+			}
+		}
+	}
+	if(s->friction_cell != 0) {
+		double *diag = malloc(sizeof(double) * s->size_rows);
+		compute_diag(s, diag);
+		int itx, ity;
+		double rx, ry;
+		if(cold) {
+			for(int r = 0; r < s->size_rows; r++) s->vx[r] = s->vy[r] = 0.;
+			s->restart_solver = 0;
+		}
+		else
+			s->restart_solver += 1;
+		cg_solve(s, diag, s->fx, s->vx, &itx, &rx);
+		cg_solve(s, diag, s->fy, s->vy, &ity, &ry);
+		s->cg.solves += 2;
+		s->cg.iters_total += itx + ity;
+		s->cg.last_iters[0] = itx;
+		s->cg.last_iters[1] = ity;
+		s->cg.last_relres[0] = rx;
+		s->cg.last_relres[1] = ry;
+		s->cg.last_cold = cold;
+		free(diag);
+	}
+	return 0;
+}
+
+/* WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:342-414) */
+static void update_directed_active_forces(orc_sim *s, ocell *p, int store) {
+	double dt = s->act.dt;
+	if(store) p->thetaQ_old = p->thetaQ + sqrt(dt) * s->zetaQ_inter * orc_rng_gaussian(&s->rng);
+	double f0 = p->tr[0], f1 = p->tr[1];
+	p->thetaQ = p->thetaQ_old - dt * s->J_Q * (atan2(f0 * p->Q01 - f1 * p->Q00, f0 * p->Q00 + f1 * p->Q01));
+	p->Q00 = cos(p->thetaQ);
+	p->Q01 = sin(p->thetaQ);
+	if(s->act.anchoring) {                 /* update_anchoring, WetPolarModel.cpp:417-430 */
+		double walls_length = 8;
+		double dist1 = ((double) p->CoM_old[1] - walls_length);
+		double dist2 = ((double) s->L - walls_length) - (double) p->CoM_old[1];
+		double theta;
+		if(dist1 < dist2) theta = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) s->L - 2 * walls_length) / 2);
+		else theta = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) s->L - 2 * walls_length) / 2));
+		double n0 = cos(theta), n1 = sin(theta);
+		p->Q00 = 0.5 * (n0 * n0 - n1 * n1);
+		p->Q01 = n0 * n1;
+	}
+}
+
+/* FD_CPUBackend::first_step (FD_CPUBackend.cpp:44-120) */
+static int first_step(orc_sim *s, int store) {
+	double J0 = s->act.J0, dt = s->act.dt;
+	for(int i = 0; i < s->N; i++) {
+		ocell *p = &s->c[i];
+		double temp[4] = { 0, 0, 0, 0 };
+		if(store) {
+			p->CoM_old[0] = p->CoM[0];
+			p->CoM_old[1] = p->CoM[1];
+			p->area_old = p->area;
+			p->sumF_old = p->sumF;
+		}
+		set_properties_to_zero(p);
+		for(int q = 0; q < p->S; q++) {
+			int k = p->map[q];
+			int row = q + s->start[i];
+			double Vx = 0., Vy = 0.;   /* no external forces */
+			double dphi = -J0 * p->mu[q] - s->vx[row] * p->dx[q] - s->vy[row] * p->dy[q]
+					- Vx * p->dx[q] - Vy * p->dy[q];
+			if(store) {
+				p->phi_old[q] = p->phi[q];
+				p->dphi_old[q] = dphi;
+			}
+			double phi = p->phi_old[q] + dt * .5 * (dphi + p->dphi_old[q]);
+			p->phi[q] = phi;
+			p->area += phi * phi;
+			p->sumF += phi;
+			int x = (((q - (q / p->LX) * p->LX) + p->offset[0]) % p->LX);
+			int y = (((q / p->LX) + p->offset[1]) % p->LY);
+			temp[0] += phi * p->cosx[x];
+			temp[1] += phi * p->sinx[x];
+			temp[2] += phi * p->cosy[y];
+			temp[3] += phi * p->siny[y];
+			if(!store && phi < 0.09 && phi > 0.01) check_borders(p, q);
+			s->phi2[k] = 0;                  /* resetSums */
+			s->sum_phi[k] = 0;
+			p->S00 += -0.5 * (p->dx[q] * p->dx[q] - p->dy[q] * p->dy[q]);
+			p->S01 += -p->dx[q] * p->dy[q];
+		}
+		p->CoM[0] = p->LX * (atan2(-temp[1] / p->sumF, -temp[0] / p->sumF) + PI_F) / (2 * PI_F);
+		p->CoM[1] = p->LY * (atan2(-temp[3] / p->sumF, -temp[2] / p->sumF) + PI_F) / (2 * PI_F);
+		if(!store)
+			if(set_positions(s, p)) return -1;
+		update_directed_active_forces(s, p, store);
+	}
+	return 0;
+}
+
+int orc_init(orc_sim *s) {
+	for(int k = 0; k < s->L * s->L; k++) {
+		s->phi2[k] = 0;
+		s->sum_phi[k] = 0;
+		s->store_size[k] = 0;
+	}
+	s->restart_solver = 0;
+	s->size_rows = 0;
+	init_field_properties(s);
+	return compute_forces(s);
+}
+
+void orc_apply_changes_after_equilibration(orc_sim *s) {
+	s->zetaQ_self = s->act.zetaQ_self;
+	s->zetaQ_inter = s->act.zetaQ_inter;
+	s->J_Q = s->act.J_Q;
+	s->friction = s->act.friction;
+	s->friction_cell = s->act.friction_cell;
+	s->beta = s->act.beta;
+}
+
+int orc_step(orc_sim *s, int nsteps) {
+	for(int n = 0; n < nsteps; n++) {
+		if(first_step(s, 1)) return -1;
+		if(compute_forces(s)) return -1;
+		if(first_step(s, 0)) return -1;
+		if(compute_forces(s)) return -1;
+	}
+	return 0;
+}
+
+void orc_set_warm_mode(orc_sim *s, int mode) { s->warm_mode = mode; }
+
+/* ---------------- accessors ------------------------------------------ */
+int orc_num_cells(orc_sim *s) { return s->N; }
+int orc_size_rows(orc_sim *s) { return s->size_rows; }
+
+int orc_get_cell(orc_sim *s, int i, orc_cell_info *o) {
+	if(i < 0 || i >= s->N) return set_err(s, "cell index %d out of range", i);
+	ocell *p = &s->c[i];
+	o->LsubX = p->LX; o->LsubY = p->LY; o->subSize = p->S;
+	o->offset[0] = p->offset[0]; o->offset[1] = p->offset[1];
+	o->corner = p->corner; o->corner_old = p->corner_old;
+	o->x_sub_left = p->x_sub_left; o->y_sub_bottom = p->y_sub_bottom;
+	o->start = s->start[i];
+	o->CoM[0] = p->CoM[0]; o->CoM[1] = p->CoM[1];
+	o->CoM_old[0] = p->CoM_old[0]; o->CoM_old[1] = p->CoM_old[1];
+	o->unrap[0] = p->unrap[0]; o->unrap[1] = p->unrap[1];
+	o->Q00 = p->Q00; o->Q01 = p->Q01; o->thetaQ = p->thetaQ; o->thetaQ_old = p->thetaQ_old;
+	o->area = p->area; o->area_old = p->area_old; o->sumF = p->sumF; o->sumF_old = p->sumF_old;
+	o->perimeter = p->perimeter; o->S00 = p->S00; o->S01 = p->S01;
+	o->total_force[0] = p->tf[0]; o->total_force[1] = p->tf[1];
+	o->total_force_repulsion[0] = p->tr[0]; o->total_force_repulsion[1] = p->tr[1];
+	return 0;
+}
+
+int orc_get_site_array(orc_sim *s, int i, int which, double *out) {
+	if(i < 0 || i >= s->N) return set_err(s, "cell index %d out of range", i);
+	ocell *p = &s->c[i];
+	for(int q = 0; q < p->S; q++) {
+		switch(which) {
+		case 0: out[q] = p->phi[q]; break;
+		case 1: out[q] = p->mu[q]; break;
+		case 2: out[q] = p->dx[q]; break;
+		case 3: out[q] = p->dy[q]; break;
+		case 4: out[q] = p->lap[q]; break;
+		case 5: out[q] = p->phi_old[q]; break;
+		case 6: out[q] = p->dphi_old[q]; break;
+		case 7: out[q] = p->rep[q]; break;
+		case 8: out[q] = (double) p->map[q]; break;
+		default: return set_err(s, "bad site array %d", which);
+		}
+	}
+	return 0;
+}
+
+int orc_get_rows(orc_sim *s, int which, double *out) {
+	double *src = which == 0 ? s->vx : which == 1 ? s->vy : which == 2 ? s->fx : which == 3 ? s->fy : NULL;
+	if(!src) return set_err(s, "bad row array %d", which);
+	memcpy(out, src, sizeof(double) * s->size_rows);
+	return 0;
+}
+
+int orc_get_box(orc_sim *s, int which, double *out) {
+	int LL = s->L * s->L;
+	for(int k = 0; k < LL; k++) {
+		if(which == 0) out[k] = s->phi2[k];
+		else if(which == 1) out[k] = s->sum_phi[k];
+		else if(which == 2) out[k] = s->store_size[k];
+		else return set_err(s, "bad box array %d", which);
+	}
+	return 0;
+}
+
+void orc_get_cg_stats(orc_sim *s, orc_cg_stats *o) { *o = s->cg; }
+
+int orc_apply_A(orc_sim *s, const double *x, double *y) {
+	double *diag = malloc(sizeof(double) * s->size_rows);
+	compute_diag(s, diag);
+	apply_A_mf(s, diag, x, y);
+	free(diag);
+	return 0;
+}
+
+/* Triplet rules of WetPolarModel.cpp:200-217 summed like setFromTriplets */
+int orc_assemble_dense(orc_sim *s, double *A) {
+	int n = s->size_rows;
+	static const int nv[4] = { 1, 3, 5, 7 };
+	static const double w[9] = { 0, 1, 0, 1, 0, 1, 0, 1, 0 };
+	memset(A, 0, sizeof(double) * (size_t) n * n);
+	for(int i = 0; i < s->N; i++) {
+		ocell *p = &s->c[i];
+		for(int q = 0; q < p->S; q++) {
+			int row = q + s->start[i];
+			A[(size_t) row * n + row] += s->friction;
+			for(int a = 0; a < 4; a++) {
+				int j = nv[a];
+				int o = s->bnbr[j + p->map[q] * 9];
+				for(int l = 0; l < s->store_size[o]; l++) {
+					A[(size_t) row * n + row] += s->friction_cell * w[j];
+					A[(size_t) row * n + s->store_idx[l + o * STORE_MAX]] += -s->friction_cell * w[j];
+				}
+			}
+		}
+	}
+	return 0;
+}

@@ -1,0 +1,106 @@
+/*
+ * rc_oracle.h -- CPU oracle for the RheoCell per-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference algorithm (FD_CPUBackend + WetPolarModel + MultiPhaseField +
+ * SquareBox + Utils::gaussian), written to check the HIP product path.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product (rheocell_amd) never links or calls it.
+ *
+ * One oracle instance = one periodic square box with N cells = one
+ * reference process (the reference keeps its RNG in process-global state;
+ * here every instance owns its drand48 state and gaussian cache).
+ */
+#ifndef RC_ORACLE_H_
+#define RC_ORACLE_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Model parameters exactly as read from the input file (the *_active
+ * values of WetPolarModel::get_settings, WetPolarModel.cpp:28-46, and the
+ * backend keys J0 / dt, FD_CPUBackend.cpp:15, FDBackend.cpp:34). */
+typedef struct orc_params {
+	double gamma, lambda, omega, mu, kappa;
+	int R;
+	int anchoring;
+	double zetaQ_self, zetaQ_inter, J_Q;
+	double friction, friction_cell, CGtolerance, wall_slip, passive_alpha, beta;
+	double J0, dt;
+	int cg_max_iter;     /* <=0: Eigen's default 2*rows */
+} orc_params;
+
+/* Per-cell scalar state (BaseField.h:122-225). */
+typedef struct orc_cell_info {
+	int LsubX, LsubY, subSize;
+	int offset[2];
+	int corner, corner_old;
+	int x_sub_left, y_sub_bottom;
+	int start;                    /* field_start_index */
+	double CoM[2], CoM_old[2];
+	double unrap[2];
+	double Q00, Q01, thetaQ, thetaQ_old;
+	double area, area_old, sumF, sumF_old, perimeter, S00, S01;
+	double total_force[2], total_force_repulsion[2];
+} orc_cell_info;
+
+typedef struct orc_cg_stats {
+	long long solves;
+	long long iters_total;        /* SpMV-iterations summed over solves */
+	int last_iters[2];            /* x, y: SpMV-iterations of the last solve */
+	double last_relres[2];
+	int last_cold;
+} orc_cg_stats;
+
+typedef struct orc_sim orc_sim;
+
+/* SimManager::init srand48(seed) + WetPolarModel::read_topology (N draws). */
+orc_sim *orc_create(int L, int N, const orc_params *p, long seed);
+void orc_destroy(orc_sim *s);
+const char *orc_last_error(orc_sim *s);
+
+/* read_next_configuration for one cell (SimBackend.cpp:269-296);
+ * consumes one drand48 (MultiPhaseField::init). Cells must be set in order. */
+int orc_set_cell(orc_sim *s, int i, int LsubX, int LsubY, double comx, double comy,
+		int offx, int offy, int corner, double unrapx, double unrapy,
+		double Q00, double Q01, const double *phi);
+
+/* FD_CPUBackend::init force evaluation (pre-equilibration parameters). */
+int orc_init(orc_sim *s);
+void orc_apply_changes_after_equilibration(orc_sim *s);
+int orc_step(orc_sim *s, int nsteps);
+
+int orc_num_cells(orc_sim *s);
+int orc_size_rows(orc_sim *s);
+int orc_get_cell(orc_sim *s, int i, orc_cell_info *out);
+/* which: 0 phi, 1 freeEnergy (mu), 2 fieldDX, 3 fieldDY, 4 laplacianPhi,
+ *        5 phi_old, 6 dphi_old, 7 Repulsion, 8 map_sub_to_box (as double) */
+int orc_get_site_array(orc_sim *s, int i, int which, double *out);
+/* velocity / force rows in field_start_index order. which: 0 vx,1 vy,2 fx,3 fy */
+int orc_get_rows(orc_sim *s, int which, double *out);
+int orc_get_box(orc_sim *s, int which, double *out); /* 0 phi2, 1 sum_phi, 2 cover count */
+void orc_get_cg_stats(orc_sim *s, orc_cg_stats *out);
+/* 0: reference rule (cold every 11th solve / size change); 1: always warm */
+void orc_set_warm_mode(orc_sim *s, int mode);
+
+/* drand48 / gaussian restatements for golden tests */
+typedef struct orc_rng { unsigned long long x; int has_next; double next; } orc_rng;
+void orc_rng_seed(orc_rng *r, long seed);
+double orc_rng_drand48(orc_rng *r);
+double orc_rng_gaussian(orc_rng *r);
+/* Patch trig table exactly as MultiPhaseField::resizing evaluates it
+ * (float argument 2*PI*i/L, double cos/sin). which: 0 cos, 1 sin */
+void orc_trig_table(int L, int which, double *out);
+
+/* Dense/matrix-free wet operator helpers for brute-force tests:
+ * y = A x for the current coverage (after orc_init or a step). */
+int orc_apply_A(orc_sim *s, const double *x, double *y);
+/* Explicit assembly following the reference triplet rules
+ * (WetPolarModel.cpp:200-217), dense row-major rows x rows. */
+int orc_assemble_dense(orc_sim *s, double *A);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,90 @@
+"""ctypes binding of the C ABI in include/rheocell.h (librheocell.so).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librheocell.so")
+
+RC_OK = 0
+RC_ERR = {1: "RC_ERR_ARG", 2: "RC_ERR_HIP", 3: "RC_ERR_OVERLAP", 4: "RC_ERR_GROWTH", 5: "RC_ERR_CG",
+          6: "RC_ERR_STATE", 7: "RC_ERR_TILE"}
+RC_WARM_REFERENCE = 0
+RC_WARM_PREVIOUS = 1
+
+EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_after_equilibration", "rc_step",
+           "rc_download_cells", "rc_get_rows", "rc_get_site_array", "rc_total_rows", "rc_cg_stats",
+           "rc_set_warm_start", "rc_timing", "rc_enable_timing", "rc_last_error", "rc_destroy"]
+
+
+class rc_model_params(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("lambda_", C.c_double), ("omega", C.c_double), ("mu", C.c_double),
+        ("kappa", C.c_double), ("R", C.c_int), ("anchoring", C.c_int),
+        ("zetaQ_self", C.c_double), ("zetaQ_inter", C.c_double), ("J_Q", C.c_double),
+        ("friction", C.c_double), ("friction_cell", C.c_double), ("CGtolerance", C.c_double),
+        ("wall_slip", C.c_double), ("passive_alpha", C.c_double), ("beta", C.c_double),
+        ("J0", C.c_double), ("dt", C.c_double), ("cg_max_iter", C.c_int),
+    ]
+
+
+class rc_box_desc(C.Structure):
+    _fields_ = [("L", C.c_int), ("ncells", C.c_int), ("params", rc_model_params)]
+
+
+class rc_cell(C.Structure):
+    _fields_ = [
+        ("LsubX", C.c_int), ("LsubY", C.c_int), ("offset", C.c_int * 2), ("corner", C.c_int),
+        ("corner_old", C.c_int), ("x_sub_left", C.c_int), ("y_sub_bottom", C.c_int),
+        ("CoM", C.c_double * 2), ("CoM_old", C.c_double * 2), ("unrap", C.c_double * 2),
+        ("Q00", C.c_double), ("Q01", C.c_double), ("thetaQ", C.c_double), ("thetaQ_old", C.c_double),
+        ("area", C.c_double), ("area_old", C.c_double), ("sumF", C.c_double), ("sumF_old", C.c_double),
+        ("perimeter", C.c_double), ("S00", C.c_double), ("S01", C.c_double),
+        ("total_force_repulsion", C.c_double * 2),
+    ]
+
+
+class rc_cg_stats_t(C.Structure):
+    _fields_ = [("solves", C.c_longlong), ("iters_total", C.c_longlong), ("last_iters", C.c_int * 2),
+                ("last_relres", C.c_double * 2), ("last_cold", C.c_int), ("last_solve_ms", C.c_double),
+                ("total_solve_ms", C.c_double)]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load librheocell.so and declare the C ABI.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("librheocell.so not found at %s: build it with `python -c 'import "
+                           "__graft_entry__ as g; g.build()'` (there is no CPU fallback)" % path)
+    L = C.CDLL(path)
+    vp, ip, dp = C.c_void_p, C.c_int, C.POINTER(C.c_double)
+    L.rc_create.argtypes = [C.POINTER(rc_box_desc), ip, C.POINTER(vp)]
+    L.rc_upload_cells.argtypes = [vp, C.POINTER(rc_cell), dp]
+    L.rc_init_forces.argtypes = [vp]
+    L.rc_apply_changes_after_equilibration.argtypes = [vp]
+    L.rc_step.argtypes = [vp, ip, dp]
+    L.rc_download_cells.argtypes = [vp, C.POINTER(rc_cell), dp]
+    L.rc_get_rows.argtypes = [vp, ip, dp]
+    L.rc_get_site_array.argtypes = [vp, ip, dp]
+    L.rc_total_rows.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.rc_cg_stats.argtypes = [vp, C.POINTER(rc_cg_stats_t)]
+    L.rc_set_warm_start.argtypes = [vp, ip]
+    L.rc_timing.argtypes = [vp, ip, dp, C.POINTER(C.c_longlong)]
+    L.rc_enable_timing.argtypes = [vp, ip]
+    L.rc_last_error.argtypes = [vp]
+    L.rc_last_error.restype = C.c_char_p
+    L.rc_destroy.argtypes = [vp]
+    L.rc_destroy.restype = None
+    for name in EXPORTS:
+        getattr(L, name)
+    _lib = L
+    return L

@@ -1,0 +1,824 @@
+// rc_api.hip -- C ABI (include/rheocell.h) over the gfx950 kernels.
+//
+// Step schedule (FD_CPUBackend::sim_step, src/Backends/FD_CPUBackend.cpp:134-160,
+// is first_step(true), compute_forces, first_step(false), compute_forces):
+//
+//  dry model (friction_cell == 0): the force evaluation and the following
+//    field update of a stage are fused into one per-cell kernel, so a stage
+//    is  k_box_gather(phi) -> k_stage<FUSED>.  The force evaluation that the
+//    reference runs at the end of a step therefore runs at the start of the
+//    next stage; mu, v, grad phi are never written to HBM and are
+//    re-materialised (k_stage<CF>, no side effects) only when a caller asks
+//    for them.
+//  wet model: a stage is k_stage<FS> (field update, recomputing mu and
+//    grad phi from phi and the box sums), then k_box_gather -> k_stage<CF>
+//    (right-hand side rows) -> Jacobi-PCG (x and y together), exactly the
+//    reference order, so the velocity rows always belong to the current phi.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rheocell.h"
+#include "rc_kernels.h"
+
+using namespace rc;
+
+struct TimedPair {
+	int kind;
+	hipEvent_t a, b;
+};
+
+struct rc_handle {
+	// configuration
+	std::vector<rc_box_desc> boxes;
+	std::vector<DevBox> dboxes;
+	std::vector<DevParams> dparams;   // 2 per box: [0] before equilibration, [1] active
+	int nboxes = 0;
+	int ncells = 0;
+	long long rows = 0;
+	int ntiles = 0;
+	long long box_sites = 0;
+	int max_S = 0, max_W = 0;
+	size_t lds_stage = 0, lds_cg = 0;
+	bool wet = false, jq = false;
+
+	// device memory
+	hipStream_t stream = nullptr;
+	DevBox *d_boxes = nullptr;
+	int *d_tile_box = nullptr;
+	DevCell *d_cells = nullptr;
+	DevParams *d_params = nullptr;
+	double *d_phiA = nullptr, *d_phiB = nullptr, *d_dphi = nullptr, *d_rep = nullptr;
+	double2 *d_v = nullptr, *d_f = nullptr, *d_r = nullptr, *d_p = nullptr;
+	double *d_trig = nullptr;
+	double *d_phi2 = nullptr, *d_diag = nullptr;
+	int *d_cover = nullptr, *d_tile_cells = nullptr, *d_tile_count = nullptr;
+	double2 *d_sp = nullptr;
+	double *d_part = nullptr;
+	CGState *d_cg = nullptr;
+	DevStatus *d_st = nullptr;
+	double *d_gauss = nullptr;
+	long long gauss_cap = 0;
+	double *d_out[4] = { nullptr, nullptr, nullptr, nullptr };
+	CGState *h_cg = nullptr;      // pinned
+	hipEvent_t cg_e0 = nullptr, cg_e1 = nullptr;
+	DevStatus *h_st = nullptr;    // pinned
+
+	// state machine
+	bool uploaded = false, initialized = false;
+	int cur_set = 0;              // parameter set currently applied
+	int cf_set = 0;               // set of the force evaluation preceding the next field update
+	int restart_solver = 0;
+	long long size_rows = 0, size_rows_old = 0;
+	int warm_mode = RC_WARM_REFERENCE;
+	int last_iters = 8;
+	rc_cg_stats_t cgs{};
+	std::string err;
+
+	// timing
+	bool timing = false;
+	std::vector<TimedPair> pairs;
+	std::vector<hipEvent_t> pool;
+	size_t pool_used = 0;
+	double t_ms[4] = { 0, 0, 0, 0 };
+	long long t_n[4] = { 0, 0, 0, 0 };
+};
+
+// ---------------------------------------------------------------------------
+static int fail(rc_handle *h, int code, const char *fmt, ...) {
+	char buf[1024];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	if(h) h->err = buf;
+	return code;
+}
+
+#define HIPCHK(h, call)                                                                  \
+	do {                                                                                 \
+		hipError_t e_ = (call);                                                          \
+		if(e_ != hipSuccess) return fail(h, RC_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+	} while(0)
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t n) {
+	*p = nullptr;
+	if(n == 0) n = 1;
+	return hipMalloc((void **) p, n * sizeof(T));
+}
+
+static void dfree(void *p) {
+	if(p) (void) hipFree(p);
+}
+
+static hipEvent_t ev_get(rc_handle *h) {
+	if(h->pool_used == h->pool.size()) {
+		hipEvent_t e;
+		(void) hipEventCreate(&e);
+		h->pool.push_back(e);
+	}
+	return h->pool[h->pool_used++];
+}
+
+struct TimeScope {
+	rc_handle *h;
+	int kind;
+	hipEvent_t b;
+	TimeScope(rc_handle *h_, int k) : h(h_), kind(k), b(nullptr) {
+		if(h->timing) {
+			hipEvent_t a = ev_get(h);
+			b = ev_get(h);
+			(void) hipEventRecord(a, h->stream);
+			h->pairs.push_back({ kind, a, b });
+		}
+	}
+	~TimeScope() {
+		if(h->timing) (void) hipEventRecord(b, h->stream);
+	}
+};
+
+static void collect_timing(rc_handle *h) {
+	if(!h->timing) return;
+	for(auto &p : h->pairs) {
+		float ms = 0.f;
+		if(hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+			h->t_ms[p.kind] += ms;
+			h->t_n[p.kind] += 1;
+		}
+	}
+	h->pairs.clear();
+	h->pool_used = 0;
+}
+
+static DevParams make_params(const rc_model_params &m, bool active) {
+	DevParams p{};
+	p.gamma = m.gamma;
+	p.lambda = m.lambda;
+	p.omega = m.omega;
+	p.mu = m.mu;
+	p.kappa = m.kappa;
+	p.a0 = PI_F * m.R * m.R;   // float arithmetic, WetPolarModel.cpp:50
+	p.p0 = 12.;                // WetPolarModel.h:42
+	p.passive_alpha = m.passive_alpha;
+	p.J0 = m.J0;
+	p.dt = m.dt;
+	p.sqrt_dt = std::sqrt(m.dt);
+	p.anchoring = m.anchoring;
+	if(active) {
+		p.zetaQ_self = m.zetaQ_self;
+		p.zetaQ_inter = m.zetaQ_inter;
+		p.J_Q = m.J_Q;
+		p.friction = m.friction;
+		p.friction_cell = m.friction_cell;
+		p.beta = m.beta;
+	}
+	else {  // WetPolarModel constructor values (WetPolarModel.cpp:4-21)
+		p.zetaQ_self = 0;
+		p.zetaQ_inter = 0;
+		p.J_Q = 0;
+		p.friction = 1.;
+		p.friction_cell = 0.;
+		p.beta = 0.;
+	}
+	return p;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" const char *rc_last_error(rc_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) {
+	if(!out || !boxes || nboxes <= 0) return RC_ERR_ARG;
+	*out = nullptr;
+	rc_handle *h = new rc_handle();
+	h->nboxes = nboxes;
+	h->boxes.assign(boxes, boxes + nboxes);
+	int cells = 0, tiles = 0;
+	long long sites = 0;
+	for(int b = 0; b < nboxes; ++b) {
+		const rc_box_desc &d = boxes[b];
+		if(d.L <= 2 || d.ncells < 0) {
+			*out = h;
+			return fail(h, RC_ERR_ARG, "box %d: invalid side %d or cell count %d", b, d.L, d.ncells);
+		}
+		if(!(d.params.wall_slip > 0)) {
+			*out = h;
+			return fail(h, RC_ERR_ARG, "box %d: wall_slip must be > 0 in a periodic square box", b);
+		}
+		if(!(d.params.friction > 0)) {
+			*out = h;
+			return fail(h, RC_ERR_ARG, "box %d: friction must be > 0", b);
+		}
+		DevBox B{};
+		B.L = d.L;
+		B.box_off = (int) sites;
+		B.tile_off = tiles;
+		B.ntx = (d.L + TW - 1) / TW;
+		B.nty = (d.L + TH - 1) / TH;
+		B.cell_begin = cells;
+		B.cell_end = cells + d.ncells;
+		h->dboxes.push_back(B);
+		h->dparams.push_back(make_params(d.params, false));
+		h->dparams.push_back(make_params(d.params, true));
+		cells += d.ncells;
+		tiles += B.ntx * B.nty;
+		sites += (long long) d.L * d.L;
+		if(d.params.friction_cell != 0) h->wet = true;
+		if(d.params.J_Q != 0) h->jq = true;
+	}
+	if(h->wet && nboxes != 1) {
+		*out = h;
+		return fail(h, RC_ERR_ARG, "the wet model (friction_cell != 0) runs one box per handle");
+	}
+	for(int b = 1; b < nboxes; ++b)
+		if((boxes[b].params.friction_cell != 0) != h->wet) {
+			*out = h;
+			return fail(h, RC_ERR_ARG, "all boxes of a handle must share the dry/wet model");
+		}
+	h->ncells = cells;
+	h->ntiles = tiles;
+	h->box_sites = sites;
+	*out = h;
+	HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+	HIPCHK(h, dalloc(&h->d_boxes, nboxes));
+	HIPCHK(h, dalloc(&h->d_params, 2 * nboxes));
+	HIPCHK(h, dalloc(&h->d_cells, cells));
+	HIPCHK(h, dalloc(&h->d_phi2, sites));
+	HIPCHK(h, dalloc(&h->d_cover, sites));
+	HIPCHK(h, dalloc(&h->d_st, 1));
+	HIPCHK(h, hipHostMalloc((void **) &h->h_st, sizeof(DevStatus)));
+	HIPCHK(h, hipHostMalloc((void **) &h->h_cg, sizeof(CGState)));
+	HIPCHK(h, hipEventCreate(&h->cg_e0));
+	HIPCHK(h, hipEventCreate(&h->cg_e1));
+	std::vector<int> tile_box(tiles);
+	for(int b = 0; b < nboxes; ++b)
+		for(int t = 0; t < h->dboxes[b].ntx * h->dboxes[b].nty; ++t) tile_box[h->dboxes[b].tile_off + t] = b;
+	HIPCHK(h, dalloc(&h->d_tile_box, tiles));
+	HIPCHK(h, hipMemcpy(h->d_tile_box, tile_box.data(), sizeof(int) * tiles, hipMemcpyHostToDevice));
+	HIPCHK(h, hipMemcpy(h->d_boxes, h->dboxes.data(), sizeof(DevBox) * nboxes, hipMemcpyHostToDevice));
+	HIPCHK(h, hipMemcpy(h->d_params, h->dparams.data(), sizeof(DevParams) * 2 * nboxes, hipMemcpyHostToDevice));
+	HIPCHK(h, hipMemset(h->d_st, 0, sizeof(DevStatus)));
+	HIPCHK(h, hipMemset(h->d_phi2, 0, sizeof(double) * sites));
+	if(h->wet) {
+		HIPCHK(h, dalloc(&h->d_tile_cells, (size_t) tiles * MAXT));
+		HIPCHK(h, dalloc(&h->d_tile_count, tiles));
+		HIPCHK(h, dalloc(&h->d_diag, sites));
+		HIPCHK(h, dalloc(&h->d_sp, sites));
+		HIPCHK(h, dalloc(&h->d_part, (size_t) 6 * std::max(cells, 1)));
+		HIPCHK(h, dalloc(&h->d_cg, 1));
+		HIPCHK(h, hipMemset(h->d_cg, 0, sizeof(CGState)));
+	}
+	return RC_OK;
+}
+
+extern "C" void rc_destroy(rc_handle *h) {
+	if(!h) return;
+	if(h->stream) (void) hipStreamSynchronize(h->stream);
+	dfree(h->d_boxes); dfree(h->d_tile_box); dfree(h->d_cells); dfree(h->d_params);
+	dfree(h->d_phiA); dfree(h->d_phiB); dfree(h->d_dphi); dfree(h->d_rep);
+	dfree(h->d_v); dfree(h->d_f); dfree(h->d_r); dfree(h->d_p); dfree(h->d_trig);
+	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_count);
+	dfree(h->d_sp); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
+	for(auto p : h->d_out) dfree(p);
+	if(h->h_cg) (void) hipHostFree(h->h_cg);
+	if(h->h_st) (void) hipHostFree(h->h_st);
+	for(auto e : h->pool) (void) hipEventDestroy(e);
+	if(h->cg_e0) (void) hipEventDestroy(h->cg_e0);
+	if(h->cg_e1) (void) hipEventDestroy(h->cg_e1);
+	if(h->stream) (void) hipStreamDestroy(h->stream);
+	delete h;
+}
+
+// Patch trig table of MultiPhaseField::resizing (MultiPhaseField.cpp:54-65):
+// cos(2*PI*i/L) with the argument evaluated in float, cos/sin in double.
+static void trig_table(int L, std::vector<double> &out) {
+	for(int i = 0; i < L; i++) {
+		float a = 2 * PI_F * i / L;
+		out.push_back(std::cos((double) a));
+	}
+	for(int i = 0; i < L; i++) {
+		float a = 2 * PI_F * i / L;
+		out.push_back(std::sin((double) a));
+	}
+}
+
+extern "C" int rc_upload_cells(rc_handle *h, const rc_cell *cells, const double *phi) {
+	if(!h) return RC_ERR_ARG;
+	if(!cells || !phi) return fail(h, RC_ERR_ARG, "null cells/phi");
+	std::vector<DevCell> dc(h->ncells);
+	std::vector<double> trig;
+	std::map<int, int> trig_off;
+	long long rows = 0;
+	int maxS = 0, maxW = 0;
+	for(int b = 0; b < h->nboxes; ++b) {
+		const DevBox &B = h->dboxes[b];
+		for(int i = B.cell_begin; i < B.cell_end; ++i) {
+			const rc_cell &c = cells[i];
+			DevCell &d = dc[i];
+			if(c.LsubX < 3 || c.LsubY < 3 || c.LsubX > B.L || c.LsubY > B.L)
+				return fail(h, RC_ERR_ARG, "cell %d: patch %dx%d does not fit box %d", i, c.LsubX, c.LsubY, B.L);
+			if(c.offset[0] < 0 || c.offset[0] >= c.LsubX || c.offset[1] < 0 || c.offset[1] >= c.LsubY)
+				return fail(h, RC_ERR_ARG, "cell %d: offset out of range", i);
+			if(c.corner < 0 || c.corner >= B.L * B.L) return fail(h, RC_ERR_ARG, "cell %d: corner out of range", i);
+			d.LX = c.LsubX;
+			d.LY = c.LsubY;
+			d.S = c.LsubX * c.LsubY;
+			d.start = (int) rows;
+			d.offx = c.offset[0];
+			d.offy = c.offset[1];
+			d.corner = c.corner;
+			d.corner_old = c.corner_old;
+			d.x_sub_left = c.x_sub_left;
+			d.y_sub_bottom = c.y_sub_bottom;
+			d.box = b;
+			for(int side : { c.LsubX, c.LsubY }) {
+				if(!trig_off.count(side)) {
+					trig_off[side] = (int) trig.size();
+					trig_table(side, trig);
+				}
+			}
+			d.trig_x = trig_off[c.LsubX];
+			d.trig_y = trig_off[c.LsubY];
+			d.CoMx = c.CoM[0];
+			d.CoMy = c.CoM[1];
+			d.CoM_oldx = c.CoM_old[0];
+			d.CoM_oldy = c.CoM_old[1];
+			d.unrapx = c.unrap[0];
+			d.unrapy = c.unrap[1];
+			d.Q00 = c.Q00;
+			d.Q01 = c.Q01;
+			d.thetaQ = c.thetaQ;
+			d.thetaQ_old = c.thetaQ_old;
+			d.area = c.area;
+			d.area_old = c.area_old;
+			d.sumF = c.sumF;
+			d.sumF_old = c.sumF_old;
+			d.perimeter = c.perimeter;
+			d.S00 = c.S00;
+			d.S01 = c.S01;
+			d.trx = c.total_force_repulsion[0];
+			d.try_ = c.total_force_repulsion[1];
+			rows += d.S;
+			maxS = std::max(maxS, d.S);
+			maxW = std::max(maxW, (d.LX + 2) * (d.LY + 2));
+		}
+	}
+	if(rows >= (1ll << 31)) return fail(h, RC_ERR_ARG, "too many patch sites (%lld)", rows);
+	if(rows != h->rows) {
+		dfree(h->d_phiA); dfree(h->d_phiB); dfree(h->d_dphi); dfree(h->d_rep); dfree(h->d_v);
+		dfree(h->d_f); dfree(h->d_r); dfree(h->d_p);
+		for(auto &p : h->d_out) { dfree(p); p = nullptr; }
+		h->d_phiA = h->d_phiB = h->d_dphi = h->d_rep = nullptr;
+		h->d_v = h->d_f = h->d_r = h->d_p = nullptr;
+		HIPCHK(h, dalloc(&h->d_phiA, rows));
+		HIPCHK(h, dalloc(&h->d_phiB, rows));
+		HIPCHK(h, dalloc(&h->d_dphi, rows));
+		HIPCHK(h, dalloc(&h->d_v, rows));
+		if(h->jq) HIPCHK(h, dalloc(&h->d_rep, rows));
+		if(h->wet) {
+			HIPCHK(h, dalloc(&h->d_f, rows));
+			HIPCHK(h, dalloc(&h->d_r, rows));
+			HIPCHK(h, dalloc(&h->d_p, rows));
+		}
+		h->rows = rows;
+	}
+	if(h->jq) HIPCHK(h, hipMemsetAsync(h->d_rep, 0, sizeof(double) * rows, h->stream));
+	HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * rows, h->stream));
+	dfree(h->d_trig);
+	HIPCHK(h, dalloc(&h->d_trig, trig.size()));
+	HIPCHK(h, hipMemcpyAsync(h->d_trig, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice, h->stream));
+	HIPCHK(h, hipMemcpyAsync(h->d_cells, dc.data(), sizeof(DevCell) * h->ncells, hipMemcpyHostToDevice, h->stream));
+	HIPCHK(h, hipMemcpyAsync(h->d_phiA, phi, sizeof(double) * rows, hipMemcpyHostToDevice, h->stream));
+	HIPCHK(h, hipMemsetAsync(h->d_st, 0, sizeof(DevStatus), h->stream));
+	h->max_S = maxS;
+	h->max_W = maxW;
+	h->lds_stage = sizeof(double) * (size_t) (2 * maxS + maxW + (h->jq ? maxS : 0));
+	h->lds_cg = sizeof(double2) * (size_t) maxW;
+	if(h->lds_stage > 160 * 1024) return fail(h, RC_ERR_ARG, "patch of %d sites needs %zu B of LDS", maxS, h->lds_stage);
+	HIPCHK(h, stage_set_lds_limit(std::max<size_t>(h->lds_stage, 64 * 1024)));
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	h->uploaded = true;
+	h->initialized = false;
+	return RC_OK;
+}
+
+// ---------------------------------------------------------------------------
+static StageArgs stage_args(rc_handle *h) {
+	StageArgs a{};
+	a.boxes = h->d_boxes;
+	a.cells = h->d_cells;
+	a.params = h->d_params;
+	a.trig = h->d_trig;
+	a.phi2 = h->d_phi2;
+	a.vrows = h->d_v;
+	a.frows = h->d_f;
+	a.rep = h->d_rep;
+	a.jq = h->jq ? 1 : 0;
+	a.dphi_old = h->d_dphi;
+	a.st = h->d_st;
+	return a;
+}
+
+static int run_gather(rc_handle *h, const double *phi, bool lists) {
+	GatherArgs g{};
+	g.boxes = h->d_boxes;
+	g.tile_box = h->d_tile_box;
+	g.cells = h->d_cells;
+	g.phi = phi;
+	g.phi2 = h->d_phi2;
+	g.cover = h->d_cover;
+	g.tile_cells = h->d_tile_cells;
+	g.tile_count = h->d_tile_count;
+	g.store_lists = lists ? 1 : 0;
+	g.st = h->d_st;
+	TimeScope ts(h, 0);
+	HIPCHK(h, launch_gather(g, h->ntiles, h->stream));
+	return RC_OK;
+}
+
+static int run_stage(rc_handle *h, int mode, const StageArgs &a) {
+	TimeScope ts(h, 1);
+	HIPCHK(h, launch_stage(mode, a, h->ncells, h->lds_stage, h->stream));
+	return RC_OK;
+}
+
+static CGArgs cg_args(rc_handle *h) {
+	CGArgs c{};
+	c.boxes = h->d_boxes;
+	c.tile_box = h->d_tile_box;
+	c.cells = h->d_cells;
+	c.params = h->d_params;
+	c.set = h->cur_set;
+	c.tile_cells = h->d_tile_cells;
+	c.tile_count = h->d_tile_count;
+	c.cover = h->d_cover;
+	c.diag = h->d_diag;
+	c.sp = h->d_sp;
+	c.x = h->d_v;
+	c.r = h->d_r;
+	c.p = h->d_p;
+	c.b = h->d_f;
+	c.part = h->d_part;
+	c.cg = h->d_cg;
+	c.tol = h->boxes[0].params.CGtolerance > 0 ? h->boxes[0].params.CGtolerance : 2.220446049250313e-16;
+	const int mi = h->boxes[0].params.cg_max_iter;
+	c.max_iter = mi > 0 ? mi : (int) std::min<long long>(2 * h->rows, 1ll << 30);
+	c.ncells = h->ncells;
+	c.st = h->d_st;
+	return c;
+}
+
+// Eigen ConjugateGradient solve (solve() when cold, solveWithGuess() otherwise)
+static int cg_solve(rc_handle *h, bool cold) {
+	CGArgs c = cg_args(h);
+	hipEvent_t e0 = h->cg_e0, e1 = h->cg_e1;
+	HIPCHK(h, hipEventRecord(e0, h->stream));
+	{
+		TimeScope ts(h, 2);
+		if(cold) HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * h->rows, h->stream));
+		HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream));
+		HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
+		HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->lds_cg, h->stream));
+		HIPCHK(h, launch_cg_scalar(CG_SC_INIT, c, h->stream));
+		HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream));
+		int batch = std::max(2, h->last_iters - 1);
+		long long done_iters = 0;
+		for(;;) {
+			for(int it = 0; it < batch; ++it) {
+				HIPCHK(h, launch_cg_row(CG_ROW_SPMV, c, h->lds_cg, h->stream));
+				HIPCHK(h, launch_cg_scalar(CG_SC_ALPHA, c, h->stream));
+				HIPCHK(h, launch_cg_row(CG_ROW_UPDATE, c, h->lds_cg, h->stream));
+				HIPCHK(h, launch_cg_scalar(CG_SC_BETA, c, h->stream));
+				HIPCHK(h, launch_cg_gather(CG_GATHER_P_UPDATE, c, h->ntiles, h->stream));
+			}
+			done_iters += batch;
+			HIPCHK(h, hipMemcpyAsync(h->h_cg, h->d_cg, sizeof(CGState), hipMemcpyDeviceToHost, h->stream));
+			HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
+			HIPCHK(h, hipStreamSynchronize(h->stream));
+			if(h->h_st->flags) break;
+			if(!h->h_cg->active) break;
+			if(done_iters >= c.max_iter + 2) break;
+			batch = 2;
+		}
+		const CGState &s = *h->h_cg;
+		int mask = (s.zero_rhs[0] ? 1 : 0) | (s.zero_rhs[1] ? 2 : 0);
+		if(mask) HIPCHK(h, launch_zero_rows(h->d_v, h->rows, mask, h->stream));
+	}
+	HIPCHK(h, hipEventRecord(e1, h->stream));
+	HIPCHK(h, hipEventSynchronize(e1));
+	float ms = 0.f;
+	(void) hipEventElapsedTime(&ms, e0, e1);
+	const CGState &s = *h->h_cg;
+	h->cgs.solves += 2;
+	h->cgs.iters_total += s.iters[0] + s.iters[1];
+	h->cgs.last_iters[0] = s.iters[0];
+	h->cgs.last_iters[1] = s.iters[1];
+	for(int k = 0; k < 2; ++k) h->cgs.last_relres[k] = s.bb[k] > 0 ? std::sqrt(s.rn2[k] / s.bb[k]) : 0.;
+	h->cgs.last_cold = cold ? 1 : 0;
+	h->cgs.last_solve_ms = ms;
+	h->cgs.total_solve_ms += ms;
+	h->last_iters = std::max(s.iters[0], s.iters[1]);
+	if(s.iters[0] >= c.max_iter || s.iters[1] >= c.max_iter) h->cgs.last_cold |= 2;
+	return RC_OK;
+}
+
+// WetPolarModel::begin_energy_computation(fields) on the wet path: box sums,
+// right-hand side (or dry velocity) and, for the wet solve, the CG.
+static int force_eval(rc_handle *h, double *phi, bool pure) {
+	int rc;
+	if((rc = run_gather(h, phi, h->wet))) return rc;
+	StageArgs a = stage_args(h);
+	a.cf_set = h->cur_set;
+	a.fs_set = h->cur_set;
+	a.phi_c = phi;
+	a.pure = pure ? 1 : 0;
+	if((rc = run_stage(h, MODE_CF, a))) return rc;
+	// restart bookkeeping of WetPolarModel.cpp:151-157
+	if(h->restart_solver < 10 || h->warm_mode == RC_WARM_PREVIOUS) h->size_rows_old = h->size_rows;
+	else {
+		h->size_rows_old = 0;
+		h->restart_solver = 0;
+	}
+	h->size_rows = h->rows;
+	const bool cold = h->size_rows != h->size_rows_old;
+	if(h->dparams[2 * 0 + h->cur_set].friction_cell != 0) {
+		if(cold) h->restart_solver = 0;
+		else h->restart_solver += 1;
+		if((rc = cg_solve(h, cold))) return rc;
+	}
+	h->cf_set = h->cur_set;
+	return RC_OK;
+}
+
+static int check_status(rc_handle *h) {
+	HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	const unsigned f = h->h_st->flags;
+	if(f & ST_OVERLAP) return fail(h, RC_ERR_OVERLAP, "Too many field patches overlap at box site %d", h->h_st->site);
+	if(f & ST_TILE) return fail(h, RC_ERR_TILE, "more than %d patches intersect one 32x8 box tile", MAXT);
+	if(f & ST_GROWTH)
+		return fail(h, RC_ERR_GROWTH, "patch of cell %d must grow (MultiPhaseField::set_positions growth branch)", h->h_st->first_cell);
+	return RC_OK;
+}
+
+extern "C" int rc_init_forces(rc_handle *h) {
+	if(!h) return RC_ERR_ARG;
+	if(!h->uploaded) return fail(h, RC_ERR_STATE, "rc_init_forces before rc_upload_cells");
+	h->cur_set = 0;
+	h->cf_set = 0;
+	h->restart_solver = 0;
+	h->size_rows = 0;
+	h->size_rows_old = 0;
+	if(h->wet) {
+		int rc = force_eval(h, h->d_phiA, false);
+		if(rc) return rc;
+	}
+	h->initialized = true;
+	return check_status(h);
+}
+
+extern "C" int rc_apply_changes_after_equilibration(rc_handle *h) {
+	if(!h) return RC_ERR_ARG;
+	h->cur_set = 1;
+	return RC_OK;
+}
+
+extern "C" int rc_set_warm_start(rc_handle *h, int mode) {
+	if(!h) return RC_ERR_ARG;
+	if(mode != RC_WARM_REFERENCE && mode != RC_WARM_PREVIOUS) return fail(h, RC_ERR_ARG, "unknown warm-start mode %d", mode);
+	h->warm_mode = mode;
+	return RC_OK;
+}
+
+static int step_dry(rc_handle *h, const double *gauss_dev) {
+	int rc;
+	// predictor
+	if((rc = run_gather(h, h->d_phiA, false))) return rc;
+	StageArgs a = stage_args(h);
+	a.cf_set = h->cf_set;
+	a.fs_set = h->cur_set;
+	a.store = 1;
+	a.phi_c = h->d_phiA;
+	a.phi_out = h->d_phiB;
+	a.gauss = gauss_dev;
+	if((rc = run_stage(h, MODE_FUSED, a))) return rc;
+	// corrector
+	if((rc = run_gather(h, h->d_phiB, false))) return rc;
+	a.cf_set = h->cur_set;
+	a.store = 0;
+	a.phi_c = h->d_phiB;
+	a.phi_old = h->d_phiA;
+	a.phi_out = h->d_phiA;
+	a.gauss = nullptr;
+	if((rc = run_stage(h, MODE_FUSED, a))) return rc;
+	h->cf_set = h->cur_set;
+	return RC_OK;
+}
+
+static int step_wet(rc_handle *h, const double *gauss_dev) {
+	int rc;
+	StageArgs a = stage_args(h);
+	a.cf_set = h->cf_set;
+	a.fs_set = h->cur_set;
+	a.store = 1;
+	a.phi_c = h->d_phiA;
+	a.phi_out = h->d_phiB;
+	a.gauss = gauss_dev;
+	if((rc = run_stage(h, MODE_FS, a))) return rc;
+	if((rc = force_eval(h, h->d_phiB, false))) return rc;
+	a.cf_set = h->cf_set;
+	a.fs_set = h->cur_set;
+	a.store = 0;
+	a.phi_c = h->d_phiB;
+	a.phi_old = h->d_phiA;
+	a.phi_out = h->d_phiA;
+	a.gauss = nullptr;
+	if((rc = run_stage(h, MODE_FS, a))) return rc;
+	if((rc = force_eval(h, h->d_phiA, false))) return rc;
+	return RC_OK;
+}
+
+extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
+	if(!h) return RC_ERR_ARG;
+	if(!h->initialized) return fail(h, RC_ERR_STATE, "rc_step before rc_init_forces");
+	if(nsteps < 0) return fail(h, RC_ERR_ARG, "negative step count");
+	const int chunk = 256;
+	hipEvent_t t0 = nullptr, t1 = nullptr;
+	if(h->timing) {
+		t0 = ev_get(h);
+		t1 = ev_get(h);
+		HIPCHK(h, hipEventRecord(t0, h->stream));
+	}
+	for(int s0 = 0; s0 < nsteps; s0 += chunk) {
+		const int ns = std::min(chunk, nsteps - s0);
+		if(gauss) {
+			const long long need = (long long) ns * h->ncells;
+			if(need > h->gauss_cap) {
+				HIPCHK(h, hipStreamSynchronize(h->stream));
+				dfree(h->d_gauss);
+				h->d_gauss = nullptr;
+				HIPCHK(h, dalloc(&h->d_gauss, need));
+				h->gauss_cap = need;
+			}
+			HIPCHK(h, hipMemcpyAsync(h->d_gauss, gauss + (size_t) s0 * h->ncells, sizeof(double) * need,
+					hipMemcpyHostToDevice, h->stream));
+		}
+		for(int s = 0; s < ns; ++s) {
+			const double *g = gauss ? h->d_gauss + (size_t) s * h->ncells : nullptr;
+			int rc = h->wet ? step_wet(h, g) : step_dry(h, g);
+			if(rc) return rc;
+		}
+		if(gauss && s0 + chunk < nsteps) HIPCHK(h, hipStreamSynchronize(h->stream));  // d_gauss is reused
+	}
+	if(h->timing) HIPCHK(h, hipEventRecord(t1, h->stream));
+	int rc = check_status(h);
+	if(h->timing) {
+		float ms = 0.f;
+		(void) hipEventElapsedTime(&ms, t0, t1);
+		h->t_ms[3] += ms;
+		h->t_n[3] += 1;
+		collect_timing(h);
+	}
+	return rc;
+}
+
+// the force evaluation of the current state without side effects (dry model)
+static int materialize(rc_handle *h, bool site_outputs) {
+	if(!h->initialized) return fail(h, RC_ERR_STATE, "no force evaluation before rc_init_forces");
+	if(h->wet && !site_outputs) return RC_OK;
+	if(site_outputs) {
+		for(auto &p : h->d_out)
+			if(!p) HIPCHK(h, dalloc(&p, h->rows));
+	}
+	int rc;
+	if((rc = run_gather(h, h->d_phiA, false))) return rc;
+	StageArgs a = stage_args(h);
+	a.cf_set = h->cf_set;
+	a.fs_set = h->cf_set;
+	a.phi_c = h->d_phiA;
+	a.pure = 1;
+	if(h->wet) {
+		// keep the solved velocity: divert the force rows to scratch
+		a.frows = h->d_r;
+		a.vrows = h->d_r;
+	}
+	if(site_outputs) {
+		a.out_mu = h->d_out[0];
+		a.out_dx = h->d_out[1];
+		a.out_dy = h->d_out[2];
+		a.out_lap = h->d_out[3];
+	}
+	if((rc = run_stage(h, MODE_CF, a))) return rc;
+	return check_status(h);
+}
+
+extern "C" int rc_download_cells(rc_handle *h, rc_cell *cells, double *phi) {
+	if(!h) return RC_ERR_ARG;
+	if(!h->uploaded) return fail(h, RC_ERR_STATE, "no cells uploaded");
+	int rc;
+	if(h->initialized && !h->wet)
+		if((rc = materialize(h, false))) return rc;
+	std::vector<DevCell> dc(h->ncells);
+	HIPCHK(h, hipMemcpyAsync(dc.data(), h->d_cells, sizeof(DevCell) * h->ncells, hipMemcpyDeviceToHost, h->stream));
+	if(phi) HIPCHK(h, hipMemcpyAsync(phi, h->d_phiA, sizeof(double) * h->rows, hipMemcpyDeviceToHost, h->stream));
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	if(cells) {
+		for(int i = 0; i < h->ncells; ++i) {
+			const DevCell &d = dc[i];
+			rc_cell &c = cells[i];
+			c.LsubX = d.LX;
+			c.LsubY = d.LY;
+			c.offset[0] = d.offx;
+			c.offset[1] = d.offy;
+			c.corner = d.corner;
+			c.corner_old = d.corner_old;
+			c.x_sub_left = d.x_sub_left;
+			c.y_sub_bottom = d.y_sub_bottom;
+			c.CoM[0] = d.CoMx;
+			c.CoM[1] = d.CoMy;
+			c.CoM_old[0] = d.CoM_oldx;
+			c.CoM_old[1] = d.CoM_oldy;
+			c.unrap[0] = d.unrapx;
+			c.unrap[1] = d.unrapy;
+			c.Q00 = d.Q00;
+			c.Q01 = d.Q01;
+			c.thetaQ = d.thetaQ;
+			c.thetaQ_old = d.thetaQ_old;
+			c.area = d.area;
+			c.area_old = d.area_old;
+			c.sumF = d.sumF;
+			c.sumF_old = d.sumF_old;
+			c.perimeter = d.perimeter;
+			c.S00 = d.S00;
+			c.S01 = d.S01;
+			c.total_force_repulsion[0] = d.trx;
+			c.total_force_repulsion[1] = d.try_;
+		}
+	}
+	return RC_OK;
+}
+
+extern "C" int rc_get_rows(rc_handle *h, int which, double *out) {
+	if(!h) return RC_ERR_ARG;
+	if(!out || which < 0 || which > 3) return fail(h, RC_ERR_ARG, "bad rows request %d", which);
+	if(which >= 2 && !h->wet) return fail(h, RC_ERR_ARG, "force rows exist only for the wet model");
+	int rc;
+	if(!h->wet && (rc = materialize(h, false))) return rc;
+	std::vector<double2> tmp(h->rows);
+	const double2 *src = which < 2 ? h->d_v : h->d_f;
+	HIPCHK(h, hipMemcpyAsync(tmp.data(), src, sizeof(double2) * h->rows, hipMemcpyDeviceToHost, h->stream));
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	for(long long i = 0; i < h->rows; ++i) out[i] = (which & 1) ? tmp[i].y : tmp[i].x;
+	return RC_OK;
+}
+
+extern "C" int rc_get_site_array(rc_handle *h, int which, double *out) {
+	if(!h) return RC_ERR_ARG;
+	if(!out || which < 0 || which > 4) return fail(h, RC_ERR_ARG, "bad site array %d", which);
+	int rc;
+	if(which == 0) {
+		HIPCHK(h, hipMemcpyAsync(out, h->d_phiA, sizeof(double) * h->rows, hipMemcpyDeviceToHost, h->stream));
+		HIPCHK(h, hipStreamSynchronize(h->stream));
+		return RC_OK;
+	}
+	if((rc = materialize(h, true))) return rc;
+	HIPCHK(h, hipMemcpyAsync(out, h->d_out[which - 1], sizeof(double) * h->rows, hipMemcpyDeviceToHost, h->stream));
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	return RC_OK;
+}
+
+extern "C" int rc_total_rows(rc_handle *h, long long *rows) {
+	if(!h || !rows) return RC_ERR_ARG;
+	*rows = h->rows;
+	return RC_OK;
+}
+
+extern "C" int rc_cg_stats(rc_handle *h, rc_cg_stats_t *out) {
+	if(!h || !out) return RC_ERR_ARG;
+	*out = h->cgs;
+	return RC_OK;
+}
+
+extern "C" int rc_enable_timing(rc_handle *h, int on) {
+	if(!h) return RC_ERR_ARG;
+	h->timing = on != 0;
+	for(int k = 0; k < 4; ++k) {
+		h->t_ms[k] = 0;
+		h->t_n[k] = 0;
+	}
+	return RC_OK;
+}
+
+extern "C" int rc_timing(rc_handle *h, int kind, double *ms, long long *launches) {
+	if(!h || kind < 0 || kind > 3) return RC_ERR_ARG;
+	if(ms) *ms = h->t_ms[kind];
+	if(launches) *launches = h->t_n[kind];
+	return RC_OK;
+}

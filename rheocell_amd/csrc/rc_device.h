@@ -1,0 +1,65 @@
+// rc_device.h -- device-side data model of the MI355X RheoCell hot path.
+//
+// HBM layout (one handle):
+//   rows   : every per-site vector is one array over all patch sites of all
+//            cells, cell-major in cell-index order, storage order inside a
+//            patch (the field_start_index layout of WetPolarModel.cpp:160-162).
+//   cells  : array of DevCell (per-cell scalars, BaseField.h:122-225).
+//   boxes  : each replica's L x L box-grid arrays at box_off (row-major y*L+x).
+//   tiles  : 32 x 8 box tiles, tile_off per box; list of intersecting cells.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace rc {
+
+constexpr float PI_F = 3.141592653589793238462643f;  // src/defs.h:4 keeps PI a float
+constexpr int TW = 32;           // box tile width  (one 64-lane wave = 2 rows)
+constexpr int TH = 8;            // box tile height (256 sites per tile)
+constexpr int MAXT = 128;        // cells intersecting one tile (bounded by 20 x 6)
+constexpr int STORE_MAX = 20;    // WetPolarModel.cpp:51
+constexpr int BORDER = 4;        // BaseField.cpp:5
+constexpr int CELL_THREADS = 256;
+constexpr int NRED = 11;         // per-cell reductions of the field update
+
+// status flags written by kernels; every kernel returns at once when set
+constexpr unsigned ST_OVERLAP = 1u;
+constexpr unsigned ST_GROWTH = 2u;
+constexpr unsigned ST_TILE = 4u;
+
+struct DevParams {
+	double gamma, lambda, omega, mu, kappa, a0, p0, beta, passive_alpha;
+	double zetaQ_self, zetaQ_inter, J_Q, friction, friction_cell, J0, dt, sqrt_dt;
+	int anchoring, pad;
+};
+
+struct DevCell {
+	int LX, LY, S, start;
+	int offx, offy, corner, corner_old;
+	int x_sub_left, y_sub_bottom, box, trig_x;
+	int trig_y, pad0;
+	double CoMx, CoMy, CoM_oldx, CoM_oldy;
+	double unrapx, unrapy;
+	double Q00, Q01, thetaQ, thetaQ_old;
+	double area, area_old, sumF, sumF_old, perimeter, S00, S01;
+	double trx, try_;
+};
+
+struct DevBox {
+	int L, box_off, tile_off, ntx, nty, cell_begin, cell_end, pad;
+};
+
+struct DevStatus {
+	unsigned flags;
+	int first_cell;
+	int site;
+	int count;
+};
+
+// CG state of the two simultaneous solves (x and y components share A)
+struct CGState {
+	double bb[2], thr[2], absNew[2], alpha[2], beta[2], rn2[2];
+	int done[2], iters[2], zero_rhs[2], max_iter, active;
+};
+
+}  // namespace rc

@@ -1,0 +1,80 @@
+// rc_kernels.h -- launch interface of the HIP kernels (rc_kernels.hip).
+#pragma once
+
+#include "rc_device.h"
+
+namespace rc {
+
+enum StageMode { MODE_FUSED = 0, MODE_CF = 1, MODE_FS = 2 };
+
+// Arguments of the per-cell stage kernel (one workgroup per cell).
+struct StageArgs {
+	const DevBox *boxes;
+	DevCell *cells;
+	const DevParams *params;     // params[2*box + set]
+	int cf_set, fs_set;          // parameter sets of the force evaluation / field update
+	int store;                   // 1: predictor (first_step(true)), 0: corrector
+	int pure;                    // MODE_CF: no side effects on cells / Repulsion
+	int jq;                      // accumulate Repulsion and CIL sums (J_Q != 0)
+	int write_v;                 // MODE_FUSED: also write v rows
+	const double *phi_c;         // phi entering the force evaluation
+	double *phi_out;             // field update output
+	const double *phi_old;       // corrector: phi_n
+	double *dphi_old;            // predictor: written, corrector: read
+	const double *phi2;          // box sum of phi^2
+	const double *trig;          // patch trig tables
+	double2 *vrows;              // velocity rows (read in MODE_FS, written by dry MODE_CF / write_v)
+	double2 *frows;              // wet MODE_CF: force rows (CG right-hand side)
+	double *rep;                 // Repulsion rows (jq)
+	double *out_mu, *out_dx, *out_dy, *out_lap;   // optional site outputs (MODE_CF)
+	const double *gauss;         // predictor noise, one per cell
+	DevStatus *st;
+};
+
+struct GatherArgs {
+	const DevBox *boxes;
+	const int *tile_box;
+	const DevCell *cells;
+	const double *phi;
+	double *phi2;
+	int *cover;
+	int *tile_cells;             // written when store_lists
+	int *tile_count;
+	int store_lists;
+	DevStatus *st;
+};
+
+enum CGGatherMode { CG_GATHER_X = 0, CG_GATHER_P_FIRST = 1, CG_GATHER_P_UPDATE = 2 };
+enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_SPMV = 1, CG_ROW_UPDATE = 2 };
+enum CGScalarMode { CG_SC_INIT = 0, CG_SC_ALPHA = 1, CG_SC_BETA = 2 };
+
+struct CGArgs {
+	const DevBox *boxes;
+	const int *tile_box;
+	const DevCell *cells;
+	const DevParams *params;     // wet parameters (params[2*0 + set])
+	int set;
+	const int *tile_cells;
+	const int *tile_count;
+	const int *cover;
+	double *diag;                // D(k) = xi + eta * sum of the 4 neighbour covers
+	double2 *sp;                 // box sums of the gathered vector
+	double2 *x, *r, *p, *b;
+	double *part;                // per-cell partial sums, 6 per cell
+	CGState *cg;
+	double tol;
+	int max_iter;
+	int ncells;
+	DevStatus *st;
+};
+
+hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s);
+hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds_bytes, hipStream_t s);
+hipError_t launch_cg_diag(const CGArgs &a, int nbox_sites, hipStream_t s);
+hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s);
+hipError_t launch_cg_row(int mode, const CGArgs &a, size_t lds_bytes, hipStream_t s);
+hipError_t launch_cg_scalar(int mode, const CGArgs &a, hipStream_t s);
+hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
+hipError_t stage_set_lds_limit(size_t bytes);
+
+}  // namespace rc

@@ -1,0 +1,753 @@
+// rc_kernels.hip -- CDNA4 (gfx950) kernels of the RheoCell per-step hot path.
+//
+// k_box_gather   WetPolarModel::computeGlobalSums box part
+//                (src/Interactions/WetPolarModel.cpp:236-260) as an
+//                atomic-free gather: one thread per box site, one workgroup
+//                per 32x8 box tile, covering patches visited in cell-index
+//                order so the sum order equals the reference scatter order.
+// k_stage<MODE>  one workgroup per cell, patch staged in LDS:
+//                BaseInteraction::derivX/derivY/Laplacian
+//                (src/Interactions/BaseInteraction.h:196-263),
+//                WetPolarModel::f_interaction / calc_internal_forces
+//                (WetPolarModel.cpp:262-339), the dry velocity v = f/xi
+//                (WetPolarModel.cpp:194-198), FD_CPUBackend::first_step
+//                (src/Backends/FD_CPUBackend.cpp:44-120) including the
+//                per-cell reductions, MultiPhaseField::set_positions
+//                recentering (src/Fields/MultiPhaseField.cpp:223-247) and
+//                WetPolarModel::updateDirectedActiveForces (:342-414).
+// k_cg_*         matrix-free Jacobi-PCG for the wet velocity solve
+//                (WetPolarModel.cpp:200-233); restates Eigen's
+//                ConjugateGradient iteration with both components at once.
+//
+// All arithmetic is FP64 and is compiled with -ffp-contract=off so that
+// every per-site expression is the IEEE evaluation of the reference source.
+#include "rc_kernels.h"
+
+#include <float.h>
+
+namespace rc {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ inline int box_x(int side, int site, int distX) {  // SquareBox.cpp:31-37
+	int x = (site - ((site / side) * side)) + distX;
+	while(x < 0) x += side;
+	while(x >= side) x -= side;
+	return x;
+}
+__device__ inline int box_y(int side, int site, int distY) {  // SquareBox.cpp:39-45
+	int y = (site / side) + distY;
+	while(y < 0) y += side;
+	while(y >= side) y -= side;
+	return y;
+}
+__device__ inline int box_elem(int side, int site, int dX, int dY) {
+	return box_x(side, site, dX) + box_y(side, site, dY) * side;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+	for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+	return v;
+}
+__device__ inline int wave_min(int v) {
+#pragma unroll
+	for(int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+	return v;
+}
+
+// Sum NV values over the 256-thread block; the result is valid in thread 0.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double *s_red) {
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+	for(int j = 0; j < NV; ++j) v[j] = wave_sum(v[j]);
+	if(lane == 0) {
+#pragma unroll
+		for(int j = 0; j < NV; ++j) s_red[wave * NV + j] = v[j];
+	}
+	__syncthreads();
+	if(threadIdx.x == 0) {
+#pragma unroll
+		for(int j = 0; j < NV; ++j) v[j] = ((s_red[j] + s_red[NV + j]) + s_red[2 * NV + j]) + s_red[3 * NV + j];
+	}
+	__syncthreads();
+}
+
+__device__ inline bool circ_overlap(int a0, int alen, int b0, int blen, int L) {
+	// do [a0, a0+alen) and [b0, b0+blen) intersect on the circle of length L?
+	int d1 = b0 - a0;
+	if(d1 < 0) d1 += L;
+	int d2 = a0 - b0;
+	if(d2 < 0) d2 += L;
+	return d1 < alen || d2 < blen;
+}
+
+// ---------------------------------------------------------------------------
+// k_box_gather: phi2[k] = sum over covering patches (cell order) of phi^2,
+// cover[k] = number of covering patches (the reference's overlap list size)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
+	if(a.st->flags) return;
+	__shared__ int s_cell[MAXT], s_cx[MAXT], s_cy[MAXT], s_lx[MAXT], s_ly[MAXT], s_ox[MAXT], s_oy[MAXT],
+			s_start[MAXT];
+	__shared__ int s_wcnt[4];
+	__shared__ int s_n;
+	const int tile = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const DevBox B = a.boxes[a.tile_box[tile]];
+	const int t = tile - B.tile_off, L = B.L;
+	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
+	const int tw = min(TW, L - x0), th = min(TH, L - y0);
+	if(tid == 0) s_n = 0;
+	__syncthreads();
+	for(int base = B.cell_begin; base < B.cell_end; base += 256) {
+		const int c = base + tid;
+		bool hit = false;
+		int cx = 0, cy = 0, lx = 0, ly = 0, ox = 0, oy = 0, st = 0;
+		if(c < B.cell_end) {
+			const DevCell &C = a.cells[c];
+			cx = C.corner % L;
+			cy = C.corner / L;
+			lx = C.LX; ly = C.LY; ox = C.offx; oy = C.offy; st = C.start;
+			hit = circ_overlap(cx, lx, x0, tw, L) && circ_overlap(cy, ly, y0, th, L);
+		}
+		const unsigned long long m = __ballot(hit);
+		const int pre = __popcll(m & ((1ull << lane) - 1ull));
+		if(lane == 0) s_wcnt[wave] = __popcll(m);
+		__syncthreads();
+		int off = s_n;
+		for(int w = 0; w < wave; ++w) off += s_wcnt[w];
+		const int pos = off + pre;
+		if(hit && pos < MAXT) {
+			s_cell[pos] = c; s_cx[pos] = cx; s_cy[pos] = cy; s_lx[pos] = lx; s_ly[pos] = ly;
+			s_ox[pos] = ox; s_oy[pos] = oy; s_start[pos] = st;
+		}
+		__syncthreads();
+		if(tid == 0) s_n += s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+		__syncthreads();
+	}
+	const int n = s_n;
+	if(n > MAXT) {
+		if(tid == 0) atomicOr(&a.st->flags, ST_TILE);
+		return;
+	}
+	if(a.store_lists) {
+		for(int j = tid; j < n; j += 256) a.tile_cells[tile * MAXT + j] = s_cell[j];
+		if(tid == 0) a.tile_count[tile] = n;
+	}
+	const int sx = tid % TW, sy = tid / TW;
+	if(sx >= tw || sy >= th) return;
+	const int x = x0 + sx, y = y0 + sy;
+	double acc = 0.0;
+	int cnt = 0;
+	for(int j = 0; j < n; ++j) {
+		int lx = x - s_cx[j];
+		if(lx < 0) lx += L;
+		int ly = y - s_cy[j];
+		if(ly < 0) ly += L;
+		const int LXj = s_lx[j], LYj = s_ly[j];
+		if(lx < LXj && ly < LYj) {
+			int qx = lx - s_ox[j];
+			if(qx < 0) qx += LXj;
+			int qy = ly - s_oy[j];
+			if(qy < 0) qy += LYj;
+			const double v = a.phi[s_start[j] + qy * LXj + qx];
+			acc += v * v;
+			++cnt;
+		}
+	}
+	const int k = B.box_off + y * L + x;
+	a.phi2[k] = acc;
+	a.cover[k] = cnt;
+	if(cnt > STORE_MAX) {
+		if(atomicOr(&a.st->flags, ST_OVERLAP) == 0u) a.st->site = k;
+	}
+}
+
+hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s) {
+	hipLaunchKernelGGL(k_box_gather, dim3(ntiles), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// k_stage<MODE>: force evaluation (+ field update) of one cell
+// ---------------------------------------------------------------------------
+struct SiteIdx {
+	int iE, iW, iN, iS, lx, ly, w;
+};
+
+__device__ inline SiteIdx site_index(int q, int LX, int LY, int offx, int offy, int WX) {
+	SiteIdx r;
+	const int qy = q / LX, qx = q - qy * LX;
+	const int xe = (qx + 1 == LX) ? 0 : qx + 1;
+	const int xw = (qx == 0) ? LX - 1 : qx - 1;
+	const int yn = (qy + 1 == LY) ? 0 : qy + 1;
+	const int ys = (qy == 0) ? LY - 1 : qy - 1;
+	r.iE = qy * LX + xe;
+	r.iW = qy * LX + xw;
+	r.iN = yn * LX + qx;
+	r.iS = ys * LX + qx;
+	int lx = qx + offx;
+	if(lx >= LX) lx -= LX;
+	int ly = qy + offy;
+	if(ly >= LY) ly -= LY;
+	r.lx = lx;
+	r.ly = ly;
+	r.w = (ly + 1) * WX + (lx + 1);
+	return r;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
+	if(a.st->flags) return;
+	extern __shared__ double smem[];
+	__shared__ double s_red[4 * NRED];
+	__shared__ int s_redi[8];
+	__shared__ double s_bc;
+	const int ci = blockIdx.x, tid = threadIdx.x;
+	const DevCell C = a.cells[ci];
+	const DevBox B = a.boxes[C.box];
+	const DevParams P = a.params[2 * C.box + a.cf_set];
+	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L, WX = LX + 2, WY = LY + 2;
+	double *s_phi = smem;
+	double *s_win = smem + S;
+	double *s_mu = s_win + WX * WY;
+	double *s_rep = s_mu + S;
+	const bool do_rep = a.jq && MODE != MODE_FS && !a.pure;
+
+	// stage the patch and the (LX+2)x(LY+2) box window of sum phi^2
+	const double *phic = a.phi_c + C.start;
+	for(int q = tid; q < S; q += CELL_THREADS) s_phi[q] = phic[q];
+	const int cx = C.corner % L, cy = C.corner / L;
+	for(int w = tid; w < WX * WY; w += CELL_THREADS) {
+		const int wy = w / WX, wx = w - wy * WX;
+		int bx = cx + wx - 1;
+		if(bx < 0) bx += L;
+		else if(bx >= L) bx -= L;
+		int by = cy + wy - 1;
+		if(by < 0) by += L;
+		else if(by >= L) by -= L;
+		s_win[w] = a.phi2[B.box_off + by * L + bx];
+	}
+	__syncthreads();
+
+	// phase 1: perimeter = sum |grad phi|^2 (computeGlobalSums, WetPolarModel.cpp:243-246)
+	{
+		double per[1] = { 0.0 };
+		for(int q = tid; q < S; q += CELL_THREADS) {
+			const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
+			const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
+			const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
+			per[0] += dx * dx + dy * dy;
+		}
+		block_sum<1>(per, s_red);
+		if(tid == 0) s_bc = per[0];
+		__syncthreads();
+	}
+	const double perimeter = s_bc;
+
+	// phase 2: chemical potential mu = dF/dphi (f_interaction, WetPolarModel.cpp:262-301)
+	for(int q = tid; q < S; q += CELL_THREADS) {
+		const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
+		const double ph = s_phi[q];
+		const double lap = s_phi[I.iN] + s_phi[I.iS] + s_phi[I.iE] + s_phi[I.iW] - 4 * ph;
+		const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
+		const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
+		const double p2 = s_win[I.w];
+		const double laplacianSquare = s_win[I.w + 1] + s_win[I.w + WX] + s_win[I.w - 1] + s_win[I.w - WX] - 4. * p2;
+		const double CH = P.gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / P.lambda - 2 * P.lambda * lap);
+		const double A = -4 * (P.mu / P.a0) * (1 - C.area / P.a0) * ph;
+		const double Rep = 4 * (P.kappa / P.lambda) * ph * (p2 - ph * ph);
+		const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
+		const double dd = laplacianSquare - lsquare;
+		const double suppress = dd / sqrt(1 + dd * dd);
+		const double Adh = -4 * P.lambda * P.omega * suppress * ph;
+		const double Pp = 4 * (P.beta / P.p0) * (1 - perimeter / P.p0) * lap;
+		const double V = CH + A + Rep + Adh + Pp;
+		s_mu[q] = V;
+		if(MODE == MODE_CF && a.out_mu) {
+			a.out_mu[C.start + q] = V;
+			a.out_dx[C.start + q] = dx;
+			a.out_dy[C.start + q] = dy;
+			a.out_lap[C.start + q] = lap;
+		}
+		if(do_rep) {
+			const double rn = a.rep[C.start + q] + Rep;
+			a.rep[C.start + q] = rn;
+			s_rep[q] = rn;
+		}
+	}
+	__syncthreads();
+
+	// phase 3: forces (calc_internal_forces) and, unless MODE_CF, the Heun update
+	const DevParams Q = a.params[2 * C.box + a.fs_set];
+	double red[NRED];
+#pragma unroll
+	for(int j = 0; j < NRED; ++j) red[j] = 0.0;
+	int xmin = LX, ymin = LY;
+	const double *cosx = a.trig + C.trig_x, *sinx = cosx + LX;
+	const double *cosy = a.trig + C.trig_y, *siny = cosy + LY;
+	const bool dry_cf = P.friction_cell == 0;
+	for(int q = tid; q < S; q += CELL_THREADS) {
+		const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
+		const int row = C.start + q;
+		const double ph = s_phi[q];
+		const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
+		const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
+		double vx, vy;
+		if(MODE == MODE_FS) {
+			const double2 v = a.vrows[row];
+			vx = v.x;
+			vy = v.y;
+		}
+		else {
+			const double f_passive_x = (-1) * 0.5 * (s_mu[I.iE] - s_mu[I.iW]);
+			const double f_passive_y = (-1) * 0.5 * (s_mu[I.iN] - s_mu[I.iS]);
+			if(do_rep) {
+				red[8] += (-1) * 0.5 * (s_rep[I.iE] - s_rep[I.iW]) * ph;
+				red[9] += (-1) * 0.5 * (s_rep[I.iN] - s_rep[I.iS]) * ph;
+			}
+			double Theta = (-dx * C.Q00) < 0 ? 0.0 : 1.0;
+			const double fQx = P.lambda * (-dx * C.Q00) * Theta * C.Q00;
+			Theta = (-dy * C.Q01) < 0 ? 0.0 : 1.0;
+			const double fQy = P.lambda * (-dy * C.Q01) * Theta * C.Q01;
+			const double fx = f_passive_x * P.passive_alpha + fQx * P.zetaQ_self;
+			const double fy = f_passive_y * P.passive_alpha + fQy * P.zetaQ_self;
+			vx = fx / P.friction;
+			vy = fy / P.friction;
+			if(MODE == MODE_CF) {
+				if(dry_cf) a.vrows[row] = make_double2(vx, vy);
+				else a.frows[row] = make_double2(fx, fy);
+				continue;
+			}
+			if(a.write_v) a.vrows[row] = make_double2(vx, vy);
+		}
+		// FD_CPUBackend::first_step body (FD_CPUBackend.cpp:66-101)
+		const double mu = s_mu[q];
+		const double dphi = -Q.J0 * mu - vx * dx - vy * dy;
+		double phn;
+		if(a.store) {
+			a.dphi_old[row] = dphi;
+			phn = ph + Q.dt * .5 * (dphi + dphi);
+		}
+		else {
+			phn = a.phi_old[row] + Q.dt * .5 * (dphi + a.dphi_old[row]);
+		}
+		a.phi_out[row] = phn;
+		red[0] += phn * phn;
+		red[1] += phn;
+		red[2] += phn * cosx[I.lx];
+		red[3] += phn * sinx[I.lx];
+		red[4] += phn * cosy[I.ly];
+		red[5] += phn * siny[I.ly];
+		if(!a.store && phn < 0.09 && phn > 0.01) {
+			xmin = min(xmin, I.lx);
+			ymin = min(ymin, I.ly);
+		}
+		red[6] += -0.5 * (dx * dx - dy * dy);
+		red[7] += -dx * dy;
+	}
+
+	if(MODE == MODE_CF) {
+		if(a.pure) {
+			if(tid == 0) a.cells[ci].perimeter = perimeter;
+			return;
+		}
+		double tr[2] = { red[8], red[9] };
+		block_sum<2>(tr, s_red);
+		if(tid == 0) {
+			DevCell &G = a.cells[ci];
+			G.perimeter = perimeter;
+			G.trx = tr[0];
+			G.try_ = tr[1];
+		}
+		return;
+	}
+
+	block_sum<NRED>(red, s_red);
+	{
+		const int lane = tid & 63, wave = tid >> 6;
+		const int mx = wave_min(xmin), my = wave_min(ymin);
+		if(lane == 0) {
+			s_redi[wave] = mx;
+			s_redi[4 + wave] = my;
+		}
+		__syncthreads();
+	}
+	if(tid != 0) return;
+
+	// ---- per-cell tail, one lane ----
+	DevCell &G = a.cells[ci];
+	const double trx = MODE == MODE_FUSED ? red[8] : C.trx;
+	const double try_ = MODE == MODE_FUSED ? red[9] : C.try_;
+	if(a.store) {
+		G.CoM_oldx = C.CoMx;
+		G.CoM_oldy = C.CoMy;
+		G.area_old = C.area;
+		G.sumF_old = C.sumF;
+	}
+	const double area = red[0], sumF = red[1];
+	G.area = area;
+	G.sumF = sumF;
+	G.S00 = red[6];
+	G.S01 = red[7];
+	G.perimeter = 0.;
+	int xsl = LX, ysb = LY;
+	if(!a.store) {
+		xsl = min(min(s_redi[0], s_redi[1]), min(s_redi[2], s_redi[3]));
+		ysb = min(min(s_redi[4], s_redi[5]), min(s_redi[6], s_redi[7]));
+	}
+	G.x_sub_left = xsl;
+	G.y_sub_bottom = ysb;
+	double CoMx = LX * (atan2(-red[3] / sumF, -red[2] / sumF) + PI_F) / (2 * PI_F);
+	double CoMy = LY * (atan2(-red[5] / sumF, -red[4] / sumF) + PI_F) / (2 * PI_F);
+	if(!a.store) {
+		// MultiPhaseField::set_positions (MultiPhaseField.cpp:223-247)
+		if(xsl >= BORDER && ysb >= BORDER) {
+			const int site = box_elem(L, C.corner, (int) CoMx, (int) CoMy);
+			CoMy = (int) (site / L) + (CoMy - (int) CoMy);
+			CoMx = (int) (site - (int) (site / L) * L) + (CoMx - (int) CoMx);
+			const double co0 = box_x(L, C.corner, 0), co1 = box_y(L, C.corner, 0);
+			const int corner = box_elem(L, site, (int) -LX / 2, (int) -LY / 2);
+			const double cn0 = box_x(L, corner, 0), cn1 = box_y(L, corner, 0);
+			const double d0 = cn0 - co0 - rint((cn0 - co0) / L) * L;
+			const double d1 = cn1 - co1 - rint((cn1 - co1) / L) * L;
+			G.corner_old = C.corner;
+			G.corner = corner;
+			G.unrapx = C.unrapx + d0;
+			G.unrapy = C.unrapy + d1;
+			G.offx = (int) (C.offx + LX - d0) % LX;
+			G.offy = (int) (C.offy + LY - d1) % LY;
+		}
+		else {
+			// patch growth: left to the host (flag and halt)
+			if(atomicOr(&a.st->flags, ST_GROWTH) == 0u) a.st->first_cell = ci;
+		}
+	}
+	G.CoMx = CoMx;
+	G.CoMy = CoMy;
+	// WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:365-396)
+	double th_old = C.thetaQ_old;
+	if(a.store) {
+		const double g = a.gauss ? a.gauss[ci] : 0.0;
+		th_old = C.thetaQ + Q.sqrt_dt * Q.zetaQ_inter * g;
+	}
+	const double th = th_old - Q.dt * Q.J_Q * (atan2(trx * C.Q01 - try_ * C.Q00, trx * C.Q00 + try_ * C.Q01));
+	G.thetaQ_old = th_old;
+	G.thetaQ = th;
+	double Q00 = cos(th), Q01 = sin(th);
+	if(Q.anchoring) {  // update_anchoring (WetPolarModel.cpp:417-430)
+		const double walls_length = 8;
+		const double com_old_y = a.store ? C.CoMy : C.CoM_oldy;
+		const double dist1 = (com_old_y - walls_length);
+		const double dist2 = ((double) L - walls_length) - com_old_y;
+		double theta;
+		if(dist1 < dist2) theta = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
+		else theta = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) L - 2 * walls_length) / 2));
+		const double n0 = cos(theta), n1 = sin(theta);
+		Q00 = 0.5 * (n0 * n0 - n1 * n1);
+		Q01 = n0 * n1;
+	}
+	G.Q00 = Q00;
+	G.Q01 = Q01;
+	if(MODE == MODE_FUSED) {
+		G.trx = trx;
+		G.try_ = try_;
+	}
+}
+
+hipError_t stage_set_lds_limit(size_t bytes) {
+	hipError_t e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_FUSED>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_CF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
+	return hipFuncSetAttribute((const void *) k_stage<MODE_FS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes);
+}
+
+hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
+	switch(mode) {
+	case MODE_FUSED: hipLaunchKernelGGL(k_stage<MODE_FUSED>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	case MODE_CF: hipLaunchKernelGGL(k_stage<MODE_CF>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	default: hipLaunchKernelGGL(k_stage<MODE_FS>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	}
+	return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Wet velocity solve: A v = f, (A v)_i = D(k_i) v_i - eta * sum_{d in S,W,E,N}
+// Sv(nbr_d(k_i)), Sv = box sum of v over the rows living at a site.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cg_diag(CGArgs a, int nsites) {
+	const int k = blockIdx.x * 256 + threadIdx.x;
+	if(k >= nsites) return;
+	const DevBox B = a.boxes[0];
+	const DevParams P = a.params[a.set];
+	const int L = B.L, y = k / L, x = k - y * L;
+	const int xe = x + 1 == L ? 0 : x + 1, xw = x == 0 ? L - 1 : x - 1;
+	const int yn = y + 1 == L ? 0 : y + 1, ys = y == 0 ? L - 1 : y - 1;
+	// triplet order of WetPolarModel.cpp:206-215: neighbours 1,3,5,7 = S,W,E,N
+	const int nb[4] = { ys * L + x, y * L + xw, y * L + xe, yn * L + x };
+	double d = P.friction;
+	for(int j = 0; j < 4; ++j) {
+		const int m = a.cover[B.box_off + nb[j]];
+		for(int l = 0; l < m; ++l) d += P.friction_cell * 1.0;
+	}
+	a.diag[B.box_off + k] = d;
+}
+
+hipError_t launch_cg_diag(const CGArgs &a, int nsites, hipStream_t s) {
+	hipLaunchKernelGGL(k_cg_diag, dim3((nsites + 255) / 256), dim3(256), 0, s, a, nsites);
+	return hipGetLastError();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
+	if(a.st->flags) return;
+	const CGState *cg = a.cg;
+	if(MODE != CG_GATHER_X && !cg->active) return;
+	__shared__ int s_cx[MAXT], s_cy[MAXT], s_lx[MAXT], s_ly[MAXT], s_ox[MAXT], s_oy[MAXT], s_start[MAXT];
+	const int tile = blockIdx.x, tid = threadIdx.x;
+	const DevBox B = a.boxes[a.tile_box[tile]];
+	const int t = tile - B.tile_off, L = B.L;
+	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
+	const int tw = min(TW, L - x0), th = min(TH, L - y0);
+	const int n = a.tile_count[tile];
+	for(int j = tid; j < n; j += 256) {
+		const DevCell &C = a.cells[a.tile_cells[tile * MAXT + j]];
+		s_cx[j] = C.corner % L; s_cy[j] = C.corner / L; s_lx[j] = C.LX; s_ly[j] = C.LY;
+		s_ox[j] = C.offx; s_oy[j] = C.offy; s_start[j] = C.start;
+	}
+	__syncthreads();
+	const int sx = tid % TW, sy = tid / TW;
+	if(sx >= tw || sy >= th) return;
+	const int x = x0 + sx, y = y0 + sy;
+	const int k = B.box_off + y * L + x;
+	double inv = 0., b0 = 0., b1 = 0.;
+	bool d0 = false, d1 = false;
+	if(MODE != CG_GATHER_X) {
+		inv = 1. / a.diag[k];
+		d0 = cg->done[0]; d1 = cg->done[1];
+		b0 = cg->beta[0]; b1 = cg->beta[1];
+	}
+	double2 acc = make_double2(0., 0.);
+	for(int j = 0; j < n; ++j) {
+		int lx = x - s_cx[j];
+		if(lx < 0) lx += L;
+		int ly = y - s_cy[j];
+		if(ly < 0) ly += L;
+		const int LXj = s_lx[j], LYj = s_ly[j];
+		if(lx < LXj && ly < LYj) {
+			int qx = lx - s_ox[j];
+			if(qx < 0) qx += LXj;
+			int qy = ly - s_oy[j];
+			if(qy < 0) qy += LYj;
+			const int row = s_start[j] + qy * LXj + qx;
+			double2 v;
+			if(MODE == CG_GATHER_X) v = a.x[row];
+			else {
+				const double2 rr = a.r[row];
+				if(MODE == CG_GATHER_P_FIRST) {
+					v = make_double2(inv * rr.x, inv * rr.y);
+				}
+				else {
+					const double2 po = a.p[row];
+					v.x = d0 ? po.x : inv * rr.x + b0 * po.x;
+					v.y = d1 ? po.y : inv * rr.y + b1 * po.y;
+				}
+				a.p[row] = v;
+			}
+			acc.x += v.x;
+			acc.y += v.y;
+		}
+	}
+	a.sp[k] = acc;
+}
+
+hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s) {
+	switch(mode) {
+	case CG_GATHER_X: hipLaunchKernelGGL(k_cg_gather<CG_GATHER_X>, dim3(ntiles), dim3(256), 0, s, a); break;
+	case CG_GATHER_P_FIRST: hipLaunchKernelGGL(k_cg_gather<CG_GATHER_P_FIRST>, dim3(ntiles), dim3(256), 0, s, a); break;
+	default: hipLaunchKernelGGL(k_cg_gather<CG_GATHER_P_UPDATE>, dim3(ntiles), dim3(256), 0, s, a); break;
+	}
+	return hipGetLastError();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
+	if(a.st->flags) return;
+	const CGState *cg = a.cg;
+	if(MODE != CG_ROW_RESID && !cg->active) return;
+	extern __shared__ double smem[];
+	__shared__ double s_red[4 * 6];
+	double2 *s_win = reinterpret_cast<double2 *>(smem);
+	const int ci = blockIdx.x, tid = threadIdx.x;
+	const DevCell C = a.cells[ci];
+	const DevBox B = a.boxes[C.box];
+	const double eta = a.params[a.set].friction_cell;
+	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L, WX = LX + 2, WY = LY + 2;
+	const int cx = C.corner % L, cy = C.corner / L;
+	for(int w = tid; w < WX * WY; w += 256) {
+		const int wy = w / WX, wx = w - wy * WX;
+		int bx = cx + wx - 1;
+		if(bx < 0) bx += L;
+		else if(bx >= L) bx -= L;
+		int by = cy + wy - 1;
+		if(by < 0) by += L;
+		else if(by >= L) by -= L;
+		s_win[w] = a.sp[B.box_off + by * L + bx];
+	}
+	double al0 = 0., al1 = 0.;
+	bool dn0 = false, dn1 = false;
+	if(MODE == CG_ROW_UPDATE) {
+		al0 = cg->alpha[0]; al1 = cg->alpha[1];
+		dn0 = cg->done[0]; dn1 = cg->done[1];
+	}
+	__syncthreads();
+	double red[6] = { 0., 0., 0., 0., 0., 0. };
+	for(int q = tid; q < S; q += 256) {
+		const int qy = q / LX, qx = q - qy * LX;
+		int lx = qx + C.offx;
+		if(lx >= LX) lx -= LX;
+		int ly = qy + C.offy;
+		if(ly >= LY) ly -= LY;
+		const int w = (ly + 1) * WX + (lx + 1);
+		int bx = cx + lx;
+		if(bx >= L) bx -= L;
+		int by = cy + ly;
+		if(by >= L) by -= L;
+		const double D = a.diag[B.box_off + by * L + bx];
+		const int row = C.start + q;
+		const double2 vin = MODE == CG_ROW_RESID ? a.x[row] : a.p[row];
+		const double2 sS = s_win[w - WX], sW = s_win[w - 1], sE = s_win[w + 1], sN = s_win[w + WX];
+		double2 t;
+		t.x = D * vin.x - eta * sS.x - eta * sW.x - eta * sE.x - eta * sN.x;
+		t.y = D * vin.y - eta * sS.y - eta * sW.y - eta * sE.y - eta * sN.y;
+		if(MODE == CG_ROW_RESID) {
+			const double2 bb = a.b[row];
+			const double2 rr = make_double2(bb.x - t.x, bb.y - t.y);
+			a.r[row] = rr;
+			const double inv = 1. / D;
+			red[0] += bb.x * bb.x; red[1] += bb.y * bb.y;
+			red[2] += rr.x * rr.x; red[3] += rr.y * rr.y;
+			red[4] += rr.x * (inv * rr.x); red[5] += rr.y * (inv * rr.y);
+		}
+		else if(MODE == CG_ROW_SPMV) {
+			red[0] += vin.x * t.x;
+			red[1] += vin.y * t.y;
+		}
+		else {
+			double2 xx = a.x[row], rr = a.r[row];
+			if(!dn0) { xx.x += al0 * vin.x; rr.x -= al0 * t.x; }
+			if(!dn1) { xx.y += al1 * vin.y; rr.y -= al1 * t.y; }
+			a.x[row] = xx;
+			a.r[row] = rr;
+			const double inv = 1. / D;
+			red[0] += rr.x * rr.x; red[1] += rr.y * rr.y;
+			red[2] += rr.x * (inv * rr.x); red[3] += rr.y * (inv * rr.y);
+		}
+	}
+	block_sum<6>(red, s_red);
+	if(tid == 0) {
+#pragma unroll
+		for(int j = 0; j < 6; ++j) a.part[ci * 6 + j] = red[j];
+	}
+}
+
+hipError_t launch_cg_row(int mode, const CGArgs &a, size_t lds, hipStream_t s) {
+	switch(mode) {
+	case CG_ROW_RESID: hipLaunchKernelGGL(k_cg_row<CG_ROW_RESID>, dim3(a.ncells), dim3(256), lds, s, a); break;
+	case CG_ROW_SPMV: hipLaunchKernelGGL(k_cg_row<CG_ROW_SPMV>, dim3(a.ncells), dim3(256), lds, s, a); break;
+	default: hipLaunchKernelGGL(k_cg_row<CG_ROW_UPDATE>, dim3(a.ncells), dim3(256), lds, s, a); break;
+	}
+	return hipGetLastError();
+}
+
+// one workgroup: deterministic reduction of the per-cell partials + scalar
+// recurrences of the CG (Eigen ConjugateGradient.h loop)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_cg_scalar(CGArgs a) {
+	if(a.st->flags) return;
+	CGState *cg = a.cg;
+	if(MODE != CG_SC_INIT && !cg->active) return;
+	__shared__ double s_red[4 * 6];
+	double red[6] = { 0., 0., 0., 0., 0., 0. };
+	for(int c = threadIdx.x; c < a.ncells; c += 256) {
+#pragma unroll
+		for(int j = 0; j < 6; ++j) red[j] += a.part[c * 6 + j];
+	}
+	block_sum<6>(red, s_red);
+	if(threadIdx.x != 0) return;
+	if(MODE == CG_SC_INIT) {
+		for(int c = 0; c < 2; ++c) {
+			const double bb = red[c], rr = red[2 + c], rz = red[4 + c];
+			cg->bb[c] = bb;
+			cg->iters[c] = 0;
+			cg->beta[c] = 0.;
+			cg->alpha[c] = 0.;
+			cg->zero_rhs[c] = 0;
+			if(bb == 0) {
+				cg->zero_rhs[c] = 1;
+				cg->done[c] = 1;
+				cg->rn2[c] = 0.;
+				continue;
+			}
+			cg->max_iter = a.max_iter;
+			double thr = a.tol * a.tol * bb;
+			if(thr < DBL_MIN) thr = DBL_MIN;
+			cg->thr[c] = thr;
+			cg->rn2[c] = rr;
+			if(rr < thr) {
+				cg->done[c] = 1;
+				continue;
+			}
+			cg->done[c] = 0;
+			cg->absNew[c] = rz;
+		}
+	}
+	else if(MODE == CG_SC_ALPHA) {
+		for(int c = 0; c < 2; ++c)
+			if(!cg->done[c]) cg->alpha[c] = cg->absNew[c] / red[c];
+	}
+	else {
+		for(int c = 0; c < 2; ++c) {
+			if(cg->done[c]) continue;
+			cg->iters[c] += 1;
+			const double rn2 = red[c];
+			cg->rn2[c] = rn2;
+			if(rn2 < cg->thr[c] || cg->iters[c] >= cg->max_iter) {
+				cg->done[c] = 1;
+				continue;
+			}
+			const double absOld = cg->absNew[c];
+			cg->absNew[c] = red[2 + c];
+			cg->beta[c] = cg->absNew[c] / absOld;
+		}
+	}
+	cg->active = !(cg->done[0] && cg->done[1]);
+}
+
+hipError_t launch_cg_scalar(int mode, const CGArgs &a, hipStream_t s) {
+	switch(mode) {
+	case CG_SC_INIT: hipLaunchKernelGGL(k_cg_scalar<CG_SC_INIT>, dim3(1), dim3(256), 0, s, a); break;
+	case CG_SC_ALPHA: hipLaunchKernelGGL(k_cg_scalar<CG_SC_ALPHA>, dim3(1), dim3(256), 0, s, a); break;
+	default: hipLaunchKernelGGL(k_cg_scalar<CG_SC_BETA>, dim3(1), dim3(256), 0, s, a); break;
+	}
+	return hipGetLastError();
+}
+
+__global__ void k_zero_rows(double2 *x, long long rows, int mask) {
+	const long long i = (long long) blockIdx.x * 256 + threadIdx.x;
+	if(i >= rows) return;
+	double2 v = x[i];
+	if(mask & 1) v.x = 0.;
+	if(mask & 2) v.y = 0.;
+	x[i] = v;
+}
+
+hipError_t launch_zero_rows(double2 *x, long long rows, int mask, hipStream_t s) {
+	if(rows <= 0) return hipSuccess;
+	hipLaunchKernelGGL(k_zero_rows, dim3((unsigned) ((rows + 255) / 256)), dim3(256), 0, s, x, rows, mask);
+	return hipGetLastError();
+}
+
+}  // namespace rc

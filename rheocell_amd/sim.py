@@ -1,0 +1,197 @@
+"""Host-side mirror of the reference backend for Python callers.
+
+``Simulation`` plays the part of ``SimBackend``/``FD_CPUBackend`` with the
+``WetPolarModel`` interaction (src/Backends/SimBackend.cpp:171-314,
+src/Backends/FD_CPUBackend.cpp:19-160) on top of the C ABI: it reads confs
+with the reference semantics, owns the random streams and drives
+``rc_step``.  All simulation arithmetic runs in librheocell.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from .conf import CellConf, Conf, RCexception
+from .rng import ReferenceStream
+
+_KEY_DEFAULTS = {  # WetPolarModel constructor / BaseInteraction defaults
+    "gamma": 0.06, "lambda": 2.5, "omega": 0.004, "mu": 20.0, "kappa": 0.1, "CGtolerance": 1e-4,
+    "wall_slip": 0.5, "passive_alpha": 1.0, "J0": 1.0, "anchoring": 0,
+}
+_REQUIRED = ["R", "dt", "zetaQ_self", "zetaQ_inter", "J_Q", "friction", "friction_cell", "beta"]
+
+
+def model_params(keys: Dict[str, object], cg_max_iter: int = 0) -> _lib.rc_model_params:
+    """Input keys -> rc_model_params.  The *_active keys are mandatory here:
+    the reference leaves those members uninitialised when they are absent
+    (WetPolarModel.cpp:4-21 vs :37-46)."""
+    for k in _REQUIRED:
+        if k not in keys:
+            raise RCexception("key '%s' is required (the reference leaves it uninitialised)" % k)
+    g = lambda k: float(keys.get(k, _KEY_DEFAULTS.get(k, 0.0)))  # noqa: E731
+    if str(keys.get("interaction_type", "wetpolarmodel")).lower() != "wetpolarmodel":
+        raise RCexception("interaction_type '%s' is not supported by the HIP backend" % keys.get("interaction_type"))
+    if str(keys.get("box", "square")).lower() != "square":
+        raise RCexception("only box = square is supported")
+    if int(float(keys.get("external_forces", 0))) != 0:
+        raise RCexception("external_forces are not supported by the HIP backend")
+    return _lib.rc_model_params(
+        gamma=g("gamma"), lambda_=g("lambda"), omega=g("omega"), mu=g("mu"), kappa=g("kappa"),
+        R=int(float(keys["R"])), anchoring=int(float(keys.get("anchoring", 0))),
+        zetaQ_self=g("zetaQ_self"), zetaQ_inter=g("zetaQ_inter"), J_Q=g("J_Q"), friction=g("friction"),
+        friction_cell=g("friction_cell"), CGtolerance=g("CGtolerance"), wall_slip=g("wall_slip"),
+        passive_alpha=g("passive_alpha"), beta=g("beta"), J0=g("J0"), dt=g("dt"), cg_max_iter=cg_max_iter)
+
+
+def cell_from_conf(c: CellConf) -> _lib.rc_cell:
+    """read_next_configuration per-cell state (SimBackend.cpp:273-296)."""
+    x = _lib.rc_cell()
+    x.LsubX, x.LsubY = c.LsubX, c.LsubY
+    x.offset[0], x.offset[1] = c.offset
+    x.corner = x.corner_old = c.corner
+    x.x_sub_left, x.y_sub_bottom = c.LsubX, c.LsubY
+    x.CoM[0], x.CoM[1] = c.CoM
+    x.CoM_old[0], x.CoM_old[1] = c.CoM
+    x.unrap[0], x.unrap[1] = c.unrap
+    x.Q00, x.Q01 = c.Q00, c.Q01
+    x.thetaQ = 0.5 * math.atan2(2 * c.Q01, 2 * c.Q00)
+    x.thetaQ_old = x.thetaQ
+    area = 0.0
+    sumF = 0.0
+    for v in c.phi.tolist():  # sequential sums, as the reader does
+        area += v * v
+        sumF += v
+    x.area, x.sumF = area, sumF
+    return x
+
+
+class Simulation:
+    """One handle: one or more independent boxes advanced together."""
+
+    def __init__(self, boxes: Sequence[Tuple[Conf, Dict[str, object]]], seeds: Optional[Sequence[int]] = None,
+                 cg_max_iter: int = 0, init: bool = True, equilibrate: bool = True):
+        self.L = _lib.load()
+        self.confs = [b[0] for b in boxes]
+        self.keys = [b[1] for b in boxes]
+        descs = (_lib.rc_box_desc * len(boxes))()
+        for i, (conf, keys) in enumerate(boxes):
+            if conf.Lx != conf.Ly:
+                raise RCexception("The box in the configuration file is not a square")
+            descs[i].L = conf.Lx
+            descs[i].ncells = len(conf.cells)
+            descs[i].params = model_params(keys, cg_max_iter)
+        self.h = C.c_void_p()
+        rc = self.L.rc_create(descs, len(boxes), C.byref(self.h))
+        self._check(rc)
+        cells = [c for conf in self.confs for c in conf.cells]
+        self.ncells = len(cells)
+        self.counts = [len(conf.cells) for conf in self.confs]
+        arr = (_lib.rc_cell * self.ncells)()
+        for i, c in enumerate(cells):
+            arr[i] = cell_from_conf(c)
+        phi = np.ascontiguousarray(np.concatenate([c.phi for c in cells]) if cells else np.zeros(1))
+        self._check(self.L.rc_upload_cells(self.h, arr, phi.ctypes.data_as(C.POINTER(C.c_double))))
+        sd = [int(k.get("seed", 1)) for k in self.keys] if seeds is None else list(seeds)
+        self.streams = [ReferenceStream(s, n) for s, n in zip(sd, self.counts)]
+        self.step_count = 0
+        if init:
+            self.init_forces()
+            if equilibrate:
+                self.apply_changes_after_equilibration()
+
+    # --- errors -----------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != _lib.RC_OK:
+            msg = self.L.rc_last_error(self.h).decode() if self.h else ""
+            raise RCexception("%s: %s" % (_lib.RC_ERR.get(rc, rc), msg))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.rc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- protocol of the reference backend --------------------------------
+    def init_forces(self):
+        self._check(self.L.rc_init_forces(self.h))
+
+    def apply_changes_after_equilibration(self):
+        self._check(self.L.rc_apply_changes_after_equilibration(self.h))
+
+    def set_warm_start(self, mode: int):
+        self._check(self.L.rc_set_warm_start(self.h, mode))
+
+    def gaussians(self, nsteps: int) -> np.ndarray:
+        parts = [s.gaussians(nsteps) for s in self.streams]
+        return np.ascontiguousarray(np.concatenate(parts, axis=1))
+
+    def step(self, nsteps: int = 1, gauss: Optional[np.ndarray] = None):
+        if gauss is None:
+            gauss = self.gaussians(nsteps)
+        gauss = np.ascontiguousarray(gauss, dtype=np.float64)
+        self._check(self.L.rc_step(self.h, nsteps, gauss.ctypes.data_as(C.POINTER(C.c_double))))
+        self.step_count += nsteps
+
+    # --- sync / observables -----------------------------------------------
+    def total_rows(self) -> int:
+        n = C.c_longlong()
+        self._check(self.L.rc_total_rows(self.h, C.byref(n)))
+        return n.value
+
+    def download(self) -> Tuple[List[_lib.rc_cell], np.ndarray]:
+        arr = (_lib.rc_cell * self.ncells)()
+        phi = np.empty(self.total_rows())
+        self._check(self.L.rc_download_cells(self.h, arr, phi.ctypes.data_as(C.POINTER(C.c_double))))
+        return list(arr), phi
+
+    def rows(self, which: int) -> np.ndarray:
+        out = np.empty(self.total_rows())
+        self._check(self.L.rc_get_rows(self.h, which, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def site_array(self, which: int) -> np.ndarray:
+        out = np.empty(self.total_rows())
+        self._check(self.L.rc_get_site_array(self.h, which, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def cg_stats(self) -> _lib.rc_cg_stats_t:
+        st = _lib.rc_cg_stats_t()
+        self._check(self.L.rc_cg_stats(self.h, C.byref(st)))
+        return st
+
+    def enable_timing(self, on: bool = True):
+        self._check(self.L.rc_enable_timing(self.h, 1 if on else 0))
+
+    def timing(self, kind: int) -> Tuple[float, int]:
+        ms = C.c_double()
+        n = C.c_longlong()
+        self._check(self.L.rc_timing(self.h, kind, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def to_conf(self, box: int = 0) -> Conf:
+        """Configuration observable of one box (Configuration.cpp:46-55)."""
+        cells, phi = self.download()
+        b0 = sum(self.counts[:box])
+        out = []
+        start = sum(c.LsubX * c.LsubY for c in cells[:b0])
+        L = self.confs[box].Lx
+        for c in cells[b0:b0 + self.counts[box]]:
+            S = c.LsubX * c.LsubY
+            q = np.arange(S)
+            lx = (q % c.LsubX + c.offset[0]) % c.LsubX
+            ly = (q // c.LsubX + c.offset[1]) % c.LsubY
+            cx, cy = c.corner % L, c.corner // L
+            k = (cx + lx) % L + ((cy + ly) % L) * L
+            out.append(CellConf(c.LsubX, c.LsubY, (c.CoM[0], c.CoM[1]), (c.offset[0], c.offset[1]), c.corner,
+                                (c.unrap[0], c.unrap[1]), c.Q00, c.Q01, k.astype(np.int64), phi[start:start + S].copy()))
+            start += S
+        return Conf(self.confs[box].step + self.step_count, L, L, out)
