@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the RheoCell hot path on one MI355X.
+
+Default workload: the wet (viscous) model, periodic 512x512 box, 1600 cells
+(jittered 40x40 lattice, R=8, 41x41 tanh patches), FP64, matrix-free
+Jacobi-PCG to 1e-8 relative residual warm-started by the reference rule.
+A "step" is one full Heun step (2 force evaluations + 2 field updates +
+4 CG solves); value = cells * steps / wall seconds (cell-steps/s) with all
+inputs resident in HBM.
+
+Prints ONE JSON line.  Also measures, outside the timed region:
+  * roofline: per-kernel device time with HIP events on the library stream;
+    achieved bandwidth of the dominant kernel from its algorithmic bytes;
+  * cpu_baseline: the C oracle (reference-faithful port, -O3 -ffast-math,
+    1 core) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+# algorithmic bytes per launch, (bytes per patch-site row, bytes per box site)
+BYTES = {
+    "box_gather": (8, 12),          # read phi; write sum phi^2 and cover count
+    "stage_fused_pred": (24, 8),    # read phi_n; write phi*, dphi_old; box window
+    "stage_fused_corr": (32, 8),    # read phi*, phi_n, dphi_old; write phi_n+1
+    "stage_cf": (24, 8),            # read phi; write f (2 doubles)
+    "stage_fs_pred": (40, 8),       # read phi, v; write phi*, dphi_old
+    "stage_fs_corr": (48, 8),       # read phi*, v, phi_n, dphi_old; write phi_n+1
+    "cg_diag": (0, 12),
+    "cg_gather_x": (16, 16),        # read x; write box sums
+    "cg_gather_p_first": (32, 24),  # read r; write p; read diag, write box sums
+    "cg_gather_p": (48, 24),        # read r, p; write p; read diag, write box sums
+    "cg_row_resid": (48, 24),       # read x, b; write r; box window + diag
+    "cg_row_spmv": (16, 24),        # read p; box window + diag
+    "cg_row_update": (80, 24),      # read x, p, r; write x, r; box window + diag
+    "cg_scalar": (0, 0),
+}
+CG_SWEEP_KINDS = ("cg_row_spmv", "cg_row_update")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="wet512_1600")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timing", action="store_true")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(conf, keys, steps):
+    from oracle import OracleSim  # the checker, timed here as the CPU port
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except Exception:
+        pass
+    s = OracleSim(conf, keys, flavour="fast")
+    t0 = time.perf_counter()
+    s.step(steps)
+    dt = time.perf_counter() - t0
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    st = s.cg_stats()
+    return {
+        "value": len(conf.cells) * steps / dt,
+        "unit": "cell-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "%s, first %d Heun step(s) after init, oracle/liborc_fast.so (-O3 -ffast-math), 1 thread on %s; "
+                  "%.1f s, CG iterations %d over %d solves" % (keys.get("_name", "workload"), steps, cpu, dt,
+                                                               st.iters_total, st.solves),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from rheocell_amd import confgen, _lib
+    from rheocell_amd.sim import Simulation
+
+    log("[bench] generating %s (seed %d)" % (args.workload, args.seed))
+    conf, keys = confgen.workload(args.workload, seed=args.seed)
+    keys["_name"] = args.workload
+    ncells = len(conf.cells)
+    L = conf.Lx
+    wet = float(keys["friction_cell"]) != 0
+    sim = Simulation([(conf, keys)])
+    rows = sim.total_rows()
+    gauss = sim.gaussians(args.warmup + args.steps + args.timing_steps)
+    log("[bench] %d cells, %d rows, warmup %d" % (ncells, rows, args.warmup))
+    if args.warmup:
+        sim.step(args.warmup, gauss[:args.warmup])
+    st0 = sim.cg_stats()
+    log("[bench] timed %d steps" % args.steps)
+    t0 = time.perf_counter()
+    sim.step(args.steps, gauss[args.warmup:args.warmup + args.steps])
+    t1 = time.perf_counter()
+    st1 = sim.cg_stats()
+    secs = t1 - t0
+    value = ncells * args.steps / secs
+    ms_step = 1e3 * secs / args.steps
+    log("[bench] %.3f ms/step, %.4g cell-steps/s" % (ms_step, value))
+
+    roof = None
+    kernels = {}
+    if not args.no_timing and args.timing_steps > 0:
+        sim.enable_timing(True)
+        st_a = sim.cg_stats()
+        sim.step(args.timing_steps, gauss[args.warmup + args.steps:])
+        st_b = sim.cg_stats()
+        sweeps = st_b.sweeps_total - st_a.sweeps_total
+        for i, name in enumerate(_lib.T_KINDS):
+            ms, n = sim.timing(i)
+            if n:
+                kernels[name] = {"ms_total": ms, "launches": n}
+        sim.enable_timing(False)
+        best = None
+        for name, d in kernels.items():
+            if name not in BYTES:
+                continue
+            if best is None or d["ms_total"] > kernels[best]["ms_total"]:
+                best = name
+        if best:
+            d = kernels[best]
+            per_row, per_site = BYTES[best]
+            bytes_launch = per_row * rows + per_site * L * L
+            useful = sweeps if best in CG_SWEEP_KINDS else d["launches"]
+            avg_us = 1e3 * d["ms_total"] / d["launches"]
+            achieved = bytes_launch * useful / (d["ms_total"] * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                    "kernel": _lib.T_SYMBOL.get(best, best), "kind": best,
+                    "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3),
+                    "launches": d["launches"], "useful_launches": useful,
+                    "bytes_model": "%d B/row x %d rows + %d B/site x %d box sites" % (per_row, rows, per_site, L * L)}
+        log("[bench] kernels: " + json.dumps({k: (round(v["ms_total"], 3), v["launches"]) for k, v in kernels.items()}))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline: %d step(s) of the oracle on 1 core" % args.cpu_steps)
+        cpu = cpu_baseline(conf, keys, args.cpu_steps)
+        log("[bench] CPU %.4g cell-steps/s" % cpu["value"])
+
+    solves = st1.solves - st0.solves
+    iters = st1.iters_total - st0.iters_total
+    out = {
+        "metric": "cell-steps/s, 512^2 wet, 1600 cells" if args.workload == "wet512_1600" else "cell-steps/s, %s" % args.workload,
+        "value": round(value, 2),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (jittered-lattice tanh cells, seed %d)" % args.seed,
+        "config": {"workload": args.workload, "box": L, "cells": ncells, "patch": "41x41", "rows": rows,
+                   "model": "wetpolarmodel", "friction_cell": float(keys["friction_cell"]),
+                   "cg_tolerance": float(keys["CGtolerance"]) if wet else None,
+                   "warm_start": "previous solution, cold every 11th solve (reference rule)" if wet else None},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if wet:
+        out["cg_ms_per_step"] = round((st1.total_solve_ms - st0.total_solve_ms) / args.steps, 4)
+        out["cg_iters_per_solve"] = round(iters / max(solves, 1), 2)
+    if kernels:
+        out["kernels_ms"] = {k: round(v["ms_total"], 4) for k, v in kernels.items()}
+    print(json.dumps(out), flush=True)
+    sim.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -96,8 +96,10 @@ typedef struct rc_cg_stats_t {
 	int last_iters[2];               /* x, y of the last solve */
 	double last_relres[2];
 	int last_cold;
-	double last_solve_ms;            /* device time of the last x+y solve */
+	double last_solve_ms;            /* wall time of the last x+y solve (HIP events) */
 	double total_solve_ms;
+	long long sweeps_total;          /* launches of the iteration kernels that did work:
+	                                    sum over solves of max(iters_x, iters_y) */
 } rc_cg_stats_t;
 
 #define RC_WARM_REFERENCE 0   /* previous solution, cold restart every 11th solve (reference rule) */
@@ -123,10 +125,27 @@ int rc_get_site_array(rc_handle *h, int which, double *out);
 int rc_total_rows(rc_handle *h, long long *rows);
 int rc_cg_stats(rc_handle *h, rc_cg_stats_t *out);
 int rc_set_warm_start(rc_handle *h, int mode);
-/* Device time (ms) of the stencil/field kernels over the last rc_step call,
- * measured with HIP events on the library's stream; kind: 0 box gather,
- * 1 per-cell stage kernels, 2 CG kernels, 3 whole rc_step. launches gets the
- * number of launches of that kind. */
+/* Kernel timing with HIP events on the library's stream (enabled by
+ * rc_enable_timing; adds an event pair around every launch).  rc_timing
+ * returns the summed device time (ms) and launch count of one kind since
+ * timing was enabled. */
+#define RC_T_BOX_GATHER 0          /* k_box_gather: box sums of phi^2 */
+#define RC_T_STAGE_FUSED_PRED 1    /* dry: force evaluation + predictor update */
+#define RC_T_STAGE_FUSED_CORR 2    /* dry: force evaluation + corrector update */
+#define RC_T_STAGE_CF 3            /* force evaluation only (wet RHS / dry velocity) */
+#define RC_T_STAGE_FS_PRED 4       /* wet: predictor field update */
+#define RC_T_STAGE_FS_CORR 5       /* wet: corrector field update */
+#define RC_T_CG_DIAG 6
+#define RC_T_CG_GATHER_X 7
+#define RC_T_CG_GATHER_P 8         /* p = z + beta p fused with the box gather of p */
+#define RC_T_CG_ROW_RESID 9
+#define RC_T_CG_ROW_SPMV 10        /* p.Ap */
+#define RC_T_CG_ROW_UPDATE 11      /* x += alpha p, r -= alpha Ap, r.r, r.z */
+#define RC_T_CG_SCALAR 12
+#define RC_T_CG_SOLVE 13           /* a whole x+y solve */
+#define RC_T_STEP 14               /* a whole rc_step call */
+#define RC_T_CG_GATHER_P_FIRST 15  /* p = z fused with the box gather of p */
+#define RC_NKINDS 16
 int rc_timing(rc_handle *h, int kind, double *ms, long long *launches);
 int rc_enable_timing(rc_handle *h, int on);
 const char *rc_last_error(rc_handle *h);

@@ -84,6 +84,8 @@ def _lib(flavour: str = "strict") -> C.CDLL:
     L.orc_trig_table.argtypes = [ip, ip, dp]
     L.orc_apply_A.argtypes = [vp, dp, dp]
     L.orc_assemble_dense.argtypes = [vp, dp]
+    L.orc_test_nbrs.argtypes = [ip, ip, C.POINTER(C.c_int)]
+    L.orc_test_set_positions.argtypes = [ip, C.POINTER(C.c_int), dp, dp, ip]
     _LIBS[flavour] = L
     return L
 
@@ -195,3 +197,25 @@ def trig_table(L: int, which: int, flavour: str = "strict") -> np.ndarray:
     out = np.empty(L)
     _lib(flavour).orc_trig_table(L, which, _dp(out))
     return out
+
+
+def test_nbrs(LX: int, LY: int, flavour: str = "strict") -> np.ndarray:
+    out = np.empty(9 * LX * LY, dtype=np.int32)
+    _lib(flavour).orc_test_nbrs(LX, LY, out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out
+
+
+def test_set_positions(Lbox, LX, LY, offx, offy, corner, ux, uy, comx, comy, xsl, ysb, phi, flavour="strict"):
+    """MultiPhaseField::set_positions(box) restated; returns the new state."""
+    ints = np.array([LX, LY, offx, offy, corner, corner, xsl, ysb], dtype=np.int32)
+    dbl = np.array([ux, uy, comx, comy, 0.0, 0.0])
+    cap = 4 * max(LX * LY, 64 * 64)
+    buf = np.zeros(cap)
+    buf[:LX * LY] = phi
+    rc = _lib(flavour).orc_test_set_positions(Lbox, ints.ctypes.data_as(C.POINTER(C.c_int)), _dp(dbl), _dp(buf), cap)
+    if rc != 0:
+        raise RuntimeError("set_positions failed")
+    S = int(ints[0]) * int(ints[1])
+    return {"LX": int(ints[0]), "LY": int(ints[1]), "offset": [int(ints[2]), int(ints[3])], "corner": int(ints[4]),
+            "corner_old": int(ints[5]), "unrap": [dbl[0], dbl[1]], "CoM": [dbl[2], dbl[3]], "area": dbl[4],
+            "phi": buf[:S].copy()}

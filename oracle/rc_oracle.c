@@ -29,6 +29,7 @@
  * against dense numpy solves.  Eigen's exact CG iterate sequence is parity
  * unpinned (Eigen is absent from this image).
  */
+#define _GNU_SOURCE 1
 #include "rc_oracle.h"
 
 #include <float.h>
@@ -126,10 +127,14 @@ double orc_rng_gaussian(orc_rng *r) {
 }
 
 void orc_trig_table(int L, int which, double *out) {
-	/* MultiPhaseField.cpp:58-65: cos(2*PI*i/LsubX), argument in float */
+	/* MultiPhaseField.cpp:58-65: cos(2*PI*i/LsubX) and sin(...) of the same
+	 * float argument; the reference build merges the pair into one glibc
+	 * sincos call, which is what is evaluated here. */
 	for(int i = 0; i < L; i++) {
 		float a = 2 * PI_F * i / L;
-		out[i] = which == 0 ? cos(a) : sin(a);
+		double sv, cv;
+		sincos((double) a, &sv, &cv);
+		out[i] = which == 0 ? cv : sv;
 	}
 }
 
@@ -827,4 +832,37 @@ int orc_assemble_dense(orc_sim *s, double *A) {
 		}
 	}
 	return 0;
+}
+
+/* ---------------- test hooks ----------------------------------------- */
+void orc_test_nbrs(int LX, int LY, int *out) { periodic_nbrs(LX, LY, out); }
+
+int orc_test_set_positions(int Lbox, int *ints, double *dbl, double *phi, int phi_cap) {
+	orc_sim tmp;
+	memset(&tmp, 0, sizeof(tmp));
+	tmp.L = Lbox;
+	ocell p;
+	memset(&p, 0, sizeof(p));
+	p.LX = ints[0]; p.LY = ints[1]; p.S = p.LX * p.LY;
+	p.offset[0] = ints[2]; p.offset[1] = ints[3];
+	p.corner = ints[4]; p.corner_old = ints[5];
+	p.x_sub_left = ints[6]; p.y_sub_bottom = ints[7];
+	p.unrap[0] = dbl[0]; p.unrap[1] = dbl[1];
+	p.CoM[0] = dbl[2]; p.CoM[1] = dbl[3];
+	p.area = dbl[4];
+	cell_resizing(&p, 0, p.S);
+	memcpy(p.phi, phi, sizeof(double) * p.S);
+	periodic_nbrs(p.LX, p.LY, p.nbr);
+	int rc = set_positions(&tmp, &p);
+	if(rc == 0) {
+		if(p.S > phi_cap) rc = -1;
+		else {
+			ints[0] = p.LX; ints[1] = p.LY; ints[2] = p.offset[0]; ints[3] = p.offset[1];
+			ints[4] = p.corner; ints[5] = p.corner_old; ints[6] = p.x_sub_left; ints[7] = p.y_sub_bottom;
+			dbl[0] = p.unrap[0]; dbl[1] = p.unrap[1]; dbl[2] = p.CoM[0]; dbl[3] = p.CoM[1]; dbl[4] = p.area;
+			memcpy(phi, p.phi, sizeof(double) * p.S);
+		}
+	}
+	cell_free(&p);
+	return rc;
 }

@@ -100,6 +100,14 @@ int orc_apply_A(orc_sim *s, const double *x, double *y);
  * (WetPolarModel.cpp:200-217), dense row-major rows x rows. */
 int orc_assemble_dense(orc_sim *s, double *A);
 
+/* Test hooks for golden checks of the geometry restatement. */
+void orc_test_nbrs(int LX, int LY, int *out);          /* 9*LX*LY */
+/* MultiPhaseField::set_positions(box) on one cell; in/out state arrays:
+ * ints[8] = {LX, LY, offx, offy, corner, corner_old, xsl, ysb}
+ * dbl[6]  = {unrapx, unrapy, comx, comy, area, 0}; phi holds max(S_in, S_out)
+ * values.  Returns 0 or -1. */
+int orc_test_set_positions(int Lbox, int *ints, double *dbl, double *phi, int phi_cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,17 @@ RC_ERR = {1: "RC_ERR_ARG", 2: "RC_ERR_HIP", 3: "RC_ERR_OVERLAP", 4: "RC_ERR_GROW
 RC_WARM_REFERENCE = 0
 RC_WARM_PREVIOUS = 1
 
+# timing kinds (rheocell.h RC_T_*)
+T_KINDS = ["box_gather", "stage_fused_pred", "stage_fused_corr", "stage_cf", "stage_fs_pred", "stage_fs_corr",
+           "cg_diag", "cg_gather_x", "cg_gather_p", "cg_row_resid", "cg_row_spmv", "cg_row_update", "cg_scalar",
+           "cg_solve", "step", "cg_gather_p_first"]
+# kernel symbol of each timing kind (as rocprofv3 names it)
+T_SYMBOL = {"box_gather": "k_box_gather", "stage_fused_pred": "k_stage<0>", "stage_fused_corr": "k_stage<0>",
+            "stage_cf": "k_stage<1>", "stage_fs_pred": "k_stage<2>", "stage_fs_corr": "k_stage<2>",
+            "cg_diag": "k_cg_diag", "cg_gather_x": "k_cg_gather<0>", "cg_gather_p_first": "k_cg_gather<1>",
+            "cg_gather_p": "k_cg_gather<2>", "cg_row_resid": "k_cg_row<0>", "cg_row_spmv": "k_cg_row<1>",
+            "cg_row_update": "k_cg_row<2>", "cg_scalar": "k_cg_scalar"}
+
 EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_after_equilibration", "rc_step",
            "rc_download_cells", "rc_get_rows", "rc_get_site_array", "rc_total_rows", "rc_cg_stats",
            "rc_set_warm_start", "rc_timing", "rc_enable_timing", "rc_last_error", "rc_destroy"]
@@ -51,7 +62,7 @@ class rc_cell(C.Structure):
 class rc_cg_stats_t(C.Structure):
     _fields_ = [("solves", C.c_longlong), ("iters_total", C.c_longlong), ("last_iters", C.c_int * 2),
                 ("last_relres", C.c_double * 2), ("last_cold", C.c_int), ("last_solve_ms", C.c_double),
-                ("total_solve_ms", C.c_double)]
+                ("total_solve_ms", C.c_double), ("sweeps_total", C.c_longlong)]
 
 
 _lib = None

@@ -87,8 +87,8 @@ struct rc_handle {
 	std::vector<TimedPair> pairs;
 	std::vector<hipEvent_t> pool;
 	size_t pool_used = 0;
-	double t_ms[4] = { 0, 0, 0, 0 };
-	long long t_n[4] = { 0, 0, 0, 0 };
+	double t_ms[RC_NKINDS] = {};
+	long long t_n[RC_NKINDS] = {};
 };
 
 // ---------------------------------------------------------------------------
@@ -297,16 +297,16 @@ extern "C" void rc_destroy(rc_handle *h) {
 }
 
 // Patch trig table of MultiPhaseField::resizing (MultiPhaseField.cpp:54-65):
-// cos(2*PI*i/L) with the argument evaluated in float, cos/sin in double.
+// cos/sin(2*PI*i/L) with the argument evaluated in float; the reference build
+// evaluates each cos/sin pair of one argument with a single glibc sincos call.
 static void trig_table(int L, std::vector<double> &out) {
+	std::vector<double> c(L), s(L);
 	for(int i = 0; i < L; i++) {
 		float a = 2 * PI_F * i / L;
-		out.push_back(std::cos((double) a));
+		sincos((double) a, &s[i], &c[i]);
 	}
-	for(int i = 0; i < L; i++) {
-		float a = 2 * PI_F * i / L;
-		out.push_back(std::sin((double) a));
-	}
+	out.insert(out.end(), c.begin(), c.end());
+	out.insert(out.end(), s.begin(), s.end());
 }
 
 extern "C" int rc_upload_cells(rc_handle *h, const rc_cell *cells, const double *phi) {
@@ -438,13 +438,16 @@ static int run_gather(rc_handle *h, const double *phi, bool lists) {
 	g.tile_count = h->d_tile_count;
 	g.store_lists = lists ? 1 : 0;
 	g.st = h->d_st;
-	TimeScope ts(h, 0);
+	TimeScope ts(h, RC_T_BOX_GATHER);
 	HIPCHK(h, launch_gather(g, h->ntiles, h->stream));
 	return RC_OK;
 }
 
 static int run_stage(rc_handle *h, int mode, const StageArgs &a) {
-	TimeScope ts(h, 1);
+	const int kind = mode == MODE_CF ? RC_T_STAGE_CF
+			: mode == MODE_FUSED ? (a.store ? RC_T_STAGE_FUSED_PRED : RC_T_STAGE_FUSED_CORR)
+			: (a.store ? RC_T_STAGE_FS_PRED : RC_T_STAGE_FS_CORR);
+	TimeScope ts(h, kind);
 	HIPCHK(h, launch_stage(mode, a, h->ncells, h->lds_stage, h->stream));
 	return RC_OK;
 }
@@ -481,22 +484,22 @@ static int cg_solve(rc_handle *h, bool cold) {
 	hipEvent_t e0 = h->cg_e0, e1 = h->cg_e1;
 	HIPCHK(h, hipEventRecord(e0, h->stream));
 	{
-		TimeScope ts(h, 2);
+		TimeScope ts(h, RC_T_CG_SOLVE);
 		if(cold) HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * h->rows, h->stream));
-		HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream));
-		HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
-		HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->lds_cg, h->stream));
-		HIPCHK(h, launch_cg_scalar(CG_SC_INIT, c, h->stream));
-		HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream));
+		{ TimeScope t(h, RC_T_CG_DIAG); HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->lds_cg, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_SCALAR); HIPCHK(h, launch_cg_scalar(CG_SC_INIT, c, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_GATHER_P_FIRST); HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream)); }
 		int batch = std::max(2, h->last_iters - 1);
 		long long done_iters = 0;
 		for(;;) {
 			for(int it = 0; it < batch; ++it) {
-				HIPCHK(h, launch_cg_row(CG_ROW_SPMV, c, h->lds_cg, h->stream));
-				HIPCHK(h, launch_cg_scalar(CG_SC_ALPHA, c, h->stream));
-				HIPCHK(h, launch_cg_row(CG_ROW_UPDATE, c, h->lds_cg, h->stream));
-				HIPCHK(h, launch_cg_scalar(CG_SC_BETA, c, h->stream));
-				HIPCHK(h, launch_cg_gather(CG_GATHER_P_UPDATE, c, h->ntiles, h->stream));
+				{ TimeScope t(h, RC_T_CG_ROW_SPMV); HIPCHK(h, launch_cg_row(CG_ROW_SPMV, c, h->lds_cg, h->stream)); }
+				{ TimeScope t(h, RC_T_CG_SCALAR); HIPCHK(h, launch_cg_scalar(CG_SC_ALPHA, c, h->stream)); }
+				{ TimeScope t(h, RC_T_CG_ROW_UPDATE); HIPCHK(h, launch_cg_row(CG_ROW_UPDATE, c, h->lds_cg, h->stream)); }
+				{ TimeScope t(h, RC_T_CG_SCALAR); HIPCHK(h, launch_cg_scalar(CG_SC_BETA, c, h->stream)); }
+				{ TimeScope t(h, RC_T_CG_GATHER_P); HIPCHK(h, launch_cg_gather(CG_GATHER_P_UPDATE, c, h->ntiles, h->stream)); }
 			}
 			done_iters += batch;
 			HIPCHK(h, hipMemcpyAsync(h->h_cg, h->d_cg, sizeof(CGState), hipMemcpyDeviceToHost, h->stream));
@@ -518,6 +521,7 @@ static int cg_solve(rc_handle *h, bool cold) {
 	const CGState &s = *h->h_cg;
 	h->cgs.solves += 2;
 	h->cgs.iters_total += s.iters[0] + s.iters[1];
+	h->cgs.sweeps_total += std::max(s.iters[0], s.iters[1]);
 	h->cgs.last_iters[0] = s.iters[0];
 	h->cgs.last_iters[1] = s.iters[1];
 	for(int k = 0; k < 2; ++k) h->cgs.last_relres[k] = s.bb[k] > 0 ? std::sqrt(s.rn2[k] / s.bb[k]) : 0.;
@@ -682,8 +686,8 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 	if(h->timing) {
 		float ms = 0.f;
 		(void) hipEventElapsedTime(&ms, t0, t1);
-		h->t_ms[3] += ms;
-		h->t_n[3] += 1;
+		h->t_ms[RC_T_STEP] += ms;
+		h->t_n[RC_T_STEP] += 1;
 		collect_timing(h);
 	}
 	return rc;
@@ -809,7 +813,7 @@ extern "C" int rc_cg_stats(rc_handle *h, rc_cg_stats_t *out) {
 extern "C" int rc_enable_timing(rc_handle *h, int on) {
 	if(!h) return RC_ERR_ARG;
 	h->timing = on != 0;
-	for(int k = 0; k < 4; ++k) {
+	for(int k = 0; k < RC_NKINDS; ++k) {
 		h->t_ms[k] = 0;
 		h->t_n[k] = 0;
 	}
@@ -817,7 +821,7 @@ extern "C" int rc_enable_timing(rc_handle *h, int on) {
 }
 
 extern "C" int rc_timing(rc_handle *h, int kind, double *ms, long long *launches) {
-	if(!h || kind < 0 || kind > 3) return RC_ERR_ARG;
+	if(!h || kind < 0 || kind >= RC_NKINDS) return RC_ERR_ARG;
 	if(ms) *ms = h->t_ms[kind];
 	if(launches) *launches = h->t_n[kind];
 	return RC_OK;

@@ -1,0 +1,95 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/ref_fields.json from the reference's own code.
+
+Runs oracle/_ref/ref_fields (oracle/ref_driver.cpp linked against the
+reference's MultiPhaseField, SquareBox and Utils sources, compiled from
+/root/reference by oracle/build_ref.sh) and, for the trig tables only, the
+same driver compiled with the reference's Release flags (ref_fields_fast).
+The fixture holds inputs and outputs only.
+
+Usage: bash oracle/build_ref.sh && python tests/golden/make_golden.py
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_fields")
+REF_FAST = os.path.join(ROOT, "oracle", "_ref", "ref_fields_fast")
+
+
+def run(binary, cmds):
+    out = subprocess.run([binary], input="\n".join(cmds) + "\n", capture_output=True, text=True, check=True).stdout
+    return [json.loads(l) for l in out.splitlines() if l.strip()]
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("build oracle/_ref first: bash oracle/build_ref.sh")
+    rng = np.random.default_rng(2024)
+    cmds = ["trig %d" % L for L in range(29, 50)]
+    cmds += ["rng %d 64" % s for s in (1, 4982, 123456789)]
+    cmds += ["gauss %d 64" % s for s in (1, 4982, 123456789)]
+    sub = [(64, 5, 4, 2, 1, 4090), (140, 41, 41, 0, 0, 99 * 140 + 130), (140, 41, 41, 17, 33, 139),
+           (512, 41, 41, 40, 40, 511 * 512 + 511), (128, 29, 49, 3, 48, 5000)]
+    cmds += ["subindex %d %d %d %d %d %d" % s for s in sub]
+    cmds += ["nbrsub 5 4", "nbrsub 41 41", "nbrsub 29 33"]
+    setpos_inputs = []
+    # recentering: random CoM within the patch, corners near box edges
+    for t in range(24):
+        Lb = int(rng.choice([64, 140, 512]))
+        LX = int(rng.choice([29, 41, 49]))
+        LY = LX if t % 3 else int(rng.choice([29, 41]))
+        ox, oy = int(rng.integers(LX)), int(rng.integers(LY))
+        corner = int(rng.integers(Lb)) + Lb * int(rng.integers(Lb))
+        if t % 4 == 0:
+            corner = (Lb - 3) + Lb * (Lb - 5)
+        ux, uy = float(rng.integers(-50, 50)), float(rng.integers(-50, 50))
+        cx, cy = float(rng.uniform(0, LX)), float(rng.uniform(0, LY))
+        setpos_inputs.append((Lb, LX, LY, ox, oy, corner, ux, uy, cx, cy, 9, 9, np.zeros(LX * LY)))
+    # growth / reshaping branch (x_sub_left or y_sub_bottom < border = 4)
+    for t in range(10):
+        Lb = 64
+        LX = LY = int(rng.choice([12, 16, 21]))
+        ox, oy = int(rng.integers(LX)), int(rng.integers(LY))
+        corner = int(rng.integers(Lb)) + Lb * int(rng.integers(Lb))
+        xsl = int(rng.integers(0, 4)) if t % 2 == 0 else int(rng.integers(4, LX // 2))
+        ysb = int(rng.integers(0, 4)) if t % 2 == 1 or t % 3 == 0 else int(rng.integers(4, LY // 2))
+        phi = np.round(rng.uniform(0, 1, LX * LY), 12)
+        setpos_inputs.append((Lb, LX, LY, ox, oy, corner, 3.0, -7.0, LX / 2 + 0.3, LY / 2 - 0.2, xsl, ysb, phi))
+    for (Lb, LX, LY, ox, oy, corner, ux, uy, cx, cy, xsl, ysb, phi) in setpos_inputs:
+        cmds.append("setpos %d %d %d %d %d %d %.17g %.17g %.17g %.17g %d %d " % (Lb, LX, LY, ox, oy, corner, ux, uy, cx, cy, xsl, ysb)
+                    + " ".join("%.17g" % v for v in phi))
+    res = run(REF, cmds)
+    fast = run(REF_FAST, ["trig %d" % L for L in range(29, 50)])
+    setpos = []
+    k = 0
+    for r in res:
+        if r["cmd"] == "setpos":
+            inp = setpos_inputs[k]
+            k += 1
+            r["input"] = {"Lbox": inp[0], "LX": inp[1], "LY": inp[2], "offset": [inp[3], inp[4]], "corner": inp[5],
+                          "unrap": [inp[6], inp[7]], "CoM": [inp[8], inp[9]], "xsl": inp[10], "ysb": inp[11],
+                          "phi": [float(v) for v in inp[12]]}
+            setpos.append(r)
+    out = {
+        "source": "reference MultiPhaseField/SquareBox/Utils compiled from source (oracle/build_ref.sh)",
+        "trig": [r for r in res if r["cmd"] == "trig"],
+        "trig_fast": fast,
+        "rng": [r for r in res if r["cmd"] == "rng"],
+        "gauss": [r for r in res if r["cmd"] == "gauss"],
+        "subindex": [r for r in res if r["cmd"] == "subindex"],
+        "nbrsub": [r for r in res if r["cmd"] == "nbrsub"],
+        "setpos": setpos,
+    }
+    with open(os.path.join(HERE, "ref_fields.json"), "w") as f:
+        json.dump(out, f, separators=(",", ":"))
+    print("wrote", os.path.join(HERE, "ref_fields.json"))
+
+
+if __name__ == "__main__":
+    main()
