@@ -83,6 +83,30 @@ WORKLOADS = {
 }
 
 
+def replica_ensemble(nrep: int = 32, seed: int = 1, coverage: float = 0.9):
+    """Independent dry replicas: box sides from {128,...,384}, cell radii
+    U[6,10] with patch side 5R-1 rounded to odd (29..49), ~90% area
+    coverage, own seed and zetaQ_self in [0.05, 0.2] per replica."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(nrep):
+        L = int(rng.choice([128, 192, 256, 320, 384]))
+        n_target = coverage * L * L / (math.pi * 8.0 ** 2)
+        nside = max(1, int(round(math.sqrt(n_target))))
+        radii = rng.uniform(6, 10, nside * nside)
+        sides = [int(2 * round((5 * R - 1) / 2.0) + 1) if int(round(5 * R - 1)) % 2 == 0 else int(round(5 * R - 1))
+                 for R in radii]
+        sides = [min(49, max(29, s)) for s in sides]
+        rs = int(rng.integers(1, 2 ** 31 - 1))
+        conf = lattice_conf(L, nside, rs, radii=[float(x) for x in radii], sides=sides)
+        keys = dict(BASE_KEYS)
+        keys["seed"] = rs
+        keys["zetaQ_self"] = float(rng.uniform(0.05, 0.2))
+        keys["steps"] = 1000
+        out.append((conf, keys))
+    return out
+
+
 def workload(name: str, seed: int = 1):
     L, nside, steps, fc = WORKLOADS[name]
     conf = lattice_conf(L, nside, seed)

@@ -309,9 +309,9 @@ static void trig_table(int L, std::vector<double> &out) {
 	out.insert(out.end(), s.begin(), s.end());
 }
 
-extern "C" int rc_upload_cells(rc_handle *h, const rc_cell *cells, const double *phi) {
-	if(!h) return RC_ERR_ARG;
-	if(!cells || !phi) return fail(h, RC_ERR_ARG, "null cells/phi");
+// Lays out the rows of all cells (field_start_index order), the trig pool and
+// the cell table on the device.  rep: Repulsion rows to keep (J_Q != 0), or null.
+static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, const double *rep) {
 	std::vector<DevCell> dc(h->ncells);
 	std::vector<double> trig;
 	std::map<int, int> trig_off;
@@ -389,7 +389,10 @@ extern "C" int rc_upload_cells(rc_handle *h, const rc_cell *cells, const double 
 		}
 		h->rows = rows;
 	}
-	if(h->jq) HIPCHK(h, hipMemsetAsync(h->d_rep, 0, sizeof(double) * rows, h->stream));
+	if(h->jq) {
+		if(rep) HIPCHK(h, hipMemcpyAsync(h->d_rep, rep, sizeof(double) * rows, hipMemcpyHostToDevice, h->stream));
+		else HIPCHK(h, hipMemsetAsync(h->d_rep, 0, sizeof(double) * rows, h->stream));
+	}
 	HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * rows, h->stream));
 	dfree(h->d_trig);
 	HIPCHK(h, dalloc(&h->d_trig, trig.size()));
@@ -404,6 +407,14 @@ extern "C" int rc_upload_cells(rc_handle *h, const rc_cell *cells, const double 
 	if(h->lds_stage > 160 * 1024) return fail(h, RC_ERR_ARG, "patch of %d sites needs %zu B of LDS", maxS, h->lds_stage);
 	HIPCHK(h, stage_set_lds_limit(std::max<size_t>(h->lds_stage, 64 * 1024)));
 	HIPCHK(h, hipStreamSynchronize(h->stream));
+	return RC_OK;
+}
+
+extern "C" int rc_upload_cells(rc_handle *h, const rc_cell *cells, const double *phi) {
+	if(!h) return RC_ERR_ARG;
+	if(!cells || !phi) return fail(h, RC_ERR_ARG, "null cells/phi");
+	int rc = upload_layout(h, cells, phi, nullptr);
+	if(rc) return rc;
 	h->uploaded = true;
 	h->initialized = false;
 	return RC_OK;
@@ -601,7 +612,7 @@ extern "C" int rc_set_warm_start(rc_handle *h, int mode) {
 	return RC_OK;
 }
 
-static int step_dry(rc_handle *h, const double *gauss_dev) {
+static int step_dry(rc_handle *h, const double *gauss_dev, int step) {
 	int rc;
 	// predictor
 	if((rc = run_gather(h, h->d_phiA, false))) return rc;
@@ -609,6 +620,7 @@ static int step_dry(rc_handle *h, const double *gauss_dev) {
 	a.cf_set = h->cf_set;
 	a.fs_set = h->cur_set;
 	a.store = 1;
+	a.step = step;
 	a.phi_c = h->d_phiA;
 	a.phi_out = h->d_phiB;
 	a.gauss = gauss_dev;
@@ -626,17 +638,20 @@ static int step_dry(rc_handle *h, const double *gauss_dev) {
 	return RC_OK;
 }
 
-static int step_wet(rc_handle *h, const double *gauss_dev) {
+// returns RC_OK, an error, or -1 when the device halted (status flag set)
+static int step_wet(rc_handle *h, const double *gauss_dev, int step) {
 	int rc;
 	StageArgs a = stage_args(h);
 	a.cf_set = h->cf_set;
 	a.fs_set = h->cur_set;
 	a.store = 1;
+	a.step = step;
 	a.phi_c = h->d_phiA;
 	a.phi_out = h->d_phiB;
 	a.gauss = gauss_dev;
 	if((rc = run_stage(h, MODE_FS, a))) return rc;
 	if((rc = force_eval(h, h->d_phiB, false))) return rc;
+	if(h->h_st->flags) return -1;
 	a.cf_set = h->cf_set;
 	a.fs_set = h->cur_set;
 	a.store = 0;
@@ -645,7 +660,157 @@ static int step_wet(rc_handle *h, const double *gauss_dev) {
 	a.phi_out = h->d_phiA;
 	a.gauss = nullptr;
 	if((rc = run_stage(h, MODE_FS, a))) return rc;
+	// a growth request of this corrector must be served before the force
+	// evaluation that follows it
+	HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	if(h->h_st->flags) return -1;
 	if((rc = force_eval(h, h->d_phiA, false))) return rc;
+	if(h->h_st->flags) return -1;
+	return RC_OK;
+}
+
+static void to_rc_cell(const DevCell &d, rc_cell &c) {
+	c.LsubX = d.LX;
+	c.LsubY = d.LY;
+	c.offset[0] = d.offx;
+	c.offset[1] = d.offy;
+	c.corner = d.corner;
+	c.corner_old = d.corner_old;
+	c.x_sub_left = d.x_sub_left;
+	c.y_sub_bottom = d.y_sub_bottom;
+	c.CoM[0] = d.CoMx;
+	c.CoM[1] = d.CoMy;
+	c.CoM_old[0] = d.CoM_oldx;
+	c.CoM_old[1] = d.CoM_oldy;
+	c.unrap[0] = d.unrapx;
+	c.unrap[1] = d.unrapy;
+	c.Q00 = d.Q00;
+	c.Q01 = d.Q01;
+	c.thetaQ = d.thetaQ;
+	c.thetaQ_old = d.thetaQ_old;
+	c.area = d.area;
+	c.area_old = d.area_old;
+	c.sumF = d.sumF;
+	c.sumF_old = d.sumF_old;
+	c.perimeter = d.perimeter;
+	c.S00 = d.S00;
+	c.S01 = d.S01;
+	c.total_force_repulsion[0] = d.trx;
+	c.total_force_repulsion[1] = d.try_;
+}
+
+static int download_raw(rc_handle *h, std::vector<rc_cell> &cells, std::vector<double> &phi, std::vector<double> *rep) {
+	std::vector<DevCell> dc(h->ncells);
+	cells.resize(h->ncells);
+	phi.resize(h->rows);
+	HIPCHK(h, hipMemcpyAsync(dc.data(), h->d_cells, sizeof(DevCell) * h->ncells, hipMemcpyDeviceToHost, h->stream));
+	HIPCHK(h, hipMemcpyAsync(phi.data(), h->d_phiA, sizeof(double) * h->rows, hipMemcpyDeviceToHost, h->stream));
+	if(rep && h->jq) {
+		rep->resize(h->rows);
+		HIPCHK(h, hipMemcpyAsync(rep->data(), h->d_rep, sizeof(double) * h->rows, hipMemcpyDeviceToHost, h->stream));
+	}
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	for(int i = 0; i < h->ncells; ++i) to_rc_cell(dc[i], cells[i]);
+	return RC_OK;
+}
+
+// Host part of MultiPhaseField::set_positions (MultiPhaseField.cpp:248-310):
+// the patch is re-cut around the interface bounding box, border 4 sites.
+static int grow_patch(rc_handle *h, rc_cell &c, const double *phi, std::vector<double> &out, int side) {
+	auto bx = [side](int site, int d) {
+		int x = (site - ((site / side) * side)) + d;
+		while(x < 0) x += side;
+		while(x >= side) x -= side;
+		return x;
+	};
+	auto by = [side](int site, int d) {
+		int y = (site / side) + d;
+		while(y < 0) y += side;
+		while(y >= side) y -= side;
+		return y;
+	};
+	const int LX = c.LsubX, LY = c.LsubY;
+	const int nLX = LX - (2 * c.x_sub_left) + (2 * BORDER);
+	const int nLY = LY - (2 * c.y_sub_bottom) + (2 * BORDER);
+	if(nLX < 3 || nLY < 3 || nLX > side || nLY > side)
+		return fail(h, RC_ERR_GROWTH, "patch growth to %dx%d does not fit box %d", nLX, nLY, side);
+	const int nS = nLX * nLY;
+	const int ncorner = bx(c.corner, -(-(2 * c.x_sub_left) + (2 * BORDER)) / 2)
+			+ by(c.corner, -(-(2 * c.y_sub_bottom) + (2 * BORDER)) / 2) * side;
+	out.assign(nS, 0.);
+	double area = 0;
+	int sy, sx, dimy, dimx;
+	if(nLY < LY) { sy = (LY - nLY) / 2; dimy = nLY; } else { sy = 0; dimy = LY; }
+	if(nLX < LX) { sx = (LX - nLX) / 2; dimx = nLX; } else { sx = 0; dimx = LX; }
+	for(int i = sy; i < dimy; i++) {
+		const int yy = by(c.corner, i);
+		int off_y = i - c.offset[1];
+		if(off_y < 0) off_y += LY;
+		for(int j = sx; j < dimx; j++) {
+			const int xx = bx(c.corner, j);
+			int dx = (ncorner - (ncorner / side) * side) - xx;
+			if(dx > side / 2) dx -= side;
+			else if(dx < -side / 2) dx += side;
+			int dy = (ncorner / side) - yy;
+			if(dy > side / 2) dy -= side;
+			else if(dy < -side / 2) dy += side;
+			int off_x = j - c.offset[0];
+			if(off_x < 0) off_x += LX;
+			const int dst = std::abs(dx) + std::abs(dy) * nLX;
+			if(dst < 0 || dst >= nS) return fail(h, RC_ERR_GROWTH, "patch growth writes outside the new patch");
+			const double v = phi[off_x + off_y * LX];
+			out[dst] = v;
+			area += v * v;
+		}
+	}
+	const double c0x = bx(c.corner, 0), c0y = by(c.corner, 0);
+	const double c1x = bx(ncorner, 0), c1y = by(ncorner, 0);
+	c.unrap[0] += c1x - c0x - std::rint((c1x - c0x) / side) * side;
+	c.unrap[1] += c1y - c0y - std::rint((c1y - c0y) / side) * side;
+	c.area = area;
+	c.LsubX = nLX;
+	c.LsubY = nLY;
+	c.corner_old = c.corner;
+	c.corner = ncorner;
+	c.offset[0] = 0;
+	c.offset[1] = 0;
+	const int site = bx(ncorner, nLX / 2) + by(ncorner, nLY / 2) * side;
+	c.CoM[1] = (int) (site / side);
+	c.CoM[0] = (int) (site - (int) (site / side) * side);
+	return RC_OK;
+}
+
+// serves a growth request raised by the device: re-cuts the flagged patches,
+// re-lays out the rows and clears the halt
+static int serve_growth(rc_handle *h) {
+	std::vector<rc_cell> cells;
+	std::vector<double> phi, rep, nphi, nrep, tmp;
+	int rc;
+	if((rc = download_raw(h, cells, phi, &rep))) return rc;
+	long long start = 0;
+	for(int b = 0; b < h->nboxes; ++b) {
+		const DevBox &B = h->dboxes[b];
+		for(int i = B.cell_begin; i < B.cell_end; ++i) {
+			rc_cell &c = cells[i];
+			const int S = c.LsubX * c.LsubY;
+			if(c.x_sub_left >= BORDER && c.y_sub_bottom >= BORDER) {
+				nphi.insert(nphi.end(), phi.begin() + start, phi.begin() + start + S);
+				if(h->jq) nrep.insert(nrep.end(), rep.begin() + start, rep.begin() + start + S);
+			}
+			else {
+				if((rc = grow_patch(h, c, phi.data() + start, tmp, B.L))) return rc;
+				nphi.insert(nphi.end(), tmp.begin(), tmp.end());
+				if(h->jq) {  // std::vector::resize keeps the first min(old, new) entries
+					const int nS = c.LsubX * c.LsubY;
+					for(int q = 0; q < nS; ++q) nrep.push_back(q < S ? rep[start + q] : 0.);
+				}
+			}
+			start += S;
+		}
+	}
+	if((rc = upload_layout(h, cells.data(), nphi.data(), h->jq ? nrep.data() : nullptr))) return rc;
+	h->h_st->flags = 0;
 	return RC_OK;
 }
 
@@ -660,7 +825,8 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 		t1 = ev_get(h);
 		HIPCHK(h, hipEventRecord(t0, h->stream));
 	}
-	for(int s0 = 0; s0 < nsteps; s0 += chunk) {
+	int rc = RC_OK;
+	for(int s0 = 0; s0 < nsteps && rc == RC_OK; s0 += chunk) {
 		const int ns = std::min(chunk, nsteps - s0);
 		if(gauss) {
 			const long long need = (long long) ns * h->ncells;
@@ -674,15 +840,40 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 			HIPCHK(h, hipMemcpyAsync(h->d_gauss, gauss + (size_t) s0 * h->ncells, sizeof(double) * need,
 					hipMemcpyHostToDevice, h->stream));
 		}
-		for(int s = 0; s < ns; ++s) {
-			const double *g = gauss ? h->d_gauss + (size_t) s * h->ncells : nullptr;
-			int rc = h->wet ? step_wet(h, g) : step_dry(h, g);
-			if(rc) return rc;
+		int s = 0;
+		while(s < ns) {
+			bool halted = false;
+			for(; s < ns; ++s) {
+				const double *g = gauss ? h->d_gauss + (size_t) s * h->ncells : nullptr;
+				rc = h->wet ? step_wet(h, g, s) : step_dry(h, g, s);
+				if(rc == -1) {
+					halted = true;
+					rc = RC_OK;
+					break;
+				}
+				if(rc) return rc;
+			}
+			HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
+			HIPCHK(h, hipStreamSynchronize(h->stream));
+			const DevStatus st = *h->h_st;
+			if(st.flags == ST_GROWTH) {
+				// the device halted after the corrector of step st.step
+				if((rc = serve_growth(h))) return rc;
+				if(h->wet) {
+					if((rc = force_eval(h, h->d_phiA, false))) return rc;
+				}
+				else
+					h->cf_set = h->cur_set;
+				s = st.step + 1;
+				continue;
+			}
+			if(st.flags) break;
+			(void) halted;
 		}
 		if(gauss && s0 + chunk < nsteps) HIPCHK(h, hipStreamSynchronize(h->stream));  // d_gauss is reused
 	}
 	if(h->timing) HIPCHK(h, hipEventRecord(t1, h->stream));
-	int rc = check_status(h);
+	rc = check_status(h);
 	if(h->timing) {
 		float ms = 0.f;
 		(void) hipEventElapsedTime(&ms, t0, t1);
@@ -729,43 +920,11 @@ extern "C" int rc_download_cells(rc_handle *h, rc_cell *cells, double *phi) {
 	int rc;
 	if(h->initialized && !h->wet)
 		if((rc = materialize(h, false))) return rc;
-	std::vector<DevCell> dc(h->ncells);
-	HIPCHK(h, hipMemcpyAsync(dc.data(), h->d_cells, sizeof(DevCell) * h->ncells, hipMemcpyDeviceToHost, h->stream));
-	if(phi) HIPCHK(h, hipMemcpyAsync(phi, h->d_phiA, sizeof(double) * h->rows, hipMemcpyDeviceToHost, h->stream));
-	HIPCHK(h, hipStreamSynchronize(h->stream));
-	if(cells) {
-		for(int i = 0; i < h->ncells; ++i) {
-			const DevCell &d = dc[i];
-			rc_cell &c = cells[i];
-			c.LsubX = d.LX;
-			c.LsubY = d.LY;
-			c.offset[0] = d.offx;
-			c.offset[1] = d.offy;
-			c.corner = d.corner;
-			c.corner_old = d.corner_old;
-			c.x_sub_left = d.x_sub_left;
-			c.y_sub_bottom = d.y_sub_bottom;
-			c.CoM[0] = d.CoMx;
-			c.CoM[1] = d.CoMy;
-			c.CoM_old[0] = d.CoM_oldx;
-			c.CoM_old[1] = d.CoM_oldy;
-			c.unrap[0] = d.unrapx;
-			c.unrap[1] = d.unrapy;
-			c.Q00 = d.Q00;
-			c.Q01 = d.Q01;
-			c.thetaQ = d.thetaQ;
-			c.thetaQ_old = d.thetaQ_old;
-			c.area = d.area;
-			c.area_old = d.area_old;
-			c.sumF = d.sumF;
-			c.sumF_old = d.sumF_old;
-			c.perimeter = d.perimeter;
-			c.S00 = d.S00;
-			c.S01 = d.S01;
-			c.total_force_repulsion[0] = d.trx;
-			c.total_force_repulsion[1] = d.try_;
-		}
-	}
+	std::vector<rc_cell> c;
+	std::vector<double> p;
+	if((rc = download_raw(h, c, p, nullptr))) return rc;
+	if(cells) std::copy(c.begin(), c.end(), cells);
+	if(phi) std::copy(p.begin(), p.end(), phi);
 	return RC_OK;
 }
 

@@ -53,7 +53,7 @@ struct DevStatus {
 	unsigned flags;
 	int first_cell;
 	int site;
-	int count;
+	int step;        // step index (within the current rc_step call) that raised the flag
 };
 
 // CG state of the two simultaneous solves (x and y components share A)

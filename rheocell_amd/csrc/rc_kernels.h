@@ -14,6 +14,7 @@ struct StageArgs {
 	const DevParams *params;     // params[2*box + set]
 	int cf_set, fs_set;          // parameter sets of the force evaluation / field update
 	int store;                   // 1: predictor (first_step(true)), 0: corrector
+	int step;                    // step index, recorded with a growth request
 	int pure;                    // MODE_CF: no side effects on cells / Repulsion
 	int jq;                      // accumulate Repulsion and CIL sums (J_Q != 0)
 	int write_v;                 // MODE_FUSED: also write v rows

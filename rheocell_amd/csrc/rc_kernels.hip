@@ -421,8 +421,12 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 			G.offy = (int) (C.offy + LY - d1) % LY;
 		}
 		else {
-			// patch growth: left to the host (flag and halt)
-			if(atomicOr(&a.st->flags, ST_GROWTH) == 0u) a.st->first_cell = ci;
+			// patch growth (MultiPhaseField.cpp:248-310) is done by the host:
+			// flag it and halt; every later kernel returns at entry
+			if(atomicOr(&a.st->flags, ST_GROWTH) == 0u) {
+				a.st->first_cell = ci;
+				a.st->step = a.step;
+			}
 		}
 	}
 	G.CoMx = CoMx;

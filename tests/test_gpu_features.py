@@ -1,0 +1,170 @@
+"""GPU path vs the oracle across model options, batched uneven replicas,
+invariants and the reference's error conditions."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import OracleSim
+from rheocell_amd import confgen
+from rheocell_amd.conf import CellConf, Conf, RCexception
+from rheocell_amd.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def _max_dphi(sim, orcs):
+    cells, phi = sim.download()
+    start = 0
+    worst = 0.0
+    i = 0
+    for o in orcs:
+        for j in range(o.N):
+            c = cells[i]
+            S = c.LsubX * c.LsubY
+            worst = max(worst, float(np.abs(phi[start:start + S] - o.phi(j)).max()))
+            oc = o.cell(j)
+            assert (c.offset[0], c.offset[1], c.corner) == (oc.offset[0], oc.offset[1], oc.corner)
+            start += S
+            i += 1
+    return worst
+
+
+@pytest.mark.parametrize("opts", [dict(J_Q=0.5), dict(omega=0.02), dict(beta=0.3), dict(anchoring=1),
+                                  dict(J_Q=0.2, omega=0.01, beta=0.1, zetaQ_inter=0.3)],
+                         ids=["JQ", "omega", "beta", "anchoring", "all"])
+def test_dry_options(opts):
+    conf = confgen.lattice_conf(96, 4, seed=11)
+    keys = dict(confgen.BASE_KEYS, seed=11, **opts)
+    sim = Simulation([(conf, keys)])
+    orc = OracleSim(conf, keys)
+    sim.step(20)
+    orc.step(20)
+    w = _max_dphi(sim, [orc])
+    print(opts, "max|dphi| =", w)
+    assert w <= 1e-10
+    dv = np.abs(sim.rows(0) - orc.rows(0)).max()
+    assert dv <= 1e-10
+
+
+def test_uneven_replicas_one_launch():
+    rng = np.random.default_rng(21)
+    boxes, orcs = [], []
+    for b, L in enumerate((96, 128, 64)):
+        n = {96: 3, 128: 4, 64: 2}[L]
+        sides = [int(s) for s in rng.choice([29, 33, 41, 49], n * n)]
+        radii = [float(r) for r in rng.uniform(6, 10, n * n)]
+        conf = confgen.lattice_conf(L, n, seed=100 + b, radii=radii, sides=sides)
+        keys = dict(confgen.BASE_KEYS, seed=100 + b, zetaQ_self=float(rng.uniform(0.05, 0.2)))
+        boxes.append((conf, keys))
+        orcs.append(OracleSim(conf, keys))
+    sim = Simulation(boxes)
+    sim.step(15)
+    for o in orcs:
+        o.step(15)
+    w = _max_dphi(sim, orcs)
+    print("3 replicas, uneven patches: max|dphi| =", w)
+    assert w <= 1e-10
+
+
+def test_translation_invariance():
+    conf = confgen.lattice_conf(128, 3, seed=8)
+    sh = (37, 91)
+    cells = []
+    for c in conf.cells:
+        cx, cy = c.corner % 128, c.corner // 128
+        corner = (cx + sh[0]) % 128 + ((cy + sh[1]) % 128) * 128
+        cells.append(CellConf(c.LsubX, c.LsubY, ((c.CoM[0] + sh[0]) % 128, (c.CoM[1] + sh[1]) % 128), c.offset,
+                              corner, (c.unrap[0] + sh[0], c.unrap[1] + sh[1]), c.Q00, c.Q01, c.box_index, c.phi))
+    keys = dict(confgen.BASE_KEYS, seed=8)
+    a = Simulation([(conf, keys)])
+    b = Simulation([(Conf(0, 128, 128, cells), keys)])
+    a.step(12)
+    b.step(12)
+    assert np.array_equal(a.download()[1], b.download()[1])
+
+
+def test_site_arrays_dry_and_wet():
+    for fc in (0, 1):
+        conf = confgen.lattice_conf(64, 3, seed=13)
+        keys = dict(confgen.BASE_KEYS, seed=13, friction_cell=fc, CGtolerance=1e-13)
+        sim = Simulation([(conf, keys)])
+        orc = OracleSim(conf, keys)
+        sim.step(3)
+        orc.step(3)
+        for which in (1, 2, 3, 4):
+            ref = np.concatenate([orc.site(i, which) for i in range(orc.N)])
+            got = sim.site_array(which)
+            assert np.abs(got - ref).max() <= 1e-10, (fc, which)
+
+
+def test_equilibration_then_active():
+    conf = confgen.lattice_conf(64, 3, seed=17)
+    keys = dict(confgen.BASE_KEYS, seed=17, zetaQ_self=0.3, J_Q=0.1)
+    sim = Simulation([(conf, keys)], equilibrate=False)
+    orc = OracleSim(conf, keys, equilibrate=False)
+    sim.step(4)
+    orc.step(4)
+    sim.apply_changes_after_equilibration()
+    orc.L.orc_apply_changes_after_equilibration(orc.h)
+    sim.step(6)
+    orc.step(6)
+    assert _max_dphi(sim, [orc]) <= 1e-10
+
+
+def test_wet_reference_tolerance():
+    conf = confgen.lattice_conf(64, 3, seed=19)
+    keys = dict(confgen.BASE_KEYS, seed=19, friction_cell=1, CGtolerance=1e-8)
+    sim = Simulation([(conf, keys)])
+    orc = OracleSim(conf, keys)
+    for _ in range(12):  # crosses the cold restart after 10 warm solves
+        sim.step(1)
+        orc.step(1)
+        gi, oi = list(sim.cg_stats().last_iters), list(orc.cg_stats().last_iters)
+        assert all(abs(a - b) <= 2 for a, b in zip(gi, oi)), (gi, oi)
+        assert max(sim.cg_stats().last_relres) < 1e-8
+    w = _max_dphi(sim, [orc])
+    print("wet @1e-8, 12 steps: max|dphi| =", w)
+    assert w < 1e-7
+
+
+def test_overlap_limit_raises():
+    # 21 patches covering the same sites: the reference throws
+    L = 64
+    cells = [confgen.tanh_cell(L, 32.0 + 0.01 * i, 32.0, 8.0, 2.0, 41, 0.1 * i) for i in range(21)]
+    conf = Conf(0, L, L, cells)
+    keys = dict(confgen.BASE_KEYS, seed=1)
+    with pytest.raises(RuntimeError, match="Too many field patches overlap"):
+        OracleSim(conf, keys)
+    with pytest.raises(RCexception, match="RC_ERR_OVERLAP"):
+        sim = Simulation([(conf, keys)])
+        sim.step(1)
+
+
+def _growth_conf(L=96):
+    # a cell whose interface comes closer than 4 sites to the patch's left edge
+    c = confgen.tanh_cell(L, 40.0, 48.0, 8.0, 2.0, 41, 0.2)
+    phi = np.roll(c.phi.reshape(41, 41), -9, axis=1)
+    phi[:, -9:] = 0.0
+    cell = CellConf(41, 41, c.CoM, (0, 0), (c.corner + 9) % (L * L), (c.unrap[0] + 9, c.unrap[1]), c.Q00, c.Q01,
+                    c.box_index, phi.ravel().copy())
+    other = confgen.tanh_cell(L, 80.0, 20.0, 8.0, 2.0, 41, -1.0)
+    return Conf(0, L, L, [cell, other])
+
+
+@pytest.mark.parametrize("fc", [0, 1], ids=["dry", "wet"])
+def test_patch_growth_parity(fc):
+    conf = _growth_conf()
+    keys = dict(confgen.BASE_KEYS, seed=1, friction_cell=fc, CGtolerance=1e-13, J_Q=0.3)
+    orc = OracleSim(conf, keys)
+    sim = Simulation([(conf, keys)])
+    sim.step(1)
+    orc.step(1)
+    cells, _ = sim.download()
+    assert (cells[0].LsubX, cells[0].LsubY) == (orc.cell(0).LsubX, orc.cell(0).LsubY) != (41, 41)
+    sim.step(5)
+    orc.step(5)
+    w = _max_dphi(sim, [orc])
+    print("growth (%s): patch %dx%d, max|dphi| = %.3e" % ("wet" if fc else "dry", cells[0].LsubX, cells[0].LsubY, w))
+    assert w <= 1e-10
+    assert np.abs(sim.rows(0) - orc.rows(0)).max() <= 1e-10
