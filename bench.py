@@ -37,14 +37,15 @@ BYTES = {
     "stage_fs_corr": (48, 8),       # read phi*, v, phi_n, dphi_old; write phi_n+1
     "cg_diag": (0, 12),
     "cg_gather_x": (16, 16),        # read x; write box sums
-    "cg_gather_p_first": (32, 24),  # read r; write p; read diag, write box sums
-    "cg_gather_p": (48, 24),        # read r, p; write p; read diag, write box sums
-    "cg_row_resid": (48, 24),       # read x, b; write r; box window + diag
-    "cg_row_spmv": (16, 24),        # read p; box window + diag
-    "cg_row_update": (80, 24),      # read x, p, r; write x, r; box window + diag
-    "cg_scalar": (0, 0),
+    "cg_gather_p_first": (32, 40),  # read r; write p; read diag, write box sums of p, p^2
+    "cg_gather_p": (48, 40),        # read r, p; write p; read diag, write box sums of p, p^2
+    "cg_row_resid": (48, 24),       # read x, b; write r; read N4 and diag
+    "cg_box_n4": (0, 32),           # read box sums; write N4
+    "cg_box_pap": (0, 56),          # read box sums, P2, diag; write N4
+    "cg_row_update": (80, 24),      # read x, p, r; write x, r; read N4 and diag
+    "cg_reduce": (0, 0),
 }
-CG_SWEEP_KINDS = ("cg_row_spmv", "cg_row_update")
+CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update")
 
 
 def parse():

@@ -126,7 +126,9 @@ int rc_total_rows(rc_handle *h, long long *rows);
 int rc_cg_stats(rc_handle *h, rc_cg_stats_t *out);
 int rc_set_warm_start(rc_handle *h, int mode);
 /* Kernel timing with HIP events on the library's stream (enabled by
- * rc_enable_timing; adds an event pair around every launch).  rc_timing
+ * rc_enable_timing).  Single-kernel kinds record their events at kernel
+ * start and end (hipExtLaunchKernel), so they time the kernel alone;
+ * RC_T_CG_SOLVE and RC_T_STEP bracket a whole group of launches.  rc_timing
  * returns the summed device time (ms) and launch count of one kind since
  * timing was enabled. */
 #define RC_T_BOX_GATHER 0          /* k_box_gather: box sums of phi^2 */
@@ -139,13 +141,14 @@ int rc_set_warm_start(rc_handle *h, int mode);
 #define RC_T_CG_GATHER_X 7
 #define RC_T_CG_GATHER_P 8         /* p = z + beta p fused with the box gather of p */
 #define RC_T_CG_ROW_RESID 9
-#define RC_T_CG_ROW_SPMV 10        /* p.Ap */
+#define RC_T_CG_BOX_PAP 10         /* box pass: N4 of p and p.Ap -> alpha */
 #define RC_T_CG_ROW_UPDATE 11      /* x += alpha p, r -= alpha Ap, r.r, r.z */
-#define RC_T_CG_SCALAR 12
+#define RC_T_CG_BOX_N4 12          /* box pass: N4 of the initial guess */
 #define RC_T_CG_SOLVE 13           /* a whole x+y solve */
 #define RC_T_STEP 14               /* a whole rc_step call */
 #define RC_T_CG_GATHER_P_FIRST 15  /* p = z fused with the box gather of p */
-#define RC_NKINDS 16
+#define RC_T_CG_REDUCE 16          /* one-workgroup reduction of CG partial sums */
+#define RC_NKINDS 17
 int rc_timing(rc_handle *h, int kind, double *ms, long long *launches);
 int rc_enable_timing(rc_handle *h, int on);
 const char *rc_last_error(rc_handle *h);

@@ -60,7 +60,8 @@ struct rc_handle {
 	double *d_trig = nullptr;
 	double *d_phi2 = nullptr, *d_diag = nullptr;
 	int *d_cover = nullptr, *d_tile_cells = nullptr, *d_tile_count = nullptr;
-	double2 *d_sp = nullptr;
+	double2 *d_sp = nullptr, *d_p2 = nullptr, *d_n4 = nullptr;
+	double *d_part_box = nullptr;
 	double *d_part = nullptr;
 	CGState *d_cg = nullptr;
 	DevStatus *d_st = nullptr;
@@ -131,17 +132,22 @@ static hipEvent_t ev_get(rc_handle *h) {
 struct TimeScope {
 	rc_handle *h;
 	int kind;
+	bool group;
 	hipEvent_t b;
-	TimeScope(rc_handle *h_, int k) : h(h_), kind(k), b(nullptr) {
+	TimeScope(rc_handle *h_, int k) : h(h_), kind(k), group(k == RC_T_CG_SOLVE || k == RC_T_STEP), b(nullptr) {
 		if(h->timing) {
 			hipEvent_t a = ev_get(h);
 			b = ev_get(h);
-			(void) hipEventRecord(a, h->stream);
 			h->pairs.push_back({ kind, a, b });
+			if(group) (void) hipEventRecord(a, h->stream);
+			else set_launch_events(a, b);
 		}
 	}
 	~TimeScope() {
-		if(h->timing) (void) hipEventRecord(b, h->stream);
+		if(h->timing) {
+			if(group) (void) hipEventRecord(b, h->stream);
+			else set_launch_events(nullptr, nullptr);
+		}
 	}
 };
 
@@ -271,6 +277,9 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, dalloc(&h->d_tile_count, tiles));
 		HIPCHK(h, dalloc(&h->d_diag, sites));
 		HIPCHK(h, dalloc(&h->d_sp, sites));
+		HIPCHK(h, dalloc(&h->d_p2, sites));
+		HIPCHK(h, dalloc(&h->d_n4, sites));
+		HIPCHK(h, dalloc(&h->d_part_box, 2 * ((sites + 255) / 256)));
 		HIPCHK(h, dalloc(&h->d_part, (size_t) 6 * std::max(cells, 1)));
 		HIPCHK(h, dalloc(&h->d_cg, 1));
 		HIPCHK(h, hipMemset(h->d_cg, 0, sizeof(CGState)));
@@ -285,7 +294,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_phiA); dfree(h->d_phiB); dfree(h->d_dphi); dfree(h->d_rep);
 	dfree(h->d_v); dfree(h->d_f); dfree(h->d_r); dfree(h->d_p); dfree(h->d_trig);
 	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_count);
-	dfree(h->d_sp); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
+	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
@@ -475,6 +484,9 @@ static CGArgs cg_args(rc_handle *h) {
 	c.cover = h->d_cover;
 	c.diag = h->d_diag;
 	c.sp = h->d_sp;
+	c.p2 = h->d_p2;
+	c.n4 = h->d_n4;
+	c.part_box = h->d_part_box;
 	c.x = h->d_v;
 	c.r = h->d_r;
 	c.p = h->d_p;
@@ -497,19 +509,21 @@ static int cg_solve(rc_handle *h, bool cold) {
 	{
 		TimeScope ts(h, RC_T_CG_SOLVE);
 		if(cold) HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * h->rows, h->stream));
+		const int nbox_blk = (int) ((h->box_sites + 255) / 256);
 		{ TimeScope t(h, RC_T_CG_DIAG); HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream)); }
 		{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->lds_cg, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_SCALAR); HIPCHK(h, launch_cg_scalar(CG_SC_INIT, c, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream)); }
 		{ TimeScope t(h, RC_T_CG_GATHER_P_FIRST); HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream)); }
 		int batch = std::max(2, h->last_iters - 1);
 		long long done_iters = 0;
 		for(;;) {
 			for(int it = 0; it < batch; ++it) {
-				{ TimeScope t(h, RC_T_CG_ROW_SPMV); HIPCHK(h, launch_cg_row(CG_ROW_SPMV, c, h->lds_cg, h->stream)); }
-				{ TimeScope t(h, RC_T_CG_SCALAR); HIPCHK(h, launch_cg_scalar(CG_SC_ALPHA, c, h->stream)); }
-				{ TimeScope t(h, RC_T_CG_ROW_UPDATE); HIPCHK(h, launch_cg_row(CG_ROW_UPDATE, c, h->lds_cg, h->stream)); }
-				{ TimeScope t(h, RC_T_CG_SCALAR); HIPCHK(h, launch_cg_scalar(CG_SC_BETA, c, h->stream)); }
+				{ TimeScope t(h, RC_T_CG_BOX_PAP); HIPCHK(h, launch_cg_box(1, c, (int) h->box_sites, h->stream)); }
+				{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(1, c, nbox_blk, h->stream)); }
+				{ TimeScope t(h, RC_T_CG_ROW_UPDATE); HIPCHK(h, launch_cg_row(CG_ROW_UPDATE, c, h->stream)); }
+				{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(2, c, h->ncells, h->stream)); }
 				{ TimeScope t(h, RC_T_CG_GATHER_P); HIPCHK(h, launch_cg_gather(CG_GATHER_P_UPDATE, c, h->ntiles, h->stream)); }
 			}
 			done_iters += batch;

@@ -46,8 +46,7 @@ struct GatherArgs {
 };
 
 enum CGGatherMode { CG_GATHER_X = 0, CG_GATHER_P_FIRST = 1, CG_GATHER_P_UPDATE = 2 };
-enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_SPMV = 1, CG_ROW_UPDATE = 2 };
-enum CGScalarMode { CG_SC_INIT = 0, CG_SC_ALPHA = 1, CG_SC_BETA = 2 };
+enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_UPDATE = 2 };
 
 struct CGArgs {
 	const DevBox *boxes;
@@ -60,8 +59,11 @@ struct CGArgs {
 	const int *cover;
 	double *diag;                // D(k) = xi + eta * sum of the 4 neighbour covers
 	double2 *sp;                 // box sums of the gathered vector
+	double2 *p2;                 // box sums of p^2
+	double2 *n4;                 // sum of sp over the 4 box neighbours
 	double2 *x, *r, *p, *b;
 	double *part;                // per-cell partial sums, 6 per cell
+	double *part_box;            // per-workgroup partials of the box pass, 2 each
 	CGState *cg;
 	double tol;
 	int max_iter;
@@ -73,9 +75,13 @@ hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s);
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds_bytes, hipStream_t s);
 hipError_t launch_cg_diag(const CGArgs &a, int nbox_sites, hipStream_t s);
 hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s);
-hipError_t launch_cg_row(int mode, const CGArgs &a, size_t lds_bytes, hipStream_t s);
-hipError_t launch_cg_scalar(int mode, const CGArgs &a, hipStream_t s);
+hipError_t launch_cg_box(int pap, const CGArgs &a, int nsites, hipStream_t s);
+hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s);
+// what: 0 after the residual pass, 1 after p.Ap, 2 after the x,r update
+hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s);
 hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
 hipError_t stage_set_lds_limit(size_t bytes);
+// events recorded by the next launches at kernel start / end (null: off)
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
 
 }  // namespace rc

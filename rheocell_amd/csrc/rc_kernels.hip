@@ -23,9 +23,24 @@
 // every per-site expression is the IEEE evaluation of the reference source.
 #include "rc_kernels.h"
 
+#include <hip/hip_ext.h>
+
 #include <float.h>
 
 namespace rc {
+
+// Optional per-launch timing: when set, the next launches record these events
+// at kernel start/end (hipExtLaunchKernel), i.e. pure kernel durations.
+static thread_local hipEvent_t t_ev_start = nullptr, t_ev_stop = nullptr;
+void set_launch_events(hipEvent_t start, hipEvent_t stop) {
+	t_ev_start = start;
+	t_ev_stop = stop;
+}
+#define RC_LAUNCH(K, G, B, SH, S, ...)                                                         \
+	do {                                                                                       \
+		if(t_ev_start) hipExtLaunchKernelGGL(K, G, B, SH, S, t_ev_start, t_ev_stop, 0, __VA_ARGS__); \
+		else hipLaunchKernelGGL(K, G, B, SH, S, __VA_ARGS__);                                  \
+	} while(0)
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -166,7 +181,7 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 }
 
 hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s) {
-	hipLaunchKernelGGL(k_box_gather, dim3(ntiles), dim3(256), 0, s, a);
+	RC_LAUNCH(k_box_gather, dim3(ntiles), dim3(256), 0, s, a);
 	return hipGetLastError();
 }
 
@@ -470,16 +485,22 @@ hipError_t stage_set_lds_limit(size_t bytes) {
 
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
 	switch(mode) {
-	case MODE_FUSED: hipLaunchKernelGGL(k_stage<MODE_FUSED>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	case MODE_CF: hipLaunchKernelGGL(k_stage<MODE_CF>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	default: hipLaunchKernelGGL(k_stage<MODE_FS>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	case MODE_FUSED: RC_LAUNCH(k_stage<MODE_FUSED>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	case MODE_CF: RC_LAUNCH(k_stage<MODE_CF>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	default: RC_LAUNCH(k_stage<MODE_FS>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
 	}
 	return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Wet velocity solve: A v = f, (A v)_i = D(k_i) v_i - eta * sum_{d in S,W,E,N}
-// Sv(nbr_d(k_i)), Sv = box sum of v over the rows living at a site.
+// Wet velocity solve: A v = f, (A v)_i = D(k_i) v_i - eta * N4(k_i), with
+// Sv(k) = sum of v over the rows living at box site k and
+// N4(k) = Sv(S) + Sv(W) + Sv(E) + Sv(N).  Per CG iteration:
+//   k_cg_gather<P>  p = z + beta p (z = r / D), Sp(k), P2(k) = sum p^2 at k
+//   k_cg_box<1>     N4(k);  p.Ap = sum_k D(k) P2(k) - eta Sp(k) N4(k)  -> alpha
+//   k_cg_row<UPD>   x += alpha p, r -= alpha (D p - eta N4), r.r, r.z  -> beta
+// Reductions: per-workgroup partials, summed in a fixed order by a
+// one-workgroup k_cg_reduce launch that also runs the scalar recurrences.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cg_diag(CGArgs a, int nsites) {
 	const int k = blockIdx.x * 256 + threadIdx.x;
@@ -500,7 +521,116 @@ __global__ __launch_bounds__(256) void k_cg_diag(CGArgs a, int nsites) {
 }
 
 hipError_t launch_cg_diag(const CGArgs &a, int nsites, hipStream_t s) {
-	hipLaunchKernelGGL(k_cg_diag, dim3((nsites + 255) / 256), dim3(256), 0, s, a, nsites);
+	RC_LAUNCH(k_cg_diag, dim3((nsites + 255) / 256), dim3(256), 0, s, a, nsites);
+	return hipGetLastError();
+}
+
+// Block partials (valid in thread 0) -> part[j*nblk + blockIdx] (SoA); the
+// next launch, k_cg_reduce, sums them in a fixed order (deterministic).  A
+// last-workgroup ticket would save that launch, but its agent-scope release
+// costs an L2 write-back per workgroup on this multi-XCD part (measured 2x
+// slower per CG iteration).
+template <int NV>
+__device__ inline void block_partials(const double (&v)[NV], double *part) {
+	if(threadIdx.x == 0) {
+#pragma unroll
+		for(int j = 0; j < NV; ++j) part[j * gridDim.x + blockIdx.x] = v[j];
+	}
+}
+
+// scalar recurrences of Eigen's ConjugateGradient loop
+__device__ void cg_after_resid(CGState *cg, const double *tot, double tol, int max_iter) {
+	cg->max_iter = max_iter;
+	for(int c = 0; c < 2; ++c) {
+		const double bb = tot[c], rr = tot[2 + c], rz = tot[4 + c];
+		cg->bb[c] = bb;
+		cg->iters[c] = 0;
+		cg->beta[c] = 0.;
+		cg->alpha[c] = 0.;
+		cg->zero_rhs[c] = 0;
+		if(bb == 0) {
+			cg->zero_rhs[c] = 1;
+			cg->done[c] = 1;
+			cg->rn2[c] = 0.;
+			continue;
+		}
+		double thr = tol * tol * bb;
+		if(thr < DBL_MIN) thr = DBL_MIN;
+		cg->thr[c] = thr;
+		cg->rn2[c] = rr;
+		if(rr < thr) {
+			cg->done[c] = 1;
+			continue;
+		}
+		cg->done[c] = 0;
+		cg->absNew[c] = rz;
+	}
+	cg->active = !(cg->done[0] && cg->done[1]);
+}
+__device__ void cg_after_pap(CGState *cg, const double *tot) {
+	for(int c = 0; c < 2; ++c)
+		if(!cg->done[c]) cg->alpha[c] = cg->absNew[c] / tot[c];
+}
+__device__ void cg_after_update(CGState *cg, const double *tot) {
+	for(int c = 0; c < 2; ++c) {
+		if(cg->done[c]) continue;
+		cg->iters[c] += 1;
+		const double rn2 = tot[c];
+		cg->rn2[c] = rn2;
+		if(rn2 < cg->thr[c] || cg->iters[c] >= cg->max_iter) {
+			cg->done[c] = 1;
+			continue;
+		}
+		const double absOld = cg->absNew[c];
+		cg->absNew[c] = tot[2 + c];
+		cg->beta[c] = cg->absNew[c] / absOld;
+	}
+	cg->active = !(cg->done[0] && cg->done[1]);
+}
+
+// one workgroup of 1024: sums SoA partials of the previous kernel
+template <int WHAT>
+__global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
+	if(a.st->flags) return;
+	CGState *cg = a.cg;
+	if(WHAT != 0 && !cg->active) return;
+	constexpr int NV = WHAT == 0 ? 6 : WHAT == 1 ? 2 : 4;
+	const double *part = WHAT == 1 ? a.part_box : a.part;
+	__shared__ double s_red[16 * NV];
+	double acc[NV];
+#pragma unroll
+	for(int j = 0; j < NV; ++j) acc[j] = 0.;
+	for(int b = threadIdx.x; b < nblk; b += 1024) {
+#pragma unroll
+		for(int j = 0; j < NV; ++j) acc[j] += part[j * nblk + b];
+	}
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+	for(int j = 0; j < NV; ++j) acc[j] = wave_sum(acc[j]);
+	if(lane == 0) {
+#pragma unroll
+		for(int j = 0; j < NV; ++j) s_red[wave * NV + j] = acc[j];
+	}
+	__syncthreads();
+	if(threadIdx.x != 0) return;
+	double tot[NV];
+#pragma unroll
+	for(int j = 0; j < NV; ++j) {
+		double t = 0.;
+		for(int w = 0; w < 16; ++w) t += s_red[w * NV + j];
+		tot[j] = t;
+	}
+	if(WHAT == 0) cg_after_resid(cg, tot, a.tol, a.max_iter);
+	else if(WHAT == 1) cg_after_pap(cg, tot);
+	else cg_after_update(cg, tot);
+}
+
+hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s) {
+	switch(what) {
+	case 0: RC_LAUNCH(k_cg_reduce<0>, dim3(1), dim3(1024), 0, s, a, nblk); break;
+	case 1: RC_LAUNCH(k_cg_reduce<1>, dim3(1), dim3(1024), 0, s, a, nblk); break;
+	default: RC_LAUNCH(k_cg_reduce<2>, dim3(1), dim3(1024), 0, s, a, nblk); break;
+	}
 	return hipGetLastError();
 }
 
@@ -529,11 +659,11 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 	double inv = 0., b0 = 0., b1 = 0.;
 	bool d0 = false, d1 = false;
 	if(MODE != CG_GATHER_X) {
-		inv = 1. / a.diag[k];
+		inv = 1. / a.diag[k];   // Jacobi: z = invdiag .* r
 		d0 = cg->done[0]; d1 = cg->done[1];
 		b0 = cg->beta[0]; b1 = cg->beta[1];
 	}
-	double2 acc = make_double2(0., 0.);
+	double2 acc = make_double2(0., 0.), acc2 = make_double2(0., 0.);
 	for(int j = 0; j < n; ++j) {
 		int lx = x - s_cx[j];
 		if(lx < 0) lx += L;
@@ -559,85 +689,110 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 					v.y = d1 ? po.y : inv * rr.y + b1 * po.y;
 				}
 				a.p[row] = v;
+				acc2.x += v.x * v.x;
+				acc2.y += v.y * v.y;
 			}
 			acc.x += v.x;
 			acc.y += v.y;
 		}
 	}
 	a.sp[k] = acc;
+	if(MODE != CG_GATHER_X) a.p2[k] = acc2;
 }
 
 hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s) {
 	switch(mode) {
-	case CG_GATHER_X: hipLaunchKernelGGL(k_cg_gather<CG_GATHER_X>, dim3(ntiles), dim3(256), 0, s, a); break;
-	case CG_GATHER_P_FIRST: hipLaunchKernelGGL(k_cg_gather<CG_GATHER_P_FIRST>, dim3(ntiles), dim3(256), 0, s, a); break;
-	default: hipLaunchKernelGGL(k_cg_gather<CG_GATHER_P_UPDATE>, dim3(ntiles), dim3(256), 0, s, a); break;
+	case CG_GATHER_X: RC_LAUNCH(k_cg_gather<CG_GATHER_X>, dim3(ntiles), dim3(256), 0, s, a); break;
+	case CG_GATHER_P_FIRST: RC_LAUNCH(k_cg_gather<CG_GATHER_P_FIRST>, dim3(ntiles), dim3(256), 0, s, a); break;
+	default: RC_LAUNCH(k_cg_gather<CG_GATHER_P_UPDATE>, dim3(ntiles), dim3(256), 0, s, a); break;
 	}
+	return hipGetLastError();
+}
+
+// N4(k) and, for PAP, p.Ap reduced to alpha (Eigen: alpha = absNew / p.dot(A p))
+template <int PAP>
+__global__ __launch_bounds__(256) void k_cg_box(CGArgs a, int nsites) {
+	if(a.st->flags) return;
+	CGState *cg = a.cg;
+	if(PAP && !cg->active) return;
+	__shared__ double s_red[4 * 2];
+	const int k = blockIdx.x * 256 + threadIdx.x;
+	const DevBox B = a.boxes[0];
+	const double eta = a.params[a.set].friction_cell;
+	double part[2] = { 0., 0. };
+	if(k < nsites) {
+		const int L = B.L, y = k / L, x = k - y * L;
+		const int xe = x + 1 == L ? 0 : x + 1, xw = x == 0 ? L - 1 : x - 1;
+		const int yn = y + 1 == L ? 0 : y + 1, ys = y == 0 ? L - 1 : y - 1;
+		const int o = B.box_off;
+		const double2 sS = a.sp[o + ys * L + x], sW = a.sp[o + y * L + xw], sE = a.sp[o + y * L + xe],
+				sN = a.sp[o + yn * L + x];
+		const double2 n4 = make_double2(sS.x + sW.x + sE.x + sN.x, sS.y + sW.y + sE.y + sN.y);
+		a.n4[o + k] = n4;
+		if(PAP) {
+			const double D = a.diag[o + k];
+			const double2 sp = a.sp[o + k], p2 = a.p2[o + k];
+			part[0] = D * p2.x - eta * sp.x * n4.x;
+			part[1] = D * p2.y - eta * sp.y * n4.y;
+		}
+	}
+	if(!PAP) return;
+	block_sum<2>(part, s_red);
+	block_partials<2>(part, a.part_box);
+}
+
+hipError_t launch_cg_box(int pap, const CGArgs &a, int nsites, hipStream_t s) {
+	const dim3 g((nsites + 255) / 256);
+	if(pap) RC_LAUNCH(k_cg_box<1>, g, dim3(256), 0, s, a, nsites);
+	else RC_LAUNCH(k_cg_box<0>, g, dim3(256), 0, s, a, nsites);
 	return hipGetLastError();
 }
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 	if(a.st->flags) return;
-	const CGState *cg = a.cg;
+	CGState *cg = a.cg;
 	if(MODE != CG_ROW_RESID && !cg->active) return;
-	extern __shared__ double smem[];
 	__shared__ double s_red[4 * 6];
-	double2 *s_win = reinterpret_cast<double2 *>(smem);
 	const int ci = blockIdx.x, tid = threadIdx.x;
 	const DevCell C = a.cells[ci];
 	const DevBox B = a.boxes[C.box];
 	const double eta = a.params[a.set].friction_cell;
-	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L, WX = LX + 2, WY = LY + 2;
+	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L;
 	const int cx = C.corner % L, cy = C.corner / L;
-	for(int w = tid; w < WX * WY; w += 256) {
-		const int wy = w / WX, wx = w - wy * WX;
-		int bx = cx + wx - 1;
-		if(bx < 0) bx += L;
-		else if(bx >= L) bx -= L;
-		int by = cy + wy - 1;
-		if(by < 0) by += L;
-		else if(by >= L) by -= L;
-		s_win[w] = a.sp[B.box_off + by * L + bx];
-	}
 	double al0 = 0., al1 = 0.;
 	bool dn0 = false, dn1 = false;
 	if(MODE == CG_ROW_UPDATE) {
 		al0 = cg->alpha[0]; al1 = cg->alpha[1];
 		dn0 = cg->done[0]; dn1 = cg->done[1];
 	}
-	__syncthreads();
 	double red[6] = { 0., 0., 0., 0., 0., 0. };
 	for(int q = tid; q < S; q += 256) {
 		const int qy = q / LX, qx = q - qy * LX;
-		int lx = qx + C.offx;
-		if(lx >= LX) lx -= LX;
-		int ly = qy + C.offy;
-		if(ly >= LY) ly -= LY;
-		const int w = (ly + 1) * WX + (lx + 1);
-		int bx = cx + lx;
+		int bx = qx + C.offx;
+		if(bx >= LX) bx -= LX;
+		bx += cx;
 		if(bx >= L) bx -= L;
-		int by = cy + ly;
+		int by = qy + C.offy;
+		if(by >= LY) by -= LY;
+		by += cy;
 		if(by >= L) by -= L;
-		const double D = a.diag[B.box_off + by * L + bx];
+		const int k = B.box_off + by * L + bx;
+		const double D = a.diag[k];
+		const double2 n4 = a.n4[k];
 		const int row = C.start + q;
 		const double2 vin = MODE == CG_ROW_RESID ? a.x[row] : a.p[row];
-		const double2 sS = s_win[w - WX], sW = s_win[w - 1], sE = s_win[w + 1], sN = s_win[w + WX];
 		double2 t;
-		t.x = D * vin.x - eta * sS.x - eta * sW.x - eta * sE.x - eta * sN.x;
-		t.y = D * vin.y - eta * sS.y - eta * sW.y - eta * sE.y - eta * sN.y;
+		t.x = D * vin.x - eta * n4.x;
+		t.y = D * vin.y - eta * n4.y;
+		const double inv = 1. / D;
 		if(MODE == CG_ROW_RESID) {
 			const double2 bb = a.b[row];
 			const double2 rr = make_double2(bb.x - t.x, bb.y - t.y);
 			a.r[row] = rr;
-			const double inv = 1. / D;
 			red[0] += bb.x * bb.x; red[1] += bb.y * bb.y;
 			red[2] += rr.x * rr.x; red[3] += rr.y * rr.y;
 			red[4] += rr.x * (inv * rr.x); red[5] += rr.y * (inv * rr.y);
-		}
-		else if(MODE == CG_ROW_SPMV) {
-			red[0] += vin.x * t.x;
-			red[1] += vin.y * t.y;
 		}
 		else {
 			double2 xx = a.x[row], rr = a.r[row];
@@ -645,97 +800,24 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 			if(!dn1) { xx.y += al1 * vin.y; rr.y -= al1 * t.y; }
 			a.x[row] = xx;
 			a.r[row] = rr;
-			const double inv = 1. / D;
 			red[0] += rr.x * rr.x; red[1] += rr.y * rr.y;
 			red[2] += rr.x * (inv * rr.x); red[3] += rr.y * (inv * rr.y);
 		}
 	}
-	block_sum<6>(red, s_red);
-	if(tid == 0) {
-#pragma unroll
-		for(int j = 0; j < 6; ++j) a.part[ci * 6 + j] = red[j];
-	}
-}
-
-hipError_t launch_cg_row(int mode, const CGArgs &a, size_t lds, hipStream_t s) {
-	switch(mode) {
-	case CG_ROW_RESID: hipLaunchKernelGGL(k_cg_row<CG_ROW_RESID>, dim3(a.ncells), dim3(256), lds, s, a); break;
-	case CG_ROW_SPMV: hipLaunchKernelGGL(k_cg_row<CG_ROW_SPMV>, dim3(a.ncells), dim3(256), lds, s, a); break;
-	default: hipLaunchKernelGGL(k_cg_row<CG_ROW_UPDATE>, dim3(a.ncells), dim3(256), lds, s, a); break;
-	}
-	return hipGetLastError();
-}
-
-// one workgroup: deterministic reduction of the per-cell partials + scalar
-// recurrences of the CG (Eigen ConjugateGradient.h loop)
-template <int MODE>
-__global__ __launch_bounds__(256) void k_cg_scalar(CGArgs a) {
-	if(a.st->flags) return;
-	CGState *cg = a.cg;
-	if(MODE != CG_SC_INIT && !cg->active) return;
-	__shared__ double s_red[4 * 6];
-	double red[6] = { 0., 0., 0., 0., 0., 0. };
-	for(int c = threadIdx.x; c < a.ncells; c += 256) {
-#pragma unroll
-		for(int j = 0; j < 6; ++j) red[j] += a.part[c * 6 + j];
-	}
-	block_sum<6>(red, s_red);
-	if(threadIdx.x != 0) return;
-	if(MODE == CG_SC_INIT) {
-		for(int c = 0; c < 2; ++c) {
-			const double bb = red[c], rr = red[2 + c], rz = red[4 + c];
-			cg->bb[c] = bb;
-			cg->iters[c] = 0;
-			cg->beta[c] = 0.;
-			cg->alpha[c] = 0.;
-			cg->zero_rhs[c] = 0;
-			if(bb == 0) {
-				cg->zero_rhs[c] = 1;
-				cg->done[c] = 1;
-				cg->rn2[c] = 0.;
-				continue;
-			}
-			cg->max_iter = a.max_iter;
-			double thr = a.tol * a.tol * bb;
-			if(thr < DBL_MIN) thr = DBL_MIN;
-			cg->thr[c] = thr;
-			cg->rn2[c] = rr;
-			if(rr < thr) {
-				cg->done[c] = 1;
-				continue;
-			}
-			cg->done[c] = 0;
-			cg->absNew[c] = rz;
-		}
-	}
-	else if(MODE == CG_SC_ALPHA) {
-		for(int c = 0; c < 2; ++c)
-			if(!cg->done[c]) cg->alpha[c] = cg->absNew[c] / red[c];
+	if(MODE == CG_ROW_RESID) {
+		block_sum<6>(red, s_red);
+		block_partials<6>(red, a.part);
 	}
 	else {
-		for(int c = 0; c < 2; ++c) {
-			if(cg->done[c]) continue;
-			cg->iters[c] += 1;
-			const double rn2 = red[c];
-			cg->rn2[c] = rn2;
-			if(rn2 < cg->thr[c] || cg->iters[c] >= cg->max_iter) {
-				cg->done[c] = 1;
-				continue;
-			}
-			const double absOld = cg->absNew[c];
-			cg->absNew[c] = red[2 + c];
-			cg->beta[c] = cg->absNew[c] / absOld;
-		}
+		double r4[4] = { red[0], red[1], red[2], red[3] };
+		block_sum<4>(r4, s_red);
+		block_partials<4>(r4, a.part);
 	}
-	cg->active = !(cg->done[0] && cg->done[1]);
 }
 
-hipError_t launch_cg_scalar(int mode, const CGArgs &a, hipStream_t s) {
-	switch(mode) {
-	case CG_SC_INIT: hipLaunchKernelGGL(k_cg_scalar<CG_SC_INIT>, dim3(1), dim3(256), 0, s, a); break;
-	case CG_SC_ALPHA: hipLaunchKernelGGL(k_cg_scalar<CG_SC_ALPHA>, dim3(1), dim3(256), 0, s, a); break;
-	default: hipLaunchKernelGGL(k_cg_scalar<CG_SC_BETA>, dim3(1), dim3(256), 0, s, a); break;
-	}
+hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s) {
+	if(mode == CG_ROW_RESID) RC_LAUNCH(k_cg_row<CG_ROW_RESID>, dim3(a.ncells), dim3(256), 0, s, a);
+	else RC_LAUNCH(k_cg_row<CG_ROW_UPDATE>, dim3(a.ncells), dim3(256), 0, s, a);
 	return hipGetLastError();
 }
 
@@ -750,7 +832,7 @@ __global__ void k_zero_rows(double2 *x, long long rows, int mask) {
 
 hipError_t launch_zero_rows(double2 *x, long long rows, int mask, hipStream_t s) {
 	if(rows <= 0) return hipSuccess;
-	hipLaunchKernelGGL(k_zero_rows, dim3((unsigned) ((rows + 255) / 256)), dim3(256), 0, s, x, rows, mask);
+	RC_LAUNCH(k_zero_rows, dim3((unsigned) ((rows + 255) / 256)), dim3(256), 0, s, x, rows, mask);
 	return hipGetLastError();
 }
 

@@ -168,3 +168,15 @@ def test_patch_growth_parity(fc):
     print("growth (%s): patch %dx%d, max|dphi| = %.3e" % ("wet" if fc else "dry", cells[0].LsubX, cells[0].LsubY, w))
     assert w <= 1e-10
     assert np.abs(sim.rows(0) - orc.rows(0)).max() <= 1e-10
+
+
+def test_wet_solve_is_deterministic():
+    conf = confgen.lattice_conf(96, 5, seed=23)
+    keys = dict(confgen.BASE_KEYS, seed=23, friction_cell=1, CGtolerance=1e-8)
+    a = Simulation([(conf, keys)])
+    b = Simulation([(conf, keys)])
+    a.step(4)
+    b.step(4)
+    assert np.array_equal(a.download()[1], b.download()[1])
+    assert np.array_equal(a.rows(0), b.rows(0))
+    assert a.cg_stats().iters_total == b.cg_stats().iters_total
