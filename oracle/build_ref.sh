@@ -13,6 +13,11 @@ if [ ! -d "$REF" ]; then
   exit 0
 fi
 mkdir -p "$OUT"
+if [ -x "$OUT/ref_fields" ] && [ -x "$OUT/ref_fields_fast" ] && [ "$OUT/ref_fields" -nt "$HERE/ref_driver.cpp" ] \
+   && [ "$OUT/ref_fields" -nt "$0" ] && [ -z "${FORCE:-}" ]; then
+  echo "oracle/_ref is up to date (FORCE=1 rebuilds)"
+  exit 0
+fi
 SRCS="$REF/Fields/MultiPhaseField.cpp $REF/Fields/BaseField.cpp $REF/Boxes/SquareBox.cpp $REF/Boxes/BaseBox.cpp \
   $REF/Utilities/RCexception.cpp $REF/Utilities/Utils.cpp $REF/Utilities/Logger.cpp \
   $REF/Utilities/parse_input/parse_input.cpp $REF/Utilities/parse_input/getline.c"

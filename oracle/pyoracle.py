@@ -120,7 +120,7 @@ class OracleSim:
         for i, c in enumerate(conf.cells):
             phi = np.ascontiguousarray(c.phi, dtype=np.float64)
             rc = self.L.orc_set_cell(self.h, i, c.LsubX, c.LsubY, c.CoM[0], c.CoM[1], c.offset[0], c.offset[1],
-                                     c.corner, c.unrap[0], c.unrap[1], c.Q00, c.Q01, _dp(phi))
+                                     c.corner, int(c.unrap[0]), int(c.unrap[1]), c.Q00, c.Q01, _dp(phi))
             self._check(rc)
         if init:
             self._check(self.L.orc_init(self.h))

@@ -19,6 +19,26 @@ HIPCC_FLAGS = [
 ]
 
 
+HOST_SOURCES = [os.path.join(HERE, "host", "rc_host.cpp"), os.path.join(HERE, "host", "rheoCell_hip.cpp")]
+HOST_EXE = os.path.join(HERE, "rheoCell_hip")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ drop-in (FD_HIPBackend + SimManager + main), linked to librheocell.so."""
+    tmp = HOST_EXE + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")] + HOST_SOURCES + [
+        "-L" + HERE, "-lrheocell", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib",
+        "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building rheoCell_hip")
+    os.replace(tmp, HOST_EXE)
+    if verbose:
+        print("built", HOST_EXE)
+    return HOST_EXE
+
+
 def hipcc() -> str:
     for p in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
         if p and (os.path.isabs(p) and os.path.exists(p) or not os.path.isabs(p)):
@@ -41,3 +61,4 @@ def build_library(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_host(verbose=True)

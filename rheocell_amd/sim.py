@@ -56,7 +56,7 @@ def cell_from_conf(c: CellConf) -> _lib.rc_cell:
     x.x_sub_left, x.y_sub_bottom = c.LsubX, c.LsubY
     x.CoM[0], x.CoM[1] = c.CoM
     x.CoM_old[0], x.CoM_old[1] = c.CoM
-    x.unrap[0], x.unrap[1] = c.unrap
+    x.unrap[0], x.unrap[1] = int(c.unrap[0]), int(c.unrap[1])  # set_positions takes ints
     x.Q00, x.Q01 = c.Q00, c.Q01
     x.thetaQ = 0.5 * math.atan2(2 * c.Q01, 2 * c.Q00)
     x.thetaQ_old = x.thetaQ
