@@ -100,10 +100,14 @@ typedef struct rc_cg_stats_t {
 	double total_solve_ms;
 	long long sweeps_total;          /* launches of the iteration kernels that did work:
 	                                    sum over solves of max(iters_x, iters_y) */
+	long long learned_used;          /* solves that started from the network's guess */
+	long long learned_tried;         /* solves where the network's guess was evaluated */
 } rc_cg_stats_t;
 
 #define RC_WARM_REFERENCE 0   /* previous solution, cold restart every 11th solve (reference rule) */
 #define RC_WARM_PREVIOUS 1    /* previous solution, never a periodic cold restart */
+#define RC_WARM_LEARNED 2     /* guess of the attached network; the previous solution when
+                                 its initial residual is smaller (needs rc_attach_net) */
 
 typedef struct rc_handle rc_handle;
 
@@ -153,6 +157,41 @@ int rc_timing(rc_handle *h, int kind, double *ms, long long *launches);
 int rc_enable_timing(rc_handle *h, int on);
 const char *rc_last_error(rc_handle *h);
 void rc_destroy(rc_handle *h);
+
+/* ---- Learned CG warm start (no counterpart in the reference) ----------
+ * A 6-layer 3x3 convolutional network (32 channels, circular padding, ReLU,
+ * BF16 weights and activations, FP32 accumulation on MFMA) maps box fields --
+ * the change since the previous solve of the force sums (fx, fy summed over
+ * the patches covering a site) and sum phi^2 -- to the change of the box sums
+ * of the velocity.  Rows follow from v_i = (f_i + eta N4(k_i)) / D(k_i); the
+ * guess enters by an A-norm line search from the previous solution and is
+ * dropped when it does not lower the initial residual.
+ * Tensors are host float arrays, pixel-major: x [N][H][W][3], y [N][H][W][2].
+ * Weights: w [6][9][32][32] (layer, tap = 3*(dy+1)+(dx+1), out ch, in ch),
+ * b [6][32]; entries of absent channels (inputs >= 3 of layer 0, outputs >= 2
+ * of layer 5) are kept at zero. */
+typedef struct rc_net rc_net;
+int rc_net_create(int N, int H, int W, rc_net **out);
+int rc_net_set_params(rc_net *n, const float *w, const float *b);
+int rc_net_get_params(rc_net *n, float *w, float *b);
+int rc_net_forward(rc_net *n, const float *x, float *out);
+/* forward + MSE loss + backward, no update (gradient checks) */
+int rc_net_grads(rc_net *n, const float *x, const float *y, float *gw, float *gb, double *loss);
+/* forward + backward + one Adam step (beta 0.9 / 0.999) on the GPU */
+int rc_net_train_step(rc_net *n, const float *x, const float *y, float lr, double *loss);
+const char *rc_net_last_error(rc_net *n);
+void rc_net_destroy(rc_net *n);
+/* Use net (created with N = 1, H = W = box side) for RC_WARM_LEARNED.
+ * scale[0]: multiplier of the force sums, scale[1]: of sum phi^2,
+ * scale[2]: multiplier of the network output (-> velocity box sums). */
+int rc_attach_net(rc_handle *h, rc_net *n, const float scale[3]);
+/* Box fields of the last force evaluation, [L*L][5]: sum fx, sum fy,
+ * sum phi^2, sum vx, sum vy (wet model, one box). */
+int rc_snapshot_box(rc_handle *h, float *out);
+/* Record the box fields of every following wet solve (training data for the
+ * network), up to capacity solves; rc_get_recorded copies [count][L*L][5]. */
+int rc_record_solves(rc_handle *h, int capacity);
+int rc_get_recorded(rc_handle *h, float *out, int *count);
 
 #ifdef __cplusplus
 }

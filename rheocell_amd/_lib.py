@@ -15,6 +15,7 @@ RC_ERR = {1: "RC_ERR_ARG", 2: "RC_ERR_HIP", 3: "RC_ERR_OVERLAP", 4: "RC_ERR_GROW
           6: "RC_ERR_STATE", 7: "RC_ERR_TILE"}
 RC_WARM_REFERENCE = 0
 RC_WARM_PREVIOUS = 1
+RC_WARM_LEARNED = 2
 
 # timing kinds (rheocell.h RC_T_*)
 T_KINDS = ["box_gather", "stage_fused_pred", "stage_fused_corr", "stage_cf", "stage_fs_pred", "stage_fs_corr",
@@ -29,7 +30,10 @@ T_SYMBOL = {"box_gather": "k_box_gather", "stage_fused_pred": "k_stage<0>", "sta
 
 EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_after_equilibration", "rc_step",
            "rc_download_cells", "rc_get_rows", "rc_get_site_array", "rc_total_rows", "rc_cg_stats",
-           "rc_set_warm_start", "rc_timing", "rc_enable_timing", "rc_last_error", "rc_destroy"]
+           "rc_set_warm_start", "rc_timing", "rc_enable_timing", "rc_last_error", "rc_destroy",
+           "rc_net_create", "rc_net_set_params", "rc_net_get_params", "rc_net_forward", "rc_net_grads",
+           "rc_net_train_step", "rc_net_last_error", "rc_net_destroy", "rc_attach_net", "rc_snapshot_box",
+           "rc_record_solves", "rc_get_recorded"]
 
 
 class rc_model_params(C.Structure):
@@ -62,7 +66,8 @@ class rc_cell(C.Structure):
 class rc_cg_stats_t(C.Structure):
     _fields_ = [("solves", C.c_longlong), ("iters_total", C.c_longlong), ("last_iters", C.c_int * 2),
                 ("last_relres", C.c_double * 2), ("last_cold", C.c_int), ("last_solve_ms", C.c_double),
-                ("total_solve_ms", C.c_double), ("sweeps_total", C.c_longlong)]
+                ("total_solve_ms", C.c_double), ("sweeps_total", C.c_longlong),
+                ("learned_used", C.c_longlong), ("learned_tried", C.c_longlong)]
 
 
 _lib = None
@@ -95,6 +100,21 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.rc_last_error.restype = C.c_char_p
     L.rc_destroy.argtypes = [vp]
     L.rc_destroy.restype = None
+    fp = C.POINTER(C.c_float)
+    L.rc_net_create.argtypes = [ip, ip, ip, C.POINTER(vp)]
+    L.rc_net_set_params.argtypes = [vp, fp, fp]
+    L.rc_net_get_params.argtypes = [vp, fp, fp]
+    L.rc_net_forward.argtypes = [vp, fp, fp]
+    L.rc_net_grads.argtypes = [vp, fp, fp, fp, fp, dp]
+    L.rc_net_train_step.argtypes = [vp, fp, fp, C.c_float, dp]
+    L.rc_net_last_error.argtypes = [vp]
+    L.rc_net_last_error.restype = C.c_char_p
+    L.rc_net_destroy.argtypes = [vp]
+    L.rc_net_destroy.restype = None
+    L.rc_attach_net.argtypes = [vp, vp, fp]
+    L.rc_snapshot_box.argtypes = [vp, fp]
+    L.rc_record_solves.argtypes = [vp, ip]
+    L.rc_get_recorded.argtypes = [vp, fp, C.POINTER(C.c_int)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L

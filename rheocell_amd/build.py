@@ -7,7 +7,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = [os.path.join(HERE, "csrc", "rc_kernels.hip"), os.path.join(HERE, "csrc", "rc_api.hip")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("rc_kernels.hip", "rc_api.hip", "rc_conv.hip", "rc_net.hip")]
 OUT = os.path.join(HERE, "librheocell.so")
 
 HIPCC_FLAGS = [

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <map>
 #include <string>
@@ -27,6 +28,7 @@
 
 #include "../../include/rheocell.h"
 #include "rc_kernels.h"
+#include "rc_net.h"
 
 using namespace rc;
 
@@ -54,6 +56,7 @@ struct rc_handle {
 	DevBox *d_boxes = nullptr;
 	int *d_tile_box = nullptr;
 	DevCell *d_cells = nullptr;
+	int4 *d_geo = nullptr;        // compact per-cell geometry for the box gathers
 	DevParams *d_params = nullptr;
 	double *d_phiA = nullptr, *d_phiB = nullptr, *d_dphi = nullptr, *d_rep = nullptr;
 	double2 *d_v = nullptr, *d_f = nullptr, *d_r = nullptr, *d_p = nullptr;
@@ -74,6 +77,8 @@ struct rc_handle {
 
 	// state machine
 	bool uploaded = false, initialized = false;
+	bool lists_valid = false;     // tile lists match the current patch geometry
+	int npt = 7;
 	int cur_set = 0;              // parameter set currently applied
 	int cf_set = 0;               // set of the force evaluation preceding the next field update
 	int restart_solver = 0;
@@ -82,6 +87,16 @@ struct rc_handle {
 	int last_iters = 8;
 	rc_cg_stats_t cgs{};
 	std::string err;
+
+	// learned warm start
+	rc_net *net = nullptr;
+	float scale[3] = { 1.f, 1.f, 1.f };
+	double2 *d_xg = nullptr, *d_sfprev = nullptr;
+	double *d_rr = nullptr;
+	bool have_sfprev = false, pending_learned_stat = false;
+	float *d_feat = nullptr, *d_snap = nullptr;
+	float *d_rec = nullptr;         // recorded solves (training data): [cap][L*L][5]
+	int rec_cap = 0, rec_n = 0;
 
 	// timing
 	bool timing = false;
@@ -256,6 +271,9 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 	HIPCHK(h, dalloc(&h->d_boxes, nboxes));
 	HIPCHK(h, dalloc(&h->d_params, 2 * nboxes));
 	HIPCHK(h, dalloc(&h->d_cells, cells));
+	HIPCHK(h, dalloc(&h->d_geo, cells));
+	HIPCHK(h, dalloc(&h->d_tile_cells, (size_t) tiles * MAXT));
+	HIPCHK(h, dalloc(&h->d_tile_count, tiles));
 	HIPCHK(h, dalloc(&h->d_phi2, sites));
 	HIPCHK(h, dalloc(&h->d_cover, sites));
 	HIPCHK(h, dalloc(&h->d_st, 1));
@@ -273,8 +291,6 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 	HIPCHK(h, hipMemset(h->d_st, 0, sizeof(DevStatus)));
 	HIPCHK(h, hipMemset(h->d_phi2, 0, sizeof(double) * sites));
 	if(h->wet) {
-		HIPCHK(h, dalloc(&h->d_tile_cells, (size_t) tiles * MAXT));
-		HIPCHK(h, dalloc(&h->d_tile_count, tiles));
 		HIPCHK(h, dalloc(&h->d_diag, sites));
 		HIPCHK(h, dalloc(&h->d_sp, sites));
 		HIPCHK(h, dalloc(&h->d_p2, sites));
@@ -290,12 +306,13 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 extern "C" void rc_destroy(rc_handle *h) {
 	if(!h) return;
 	if(h->stream) (void) hipStreamSynchronize(h->stream);
-	dfree(h->d_boxes); dfree(h->d_tile_box); dfree(h->d_cells); dfree(h->d_params);
+	dfree(h->d_boxes); dfree(h->d_tile_box); dfree(h->d_cells); dfree(h->d_geo); dfree(h->d_params);
 	dfree(h->d_phiA); dfree(h->d_phiB); dfree(h->d_dphi); dfree(h->d_rep);
 	dfree(h->d_v); dfree(h->d_f); dfree(h->d_r); dfree(h->d_p); dfree(h->d_trig);
 	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_count);
 	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
+	dfree(h->d_xg); dfree(h->d_feat); dfree(h->d_snap); dfree(h->d_sfprev); dfree(h->d_rr); dfree(h->d_rec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
 	for(auto e : h->pool) (void) hipEventDestroy(e);
@@ -397,6 +414,11 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			HIPCHK(h, dalloc(&h->d_p, rows));
 		}
 		h->rows = rows;
+		if(h->d_xg) {
+			dfree(h->d_xg);
+			h->d_xg = nullptr;
+			HIPCHK(h, dalloc(&h->d_xg, rows));
+		}
 	}
 	if(h->jq) {
 		if(rep) HIPCHK(h, hipMemcpyAsync(h->d_rep, rep, sizeof(double) * rows, hipMemcpyHostToDevice, h->stream));
@@ -407,11 +429,18 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 	HIPCHK(h, dalloc(&h->d_trig, trig.size()));
 	HIPCHK(h, hipMemcpyAsync(h->d_trig, trig.data(), sizeof(double) * trig.size(), hipMemcpyHostToDevice, h->stream));
 	HIPCHK(h, hipMemcpyAsync(h->d_cells, dc.data(), sizeof(DevCell) * h->ncells, hipMemcpyHostToDevice, h->stream));
+	std::vector<int4> geo(h->ncells);
+	for(int i = 0; i < h->ncells; ++i)
+		geo[i] = make_int4(dc[i].corner, dc[i].LX | (dc[i].LY << 16), dc[i].offx | (dc[i].offy << 16), dc[i].start);
+	HIPCHK(h, hipMemcpyAsync(h->d_geo, geo.data(), sizeof(int4) * h->ncells, hipMemcpyHostToDevice, h->stream));
+	h->lists_valid = false;
 	HIPCHK(h, hipMemcpyAsync(h->d_phiA, phi, sizeof(double) * rows, hipMemcpyHostToDevice, h->stream));
 	HIPCHK(h, hipMemsetAsync(h->d_st, 0, sizeof(DevStatus), h->stream));
 	h->max_S = maxS;
 	h->max_W = maxW;
-	h->lds_stage = sizeof(double) * (size_t) (2 * maxS + maxW + (h->jq ? maxS : 0));
+	h->lds_stage = sizeof(double) * (size_t) (maxS + maxW + (h->jq ? maxS : 0));
+	h->npt = stage_npt(maxS);
+	if(h->npt < 0 || std::max(maxS, maxW) >= (1 << 16)) return fail(h, RC_ERR_ARG, "patch of %d sites is larger than the 64x64 supported", maxS);
 	h->lds_cg = sizeof(double2) * (size_t) maxW;
 	if(h->lds_stage > 160 * 1024) return fail(h, RC_ERR_ARG, "patch of %d sites needs %zu B of LDS", maxS, h->lds_stage);
 	HIPCHK(h, stage_set_lds_limit(std::max<size_t>(h->lds_stage, 64 * 1024)));
@@ -442,24 +471,29 @@ static StageArgs stage_args(rc_handle *h) {
 	a.rep = h->d_rep;
 	a.jq = h->jq ? 1 : 0;
 	a.dphi_old = h->d_dphi;
+	a.geo = h->d_geo;
+	a.npt = h->npt;
 	a.st = h->d_st;
 	return a;
 }
 
-static int run_gather(rc_handle *h, const double *phi, bool lists) {
+// box sums of phi^2 (k_box_gather); the tile lists are rebuilt only after the
+// patches moved (corrector recentering, upload, growth)
+static int run_gather(rc_handle *h, const double *phi) {
 	GatherArgs g{};
 	g.boxes = h->d_boxes;
 	g.tile_box = h->d_tile_box;
-	g.cells = h->d_cells;
+	g.geo = h->d_geo;
 	g.phi = phi;
 	g.phi2 = h->d_phi2;
 	g.cover = h->d_cover;
 	g.tile_cells = h->d_tile_cells;
 	g.tile_count = h->d_tile_count;
-	g.store_lists = lists ? 1 : 0;
+	g.build_lists = h->lists_valid ? 0 : 1;
 	g.st = h->d_st;
 	TimeScope ts(h, RC_T_BOX_GATHER);
 	HIPCHK(h, launch_gather(g, h->ntiles, h->stream));
+	h->lists_valid = true;
 	return RC_OK;
 }
 
@@ -469,6 +503,7 @@ static int run_stage(rc_handle *h, int mode, const StageArgs &a) {
 			: (a.store ? RC_T_STAGE_FS_PRED : RC_T_STAGE_FS_CORR);
 	TimeScope ts(h, kind);
 	HIPCHK(h, launch_stage(mode, a, h->ncells, h->lds_stage, h->stream));
+	if(mode != MODE_CF && !a.store) h->lists_valid = false;   // the corrector recentres patches
 	return RC_OK;
 }
 
@@ -477,6 +512,7 @@ static CGArgs cg_args(rc_handle *h) {
 	c.boxes = h->d_boxes;
 	c.tile_box = h->d_tile_box;
 	c.cells = h->d_cells;
+	c.geo = h->d_geo;
 	c.params = h->d_params;
 	c.set = h->cur_set;
 	c.tile_cells = h->d_tile_cells;
@@ -501,6 +537,87 @@ static CGArgs cg_args(rc_handle *h) {
 	return c;
 }
 
+// initial residual |b - A x|^2 (x + y components) of the rows in c.x
+static int resid_norm(rc_handle *h, const CGArgs &c, double *rn2) {
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
+	HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream));
+	HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream));
+	HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream));
+	HIPCHK(h, hipMemcpyAsync(h->h_cg, h->d_cg, sizeof(CGState), hipMemcpyDeviceToHost, h->stream));
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	*rn2 = h->h_cg->rn2[0] + h->h_cg->rn2[1];
+	return RC_OK;
+}
+
+// The learned initial guess.  The network maps the change of the force box
+// sums since the previous solve (and sum phi^2) to the change of the velocity
+// box sums; rows follow from v_i = (f_i + eta N4(k_i)) / D(k_i) (exact for
+// exact box sums).  The guess enters through an A-norm line search from the
+// previous solution x along d = x_net - x, with A d = r0 - r_net; the step is
+// dropped (previous guess kept) when it does not lower the initial residual.
+// Leaves x, r and the CG set-up of the chosen start in place.
+static int learned_guess(rc_handle *h, CGArgs c) {
+	const int ns = (int) h->box_sites;
+	CGArgs g = c;
+	g.x = h->d_f;
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, g, h->ntiles, h->stream));
+	HIPCHK(h, launch_box_features(g, h->d_sfprev, h->have_sfprev ? 1 : 0, h->d_phi2, h->d_feat, h->scale[0], h->scale[1], ns,
+			h->stream));
+	h->have_sfprev = true;
+	if(net_forward_device(h->net, h->d_feat, h->stream))
+		return fail(h, RC_ERR_HIP, "network forward pass failed: %s", rc_net_last_error(h->net));
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));     // sum of the previous solution
+	HIPCHK(h, launch_box_add_net(c, net_output(h->net), h->scale[2], ns, h->stream));
+	HIPCHK(h, launch_cg_box(0, c, ns, h->stream));
+	CGArgs q = c;
+	q.r = h->d_xg;
+	HIPCHK(h, launch_cg_row(CG_ROW_GUESS, q, h->stream));
+	// r0 = b - A x (into r), rg = b - A x_net (into p), their norms
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
+	HIPCHK(h, launch_cg_box(0, c, ns, h->stream));
+	HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream));
+	HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream));
+	HIPCHK(h, hipMemcpyAsync(h->d_rr, reinterpret_cast<char *>(h->d_cg) + offsetof(CGState, rn2), 2 * sizeof(double),
+			hipMemcpyDeviceToDevice, h->stream));
+	CGArgs x = c;
+	x.x = h->d_xg;
+	x.r = h->d_p;
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, x, h->ntiles, h->stream));
+	HIPCHK(h, launch_cg_box(0, x, ns, h->stream));
+	HIPCHK(h, launch_cg_row(CG_ROW_RESID, x, h->stream));
+	HIPCHK(h, launch_cg_reduce(0, x, h->ncells, h->stream));
+	HIPCHK(h, hipMemcpyAsync(h->d_rr + 2, reinterpret_cast<char *>(h->d_cg) + offsetof(CGState, rn2), 2 * sizeof(double),
+			hipMemcpyDeviceToDevice, h->stream));
+	CGArgs l = c;
+	l.xg = h->d_xg;
+	l.rg = h->d_p;
+	l.rr0 = h->d_rr;
+	l.rrg = h->d_rr + 2;
+	HIPCHK(h, launch_cg_row(CG_ROW_LINE1, l, h->stream));
+	HIPCHK(h, launch_cg_reduce(3, l, h->ncells, h->stream));
+	HIPCHK(h, launch_cg_row(CG_ROW_LINE2, l, h->stream));
+	HIPCHK(h, launch_cg_reduce(0, l, h->ncells, h->stream));
+	h->cgs.learned_tried += 1;
+	h->pending_learned_stat = true;
+	return -1;
+}
+
+// appends (sum f, sum phi^2, sum v) of the solve just finished to the record
+static int record_solve(rc_handle *h) {
+	const int ns = (int) h->box_sites;
+	CGArgs c = cg_args(h);
+	float *dst = h->d_rec + (size_t) h->rec_n * 5 * ns;
+	c.x = h->d_f;
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 0, ns, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 2, ns, h->stream));
+	c.x = h->d_v;
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 3, ns, h->stream));
+	h->rec_n += 1;
+	return RC_OK;
+}
+
 // Eigen ConjugateGradient solve (solve() when cold, solveWithGuess() otherwise)
 static int cg_solve(rc_handle *h, bool cold) {
 	CGArgs c = cg_args(h);
@@ -511,10 +628,19 @@ static int cg_solve(rc_handle *h, bool cold) {
 		if(cold) HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * h->rows, h->stream));
 		const int nbox_blk = (int) ((h->box_sites + 255) / 256);
 		{ TimeScope t(h, RC_T_CG_DIAG); HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream)); }
+		bool need_resid = true;
+		if(h->warm_mode == RC_WARM_LEARNED && h->net) {
+			int rc = learned_guess(h, c);
+			if(rc < 0) need_resid = false;   // the guess won; its residual is in place
+			else if(rc) return rc;
+			c.x = h->d_v;
+		}
+		if(need_resid) {
+			{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
+			{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
+			{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }
+			{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream)); }
+		}
 		{ TimeScope t(h, RC_T_CG_GATHER_P_FIRST); HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream)); }
 		int batch = std::max(2, h->last_iters - 1);
 		long long done_iters = 0;
@@ -544,6 +670,14 @@ static int cg_solve(rc_handle *h, bool cold) {
 	float ms = 0.f;
 	(void) hipEventElapsedTime(&ms, e0, e1);
 	const CGState &s = *h->h_cg;
+	if(h->pending_learned_stat) {
+		if(s.alpha0[0] != 0 || s.alpha0[1] != 0) h->cgs.learned_used += 1;
+		h->pending_learned_stat = false;
+	}
+	if(h->d_rec && h->rec_n < h->rec_cap) {
+		int rc = record_solve(h);
+		if(rc) return rc;
+	}
 	h->cgs.solves += 2;
 	h->cgs.iters_total += s.iters[0] + s.iters[1];
 	h->cgs.sweeps_total += std::max(s.iters[0], s.iters[1]);
@@ -562,7 +696,7 @@ static int cg_solve(rc_handle *h, bool cold) {
 // right-hand side (or dry velocity) and, for the wet solve, the CG.
 static int force_eval(rc_handle *h, double *phi, bool pure) {
 	int rc;
-	if((rc = run_gather(h, phi, h->wet))) return rc;
+	if((rc = run_gather(h, phi))) return rc;
 	StageArgs a = stage_args(h);
 	a.cf_set = h->cur_set;
 	a.fs_set = h->cur_set;
@@ -570,7 +704,7 @@ static int force_eval(rc_handle *h, double *phi, bool pure) {
 	a.pure = pure ? 1 : 0;
 	if((rc = run_stage(h, MODE_CF, a))) return rc;
 	// restart bookkeeping of WetPolarModel.cpp:151-157
-	if(h->restart_solver < 10 || h->warm_mode == RC_WARM_PREVIOUS) h->size_rows_old = h->size_rows;
+	if(h->restart_solver < 10 || h->warm_mode != RC_WARM_REFERENCE) h->size_rows_old = h->size_rows;
 	else {
 		h->size_rows_old = 0;
 		h->restart_solver = 0;
@@ -621,7 +755,9 @@ extern "C" int rc_apply_changes_after_equilibration(rc_handle *h) {
 
 extern "C" int rc_set_warm_start(rc_handle *h, int mode) {
 	if(!h) return RC_ERR_ARG;
-	if(mode != RC_WARM_REFERENCE && mode != RC_WARM_PREVIOUS) return fail(h, RC_ERR_ARG, "unknown warm-start mode %d", mode);
+	if(mode != RC_WARM_REFERENCE && mode != RC_WARM_PREVIOUS && mode != RC_WARM_LEARNED)
+		return fail(h, RC_ERR_ARG, "unknown warm-start mode %d", mode);
+	if(mode == RC_WARM_LEARNED && !h->net) return fail(h, RC_ERR_STATE, "RC_WARM_LEARNED needs rc_attach_net first");
 	h->warm_mode = mode;
 	return RC_OK;
 }
@@ -629,7 +765,7 @@ extern "C" int rc_set_warm_start(rc_handle *h, int mode) {
 static int step_dry(rc_handle *h, const double *gauss_dev, int step) {
 	int rc;
 	// predictor
-	if((rc = run_gather(h, h->d_phiA, false))) return rc;
+	if((rc = run_gather(h, h->d_phiA))) return rc;
 	StageArgs a = stage_args(h);
 	a.cf_set = h->cf_set;
 	a.fs_set = h->cur_set;
@@ -640,7 +776,7 @@ static int step_dry(rc_handle *h, const double *gauss_dev, int step) {
 	a.gauss = gauss_dev;
 	if((rc = run_stage(h, MODE_FUSED, a))) return rc;
 	// corrector
-	if((rc = run_gather(h, h->d_phiB, false))) return rc;
+	if((rc = run_gather(h, h->d_phiB))) return rc;
 	a.cf_set = h->cur_set;
 	a.store = 0;
 	a.phi_c = h->d_phiB;
@@ -907,7 +1043,7 @@ static int materialize(rc_handle *h, bool site_outputs) {
 			if(!p) HIPCHK(h, dalloc(&p, h->rows));
 	}
 	int rc;
-	if((rc = run_gather(h, h->d_phiA, false))) return rc;
+	if((rc = run_gather(h, h->d_phiA))) return rc;
 	StageArgs a = stage_args(h);
 	a.cf_set = h->cf_set;
 	a.fs_set = h->cf_set;
@@ -997,5 +1133,68 @@ extern "C" int rc_timing(rc_handle *h, int kind, double *ms, long long *launches
 	if(!h || kind < 0 || kind >= RC_NKINDS) return RC_ERR_ARG;
 	if(ms) *ms = h->t_ms[kind];
 	if(launches) *launches = h->t_n[kind];
+	return RC_OK;
+}
+
+extern "C" int rc_attach_net(rc_handle *h, rc_net *n, const float scale[3]) {
+	if(!h) return RC_ERR_ARG;
+	if(!h->wet || h->nboxes != 1) return fail(h, RC_ERR_ARG, "the learned warm start needs the wet model on one box");
+	if(!n) {
+		h->net = nullptr;
+		if(h->warm_mode == RC_WARM_LEARNED) h->warm_mode = RC_WARM_PREVIOUS;
+		return RC_OK;
+	}
+	int N, H, W;
+	net_dims(n, &N, &H, &W);
+	if(N != 1 || H != h->boxes[0].L || W != h->boxes[0].L)
+		return fail(h, RC_ERR_ARG, "network shape %dx%dx%d does not match the %d^2 box", N, H, W, h->boxes[0].L);
+	if(!h->d_xg) {
+		HIPCHK(h, dalloc(&h->d_xg, std::max<long long>(h->rows, 1)));
+		HIPCHK(h, dalloc(&h->d_feat, 3 * h->box_sites));
+		HIPCHK(h, dalloc(&h->d_sfprev, h->box_sites));
+		HIPCHK(h, dalloc(&h->d_rr, 4));
+	}
+	h->have_sfprev = false;
+	h->net = n;
+	for(int k = 0; k < 3; ++k) h->scale[k] = scale ? scale[k] : 1.f;
+	return RC_OK;
+}
+
+extern "C" int rc_snapshot_box(rc_handle *h, float *out) {
+	if(!h || !out) return RC_ERR_ARG;
+	if(!h->wet || h->nboxes != 1) return fail(h, RC_ERR_ARG, "snapshots exist for the wet model on one box");
+	if(!h->initialized) return fail(h, RC_ERR_STATE, "no force evaluation yet");
+	const int ns = (int) h->box_sites;
+	if(!h->d_snap) HIPCHK(h, dalloc(&h->d_snap, 5 * (size_t) ns));
+	CGArgs c = cg_args(h);
+	c.x = h->d_f;
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_snap, 0, ns, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_snap, 2, ns, h->stream));
+	c.x = h->d_v;
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_snap, 3, ns, h->stream));
+	HIPCHK(h, hipMemcpyAsync(out, h->d_snap, sizeof(float) * 5 * (size_t) ns, hipMemcpyDeviceToHost, h->stream));
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	return RC_OK;
+}
+
+extern "C" int rc_record_solves(rc_handle *h, int capacity) {
+	if(!h || capacity < 0) return RC_ERR_ARG;
+	if(!h->wet || h->nboxes != 1) return fail(h, RC_ERR_ARG, "recording needs the wet model on one box");
+	dfree(h->d_rec);
+	h->d_rec = nullptr;
+	h->rec_cap = h->rec_n = 0;
+	if(capacity == 0) return RC_OK;
+	HIPCHK(h, dalloc(&h->d_rec, (size_t) capacity * 5 * h->box_sites));
+	h->rec_cap = capacity;
+	return RC_OK;
+}
+
+extern "C" int rc_get_recorded(rc_handle *h, float *out, int *count) {
+	if(!h || !count) return RC_ERR_ARG;
+	*count = h->rec_n;
+	if(out && h->rec_n)
+		HIPCHK(h, hipMemcpy(out, h->d_rec, sizeof(float) * (size_t) h->rec_n * 5 * h->box_sites, hipMemcpyDeviceToHost));
 	return RC_OK;
 }

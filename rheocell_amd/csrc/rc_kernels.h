@@ -29,29 +29,33 @@ struct StageArgs {
 	double *rep;                 // Repulsion rows (jq)
 	double *out_mu, *out_dx, *out_dy, *out_lap;   // optional site outputs (MODE_CF)
 	const double *gauss;         // predictor noise, one per cell
+	int4 *geo;                   // compact geometry, rewritten at recentering
+	int npt;                     // sites per thread (stage_npt of the largest patch)
 	DevStatus *st;
 };
 
+// compact per-cell geometry: {corner, LX | LY << 16, offx | offy << 16, row start}
 struct GatherArgs {
 	const DevBox *boxes;
 	const int *tile_box;
-	const DevCell *cells;
+	const int4 *geo;
 	const double *phi;
 	double *phi2;
 	int *cover;
-	int *tile_cells;             // written when store_lists
+	int *tile_cells;             // cells intersecting each tile, cell order
 	int *tile_count;
-	int store_lists;
+	int build_lists;             // 1: rebuild the lists from geo, 0: reuse them
 	DevStatus *st;
 };
 
 enum CGGatherMode { CG_GATHER_X = 0, CG_GATHER_P_FIRST = 1, CG_GATHER_P_UPDATE = 2 };
-enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_UPDATE = 2 };
+enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_UPDATE = 2, CG_ROW_GUESS = 3, CG_ROW_LINE1 = 4, CG_ROW_LINE2 = 5 };
 
 struct CGArgs {
 	const DevBox *boxes;
 	const int *tile_box;
 	const DevCell *cells;
+	const int4 *geo;
 	const DevParams *params;     // wet parameters (params[2*0 + set])
 	int set;
 	const int *tile_cells;
@@ -62,10 +66,12 @@ struct CGArgs {
 	double2 *p2;                 // box sums of p^2
 	double2 *n4;                 // sum of sp over the 4 box neighbours
 	double2 *x, *r, *p, *b;
+	double2 *xg, *rg;            // learned guess rows and their residual
 	double *part;                // per-cell partial sums, 6 per cell
 	double *part_box;            // per-workgroup partials of the box pass, 2 each
 	CGState *cg;
 	double tol;
+	const double *rr0, *rrg;     // |r0|^2, |rg|^2 per component (line search)
 	int max_iter;
 	int ncells;
 	DevStatus *st;
@@ -77,10 +83,18 @@ hipError_t launch_cg_diag(const CGArgs &a, int nbox_sites, hipStream_t s);
 hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s);
 hipError_t launch_cg_box(int pap, const CGArgs &a, int nsites, hipStream_t s);
 hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s);
-// what: 0 after the residual pass, 1 after p.Ap, 2 after the x,r update
+// what: 0 after the residual pass, 1 after p.Ap, 2 after the x,r update, 3 line search
 hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s);
+// learned guess helpers: features ((sum f - previous sum f) * sf, sum phi^2 * sp2) -> feat [L*L][3]
+// and sfprev <- sum f; sp += net output * sv
+hipError_t launch_box_features(const CGArgs &a, double2 *sfprev, int have_prev, const double *phi2, float *feat, float sf,
+		float sp2, int nsites, hipStream_t s);
+hipError_t launch_box_add_net(const CGArgs &a, const float *out, float sv, int nsites, hipStream_t s);
+hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap, int chan, int nsites, hipStream_t s);
 hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
 hipError_t stage_set_lds_limit(size_t bytes);
+// sites-per-thread instantiation for a largest patch of max_sites (-1: too large)
+int stage_npt(int max_sites);
 // events recorded by the next launches at kernel start / end (null: off)
 void set_launch_events(hipEvent_t start, hipEvent_t stop);
 

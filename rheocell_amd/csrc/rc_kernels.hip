@@ -100,77 +100,120 @@ __device__ inline bool circ_overlap(int a0, int alen, int b0, int blen, int L) {
 }
 
 // ---------------------------------------------------------------------------
-// k_box_gather: phi2[k] = sum over covering patches (cell order) of phi^2,
-// cover[k] = number of covering patches (the reference's overlap list size)
+// Tile lists: cells whose patch intersects a 32x8 box tile, in cell order.
+// Built from the compact per-cell geometry (geo) by one pass over the
+// box's cells; the corrector's gather reuses the predictor's lists (the
+// patches only move at the corrector's recentering).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
-	if(a.st->flags) return;
-	__shared__ int s_cell[MAXT], s_cx[MAXT], s_cy[MAXT], s_lx[MAXT], s_ly[MAXT], s_ox[MAXT], s_oy[MAXT],
-			s_start[MAXT];
-	__shared__ int s_wcnt[4];
-	__shared__ int s_n;
-	const int tile = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const DevBox B = a.boxes[a.tile_box[tile]];
-	const int t = tile - B.tile_off, L = B.L;
-	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
-	const int tw = min(TW, L - x0), th = min(TH, L - y0);
-	if(tid == 0) s_n = 0;
+__device__ inline void unpack_geo(const int4 g, int L, int &cx, int &cy, int &lx, int &ly, int &ox, int &oy, int &st) {
+	cx = g.x % L;
+	cy = g.x / L;
+	lx = g.y & 0xFFFF;
+	ly = g.y >> 16;
+	ox = g.z & 0xFFFF;
+	oy = g.z >> 16;
+	st = g.w;
+}
+
+// loads the tile's list into LDS (building it when build != 0); returns its length
+__device__ int tile_list(const GatherArgs &a, int tile, const DevBox &B, int x0, int y0, int tw, int th, int build,
+		int *s_cx, int *s_cy, int *s_lx, int *s_ly, int *s_ox, int *s_oy, int *s_start, int *s_wcnt, int *s_n) {
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, L = B.L;
+	if(!build) {
+		const int n = a.tile_count[tile];
+		for(int j = tid; j < n; j += 256)
+			unpack_geo(a.geo[a.tile_cells[tile * MAXT + j]], L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
+		__syncthreads();
+		return n;
+	}
+	if(tid == 0) *s_n = 0;
 	__syncthreads();
 	for(int base = B.cell_begin; base < B.cell_end; base += 256) {
 		const int c = base + tid;
 		bool hit = false;
 		int cx = 0, cy = 0, lx = 0, ly = 0, ox = 0, oy = 0, st = 0;
 		if(c < B.cell_end) {
-			const DevCell &C = a.cells[c];
-			cx = C.corner % L;
-			cy = C.corner / L;
-			lx = C.LX; ly = C.LY; ox = C.offx; oy = C.offy; st = C.start;
+			unpack_geo(a.geo[c], L, cx, cy, lx, ly, ox, oy, st);
 			hit = circ_overlap(cx, lx, x0, tw, L) && circ_overlap(cy, ly, y0, th, L);
 		}
 		const unsigned long long m = __ballot(hit);
 		const int pre = __popcll(m & ((1ull << lane) - 1ull));
 		if(lane == 0) s_wcnt[wave] = __popcll(m);
 		__syncthreads();
-		int off = s_n;
+		int off = *s_n;
 		for(int w = 0; w < wave; ++w) off += s_wcnt[w];
 		const int pos = off + pre;
 		if(hit && pos < MAXT) {
-			s_cell[pos] = c; s_cx[pos] = cx; s_cy[pos] = cy; s_lx[pos] = lx; s_ly[pos] = ly;
+			s_cx[pos] = cx; s_cy[pos] = cy; s_lx[pos] = lx; s_ly[pos] = ly;
 			s_ox[pos] = ox; s_oy[pos] = oy; s_start[pos] = st;
+			a.tile_cells[tile * MAXT + pos] = c;
 		}
 		__syncthreads();
-		if(tid == 0) s_n += s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+		if(tid == 0) *s_n += s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
 		__syncthreads();
 	}
-	const int n = s_n;
+	const int n = *s_n;
+	if(tid == 0) a.tile_count[tile] = min(n, MAXT);
+	return n;
+}
+
+// row of the patch site of cell j covering box site (x, y), or -1
+__device__ inline int covering_row(int x, int y, int L, int cx, int cy, int LXj, int LYj, int ox, int oy, int st) {
+	int lx = x - cx;
+	if(lx < 0) lx += L;
+	int ly = y - cy;
+	if(ly < 0) ly += L;
+	if(lx >= LXj || ly >= LYj) return -1;
+	int qx = lx - ox;
+	if(qx < 0) qx += LXj;
+	int qy = ly - oy;
+	if(qy < 0) qy += LYj;
+	return st + qy * LXj + qx;
+}
+
+constexpr int GCHUNK = 8;   // covering patches examined per batch of independent loads
+
+// ---------------------------------------------------------------------------
+// k_box_gather: phi2[k] = sum over covering patches (cell order) of phi^2,
+// cover[k] = number of covering patches (the reference's overlap list size)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
+	if(a.st->flags) return;
+	__shared__ int s_cx[MAXT], s_cy[MAXT], s_lx[MAXT], s_ly[MAXT], s_ox[MAXT], s_oy[MAXT], s_start[MAXT];
+	__shared__ int s_wcnt[4];
+	__shared__ int s_n;
+	const int tile = blockIdx.x, tid = threadIdx.x;
+	const DevBox B = a.boxes[a.tile_box[tile]];
+	const int t = tile - B.tile_off, L = B.L;
+	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
+	const int tw = min(TW, L - x0), th = min(TH, L - y0);
+	const int n = tile_list(a, tile, B, x0, y0, tw, th, a.build_lists, s_cx, s_cy, s_lx, s_ly, s_ox, s_oy, s_start,
+			s_wcnt, &s_n);
 	if(n > MAXT) {
 		if(tid == 0) atomicOr(&a.st->flags, ST_TILE);
 		return;
-	}
-	if(a.store_lists) {
-		for(int j = tid; j < n; j += 256) a.tile_cells[tile * MAXT + j] = s_cell[j];
-		if(tid == 0) a.tile_count[tile] = n;
 	}
 	const int sx = tid % TW, sy = tid / TW;
 	if(sx >= tw || sy >= th) return;
 	const int x = x0 + sx, y = y0 + sy;
 	double acc = 0.0;
 	int cnt = 0;
-	for(int j = 0; j < n; ++j) {
-		int lx = x - s_cx[j];
-		if(lx < 0) lx += L;
-		int ly = y - s_cy[j];
-		if(ly < 0) ly += L;
-		const int LXj = s_lx[j], LYj = s_ly[j];
-		if(lx < LXj && ly < LYj) {
-			int qx = lx - s_ox[j];
-			if(qx < 0) qx += LXj;
-			int qy = ly - s_oy[j];
-			if(qy < 0) qy += LYj;
-			const double v = a.phi[s_start[j] + qy * LXj + qx];
-			acc += v * v;
-			++cnt;
+	for(int base = 0; base < n; base += GCHUNK) {
+		int row[GCHUNK];
+		double v[GCHUNK];
+#pragma unroll
+		for(int k = 0; k < GCHUNK; ++k) {
+			const int j = base + k;
+			row[k] = j < n ? covering_row(x, y, L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]) : -1;
 		}
+#pragma unroll
+		for(int k = 0; k < GCHUNK; ++k) v[k] = row[k] >= 0 ? a.phi[row[k]] : 0.;
+#pragma unroll
+		for(int k = 0; k < GCHUNK; ++k)
+			if(row[k] >= 0) {
+				acc += v[k] * v[k];
+				++cnt;
+			}
 	}
 	const int k = B.box_off + y * L + x;
 	a.phi2[k] = acc;
@@ -213,7 +256,9 @@ __device__ inline SiteIdx site_index(int q, int LX, int LY, int offx, int offy, 
 	return r;
 }
 
-template <int MODE>
+// NPT = sites per thread (compile time) so that the per-site registers and the
+// early loads of the corrector rows are fully unrolled.
+template <int MODE, int NPT>
 __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	if(a.st->flags) return;
 	extern __shared__ double smem[];
@@ -226,11 +271,28 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	const DevParams P = a.params[2 * C.box + a.cf_set];
 	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L, WX = LX + 2, WY = LY + 2;
 	double *s_phi = smem;
-	double *s_win = smem + S;
-	double *s_mu = s_win + WX * WY;
-	double *s_rep = s_mu + S;
+	double *s_win = smem + S;      // box window of sum phi^2, then mu (aliased)
+	double *s_mu = s_win;
+	double *s_rep = s_win + WX * WY;
 	const bool do_rep = a.jq && MODE != MODE_FS && !a.pure;
+	const bool fs = MODE != MODE_CF;
 
+	// early loads of the rows the field update reads
+	double pold[NPT], dold[NPT];
+	double2 vv[NPT];
+#pragma unroll
+	for(int j = 0; j < NPT; ++j) {
+		const int q = tid + j * CELL_THREADS;
+		pold[j] = dold[j] = 0.;
+		vv[j] = make_double2(0., 0.);
+		if(q < S) {
+			if(fs && !a.store) {
+				pold[j] = a.phi_old[C.start + q];
+				dold[j] = a.dphi_old[C.start + q];
+			}
+			if(MODE == MODE_FS) vv[j] = a.vrows[C.start + q];
+		}
+	}
 	// stage the patch and the (LX+2)x(LY+2) box window of sum phi^2
 	const double *phic = a.phi_c + C.start;
 	for(int q = tid; q < S; q += CELL_THREADS) s_phi[q] = phic[q];
@@ -247,51 +309,82 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	}
 	__syncthreads();
 
-	// phase 1: perimeter = sum |grad phi|^2 (computeGlobalSums, WetPolarModel.cpp:243-246)
-	{
+	// phase 1: perimeter = sum |grad phi|^2 (computeGlobalSums, WetPolarModel.cpp:243-246),
+	// needed before mu only by the perimeter term (beta != 0)
+	double perimeter = 0.;
+	if(P.beta != 0) {
 		double per[1] = { 0.0 };
-		for(int q = tid; q < S; q += CELL_THREADS) {
-			const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
-			const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
-			const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
-			per[0] += dx * dx + dy * dy;
+#pragma unroll
+		for(int j = 0; j < NPT; ++j) {
+			const int q = tid + j * CELL_THREADS;
+			if(q < S) {
+				const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
+				const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
+				const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
+				per[0] += dx * dx + dy * dy;
+			}
 		}
 		block_sum<1>(per, s_red);
 		if(tid == 0) s_bc = per[0];
 		__syncthreads();
+		perimeter = s_bc;
 	}
-	const double perimeter = s_bc;
 
-	// phase 2: chemical potential mu = dF/dphi (f_interaction, WetPolarModel.cpp:262-301)
-	for(int q = tid; q < S; q += CELL_THREADS) {
-		const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
-		const double ph = s_phi[q];
-		const double lap = s_phi[I.iN] + s_phi[I.iS] + s_phi[I.iE] + s_phi[I.iW] - 4 * ph;
-		const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
-		const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
-		const double p2 = s_win[I.w];
-		const double laplacianSquare = s_win[I.w + 1] + s_win[I.w + WX] + s_win[I.w - 1] + s_win[I.w - WX] - 4. * p2;
-		const double CH = P.gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / P.lambda - 2 * P.lambda * lap);
-		const double A = -4 * (P.mu / P.a0) * (1 - C.area / P.a0) * ph;
-		const double Rep = 4 * (P.kappa / P.lambda) * ph * (p2 - ph * ph);
-		const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
-		const double dd = laplacianSquare - lsquare;
-		const double suppress = dd / sqrt(1 + dd * dd);
-		const double Adh = -4 * P.lambda * P.omega * suppress * ph;
-		const double Pp = 4 * (P.beta / P.p0) * (1 - perimeter / P.p0) * lap;
-		const double V = CH + A + Rep + Adh + Pp;
-		s_mu[q] = V;
-		if(MODE == MODE_CF && a.out_mu) {
-			a.out_mu[C.start + q] = V;
-			a.out_dx[C.start + q] = dx;
-			a.out_dy[C.start + q] = dy;
-			a.out_lap[C.start + q] = lap;
+	// phase 2: chemical potential mu = dF/dphi (f_interaction, WetPolarModel.cpp:262-301).
+	// With omega == 0 (beta == 0) the adhesion (perimeter) term is exactly +-0 and is not evaluated.
+	double mu[NPT];
+	double per2[1] = { 0.0 };
+#pragma unroll
+	for(int j = 0; j < NPT; ++j) {
+		const int q = tid + j * CELL_THREADS;
+		mu[j] = 0.;
+		if(q < S) {
+			const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
+			const double ph = s_phi[q];
+			const double lap = s_phi[I.iN] + s_phi[I.iS] + s_phi[I.iE] + s_phi[I.iW] - 4 * ph;
+			const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
+			const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
+			if(MODE == MODE_CF && P.beta == 0) per2[0] += dx * dx + dy * dy;
+			const double p2 = s_win[I.w];
+			const double CH = P.gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / P.lambda - 2 * P.lambda * lap);
+			const double A = -4 * (P.mu / P.a0) * (1 - C.area / P.a0) * ph;
+			const double Rep = 4 * (P.kappa / P.lambda) * ph * (p2 - ph * ph);
+			double Adh = 0.;
+			if(P.omega != 0) {
+				const double laplacianSquare = s_win[I.w + 1] + s_win[I.w + WX] + s_win[I.w - 1] + s_win[I.w - WX] - 4. * p2;
+				const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
+				const double dd = laplacianSquare - lsquare;
+				const double suppress = dd / sqrt(1 + dd * dd);
+				Adh = -4 * P.lambda * P.omega * suppress * ph;
+			}
+			double Pp = 0.;
+			if(P.beta != 0) Pp = 4 * (P.beta / P.p0) * (1 - perimeter / P.p0) * lap;
+			const double V = CH + A + Rep + Adh + Pp;
+			mu[j] = V;
+			if(MODE == MODE_CF && a.out_mu) {
+				a.out_mu[C.start + q] = V;
+				a.out_dx[C.start + q] = dx;
+				a.out_dy[C.start + q] = dy;
+				a.out_lap[C.start + q] = lap;
+			}
+			if(do_rep) {
+				const double rn = a.rep[C.start + q] + Rep;
+				a.rep[C.start + q] = rn;
+				s_rep[q] = rn;
+			}
 		}
-		if(do_rep) {
-			const double rn = a.rep[C.start + q] + Rep;
-			a.rep[C.start + q] = rn;
-			s_rep[q] = rn;
-		}
+	}
+	if(MODE == MODE_CF && P.beta == 0) {
+		block_sum<1>(per2, s_red);
+		if(tid == 0) s_bc = per2[0];
+		__syncthreads();
+		perimeter = s_bc;
+	}
+	__syncthreads();  // every read of the sum phi^2 window is done: reuse it for mu
+#pragma unroll
+	for(int j = 0; j < NPT; ++j) {
+		const int q = tid + j * CELL_THREADS;
+		if(q < S) s_mu[q] = mu[j];
 	}
 	__syncthreads();
 
@@ -304,7 +397,10 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	const double *cosx = a.trig + C.trig_x, *sinx = cosx + LX;
 	const double *cosy = a.trig + C.trig_y, *siny = cosy + LY;
 	const bool dry_cf = P.friction_cell == 0;
-	for(int q = tid; q < S; q += CELL_THREADS) {
+#pragma unroll
+	for(int j = 0; j < NPT; ++j) {
+		const int q = tid + j * CELL_THREADS;
+		if(q >= S) continue;
 		const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
 		const int row = C.start + q;
 		const double ph = s_phi[q];
@@ -312,9 +408,8 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 		const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
 		double vx, vy;
 		if(MODE == MODE_FS) {
-			const double2 v = a.vrows[row];
-			vx = v.x;
-			vy = v.y;
+			vx = vv[j].x;
+			vy = vv[j].y;
 		}
 		else {
 			const double f_passive_x = (-1) * 0.5 * (s_mu[I.iE] - s_mu[I.iW]);
@@ -339,15 +434,14 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 			if(a.write_v) a.vrows[row] = make_double2(vx, vy);
 		}
 		// FD_CPUBackend::first_step body (FD_CPUBackend.cpp:66-101)
-		const double mu = s_mu[q];
-		const double dphi = -Q.J0 * mu - vx * dx - vy * dy;
+		const double dphi = -Q.J0 * mu[j] - vx * dx - vy * dy;
 		double phn;
 		if(a.store) {
 			a.dphi_old[row] = dphi;
 			phn = ph + Q.dt * .5 * (dphi + dphi);
 		}
 		else {
-			phn = a.phi_old[row] + Q.dt * .5 * (dphi + a.dphi_old[row]);
+			phn = pold[j] + Q.dt * .5 * (dphi + dold[j]);
 		}
 		a.phi_out[row] = phn;
 		red[0] += phn * phn;
@@ -388,25 +482,57 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 			s_redi[wave] = mx;
 			s_redi[4 + wave] = my;
 		}
+		if(tid == 0) {
+#pragma unroll
+			for(int j = 0; j < NRED; ++j) s_red[j] = red[j];
+		}
 		__syncthreads();
 	}
-	if(tid != 0) return;
+	if(tid >= 4) return;
 
-	// ---- per-cell tail, one lane ----
+	// ---- per-cell tail: lanes 0/1 take the two CoM atan2, lanes 2/3 the
+	// polarity angle and its cos / sin; lane 0 writes the cell ----
+	const double sumF = s_red[1];
+	const double trx = MODE == MODE_FUSED ? s_red[8] : C.trx;
+	const double try_ = MODE == MODE_FUSED ? s_red[9] : C.try_;
+	double val = 0.;
+	double th_old = C.thetaQ_old, th = 0.;
+	if(tid == 0) val = LX * (atan2(-s_red[3] / sumF, -s_red[2] / sumF) + PI_F) / (2 * PI_F);
+	else if(tid == 1) val = LY * (atan2(-s_red[5] / sumF, -s_red[4] / sumF) + PI_F) / (2 * PI_F);
+	else {
+		// WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:365-396)
+		if(a.store) {
+			const double g = a.gauss ? a.gauss[ci] : 0.0;
+			th_old = C.thetaQ + Q.sqrt_dt * Q.zetaQ_inter * g;
+		}
+		th = th_old - Q.dt * Q.J_Q * (atan2(trx * C.Q01 - try_ * C.Q00, trx * C.Q00 + try_ * C.Q01));
+		double arg = th;
+		if(Q.anchoring) {  // update_anchoring (WetPolarModel.cpp:417-430)
+			const double walls_length = 8;
+			const double com_old_y = a.store ? C.CoMy : C.CoM_oldy;
+			const double dist1 = (com_old_y - walls_length);
+			const double dist2 = ((double) L - walls_length) - com_old_y;
+			if(dist1 < dist2) arg = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
+			else arg = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) L - 2 * walls_length) / 2));
+		}
+		val = tid == 2 ? cos(arg) : sin(arg);
+	}
+	double CoMx = __shfl(val, 0, 64), CoMy = __shfl(val, 1, 64);
+	const double c2 = __shfl(val, 2, 64), s3 = __shfl(val, 3, 64);
+	th = __shfl(th, 2, 64);
+	th_old = __shfl(th_old, 2, 64);
+	if(tid != 0) return;
 	DevCell &G = a.cells[ci];
-	const double trx = MODE == MODE_FUSED ? red[8] : C.trx;
-	const double try_ = MODE == MODE_FUSED ? red[9] : C.try_;
 	if(a.store) {
 		G.CoM_oldx = C.CoMx;
 		G.CoM_oldy = C.CoMy;
 		G.area_old = C.area;
 		G.sumF_old = C.sumF;
 	}
-	const double area = red[0], sumF = red[1];
-	G.area = area;
+	G.area = s_red[0];
 	G.sumF = sumF;
-	G.S00 = red[6];
-	G.S01 = red[7];
+	G.S00 = s_red[6];
+	G.S01 = s_red[7];
 	G.perimeter = 0.;
 	int xsl = LX, ysb = LY;
 	if(!a.store) {
@@ -415,8 +541,6 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	}
 	G.x_sub_left = xsl;
 	G.y_sub_bottom = ysb;
-	double CoMx = LX * (atan2(-red[3] / sumF, -red[2] / sumF) + PI_F) / (2 * PI_F);
-	double CoMy = LY * (atan2(-red[5] / sumF, -red[4] / sumF) + PI_F) / (2 * PI_F);
 	if(!a.store) {
 		// MultiPhaseField::set_positions (MultiPhaseField.cpp:223-247)
 		if(xsl >= BORDER && ysb >= BORDER) {
@@ -428,12 +552,15 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 			const double cn0 = box_x(L, corner, 0), cn1 = box_y(L, corner, 0);
 			const double d0 = cn0 - co0 - rint((cn0 - co0) / L) * L;
 			const double d1 = cn1 - co1 - rint((cn1 - co1) / L) * L;
+			const int offx = (int) (C.offx + LX - d0) % LX;
+			const int offy = (int) (C.offy + LY - d1) % LY;
 			G.corner_old = C.corner;
 			G.corner = corner;
 			G.unrapx = C.unrapx + d0;
 			G.unrapy = C.unrapy + d1;
-			G.offx = (int) (C.offx + LX - d0) % LX;
-			G.offy = (int) (C.offy + LY - d1) % LY;
+			G.offx = offx;
+			G.offy = offy;
+			a.geo[ci] = make_int4(corner, LX | (LY << 16), offx | (offy << 16), C.start);
 		}
 		else {
 			// patch growth (MultiPhaseField.cpp:248-310) is done by the host:
@@ -446,49 +573,57 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	}
 	G.CoMx = CoMx;
 	G.CoMy = CoMy;
-	// WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:365-396)
-	double th_old = C.thetaQ_old;
-	if(a.store) {
-		const double g = a.gauss ? a.gauss[ci] : 0.0;
-		th_old = C.thetaQ + Q.sqrt_dt * Q.zetaQ_inter * g;
-	}
-	const double th = th_old - Q.dt * Q.J_Q * (atan2(trx * C.Q01 - try_ * C.Q00, trx * C.Q00 + try_ * C.Q01));
 	G.thetaQ_old = th_old;
 	G.thetaQ = th;
-	double Q00 = cos(th), Q01 = sin(th);
-	if(Q.anchoring) {  // update_anchoring (WetPolarModel.cpp:417-430)
-		const double walls_length = 8;
-		const double com_old_y = a.store ? C.CoMy : C.CoM_oldy;
-		const double dist1 = (com_old_y - walls_length);
-		const double dist2 = ((double) L - walls_length) - com_old_y;
-		double theta;
-		if(dist1 < dist2) theta = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
-		else theta = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) L - 2 * walls_length) / 2));
-		const double n0 = cos(theta), n1 = sin(theta);
-		Q00 = 0.5 * (n0 * n0 - n1 * n1);
-		Q01 = n0 * n1;
+	if(Q.anchoring) {
+		G.Q00 = 0.5 * (c2 * c2 - s3 * s3);
+		G.Q01 = c2 * s3;
 	}
-	G.Q00 = Q00;
-	G.Q01 = Q01;
+	else {
+		G.Q00 = c2;
+		G.Q01 = s3;
+	}
 	if(MODE == MODE_FUSED) {
 		G.trx = trx;
 		G.try_ = try_;
 	}
 }
 
+template <int MODE>
+static hipError_t set_lds(size_t bytes) {
+	hipError_t e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
+	return hipFuncSetAttribute((const void *) k_stage<MODE, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes);
+}
+
 hipError_t stage_set_lds_limit(size_t bytes) {
 	hipError_t e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_FUSED>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_CF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
-	return hipFuncSetAttribute((const void *) k_stage<MODE_FS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes);
+	if((e = set_lds<MODE_FUSED>(bytes)) != hipSuccess) return e;
+	if((e = set_lds<MODE_CF>(bytes)) != hipSuccess) return e;
+	return set_lds<MODE_FS>(bytes);
+}
+
+int stage_npt(int max_sites) {
+	const int n = (max_sites + CELL_THREADS - 1) / CELL_THREADS;
+	return n <= 4 ? 4 : n <= 7 ? 7 : n <= 10 ? 10 : n <= 16 ? 16 : -1;
+}
+
+template <int MODE>
+static void launch_mode(const StageArgs &a, int ncells, size_t lds, hipStream_t s, int npt) {
+	switch(npt) {
+	case 4: RC_LAUNCH((k_stage<MODE, 4>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	case 7: RC_LAUNCH((k_stage<MODE, 7>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	case 10: RC_LAUNCH((k_stage<MODE, 10>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	default: RC_LAUNCH((k_stage<MODE, 16>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	}
 }
 
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
-	switch(mode) {
-	case MODE_FUSED: RC_LAUNCH(k_stage<MODE_FUSED>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	case MODE_CF: RC_LAUNCH(k_stage<MODE_CF>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	default: RC_LAUNCH(k_stage<MODE_FS>, dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	}
+	if(mode == MODE_FUSED) launch_mode<MODE_FUSED>(a, ncells, lds, s, a.npt);
+	else if(mode == MODE_CF) launch_mode<MODE_CF>(a, ncells, lds, s, a.npt);
+	else launch_mode<MODE_FS>(a, ncells, lds, s, a.npt);
 	return hipGetLastError();
 }
 
@@ -567,6 +702,18 @@ __device__ void cg_after_resid(CGState *cg, const double *tot, double tol, int m
 	}
 	cg->active = !(cg->done[0] && cg->done[1]);
 }
+// alpha0 = (d.r0)/(d.A d) (A-norm optimal step along the learned correction),
+// kept only when it lowers |r|^2 = |(1-a) r0 + a rg|^2 below |r0|^2; needs
+// |r0|^2 and |rg|^2 in rn2 / thr (set by the two residual passes)
+__device__ void cg_after_line(CGState *cg, const double *tot, const double *rr0, const double *rrg) {
+	for(int c = 0; c < 2; ++c) {
+		const double dr = tot[c], dad = tot[2 + c], r0g = tot[4 + c];
+		double al = dad > 0 ? dr / dad : 0.;
+		const double rr = (1 - al) * (1 - al) * rr0[c] + 2 * al * (1 - al) * r0g + al * al * rrg[c];
+		if(!(rr < rr0[c])) al = 0.;
+		cg->alpha0[c] = al;
+	}
+}
 __device__ void cg_after_pap(CGState *cg, const double *tot) {
 	for(int c = 0; c < 2; ++c)
 		if(!cg->done[c]) cg->alpha[c] = cg->absNew[c] / tot[c];
@@ -593,8 +740,8 @@ template <int WHAT>
 __global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
 	if(a.st->flags) return;
 	CGState *cg = a.cg;
-	if(WHAT != 0 && !cg->active) return;
-	constexpr int NV = WHAT == 0 ? 6 : WHAT == 1 ? 2 : 4;
+	if((WHAT == 1 || WHAT == 2) && !cg->active) return;
+	constexpr int NV = WHAT == 0 || WHAT == 3 ? 6 : WHAT == 1 ? 2 : 4;
 	const double *part = WHAT == 1 ? a.part_box : a.part;
 	__shared__ double s_red[16 * NV];
 	double acc[NV];
@@ -622,6 +769,7 @@ __global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
 	}
 	if(WHAT == 0) cg_after_resid(cg, tot, a.tol, a.max_iter);
 	else if(WHAT == 1) cg_after_pap(cg, tot);
+	else if(WHAT == 3) cg_after_line(cg, tot, a.rr0, a.rrg);
 	else cg_after_update(cg, tot);
 }
 
@@ -629,6 +777,7 @@ hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s) 
 	switch(what) {
 	case 0: RC_LAUNCH(k_cg_reduce<0>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	case 1: RC_LAUNCH(k_cg_reduce<1>, dim3(1), dim3(1024), 0, s, a, nblk); break;
+	case 3: RC_LAUNCH(k_cg_reduce<3>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	default: RC_LAUNCH(k_cg_reduce<2>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	}
 	return hipGetLastError();
@@ -646,11 +795,8 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
 	const int tw = min(TW, L - x0), th = min(TH, L - y0);
 	const int n = a.tile_count[tile];
-	for(int j = tid; j < n; j += 256) {
-		const DevCell &C = a.cells[a.tile_cells[tile * MAXT + j]];
-		s_cx[j] = C.corner % L; s_cy[j] = C.corner / L; s_lx[j] = C.LX; s_ly[j] = C.LY;
-		s_ox[j] = C.offx; s_oy[j] = C.offy; s_start[j] = C.start;
-	}
+	for(int j = tid; j < n; j += 256)
+		unpack_geo(a.geo[a.tile_cells[tile * MAXT + j]], L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
 	__syncthreads();
 	const int sx = tid % TW, sy = tid / TW;
 	if(sx >= tw || sy >= th) return;
@@ -664,31 +810,38 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 		b0 = cg->beta[0]; b1 = cg->beta[1];
 	}
 	double2 acc = make_double2(0., 0.), acc2 = make_double2(0., 0.);
-	for(int j = 0; j < n; ++j) {
-		int lx = x - s_cx[j];
-		if(lx < 0) lx += L;
-		int ly = y - s_cy[j];
-		if(ly < 0) ly += L;
-		const int LXj = s_lx[j], LYj = s_ly[j];
-		if(lx < LXj && ly < LYj) {
-			int qx = lx - s_ox[j];
-			if(qx < 0) qx += LXj;
-			int qy = ly - s_oy[j];
-			if(qy < 0) qy += LYj;
-			const int row = s_start[j] + qy * LXj + qx;
-			double2 v;
-			if(MODE == CG_GATHER_X) v = a.x[row];
-			else {
-				const double2 rr = a.r[row];
-				if(MODE == CG_GATHER_P_FIRST) {
-					v = make_double2(inv * rr.x, inv * rr.y);
-				}
+	for(int base = 0; base < n; base += GCHUNK) {
+		int row[GCHUNK];
+		double2 rv[GCHUNK], pv[GCHUNK];
+#pragma unroll
+		for(int q = 0; q < GCHUNK; ++q) {
+			const int j = base + q;
+			row[q] = j < n ? covering_row(x, y, L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]) : -1;
+		}
+#pragma unroll
+		for(int q = 0; q < GCHUNK; ++q) {
+			rv[q] = make_double2(0., 0.);
+			pv[q] = make_double2(0., 0.);
+			if(row[q] >= 0) {
+				if(MODE == CG_GATHER_X) rv[q] = a.x[row[q]];
 				else {
-					const double2 po = a.p[row];
-					v.x = d0 ? po.x : inv * rr.x + b0 * po.x;
-					v.y = d1 ? po.y : inv * rr.y + b1 * po.y;
+					rv[q] = a.r[row[q]];
+					if(MODE == CG_GATHER_P_UPDATE) pv[q] = a.p[row[q]];
 				}
-				a.p[row] = v;
+			}
+		}
+#pragma unroll
+		for(int q = 0; q < GCHUNK; ++q) {
+			if(row[q] < 0) continue;
+			double2 v;
+			if(MODE == CG_GATHER_X) v = rv[q];
+			else {
+				if(MODE == CG_GATHER_P_FIRST) v = make_double2(inv * rv[q].x, inv * rv[q].y);
+				else {
+					v.x = d0 ? pv[q].x : inv * rv[q].x + b0 * pv[q].x;
+					v.y = d1 ? pv[q].y : inv * rv[q].y + b1 * pv[q].y;
+				}
+				a.p[row[q]] = v;
 				acc2.x += v.x * v.x;
 				acc2.y += v.y * v.y;
 			}
@@ -752,7 +905,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 	if(a.st->flags) return;
 	CGState *cg = a.cg;
-	if(MODE != CG_ROW_RESID && !cg->active) return;
+	if(MODE == CG_ROW_UPDATE && !cg->active) return;
 	__shared__ double s_red[4 * 6];
 	const int ci = blockIdx.x, tid = threadIdx.x;
 	const DevCell C = a.cells[ci];
@@ -781,6 +934,34 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 		const double D = a.diag[k];
 		const double2 n4 = a.n4[k];
 		const int row = C.start + q;
+		if(MODE == CG_ROW_LINE1 || MODE == CG_ROW_LINE2) {
+			const double inv = 1. / D;
+			// line search along d = xg - x (previous solution): r0 = b - A x, rg = b - A xg, A d = r0 - rg
+			const double2 x0 = a.x[row], xg = a.xg[row], r0 = a.r[row], rg = a.rg[row];
+			const double2 d = make_double2(xg.x - x0.x, xg.y - x0.y);
+			if(MODE == CG_ROW_LINE1) {
+				red[0] += d.x * r0.x; red[1] += d.y * r0.y;
+				red[2] += d.x * (r0.x - rg.x); red[3] += d.y * (r0.y - rg.y);
+				red[4] += r0.x * rg.x; red[5] += r0.y * rg.y;
+				continue;
+			}
+			const double a0 = cg->alpha0[0], a1 = cg->alpha0[1];
+			const double2 xn = make_double2(x0.x + a0 * d.x, x0.y + a1 * d.y);
+			const double2 rn = make_double2((1. - a0) * r0.x + a0 * rg.x, (1. - a1) * r0.y + a1 * rg.y);
+			a.x[row] = xn;
+			a.r[row] = rn;
+			const double2 bb = a.b[row];
+			red[0] += bb.x * bb.x; red[1] += bb.y * bb.y;
+			red[2] += rn.x * rn.x; red[3] += rn.y * rn.y;
+			red[4] += rn.x * (inv * rn.x); red[5] += rn.y * (inv * rn.y);
+			continue;
+		}
+		if(MODE == CG_ROW_GUESS) {
+			// rows from the box sums: v_i = (f_i + eta N4(k_i)) / D(k_i)
+			const double2 f = a.b[row];
+			a.r[row] = make_double2((f.x + eta * n4.x) / D, (f.y + eta * n4.y) / D);
+			continue;
+		}
 		const double2 vin = MODE == CG_ROW_RESID ? a.x[row] : a.p[row];
 		double2 t;
 		t.x = D * vin.x - eta * n4.x;
@@ -804,7 +985,8 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 			red[2] += rr.x * (inv * rr.x); red[3] += rr.y * (inv * rr.y);
 		}
 	}
-	if(MODE == CG_ROW_RESID) {
+	if(MODE == CG_ROW_GUESS) return;
+	if(MODE == CG_ROW_RESID || MODE == CG_ROW_LINE1 || MODE == CG_ROW_LINE2) {
 		block_sum<6>(red, s_red);
 		block_partials<6>(red, a.part);
 	}
@@ -817,7 +999,58 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 
 hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s) {
 	if(mode == CG_ROW_RESID) RC_LAUNCH(k_cg_row<CG_ROW_RESID>, dim3(a.ncells), dim3(256), 0, s, a);
+	else if(mode == CG_ROW_GUESS) RC_LAUNCH(k_cg_row<CG_ROW_GUESS>, dim3(a.ncells), dim3(256), 0, s, a);
+	else if(mode == CG_ROW_LINE1) RC_LAUNCH(k_cg_row<CG_ROW_LINE1>, dim3(a.ncells), dim3(256), 0, s, a);
+	else if(mode == CG_ROW_LINE2) RC_LAUNCH(k_cg_row<CG_ROW_LINE2>, dim3(a.ncells), dim3(256), 0, s, a);
 	else RC_LAUNCH(k_cg_row<CG_ROW_UPDATE>, dim3(a.ncells), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+// network input per box site: the change of the force box sums since the
+// previous solve and sum phi^2; then remembers the force sums
+__global__ void k_box_features(CGArgs a, double2 *sfprev, int have_prev, const double *phi2, float *feat, float sf,
+		float sp2, int nsites) {
+	const int k = blockIdx.x * 256 + threadIdx.x;
+	if(k >= nsites) return;
+	const double2 v = a.sp[k];
+	const double2 o = have_prev ? sfprev[k] : v;
+	feat[3 * k + 0] = (float) ((v.x - o.x) * sf);
+	feat[3 * k + 1] = (float) ((v.y - o.y) * sf);
+	feat[3 * k + 2] = (float) (phi2[k] * sp2);
+	sfprev[k] = v;
+}
+hipError_t launch_box_features(const CGArgs &a, double2 *sfprev, int have_prev, const double *phi2, float *feat, float sf,
+		float sp2, int nsites, hipStream_t s) {
+	RC_LAUNCH(k_box_features, dim3((nsites + 255) / 256), dim3(256), 0, s, a, sfprev, have_prev, phi2, feat, sf, sp2, nsites);
+	return hipGetLastError();
+}
+
+// predicted velocity box sums: previous solution's sums + network change
+__global__ void k_box_add_net(CGArgs a, const float *out, float sv, int nsites) {
+	const int k = blockIdx.x * 256 + threadIdx.x;
+	if(k >= nsites) return;
+	const double2 v = a.sp[k];
+	a.sp[k] = make_double2(v.x + (double) out[2 * k] * sv, v.y + (double) out[2 * k + 1] * sv);
+}
+hipError_t launch_box_add_net(const CGArgs &a, const float *out, float sv, int nsites, hipStream_t s) {
+	RC_LAUNCH(k_box_add_net, dim3((nsites + 255) / 256), dim3(256), 0, s, a, out, sv, nsites);
+	return hipGetLastError();
+}
+
+// snapshot channels: chan 0 -> (sp.x, sp.y) into [k][0..1], chan 2 -> phi2 into [k][2],
+// chan 3 -> (sp.x, sp.y) into [k][3..4]
+__global__ void k_box_snapshot(CGArgs a, const double *phi2, float *snap, int chan, int nsites) {
+	const int k = blockIdx.x * 256 + threadIdx.x;
+	if(k >= nsites) return;
+	if(chan == 2) snap[5 * k + 2] = (float) phi2[k];
+	else {
+		const double2 v = a.sp[k];
+		snap[5 * k + chan] = (float) v.x;
+		snap[5 * k + chan + 1] = (float) v.y;
+	}
+}
+hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap, int chan, int nsites, hipStream_t s) {
+	RC_LAUNCH(k_box_snapshot, dim3((nsites + 255) / 256), dim3(256), 0, s, a, phi2, snap, chan, nsites);
 	return hipGetLastError();
 }
 
