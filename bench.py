@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "rheocell_amd", "weights", "warmstart_wet256.npz"),
+                    help="learned warm-start weights (wet workloads; skipped when absent)")
+    ap.add_argument("--cg-steps", type=int, default=20, help="steps of the learned-vs-previous CG comparison")
     return ap.parse_args()
 
 
@@ -95,6 +98,38 @@ def cpu_baseline(conf, keys, steps):
                   "%.1f s, CG iterations %d over %d solves" % (keys.get("_name", "workload"), steps, cpu, dt,
                                                                st.iters_total, st.solves),
     }
+
+
+def cg_compare(conf, keys, weights, warmup, steps):
+    """CG wall time per step with the previous-solution warm start and with the
+    learned warm start, from the same initial state (same random stream)."""
+    from rheocell_amd import _lib, learned
+    from rheocell_amd.sim import Simulation
+    w, b, scale = learned.load(weights)
+    out = {}
+    for mode in ("previous", "learned"):
+        sim = Simulation([(conf, keys)])
+        g = sim.gaussians(warmup + steps)
+        if mode == "learned":
+            learned.attach(sim, w, b, scale)
+        else:
+            sim.set_warm_start(_lib.RC_WARM_PREVIOUS)
+        sim.step(warmup, g[:warmup])
+        s0 = sim.cg_stats()
+        t0 = time.perf_counter()
+        sim.step(steps, g[warmup:])
+        dt = time.perf_counter() - t0
+        s1 = sim.cg_stats()
+        out[mode] = {"cg_ms_per_step": round((s1.total_solve_ms - s0.total_solve_ms) / steps, 4),
+                     "ms_per_step": round(1e3 * dt / steps, 4),
+                     "cg_iters_per_solve": round((s1.iters_total - s0.iters_total) / max(s1.solves - s0.solves, 1), 2)}
+        if mode == "learned":
+            out[mode]["net_step_taken"] = "%d/%d" % (s1.learned_used - s0.learned_used, s1.learned_tried - s0.learned_tried)
+        sim.close()
+    out["cg_time_ratio_previous_over_learned"] = round(out["previous"]["cg_ms_per_step"] /
+                                                       max(out["learned"]["cg_ms_per_step"], 1e-9), 3)
+    out["weights"] = os.path.relpath(weights, ROOT)
+    return out
 
 
 def main():
@@ -164,6 +199,12 @@ def main():
                     "bytes_model": "%d B/row x %d rows + %d B/site x %d box sites" % (per_row, rows, per_site, L * L)}
         log("[bench] kernels: " + json.dumps({k: (round(v["ms_total"], 3), v["launches"]) for k, v in kernels.items()}))
 
+    cgcmp = None
+    if wet and args.weights and os.path.exists(args.weights) and args.cg_steps > 0:
+        log("[bench] CG warm starts: previous solution vs learned, %d steps each" % args.cg_steps)
+        cgcmp = cg_compare(conf, keys, args.weights, args.warmup, args.cg_steps)
+        log("[bench] " + json.dumps(cgcmp))
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline: %d step(s) of the oracle on 1 core" % args.cpu_steps)
@@ -195,6 +236,8 @@ def main():
     if wet:
         out["cg_ms_per_step"] = round((st1.total_solve_ms - st0.total_solve_ms) / args.steps, 4)
         out["cg_iters_per_solve"] = round(iters / max(solves, 1), 2)
+    if cgcmp:
+        out["cg_warm_start"] = cgcmp
     if kernels:
         out["kernels_ms"] = {k: round(v["ms_total"], 4) for k, v in kernels.items()}
     print(json.dumps(out), flush=True)

@@ -161,11 +161,11 @@ void rc_destroy(rc_handle *h);
 /* ---- Learned CG warm start (no counterpart in the reference) ----------
  * A 6-layer 3x3 convolutional network (32 channels, circular padding, ReLU,
  * BF16 weights and activations, FP32 accumulation on MFMA) maps box fields --
- * the change since the previous solve of the force sums (fx, fy summed over
- * the patches covering a site) and sum phi^2 -- to the change of the box sums
- * of the velocity.  Rows follow from v_i = (f_i + eta N4(k_i)) / D(k_i); the
- * guess enters by an A-norm line search from the previous solution and is
- * dropped when it does not lower the initial residual.
+ * the residual force r0 = f - A v_prev of the previous solution (fx, fy summed
+ * over the patches covering a site) and sum phi^2 -- to the box sums of the
+ * velocity correction e = A^-1 r0.  Rows follow from e_i = (r0_i + eta N4(k_i))
+ * / D(k_i); the guess enters by an A-norm line search from the previous
+ * solution and is dropped when it does not lower the initial residual.
  * Tensors are host float arrays, pixel-major: x [N][H][W][3], y [N][H][W][2].
  * Weights: w [6][9][32][32] (layer, tap = 3*(dy+1)+(dx+1), out ch, in ch),
  * b [6][32]; entries of absent channels (inputs >= 3 of layer 0, outputs >= 2
@@ -188,8 +188,9 @@ int rc_attach_net(rc_handle *h, rc_net *n, const float scale[3]);
 /* Box fields of the last force evaluation, [L*L][5]: sum fx, sum fy,
  * sum phi^2, sum vx, sum vy (wet model, one box). */
 int rc_snapshot_box(rc_handle *h, float *out);
-/* Record the box fields of every following wet solve (training data for the
- * network), up to capacity solves; rc_get_recorded copies [count][L*L][5]. */
+/* Record every following wet solve (training data for the network), up to
+ * capacity solves; rc_get_recorded copies [count][L*L][5]: box sums of the
+ * initial residual r0 (x, y), sum phi^2, box sums of the correction (x, y). */
 int rc_record_solves(rc_handle *h, int capacity);
 int rc_get_recorded(rc_handle *h, float *out, int *count);
 

@@ -91,9 +91,9 @@ struct rc_handle {
 	// learned warm start
 	rc_net *net = nullptr;
 	float scale[3] = { 1.f, 1.f, 1.f };
-	double2 *d_xg = nullptr, *d_sfprev = nullptr;
+	double2 *d_xg = nullptr;
 	double *d_rr = nullptr;
-	bool have_sfprev = false, pending_learned_stat = false;
+	bool pending_learned_stat = false, rec_open = false;
 	float *d_feat = nullptr, *d_snap = nullptr;
 	float *d_rec = nullptr;         // recorded solves (training data): [cap][L*L][5]
 	int rec_cap = 0, rec_n = 0;
@@ -312,7 +312,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_count);
 	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
-	dfree(h->d_xg); dfree(h->d_feat); dfree(h->d_snap); dfree(h->d_sfprev); dfree(h->d_rr); dfree(h->d_rec);
+	dfree(h->d_xg); dfree(h->d_feat); dfree(h->d_snap); dfree(h->d_rr); dfree(h->d_rec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
 	for(auto e : h->pool) (void) hipEventDestroy(e);
@@ -549,36 +549,35 @@ static int resid_norm(rc_handle *h, const CGArgs &c, double *rn2) {
 	return RC_OK;
 }
 
-// The learned initial guess.  The network maps the change of the force box
-// sums since the previous solve (and sum phi^2) to the change of the velocity
-// box sums; rows follow from v_i = (f_i + eta N4(k_i)) / D(k_i) (exact for
-// exact box sums).  The guess enters through an A-norm line search from the
-// previous solution x along d = x_net - x, with A d = r0 - r_net; the step is
-// dropped (previous guess kept) when it does not lower the initial residual.
-// Leaves x, r and the CG set-up of the chosen start in place.
+// The learned initial guess.  The correction e = A^-1 r0 of the previous
+// solution x (r0 = b - A x) is predicted by the network from the box sums of
+// r0 and sum phi^2 as the box sums of e; rows follow from A e = r0, i.e.
+// e_i = (r0_i + eta N4(k_i)) / D(k_i).  The guess enters by an A-norm line
+// search from x along e (A e = r0 - r_g with r_g the residual of x + e) and
+// the step is dropped when it does not lower the initial residual.  Leaves x,
+// r and the CG set-up of the chosen start in place.
 static int learned_guess(rc_handle *h, CGArgs c) {
 	const int ns = (int) h->box_sites;
-	CGArgs g = c;
-	g.x = h->d_f;
-	HIPCHK(h, launch_cg_gather(CG_GATHER_X, g, h->ntiles, h->stream));
-	HIPCHK(h, launch_box_features(g, h->d_sfprev, h->have_sfprev ? 1 : 0, h->d_phi2, h->d_feat, h->scale[0], h->scale[1], ns,
-			h->stream));
-	h->have_sfprev = true;
-	if(net_forward_device(h->net, h->d_feat, h->stream))
-		return fail(h, RC_ERR_HIP, "network forward pass failed: %s", rc_net_last_error(h->net));
-	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));     // sum of the previous solution
-	HIPCHK(h, launch_box_add_net(c, net_output(h->net), h->scale[2], ns, h->stream));
-	HIPCHK(h, launch_cg_box(0, c, ns, h->stream));
-	CGArgs q = c;
-	q.r = h->d_xg;
-	HIPCHK(h, launch_cg_row(CG_ROW_GUESS, q, h->stream));
-	// r0 = b - A x (into r), rg = b - A x_net (into p), their norms
+	// r0 = b - A x into r, |r0|^2
 	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
 	HIPCHK(h, launch_cg_box(0, c, ns, h->stream));
 	HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream));
 	HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream));
 	HIPCHK(h, hipMemcpyAsync(h->d_rr, reinterpret_cast<char *>(h->d_cg) + offsetof(CGState, rn2), 2 * sizeof(double),
 			hipMemcpyDeviceToDevice, h->stream));
+	// network: box sums of r0 and phi^2 -> box sums of the correction
+	CGArgs g = c;
+	g.x = h->d_r;
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, g, h->ntiles, h->stream));
+	HIPCHK(h, launch_box_features(g, h->d_phi2, h->d_feat, h->scale[0], h->scale[1], ns, h->stream));
+	if(net_forward_device(h->net, h->d_feat, h->stream))
+		return fail(h, RC_ERR_HIP, "network forward pass failed: %s", rc_net_last_error(h->net));
+	HIPCHK(h, launch_box_from_net(c, net_output(h->net), h->scale[2], ns, h->stream));
+	HIPCHK(h, launch_cg_box(0, c, ns, h->stream));
+	CGArgs q = c;
+	q.xg = h->d_xg;
+	HIPCHK(h, launch_cg_row(CG_ROW_GUESS, q, h->stream));
+	// r_g = b - A x_g into p, |r_g|^2
 	CGArgs x = c;
 	x.x = h->d_xg;
 	x.r = h->d_p;
@@ -602,19 +601,33 @@ static int learned_guess(rc_handle *h, CGArgs c) {
 	return -1;
 }
 
-// appends (sum f, sum phi^2, sum v) of the solve just finished to the record
-static int record_solve(rc_handle *h) {
+// Training record of one solve, [L*L][5]: box sums of the initial residual
+// r0 = b - A x_prev, sum phi^2, box sums of the correction x - x_prev.
+// record_start runs after the initial residual pass, record_end after the solve.
+static int record_start(rc_handle *h, const CGArgs &c0) {
 	const int ns = (int) h->box_sites;
-	CGArgs c = cg_args(h);
+	CGArgs c = c0;
 	float *dst = h->d_rec + (size_t) h->rec_n * 5 * ns;
-	c.x = h->d_f;
+	HIPCHK(h, hipMemcpyAsync(h->d_xg, h->d_v, sizeof(double2) * h->rows, hipMemcpyDeviceToDevice, h->stream));
+	c.x = h->d_r;
 	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
 	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 0, ns, h->stream));
 	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 2, ns, h->stream));
+	h->rec_open = true;
+	return RC_OK;
+}
+static int record_end(rc_handle *h) {
+	const int ns = (int) h->box_sites;
+	CGArgs c = cg_args(h);
+	float *dst = h->d_rec + (size_t) h->rec_n * 5 * ns;
 	c.x = h->d_v;
 	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
 	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 3, ns, h->stream));
+	c.x = h->d_xg;
+	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 5, ns, h->stream));
 	h->rec_n += 1;
+	h->rec_open = false;
 	return RC_OK;
 }
 
@@ -640,6 +653,10 @@ static int cg_solve(rc_handle *h, bool cold) {
 			{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
 			{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }
 			{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream)); }
+			if(h->d_rec && h->rec_n < h->rec_cap) {
+				int rc = record_start(h, c);
+				if(rc) return rc;
+			}
 		}
 		{ TimeScope t(h, RC_T_CG_GATHER_P_FIRST); HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream)); }
 		int batch = std::max(2, h->last_iters - 1);
@@ -674,8 +691,8 @@ static int cg_solve(rc_handle *h, bool cold) {
 		if(s.alpha0[0] != 0 || s.alpha0[1] != 0) h->cgs.learned_used += 1;
 		h->pending_learned_stat = false;
 	}
-	if(h->d_rec && h->rec_n < h->rec_cap) {
-		int rc = record_solve(h);
+	if(h->rec_open) {
+		int rc = record_end(h);
 		if(rc) return rc;
 	}
 	h->cgs.solves += 2;
@@ -1151,10 +1168,8 @@ extern "C" int rc_attach_net(rc_handle *h, rc_net *n, const float scale[3]) {
 	if(!h->d_xg) {
 		HIPCHK(h, dalloc(&h->d_xg, std::max<long long>(h->rows, 1)));
 		HIPCHK(h, dalloc(&h->d_feat, 3 * h->box_sites));
-		HIPCHK(h, dalloc(&h->d_sfprev, h->box_sites));
 		HIPCHK(h, dalloc(&h->d_rr, 4));
 	}
-	h->have_sfprev = false;
 	h->net = n;
 	for(int k = 0; k < 3; ++k) h->scale[k] = scale ? scale[k] : 1.f;
 	return RC_OK;
@@ -1187,6 +1202,7 @@ extern "C" int rc_record_solves(rc_handle *h, int capacity) {
 	h->rec_cap = h->rec_n = 0;
 	if(capacity == 0) return RC_OK;
 	HIPCHK(h, dalloc(&h->d_rec, (size_t) capacity * 5 * h->box_sites));
+	if(!h->d_xg) HIPCHK(h, dalloc(&h->d_xg, std::max<long long>(h->rows, 1)));
 	h->rec_cap = capacity;
 	return RC_OK;
 }

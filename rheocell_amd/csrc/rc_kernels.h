@@ -85,11 +85,10 @@ hipError_t launch_cg_box(int pap, const CGArgs &a, int nsites, hipStream_t s);
 hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s);
 // what: 0 after the residual pass, 1 after p.Ap, 2 after the x,r update, 3 line search
 hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s);
-// learned guess helpers: features ((sum f - previous sum f) * sf, sum phi^2 * sp2) -> feat [L*L][3]
-// and sfprev <- sum f; sp += net output * sv
-hipError_t launch_box_features(const CGArgs &a, double2 *sfprev, int have_prev, const double *phi2, float *feat, float sf,
-		float sp2, int nsites, hipStream_t s);
-hipError_t launch_box_add_net(const CGArgs &a, const float *out, float sv, int nsites, hipStream_t s);
+// learned guess helpers: features (sum r0 * sf, sum phi^2 * sp2) -> feat [L*L][3]; sp = net output * sv
+hipError_t launch_box_features(const CGArgs &a, const double *phi2, float *feat, float sf, float sp2, int nsites,
+		hipStream_t s);
+hipError_t launch_box_from_net(const CGArgs &a, const float *out, float sv, int nsites, hipStream_t s);
 hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap, int chan, int nsites, hipStream_t s);
 hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
 hipError_t stage_set_lds_limit(size_t bytes);

@@ -957,9 +957,9 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 			continue;
 		}
 		if(MODE == CG_ROW_GUESS) {
-			// rows from the box sums: v_i = (f_i + eta N4(k_i)) / D(k_i)
-			const double2 f = a.b[row];
-			a.r[row] = make_double2((f.x + eta * n4.x) / D, (f.y + eta * n4.y) / D);
+			// correction rows from its box sums: A e = r0 gives e_i = (r0_i + eta N4(k_i)) / D(k_i)
+			const double2 r0 = a.r[row], x0 = a.x[row];
+			a.xg[row] = make_double2(x0.x + (r0.x + eta * n4.x) / D, x0.y + (r0.y + eta * n4.y) / D);
 			continue;
 		}
 		const double2 vin = MODE == CG_ROW_RESID ? a.x[row] : a.p[row];
@@ -1006,43 +1006,44 @@ hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s) {
 	return hipGetLastError();
 }
 
-// network input per box site: the change of the force box sums since the
-// previous solve and sum phi^2; then remembers the force sums
-__global__ void k_box_features(CGArgs a, double2 *sfprev, int have_prev, const double *phi2, float *feat, float sf,
-		float sp2, int nsites) {
+// network input per box site: the box sums of the residual force
+// r0 = b - A x_prev (a.sp) and sum phi^2
+__global__ void k_box_features(CGArgs a, const double *phi2, float *feat, float sf, float sp2, int nsites) {
 	const int k = blockIdx.x * 256 + threadIdx.x;
 	if(k >= nsites) return;
 	const double2 v = a.sp[k];
-	const double2 o = have_prev ? sfprev[k] : v;
-	feat[3 * k + 0] = (float) ((v.x - o.x) * sf);
-	feat[3 * k + 1] = (float) ((v.y - o.y) * sf);
+	feat[3 * k + 0] = (float) (v.x * sf);
+	feat[3 * k + 1] = (float) (v.y * sf);
 	feat[3 * k + 2] = (float) (phi2[k] * sp2);
-	sfprev[k] = v;
 }
-hipError_t launch_box_features(const CGArgs &a, double2 *sfprev, int have_prev, const double *phi2, float *feat, float sf,
-		float sp2, int nsites, hipStream_t s) {
-	RC_LAUNCH(k_box_features, dim3((nsites + 255) / 256), dim3(256), 0, s, a, sfprev, have_prev, phi2, feat, sf, sp2, nsites);
+hipError_t launch_box_features(const CGArgs &a, const double *phi2, float *feat, float sf, float sp2, int nsites,
+		hipStream_t s) {
+	RC_LAUNCH(k_box_features, dim3((nsites + 255) / 256), dim3(256), 0, s, a, phi2, feat, sf, sp2, nsites);
 	return hipGetLastError();
 }
 
-// predicted velocity box sums: previous solution's sums + network change
-__global__ void k_box_add_net(CGArgs a, const float *out, float sv, int nsites) {
+// network output -> box sums of the velocity correction (a.sp)
+__global__ void k_box_from_net(CGArgs a, const float *out, float sv, int nsites) {
 	const int k = blockIdx.x * 256 + threadIdx.x;
 	if(k >= nsites) return;
-	const double2 v = a.sp[k];
-	a.sp[k] = make_double2(v.x + (double) out[2 * k] * sv, v.y + (double) out[2 * k + 1] * sv);
+	a.sp[k] = make_double2((double) out[2 * k] * sv, (double) out[2 * k + 1] * sv);
 }
-hipError_t launch_box_add_net(const CGArgs &a, const float *out, float sv, int nsites, hipStream_t s) {
-	RC_LAUNCH(k_box_add_net, dim3((nsites + 255) / 256), dim3(256), 0, s, a, out, sv, nsites);
+hipError_t launch_box_from_net(const CGArgs &a, const float *out, float sv, int nsites, hipStream_t s) {
+	RC_LAUNCH(k_box_from_net, dim3((nsites + 255) / 256), dim3(256), 0, s, a, out, sv, nsites);
 	return hipGetLastError();
 }
 
 // snapshot channels: chan 0 -> (sp.x, sp.y) into [k][0..1], chan 2 -> phi2 into [k][2],
-// chan 3 -> (sp.x, sp.y) into [k][3..4]
+// chan 3 -> (sp.x, sp.y) into [k][3..4], chan 5 -> [k][3..4] -= (sp.x, sp.y)
 __global__ void k_box_snapshot(CGArgs a, const double *phi2, float *snap, int chan, int nsites) {
 	const int k = blockIdx.x * 256 + threadIdx.x;
 	if(k >= nsites) return;
 	if(chan == 2) snap[5 * k + 2] = (float) phi2[k];
+	else if(chan == 5) {
+		const double2 v = a.sp[k];
+		snap[5 * k + 3] -= (float) v.x;
+		snap[5 * k + 4] -= (float) v.y;
+	}
 	else {
 		const double2 v = a.sp[k];
 		snap[5 * k + chan] = (float) v.x;

@@ -107,7 +107,9 @@ def snapshot(sim: Simulation) -> np.ndarray:
 
 
 def record_run(L: int, nside: int, seed: int, steps: int, burn_in: int = 10) -> np.ndarray:
-    """Box fields of every wet solve of one run (2 per step), [solves, L, L, 5]."""
+    """Every wet solve of one run (2 per step), [solves, L, L, 5]: box sums of
+    the previous solution's residual (x, y), sum phi^2, box sums of the
+    correction from the previous solution to the new one (x, y)."""
     conf = confgen.lattice_conf(L, nside, seed)
     keys = dict(confgen.BASE_KEYS, friction_cell=1, CGtolerance=1e-8, seed=seed)
     sim = Simulation([(conf, keys)])
@@ -136,18 +138,14 @@ def generate_snapshots(L: int = 256, nside: int = 20, seeds: Sequence[int] = (11
 
 
 def pairs(runs: List[np.ndarray]) -> Tuple[np.ndarray, np.ndarray]:
-    """Consecutive solves: (change of force sums, sum phi^2) -> change of velocity sums."""
-    dx, dy = [], []
-    for r in runs:
-        d = r[1:] - r[:-1]
-        x = np.concatenate([d[..., 0:2], r[1:, ..., 2:3]], axis=-1)
-        dx.append(x)
-        dy.append(d[..., 3:5])
-    return np.concatenate(dx), np.concatenate(dy)
+    """Training pairs: (residual sums, sum phi^2) -> correction sums."""
+    x = np.concatenate([r[..., 0:3] for r in runs])
+    y = np.concatenate([r[..., 3:5] for r in runs])
+    return x, y
 
 
 def scales_of(x: np.ndarray, y: np.ndarray) -> np.ndarray:
-    """[force-change multiplier, sum phi^2 multiplier, output multiplier] for unit RMS."""
+    """[residual multiplier, sum phi^2 multiplier, output multiplier] for unit RMS."""
     return np.array([1.0 / np.sqrt(np.mean(x[..., 0:2] ** 2)), 1.0 / np.sqrt(np.mean(x[..., 2] ** 2)),
                      np.sqrt(np.mean(y ** 2))], dtype=np.float32)
 

@@ -68,6 +68,8 @@ def test_learned_warm_start_end_to_end():
     L = 64
     runs = learned.generate_snapshots(L=L, nside=4, seeds=(3, 4), steps=40, burn_in=5)
     assert all(r.shape == (80, L, L, 5) for r in runs)
+    # each record: residual sums, sum phi^2, correction sums; sum phi^2 > 0 inside cells
+    assert all(np.isfinite(r).all() and r[..., 2].max() > 0.5 for r in runs)
     w, b, scale, hist = learned.train(runs, epochs=20, batch=8, lr=3e-3)
     print("training loss", hist[0], "->", hist[-1])
     assert hist[-1] < hist[0]
