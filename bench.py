@@ -106,12 +106,13 @@ def cg_compare(conf, keys, weights, warmup, steps):
     from rheocell_amd import _lib, learned
     from rheocell_amd.sim import Simulation
     w, b, scale = learned.load(weights)
+    stages = learned.trained_stages(weights)
     out = {}
     for mode in ("previous", "learned"):
         sim = Simulation([(conf, keys)])
         g = sim.gaussians(warmup + steps)
         if mode == "learned":
-            learned.attach(sim, w, b, scale)
+            learned.attach(sim, w, b, scale, stages=stages)
         else:
             sim.set_warm_start(_lib.RC_WARM_PREVIOUS)
         sim.step(warmup, g[:warmup])
@@ -129,6 +130,7 @@ def cg_compare(conf, keys, weights, warmup, steps):
     out["cg_time_ratio_previous_over_learned"] = round(out["previous"]["cg_ms_per_step"] /
                                                        max(out["learned"]["cg_ms_per_step"], 1e-9), 3)
     out["weights"] = os.path.relpath(weights, ROOT)
+    out["stages"] = stages
     return out
 
 

@@ -100,14 +100,14 @@ typedef struct rc_cg_stats_t {
 	double total_solve_ms;
 	long long sweeps_total;          /* launches of the iteration kernels that did work:
 	                                    sum over solves of max(iters_x, iters_y) */
-	long long learned_used;          /* solves that started from the network's guess */
-	long long learned_tried;         /* solves where the network's guess was evaluated */
+	long long learned_used;          /* solves whose last network stage moved the start */
+	long long learned_tried;         /* solves that ran the network stages */
 } rc_cg_stats_t;
 
 #define RC_WARM_REFERENCE 0   /* previous solution, cold restart every 11th solve (reference rule) */
 #define RC_WARM_PREVIOUS 1    /* previous solution, never a periodic cold restart */
-#define RC_WARM_LEARNED 2     /* guess of the attached network; the previous solution when
-                                 its initial residual is smaller (needs rc_attach_net) */
+#define RC_WARM_LEARNED 2     /* previous solution refined by the attached network
+                                 (needs rc_attach_net; see rc_set_learned_stages) */
 
 typedef struct rc_handle rc_handle;
 
@@ -152,7 +152,8 @@ int rc_set_warm_start(rc_handle *h, int mode);
 #define RC_T_STEP 14               /* a whole rc_step call */
 #define RC_T_CG_GATHER_P_FIRST 15  /* p = z fused with the box gather of p */
 #define RC_T_CG_REDUCE 16          /* one-workgroup reduction of CG partial sums */
-#define RC_NKINDS 17
+#define RC_T_CG_LEARNED 17         /* the learned refinement stages of one solve (group) */
+#define RC_NKINDS 18
 int rc_timing(rc_handle *h, int kind, double *ms, long long *launches);
 int rc_enable_timing(rc_handle *h, int on);
 const char *rc_last_error(rc_handle *h);
@@ -161,11 +162,13 @@ void rc_destroy(rc_handle *h);
 /* ---- Learned CG warm start (no counterpart in the reference) ----------
  * A 6-layer 3x3 convolutional network (32 channels, circular padding, ReLU,
  * BF16 weights and activations, FP32 accumulation on MFMA) maps box fields --
- * the residual force r0 = f - A v_prev of the previous solution (fx, fy summed
- * over the patches covering a site) and sum phi^2 -- to the box sums of the
- * velocity correction e = A^-1 r0.  Rows follow from e_i = (r0_i + eta N4(k_i))
- * / D(k_i); the guess enters by an A-norm line search from the previous
- * solution and is dropped when it does not lower the initial residual.
+ * the residual r = f - A x of a start x (summed over the patches covering a
+ * site, divided per component by sqrt(|r_c|^2 / rows)) and sum phi^2 -- to
+ * the box sums of the correction e = A^-1 r in the same units.  Rows follow
+ * from e_i = (r_i + eta N4(k_i)) / D(k_i); x moves along e by the A-norm
+ * optimal step (e.r)/(e.A e), which never raises the A-norm error.  A solve
+ * starts from the previous solution and applies the stage rc_set_learned_stages
+ * times; the reference's CG then runs to its tolerance from the result.
  * Tensors are host float arrays, pixel-major: x [N][H][W][3], y [N][H][W][2].
  * Weights: w [6][9][32][32] (layer, tap = 3*(dy+1)+(dx+1), out ch, in ch),
  * b [6][32]; entries of absent channels (inputs >= 3 of layer 0, outputs >= 2
@@ -182,15 +185,21 @@ int rc_net_train_step(rc_net *n, const float *x, const float *y, float lr, doubl
 const char *rc_net_last_error(rc_net *n);
 void rc_net_destroy(rc_net *n);
 /* Use net (created with N = 1, H = W = box side) for RC_WARM_LEARNED.
- * scale[0]: multiplier of the force sums, scale[1]: of sum phi^2,
- * scale[2]: multiplier of the network output (-> velocity box sums). */
+ * scale[0]: multiplier of the scaled residual sums, scale[1]: of sum phi^2,
+ * scale[2]: multiplier of the network output (-> scaled correction sums). */
 int rc_attach_net(rc_handle *h, rc_net *n, const float scale[3]);
+#define RC_MAX_STAGES 8
+/* network refinement stages per solve, 1..RC_MAX_STAGES (default 1) */
+int rc_set_learned_stages(rc_handle *h, int stages);
 /* Box fields of the last force evaluation, [L*L][5]: sum fx, sum fy,
  * sum phi^2, sum vx, sum vy (wet model, one box). */
 int rc_snapshot_box(rc_handle *h, float *out);
-/* Record every following wet solve (training data for the network), up to
- * capacity solves; rc_get_recorded copies [count][L*L][5]: box sums of the
- * initial residual r0 (x, y), sum phi^2, box sums of the correction (x, y). */
+/* Record training data for the network from every following wet solve, up to
+ * capacity records: one per start the solve passes through (the previous
+ * solution, then the result of each network stage but the last).
+ * rc_get_recorded copies [count][L*L][5]: box sums of the start's residual
+ * (x, y), sum phi^2, box sums of the correction to the converged solution
+ * (x, y), both divided per component by sqrt(|r_c|^2 / rows) of that start. */
 int rc_record_solves(rc_handle *h, int capacity);
 int rc_get_recorded(rc_handle *h, float *out, int *count);
 

@@ -20,7 +20,7 @@ RC_WARM_LEARNED = 2
 # timing kinds (rheocell.h RC_T_*)
 T_KINDS = ["box_gather", "stage_fused_pred", "stage_fused_corr", "stage_cf", "stage_fs_pred", "stage_fs_corr",
            "cg_diag", "cg_gather_x", "cg_gather_p", "cg_row_resid", "cg_box_pap", "cg_row_update", "cg_box_n4",
-           "cg_solve", "step", "cg_gather_p_first", "cg_reduce"]
+           "cg_solve", "step", "cg_gather_p_first", "cg_reduce", "cg_learned"]
 # kernel symbol of each timing kind (as rocprofv3 names it)
 T_SYMBOL = {"box_gather": "k_box_gather", "stage_fused_pred": "k_stage<0>", "stage_fused_corr": "k_stage<0>",
             "stage_cf": "k_stage<1>", "stage_fs_pred": "k_stage<2>", "stage_fs_corr": "k_stage<2>",
@@ -33,7 +33,7 @@ EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_a
            "rc_set_warm_start", "rc_timing", "rc_enable_timing", "rc_last_error", "rc_destroy",
            "rc_net_create", "rc_net_set_params", "rc_net_get_params", "rc_net_forward", "rc_net_grads",
            "rc_net_train_step", "rc_net_last_error", "rc_net_destroy", "rc_attach_net", "rc_snapshot_box",
-           "rc_record_solves", "rc_get_recorded"]
+           "rc_record_solves", "rc_get_recorded", "rc_set_learned_stages"]
 
 
 class rc_model_params(C.Structure):
@@ -115,6 +115,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.rc_snapshot_box.argtypes = [vp, fp]
     L.rc_record_solves.argtypes = [vp, ip]
     L.rc_get_recorded.argtypes = [vp, fp, C.POINTER(C.c_int)]
+    L.rc_set_learned_stages.argtypes = [vp, ip]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L

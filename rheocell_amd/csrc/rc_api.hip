@@ -91,12 +91,14 @@ struct rc_handle {
 	// learned warm start
 	rc_net *net = nullptr;
 	float scale[3] = { 1.f, 1.f, 1.f };
-	double2 *d_xg = nullptr;
-	double *d_rr = nullptr;
-	bool pending_learned_stat = false, rec_open = false;
-	float *d_feat = nullptr, *d_snap = nullptr;
-	float *d_rec = nullptr;         // recorded solves (training data): [cap][L*L][5]
+	int stages = 1;                 // network refinement stages per solve
+	bool pending_learned_stat = false;
+	float *d_snap = nullptr;
+	float *d_rec = nullptr;         // training records: [cap][L*L][5]
 	int rec_cap = 0, rec_n = 0;
+	int rec_open = 0;               // records of the running solve (one per stage)
+	double2 *d_xrec = nullptr;      // x at each recorded stage [RC_MAX_STAGES][rows]
+	double *d_rnrec = nullptr;      // |r_c|^2 at each recorded stage [RC_MAX_STAGES][2]
 
 	// timing
 	bool timing = false;
@@ -149,7 +151,7 @@ struct TimeScope {
 	int kind;
 	bool group;
 	hipEvent_t b;
-	TimeScope(rc_handle *h_, int k) : h(h_), kind(k), group(k == RC_T_CG_SOLVE || k == RC_T_STEP), b(nullptr) {
+	TimeScope(rc_handle *h_, int k) : h(h_), kind(k), group(k == RC_T_CG_SOLVE || k == RC_T_STEP || k == RC_T_CG_LEARNED), b(nullptr) {
 		if(h->timing) {
 			hipEvent_t a = ev_get(h);
 			b = ev_get(h);
@@ -296,7 +298,8 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, dalloc(&h->d_p2, sites));
 		HIPCHK(h, dalloc(&h->d_n4, sites));
 		HIPCHK(h, dalloc(&h->d_part_box, 2 * ((sites + 255) / 256)));
-		HIPCHK(h, dalloc(&h->d_part, (size_t) 6 * std::max(cells, 1)));
+		// partials: 6 per cell (row passes), 2 per tile (the learned stage's gather<E>)
+		HIPCHK(h, dalloc(&h->d_part, (size_t) std::max(6 * std::max(cells, 1), 2 * tiles)));
 		HIPCHK(h, dalloc(&h->d_cg, 1));
 		HIPCHK(h, hipMemset(h->d_cg, 0, sizeof(CGState)));
 	}
@@ -312,7 +315,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_count);
 	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
-	dfree(h->d_xg); dfree(h->d_feat); dfree(h->d_snap); dfree(h->d_rr); dfree(h->d_rec);
+	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
 	for(auto e : h->pool) (void) hipEventDestroy(e);
@@ -414,10 +417,10 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			HIPCHK(h, dalloc(&h->d_p, rows));
 		}
 		h->rows = rows;
-		if(h->d_xg) {
-			dfree(h->d_xg);
-			h->d_xg = nullptr;
-			HIPCHK(h, dalloc(&h->d_xg, rows));
+		if(h->d_xrec) {
+			dfree(h->d_xrec);
+			h->d_xrec = nullptr;
+			HIPCHK(h, dalloc(&h->d_xrec, (size_t) RC_MAX_STAGES * rows));
 		}
 	}
 	if(h->jq) {
@@ -549,85 +552,81 @@ static int resid_norm(rc_handle *h, const CGArgs &c, double *rn2) {
 	return RC_OK;
 }
 
-// The learned initial guess.  The correction e = A^-1 r0 of the previous
-// solution x (r0 = b - A x) is predicted by the network from the box sums of
-// r0 and sum phi^2 as the box sums of e; rows follow from A e = r0, i.e.
-// e_i = (r0_i + eta N4(k_i)) / D(k_i).  The guess enters by an A-norm line
-// search from x along e (A e = r0 - r_g with r_g the residual of x + e) and
-// the step is dropped when it does not lower the initial residual.  Leaves x,
-// r and the CG set-up of the chosen start in place.
-static int learned_guess(rc_handle *h, CGArgs c) {
-	const int ns = (int) h->box_sites;
-	// r0 = b - A x into r, |r0|^2
-	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
-	HIPCHK(h, launch_cg_box(0, c, ns, h->stream));
-	HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream));
-	HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream));
-	HIPCHK(h, hipMemcpyAsync(h->d_rr, reinterpret_cast<char *>(h->d_cg) + offsetof(CGState, rn2), 2 * sizeof(double),
-			hipMemcpyDeviceToDevice, h->stream));
-	// network: box sums of r0 and phi^2 -> box sums of the correction
-	CGArgs g = c;
-	g.x = h->d_r;
-	HIPCHK(h, launch_cg_gather(CG_GATHER_X, g, h->ntiles, h->stream));
-	HIPCHK(h, launch_box_features(g, h->d_phi2, h->d_feat, h->scale[0], h->scale[1], ns, h->stream));
-	if(net_forward_device(h->net, h->d_feat, h->stream))
-		return fail(h, RC_ERR_HIP, "network forward pass failed: %s", rc_net_last_error(h->net));
-	HIPCHK(h, launch_box_from_net(c, net_output(h->net), h->scale[2], ns, h->stream));
-	HIPCHK(h, launch_cg_box(0, c, ns, h->stream));
-	CGArgs q = c;
-	q.xg = h->d_xg;
-	HIPCHK(h, launch_cg_row(CG_ROW_GUESS, q, h->stream));
-	// r_g = b - A x_g into p, |r_g|^2
-	CGArgs x = c;
-	x.x = h->d_xg;
-	x.r = h->d_p;
-	HIPCHK(h, launch_cg_gather(CG_GATHER_X, x, h->ntiles, h->stream));
-	HIPCHK(h, launch_cg_box(0, x, ns, h->stream));
-	HIPCHK(h, launch_cg_row(CG_ROW_RESID, x, h->stream));
-	HIPCHK(h, launch_cg_reduce(0, x, h->ncells, h->stream));
-	HIPCHK(h, hipMemcpyAsync(h->d_rr + 2, reinterpret_cast<char *>(h->d_cg) + offsetof(CGState, rn2), 2 * sizeof(double),
-			hipMemcpyDeviceToDevice, h->stream));
-	CGArgs l = c;
-	l.xg = h->d_xg;
-	l.rg = h->d_p;
-	l.rr0 = h->d_rr;
-	l.rrg = h->d_rr + 2;
-	HIPCHK(h, launch_cg_row(CG_ROW_LINE1, l, h->stream));
-	HIPCHK(h, launch_cg_reduce(3, l, h->ncells, h->stream));
-	HIPCHK(h, launch_cg_row(CG_ROW_LINE2, l, h->stream));
-	HIPCHK(h, launch_cg_reduce(0, l, h->ncells, h->stream));
-	h->cgs.learned_tried += 1;
-	h->pending_learned_stat = true;
-	return -1;
-}
-
-// Training record of one solve, [L*L][5]: box sums of the initial residual
-// r0 = b - A x_prev, sum phi^2, box sums of the correction x - x_prev.
-// record_start runs after the initial residual pass, record_end after the solve.
-static int record_start(rc_handle *h, const CGArgs &c0) {
+// One refinement stage of the learned initial guess.  With x the current
+// start and r = b - A x its residual, the network predicts the box sums of the
+// correction e = A^-1 r from the box sums of r (scaled by the residual size of
+// each component) and sum phi^2; rows follow from A e = r, i.e.
+// e_i = (r_i + eta N4(k_i)) / D(k_i).  x moves along e by the A-norm optimal
+// step alpha_c = (e.r)/(e.A e), which never raises the A-norm error, and
+// r -= alpha A e.  e.A e comes from box sums like the CG's p.Ap.  The last
+// reduction leaves the CG set-up of the new start.  Launches:
+//   gather<FEAT>  box sums of r -> network input      conv x6  network
+//   box<2>        N4 of the predicted sums
+//   gather<E>     e rows, box sums of e and e^2, e.r  box<1>  N4, e.A e
+//   reduce<3>     alpha                               row<STEP>  x, r; r.r, r.z
+//   reduce<4>     CG set-up
+static int learned_stage(rc_handle *h, const CGArgs &c0) {
 	const int ns = (int) h->box_sites;
 	CGArgs c = c0;
-	float *dst = h->d_rec + (size_t) h->rec_n * 5 * ns;
-	HIPCHK(h, hipMemcpyAsync(h->d_xg, h->d_v, sizeof(double2) * h->rows, hipMemcpyDeviceToDevice, h->stream));
+	c.act0 = net_input(h->net);
+	c.phi2 = h->d_phi2;
+	c.net_out = net_output(h->net);
+	c.sf = h->scale[0];
+	c.sp2 = h->scale[1];
+	c.sv = h->scale[2];
+	c.rows = (double) h->rows;
+	c.nbox_blk = (ns + 255) / 256;
+	HIPCHK(h, launch_cg_gather(CG_GATHER_FEAT, c, h->ntiles, h->stream));
+	if(net_forward_packed(h->net, h->stream))
+		return fail(h, RC_ERR_HIP, "network forward pass failed: %s", rc_net_last_error(h->net));
+	HIPCHK(h, launch_cg_box(2, c, ns, h->stream));
+	HIPCHK(h, launch_cg_gather(CG_GATHER_E, c, h->ntiles, h->stream));
+	HIPCHK(h, launch_cg_box(1, c, ns, h->stream));
+	HIPCHK(h, launch_cg_reduce(3, c, h->ntiles, h->stream));
+	HIPCHK(h, launch_cg_row(CG_ROW_STEP, c, h->stream));
+	HIPCHK(h, launch_cg_reduce(4, c, h->ncells, h->stream));
+	return RC_OK;
+}
+
+// Training records, [L*L][5] each: box sums of the residual r of a start x
+// (divided by the residual scale sqrt(|r_c|^2 / rows)), sum phi^2, and box
+// sums of the correction x_solution - x (same scale).  record_stage runs when
+// the residual of a start is in place (once per solve from the previous
+// solution, once per stage with the network); record_end after the solve.
+static int record_stage(rc_handle *h, const CGArgs &c0) {
+	if(h->rec_n + h->rec_open >= h->rec_cap || h->rec_open >= RC_MAX_STAGES) return RC_OK;
+	const int ns = (int) h->box_sites, s = h->rec_open;
+	const double rows = (double) h->rows;
+	CGArgs c = c0;
+	float *dst = h->d_rec + (size_t) (h->rec_n + s) * 5 * ns;
+	HIPCHK(h, hipMemcpyAsync(h->d_xrec + (size_t) s * h->rows, h->d_v, sizeof(double2) * h->rows, hipMemcpyDeviceToDevice,
+			h->stream));
+	HIPCHK(h, hipMemcpyAsync(h->d_rnrec + 2 * s, reinterpret_cast<char *>(h->d_cg) + offsetof(CGState, rn2),
+			2 * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
 	c.x = h->d_r;
 	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
-	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 0, ns, h->stream));
-	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 2, ns, h->stream));
-	h->rec_open = true;
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 0, h->d_rnrec + 2 * s, rows, ns, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 2, nullptr, rows, ns, h->stream));
+	h->rec_open += 1;
 	return RC_OK;
 }
 static int record_end(rc_handle *h) {
 	const int ns = (int) h->box_sites;
+	const double rows = (double) h->rows;
 	CGArgs c = cg_args(h);
-	float *dst = h->d_rec + (size_t) h->rec_n * 5 * ns;
 	c.x = h->d_v;
 	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
-	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 3, ns, h->stream));
-	c.x = h->d_xg;
-	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
-	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, dst, 5, ns, h->stream));
-	h->rec_n += 1;
-	h->rec_open = false;
+	for(int s = 0; s < h->rec_open; ++s)
+		HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_rec + (size_t) (h->rec_n + s) * 5 * ns, 3, h->d_rnrec + 2 * s,
+				rows, ns, h->stream));
+	for(int s = 0; s < h->rec_open; ++s) {
+		c.x = h->d_xrec + (size_t) s * h->rows;
+		HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
+		HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_rec + (size_t) (h->rec_n + s) * 5 * ns, 5, h->d_rnrec + 2 * s,
+				rows, ns, h->stream));
+	}
+	h->rec_n += h->rec_open;
+	h->rec_open = 0;
 	return RC_OK;
 }
 
@@ -641,22 +640,27 @@ static int cg_solve(rc_handle *h, bool cold) {
 		if(cold) HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * h->rows, h->stream));
 		const int nbox_blk = (int) ((h->box_sites + 255) / 256);
 		{ TimeScope t(h, RC_T_CG_DIAG); HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream)); }
-		bool need_resid = true;
-		if(h->warm_mode == RC_WARM_LEARNED && h->net) {
-			int rc = learned_guess(h, c);
-			if(rc < 0) need_resid = false;   // the guess won; its residual is in place
-			else if(rc) return rc;
-			c.x = h->d_v;
+		{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }
+		{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream)); }
+		const bool learned = h->warm_mode == RC_WARM_LEARNED && h->net;
+		if(h->d_rec) {
+			int rc = record_stage(h, c);
+			if(rc) return rc;
 		}
-		if(need_resid) {
-			{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
-			{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
-			{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }
-			{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream)); }
-			if(h->d_rec && h->rec_n < h->rec_cap) {
-				int rc = record_start(h, c);
+		if(learned) {
+			TimeScope t(h, RC_T_CG_LEARNED);
+			for(int st = 0; st < h->stages; ++st) {
+				int rc = learned_stage(h, c);
 				if(rc) return rc;
+				if(h->d_rec && st + 1 < h->stages) {
+					rc = record_stage(h, c);
+					if(rc) return rc;
+				}
 			}
+			h->cgs.learned_tried += 1;
+			h->pending_learned_stat = true;
 		}
 		{ TimeScope t(h, RC_T_CG_GATHER_P_FIRST); HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream)); }
 		int batch = std::max(2, h->last_iters - 1);
@@ -1165,11 +1169,6 @@ extern "C" int rc_attach_net(rc_handle *h, rc_net *n, const float scale[3]) {
 	net_dims(n, &N, &H, &W);
 	if(N != 1 || H != h->boxes[0].L || W != h->boxes[0].L)
 		return fail(h, RC_ERR_ARG, "network shape %dx%dx%d does not match the %d^2 box", N, H, W, h->boxes[0].L);
-	if(!h->d_xg) {
-		HIPCHK(h, dalloc(&h->d_xg, std::max<long long>(h->rows, 1)));
-		HIPCHK(h, dalloc(&h->d_feat, 3 * h->box_sites));
-		HIPCHK(h, dalloc(&h->d_rr, 4));
-	}
 	h->net = n;
 	for(int k = 0; k < 3; ++k) h->scale[k] = scale ? scale[k] : 1.f;
 	return RC_OK;
@@ -1184,11 +1183,11 @@ extern "C" int rc_snapshot_box(rc_handle *h, float *out) {
 	CGArgs c = cg_args(h);
 	c.x = h->d_f;
 	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
-	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_snap, 0, ns, h->stream));
-	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_snap, 2, ns, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_snap, 0, nullptr, 1., ns, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_snap, 2, nullptr, 1., ns, h->stream));
 	c.x = h->d_v;
 	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
-	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_snap, 3, ns, h->stream));
+	HIPCHK(h, launch_box_snapshot(c, h->d_phi2, h->d_snap, 3, nullptr, 1., ns, h->stream));
 	HIPCHK(h, hipMemcpyAsync(out, h->d_snap, sizeof(float) * 5 * (size_t) ns, hipMemcpyDeviceToHost, h->stream));
 	HIPCHK(h, hipStreamSynchronize(h->stream));
 	return RC_OK;
@@ -1198,12 +1197,25 @@ extern "C" int rc_record_solves(rc_handle *h, int capacity) {
 	if(!h || capacity < 0) return RC_ERR_ARG;
 	if(!h->wet || h->nboxes != 1) return fail(h, RC_ERR_ARG, "recording needs the wet model on one box");
 	dfree(h->d_rec);
+	dfree(h->d_xrec);
+	dfree(h->d_rnrec);
 	h->d_rec = nullptr;
-	h->rec_cap = h->rec_n = 0;
+	h->d_xrec = nullptr;
+	h->d_rnrec = nullptr;
+	h->rec_cap = h->rec_n = h->rec_open = 0;
 	if(capacity == 0) return RC_OK;
 	HIPCHK(h, dalloc(&h->d_rec, (size_t) capacity * 5 * h->box_sites));
-	if(!h->d_xg) HIPCHK(h, dalloc(&h->d_xg, std::max<long long>(h->rows, 1)));
+	HIPCHK(h, dalloc(&h->d_xrec, (size_t) RC_MAX_STAGES * std::max<long long>(h->rows, 1)));
+	HIPCHK(h, dalloc(&h->d_rnrec, 2 * RC_MAX_STAGES));
 	h->rec_cap = capacity;
+	return RC_OK;
+}
+
+extern "C" int rc_set_learned_stages(rc_handle *h, int stages) {
+	if(!h) return RC_ERR_ARG;
+	if(stages < 1 || stages > RC_MAX_STAGES)
+		return fail(h, RC_ERR_ARG, "learned stages %d outside 1..%d", stages, RC_MAX_STAGES);
+	h->stages = stages;
 	return RC_OK;
 }
 

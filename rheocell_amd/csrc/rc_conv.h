@@ -9,6 +9,14 @@ constexpr int NET_LAYERS = 6;
 constexpr int NET_C = 32;                       // channels of every stored tensor
 constexpr int NET_WLAYER = 9 * NET_C * NET_C;   // weights per layer [tap][co][ci]
 
+__device__ __host__ inline unsigned short f2bf(float f) {  // round to nearest even
+	unsigned u = __builtin_bit_cast(unsigned, f);
+	if((u & 0x7F800000u) == 0x7F800000u) return (unsigned short) ((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
+	u += 0x7FFFu + ((u >> 16) & 1u);
+	return (unsigned short) (u >> 16);
+}
+__device__ __host__ inline float bf2f(unsigned short h) { return __builtin_bit_cast(float, ((unsigned) h) << 16); }
+
 hipError_t launch_conv3x3(const unsigned short *in, const unsigned short *w, const float *bias, unsigned short *out,
 		const unsigned short *mask, float *fout, int N, int H, int W, int mode, hipStream_t s);
 hipError_t launch_conv_wgrad(const unsigned short *in, const unsigned short *g, float *part, int N, int H, int W, hipStream_t s);

@@ -22,14 +22,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __host__ inline unsigned short f2bf(float f) {  // round to nearest even
-	unsigned u = __builtin_bit_cast(unsigned, f);
-	if((u & 0x7F800000u) == 0x7F800000u) return (unsigned short) ((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
-	u += 0x7FFFu + ((u >> 16) & 1u);
-	return (unsigned short) (u >> 16);
-}
-__device__ __host__ inline float bf2f(unsigned short h) { return __builtin_bit_cast(float, ((unsigned) h) << 16); }
-
 __device__ inline bf16x8 load8(const unsigned short *p) {
 	const u16x8 v = *reinterpret_cast<const u16x8 *>(p);
 	return __builtin_bit_cast(bf16x8, v);
@@ -39,65 +31,118 @@ __device__ inline bf16x8 load8(const unsigned short *p) {
 // (dy, dx) = (tap / 3 - 1, tap % 3 - 1).  Backprop through a layer uses the
 // same kernel with the tap-mirrored, transposed weights wt[8 - tap][ci][co].
 // mode: 0 store bf16 ReLU(z); 1 store bf16 z; 2 store bf16 z masked by (mask > 0) (backprop through ReLU);
-// fout (optional): fp32 copy of channels 0..1 of z.
+// out may be null (no bf16 store); fout (optional): fp32 copy of channels 0..1 of z.
+//
+// A workgroup (4 waves) computes a CONV_ROWS x (16 CONV_TILES) pixel block:
+// wave w takes image row y0 + w and its CONV_TILES 16-pixel tiles, with the
+// layer's weights held in registers (9 taps x 2 output-channel halves = 18 A
+// operands, loaded from L2).  The block's input rows y0-1 .. y0+CONV_ROWS and
+// columns x0-1 .. x0+16 CONV_TILES (circular) are staged once in LDS, so every
+// input pixel leaves HBM/L2 about (CONV_ROWS+2)/CONV_ROWS times instead of 9
+// times.  (Staging the weights in LDS as well measured slower: the larger
+// LDS footprint cuts residency from 4 to 3 workgroups per CU.)
+// LDS rows are 64 bytes (32 bf16) with the 16-byte chunk index XOR-swizzled by
+// bits 2..3 of the row index: the 16 lanes of a 128-bit read of 16
+// consecutive rows then hit distinct banks.  Weights are the A operand and
+// pixels the B operand, so D[co][px] leaves each lane 4 consecutive channels
+// of one pixel: bf16 results go out as 8-byte stores.
+constexpr int CONV_ROWS = 4;
+constexpr int CONV_TILES = 4;
+constexpr int CONV_BW = 16 * CONV_TILES;           // block width in pixels
+constexpr int CONV_SW = CONV_BW + 2;               // staged columns
+constexpr int CONV_SH = CONV_ROWS + 2;             // staged rows
+
+__device__ inline unsigned pack2bf(float a, float b) { return (unsigned) f2bf(a) | ((unsigned) f2bf(b) << 16); }
+// element offset of 16-byte chunk c (0..3) of LDS row q
+__device__ inline int swz(int q, int c) { return q * 32 + ((c ^ ((q >> 2) & 3)) << 3); }
+
 __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restrict__ in, const unsigned short *__restrict__ w,
 		const float *__restrict__ bias, unsigned short *__restrict__ out, const unsigned short *__restrict__ mask,
 		float *__restrict__ fout, int H, int W, int mode) {
+	__shared__ __attribute__((aligned(16))) unsigned short s_in[CONV_SH * CONV_SW * 32];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int tiles_x = (W + 15) / 16;
-	const int gt = blockIdx.x * 4 + wave;           // 16-pixel tile index over (y, tx) of sample n
-	const int tx = gt % tiles_x;
-	const int y = gt / tiles_x;
+	const int bx = (W + CONV_BW - 1) / CONV_BW;
+	const int x0 = (blockIdx.x % bx) * CONV_BW, y0 = (blockIdx.x / bx) * CONV_ROWS;
 	const int n = blockIdx.y;
-	if(y >= H) return;
-	const int x0 = tx * 16;
-	const int i = lane & 15, kb = lane >> 4;
-	f32x4 acc0 = { 0.f, 0.f, 0.f, 0.f }, acc1 = { 0.f, 0.f, 0.f, 0.f };
 	const size_t img = (size_t) n * H * W;
+	// stage: CONV_SH rows x CONV_SW columns x 4 chunks of 8 channels
+	for(int e = threadIdx.x; e < CONV_SH * CONV_SW * 4; e += 256) {
+		const int ch = e & 3, q = e >> 2, col = q % CONV_SW, r = q / CONV_SW;
+		// circular; columns past W of a partial block wrap too (staged, never read)
+		const int xs = (x0 + col - 1 + W) % W, ys = (y0 + r - 1 + H) % H;
+		const uint4 v = *reinterpret_cast<const uint4 *>(in + (img + (size_t) ys * W + xs) * 32 + ch * 8);
+		*reinterpret_cast<uint4 *>(&s_in[swz(q, ch)]) = v;
+	}
+	const int i = lane & 15, kb = lane >> 4;
+	bf16x8 wa[9], wb[9];
 #pragma unroll
 	for(int tap = 0; tap < 9; ++tap) {
-		const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-		int xs = x0 + i;
-		if(xs >= W) xs = W - 1;                       // padded lanes of a partial tile (results dropped)
-		xs += dx;
-		xs = xs < 0 ? xs + W : (xs >= W ? xs - W : xs);
-		int ys = y + dy;
-		ys = ys < 0 ? ys + H : (ys >= H ? ys - H : ys);
-		const bf16x8 a = load8(in + (img + (size_t) ys * W + xs) * 32 + kb * 8);
-		const bf16x8 b0 = load8(w + ((size_t) tap * 32 + i) * 32 + kb * 8);
-		const bf16x8 b1 = load8(w + ((size_t) tap * 32 + 16 + i) * 32 + kb * 8);
-		acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0, 0, 0, 0);
-		acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1, 0, 0, 0);
+		wa[tap] = load8(w + ((size_t) tap * 32 + i) * 32 + kb * 8);
+		wb[tap] = load8(w + ((size_t) tap * 32 + 16 + i) * 32 + kb * 8);
 	}
-	// D[i][n]: pixel 4*kb + r, channel i (tile 0) / 16 + i (tile 1)
+	float bz0[4], bz1[4];
 #pragma unroll
 	for(int r = 0; r < 4; ++r) {
-		const int px = x0 + 4 * kb + r;
-		if(px >= W) continue;
-		const size_t o = (img + (size_t) y * W + px) * 32;
-		float z0 = acc0[r], z1 = acc1[r];
-		if(bias) {
-			z0 += bias[i];
-			z1 += bias[16 + i];
+		bz0[r] = bias ? bias[4 * kb + r] : 0.f;
+		bz1[r] = bias ? bias[16 + 4 * kb + r] : 0.f;
+	}
+	__syncthreads();
+	const int y = y0 + wave;
+	if(y >= H) return;
+	for(int t = 0; t < CONV_TILES; ++t) {
+		const int px = x0 + t * 16 + i;
+		if(x0 + t * 16 >= W) break;
+		f32x4 acc0 = { 0.f, 0.f, 0.f, 0.f }, acc1 = { 0.f, 0.f, 0.f, 0.f };
+		bf16x8 a[9];
+#pragma unroll
+		for(int tap = 0; tap < 9; ++tap) {
+			const int dy = tap / 3, dx = tap % 3;   // staged row wave + dy, column t*16 + i + dx
+			const u16x8 v = *reinterpret_cast<const u16x8 *>(&s_in[swz((wave + dy) * CONV_SW + t * 16 + i + dx, kb)]);
+			a[tap] = __builtin_bit_cast(bf16x8, v);
 		}
-		if(fout && i < 2) fout[(img + (size_t) y * W + px) * 2 + i] = z0;
+#pragma unroll
+		for(int tap = 0; tap < 9; ++tap) {
+			acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[tap], a[tap], acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb[tap], a[tap], acc1, 0, 0, 0);
+		}
+		// D[co][px]: lane holds channels 4 kb + r (acc0) and 16 + 4 kb + r (acc1) of pixel px
+		const size_t pix = img + (size_t) y * W + px;
+		float z0[4], z1[4];
+#pragma unroll
+		for(int r = 0; r < 4; ++r) {
+			z0[r] = acc0[r] + bz0[r];
+			z1[r] = acc1[r] + bz1[r];
+		}
+		if(fout && kb == 0) *reinterpret_cast<float2 *>(fout + pix * 2) = make_float2(z0[0], z0[1]);
+		if(!out) continue;
 		if(mode == 0) {
-			z0 = fmaxf(z0, 0.f);
-			z1 = fmaxf(z1, 0.f);
+#pragma unroll
+			for(int r = 0; r < 4; ++r) {
+				z0[r] = fmaxf(z0[r], 0.f);
+				z1[r] = fmaxf(z1[r], 0.f);
+			}
 		}
 		else if(mode == 2) {
-			if(!(bf2f(mask[o + i]) > 0.f)) z0 = 0.f;
-			if(!(bf2f(mask[o + 16 + i]) > 0.f)) z1 = 0.f;
+			const uint2 m0 = *reinterpret_cast<const uint2 *>(mask + pix * 32 + 4 * kb);
+			const uint2 m1 = *reinterpret_cast<const uint2 *>(mask + pix * 32 + 16 + 4 * kb);
+			const unsigned mm0[4] = { m0.x & 0xFFFFu, m0.x >> 16, m0.y & 0xFFFFu, m0.y >> 16 };
+			const unsigned mm1[4] = { m1.x & 0xFFFFu, m1.x >> 16, m1.y & 0xFFFFu, m1.y >> 16 };
+#pragma unroll
+			for(int r = 0; r < 4; ++r) {
+				if(!(bf2f((unsigned short) mm0[r]) > 0.f)) z0[r] = 0.f;
+				if(!(bf2f((unsigned short) mm1[r]) > 0.f)) z1[r] = 0.f;
+			}
 		}
-		out[o + i] = f2bf(z0);
-		out[o + 16 + i] = f2bf(z1);
+		*reinterpret_cast<uint2 *>(out + pix * 32 + 4 * kb) = make_uint2(pack2bf(z0[0], z0[1]), pack2bf(z0[2], z0[3]));
+		*reinterpret_cast<uint2 *>(out + pix * 32 + 16 + 4 * kb) = make_uint2(pack2bf(z1[0], z1[1]), pack2bf(z1[2], z1[3]));
 	}
 }
 
 hipError_t launch_conv3x3(const unsigned short *in, const unsigned short *w, const float *bias, unsigned short *out,
 		const unsigned short *mask, float *fout, int N, int H, int W, int mode, hipStream_t s) {
-	const int tiles = H * ((W + 15) / 16);
-	hipLaunchKernelGGL(k_conv3x3, dim3((tiles + 3) / 4, N), dim3(256), 0, s, in, w, bias, out, mask, fout, H, W, mode);
+	if(W % 16 || H < 1 || W < 1) return hipErrorInvalidValue;
+	const int blocks = ((W + CONV_BW - 1) / CONV_BW) * ((H + CONV_ROWS - 1) / CONV_ROWS);
+	hipLaunchKernelGGL(k_conv3x3, dim3(blocks, N), dim3(256), 0, s, in, w, bias, out, mask, fout, H, W, mode);
 	return hipGetLastError();
 }
 

@@ -59,7 +59,7 @@ struct DevStatus {
 // CG state of the two simultaneous solves (x and y components share A)
 struct CGState {
 	double bb[2], thr[2], absNew[2], alpha[2], beta[2], rn2[2];
-	double alpha0[2];   // line-search step along the learned correction (0: previous guess kept)
+	double alpha0[2];   // step along the correction of a learned refinement stage
 	int done[2], iters[2], zero_rhs[2], max_iter, active;
 };
 

@@ -48,8 +48,8 @@ struct GatherArgs {
 	DevStatus *st;
 };
 
-enum CGGatherMode { CG_GATHER_X = 0, CG_GATHER_P_FIRST = 1, CG_GATHER_P_UPDATE = 2 };
-enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_UPDATE = 2, CG_ROW_GUESS = 3, CG_ROW_LINE1 = 4, CG_ROW_LINE2 = 5 };
+enum CGGatherMode { CG_GATHER_X = 0, CG_GATHER_P_FIRST = 1, CG_GATHER_P_UPDATE = 2, CG_GATHER_FEAT = 3, CG_GATHER_E = 4 };
+enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_UPDATE = 2, CG_ROW_STEP = 5 };
 
 struct CGArgs {
 	const DevBox *boxes;
@@ -66,12 +66,18 @@ struct CGArgs {
 	double2 *p2;                 // box sums of p^2
 	double2 *n4;                 // sum of sp over the 4 box neighbours
 	double2 *x, *r, *p, *b;
-	double2 *xg, *rg;            // learned guess rows and their residual
+	// learned refinement stage: network input (bf16 [L*L][32]) and output (fp32 [L*L][2]),
+	// feature / output multipliers, row count (residual scale sqrt(|r_c|^2 / rows))
+	unsigned short *act0;
+	const double *phi2;
+	const float *net_out;
+	float sf, sp2, sv;
+	double rows;
+	int nbox_blk;                // workgroups of a box-site pass (partials of k_cg_box<1>)
 	double *part;                // per-cell partial sums, 6 per cell
 	double *part_box;            // per-workgroup partials of the box pass, 2 each
 	CGState *cg;
 	double tol;
-	const double *rr0, *rrg;     // |r0|^2, |rg|^2 per component (line search)
 	int max_iter;
 	int ncells;
 	DevStatus *st;
@@ -81,15 +87,17 @@ hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s);
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds_bytes, hipStream_t s);
 hipError_t launch_cg_diag(const CGArgs &a, int nbox_sites, hipStream_t s);
 hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s);
-hipError_t launch_cg_box(int pap, const CGArgs &a, int nsites, hipStream_t s);
+// mode 0: N4 of sp; 1: N4 and the p.Ap partials; 2: N4 of the network output (learned stage)
+hipError_t launch_cg_box(int mode, const CGArgs &a, int nsites, hipStream_t s);
 hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s);
-// what: 0 after the residual pass, 1 after p.Ap, 2 after the x,r update, 3 line search
+// what: 0 after the residual pass, 1 after p.Ap, 2 after the x,r update,
+// 3 step of a learned stage, 4 after a learned stage's x,r update
 hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s);
-// learned guess helpers: features (sum r0 * sf, sum phi^2 * sp2) -> feat [L*L][3]; sp = net output * sv
-hipError_t launch_box_features(const CGArgs &a, const double *phi2, float *feat, float sf, float sp2, int nsites,
-		hipStream_t s);
-hipError_t launch_box_from_net(const CGArgs &a, const float *out, float sv, int nsites, hipStream_t s);
-hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap, int chan, int nsites, hipStream_t s);
+// snapshot channels: chan 0 -> sp into [k][0..1], chan 2 -> phi2 into [k][2],
+// chan 3 -> sp into [k][3..4], chan 5 -> [k][3..4] -= sp; sp is divided by
+// sqrt(rn2_c / rows) when rn2 is given
+hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap, int chan, const double *rn2, double rows,
+		int nsites, hipStream_t s);
 hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
 hipError_t stage_set_lds_limit(size_t bytes);
 // sites-per-thread instantiation for a largest patch of max_sites (-1: too large)

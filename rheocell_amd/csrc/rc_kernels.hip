@@ -22,6 +22,7 @@
 // All arithmetic is FP64 and is compiled with -ffp-contract=off so that
 // every per-site expression is the IEEE evaluation of the reference source.
 #include "rc_kernels.h"
+#include "rc_conv.h"
 
 #include <hip/hip_ext.h>
 
@@ -702,17 +703,10 @@ __device__ void cg_after_resid(CGState *cg, const double *tot, double tol, int m
 	}
 	cg->active = !(cg->done[0] && cg->done[1]);
 }
-// alpha0 = (d.r0)/(d.A d) (A-norm optimal step along the learned correction),
-// kept only when it lowers |r|^2 = |(1-a) r0 + a rg|^2 below |r0|^2; needs
-// |r0|^2 and |rg|^2 in rn2 / thr (set by the two residual passes)
-__device__ void cg_after_line(CGState *cg, const double *tot, const double *rr0, const double *rrg) {
-	for(int c = 0; c < 2; ++c) {
-		const double dr = tot[c], dad = tot[2 + c], r0g = tot[4 + c];
-		double al = dad > 0 ? dr / dad : 0.;
-		const double rr = (1 - al) * (1 - al) * rr0[c] + 2 * al * (1 - al) * r0g + al * al * rrg[c];
-		if(!(rr < rr0[c])) al = 0.;
-		cg->alpha0[c] = al;
-	}
+// alpha0_c = (e.r)/(e.A e): the A-norm optimal step along the learned
+// correction e (zero when e.A e is not positive or the solve is done)
+__device__ void cg_after_step(CGState *cg, const double *er, const double *eae) {
+	for(int c = 0; c < 2; ++c) cg->alpha0[c] = cg->active && eae[c] > 0 ? er[c] / eae[c] : 0.;
 }
 __device__ void cg_after_pap(CGState *cg, const double *tot) {
 	for(int c = 0; c < 2; ++c)
@@ -735,15 +729,9 @@ __device__ void cg_after_update(CGState *cg, const double *tot) {
 	cg->active = !(cg->done[0] && cg->done[1]);
 }
 
-// one workgroup of 1024: sums SoA partials of the previous kernel
-template <int WHAT>
-__global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
-	if(a.st->flags) return;
-	CGState *cg = a.cg;
-	if((WHAT == 1 || WHAT == 2) && !cg->active) return;
-	constexpr int NV = WHAT == 0 || WHAT == 3 ? 6 : WHAT == 1 ? 2 : 4;
-	const double *part = WHAT == 1 ? a.part_box : a.part;
-	__shared__ double s_red[16 * NV];
+// one workgroup of 1024: sums SoA partials of the previous kernel(s) in a fixed order
+template <int NV>
+__device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[NV], double *s_red) {
 	double acc[NV];
 #pragma unroll
 	for(int j = 0; j < NV; ++j) acc[j] = 0.;
@@ -759,17 +747,38 @@ __global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
 		for(int j = 0; j < NV; ++j) s_red[wave * NV + j] = acc[j];
 	}
 	__syncthreads();
-	if(threadIdx.x != 0) return;
-	double tot[NV];
 #pragma unroll
 	for(int j = 0; j < NV; ++j) {
 		double t = 0.;
 		for(int w = 0; w < 16; ++w) t += s_red[w * NV + j];
 		tot[j] = t;
 	}
+	__syncthreads();
+}
+
+template <int WHAT>
+__global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
+	if(a.st->flags) return;
+	CGState *cg = a.cg;
+	if((WHAT == 1 || WHAT == 2) && !cg->active) return;
+	constexpr int NV = WHAT == 0 ? 6 : WHAT == 1 || WHAT == 3 ? 2 : 4;
+	__shared__ double s_red[16 * 6];
+	double tot[NV];
+	reduce_parts<NV>(WHAT == 1 ? a.part_box : a.part, nblk, tot, s_red);
+	if(WHAT == 3) {
+		double eae[2];
+		reduce_parts<2>(a.part_box, a.nbox_blk, eae, s_red);
+		if(threadIdx.x == 0) cg_after_step(cg, tot, eae);
+		return;
+	}
+	if(threadIdx.x != 0) return;
 	if(WHAT == 0) cg_after_resid(cg, tot, a.tol, a.max_iter);
 	else if(WHAT == 1) cg_after_pap(cg, tot);
-	else if(WHAT == 3) cg_after_line(cg, tot, a.rr0, a.rrg);
+	else if(WHAT == 4) {
+		// residual of the new start; |b|^2 is that of the solve's first residual pass
+		const double t6[6] = { cg->bb[0], cg->bb[1], tot[0], tot[1], tot[2], tot[3] };
+		cg_after_resid(cg, t6, a.tol, a.max_iter);
+	}
 	else cg_after_update(cg, tot);
 }
 
@@ -778,9 +787,16 @@ hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s) 
 	case 0: RC_LAUNCH(k_cg_reduce<0>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	case 1: RC_LAUNCH(k_cg_reduce<1>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	case 3: RC_LAUNCH(k_cg_reduce<3>, dim3(1), dim3(1024), 0, s, a, nblk); break;
+	case 4: RC_LAUNCH(k_cg_reduce<4>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	default: RC_LAUNCH(k_cg_reduce<2>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	}
 	return hipGetLastError();
+}
+
+// residual scale of component c: sqrt(|r_c|^2 / rows) (zero for a zero residual)
+__device__ inline double resid_scale(const CGState *cg, int c, double rows) {
+	const double r2 = cg->rn2[c];
+	return r2 > 0 ? sqrt(r2 / rows) : 0.;
 }
 
 template <int MODE>
@@ -799,6 +815,32 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 		unpack_geo(a.geo[a.tile_cells[tile * MAXT + j]], L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
 	__syncthreads();
 	const int sx = tid % TW, sy = tid / TW;
+	if(MODE == CG_GATHER_E) {
+		// e = (r + eta N4) / D per covering row (A e = r row by row); box sums of e, e^2; e.r -> partials
+		__shared__ double s_red[4 * 2];
+		double er[2] = { 0., 0. };
+		if(sx < tw && sy < th) {
+			const int k = B.box_off + (y0 + sy) * L + x0 + sx;
+			const double D = a.diag[k], eta = a.params[a.set].friction_cell;
+			const double2 n4 = a.n4[k];
+			double2 acc = make_double2(0., 0.), acc2 = make_double2(0., 0.);
+			for(int j = 0; j < n; ++j) {
+				const int row = covering_row(x0 + sx, y0 + sy, L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
+				if(row < 0) continue;
+				const double2 r = a.r[row];
+				const double2 e = make_double2((r.x + eta * n4.x) / D, (r.y + eta * n4.y) / D);
+				a.p[row] = e;
+				acc.x += e.x; acc.y += e.y;
+				acc2.x += e.x * e.x; acc2.y += e.y * e.y;
+				er[0] += e.x * r.x; er[1] += e.y * r.y;
+			}
+			a.sp[k] = acc;
+			a.p2[k] = acc2;
+		}
+		block_sum<2>(er, s_red);
+		block_partials<2>(er, a.part);
+		return;
+	}
 	if(sx >= tw || sy >= th) return;
 	const int x = x0 + sx, y = y0 + sy;
 	const int k = B.box_off + y * L + x;
@@ -824,6 +866,7 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 			pv[q] = make_double2(0., 0.);
 			if(row[q] >= 0) {
 				if(MODE == CG_GATHER_X) rv[q] = a.x[row[q]];
+				else if(MODE == CG_GATHER_FEAT) rv[q] = a.r[row[q]];
 				else {
 					rv[q] = a.r[row[q]];
 					if(MODE == CG_GATHER_P_UPDATE) pv[q] = a.p[row[q]];
@@ -834,7 +877,7 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 		for(int q = 0; q < GCHUNK; ++q) {
 			if(row[q] < 0) continue;
 			double2 v;
-			if(MODE == CG_GATHER_X) v = rv[q];
+			if(MODE == CG_GATHER_X || MODE == CG_GATHER_FEAT) v = rv[q];
 			else {
 				if(MODE == CG_GATHER_P_FIRST) v = make_double2(inv * rv[q].x, inv * rv[q].y);
 				else {
@@ -849,6 +892,19 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 			acc.y += v.y;
 		}
 	}
+	if(MODE == CG_GATHER_FEAT) {
+		// network input: residual sums over the residual scale, sum phi^2 (bf16 channels 0..7)
+		const double n0 = resid_scale(cg, 0, a.rows), n1 = resid_scale(cg, 1, a.rows);
+		const float f0 = n0 > 0 ? (float) (acc.x / n0) * a.sf : 0.f;
+		const float f1 = n1 > 0 ? (float) (acc.y / n1) * a.sf : 0.f;
+		const float f2 = (float) a.phi2[k] * a.sp2;
+		uint4 o;
+		o.x = (unsigned) f2bf(f0) | ((unsigned) f2bf(f1) << 16);
+		o.y = (unsigned) f2bf(f2);
+		o.z = o.w = 0u;
+		*reinterpret_cast<uint4 *>(a.act0 + (size_t) (k - B.box_off) * 32) = o;
+		return;
+	}
 	a.sp[k] = acc;
 	if(MODE != CG_GATHER_X) a.p2[k] = acc2;
 }
@@ -857,17 +913,21 @@ hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s
 	switch(mode) {
 	case CG_GATHER_X: RC_LAUNCH(k_cg_gather<CG_GATHER_X>, dim3(ntiles), dim3(256), 0, s, a); break;
 	case CG_GATHER_P_FIRST: RC_LAUNCH(k_cg_gather<CG_GATHER_P_FIRST>, dim3(ntiles), dim3(256), 0, s, a); break;
+	case CG_GATHER_FEAT: RC_LAUNCH(k_cg_gather<CG_GATHER_FEAT>, dim3(ntiles), dim3(256), 0, s, a); break;
+	case CG_GATHER_E: RC_LAUNCH(k_cg_gather<CG_GATHER_E>, dim3(ntiles), dim3(256), 0, s, a); break;
 	default: RC_LAUNCH(k_cg_gather<CG_GATHER_P_UPDATE>, dim3(ntiles), dim3(256), 0, s, a); break;
 	}
 	return hipGetLastError();
 }
 
 // N4(k) and, for PAP, p.Ap reduced to alpha (Eigen: alpha = absNew / p.dot(A p))
-template <int PAP>
+// MODE 0: N4 of sp; 1: N4 and the p.Ap partials; 2: N4 of the network output
+// sp = out * sv * (residual scale) of a learned stage
+template <int MODE>
 __global__ __launch_bounds__(256) void k_cg_box(CGArgs a, int nsites) {
 	if(a.st->flags) return;
 	CGState *cg = a.cg;
-	if(PAP && !cg->active) return;
+	if(MODE != 0 && !cg->active) return;
 	__shared__ double s_red[4 * 2];
 	const int k = blockIdx.x * 256 + threadIdx.x;
 	const DevBox B = a.boxes[0];
@@ -878,25 +938,36 @@ __global__ __launch_bounds__(256) void k_cg_box(CGArgs a, int nsites) {
 		const int xe = x + 1 == L ? 0 : x + 1, xw = x == 0 ? L - 1 : x - 1;
 		const int yn = y + 1 == L ? 0 : y + 1, ys = y == 0 ? L - 1 : y - 1;
 		const int o = B.box_off;
-		const double2 sS = a.sp[o + ys * L + x], sW = a.sp[o + y * L + xw], sE = a.sp[o + y * L + xe],
-				sN = a.sp[o + yn * L + x];
-		const double2 n4 = make_double2(sS.x + sW.x + sE.x + sN.x, sS.y + sW.y + sE.y + sN.y);
+		double2 n4;
+		if(MODE == 2) {
+			const float2 *out = reinterpret_cast<const float2 *>(a.net_out);
+			const float2 sS = out[ys * L + x], sW = out[y * L + xw], sE = out[y * L + xe], sN = out[yn * L + x];
+			const double f0 = a.sv * resid_scale(cg, 0, a.rows), f1 = a.sv * resid_scale(cg, 1, a.rows);
+			n4 = make_double2(f0 * ((double) sS.x + (double) sW.x + (double) sE.x + (double) sN.x),
+					f1 * ((double) sS.y + (double) sW.y + (double) sE.y + (double) sN.y));
+		}
+		else {
+			const double2 sS = a.sp[o + ys * L + x], sW = a.sp[o + y * L + xw], sE = a.sp[o + y * L + xe],
+					sN = a.sp[o + yn * L + x];
+			n4 = make_double2(sS.x + sW.x + sE.x + sN.x, sS.y + sW.y + sE.y + sN.y);
+		}
 		a.n4[o + k] = n4;
-		if(PAP) {
+		if(MODE == 1) {
 			const double D = a.diag[o + k];
 			const double2 sp = a.sp[o + k], p2 = a.p2[o + k];
 			part[0] = D * p2.x - eta * sp.x * n4.x;
 			part[1] = D * p2.y - eta * sp.y * n4.y;
 		}
 	}
-	if(!PAP) return;
+	if(MODE != 1) return;
 	block_sum<2>(part, s_red);
 	block_partials<2>(part, a.part_box);
 }
 
-hipError_t launch_cg_box(int pap, const CGArgs &a, int nsites, hipStream_t s) {
+hipError_t launch_cg_box(int mode, const CGArgs &a, int nsites, hipStream_t s) {
 	const dim3 g((nsites + 255) / 256);
-	if(pap) RC_LAUNCH(k_cg_box<1>, g, dim3(256), 0, s, a, nsites);
+	if(mode == 1) RC_LAUNCH(k_cg_box<1>, g, dim3(256), 0, s, a, nsites);
+	else if(mode == 2) RC_LAUNCH(k_cg_box<2>, g, dim3(256), 0, s, a, nsites);
 	else RC_LAUNCH(k_cg_box<0>, g, dim3(256), 0, s, a, nsites);
 	return hipGetLastError();
 }
@@ -934,32 +1005,19 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 		const double D = a.diag[k];
 		const double2 n4 = a.n4[k];
 		const int row = C.start + q;
-		if(MODE == CG_ROW_LINE1 || MODE == CG_ROW_LINE2) {
+		if(MODE == CG_ROW_STEP) {
+			// x += alpha0 e, r -= alpha0 A e (A e = D e - eta N4 of the box sums of e); r.r, r.z
 			const double inv = 1. / D;
-			// line search along d = xg - x (previous solution): r0 = b - A x, rg = b - A xg, A d = r0 - rg
-			const double2 x0 = a.x[row], xg = a.xg[row], r0 = a.r[row], rg = a.rg[row];
-			const double2 d = make_double2(xg.x - x0.x, xg.y - x0.y);
-			if(MODE == CG_ROW_LINE1) {
-				red[0] += d.x * r0.x; red[1] += d.y * r0.y;
-				red[2] += d.x * (r0.x - rg.x); red[3] += d.y * (r0.y - rg.y);
-				red[4] += r0.x * rg.x; red[5] += r0.y * rg.y;
-				continue;
-			}
 			const double a0 = cg->alpha0[0], a1 = cg->alpha0[1];
-			const double2 xn = make_double2(x0.x + a0 * d.x, x0.y + a1 * d.y);
-			const double2 rn = make_double2((1. - a0) * r0.x + a0 * rg.x, (1. - a1) * r0.y + a1 * rg.y);
-			a.x[row] = xn;
+			const double2 e = a.p[row];
+			double2 xx = a.x[row], rn = a.r[row];
+			xx.x += a0 * e.x; xx.y += a1 * e.y;
+			rn.x -= a0 * (D * e.x - eta * n4.x);
+			rn.y -= a1 * (D * e.y - eta * n4.y);
+			a.x[row] = xx;
 			a.r[row] = rn;
-			const double2 bb = a.b[row];
-			red[0] += bb.x * bb.x; red[1] += bb.y * bb.y;
-			red[2] += rn.x * rn.x; red[3] += rn.y * rn.y;
-			red[4] += rn.x * (inv * rn.x); red[5] += rn.y * (inv * rn.y);
-			continue;
-		}
-		if(MODE == CG_ROW_GUESS) {
-			// correction rows from its box sums: A e = r0 gives e_i = (r0_i + eta N4(k_i)) / D(k_i)
-			const double2 r0 = a.r[row], x0 = a.x[row];
-			a.xg[row] = make_double2(x0.x + (r0.x + eta * n4.x) / D, x0.y + (r0.y + eta * n4.y) / D);
+			red[0] += rn.x * rn.x; red[1] += rn.y * rn.y;
+			red[2] += rn.x * (inv * rn.x); red[3] += rn.y * (inv * rn.y);
 			continue;
 		}
 		const double2 vin = MODE == CG_ROW_RESID ? a.x[row] : a.p[row];
@@ -985,8 +1043,7 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 			red[2] += rr.x * (inv * rr.x); red[3] += rr.y * (inv * rr.y);
 		}
 	}
-	if(MODE == CG_ROW_GUESS) return;
-	if(MODE == CG_ROW_RESID || MODE == CG_ROW_LINE1 || MODE == CG_ROW_LINE2) {
+	if(MODE == CG_ROW_RESID) {
 		block_sum<6>(red, s_red);
 		block_partials<6>(red, a.part);
 	}
@@ -999,59 +1056,39 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 
 hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s) {
 	if(mode == CG_ROW_RESID) RC_LAUNCH(k_cg_row<CG_ROW_RESID>, dim3(a.ncells), dim3(256), 0, s, a);
-	else if(mode == CG_ROW_GUESS) RC_LAUNCH(k_cg_row<CG_ROW_GUESS>, dim3(a.ncells), dim3(256), 0, s, a);
-	else if(mode == CG_ROW_LINE1) RC_LAUNCH(k_cg_row<CG_ROW_LINE1>, dim3(a.ncells), dim3(256), 0, s, a);
-	else if(mode == CG_ROW_LINE2) RC_LAUNCH(k_cg_row<CG_ROW_LINE2>, dim3(a.ncells), dim3(256), 0, s, a);
+	else if(mode == CG_ROW_STEP) RC_LAUNCH(k_cg_row<CG_ROW_STEP>, dim3(a.ncells), dim3(256), 0, s, a);
 	else RC_LAUNCH(k_cg_row<CG_ROW_UPDATE>, dim3(a.ncells), dim3(256), 0, s, a);
 	return hipGetLastError();
 }
 
-// network input per box site: the box sums of the residual force
-// r0 = b - A x_prev (a.sp) and sum phi^2
-__global__ void k_box_features(CGArgs a, const double *phi2, float *feat, float sf, float sp2, int nsites) {
+// snapshot channels: chan 0 -> sp into [k][0..1], chan 2 -> phi2 into [k][2],
+// chan 3 -> sp into [k][3..4], chan 5 -> [k][3..4] -= sp; with rn2 given, sp_c
+// is divided by sqrt(rn2_c / rows) (zero when rn2_c is zero)
+__global__ void k_box_snapshot(CGArgs a, const double *phi2, float *snap, int chan, const double *rn2, double rows,
+		int nsites) {
 	const int k = blockIdx.x * 256 + threadIdx.x;
 	if(k >= nsites) return;
-	const double2 v = a.sp[k];
-	feat[3 * k + 0] = (float) (v.x * sf);
-	feat[3 * k + 1] = (float) (v.y * sf);
-	feat[3 * k + 2] = (float) (phi2[k] * sp2);
-}
-hipError_t launch_box_features(const CGArgs &a, const double *phi2, float *feat, float sf, float sp2, int nsites,
-		hipStream_t s) {
-	RC_LAUNCH(k_box_features, dim3((nsites + 255) / 256), dim3(256), 0, s, a, phi2, feat, sf, sp2, nsites);
-	return hipGetLastError();
-}
-
-// network output -> box sums of the velocity correction (a.sp)
-__global__ void k_box_from_net(CGArgs a, const float *out, float sv, int nsites) {
-	const int k = blockIdx.x * 256 + threadIdx.x;
-	if(k >= nsites) return;
-	a.sp[k] = make_double2((double) out[2 * k] * sv, (double) out[2 * k + 1] * sv);
-}
-hipError_t launch_box_from_net(const CGArgs &a, const float *out, float sv, int nsites, hipStream_t s) {
-	RC_LAUNCH(k_box_from_net, dim3((nsites + 255) / 256), dim3(256), 0, s, a, out, sv, nsites);
-	return hipGetLastError();
-}
-
-// snapshot channels: chan 0 -> (sp.x, sp.y) into [k][0..1], chan 2 -> phi2 into [k][2],
-// chan 3 -> (sp.x, sp.y) into [k][3..4], chan 5 -> [k][3..4] -= (sp.x, sp.y)
-__global__ void k_box_snapshot(CGArgs a, const double *phi2, float *snap, int chan, int nsites) {
-	const int k = blockIdx.x * 256 + threadIdx.x;
-	if(k >= nsites) return;
-	if(chan == 2) snap[5 * k + 2] = (float) phi2[k];
-	else if(chan == 5) {
-		const double2 v = a.sp[k];
+	if(chan == 2) {
+		snap[5 * k + 2] = (float) phi2[k];
+		return;
+	}
+	double2 v = a.sp[k];
+	if(rn2) {
+		v.x = rn2[0] > 0 ? v.x / sqrt(rn2[0] / rows) : 0.;
+		v.y = rn2[1] > 0 ? v.y / sqrt(rn2[1] / rows) : 0.;
+	}
+	if(chan == 5) {
 		snap[5 * k + 3] -= (float) v.x;
 		snap[5 * k + 4] -= (float) v.y;
 	}
 	else {
-		const double2 v = a.sp[k];
 		snap[5 * k + chan] = (float) v.x;
 		snap[5 * k + chan + 1] = (float) v.y;
 	}
 }
-hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap, int chan, int nsites, hipStream_t s) {
-	RC_LAUNCH(k_box_snapshot, dim3((nsites + 255) / 256), dim3(256), 0, s, a, phi2, snap, chan, nsites);
+hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap, int chan, const double *rn2, double rows,
+		int nsites, hipStream_t s) {
+	RC_LAUNCH(k_box_snapshot, dim3((nsites + 255) / 256), dim3(256), 0, s, a, phi2, snap, chan, rn2, rows, nsites);
 	return hipGetLastError();
 }
 

@@ -101,6 +101,7 @@ extern "C" int rc_net_create(int N, int H, int W, rc_net **out) {
 	NCHK(n, hipMemcpy(n->maskb, mb.data(), nb * sizeof(float), hipMemcpyHostToDevice));
 	NCHK(n, hipMemset(n->wf, 0, nw * sizeof(float)));
 	NCHK(n, hipMemset(n->bf, 0, nb * sizeof(float)));
+	NCHK(n, hipMemset(n->act[0], 0, (size_t) n->npix * NET_C * sizeof(unsigned short)));
 	return RC_OK;
 }
 
@@ -141,15 +142,18 @@ extern "C" int rc_net_get_params(rc_net *n, float *w, float *b) {
 	return RC_OK;
 }
 
-// forward pass on n->act[0] already packed; result in n->out (fp32, 2 channels)
+// forward pass on n->act[0] already packed; result in n->out (fp32, 2 channels);
+// the last layer stores no bf16 tensor (backward reads act[0..5] only)
 static int forward_packed(rc_net *n, hipStream_t s) {
 	for(int l = 0; l < NET_LAYERS; ++l) {
 		const bool last = l == NET_LAYERS - 1;
-		NCHK(n, launch_conv3x3(n->act[l], n->w + (size_t) l * NET_WLAYER, n->bf + l * NET_C, n->act[l + 1], nullptr,
-				last ? n->out : nullptr, n->N, n->H, n->W, last ? 1 : 0, s));
+		NCHK(n, launch_conv3x3(n->act[l], n->w + (size_t) l * NET_WLAYER, n->bf + l * NET_C, last ? nullptr : n->act[l + 1],
+				nullptr, last ? n->out : nullptr, n->N, n->H, n->W, last ? 1 : 0, s));
 	}
 	return RC_OK;
 }
+unsigned short *net_input(rc_net *n) { return n->act[0]; }
+int net_forward_packed(rc_net *n, hipStream_t s) { return forward_packed(n, s); }
 
 int net_forward_device(rc_net *n, const float *d_x, hipStream_t s) {
 	NCHK(n, launch_pack_features(d_x, n->act[0], n->npix, s));

@@ -106,34 +106,44 @@ def snapshot(sim: Simulation) -> np.ndarray:
     return out.reshape(L, L, 5)
 
 
-def record_run(L: int, nside: int, seed: int, steps: int, burn_in: int = 10) -> np.ndarray:
-    """Every wet solve of one run (2 per step), [solves, L, L, 5]: box sums of
-    the previous solution's residual (x, y), sum phi^2, box sums of the
-    correction from the previous solution to the new one (x, y)."""
+def record_run(L: int, nside: int, seed: int, steps: int, burn_in: int = 10, params=None,
+               stages: int = 1) -> np.ndarray:
+    """Training records of one wet run, [records, L, L, 5]: box sums of the
+    residual of a start (x, y), sum phi^2, box sums of the correction from the
+    start to the converged solution (x, y), both in units of the start's
+    residual scale.  Without params every solve gives one record (start = the
+    previous solution); with params = (w, b, scale) the network refines each
+    start over `stages` stages and every stage's start gives a record."""
     conf = confgen.lattice_conf(L, nside, seed)
     keys = dict(confgen.BASE_KEYS, friction_cell=1, CGtolerance=1e-8, seed=seed)
     sim = Simulation([(conf, keys)])
-    sim.set_warm_start(_lib.RC_WARM_PREVIOUS)
+    per_solve = 1
+    if params is None:
+        sim.set_warm_start(_lib.RC_WARM_PREVIOUS)
+    else:
+        attach(sim, *params, stages=stages)
+        per_solve = stages
     if burn_in:
         sim.step(burn_in)
-    sim._check(sim.L.rc_record_solves(sim.h, 2 * steps))
+    cap = 2 * steps * per_solve
+    sim._check(sim.L.rc_record_solves(sim.h, cap))
     sim.step(steps)
     n = C.c_int()
-    buf = np.empty((2 * steps, L, L, 5), dtype=np.float32)
+    buf = np.empty((cap, L, L, 5), dtype=np.float32)
     sim._check(sim.L.rc_get_recorded(sim.h, _fp(buf), C.byref(n)))
     sim.close()
     return buf[:n.value]
 
 
 def generate_snapshots(L: int = 256, nside: int = 20, seeds: Sequence[int] = (11, 12), steps: int = 200,
-                       burn_in: int = 10, log=None) -> List[np.ndarray]:
+                       burn_in: int = 10, log=None, params=None, stages: int = 1) -> List[np.ndarray]:
     """Wet-model runs (friction_cell = 1, CG to 1e-8); every solve is recorded."""
     runs = []
     for sd in seeds:
         t0 = time.time()
-        runs.append(record_run(L, nside, sd, steps, burn_in))
+        runs.append(record_run(L, nside, sd, steps, burn_in, params=params, stages=stages))
         if log:
-            log("[learned] seed %d: %d solves recorded in %.1f s" % (sd, len(runs[-1]), time.time() - t0))
+            log("[learned] seed %d: %d records in %.1f s" % (sd, len(runs[-1]), time.time() - t0))
     return runs
 
 
@@ -158,11 +168,13 @@ def normalise(x: np.ndarray, y: np.ndarray, scale: np.ndarray) -> Tuple[np.ndarr
 
 
 def train(runs: List[np.ndarray], epochs: int = 20, batch: int = 8, lr: float = 2e-3, seed: int = 0, log=None,
-          params: Optional[Tuple[np.ndarray, np.ndarray]] = None):
+          params: Optional[Tuple[np.ndarray, np.ndarray]] = None, scale: Optional[np.ndarray] = None):
     """Trains on the GPU (the library's forward/backward kernels and Adam);
-    returns (w, b, scale, history of the mean epoch loss)."""
+    returns (w, b, scale, history of the mean epoch loss).  Fine-tuning passes
+    the previous params and scale."""
     x, y = pairs(runs)
-    scale = scales_of(x, y)
+    if scale is None:
+        scale = scales_of(x, y)
     x, y = normalise(x, y, scale)
     n, H, W = x.shape[:3]
     net = ConvNet(batch, H, W)
@@ -186,8 +198,8 @@ def train(runs: List[np.ndarray], epochs: int = 20, batch: int = 8, lr: float = 
     return w, b, scale, hist
 
 
-def save(path: str, w, b, scale):
-    np.savez(path, w=w, b=b, scale=scale)
+def save(path: str, w, b, scale, stages: int = 1):
+    np.savez(path, w=w, b=b, scale=scale, stages=np.int32(stages))
 
 
 def load(path: str):
@@ -195,13 +207,21 @@ def load(path: str):
     return d["w"], d["b"], d["scale"]
 
 
-def attach(sim: Simulation, w, b, scale) -> ConvNet:
-    """Inference network on the simulation's box; enables RC_WARM_LEARNED."""
+def trained_stages(path: str) -> int:
+    """Refinement stages the weights were trained for (1 when not stored)."""
+    d = np.load(path)
+    return int(d["stages"]) if "stages" in d.files else 1
+
+
+def attach(sim: Simulation, w, b, scale, stages: int = 1) -> ConvNet:
+    """Inference network on the simulation's box; enables RC_WARM_LEARNED with
+    `stages` refinement stages per solve."""
     L = sim.confs[0].Lx
     net = ConvNet(1, L, L)
     net.set_params(w, b)
     sc = np.ascontiguousarray(scale, dtype=np.float32)
     sim._check(sim.L.rc_attach_net(sim.h, net.h, _fp(sc)))
+    sim._check(sim.L.rc_set_learned_stages(sim.h, stages))
     sim.set_warm_start(_lib.RC_WARM_LEARNED)
     sim._net = net  # keep alive
     return net

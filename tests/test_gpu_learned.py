@@ -78,7 +78,7 @@ def test_learned_warm_start_end_to_end():
     a = Simulation([(conf, tight)])
     a.set_warm_start(_lib.RC_WARM_PREVIOUS)
     bsim = Simulation([(conf, tight)])
-    learned.attach(bsim, w, b, scale)
+    learned.attach(bsim, w, b, scale, stages=3)
     for _ in range(8):
         a.step(1)
         bsim.step(1)
@@ -92,3 +92,54 @@ def test_learned_warm_start_end_to_end():
     _, pb = bsim.download()
     assert np.abs(pa - pb).max() < 1e-9
     assert np.abs(a.rows(0) - bsim.rows(0)).max() < 1e-9
+
+
+def test_stage_records_and_limits():
+    # with the network attached, every stage's start is one training record
+    L = 64
+    w, b = learned.init_params(1)
+    scale = np.array([1.0, 1.0, 0.01], dtype=np.float32)
+    recs = learned.record_run(L, 4, 5, steps=3, burn_in=2, params=(w, b, scale), stages=3)
+    assert recs.shape == (2 * 3 * 3, L, L, 5) and np.isfinite(recs).all()
+    # the scaled residual sums of a start have unit-order RMS
+    rms = np.sqrt(np.mean(recs[..., 0:2] ** 2, axis=(1, 2)))
+    assert np.all(rms > 0.1) and np.all(rms < 10)
+    conf = confgen.lattice_conf(L, 4, 5)
+    sim = Simulation([(conf, dict(confgen.BASE_KEYS, friction_cell=1, seed=5))])
+    with pytest.raises(Exception):
+        learned.attach(sim, w, b, scale, stages=_lib_max_stages() + 1)
+
+
+def _lib_max_stages():
+    return 8
+
+
+def test_trained_weights_cut_cg_iterations():
+    # the committed network (trained on 256^2 / 400-cell runs, seeds 101..) on an
+    # unseen seed of the same workload: 8 stages per solve, CG to 1e-8
+    import os
+    path = os.path.join(os.path.dirname(learned.__file__), "weights", "warmstart_wet256.npz")
+    w, b, scale = learned.load(path)
+    stages = learned.trained_stages(path)
+    assert stages == 8
+    conf = confgen.lattice_conf(256, 20, 77)
+    keys = dict(confgen.BASE_KEYS, friction_cell=1, CGtolerance=1e-8, seed=77)
+    it = {}
+    for mode in ("previous", "learned"):
+        sim = Simulation([(conf, keys)])
+        g = sim.gaussians(8)
+        if mode == "learned":
+            learned.attach(sim, w, b, scale, stages=stages)
+        else:
+            sim.set_warm_start(_lib.RC_WARM_PREVIOUS)
+        sim.step(3, g[:3])
+        s0 = sim.cg_stats()
+        sim.step(5, g[3:])
+        s1 = sim.cg_stats()
+        it[mode] = (s1.iters_total - s0.iters_total) / (s1.solves - s0.solves)
+        assert max(s1.last_relres) < 1e-8
+        sim.close()
+    print("CG iterations per solve: previous", it["previous"], "learned", it["learned"])
+    # each network stage costs about 2.3 CG iterations of device time; 8 stages
+    # must leave fewer than previous - 8 * 2.3 iterations to pay off twice over
+    assert it["learned"] < 0.25 * it["previous"]
