@@ -100,8 +100,9 @@ typedef struct rc_cg_stats_t {
 	double total_solve_ms;
 	long long sweeps_total;          /* launches of the iteration kernels that did work:
 	                                    sum over solves of max(iters_x, iters_y) */
-	long long learned_used;          /* solves whose last network stage moved the start */
-	long long learned_tried;         /* solves that ran the network stages */
+	long long learned_used;          /* solves that kept the network's start */
+	long long learned_tried;         /* solves that ran the network stages (the rest fell
+	                                    back to the previous solution in some component) */
 } rc_cg_stats_t;
 
 #define RC_WARM_REFERENCE 0   /* previous solution, cold restart every 11th solve (reference rule) */
@@ -168,7 +169,9 @@ void rc_destroy(rc_handle *h);
  * from e_i = (r_i + eta N4(k_i)) / D(k_i); x moves along e by the A-norm
  * optimal step (e.r)/(e.A e), which never raises the A-norm error.  A solve
  * starts from the previous solution and applies the stage rc_set_learned_stages
- * times; the reference's CG then runs to its tolerance from the result.
+ * times; a component whose residual ends larger than the previous solution's
+ * falls back to the previous solution.  The reference's CG then runs to its
+ * tolerance from the result.
  * Tensors are host float arrays, pixel-major: x [N][H][W][3], y [N][H][W][2].
  * Weights: w [6][9][32][32] (layer, tap = 3*(dy+1)+(dx+1), out ch, in ch),
  * b [6][32]; entries of absent channels (inputs >= 3 of layer 0, outputs >= 2

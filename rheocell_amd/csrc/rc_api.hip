@@ -93,6 +93,7 @@ struct rc_handle {
 	float scale[3] = { 1.f, 1.f, 1.f };
 	int stages = 1;                 // network refinement stages per solve
 	bool pending_learned_stat = false;
+	double2 *d_xprev = nullptr, *d_rprev = nullptr;   // previous-solution start and residual (fallback)
 	float *d_snap = nullptr;
 	float *d_rec = nullptr;         // training records: [cap][L*L][5]
 	int rec_cap = 0, rec_n = 0;
@@ -315,7 +316,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_count);
 	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
-	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xrec); dfree(h->d_rnrec);
+	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xprev); dfree(h->d_rprev); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
 	for(auto e : h->pool) (void) hipEventDestroy(e);
@@ -417,6 +418,13 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			HIPCHK(h, dalloc(&h->d_p, rows));
 		}
 		h->rows = rows;
+		if(h->d_xprev) {
+			dfree(h->d_xprev);
+			dfree(h->d_rprev);
+			h->d_xprev = h->d_rprev = nullptr;
+			HIPCHK(h, dalloc(&h->d_xprev, rows));
+			HIPCHK(h, dalloc(&h->d_rprev, rows));
+		}
 		if(h->d_xrec) {
 			dfree(h->d_xrec);
 			h->d_xrec = nullptr;
@@ -651,6 +659,12 @@ static int cg_solve(rc_handle *h, bool cold) {
 		}
 		if(learned) {
 			TimeScope t(h, RC_T_CG_LEARNED);
+			// keep the previous-solution start for the fallback
+			HIPCHK(h, hipMemcpyAsync(h->d_xprev, h->d_v, sizeof(double2) * h->rows, hipMemcpyDeviceToDevice, h->stream));
+			HIPCHK(h, hipMemcpyAsync(h->d_rprev, h->d_r, sizeof(double2) * h->rows, hipMemcpyDeviceToDevice, h->stream));
+			HIPCHK(h, hipMemcpyAsync(reinterpret_cast<char *>(h->d_cg) + offsetof(CGState, rn2_start),
+					reinterpret_cast<char *>(h->d_cg) + offsetof(CGState, rn2), 2 * sizeof(double), hipMemcpyDeviceToDevice,
+					h->stream));
 			for(int st = 0; st < h->stages; ++st) {
 				int rc = learned_stage(h, c);
 				if(rc) return rc;
@@ -659,6 +673,12 @@ static int cg_solve(rc_handle *h, bool cold) {
 					if(rc) return rc;
 				}
 			}
+			// back to the previous solution in a component whose residual got larger
+			CGArgs f = c;
+			f.xprev = h->d_xprev;
+			f.rprev = h->d_rprev;
+			HIPCHK(h, launch_cg_row(CG_ROW_REVERT, f, h->stream));
+			HIPCHK(h, launch_cg_reduce(5, f, h->ncells, h->stream));
 			h->cgs.learned_tried += 1;
 			h->pending_learned_stat = true;
 		}
@@ -692,7 +712,7 @@ static int cg_solve(rc_handle *h, bool cold) {
 	(void) hipEventElapsedTime(&ms, e0, e1);
 	const CGState &s = *h->h_cg;
 	if(h->pending_learned_stat) {
-		if(s.alpha0[0] != 0 || s.alpha0[1] != 0) h->cgs.learned_used += 1;
+		if(!s.reverted[0] && !s.reverted[1]) h->cgs.learned_used += 1;
 		h->pending_learned_stat = false;
 	}
 	if(h->rec_open) {
@@ -1169,6 +1189,10 @@ extern "C" int rc_attach_net(rc_handle *h, rc_net *n, const float scale[3]) {
 	net_dims(n, &N, &H, &W);
 	if(N != 1 || H != h->boxes[0].L || W != h->boxes[0].L)
 		return fail(h, RC_ERR_ARG, "network shape %dx%dx%d does not match the %d^2 box", N, H, W, h->boxes[0].L);
+	if(!h->d_xprev) {
+		HIPCHK(h, dalloc(&h->d_xprev, std::max<long long>(h->rows, 1)));
+		HIPCHK(h, dalloc(&h->d_rprev, std::max<long long>(h->rows, 1)));
+	}
 	h->net = n;
 	for(int k = 0; k < 3; ++k) h->scale[k] = scale ? scale[k] : 1.f;
 	return RC_OK;

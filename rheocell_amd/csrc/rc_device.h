@@ -60,7 +60,9 @@ struct DevStatus {
 struct CGState {
 	double bb[2], thr[2], absNew[2], alpha[2], beta[2], rn2[2];
 	double alpha0[2];   // step along the correction of a learned refinement stage
+	double rn2_start[2];  // |r_c|^2 of the previous-solution start (learned fallback test)
 	int done[2], iters[2], zero_rhs[2], max_iter, active;
+	int reverted[2];    // learned start dropped for the previous solution
 };
 
 }  // namespace rc

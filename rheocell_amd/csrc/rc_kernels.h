@@ -49,7 +49,7 @@ struct GatherArgs {
 };
 
 enum CGGatherMode { CG_GATHER_X = 0, CG_GATHER_P_FIRST = 1, CG_GATHER_P_UPDATE = 2, CG_GATHER_FEAT = 3, CG_GATHER_E = 4 };
-enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_UPDATE = 2, CG_ROW_STEP = 5 };
+enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_UPDATE = 2, CG_ROW_STEP = 5, CG_ROW_REVERT = 6 };
 
 struct CGArgs {
 	const DevBox *boxes;
@@ -74,6 +74,7 @@ struct CGArgs {
 	float sf, sp2, sv;
 	double rows;
 	int nbox_blk;                // workgroups of a box-site pass (partials of k_cg_box<1>)
+	const double2 *xprev, *rprev;  // previous-solution start and its residual (learned fallback)
 	double *part;                // per-cell partial sums, 6 per cell
 	double *part_box;            // per-workgroup partials of the box pass, 2 each
 	CGState *cg;
@@ -91,7 +92,8 @@ hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s
 hipError_t launch_cg_box(int mode, const CGArgs &a, int nsites, hipStream_t s);
 hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s);
 // what: 0 after the residual pass, 1 after p.Ap, 2 after the x,r update,
-// 3 step of a learned stage, 4 after a learned stage's x,r update
+// 3 step of a learned stage, 4 after a learned stage's x,r update,
+// 5 after the learned fallback pass
 hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s);
 // snapshot channels: chan 0 -> sp into [k][0..1], chan 2 -> phi2 into [k][2],
 // chan 3 -> sp into [k][3..4], chan 5 -> [k][3..4] -= sp; sp is divided by

@@ -772,6 +772,16 @@ __global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
 		return;
 	}
 	if(threadIdx.x != 0) return;
+	if(WHAT == 5) {
+		// after the fallback pass: its partials are valid only if a component reverted
+		const int r0 = cg->rn2[0] > cg->rn2_start[0], r1 = cg->rn2[1] > cg->rn2_start[1];
+		cg->reverted[0] = r0;
+		cg->reverted[1] = r1;
+		if(!r0 && !r1) return;
+		const double t6[6] = { cg->bb[0], cg->bb[1], tot[0], tot[1], tot[2], tot[3] };
+		cg_after_resid(cg, t6, a.tol, a.max_iter);
+		return;
+	}
 	if(WHAT == 0) cg_after_resid(cg, tot, a.tol, a.max_iter);
 	else if(WHAT == 1) cg_after_pap(cg, tot);
 	else if(WHAT == 4) {
@@ -788,6 +798,7 @@ hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s) 
 	case 1: RC_LAUNCH(k_cg_reduce<1>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	case 3: RC_LAUNCH(k_cg_reduce<3>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	case 4: RC_LAUNCH(k_cg_reduce<4>, dim3(1), dim3(1024), 0, s, a, nblk); break;
+	case 5: RC_LAUNCH(k_cg_reduce<5>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	default: RC_LAUNCH(k_cg_reduce<2>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	}
 	return hipGetLastError();
@@ -977,6 +988,10 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 	if(a.st->flags) return;
 	CGState *cg = a.cg;
 	if(MODE == CG_ROW_UPDATE && !cg->active) return;
+	// fallback: a component whose learned start has a larger residual than the previous solution
+	const bool rev0 = MODE == CG_ROW_REVERT && cg->rn2[0] > cg->rn2_start[0];
+	const bool rev1 = MODE == CG_ROW_REVERT && cg->rn2[1] > cg->rn2_start[1];
+	if(MODE == CG_ROW_REVERT && !rev0 && !rev1) return;
 	__shared__ double s_red[4 * 6];
 	const int ci = blockIdx.x, tid = threadIdx.x;
 	const DevCell C = a.cells[ci];
@@ -1005,6 +1020,18 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 		const double D = a.diag[k];
 		const double2 n4 = a.n4[k];
 		const int row = C.start + q;
+		if(MODE == CG_ROW_REVERT) {
+			// back to the previous solution (and its residual) in the components that got worse
+			const double inv = 1. / D;
+			double2 xx = a.x[row], rn = a.r[row];
+			if(rev0) { xx.x = a.xprev[row].x; rn.x = a.rprev[row].x; }
+			if(rev1) { xx.y = a.xprev[row].y; rn.y = a.rprev[row].y; }
+			a.x[row] = xx;
+			a.r[row] = rn;
+			red[0] += rn.x * rn.x; red[1] += rn.y * rn.y;
+			red[2] += rn.x * (inv * rn.x); red[3] += rn.y * (inv * rn.y);
+			continue;
+		}
 		if(MODE == CG_ROW_STEP) {
 			// x += alpha0 e, r -= alpha0 A e (A e = D e - eta N4 of the box sums of e); r.r, r.z
 			const double inv = 1. / D;
@@ -1057,6 +1084,7 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 hipError_t launch_cg_row(int mode, const CGArgs &a, hipStream_t s) {
 	if(mode == CG_ROW_RESID) RC_LAUNCH(k_cg_row<CG_ROW_RESID>, dim3(a.ncells), dim3(256), 0, s, a);
 	else if(mode == CG_ROW_STEP) RC_LAUNCH(k_cg_row<CG_ROW_STEP>, dim3(a.ncells), dim3(256), 0, s, a);
+	else if(mode == CG_ROW_REVERT) RC_LAUNCH(k_cg_row<CG_ROW_REVERT>, dim3(a.ncells), dim3(256), 0, s, a);
 	else RC_LAUNCH(k_cg_row<CG_ROW_UPDATE>, dim3(a.ncells), dim3(256), 0, s, a);
 	return hipGetLastError();
 }

@@ -143,3 +143,27 @@ def test_trained_weights_cut_cg_iterations():
     # each network stage costs about 2.3 CG iterations of device time; 8 stages
     # must leave fewer than previous - 8 * 2.3 iterations to pay off twice over
     assert it["learned"] < 0.25 * it["previous"]
+
+
+def test_untrained_network_falls_back_safely():
+    # a random network: the A-norm steps are small or useless, the fallback keeps
+    # the previous solution where the residual got larger, and CG still converges
+    # to the same solution as without the network
+    L = 64
+    conf = confgen.lattice_conf(L, 4, 21)
+    tight = dict(confgen.BASE_KEYS, friction_cell=1, CGtolerance=1e-12, seed=21)
+    a = Simulation([(conf, tight)])
+    a.set_warm_start(_lib.RC_WARM_PREVIOUS)
+    bsim = Simulation([(conf, tight)])
+    w, b = learned.init_params(3)
+    learned.attach(bsim, w, b, np.array([1.0, 1.0, 1.0], dtype=np.float32), stages=2)
+    for _ in range(4):
+        a.step(1)
+        bsim.step(1)
+        assert max(bsim.cg_stats().last_relres) < 1e-12
+    st = bsim.cg_stats()
+    print("kept", st.learned_used, "of", st.learned_tried)
+    assert st.learned_tried == 8 and 0 <= st.learned_used <= 8
+    _, pa = a.download()
+    _, pb = bsim.download()
+    assert np.abs(pa - pb).max() < 1e-9
