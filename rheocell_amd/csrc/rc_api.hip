@@ -182,6 +182,15 @@ static void collect_timing(rc_handle *h) {
 	h->pool_used = 0;
 }
 
+// 1/x when x is a power of two with a normal reciprocal (then y / x == y * (1/x)
+// exactly), else 0
+static double exact_rcp(double x) {
+	int e = 0;
+	const double m = std::frexp(x, &e);
+	if(!(x > 0) || m != 0.5 || e < -1020 || e > 1020) return 0.;
+	return 1. / x;
+}
+
 static DevParams make_params(const rc_model_params &m, bool active) {
 	DevParams p{};
 	p.gamma = m.gamma;
@@ -196,6 +205,7 @@ static DevParams make_params(const rc_model_params &m, bool active) {
 	p.dt = m.dt;
 	p.sqrt_dt = std::sqrt(m.dt);
 	p.anchoring = m.anchoring;
+	p.lambda_rcp = exact_rcp(m.lambda);
 	if(active) {
 		p.zetaQ_self = m.zetaQ_self;
 		p.zetaQ_inter = m.zetaQ_inter;
@@ -212,6 +222,7 @@ static DevParams make_params(const rc_model_params &m, bool active) {
 		p.friction_cell = 0.;
 		p.beta = 0.;
 	}
+	p.friction_rcp = exact_rcp(p.friction);
 	return p;
 }
 
@@ -346,7 +357,7 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 	std::vector<double> trig;
 	std::map<int, int> trig_off;
 	long long rows = 0;
-	int maxS = 0, maxW = 0;
+	int maxS = 0, maxW = 0, maxT = 0, npt = 0;
 	for(int b = 0; b < h->nboxes; ++b) {
 		const DevBox &B = h->dboxes[b];
 		for(int i = B.cell_begin; i < B.cell_end; ++i) {
@@ -398,6 +409,9 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			rows += d.S;
 			maxS = std::max(maxS, d.S);
 			maxW = std::max(maxW, (d.LX + 2) * (d.LY + 2));
+			maxT = std::max(maxT, 2 * d.LY);
+			const int n = stage_npt(d.LX, d.LY);
+			npt = n < 0 || npt < 0 ? -1 : std::max(npt, n);
 		}
 	}
 	if(rows >= (1ll << 31)) return fail(h, RC_ERR_ARG, "too many patch sites (%lld)", rows);
@@ -449,9 +463,11 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 	HIPCHK(h, hipMemsetAsync(h->d_st, 0, sizeof(DevStatus), h->stream));
 	h->max_S = maxS;
 	h->max_W = maxW;
-	h->lds_stage = sizeof(double) * (size_t) (maxS + maxW + (h->jq ? maxS : 0));
-	h->npt = stage_npt(maxS);
-	if(h->npt < 0 || std::max(maxS, maxW) >= (1 << 16)) return fail(h, RC_ERR_ARG, "patch of %d sites is larger than the 64x64 supported", maxS);
+	// patch, window / mu, y trig tables, Repulsion
+	h->lds_stage = sizeof(double) * (size_t) (maxS + maxW + maxT + (h->jq ? maxS : 0));
+	h->npt = npt;
+	if(h->npt < 0 || std::max(maxS, maxW) >= (1 << 16))
+		return fail(h, RC_ERR_ARG, "a patch of %d sites does not fit the stage kernel (at most 16 rows per thread strip)", maxS);
 	h->lds_cg = sizeof(double2) * (size_t) maxW;
 	if(h->lds_stage > 160 * 1024) return fail(h, RC_ERR_ARG, "patch of %d sites needs %zu B of LDS", maxS, h->lds_stage);
 	HIPCHK(h, stage_set_lds_limit(std::max<size_t>(h->lds_stage, 64 * 1024)));
@@ -484,6 +500,9 @@ static StageArgs stage_args(rc_handle *h) {
 	a.dphi_old = h->d_dphi;
 	a.geo = h->d_geo;
 	a.npt = h->npt;
+	a.rcp = 1;
+	for(const DevParams &p : h->dparams)
+		if(p.lambda_rcp == 0 || p.friction_rcp == 0) a.rcp = 0;
 	a.st = h->d_st;
 	return a;
 }

@@ -30,6 +30,7 @@ constexpr unsigned ST_TILE = 4u;
 struct DevParams {
 	double gamma, lambda, omega, mu, kappa, a0, p0, beta, passive_alpha;
 	double zetaQ_self, zetaQ_inter, J_Q, friction, friction_cell, J0, dt, sqrt_dt;
+	double lambda_rcp, friction_rcp;   // exact 1/lambda, 1/friction when a power of two, else 0
 	int anchoring, pad;
 };
 

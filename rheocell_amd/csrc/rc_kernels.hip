@@ -232,34 +232,26 @@ hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // k_stage<MODE>: force evaluation (+ field update) of one cell
 // ---------------------------------------------------------------------------
-struct SiteIdx {
-	int iE, iW, iN, iS, lx, ly, w;
-};
-
-__device__ inline SiteIdx site_index(int q, int LX, int LY, int offx, int offy, int WX) {
-	SiteIdx r;
-	const int qy = q / LX, qx = q - qy * LX;
-	const int xe = (qx + 1 == LX) ? 0 : qx + 1;
-	const int xw = (qx == 0) ? LX - 1 : qx - 1;
-	const int yn = (qy + 1 == LY) ? 0 : qy + 1;
-	const int ys = (qy == 0) ? LY - 1 : qy - 1;
-	r.iE = qy * LX + xe;
-	r.iW = qy * LX + xw;
-	r.iN = yn * LX + qx;
-	r.iS = ys * LX + qx;
-	int lx = qx + offx;
-	if(lx >= LX) lx -= LX;
-	int ly = qy + offy;
-	if(ly >= LY) ly -= LY;
-	r.lx = lx;
-	r.ly = ly;
-	r.w = (ly + 1) * WX + (lx + 1);
-	return r;
+// Column strips: a cell's LX x LY patch (ring-buffer coordinates, storage
+// q = qy * LX + qx) is covered by nb = CELL_THREADS / LX row blocks of
+// R = ceil(LY / nb) rows; thread t takes column qx = t % LX of block t / LX.
+// Column-dependent quantities (E/W neighbour columns, the box-window column,
+// the x trig table entries) are then per-thread constants, the N/S neighbours
+// of a run of rows come from registers, and no index is divided per site.
+__device__ __host__ inline int strip_rows(int LX, int LY) {
+	const int nb = CELL_THREADS / LX;
+	return nb > 0 ? (LY + nb - 1) / nb : 1 << 20;
 }
 
-// NPT = sites per thread (compile time) so that the per-site registers and the
-// early loads of the corrector rows are fully unrolled.
-template <int MODE, int NPT>
+// x / y; with RCP (y a power of two, rcp its exact reciprocal) the product
+// x * rcp, which rounds the same exact quotient
+template <bool RCP>
+__device__ inline double div_by(double x, double y, double rcp) { return RCP ? x * rcp : x / y; }
+
+// NPT >= the strip rows of every patch (compile time), so the per-site
+// registers are fully unrolled.  RCP: lambda and the friction of every
+// parameter set are powers of two (divisions become exact multiplications).
+template <int MODE, int NPT, bool RCP>
 __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	if(a.st->flags) return;
 	extern __shared__ double smem[];
@@ -271,44 +263,75 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	const DevBox B = a.boxes[C.box];
 	const DevParams P = a.params[2 * C.box + a.cf_set];
 	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L, WX = LX + 2, WY = LY + 2;
-	double *s_phi = smem;
-	double *s_win = smem + S;      // box window of sum phi^2, then mu (aliased)
+	double *s_phi = smem;          // patch of phi (storage order)
+	double *s_win = smem + S;      // box window of sum phi^2, then mu in storage order (aliased)
 	double *s_mu = s_win;
-	double *s_rep = s_win + WX * WY;
+	double *s_trig = s_win + WX * WY;   // cos y, sin y (LY each)
+	double *s_rep = s_trig + 2 * LY;
 	const bool do_rep = a.jq && MODE != MODE_FS && !a.pure;
 	const bool fs = MODE != MODE_CF;
 
-	// early loads of the rows the field update reads
-	double pold[NPT], dold[NPT];
-	double2 vv[NPT];
-#pragma unroll
-	for(int j = 0; j < NPT; ++j) {
-		const int q = tid + j * CELL_THREADS;
-		pold[j] = dold[j] = 0.;
-		vv[j] = make_double2(0., 0.);
-		if(q < S) {
-			if(fs && !a.store) {
-				pold[j] = a.phi_old[C.start + q];
-				dold[j] = a.dphi_old[C.start + q];
-			}
-			if(MODE == MODE_FS) vv[j] = a.vrows[C.start + q];
-		}
-	}
-	// stage the patch and the (LX+2)x(LY+2) box window of sum phi^2
+	// this thread's strip: column qx, rows qy0 .. qy0 + nr - 1
+	const int R = strip_rows(LX, LY);
+	const int qx = tid % LX, blk = tid / LX;
+	const int qy0 = blk * R;
+	const int nr = max(0, min(R, LY - qy0));
+	const int xe = qx + 1 == LX ? 0 : qx + 1, xw = qx == 0 ? LX - 1 : qx - 1;
+	int lx = qx + C.offx;                 // box-window column (ring offset applied)
+	if(lx >= LX) lx -= LX;
+	int ly0 = qy0 + C.offy;
+	while(ly0 >= LY) ly0 -= LY;
+	const int row0 = C.start + qy0 * LX + qx;
+
+	// stage the patch, the (LX+2)x(LY+2) box window of sum phi^2 and the y trig tables
 	const double *phic = a.phi_c + C.start;
 	for(int q = tid; q < S; q += CELL_THREADS) s_phi[q] = phic[q];
 	const int cx = C.corner % L, cy = C.corner / L;
-	for(int w = tid; w < WX * WY; w += CELL_THREADS) {
-		const int wy = w / WX, wx = w - wy * WX;
-		int bx = cx + wx - 1;
-		if(bx < 0) bx += L;
-		else if(bx >= L) bx -= L;
-		int by = cy + wy - 1;
-		if(by < 0) by += L;
-		else if(by >= L) by -= L;
-		s_win[w] = a.phi2[B.box_off + by * L + bx];
+	{
+		int wy = tid / WX, wx = tid - wy * WX;
+		const int dy = CELL_THREADS / WX, dx = CELL_THREADS - dy * WX;
+		for(int w = tid; w < WX * WY; w += CELL_THREADS) {
+			int bx = cx + wx - 1;
+			if(bx < 0) bx += L;
+			else if(bx >= L) bx -= L;
+			int by = cy + wy - 1;
+			if(by < 0) by += L;
+			else if(by >= L) by -= L;
+			s_win[w] = a.phi2[B.box_off + by * L + bx];
+			wx += dx;
+			wy += dy;
+			if(wx >= WX) {
+				wx -= WX;
+				++wy;
+			}
+		}
+	}
+	for(int t = tid; t < 2 * LY; t += CELL_THREADS) s_trig[t] = a.trig[C.trig_y + t];
+	double pold[NPT], dold[NPT];
+	double2 vv[NPT];
+#pragma unroll
+	for(int k = 0; k < NPT; ++k) {
+		pold[k] = dold[k] = 0.;
+		vv[k] = make_double2(0., 0.);
+		if(k < nr) {
+			if(fs && !a.store) {
+				pold[k] = a.phi_old[row0 + k * LX];
+				dold[k] = a.dphi_old[row0 + k * LX];
+			}
+			if(MODE == MODE_FS) vv[k] = a.vrows[row0 + k * LX];
+		}
 	}
 	__syncthreads();
+
+	// the strip's column of phi with one row above and below (periodic)
+	double col[NPT + 2];
+#pragma unroll
+	for(int k = 0; k < NPT + 2; ++k) {
+		int qy = qy0 + k - 1;
+		if(qy < 0) qy += LY;
+		else if(qy >= LY) qy -= LY;
+		col[k] = nr > 0 && k <= nr + 1 ? s_phi[qy * LX + qx] : 0.;
+	}
 
 	// phase 1: perimeter = sum |grad phi|^2 (computeGlobalSums, WetPolarModel.cpp:243-246),
 	// needed before mu only by the perimeter term (beta != 0)
@@ -316,14 +339,12 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	if(P.beta != 0) {
 		double per[1] = { 0.0 };
 #pragma unroll
-		for(int j = 0; j < NPT; ++j) {
-			const int q = tid + j * CELL_THREADS;
-			if(q < S) {
-				const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
-				const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
-				const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
-				per[0] += dx * dx + dy * dy;
-			}
+		for(int k = 0; k < NPT; ++k) {
+			if(k >= nr) break;
+			const int o = (qy0 + k) * LX;
+			const double dx = .5 * (s_phi[o + xe] - s_phi[o + xw]);
+			const double dy = .5 * (col[k + 2] - col[k]);
+			per[0] += dx * dx + dy * dy;
 		}
 		block_sum<1>(per, s_red);
 		if(tid == 0) s_bc = per[0];
@@ -333,45 +354,61 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 
 	// phase 2: chemical potential mu = dF/dphi (f_interaction, WetPolarModel.cpp:262-301).
 	// With omega == 0 (beta == 0) the adhesion (perimeter) term is exactly +-0 and is not evaluated.
+	// Loop-invariant factors are formed once, in the reference's evaluation order.
+	const double Acoef = -4 * (P.mu / P.a0) * (1 - C.area / P.a0);
+	const double Rcoef = 4 * (P.kappa / P.lambda);
+	const double lam2 = 2 * P.lambda;
+	const double Pcoef = P.beta != 0 ? 4 * (P.beta / P.p0) * (1 - perimeter / P.p0) : 0.;
 	double mu[NPT];
 	double per2[1] = { 0.0 };
+	{
+		int w = (ly0 + 1) * WX + (lx + 1);
+		int ly = ly0;
 #pragma unroll
-	for(int j = 0; j < NPT; ++j) {
-		const int q = tid + j * CELL_THREADS;
-		mu[j] = 0.;
-		if(q < S) {
-			const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
-			const double ph = s_phi[q];
-			const double lap = s_phi[I.iN] + s_phi[I.iS] + s_phi[I.iE] + s_phi[I.iW] - 4 * ph;
-			const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
-			const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
-			if(MODE == MODE_CF && P.beta == 0) per2[0] += dx * dx + dy * dy;
-			const double p2 = s_win[I.w];
-			const double CH = P.gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / P.lambda - 2 * P.lambda * lap);
-			const double A = -4 * (P.mu / P.a0) * (1 - C.area / P.a0) * ph;
-			const double Rep = 4 * (P.kappa / P.lambda) * ph * (p2 - ph * ph);
-			double Adh = 0.;
-			if(P.omega != 0) {
-				const double laplacianSquare = s_win[I.w + 1] + s_win[I.w + WX] + s_win[I.w - 1] + s_win[I.w - WX] - 4. * p2;
-				const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
-				const double dd = laplacianSquare - lsquare;
-				const double suppress = dd / sqrt(1 + dd * dd);
-				Adh = -4 * P.lambda * P.omega * suppress * ph;
+		for(int k = 0; k < NPT; ++k) {
+			mu[k] = 0.;
+			if(k < nr) {
+				const int o = (qy0 + k) * LX;
+				const double ph = col[k + 1];
+				const double pE = s_phi[o + xe], pW = s_phi[o + xw], pN = col[k + 2], pS = col[k];
+				const double lap = pN + pS + pE + pW - 4 * ph;
+				const double dx = .5 * (pE - pW);
+				const double dy = .5 * (pN - pS);
+				if(MODE == MODE_CF && P.beta == 0) per2[0] += dx * dx + dy * dy;
+				const double p2 = s_win[w];
+				const double CH = P.gamma * (div_by<RCP>(8 * ph * (1 - ph) * (1 - 2 * ph), P.lambda, P.lambda_rcp) - lam2 * lap);
+				const double A = Acoef * ph;
+				const double Rep = Rcoef * ph * (p2 - ph * ph);
+				double Adh = 0.;
+				if(P.omega != 0) {
+					const double laplacianSquare = s_win[w + 1] + s_win[w + WX] + s_win[w - 1] + s_win[w - WX] - 4. * p2;
+					const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
+					const double dd = laplacianSquare - lsquare;
+					const double suppress = dd / sqrt(1 + dd * dd);
+					Adh = -4 * P.lambda * P.omega * suppress * ph;
+				}
+				double Pp = 0.;
+				if(P.beta != 0) Pp = Pcoef * lap;
+				const double V = CH + A + Rep + Adh + Pp;
+				mu[k] = V;
+				const int row = row0 + k * LX;
+				if(MODE == MODE_CF && a.out_mu) {
+					a.out_mu[row] = V;
+					a.out_dx[row] = dx;
+					a.out_dy[row] = dy;
+					a.out_lap[row] = lap;
+				}
+				if(do_rep) {
+					const double rn = a.rep[row] + Rep;
+					a.rep[row] = rn;
+					s_rep[o + qx] = rn;
+				}
 			}
-			double Pp = 0.;
-			if(P.beta != 0) Pp = 4 * (P.beta / P.p0) * (1 - perimeter / P.p0) * lap;
-			const double V = CH + A + Rep + Adh + Pp;
-			mu[j] = V;
-			if(MODE == MODE_CF && a.out_mu) {
-				a.out_mu[C.start + q] = V;
-				a.out_dx[C.start + q] = dx;
-				a.out_dy[C.start + q] = dy;
-				a.out_lap[C.start + q] = lap;
-			}
-			if(do_rep) {
-				const double rn = a.rep[C.start + q] + Rep;
-				a.rep[C.start + q] = rn;
-				s_rep[q] = rn;
+			// next row of the window (ring order wraps at LY)
+			w += WX;
+			if(++ly == LY) {
+				ly = 0;
+				w -= LY * WX;
 			}
 		}
 	}
@@ -383,80 +420,101 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	}
 	__syncthreads();  // every read of the sum phi^2 window is done: reuse it for mu
 #pragma unroll
-	for(int j = 0; j < NPT; ++j) {
-		const int q = tid + j * CELL_THREADS;
-		if(q < S) s_mu[q] = mu[j];
-	}
+	for(int k = 0; k < NPT; ++k)
+		if(k < nr) s_mu[(qy0 + k) * LX + qx] = mu[k];
 	__syncthreads();
 
-	// phase 3: forces (calc_internal_forces) and, unless MODE_CF, the Heun update
+	// phase 3: forces (calc_internal_forces) and, unless MODE_CF, the Heun update.
+	// The per-cell sums (area, sumF, CoM trig sums, shape) are accumulated with
+	// FMA and the x trig sums as cos/sin(x) times the strip's sum of phi: their
+	// order already differs from the reference's sequential sum.
 	const DevParams Q = a.params[2 * C.box + a.fs_set];
 	double red[NRED];
 #pragma unroll
 	for(int j = 0; j < NRED; ++j) red[j] = 0.0;
-	int xmin = LX, ymin = LY;
-	const double *cosx = a.trig + C.trig_x, *sinx = cosx + LX;
-	const double *cosy = a.trig + C.trig_y, *siny = cosy + LY;
+	int ymin = LY;
+	bool edge = false;
+	const double *cosy = s_trig, *siny = s_trig + LY;
 	const bool dry_cf = P.friction_cell == 0;
+	double muN = 0., muS = 0.;   // mu one row above / below the strip
+	if(MODE != MODE_FS && nr > 0) {
+		int qa = qy0 + nr;
+		if(qa >= LY) qa -= LY;
+		int qb = qy0 - 1;
+		if(qb < 0) qb += LY;
+		muN = s_mu[qa * LX + qx];
+		muS = s_mu[qb * LX + qx];
+	}
+	{
+		int ly = ly0;
 #pragma unroll
-	for(int j = 0; j < NPT; ++j) {
-		const int q = tid + j * CELL_THREADS;
-		if(q >= S) continue;
-		const SiteIdx I = site_index(q, LX, LY, C.offx, C.offy, WX);
-		const int row = C.start + q;
-		const double ph = s_phi[q];
-		const double dx = .5 * (s_phi[I.iE] - s_phi[I.iW]);
-		const double dy = .5 * (s_phi[I.iN] - s_phi[I.iS]);
-		double vx, vy;
-		if(MODE == MODE_FS) {
-			vx = vv[j].x;
-			vy = vv[j].y;
-		}
-		else {
-			const double f_passive_x = (-1) * 0.5 * (s_mu[I.iE] - s_mu[I.iW]);
-			const double f_passive_y = (-1) * 0.5 * (s_mu[I.iN] - s_mu[I.iS]);
-			if(do_rep) {
-				red[8] += (-1) * 0.5 * (s_rep[I.iE] - s_rep[I.iW]) * ph;
-				red[9] += (-1) * 0.5 * (s_rep[I.iN] - s_rep[I.iS]) * ph;
+		for(int k = 0; k < NPT; ++k) {
+			if(k >= nr) break;
+			const int o = (qy0 + k) * LX;
+			const int row = row0 + k * LX;
+			const double ph = col[k + 1];
+			const double dx = .5 * (s_phi[o + xe] - s_phi[o + xw]);
+			const double dy = .5 * (col[k + 2] - col[k]);
+			double vx, vy;
+			if(MODE == MODE_FS) {
+				vx = vv[k].x;
+				vy = vv[k].y;
 			}
-			double Theta = (-dx * C.Q00) < 0 ? 0.0 : 1.0;
-			const double fQx = P.lambda * (-dx * C.Q00) * Theta * C.Q00;
-			Theta = (-dy * C.Q01) < 0 ? 0.0 : 1.0;
-			const double fQy = P.lambda * (-dy * C.Q01) * Theta * C.Q01;
-			const double fx = f_passive_x * P.passive_alpha + fQx * P.zetaQ_self;
-			const double fy = f_passive_y * P.passive_alpha + fQy * P.zetaQ_self;
-			vx = fx / P.friction;
-			vy = fy / P.friction;
-			if(MODE == MODE_CF) {
-				if(dry_cf) a.vrows[row] = make_double2(vx, vy);
-				else a.frows[row] = make_double2(fx, fy);
-				continue;
+			else {
+				const double mN = k + 1 < nr ? mu[k + 1] : muN;
+				const double mS = k > 0 ? mu[k - 1] : muS;
+				const double f_passive_x = (-1) * 0.5 * (s_mu[o + xe] - s_mu[o + xw]);
+				const double f_passive_y = (-1) * 0.5 * (mN - mS);
+				if(do_rep) {
+					int on = o + LX, os = o - LX;
+					if(qy0 + k + 1 == LY) on -= S;
+					if(qy0 + k == 0) os += S;
+					red[8] += (-1) * 0.5 * (s_rep[o + xe] - s_rep[o + xw]) * ph;
+					red[9] += (-1) * 0.5 * (s_rep[on + qx] - s_rep[os + qx]) * ph;
+				}
+				double Theta = (-dx * C.Q00) < 0 ? 0.0 : 1.0;
+				const double fQx = P.lambda * (-dx * C.Q00) * Theta * C.Q00;
+				Theta = (-dy * C.Q01) < 0 ? 0.0 : 1.0;
+				const double fQy = P.lambda * (-dy * C.Q01) * Theta * C.Q01;
+				const double fx = f_passive_x * P.passive_alpha + fQx * P.zetaQ_self;
+				const double fy = f_passive_y * P.passive_alpha + fQy * P.zetaQ_self;
+				vx = div_by<RCP>(fx, P.friction, P.friction_rcp);
+				vy = div_by<RCP>(fy, P.friction, P.friction_rcp);
+				if(MODE == MODE_CF) {
+					if(dry_cf) a.vrows[row] = make_double2(vx, vy);
+					else a.frows[row] = make_double2(fx, fy);
+					continue;
+				}
+				if(a.write_v) a.vrows[row] = make_double2(vx, vy);
 			}
-			if(a.write_v) a.vrows[row] = make_double2(vx, vy);
+			// FD_CPUBackend::first_step body (FD_CPUBackend.cpp:66-101)
+			const double dphi = -Q.J0 * mu[k] - vx * dx - vy * dy;
+			double phn;
+			if(a.store) {
+				a.dphi_old[row] = dphi;
+				phn = ph + Q.dt * .5 * (dphi + dphi);
+			}
+			else {
+				phn = pold[k] + Q.dt * .5 * (dphi + dold[k]);
+			}
+			a.phi_out[row] = phn;
+			red[0] = fma(phn, phn, red[0]);
+			red[1] += phn;
+			red[4] = fma(phn, cosy[ly], red[4]);
+			red[5] = fma(phn, siny[ly], red[5]);
+			if(!a.store && phn < 0.09 && phn > 0.01) {
+				edge = true;
+				ymin = min(ymin, ly);
+			}
+			red[6] += -0.5 * (dx * dx - dy * dy);
+			red[7] += -dx * dy;
+			if(++ly == LY) ly = 0;
 		}
-		// FD_CPUBackend::first_step body (FD_CPUBackend.cpp:66-101)
-		const double dphi = -Q.J0 * mu[j] - vx * dx - vy * dy;
-		double phn;
-		if(a.store) {
-			a.dphi_old[row] = dphi;
-			phn = ph + Q.dt * .5 * (dphi + dphi);
-		}
-		else {
-			phn = pold[j] + Q.dt * .5 * (dphi + dold[j]);
-		}
-		a.phi_out[row] = phn;
-		red[0] += phn * phn;
-		red[1] += phn;
-		red[2] += phn * cosx[I.lx];
-		red[3] += phn * sinx[I.lx];
-		red[4] += phn * cosy[I.ly];
-		red[5] += phn * siny[I.ly];
-		if(!a.store && phn < 0.09 && phn > 0.01) {
-			xmin = min(xmin, I.lx);
-			ymin = min(ymin, I.ly);
-		}
-		red[6] += -0.5 * (dx * dx - dy * dy);
-		red[7] += -dx * dy;
+	}
+	int xmin = edge ? lx : LX;
+	if(nr > 0 && MODE != MODE_CF) {
+		red[2] = red[1] * a.trig[C.trig_x + lx];
+		red[3] = red[1] * a.trig[C.trig_x + LX + lx];
 	}
 
 	if(MODE == MODE_CF) {
@@ -590,13 +648,18 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	}
 }
 
+template <int MODE, bool RCP>
+static hipError_t set_lds_r(size_t bytes) {
+	hipError_t e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 10, RCP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
+	return hipFuncSetAttribute((const void *) k_stage<MODE, 16, RCP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes);
+}
 template <int MODE>
 static hipError_t set_lds(size_t bytes) {
-	hipError_t e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
-	return hipFuncSetAttribute((const void *) k_stage<MODE, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes);
+	hipError_t e = set_lds_r<MODE, false>(bytes);
+	return e != hipSuccess ? e : set_lds_r<MODE, true>(bytes);
 }
 
 hipError_t stage_set_lds_limit(size_t bytes) {
@@ -606,19 +669,24 @@ hipError_t stage_set_lds_limit(size_t bytes) {
 	return set_lds<MODE_FS>(bytes);
 }
 
-int stage_npt(int max_sites) {
-	const int n = (max_sites + CELL_THREADS - 1) / CELL_THREADS;
+int stage_npt(int LX, int LY) {
+	const int n = strip_rows(LX, LY);
 	return n <= 4 ? 4 : n <= 7 ? 7 : n <= 10 ? 10 : n <= 16 ? 16 : -1;
 }
 
+template <int MODE, bool RCP>
+static void launch_mode_r(const StageArgs &a, int ncells, size_t lds, hipStream_t s, int npt) {
+	switch(npt) {
+	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	case 10: RC_LAUNCH((k_stage<MODE, 10, RCP>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	default: RC_LAUNCH((k_stage<MODE, 16, RCP>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+	}
+}
 template <int MODE>
 static void launch_mode(const StageArgs &a, int ncells, size_t lds, hipStream_t s, int npt) {
-	switch(npt) {
-	case 4: RC_LAUNCH((k_stage<MODE, 4>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	case 7: RC_LAUNCH((k_stage<MODE, 7>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	case 10: RC_LAUNCH((k_stage<MODE, 10>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	default: RC_LAUNCH((k_stage<MODE, 16>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	}
+	if(a.rcp) launch_mode_r<MODE, true>(a, ncells, lds, s, npt);
+	else launch_mode_r<MODE, false>(a, ncells, lds, s, npt);
 }
 
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
