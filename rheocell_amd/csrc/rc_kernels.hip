@@ -62,15 +62,41 @@ __device__ inline int box_elem(int side, int site, int dX, int dY) {
 	return box_x(side, site, dX) + box_y(side, site, dY) * side;
 }
 
+// Wave reductions on the DPP crossbar (no LDS traffic): within each 16-lane
+// row, quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror and row_mirror leave
+// the row total in every lane; the four row totals are then read with
+// v_readlane.  Every lane of the wave must be active.
+template <int CTRL>
+__device__ inline int dpp_i32(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true); }
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+	const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+	const unsigned lo = (unsigned) dpp_i32<CTRL>((int) (unsigned) u);
+	const unsigned hi = (unsigned) dpp_i32<CTRL>((int) (unsigned) (u >> 32));
+	return __builtin_bit_cast(double, ((unsigned long long) hi << 32) | lo);
+}
+__device__ inline double readlane_f64(double v, int lane) {
+	const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+	const unsigned lo = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) u, lane);
+	const unsigned hi = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) (u >> 32), lane);
+	return __builtin_bit_cast(double, ((unsigned long long) hi << 32) | lo);
+}
+constexpr int DPP_QUAD_1032 = 0xB1, DPP_QUAD_2301 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141, DPP_ROW_MIRROR = 0x140;
+// sum over the 64 lanes, the same value in every lane
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-	for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-	return v;
+	v += dpp_f64<DPP_QUAD_1032>(v);
+	v += dpp_f64<DPP_QUAD_2301>(v);
+	v += dpp_f64<DPP_ROW_HALF_MIRROR>(v);
+	v += dpp_f64<DPP_ROW_MIRROR>(v);
+	return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 __device__ inline int wave_min(int v) {
-#pragma unroll
-	for(int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-	return v;
+	v = min(v, dpp_i32<DPP_QUAD_1032>(v));
+	v = min(v, dpp_i32<DPP_QUAD_2301>(v));
+	v = min(v, dpp_i32<DPP_ROW_HALF_MIRROR>(v));
+	v = min(v, dpp_i32<DPP_ROW_MIRROR>(v));
+	return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+			min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
 // Sum NV values over the 256-thread block; the result is valid in thread 0.
@@ -287,7 +313,10 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	const double *phic = a.phi_c + C.start;
 	for(int q = tid; q < S; q += CELL_THREADS) s_phi[q] = phic[q];
 	const int cx = C.corner % L, cy = C.corner / L;
-	{
+	// the adhesion term (omega != 0) reads the 5-point box Laplacian of sum phi^2:
+	// stage the window; otherwise each site reads its own sum phi^2 from the box
+	const bool stage_win = P.omega != 0;
+	if(stage_win) {
 		int wy = tid / WX, wx = tid - wy * WX;
 		const int dy = CELL_THREADS / WX, dx = CELL_THREADS - dy * WX;
 		for(int w = tid; w < WX * WY; w += CELL_THREADS) {
@@ -364,6 +393,12 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	{
 		int w = (ly0 + 1) * WX + (lx + 1);
 		int ly = ly0;
+		// box site of the strip's first row: column bx fixed, row by advancing with ly
+		int bx = cx + lx;
+		if(bx >= L) bx -= L;
+		int by = cy + ly0;
+		if(by >= L) by -= L;
+		const double *phi2b = a.phi2 + B.box_off + bx;
 #pragma unroll
 		for(int k = 0; k < NPT; ++k) {
 			mu[k] = 0.;
@@ -375,7 +410,7 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 				const double dx = .5 * (pE - pW);
 				const double dy = .5 * (pN - pS);
 				if(MODE == MODE_CF && P.beta == 0) per2[0] += dx * dx + dy * dy;
-				const double p2 = s_win[w];
+				const double p2 = stage_win ? s_win[w] : phi2b[by * L];
 				const double CH = P.gamma * (div_by<RCP>(8 * ph * (1 - ph) * (1 - 2 * ph), P.lambda, P.lambda_rcp) - lam2 * lap);
 				const double A = Acoef * ph;
 				const double Rep = Rcoef * ph * (p2 - ph * ph);
@@ -404,11 +439,14 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 					s_rep[o + qx] = rn;
 				}
 			}
-			// next row of the window (ring order wraps at LY)
+			// next row of the window and of the box (ring order wraps at LY)
 			w += WX;
+			if(++by == L) by = 0;
 			if(++ly == LY) {
 				ly = 0;
 				w -= LY * WX;
+				by -= LY;
+				if(by < 0) by += L;
 			}
 		}
 	}
