@@ -145,13 +145,20 @@ def main():
     from rheocell_amd.sim import Simulation
 
     log("[bench] generating %s (seed %d)" % (args.workload, args.seed))
-    conf, keys = confgen.workload(args.workload, seed=args.seed)
+    if args.workload == "ensemble32":
+        boxes = confgen.replica_ensemble(32, seed=args.seed)
+    else:
+        boxes = [confgen.workload(args.workload, seed=args.seed)]
+    conf, keys = boxes[0]
     keys["_name"] = args.workload
-    ncells = len(conf.cells)
+    ncells = sum(len(c.cells) for c, _ in boxes)
     L = conf.Lx
+    box_sites = sum(c.Lx * c.Lx for c, _ in boxes)
     wet = float(keys["friction_cell"]) != 0
-    sim = Simulation([(conf, keys)])
+    sim = Simulation(boxes)
     rows = sim.total_rows()
+    if len(boxes) > 1:
+        log("[bench] %d boxes, sides %s" % (len(boxes), sorted(set(c.Lx for c, _ in boxes))))
     gauss = sim.gaussians(args.warmup + args.steps + args.timing_steps)
     log("[bench] %d cells, %d rows, warmup %d" % (ncells, rows, args.warmup))
     if args.warmup:
@@ -166,6 +173,19 @@ def main():
     value = ncells * args.steps / secs
     ms_step = 1e3 * secs / args.steps
     log("[bench] %.3f ms/step, %.4g cell-steps/s" % (ms_step, value))
+    groups = sim.stage_groups()
+    for g in groups:
+        log("[bench] stage group: %d cells, shape %dx%d, tuner us %s" % (g["cells"], g["threads"], g["rows"],
+                                                                         g["tune_us"]))
+    step_roof = None
+    if not wet:
+        # whole dry step: read phi_n, write phi*, dphi; read phi*, phi_n, dphi, write phi_n+1;
+        # the box sum phi^2 written and read twice (72 B per patch site + 32 B per box site)
+        step_bytes = 72 * rows + 32 * box_sites
+        step_roof = {"bytes_per_step": step_bytes, "achieved": round(step_bytes / (secs / args.steps) / 1e9, 1),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(step_bytes / (secs / args.steps) / 1e9 / PEAK_HBM_GBS, 4),
+                     "bytes_model": "72 B/patch site x %d + 32 B/box site x %d" % (rows, box_sites)}
 
     roof = None
     kernels = {}
@@ -189,7 +209,7 @@ def main():
         if best:
             d = kernels[best]
             per_row, per_site = BYTES[best]
-            bytes_launch = per_row * rows + per_site * L * L
+            bytes_launch = per_row * rows + per_site * box_sites
             useful = sweeps if best in CG_SWEEP_KINDS else d["launches"]
             avg_us = 1e3 * d["ms_total"] / d["launches"]
             achieved = bytes_launch * useful / (d["ms_total"] * 1e-3) / 1e9
@@ -198,7 +218,8 @@ def main():
                     "kernel": _lib.T_SYMBOL.get(best, best), "kind": best,
                     "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3),
                     "launches": d["launches"], "useful_launches": useful,
-                    "bytes_model": "%d B/row x %d rows + %d B/site x %d box sites" % (per_row, rows, per_site, L * L)}
+                    "bytes_model": "%d B/row x %d rows + %d B/site x %d box sites" % (per_row, rows, per_site,
+                                                                                      box_sites)}
         log("[bench] kernels: " + json.dumps({k: (round(v["ms_total"], 3), v["launches"]) for k, v in kernels.items()}))
 
     cgcmp = None
@@ -228,13 +249,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (jittered-lattice tanh cells, seed %d)" % args.seed,
-        "config": {"workload": args.workload, "box": L, "cells": ncells, "patch": "41x41", "rows": rows,
+        "config": {"workload": args.workload, "box": L if len(boxes) == 1 else sorted(c.Lx for c, _ in boxes),
+                   "boxes": len(boxes), "cells": ncells,
+                   "patch": "41x41" if len(boxes) == 1 else "29..49 (uneven)", "rows": rows,
                    "model": "wetpolarmodel", "friction_cell": float(keys["friction_cell"]),
                    "cg_tolerance": float(keys["CGtolerance"]) if wet else None,
                    "warm_start": "previous solution, cold every 11th solve (reference rule)" if wet else None},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "stage_groups": [{"cells": g["cells"], "threads": g["threads"], "rows": g["rows"], "tuner_us": g["tune_us"]}
+                         for g in groups],
     }
+    if step_roof:
+        out["step_roofline"] = step_roof
     if wet:
         out["cg_ms_per_step"] = round((st1.total_solve_ms - st0.total_solve_ms) / args.steps, 4)
         out["cg_iters_per_solve"] = round(iters / max(solves, 1), 2)

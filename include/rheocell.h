@@ -160,6 +160,24 @@ int rc_enable_timing(rc_handle *h, int on);
 const char *rc_last_error(rc_handle *h);
 void rc_destroy(rc_handle *h);
 
+/* ---- Stage kernel shape (no counterpart in the reference) ---------------
+ * The per-cell field-update kernel covers a patch with `threads` threads per
+ * cell (256 or 512), each thread taking one patch column and a strip of up to
+ * `rows` rows.  Cells are grouped by the smallest shape that fits their patch
+ * (uneven patch sizes: one launch per group, each with its own LDS size).
+ * Before the first rc_step after an upload or a patch growth the library
+ * times every shape that fits a group on the current fields (a
+ * side-effect-free force evaluation) and keeps the fastest, unless
+ * rc_set_stage_shape forced one.  Results agree between shapes to rounding of
+ * the per-cell sums. */
+/* force a shape on every group it fits (an error if it fits none); 0, 0: autotune */
+int rc_set_stage_shape(rc_handle *h, int threads, int rows);
+/* the groups (up to max_groups): shape[g] = {threads, rows, cells}; valid[g] =
+ * the shapes that fit it ({0, 0} padded); tune_us[g] = the tuner's mean time per
+ * valid shape in microseconds (0 before tuning).  Any output may be null. */
+int rc_stage_groups(rc_handle *h, int max_groups, int *ngroups, int (*shape)[3], int (*valid)[8][2],
+		double (*tune_us)[8]);
+
 /* ---- Learned CG warm start (no counterpart in the reference) ----------
  * A 6-layer 3x3 convolutional network (32 channels, circular padding, ReLU,
  * BF16 weights and activations, FP32 accumulation on MFMA) maps box fields --

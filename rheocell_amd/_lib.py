@@ -33,7 +33,8 @@ EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_a
            "rc_set_warm_start", "rc_timing", "rc_enable_timing", "rc_last_error", "rc_destroy",
            "rc_net_create", "rc_net_set_params", "rc_net_get_params", "rc_net_forward", "rc_net_grads",
            "rc_net_train_step", "rc_net_last_error", "rc_net_destroy", "rc_attach_net", "rc_snapshot_box",
-           "rc_record_solves", "rc_get_recorded", "rc_set_learned_stages"]
+           "rc_record_solves", "rc_get_recorded", "rc_set_learned_stages", "rc_set_stage_shape",
+           "rc_stage_groups"]
 
 
 class rc_model_params(C.Structure):
@@ -116,6 +117,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.rc_record_solves.argtypes = [vp, ip]
     L.rc_get_recorded.argtypes = [vp, fp, C.POINTER(C.c_int)]
     L.rc_set_learned_stages.argtypes = [vp, ip]
+    L.rc_set_stage_shape.argtypes = [vp, ip, ip]
+    L.rc_stage_groups.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L

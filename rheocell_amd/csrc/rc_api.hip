@@ -78,7 +78,19 @@ struct rc_handle {
 	// state machine
 	bool uploaded = false, initialized = false;
 	bool lists_valid = false;     // tile lists match the current patch geometry
-	int npt = 7;
+	// Field-update kernel groups: cells grouped by the smallest stage shape
+	// that fits their patch, one launch per group with its own shape and LDS.
+	struct StageGroup {
+		std::vector<int> cells;
+		int *d_cells = nullptr;
+		size_t lds = 0;
+		std::vector<std::pair<int, int>> valid;   // shapes that fit every patch of the group
+		int threads = 256, rows = 7;             // shape in use
+		double tune_us[8] = {};                  // tuner: mean pure force-evaluation time per shape
+	};
+	std::vector<StageGroup> groups;
+	int stage_forced[2] = { 0, 0 };     // rc_set_stage_shape; {0, 0}: autotune
+	bool stage_tuned = false;
 	int cur_set = 0;              // parameter set currently applied
 	int cf_set = 0;               // set of the force evaluation preceding the next field update
 	int restart_solver = 0;
@@ -152,7 +164,8 @@ struct TimeScope {
 	int kind;
 	bool group;
 	hipEvent_t b;
-	TimeScope(rc_handle *h_, int k) : h(h_), kind(k), group(k == RC_T_CG_SOLVE || k == RC_T_STEP || k == RC_T_CG_LEARNED), b(nullptr) {
+	TimeScope(rc_handle *h_, int k, bool several = false)
+		: h(h_), kind(k), group(several || k == RC_T_CG_SOLVE || k == RC_T_STEP || k == RC_T_CG_LEARNED), b(nullptr) {
 		if(h->timing) {
 			hipEvent_t a = ev_get(h);
 			b = ev_get(h);
@@ -327,7 +340,8 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_count);
 	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
-	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xprev); dfree(h->d_rprev); dfree(h->d_xrec); dfree(h->d_rnrec);
+	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xprev); dfree(h->d_rprev);
+	for(auto &g : h->groups) dfree(g.d_cells); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
 	for(auto e : h->pool) (void) hipEventDestroy(e);
@@ -357,7 +371,8 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 	std::vector<double> trig;
 	std::map<int, int> trig_off;
 	long long rows = 0;
-	int maxS = 0, maxW = 0, maxT = 0, npt = 0;
+	int maxS = 0, maxW = 0, maxT = 0;
+	std::vector<std::pair<int, int>> dims;
 	for(int b = 0; b < h->nboxes; ++b) {
 		const DevBox &B = h->dboxes[b];
 		for(int i = B.cell_begin; i < B.cell_end; ++i) {
@@ -410,8 +425,7 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			maxS = std::max(maxS, d.S);
 			maxW = std::max(maxW, (d.LX + 2) * (d.LY + 2));
 			maxT = std::max(maxT, 2 * d.LY);
-			const int n = stage_npt(d.LX, d.LY);
-			npt = n < 0 || npt < 0 ? -1 : std::max(npt, n);
+			dims.emplace_back(d.LX, d.LY);
 		}
 	}
 	if(rows >= (1ll << 31)) return fail(h, RC_ERR_ARG, "too many patch sites (%lld)", rows);
@@ -463,11 +477,57 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 	HIPCHK(h, hipMemsetAsync(h->d_st, 0, sizeof(DevStatus), h->stream));
 	h->max_S = maxS;
 	h->max_W = maxW;
-	// patch, window / mu, y trig tables, Repulsion
+	if(std::max(maxS, maxW) >= (1 << 16))
+		return fail(h, RC_ERR_ARG, "a patch of %d sites is too large", maxS);
+	{
+		// group the cells by the first stage shape (table order) that fits their patch
+		int shapes[16][2];
+		const int n = stage_shapes(shapes);
+		for(auto &g : h->groups) dfree(g.d_cells);
+		h->groups.assign(n, rc_handle::StageGroup());
+		for(int i = 0; i < h->ncells; ++i) {
+			const DevCell &d = dc[i];
+			int k = 0;
+			while(k < n && stage_strip_rows(d.LX, d.LY, shapes[k][0]) > shapes[k][1]) ++k;
+			if(k == n) return fail(h, RC_ERR_ARG, "cell %d: a %dx%d patch does not fit the stage kernel", i, d.LX, d.LY);
+			h->groups[k].cells.push_back(i);
+		}
+		std::vector<rc_handle::StageGroup> used;
+		for(int k = 0; k < n; ++k) {
+			rc_handle::StageGroup &g = h->groups[k];
+			if(g.cells.empty()) continue;
+			int gS = 0, gW = 0, gT = 0;
+			for(int i : g.cells) {
+				gS = std::max(gS, dc[i].S);
+				gW = std::max(gW, (dc[i].LX + 2) * (dc[i].LY + 2));
+				gT = std::max(gT, 2 * dc[i].LY);
+			}
+			// patch, window / mu, y trig tables, Repulsion
+			g.lds = sizeof(double) * (size_t) (gS + gW + gT + (h->jq ? gS : 0));
+			for(int j = k; j < n; ++j) {
+				bool ok = true;
+				for(int i : g.cells) ok = ok && stage_strip_rows(dc[i].LX, dc[i].LY, shapes[j][0]) <= shapes[j][1];
+				if(ok) g.valid.emplace_back(shapes[j][0], shapes[j][1]);
+			}
+			g.threads = g.valid[0].first;
+			g.rows = g.valid[0].second;
+			for(const auto &v : g.valid)
+				if(v.first == h->stage_forced[0] && v.second == h->stage_forced[1]) {
+					g.threads = v.first;
+					g.rows = v.second;
+				}
+			HIPCHK(h, dalloc(&g.d_cells, g.cells.size()));
+			HIPCHK(h, hipMemcpy(g.d_cells, g.cells.data(), sizeof(int) * g.cells.size(), hipMemcpyHostToDevice));
+			used.push_back(std::move(g));
+			h->groups[k].d_cells = nullptr;
+		}
+		h->groups = std::move(used);
+		bool tunable = false;
+		for(const auto &g : h->groups) tunable = tunable || g.valid.size() > 1;
+		h->stage_tuned = !tunable || h->stage_forced[0] != 0;
+	}
+	// largest group LDS: patch, window / mu, y trig tables, Repulsion
 	h->lds_stage = sizeof(double) * (size_t) (maxS + maxW + maxT + (h->jq ? maxS : 0));
-	h->npt = npt;
-	if(h->npt < 0 || std::max(maxS, maxW) >= (1 << 16))
-		return fail(h, RC_ERR_ARG, "a patch of %d sites does not fit the stage kernel (at most 16 rows per thread strip)", maxS);
 	h->lds_cg = sizeof(double2) * (size_t) maxW;
 	if(h->lds_stage > 160 * 1024) return fail(h, RC_ERR_ARG, "patch of %d sites needs %zu B of LDS", maxS, h->lds_stage);
 	HIPCHK(h, stage_set_lds_limit(std::max<size_t>(h->lds_stage, 64 * 1024)));
@@ -499,7 +559,8 @@ static StageArgs stage_args(rc_handle *h) {
 	a.jq = h->jq ? 1 : 0;
 	a.dphi_old = h->d_dphi;
 	a.geo = h->d_geo;
-	a.npt = h->npt;
+	a.threads = 256;
+	a.npt = 7;
 	a.rcp = 1;
 	for(const DevParams &p : h->dparams)
 		if(p.lambda_rcp == 0 || p.friction_rcp == 0) a.rcp = 0;
@@ -527,12 +588,27 @@ static int run_gather(rc_handle *h, const double *phi) {
 	return RC_OK;
 }
 
+// one launch per stage group, each with its shape, cell list and LDS size
+static int launch_groups(rc_handle *h, int mode, const StageArgs &a0) {
+	for(const auto &g : h->groups) {
+		StageArgs a = a0;
+		a.threads = g.threads;
+		a.npt = g.rows;
+		a.cell_list = h->groups.size() > 1 ? g.d_cells : nullptr;
+		HIPCHK(h, launch_stage(mode, a, (int) g.cells.size(), g.lds, h->stream));
+	}
+	return RC_OK;
+}
+
 static int run_stage(rc_handle *h, int mode, const StageArgs &a) {
 	const int kind = mode == MODE_CF ? RC_T_STAGE_CF
 			: mode == MODE_FUSED ? (a.store ? RC_T_STAGE_FUSED_PRED : RC_T_STAGE_FUSED_CORR)
 			: (a.store ? RC_T_STAGE_FS_PRED : RC_T_STAGE_FS_CORR);
-	TimeScope ts(h, kind);
-	HIPCHK(h, launch_stage(mode, a, h->ncells, h->lds_stage, h->stream));
+	{
+		TimeScope ts(h, kind, h->groups.size() > 1);   // several launches are timed as one
+		int rc = launch_groups(h, mode, a);
+		if(rc) return rc;
+	}
 	if(mode != MODE_CF && !a.store) h->lists_valid = false;   // the corrector recentres patches
 	return RC_OK;
 }
@@ -1024,10 +1100,110 @@ static int serve_growth(rc_handle *h) {
 	return RC_OK;
 }
 
+// Stage-shape autotuner: for each group, times the side-effect-free force
+// evaluation of its cells (pure MODE_CF on the current fields, force/velocity
+// rows diverted to scratch) with every shape that fits the group, 3 launches
+// each after a warm-up, and keeps the fastest.  Runs before the first step
+// after a layout change.
+static int autotune_stage(rc_handle *h) {
+	h->stage_tuned = true;
+	int rc;
+	if((rc = run_gather(h, h->d_phiA))) return rc;
+	const bool timing = h->timing;
+	h->timing = false;
+	hipEvent_t e0 = h->cg_e0, e1 = h->cg_e1;
+	for(auto &g : h->groups) {
+		if(g.valid.size() < 2) continue;
+		double best = 1e300;
+		std::pair<int, int> pick = g.valid[0];
+		for(size_t i = 0; i < g.valid.size(); ++i) {
+			StageArgs a = stage_args(h);
+			a.cf_set = h->cf_set;
+			a.fs_set = h->cf_set;
+			a.phi_c = h->d_phiA;
+			a.pure = 1;
+			if(h->wet) {
+				a.frows = h->d_r;
+				a.vrows = h->d_r;
+			}
+			a.threads = g.valid[i].first;
+			a.npt = g.valid[i].second;
+			a.cell_list = h->groups.size() > 1 ? g.d_cells : nullptr;
+			const int n = (int) g.cells.size();
+			HIPCHK(h, launch_stage(MODE_CF, a, n, g.lds, h->stream));
+			HIPCHK(h, hipEventRecord(e0, h->stream));
+			for(int r = 0; r < 3; ++r) HIPCHK(h, launch_stage(MODE_CF, a, n, g.lds, h->stream));
+			HIPCHK(h, hipEventRecord(e1, h->stream));
+			HIPCHK(h, hipEventSynchronize(e1));
+			float ms = 0.f;
+			HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+			const double us = 1e3 * ms / 3;
+			if(i < 8) g.tune_us[i] = us;
+			if(us < best) {
+				best = us;
+				pick = g.valid[i];
+			}
+		}
+		g.threads = pick.first;
+		g.rows = pick.second;
+	}
+	h->timing = timing;
+	return check_status(h);
+}
+
+extern "C" int rc_set_stage_shape(rc_handle *h, int threads, int rows) {
+	if(!h) return RC_ERR_ARG;
+	if(threads == 0 && rows == 0) {
+		h->stage_forced[0] = h->stage_forced[1] = 0;
+		h->stage_tuned = false;
+		return RC_OK;
+	}
+	bool any = false;
+	for(auto &g : h->groups)
+		for(const auto &v : g.valid)
+			if(v.first == threads && v.second == rows) {
+				g.threads = threads;
+				g.rows = rows;
+				any = true;
+			}
+	if(!any) return fail(h, RC_ERR_ARG, "stage shape %d threads x %d rows fits no patch group", threads, rows);
+	h->stage_forced[0] = threads;
+	h->stage_forced[1] = rows;
+	h->stage_tuned = true;
+	return RC_OK;
+}
+
+extern "C" int rc_stage_groups(rc_handle *h, int max_groups, int *ngroups, int (*shape)[3], int (*valid)[8][2],
+		double (*tune_us)[8]) {
+	if(!h || !ngroups) return RC_ERR_ARG;
+	*ngroups = (int) h->groups.size();
+	for(int k = 0; k < (int) h->groups.size() && k < max_groups; ++k) {
+		const auto &g = h->groups[k];
+		if(shape) {
+			shape[k][0] = g.threads;
+			shape[k][1] = g.rows;
+			shape[k][2] = (int) g.cells.size();
+		}
+		for(int i = 0; i < 8; ++i) {
+			const bool in = i < (int) g.valid.size();
+			if(valid) {
+				valid[k][i][0] = in ? g.valid[i].first : 0;
+				valid[k][i][1] = in ? g.valid[i].second : 0;
+			}
+			if(tune_us) tune_us[k][i] = in ? g.tune_us[i] : 0.;
+		}
+	}
+	return RC_OK;
+}
+
 extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 	if(!h) return RC_ERR_ARG;
 	if(!h->initialized) return fail(h, RC_ERR_STATE, "rc_step before rc_init_forces");
 	if(nsteps < 0) return fail(h, RC_ERR_ARG, "negative step count");
+	if(!h->stage_tuned) {
+		int rt = autotune_stage(h);
+		if(rt) return rt;
+	}
 	const int chunk = 256;
 	hipEvent_t t0 = nullptr, t1 = nullptr;
 	if(h->timing) {
@@ -1066,7 +1242,7 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 			HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
 			HIPCHK(h, hipStreamSynchronize(h->stream));
 			const DevStatus st = *h->h_st;
-			if(st.flags == ST_GROWTH) {
+			if((st.flags & ST_KIND_MASK) == ST_GROWTH) {
 				// the device halted after the corrector of step st.step
 				if((rc = serve_growth(h))) return rc;
 				if(h->wet) {

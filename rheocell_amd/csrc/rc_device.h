@@ -26,6 +26,11 @@ constexpr int NRED = 11;         // per-cell reductions of the field update
 constexpr unsigned ST_OVERLAP = 1u;
 constexpr unsigned ST_GROWTH = 2u;
 constexpr unsigned ST_TILE = 4u;
+// a growth request also ORs (step + 1) << ST_STEP_SHIFT into the flags: every
+// cell of one corrector stage writes the same bits, so one plain load tells a
+// later launch of that stage that the request is its own
+constexpr unsigned ST_KIND_MASK = 0xFFu;
+constexpr int ST_STEP_SHIFT = 8;
 
 struct DevParams {
 	double gamma, lambda, omega, mu, kappa, a0, p0, beta, passive_alpha;

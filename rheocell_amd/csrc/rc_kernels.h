@@ -30,7 +30,8 @@ struct StageArgs {
 	double *out_mu, *out_dx, *out_dy, *out_lap;   // optional site outputs (MODE_CF)
 	const double *gauss;         // predictor noise, one per cell
 	int4 *geo;                   // compact geometry, rewritten at recentering
-	int npt;                     // sites per thread (stage_npt, largest over the patches)
+	int threads, npt;            // stage shape: threads per cell, rows per thread strip (>= every patch's)
+	const int *cell_list;        // cell of each workgroup (null: workgroup i takes cell i)
 	int rcp;                     // lambda and every friction are powers of two (DevParams *_rcp)
 	DevStatus *st;
 };
@@ -104,8 +105,10 @@ hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap,
 hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
 hipError_t stage_set_lds_limit(size_t bytes);
 // sites-per-thread instantiation for a largest patch of max_sites (-1: too large)
-// sites per thread (strip rows) of the stage kernel for an LX x LY patch; -1 if too large
-int stage_npt(int LX, int LY);
+// stage kernel shapes {threads per cell, rows per thread strip}; returns their count
+int stage_shapes(int (*shapes)[2]);
+// rows per thread strip of an LX x LY patch with the given threads per cell
+int stage_strip_rows(int LX, int LY, int threads);
 // events recorded by the next launches at kernel start / end (null: off)
 void set_launch_events(hipEvent_t start, hipEvent_t stop);
 

@@ -99,8 +99,9 @@ __device__ inline int wave_min(int v) {
 			min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
-// Sum NV values over the 256-thread block; the result is valid in thread 0.
-template <int NV>
+// Sum NV values over a block of NW waves (256 threads by default); the result
+// is valid in thread 0.  s_red holds NW * NV doubles.
+template <int NV, int NW = 4>
 __device__ inline void block_sum(double (&v)[NV], double *s_red) {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -112,7 +113,12 @@ __device__ inline void block_sum(double (&v)[NV], double *s_red) {
 	__syncthreads();
 	if(threadIdx.x == 0) {
 #pragma unroll
-		for(int j = 0; j < NV; ++j) v[j] = ((s_red[j] + s_red[NV + j]) + s_red[2 * NV + j]) + s_red[3 * NV + j];
+		for(int j = 0; j < NV; ++j) {
+			double t = s_red[j];
+#pragma unroll
+			for(int w = 1; w < NW; ++w) t += s_red[w * NV + j];
+			v[j] = t;
+		}
 	}
 	__syncthreads();
 }
@@ -259,13 +265,13 @@ hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s) {
 // k_stage<MODE>: force evaluation (+ field update) of one cell
 // ---------------------------------------------------------------------------
 // Column strips: a cell's LX x LY patch (ring-buffer coordinates, storage
-// q = qy * LX + qx) is covered by nb = CELL_THREADS / LX row blocks of
-// R = ceil(LY / nb) rows; thread t takes column qx = t % LX of block t / LX.
+// q = qy * LX + qx) is covered by nb = CT / LX row blocks of R = ceil(LY / nb)
+// rows, CT the threads per cell; thread t takes column qx = t % LX of block t / LX.
 // Column-dependent quantities (E/W neighbour columns, the box-window column,
 // the x trig table entries) are then per-thread constants, the N/S neighbours
 // of a run of rows come from registers, and no index is divided per site.
-__device__ __host__ inline int strip_rows(int LX, int LY) {
-	const int nb = CELL_THREADS / LX;
+__device__ __host__ inline int strip_rows(int LX, int LY, int CT) {
+	const int nb = CT / LX;
 	return nb > 0 ? (LY + nb - 1) / nb : 1 << 20;
 }
 
@@ -274,17 +280,30 @@ __device__ __host__ inline int strip_rows(int LX, int LY) {
 template <bool RCP>
 __device__ inline double div_by(double x, double y, double rcp) { return RCP ? x * rcp : x / y; }
 
-// NPT >= the strip rows of every patch (compile time), so the per-site
-// registers are fully unrolled.  RCP: lambda and the friction of every
-// parameter set are powers of two (divisions become exact multiplications).
-template <int MODE, int NPT, bool RCP>
-__global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
-	if(a.st->flags) return;
+// CT threads per cell; NPT >= the strip rows of every patch (compile time), so
+// the per-site registers are fully unrolled.  RCP: lambda and the friction of
+// every parameter set are powers of two (divisions become exact multiplications).
+template <int MODE, int NPT, bool RCP, int CT>
+__global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
+	// Any status flag halts the device, except that a growth request raised
+	// by a corrector of this same step leaves the rest of that corrector stage
+	// to run (cells of other stage groups are launched after it); the host
+	// serves the growth once the stage is complete.  Correctors of later steps,
+	// already queued, halt.
+	{
+		const unsigned f = a.st->flags;
+		if(f) {
+			const bool same_step = MODE != MODE_CF && !a.store && (f & ST_KIND_MASK) == ST_GROWTH &&
+					(f >> ST_STEP_SHIFT) == (unsigned) (a.step + 1);
+			if(!same_step) return;
+		}
+	}
+	constexpr int NW = CT / 64;
 	extern __shared__ double smem[];
-	__shared__ double s_red[4 * NRED];
-	__shared__ int s_redi[8];
+	__shared__ double s_red[NW * NRED];
+	__shared__ int s_redi[2 * NW];
 	__shared__ double s_bc;
-	const int ci = blockIdx.x, tid = threadIdx.x;
+	const int ci = a.cell_list ? a.cell_list[blockIdx.x] : blockIdx.x, tid = threadIdx.x;
 	const DevCell C = a.cells[ci];
 	const DevBox B = a.boxes[C.box];
 	const DevParams P = a.params[2 * C.box + a.cf_set];
@@ -298,7 +317,7 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	const bool fs = MODE != MODE_CF;
 
 	// this thread's strip: column qx, rows qy0 .. qy0 + nr - 1
-	const int R = strip_rows(LX, LY);
+	const int R = strip_rows(LX, LY, CT);
 	const int qx = tid % LX, blk = tid / LX;
 	const int qy0 = blk * R;
 	const int nr = max(0, min(R, LY - qy0));
@@ -311,15 +330,15 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 
 	// stage the patch, the (LX+2)x(LY+2) box window of sum phi^2 and the y trig tables
 	const double *phic = a.phi_c + C.start;
-	for(int q = tid; q < S; q += CELL_THREADS) s_phi[q] = phic[q];
+	for(int q = tid; q < S; q += CT) s_phi[q] = phic[q];
 	const int cx = C.corner % L, cy = C.corner / L;
 	// the adhesion term (omega != 0) reads the 5-point box Laplacian of sum phi^2:
 	// stage the window; otherwise each site reads its own sum phi^2 from the box
 	const bool stage_win = P.omega != 0;
 	if(stage_win) {
 		int wy = tid / WX, wx = tid - wy * WX;
-		const int dy = CELL_THREADS / WX, dx = CELL_THREADS - dy * WX;
-		for(int w = tid; w < WX * WY; w += CELL_THREADS) {
+		const int dy = CT / WX, dx = CT - dy * WX;
+		for(int w = tid; w < WX * WY; w += CT) {
 			int bx = cx + wx - 1;
 			if(bx < 0) bx += L;
 			else if(bx >= L) bx -= L;
@@ -335,7 +354,7 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 			}
 		}
 	}
-	for(int t = tid; t < 2 * LY; t += CELL_THREADS) s_trig[t] = a.trig[C.trig_y + t];
+	for(int t = tid; t < 2 * LY; t += CT) s_trig[t] = a.trig[C.trig_y + t];
 	double pold[NPT], dold[NPT];
 	double2 vv[NPT];
 #pragma unroll
@@ -375,7 +394,7 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 			const double dy = .5 * (col[k + 2] - col[k]);
 			per[0] += dx * dx + dy * dy;
 		}
-		block_sum<1>(per, s_red);
+		block_sum<1, NW>(per, s_red);
 		if(tid == 0) s_bc = per[0];
 		__syncthreads();
 		perimeter = s_bc;
@@ -451,7 +470,7 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 		}
 	}
 	if(MODE == MODE_CF && P.beta == 0) {
-		block_sum<1>(per2, s_red);
+		block_sum<1, NW>(per2, s_red);
 		if(tid == 0) s_bc = per2[0];
 		__syncthreads();
 		perimeter = s_bc;
@@ -561,7 +580,7 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 			return;
 		}
 		double tr[2] = { red[8], red[9] };
-		block_sum<2>(tr, s_red);
+		block_sum<2, NW>(tr, s_red);
 		if(tid == 0) {
 			DevCell &G = a.cells[ci];
 			G.perimeter = perimeter;
@@ -571,13 +590,13 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 		return;
 	}
 
-	block_sum<NRED>(red, s_red);
+	block_sum<NRED, NW>(red, s_red);
 	{
 		const int lane = tid & 63, wave = tid >> 6;
 		const int mx = wave_min(xmin), my = wave_min(ymin);
 		if(lane == 0) {
 			s_redi[wave] = mx;
-			s_redi[4 + wave] = my;
+			s_redi[NW + wave] = my;
 		}
 		if(tid == 0) {
 #pragma unroll
@@ -633,8 +652,10 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	G.perimeter = 0.;
 	int xsl = LX, ysb = LY;
 	if(!a.store) {
-		xsl = min(min(s_redi[0], s_redi[1]), min(s_redi[2], s_redi[3]));
-		ysb = min(min(s_redi[4], s_redi[5]), min(s_redi[6], s_redi[7]));
+		for(int w = 0; w < NW; ++w) {
+			xsl = min(xsl, s_redi[w]);
+			ysb = min(ysb, s_redi[NW + w]);
+		}
 	}
 	G.x_sub_left = xsl;
 	G.y_sub_bottom = ysb;
@@ -661,8 +682,8 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 		}
 		else {
 			// patch growth (MultiPhaseField.cpp:248-310) is done by the host:
-			// flag it and halt; every later kernel returns at entry
-			if(atomicOr(&a.st->flags, ST_GROWTH) == 0u) {
+			// flag it, with the step, and halt
+			if(atomicOr(&a.st->flags, ST_GROWTH | ((unsigned) (a.step + 1) << ST_STEP_SHIFT)) == 0u) {
 				a.st->first_cell = ci;
 				a.st->step = a.step;
 			}
@@ -686,13 +707,20 @@ __global__ __launch_bounds__(CELL_THREADS) void k_stage(StageArgs a) {
 	}
 }
 
+// stage kernel shapes (threads per cell, rows per thread strip)
+static const int STAGE_SHAPES[][2] = { { 256, 4 }, { 256, 7 }, { 512, 4 }, { 512, 7 }, { 512, 10 }, { 512, 16 } };
+constexpr int NSTAGE_SHAPES = 6;
+
 template <int MODE, bool RCP>
 static hipError_t set_lds_r(size_t bytes) {
 	hipError_t e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 10, RCP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)) != hipSuccess) return e;
-	return hipFuncSetAttribute((const void *) k_stage<MODE, 16, RCP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes);
+	const int b = (int) bytes;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 10, RCP, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	return hipFuncSetAttribute((const void *) k_stage<MODE, 16, RCP, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
 }
 template <int MODE>
 static hipError_t set_lds(size_t bytes) {
@@ -707,30 +735,41 @@ hipError_t stage_set_lds_limit(size_t bytes) {
 	return set_lds<MODE_FS>(bytes);
 }
 
-int stage_npt(int LX, int LY) {
-	const int n = strip_rows(LX, LY);
-	return n <= 4 ? 4 : n <= 7 ? 7 : n <= 10 ? 10 : n <= 16 ? 16 : -1;
+int stage_shapes(int (*shapes)[2]) {
+	for(int i = 0; i < NSTAGE_SHAPES; ++i) {
+		shapes[i][0] = STAGE_SHAPES[i][0];
+		shapes[i][1] = STAGE_SHAPES[i][1];
+	}
+	return NSTAGE_SHAPES;
 }
+int stage_strip_rows(int LX, int LY, int threads) { return strip_rows(LX, LY, threads); }
 
 template <int MODE, bool RCP>
-static void launch_mode_r(const StageArgs &a, int ncells, size_t lds, hipStream_t s, int npt) {
-	switch(npt) {
-	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	case 10: RC_LAUNCH((k_stage<MODE, 10, RCP>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
-	default: RC_LAUNCH((k_stage<MODE, 16, RCP>), dim3(ncells), dim3(CELL_THREADS), lds, s, a); break;
+static void launch_mode_r(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
+	const dim3 g(ncells);
+	if(a.threads == 256) {
+		if(a.npt == 4) RC_LAUNCH((k_stage<MODE, 4, RCP, 256>), g, dim3(256), lds, s, a);
+		else RC_LAUNCH((k_stage<MODE, 7, RCP, 256>), g, dim3(256), lds, s, a);
+		return;
+	}
+	switch(a.npt) {
+	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP, 512>), g, dim3(512), lds, s, a); break;
+	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP, 512>), g, dim3(512), lds, s, a); break;
+	case 10: RC_LAUNCH((k_stage<MODE, 10, RCP, 512>), g, dim3(512), lds, s, a); break;
+	default: RC_LAUNCH((k_stage<MODE, 16, RCP, 512>), g, dim3(512), lds, s, a); break;
 	}
 }
 template <int MODE>
-static void launch_mode(const StageArgs &a, int ncells, size_t lds, hipStream_t s, int npt) {
-	if(a.rcp) launch_mode_r<MODE, true>(a, ncells, lds, s, npt);
-	else launch_mode_r<MODE, false>(a, ncells, lds, s, npt);
+static void launch_mode(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
+	if(a.rcp) launch_mode_r<MODE, true>(a, ncells, lds, s);
+	else launch_mode_r<MODE, false>(a, ncells, lds, s);
 }
 
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
-	if(mode == MODE_FUSED) launch_mode<MODE_FUSED>(a, ncells, lds, s, a.npt);
-	else if(mode == MODE_CF) launch_mode<MODE_CF>(a, ncells, lds, s, a.npt);
-	else launch_mode<MODE_FS>(a, ncells, lds, s, a.npt);
+	if(a.threads != 256 && a.threads != 512) return hipErrorInvalidValue;
+	if(mode == MODE_FUSED) launch_mode<MODE_FUSED>(a, ncells, lds, s);
+	else if(mode == MODE_CF) launch_mode<MODE_CF>(a, ncells, lds, s);
+	else launch_mode<MODE_FS>(a, ncells, lds, s);
 	return hipGetLastError();
 }
 

@@ -163,6 +163,26 @@ class Simulation:
         self._check(self.L.rc_get_site_array(self.h, which, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def set_stage_shape(self, threads: int = 0, rows: int = 0):
+        """Force the field-update kernel's shape (threads per cell, rows per
+        thread strip); 0, 0 lets the autotuner choose."""
+        self._check(self.L.rc_set_stage_shape(self.h, threads, rows))
+
+    def stage_groups(self) -> List[Dict]:
+        """Field-update kernel groups: shape in use, cells, valid shapes and the
+        tuner's time per valid shape (us)."""
+        ng = C.c_int()
+        shape = (C.c_int * (3 * 16))()
+        valid = (C.c_int * (16 * 16))()
+        us = (C.c_double * (16 * 8))()
+        self._check(self.L.rc_stage_groups(self.h, 16, C.byref(ng), shape, valid, us))
+        out = []
+        for g in range(min(ng.value, 16)):
+            v = [(valid[16 * g + 2 * i], valid[16 * g + 2 * i + 1]) for i in range(8) if valid[16 * g + 2 * i]]
+            out.append({"threads": shape[3 * g], "rows": shape[3 * g + 1], "cells": shape[3 * g + 2], "valid": v,
+                        "tune_us": {"%dx%d" % sv: round(us[8 * g + i], 2) for i, sv in enumerate(v)}})
+        return out
+
     def cg_stats(self) -> _lib.rc_cg_stats_t:
         st = _lib.rc_cg_stats_t()
         self._check(self.L.rc_cg_stats(self.h, C.byref(st)))

@@ -180,3 +180,50 @@ def test_wet_solve_is_deterministic():
     assert np.array_equal(a.download()[1], b.download()[1])
     assert np.array_equal(a.rows(0), b.rows(0))
     assert a.cg_stats().iters_total == b.cg_stats().iters_total
+
+
+@pytest.mark.parametrize("fc", [0, 1], ids=["dry", "wet"])
+def test_every_stage_shape_matches_oracle(fc):
+    # uneven patches 29..49 (several stage groups): every (threads per cell,
+    # strip rows) shape of the field-update kernel, forced in turn on the groups
+    # it fits, against the oracle
+    rng = np.random.default_rng(5)
+    n = 3
+    sides = [int(s) for s in rng.choice([29, 41, 49], n * n)]
+    radii = [float(r) for r in rng.uniform(6, 10, n * n)]
+    conf = confgen.lattice_conf(96, n, seed=31, radii=radii, sides=sides)
+    keys = dict(confgen.BASE_KEYS, seed=31, friction_cell=fc, CGtolerance=1e-13)
+    orc = OracleSim(conf, keys)
+    orc.step(6)
+    probe = Simulation([(conf, keys)])
+    groups = probe.stage_groups()
+    probe.close()
+    print("groups", groups)
+    assert len(groups) >= 2 and sum(g["cells"] for g in groups) == n * n
+    shapes = sorted({v for g in groups for v in g["valid"]})
+    assert len(shapes) >= 3 and all(t in (256, 512) for t, _ in shapes)
+    for t, r in shapes:
+        sim = Simulation([(conf, keys)])
+        sim.set_stage_shape(t, r)
+        sim.step(6)
+        for g in sim.stage_groups():
+            if (t, r) in g["valid"]:
+                assert (g["threads"], g["rows"]) == (t, r)
+        w = _max_dphi(sim, [orc])
+        print("shape", t, "x", r, "max|dphi| =", w)
+        assert w <= 1e-10
+        sim.close()
+
+
+def test_stage_autotuner_picks_a_timed_shape():
+    conf = confgen.lattice_conf(128, 4, seed=3)
+    sim = Simulation([(conf, dict(confgen.BASE_KEYS, seed=3))])
+    sim.step(1)
+    (g,) = sim.stage_groups()
+    print(g)
+    assert (g["threads"], g["rows"]) in g["valid"] and g["cells"] == 16
+    times = g["tune_us"]
+    assert all(v > 0 for v in times.values())
+    assert times["%dx%d" % (g["threads"], g["rows"])] == min(times.values())
+    with pytest.raises(RCexception):
+        sim.set_stage_shape(256, 4)   # 4 rows per strip do not cover a 41-row patch with 256 threads
