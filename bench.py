@@ -52,7 +52,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20, help="untimed steps (also captures the dry-step graph)")
     ap.add_argument("--workload", default="wet512_1600")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--timing-steps", type=int, default=2)
@@ -62,6 +62,7 @@ def parse():
     ap.add_argument("--weights", default=os.path.join(ROOT, "rheocell_amd", "weights", "warmstart_wet256.npz"),
                     help="learned warm-start weights (wet workloads; skipped when absent)")
     ap.add_argument("--cg-steps", type=int, default=20, help="steps of the learned-vs-previous CG comparison")
+    ap.add_argument("--no-graphs", action="store_true", help="launch dry steps one by one instead of HIP graph replays")
     return ap.parse_args()
 
 
@@ -156,6 +157,8 @@ def main():
     box_sites = sum(c.Lx * c.Lx for c, _ in boxes)
     wet = float(keys["friction_cell"]) != 0
     sim = Simulation(boxes)
+    if args.no_graphs:
+        sim.set_graphs(False)
     rows = sim.total_rows()
     if len(boxes) > 1:
         log("[bench] %d boxes, sides %s" % (len(boxes), sorted(set(c.Lx for c, _ in boxes))))
@@ -257,6 +260,7 @@ def main():
                    "warm_start": "previous solution, cold every 11th solve (reference rule)" if wet else None},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "graphs": (lambda g: {"replays": g[0], "steps_launched_singly": g[1]})(sim.graph_stats()),
         "stage_groups": [{"cells": g["cells"], "threads": g["threads"], "rows": g["rows"], "tuner_us": g["tune_us"]}
                          for g in groups],
     }

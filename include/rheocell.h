@@ -178,6 +178,14 @@ int rc_set_stage_shape(rc_handle *h, int threads, int rows);
 int rc_stage_groups(rc_handle *h, int max_groups, int *ngroups, int (*shape)[3], int (*valid)[8][2],
 		double (*tune_us)[8]);
 
+/* ---- HIP graphs (no counterpart in the reference) -----------------------
+ * Dry-model steps are replayed from a HIP graph of 16 steps (captured once per
+ * layout, stage shapes and parameter set); remainders, timing runs and the
+ * steps after a patch growth run as ordinary launches.  Results are identical
+ * either way.  Default: enabled. */
+int rc_set_graphs(rc_handle *h, int enable);
+int rc_graph_stats(rc_handle *h, long long *graph_launches, long long *eager_steps);
+
 /* ---- Learned CG warm start (no counterpart in the reference) ----------
  * A 6-layer 3x3 convolutional network (32 channels, circular padding, ReLU,
  * BF16 weights and activations, FP32 accumulation on MFMA) maps box fields --

@@ -89,6 +89,15 @@ struct rc_handle {
 		double tune_us[8] = {};                  // tuner: mean pure force-evaluation time per shape
 	};
 	std::vector<StageGroup> groups;
+
+	// HIP graph of GRAPH_STEPS dry steps, replayed per window of steps
+	bool use_graphs = true, graph_valid = false, graph_gauss = false, capturing = false;
+	int graph_set = -1;
+	hipGraphExec_t graph_exec = nullptr;
+	double *d_ggauss = nullptr;     // the window's predictor noise [GRAPH_STEPS][ncells]
+	int *d_step_base = nullptr;     // first step of the window being replayed
+	int *h_step_base = nullptr;     // pinned, one slot per window of a chunk
+	long long graph_launches = 0, eager_steps = 0;
 	int stage_forced[2] = { 0, 0 };     // rc_set_stage_shape; {0, 0}: autotune
 	bool stage_tuned = false;
 	int cur_set = 0;              // parameter set currently applied
@@ -341,7 +350,10 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
 	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xprev); dfree(h->d_rprev);
-	for(auto &g : h->groups) dfree(g.d_cells); dfree(h->d_xrec); dfree(h->d_rnrec);
+	for(auto &g : h->groups) dfree(g.d_cells);
+	if(h->graph_exec) (void) hipGraphExecDestroy(h->graph_exec);
+	dfree(h->d_ggauss); dfree(h->d_step_base);
+	if(h->h_step_base) (void) hipHostFree(h->h_step_base); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
 	for(auto e : h->pool) (void) hipEventDestroy(e);
@@ -525,6 +537,7 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 		bool tunable = false;
 		for(const auto &g : h->groups) tunable = tunable || g.valid.size() > 1;
 		h->stage_tuned = !tunable || h->stage_forced[0] != 0;
+		h->graph_valid = false;
 	}
 	// largest group LDS: patch, window / mu, y trig tables, Repulsion
 	h->lds_stage = sizeof(double) * (size_t) (maxS + maxW + maxT + (h->jq ? maxS : 0));
@@ -561,6 +574,7 @@ static StageArgs stage_args(rc_handle *h) {
 	a.geo = h->d_geo;
 	a.threads = 256;
 	a.npt = 7;
+	a.step_base = h->capturing ? h->d_step_base : nullptr;
 	a.rcp = 1;
 	for(const DevParams &p : h->dparams)
 		if(p.lambda_rcp == 0 || p.friction_rcp == 0) a.rcp = 0;
@@ -1148,6 +1162,7 @@ static int autotune_stage(rc_handle *h) {
 		g.rows = pick.second;
 	}
 	h->timing = timing;
+	h->graph_valid = false;
 	return check_status(h);
 }
 
@@ -1156,6 +1171,7 @@ extern "C" int rc_set_stage_shape(rc_handle *h, int threads, int rows) {
 	if(threads == 0 && rows == 0) {
 		h->stage_forced[0] = h->stage_forced[1] = 0;
 		h->stage_tuned = false;
+		h->graph_valid = false;
 		return RC_OK;
 	}
 	bool any = false;
@@ -1170,6 +1186,7 @@ extern "C" int rc_set_stage_shape(rc_handle *h, int threads, int rows) {
 	h->stage_forced[0] = threads;
 	h->stage_forced[1] = rows;
 	h->stage_tuned = true;
+	h->graph_valid = false;
 	return RC_OK;
 }
 
@@ -1193,6 +1210,58 @@ extern "C" int rc_stage_groups(rc_handle *h, int max_groups, int *ngroups, int (
 			if(tune_us) tune_us[k][i] = in ? g.tune_us[i] : 0.;
 		}
 	}
+	return RC_OK;
+}
+
+// Dry steps as a HIP graph: GRAPH_STEPS steps (gather, predictor groups,
+// gather, corrector groups each) captured once per layout, stage shapes and
+// parameter set, replayed per window.  The window's noise is copied into
+// d_ggauss and its first step index into d_step_base before each replay, so a
+// growth request raised inside a replay reports its absolute step.  The
+// first gather of the window always rebuilds the tile lists.
+constexpr int GRAPH_STEPS = 16;
+
+static int capture_graph(rc_handle *h, bool with_gauss) {
+	if(h->graph_exec) {
+		HIPCHK(h, hipGraphExecDestroy(h->graph_exec));
+		h->graph_exec = nullptr;
+	}
+	h->graph_valid = false;
+	if(!h->d_ggauss) HIPCHK(h, dalloc(&h->d_ggauss, (size_t) GRAPH_STEPS * std::max(h->ncells, 1)));
+	if(!h->d_step_base) HIPCHK(h, dalloc(&h->d_step_base, 1));
+	if(!h->h_step_base) HIPCHK(h, hipHostMalloc((void **) &h->h_step_base, sizeof(int) * 64, hipHostMallocDefault));
+	h->lists_valid = false;
+	h->capturing = true;
+	hipGraph_t graph = nullptr;
+	hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+	int rc = RC_OK;
+	if(e == hipSuccess) {
+		for(int s = 0; s < GRAPH_STEPS && rc == RC_OK; ++s)
+			rc = step_dry(h, with_gauss ? h->d_ggauss + (size_t) s * h->ncells : nullptr, s);
+		e = hipStreamEndCapture(h->stream, &graph);
+	}
+	h->capturing = false;
+	if(rc) return rc;
+	if(e != hipSuccess) return fail(h, RC_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+	e = hipGraphInstantiateWithFlags(&h->graph_exec, graph, 0);
+	(void) hipGraphDestroy(graph);
+	if(e != hipSuccess) return fail(h, RC_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+	h->graph_valid = true;
+	h->graph_gauss = with_gauss;
+	h->graph_set = h->cur_set;
+	return RC_OK;
+}
+
+extern "C" int rc_set_graphs(rc_handle *h, int enable) {
+	if(!h) return RC_ERR_ARG;
+	h->use_graphs = enable != 0;
+	return RC_OK;
+}
+
+extern "C" int rc_graph_stats(rc_handle *h, long long *graph_launches, long long *eager_steps) {
+	if(!h) return RC_ERR_ARG;
+	if(graph_launches) *graph_launches = h->graph_launches;
+	if(eager_steps) *eager_steps = h->eager_steps;
 	return RC_OK;
 }
 
@@ -1227,11 +1296,35 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 					hipMemcpyHostToDevice, h->stream));
 		}
 		int s = 0;
+		bool grown = false;
 		while(s < ns) {
 			bool halted = false;
+			int window = 0;
 			for(; s < ns; ++s) {
+				// graph replay of a whole window (dry model, steady parameter set, no timing)
+				if(h->use_graphs && !h->wet && !h->timing && !grown && ns - s >= GRAPH_STEPS && h->cf_set == h->cur_set &&
+						window < 64) {
+					if(!h->graph_valid || h->graph_gauss != (gauss != nullptr) || h->graph_set != h->cur_set) {
+						HIPCHK(h, hipStreamSynchronize(h->stream));
+						if((rc = capture_graph(h, gauss != nullptr))) return rc;
+					}
+					h->h_step_base[window] = s;
+					HIPCHK(h, hipMemcpyAsync(h->d_step_base, h->h_step_base + window, sizeof(int), hipMemcpyHostToDevice,
+							h->stream));
+					if(gauss)
+						HIPCHK(h, hipMemcpyAsync(h->d_ggauss, h->d_gauss + (size_t) s * h->ncells,
+								sizeof(double) * GRAPH_STEPS * h->ncells, hipMemcpyDeviceToDevice, h->stream));
+					HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+					h->graph_launches += 1;
+					h->lists_valid = false;
+					h->cf_set = h->cur_set;
+					++window;
+					s += GRAPH_STEPS - 1;
+					continue;
+				}
 				const double *g = gauss ? h->d_gauss + (size_t) s * h->ncells : nullptr;
 				rc = h->wet ? step_wet(h, g, s) : step_dry(h, g, s);
+				h->eager_steps += 1;
 				if(rc == -1) {
 					halted = true;
 					rc = RC_OK;
@@ -1251,6 +1344,7 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 				else
 					h->cf_set = h->cur_set;
 				s = st.step + 1;
+				grown = true;   // the rest of the chunk runs eagerly (its noise windows are misaligned)
 				continue;
 			}
 			if(st.flags) break;

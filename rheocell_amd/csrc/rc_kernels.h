@@ -15,6 +15,7 @@ struct StageArgs {
 	int cf_set, fs_set;          // parameter sets of the force evaluation / field update
 	int store;                   // 1: predictor (first_step(true)), 0: corrector
 	int step;                    // step index, recorded with a growth request
+	const int *step_base;        // added to step when set (graph replays: the window's first step)
 	int pure;                    // MODE_CF: no side effects on cells / Repulsion
 	int jq;                      // accumulate Repulsion and CIL sums (J_Q != 0)
 	int write_v;                 // MODE_FUSED: also write v rows

@@ -290,11 +290,12 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 	// to run (cells of other stage groups are launched after it); the host
 	// serves the growth once the stage is complete.  Correctors of later steps,
 	// already queued, halt.
+	const int step = a.step + (a.step_base ? *a.step_base : 0);
 	{
 		const unsigned f = a.st->flags;
 		if(f) {
 			const bool same_step = MODE != MODE_CF && !a.store && (f & ST_KIND_MASK) == ST_GROWTH &&
-					(f >> ST_STEP_SHIFT) == (unsigned) (a.step + 1);
+					(f >> ST_STEP_SHIFT) == (unsigned) (step + 1);
 			if(!same_step) return;
 		}
 	}
@@ -683,9 +684,9 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		else {
 			// patch growth (MultiPhaseField.cpp:248-310) is done by the host:
 			// flag it, with the step, and halt
-			if(atomicOr(&a.st->flags, ST_GROWTH | ((unsigned) (a.step + 1) << ST_STEP_SHIFT)) == 0u) {
+			if(atomicOr(&a.st->flags, ST_GROWTH | ((unsigned) (step + 1) << ST_STEP_SHIFT)) == 0u) {
 				a.st->first_cell = ci;
-				a.st->step = a.step;
+				a.st->step = step;
 			}
 		}
 	}

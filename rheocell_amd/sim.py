@@ -183,6 +183,15 @@ class Simulation:
                         "tune_us": {"%dx%d" % sv: round(us[8 * g + i], 2) for i, sv in enumerate(v)}})
         return out
 
+    def set_graphs(self, on: bool = True):
+        self._check(self.L.rc_set_graphs(self.h, 1 if on else 0))
+
+    def graph_stats(self) -> Tuple[int, int]:
+        """(graph replays, steps launched one by one)"""
+        g, e = C.c_longlong(), C.c_longlong()
+        self._check(self.L.rc_graph_stats(self.h, C.byref(g), C.byref(e)))
+        return g.value, e.value
+
     def cg_stats(self) -> _lib.rc_cg_stats_t:
         st = _lib.rc_cg_stats_t()
         self._check(self.L.rc_cg_stats(self.h, C.byref(st)))

@@ -227,3 +227,37 @@ def test_stage_autotuner_picks_a_timed_shape():
     assert times["%dx%d" % (g["threads"], g["rows"])] == min(times.values())
     with pytest.raises(RCexception):
         sim.set_stage_shape(256, 4)   # 4 rows per strip do not cover a 41-row patch with 256 threads
+
+
+@pytest.mark.parametrize("grow", [False, True], ids=["steady", "growth"])
+def test_graph_replays_match_eager_steps(grow):
+    # dry steps replayed from the 16-step HIP graph against the same steps
+    # launched one by one: identical fields after every chunk (several replays
+    # with host synchronisation between them); with growth the chunk falls
+    # back to ordinary launches after the growth is served
+    if grow:
+        rng = np.random.default_rng(21)
+        sides = [int(s) for s in rng.choice([29, 33, 41, 49], 9)]
+        radii = [float(r) for r in rng.uniform(6, 10, 9)]
+        conf = confgen.lattice_conf(96, 3, seed=100, radii=radii, sides=sides)
+        keys = dict(confgen.BASE_KEYS, seed=100)
+    else:
+        conf = confgen.lattice_conf(128, 4, seed=12)
+        keys = dict(confgen.BASE_KEYS, seed=12)
+    a = Simulation([(conf, keys)])
+    b = Simulation([(conf, keys)])
+    b.set_graphs(False)
+    for chunk in (20, 33, 16):
+        a.step(chunk)
+        b.step(chunk)
+        ca, pa = a.download()
+        cb, pb = b.download()
+        assert pa.shape == pb.shape and np.array_equal(pa, pb)
+        assert [(c.corner, c.LsubX, c.offset[0], c.offset[1]) for c in ca] == \
+               [(c.corner, c.LsubX, c.offset[0], c.offset[1]) for c in cb]
+    ga, ea = a.graph_stats()
+    gb, eb = b.graph_stats()
+    print("graph replays", ga, "eager steps", ea, "| reference run eager steps", eb)
+    # steps queued after a growth request halt and are launched again once it is served
+    assert gb == 0 and (eb == 69 if not grow else eb >= 69)
+    assert ga >= 1 and ga * 16 + ea >= 69
