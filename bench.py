@@ -47,6 +47,19 @@ BYTES = {
 }
 CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update")
 
+# HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
+# dispatch, mean over the launches that did work), scaled by what the same
+# counters read for a plain 256 MiB device copy (bench.py --probe-copy 256):
+# FETCH_SIZE 131085 KiB and WRITE_SIZE 262144 KiB for 262144 KiB moved each
+# way, i.e. reads x 2.0, writes x 1.0.  Recorded in profiles/.
+READ_SCALE, WRITE_SCALE = 262144.0 / 131085.0, 1.0
+TRAFFIC_KIB = {
+    ("wet512_1600", "cg_row_update"): (88125.3, 85192.0),
+    ("wet512_1600", "cg_gather_p"): (59543.8, 52540.9),
+    ("dry512_1024", "stage_fused_pred"): (10338.6, 27136.4),
+    ("dry512_1024", "stage_fused_corr"): (23884.0, 13712.0),
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -63,6 +76,8 @@ def parse():
                     help="learned warm-start weights (wet workloads; skipped when absent)")
     ap.add_argument("--cg-steps", type=int, default=20, help="steps of the learned-vs-previous CG comparison")
     ap.add_argument("--no-graphs", action="store_true", help="launch dry steps one by one instead of HIP graph replays")
+    ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
+                    help="only fill and copy a MIB-sized device buffer 3 times (counter calibration), then exit")
     return ap.parse_args()
 
 
@@ -135,8 +150,26 @@ def cg_compare(conf, keys, weights, warmup, steps):
     return out
 
 
+def probe_copy(mib):
+    """A plain fill + device-to-device copy of known size: the reference for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of the path's kernels."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.empty(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe_copy_bytes_read": n * 8, "probe_copy_bytes_written": n * 8, "copies": 3}))
+
+
 def main():
     args = parse()
+    if args.probe_copy:
+        probe_copy(args.probe_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -216,8 +249,10 @@ def main():
             useful = sweeps if best in CG_SWEEP_KINDS else d["launches"]
             avg_us = 1e3 * d["ms_total"] / d["launches"]
             achieved = bytes_launch * useful / (d["ms_total"] * 1e-3) / 1e9
+            tr = TRAFFIC_KIB.get((args.workload, best))
+            traffic = round(1024 * (tr[0] * READ_SCALE + tr[1] * WRITE_SCALE)) if tr else None
             roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                    "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                     "kernel": _lib.T_SYMBOL.get(best, best), "kind": best,
                     "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3),
                     "launches": d["launches"], "useful_launches": useful,

@@ -1114,11 +1114,12 @@ static int serve_growth(rc_handle *h) {
 	return RC_OK;
 }
 
-// Stage-shape autotuner: for each group, times the side-effect-free force
-// evaluation of its cells (pure MODE_CF on the current fields, force/velocity
-// rows diverted to scratch) with every shape that fits the group, 3 launches
-// each after a warm-up, and keeps the fastest.  Runs before the first step
-// after a layout change.
+// Stage-shape autotuner: for each group, times the predictor stage of its
+// cells (FUSED dry / FS wet) on the current fields with every shape that fits
+// the group, 3 launches each after a warm-up, and keeps the fastest.  The
+// timing launches are side-effect free: they write phi* and dphi rows (the
+// predictor's own outputs, recomputed by the next step) and no cell state or
+// Repulsion.  Runs before the first step after a layout change.
 static int autotune_stage(rc_handle *h) {
 	h->stage_tuned = true;
 	int rc;
@@ -1133,20 +1134,21 @@ static int autotune_stage(rc_handle *h) {
 		for(size_t i = 0; i < g.valid.size(); ++i) {
 			StageArgs a = stage_args(h);
 			a.cf_set = h->cf_set;
-			a.fs_set = h->cf_set;
-			a.phi_c = h->d_phiA;
+			a.fs_set = h->cur_set;
+			a.store = 1;
 			a.pure = 1;
-			if(h->wet) {
-				a.frows = h->d_r;
-				a.vrows = h->d_r;
-			}
+			a.write_v = 0;
+			a.phi_c = h->d_phiA;
+			a.phi_out = h->d_phiB;
+			a.gauss = nullptr;
 			a.threads = g.valid[i].first;
 			a.npt = g.valid[i].second;
 			a.cell_list = h->groups.size() > 1 ? g.d_cells : nullptr;
 			const int n = (int) g.cells.size();
-			HIPCHK(h, launch_stage(MODE_CF, a, n, g.lds, h->stream));
+			const int mode = h->wet ? MODE_FS : MODE_FUSED;
+			HIPCHK(h, launch_stage(mode, a, n, g.lds, h->stream));
 			HIPCHK(h, hipEventRecord(e0, h->stream));
-			for(int r = 0; r < 3; ++r) HIPCHK(h, launch_stage(MODE_CF, a, n, g.lds, h->stream));
+			for(int r = 0; r < 3; ++r) HIPCHK(h, launch_stage(mode, a, n, g.lds, h->stream));
 			HIPCHK(h, hipEventRecord(e1, h->stream));
 			HIPCHK(h, hipEventSynchronize(e1));
 			float ms = 0.f;

@@ -16,7 +16,8 @@ struct StageArgs {
 	int store;                   // 1: predictor (first_step(true)), 0: corrector
 	int step;                    // step index, recorded with a growth request
 	const int *step_base;        // added to step when set (graph replays: the window's first step)
-	int pure;                    // MODE_CF: no side effects on cells / Repulsion
+	int pure;                    // no side effects on cells / Repulsion (MODE_CF: materialise;
+	                             // predictor modes: shape tuner, outputs to the predictor's scratch rows)
 	int jq;                      // accumulate Repulsion and CIL sums (J_Q != 0)
 	int write_v;                 // MODE_FUSED: also write v rows
 	const double *phi_c;         // phi entering the force evaluation

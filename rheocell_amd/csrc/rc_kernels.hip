@@ -639,6 +639,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 	th = __shfl(th, 2, 64);
 	th_old = __shfl(th_old, 2, 64);
 	if(tid != 0) return;
+	if(a.pure) return;   // timing run (shape tuner): no cell state is written
 	DevCell &G = a.cells[ci];
 	if(a.store) {
 		G.CoM_oldx = C.CoMx;
