@@ -85,7 +85,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(conf, keys, steps):
+def cpu_baseline(conf, keys, steps, budget_s=15.0):
+    """At least `steps` Heun steps, more while the first one leaves room in a
+    ~budget_s window (a bounded sample of the same workload)."""
     from oracle import OracleSim  # the checker, timed here as the CPU port
     try:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
@@ -93,7 +95,12 @@ def cpu_baseline(conf, keys, steps):
         pass
     s = OracleSim(conf, keys, flavour="fast")
     t0 = time.perf_counter()
-    s.step(steps)
+    s.step(1)
+    t1 = time.perf_counter() - t0
+    more = max(steps - 1, min(int(budget_s / max(t1, 1e-6)) - 1, 2000))
+    if more > 0:
+        s.step(more)
+    steps = 1 + more
     dt = time.perf_counter() - t0
     cpu = "unknown"
     try:
@@ -184,7 +191,8 @@ def main():
     else:
         boxes = [confgen.workload(args.workload, seed=args.seed)]
     conf, keys = boxes[0]
-    keys["_name"] = args.workload
+    keys["_name"] = args.workload if len(boxes) == 1 else "%s replica 0 (%d cells, %d^2 box)" % (
+        args.workload, len(conf.cells), conf.Lx)
     ncells = sum(len(c.cells) for c, _ in boxes)
     L = conf.Lx
     box_sites = sum(c.Lx * c.Lx for c, _ in boxes)
@@ -296,6 +304,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "graphs": (lambda g: {"replays": g[0], "steps_launched_singly": g[1]})(sim.graph_stats()),
+        "growths": (lambda g: {"served": g[0], "host_ms": round(g[1], 1)})(sim.growth_stats()),
         "stage_groups": [{"cells": g["cells"], "threads": g["threads"], "rows": g["rows"], "tuner_us": g["tune_us"]}
                          for g in groups],
     }

@@ -185,6 +185,9 @@ int rc_stage_groups(rc_handle *h, int max_groups, int *ngroups, int (*shape)[3],
  * either way.  Default: enabled. */
 int rc_set_graphs(rc_handle *h, int enable);
 int rc_graph_stats(rc_handle *h, long long *graph_launches, long long *eager_steps);
+/* patch growths served inside rc_step (MultiPhaseField.cpp:248-310 on the
+ * host: download, re-cut, re-layout) and the host time they took */
+int rc_growth_stats(rc_handle *h, long long *growths, double *host_ms);
 
 /* ---- Learned CG warm start (no counterpart in the reference) ----------
  * A 6-layer 3x3 convolutional network (32 channels, circular padding, ReLU,

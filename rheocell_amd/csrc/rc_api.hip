@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -81,6 +82,7 @@ struct rc_handle {
 	// Field-update kernel groups: cells grouped by the smallest stage shape
 	// that fits their patch, one launch per group with its own shape and LDS.
 	struct StageGroup {
+		int key = 0;                             // index of the first fitting shape (the group's size class)
 		std::vector<int> cells;
 		int *d_cells = nullptr;
 		size_t lds = 0;
@@ -98,6 +100,8 @@ struct rc_handle {
 	int *d_step_base = nullptr;     // first step of the window being replayed
 	int *h_step_base = nullptr;     // pinned, one slot per window of a chunk
 	long long graph_launches = 0, eager_steps = 0;
+	long long growths = 0;          // growth requests served (host re-cut + re-layout)
+	double growth_ms = 0.;          // host time spent serving them
 	int stage_forced[2] = { 0, 0 };     // rc_set_stage_shape; {0, 0}: autotune
 	bool stage_tuned = false;
 	int cur_set = 0;              // parameter set currently applied
@@ -378,7 +382,10 @@ static void trig_table(int L, std::vector<double> &out) {
 
 // Lays out the rows of all cells (field_start_index order), the trig pool and
 // the cell table on the device.  rep: Repulsion rows to keep (J_Q != 0), or null.
-static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, const double *rep) {
+// phi / rep: rows in the new layout, host arrays, or device arrays when on_device
+static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, const double *rep,
+		bool on_device = false) {
+	const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
 	std::vector<DevCell> dc(h->ncells);
 	std::vector<double> trig;
 	std::map<int, int> trig_off;
@@ -472,7 +479,7 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 		}
 	}
 	if(h->jq) {
-		if(rep) HIPCHK(h, hipMemcpyAsync(h->d_rep, rep, sizeof(double) * rows, hipMemcpyHostToDevice, h->stream));
+		if(rep) HIPCHK(h, hipMemcpyAsync(h->d_rep, rep, sizeof(double) * rows, kind, h->stream));
 		else HIPCHK(h, hipMemsetAsync(h->d_rep, 0, sizeof(double) * rows, h->stream));
 	}
 	HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * rows, h->stream));
@@ -485,7 +492,7 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 		geo[i] = make_int4(dc[i].corner, dc[i].LX | (dc[i].LY << 16), dc[i].offx | (dc[i].offy << 16), dc[i].start);
 	HIPCHK(h, hipMemcpyAsync(h->d_geo, geo.data(), sizeof(int4) * h->ncells, hipMemcpyHostToDevice, h->stream));
 	h->lists_valid = false;
-	HIPCHK(h, hipMemcpyAsync(h->d_phiA, phi, sizeof(double) * rows, hipMemcpyHostToDevice, h->stream));
+	HIPCHK(h, hipMemcpyAsync(h->d_phiA, phi, sizeof(double) * rows, kind, h->stream));
 	HIPCHK(h, hipMemsetAsync(h->d_st, 0, sizeof(DevStatus), h->stream));
 	h->max_S = maxS;
 	h->max_W = maxW;
@@ -495,6 +502,10 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 		// group the cells by the first stage shape (table order) that fits their patch
 		int shapes[16][2];
 		const int n = stage_shapes(shapes);
+		// shapes chosen for the previous layout (kept where they still fit: no re-tuning)
+		std::map<int, std::pair<int, int>> prev;
+		const bool prev_tuned = h->stage_tuned && !h->groups.empty();
+		for(auto &g : h->groups) prev[g.key] = { g.threads, g.rows };
 		for(auto &g : h->groups) dfree(g.d_cells);
 		h->groups.assign(n, rc_handle::StageGroup());
 		for(int i = 0; i < h->ncells; ++i) {
@@ -505,6 +516,7 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			h->groups[k].cells.push_back(i);
 		}
 		std::vector<rc_handle::StageGroup> used;
+		bool all_kept = prev_tuned;
 		for(int k = 0; k < n; ++k) {
 			rc_handle::StageGroup &g = h->groups[k];
 			if(g.cells.empty()) continue;
@@ -521,13 +533,22 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 				for(int i : g.cells) ok = ok && stage_strip_rows(dc[i].LX, dc[i].LY, shapes[j][0]) <= shapes[j][1];
 				if(ok) g.valid.emplace_back(shapes[j][0], shapes[j][1]);
 			}
+			g.key = k;
 			g.threads = g.valid[0].first;
 			g.rows = g.valid[0].second;
-			for(const auto &v : g.valid)
+			bool kept = false;
+			for(const auto &v : g.valid) {
 				if(v.first == h->stage_forced[0] && v.second == h->stage_forced[1]) {
 					g.threads = v.first;
 					g.rows = v.second;
 				}
+				if(prev_tuned && prev.count(k) && prev[k] == v && h->stage_forced[0] == 0) {
+					g.threads = v.first;
+					g.rows = v.second;
+					kept = true;
+				}
+			}
+			if(!kept) all_kept = false;
 			HIPCHK(h, dalloc(&g.d_cells, g.cells.size()));
 			HIPCHK(h, hipMemcpy(g.d_cells, g.cells.data(), sizeof(int) * g.cells.size(), hipMemcpyHostToDevice));
 			used.push_back(std::move(g));
@@ -536,7 +557,7 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 		h->groups = std::move(used);
 		bool tunable = false;
 		for(const auto &g : h->groups) tunable = tunable || g.valid.size() > 1;
-		h->stage_tuned = !tunable || h->stage_forced[0] != 0;
+		h->stage_tuned = !tunable || h->stage_forced[0] != 0 || all_kept;
 		h->graph_valid = false;
 	}
 	// largest group LDS: patch, window / mu, y trig tables, Repulsion
@@ -1084,32 +1105,72 @@ static int grow_patch(rc_handle *h, rc_cell &c, const double *phi, std::vector<d
 // serves a growth request raised by the device: re-cuts the flagged patches,
 // re-lays out the rows and clears the halt
 static int serve_growth(rc_handle *h) {
-	std::vector<rc_cell> cells;
-	std::vector<double> phi, rep, nphi, nrep, tmp;
+	// cell state; the rows of the cells that grow; every other cell's rows are
+	// moved to their new offsets on the device (k_repack_rows)
+	std::vector<DevCell> dc(h->ncells);
+	HIPCHK(h, hipMemcpyAsync(dc.data(), h->d_cells, sizeof(DevCell) * h->ncells, hipMemcpyDeviceToHost, h->stream));
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	std::vector<rc_cell> cells(h->ncells);
+	for(int i = 0; i < h->ncells; ++i) to_rc_cell(dc[i], cells[i]);
+	std::vector<int4> map;
+	struct Up {
+		long long start;
+		std::vector<double> phi, rep;
+	};
+	std::vector<Up> ups;
+	long long nrows = 0;
 	int rc;
-	if((rc = download_raw(h, cells, phi, &rep))) return rc;
-	long long start = 0;
 	for(int b = 0; b < h->nboxes; ++b) {
 		const DevBox &B = h->dboxes[b];
 		for(int i = B.cell_begin; i < B.cell_end; ++i) {
 			rc_cell &c = cells[i];
 			const int S = c.LsubX * c.LsubY;
+			const long long old = dc[i].start;
 			if(c.x_sub_left >= BORDER && c.y_sub_bottom >= BORDER) {
-				nphi.insert(nphi.end(), phi.begin() + start, phi.begin() + start + S);
-				if(h->jq) nrep.insert(nrep.end(), rep.begin() + start, rep.begin() + start + S);
+				map.push_back(make_int4((int) old, (int) nrows, S, 0));
+				nrows += S;
+				continue;
 			}
-			else {
-				if((rc = grow_patch(h, c, phi.data() + start, tmp, B.L))) return rc;
-				nphi.insert(nphi.end(), tmp.begin(), tmp.end());
-				if(h->jq) {  // std::vector::resize keeps the first min(old, new) entries
-					const int nS = c.LsubX * c.LsubY;
-					for(int q = 0; q < nS; ++q) nrep.push_back(q < S ? rep[start + q] : 0.);
-				}
+			std::vector<double> ophi(S), orep;
+			HIPCHK(h, hipMemcpy(ophi.data(), h->d_phiA + old, sizeof(double) * S, hipMemcpyDeviceToHost));
+			if(h->jq) {
+				orep.resize(S);
+				HIPCHK(h, hipMemcpy(orep.data(), h->d_rep + old, sizeof(double) * S, hipMemcpyDeviceToHost));
 			}
-			start += S;
+			Up u;
+			u.start = nrows;
+			if((rc = grow_patch(h, c, ophi.data(), u.phi, B.L))) return rc;
+			const int nS = c.LsubX * c.LsubY;
+			if(h->jq) {  // std::vector::resize keeps the first min(old, new) entries
+				u.rep.assign(nS, 0.);
+				for(int q = 0; q < std::min(S, nS); ++q) u.rep[q] = orep[q];
+			}
+			nrows += nS;
+			ups.push_back(std::move(u));
 		}
 	}
-	if((rc = upload_layout(h, cells.data(), nphi.data(), h->jq ? nrep.data() : nullptr))) return rc;
+	double *d_nphi = nullptr, *d_nrep = nullptr;
+	int4 *d_map = nullptr;
+	HIPCHK(h, dalloc(&d_nphi, nrows));
+	if(h->jq) HIPCHK(h, dalloc(&d_nrep, nrows));
+	HIPCHK(h, dalloc(&d_map, map.size()));
+	HIPCHK(h, hipMemcpyAsync(d_map, map.data(), sizeof(int4) * map.size(), hipMemcpyHostToDevice, h->stream));
+	HIPCHK(h, launch_repack_rows(h->d_phiA, d_nphi, h->jq ? h->d_rep : nullptr, d_nrep, d_map, (int) map.size(),
+			h->stream));
+	for(const Up &u : ups) {
+		HIPCHK(h, hipMemcpyAsync(d_nphi + u.start, u.phi.data(), sizeof(double) * u.phi.size(), hipMemcpyHostToDevice,
+				h->stream));
+		if(h->jq)
+			HIPCHK(h, hipMemcpyAsync(d_nrep + u.start, u.rep.data(), sizeof(double) * u.rep.size(),
+					hipMemcpyHostToDevice, h->stream));
+	}
+	HIPCHK(h, hipStreamSynchronize(h->stream));
+	rc = upload_layout(h, cells.data(), d_nphi, d_nrep, true);
+	(void) hipStreamSynchronize(h->stream);
+	dfree(d_nphi);
+	dfree(d_nrep);
+	dfree(d_map);
+	if(rc) return rc;
 	h->h_st->flags = 0;
 	return RC_OK;
 }
@@ -1267,6 +1328,13 @@ extern "C" int rc_graph_stats(rc_handle *h, long long *graph_launches, long long
 	return RC_OK;
 }
 
+extern "C" int rc_growth_stats(rc_handle *h, long long *growths, double *host_ms) {
+	if(!h) return RC_ERR_ARG;
+	if(growths) *growths = h->growths;
+	if(host_ms) *host_ms = h->growth_ms;
+	return RC_OK;
+}
+
 extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 	if(!h) return RC_ERR_ARG;
 	if(!h->initialized) return fail(h, RC_ERR_STATE, "rc_step before rc_init_forces");
@@ -1339,7 +1407,12 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 			const DevStatus st = *h->h_st;
 			if((st.flags & ST_KIND_MASK) == ST_GROWTH) {
 				// the device halted after the corrector of step st.step
-				if((rc = serve_growth(h))) return rc;
+				{
+					const auto g0 = std::chrono::steady_clock::now();
+					if((rc = serve_growth(h))) return rc;
+					h->growths += 1;
+					h->growth_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
+				}
 				if(h->wet) {
 					if((rc = force_eval(h, h->d_phiA, false))) return rc;
 				}

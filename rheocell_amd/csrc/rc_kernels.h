@@ -105,6 +105,9 @@ hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s);
 hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap, int chan, const double *rn2, double rows,
 		int nsites, hipStream_t s);
 hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
+// map[j] = {old row start, new row start, rows}: dst[new + q] = src[old + q] (and src2 -> dst2 if set)
+hipError_t launch_repack_rows(const double *src, double *dst, const double *src2, double *dst2, const int4 *map, int n,
+		hipStream_t s);
 hipError_t stage_set_lds_limit(size_t bytes);
 // sites-per-thread instantiation for a largest patch of max_sites (-1: too large)
 // stage kernel shapes {threads per cell, rows per thread strip}; returns their count

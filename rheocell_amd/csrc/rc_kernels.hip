@@ -1267,6 +1267,25 @@ hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap,
 	return hipGetLastError();
 }
 
+// Row repack after a re-layout: for listed cell j, copy its map[j].z rows
+// from src + map[j].x to dst + map[j].y (and the same for src2 / dst2 when set).
+__global__ __launch_bounds__(256) void k_repack_rows(const double *__restrict__ src, double *__restrict__ dst,
+		const double *__restrict__ src2, double *__restrict__ dst2, const int4 *__restrict__ map, int n) {
+	const int j = blockIdx.x;
+	if(j >= n) return;
+	const int4 m = map[j];
+	for(int q = threadIdx.x; q < m.z; q += 256) {
+		dst[m.y + q] = src[m.x + q];
+		if(src2) dst2[m.y + q] = src2[m.x + q];
+	}
+}
+hipError_t launch_repack_rows(const double *src, double *dst, const double *src2, double *dst2, const int4 *map, int n,
+		hipStream_t s) {
+	if(n <= 0) return hipSuccess;
+	RC_LAUNCH(k_repack_rows, dim3(n), dim3(256), 0, s, src, dst, src2, dst2, map, n);
+	return hipGetLastError();
+}
+
 __global__ void k_zero_rows(double2 *x, long long rows, int mask) {
 	const long long i = (long long) blockIdx.x * 256 + threadIdx.x;
 	if(i >= rows) return;

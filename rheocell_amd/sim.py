@@ -192,6 +192,12 @@ class Simulation:
         self._check(self.L.rc_graph_stats(self.h, C.byref(g), C.byref(e)))
         return g.value, e.value
 
+    def growth_stats(self) -> Tuple[int, float]:
+        """(patch growths served, host milliseconds spent on them)"""
+        n, ms = C.c_longlong(), C.c_double()
+        self._check(self.L.rc_growth_stats(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
     def cg_stats(self) -> _lib.rc_cg_stats_t:
         st = _lib.rc_cg_stats_t()
         self._check(self.L.rc_cg_stats(self.h, C.byref(st)))
