@@ -674,6 +674,7 @@ static CGArgs cg_args(rc_handle *h) {
 	const int mi = h->boxes[0].params.cg_max_iter;
 	c.max_iter = mi > 0 ? mi : (int) std::min<long long>(2 * h->rows, 1ll << 30);
 	c.ncells = h->ncells;
+	c.nbox_blk = (int) ((h->box_sites + 255) / 256);
 	c.st = h->d_st;
 	return c;
 }
@@ -818,7 +819,6 @@ static int cg_solve(rc_handle *h, bool cold) {
 		for(;;) {
 			for(int it = 0; it < batch; ++it) {
 				{ TimeScope t(h, RC_T_CG_BOX_PAP); HIPCHK(h, launch_cg_box(1, c, (int) h->box_sites, h->stream)); }
-				{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(1, c, nbox_blk, h->stream)); }
 				{ TimeScope t(h, RC_T_CG_ROW_UPDATE); HIPCHK(h, launch_cg_row(CG_ROW_UPDATE, c, h->stream)); }
 				{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(2, c, h->ncells, h->stream)); }
 				{ TimeScope t(h, RC_T_CG_GATHER_P); HIPCHK(h, launch_cg_gather(CG_GATHER_P_UPDATE, c, h->ntiles, h->stream)); }

@@ -430,7 +430,12 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 				const double dx = .5 * (pE - pW);
 				const double dy = .5 * (pN - pS);
 				if(MODE == MODE_CF && P.beta == 0) per2[0] += dx * dx + dy * dy;
-				const double p2 = stage_win ? s_win[w] : phi2b[by * L];
+				// Both loads always issue and the value is selected: a select of the
+				// two pointers would become one generic-address (flat) load.  The
+				// window slot read without staging is in range and discarded.
+				const double p2g = phi2b[by * L];
+				const double p2w = s_win[w];
+				const double p2 = stage_win ? p2w : p2g;
 				const double CH = P.gamma * (div_by<RCP>(8 * ph * (1 - ph) * (1 - 2 * ph), P.lambda, P.lambda_rcp) - lam2 * lap);
 				const double A = Acoef * ph;
 				const double Rep = Rcoef * ph * (p2 - ph * ph);
@@ -876,13 +881,14 @@ __device__ void cg_after_update(CGState *cg, const double *tot) {
 	cg->active = !(cg->done[0] && cg->done[1]);
 }
 
-// one workgroup of 1024: sums SoA partials of the previous kernel(s) in a fixed order
-template <int NV>
+// sums SoA partials of the previous kernel(s) over a block of NT threads in a
+// fixed order (deterministic, the same in every workgroup); every thread gets tot
+template <int NV, int NT = 1024>
 __device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[NV], double *s_red) {
 	double acc[NV];
 #pragma unroll
 	for(int j = 0; j < NV; ++j) acc[j] = 0.;
-	for(int b = threadIdx.x; b < nblk; b += 1024) {
+	for(int b = threadIdx.x; b < nblk; b += NT) {
 #pragma unroll
 		for(int j = 0; j < NV; ++j) acc[j] += part[j * nblk + b];
 	}
@@ -897,7 +903,7 @@ __device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[
 #pragma unroll
 	for(int j = 0; j < NV; ++j) {
 		double t = 0.;
-		for(int w = 0; w < 16; ++w) t += s_red[w * NV + j];
+		for(int w = 0; w < NT / 64; ++w) t += s_red[w * NV + j];
 		tot[j] = t;
 	}
 	__syncthreads();
@@ -1149,8 +1155,17 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 	double al0 = 0., al1 = 0.;
 	bool dn0 = false, dn1 = false;
 	if(MODE == CG_ROW_UPDATE) {
-		al0 = cg->alpha[0]; al1 = cg->alpha[1];
+		// alpha = absNew / p.Ap (cg_after_pap), every workgroup summing the box
+		// pass's partials in the same fixed order: no separate reduction launch
+		double pap[2];
+		reduce_parts<2, 256>(a.part_box, a.nbox_blk, pap, s_red);
 		dn0 = cg->done[0]; dn1 = cg->done[1];
+		if(!dn0) al0 = cg->absNew[0] / pap[0];
+		if(!dn1) al1 = cg->absNew[1] / pap[1];
+		if(ci == 0 && tid == 0) {
+			if(!dn0) cg->alpha[0] = al0;
+			if(!dn1) cg->alpha[1] = al1;
+		}
 	}
 	double red[6] = { 0., 0., 0., 0., 0., 0. };
 	for(int q = tid; q < S; q += 256) {
