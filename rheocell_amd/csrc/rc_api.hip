@@ -64,6 +64,7 @@ struct rc_handle {
 	double *d_trig = nullptr;
 	double *d_phi2 = nullptr, *d_diag = nullptr;
 	int *d_cover = nullptr, *d_tile_cells = nullptr, *d_tile_count = nullptr;
+	int4 *d_tile_geo = nullptr;
 	double2 *d_sp = nullptr, *d_p2 = nullptr, *d_n4 = nullptr;
 	double *d_part_box = nullptr;
 	double *d_part = nullptr;
@@ -313,6 +314,7 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 	HIPCHK(h, dalloc(&h->d_cells, cells));
 	HIPCHK(h, dalloc(&h->d_geo, cells));
 	HIPCHK(h, dalloc(&h->d_tile_cells, (size_t) tiles * MAXT));
+	HIPCHK(h, dalloc(&h->d_tile_geo, (size_t) tiles * MAXT));
 	HIPCHK(h, dalloc(&h->d_tile_count, tiles));
 	HIPCHK(h, dalloc(&h->d_phi2, sites));
 	HIPCHK(h, dalloc(&h->d_cover, sites));
@@ -350,7 +352,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_boxes); dfree(h->d_tile_box); dfree(h->d_cells); dfree(h->d_geo); dfree(h->d_params);
 	dfree(h->d_phiA); dfree(h->d_phiB); dfree(h->d_dphi); dfree(h->d_rep);
 	dfree(h->d_v); dfree(h->d_f); dfree(h->d_r); dfree(h->d_p); dfree(h->d_trig);
-	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_count);
+	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_geo); dfree(h->d_tile_count);
 	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
 	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xprev); dfree(h->d_rprev);
@@ -614,6 +616,7 @@ static int run_gather(rc_handle *h, const double *phi) {
 	g.phi2 = h->d_phi2;
 	g.cover = h->d_cover;
 	g.tile_cells = h->d_tile_cells;
+	g.tile_geo = h->d_tile_geo;
 	g.tile_count = h->d_tile_count;
 	g.build_lists = h->lists_valid ? 0 : 1;
 	g.st = h->d_st;
@@ -657,6 +660,7 @@ static CGArgs cg_args(rc_handle *h) {
 	c.params = h->d_params;
 	c.set = h->cur_set;
 	c.tile_cells = h->d_tile_cells;
+	c.tile_geo = h->d_tile_geo;
 	c.tile_count = h->d_tile_count;
 	c.cover = h->d_cover;
 	c.diag = h->d_diag;

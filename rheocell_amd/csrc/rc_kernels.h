@@ -47,6 +47,7 @@ struct GatherArgs {
 	double *phi2;
 	int *cover;
 	int *tile_cells;             // cells intersecting each tile, cell order
+	int4 *tile_geo;              // their geometry (as geo), written with the list
 	int *tile_count;
 	int build_lists;             // 1: rebuild the lists from geo, 0: reuse them
 	DevStatus *st;
@@ -63,6 +64,7 @@ struct CGArgs {
 	const DevParams *params;     // wet parameters (params[2*0 + set])
 	int set;
 	const int *tile_cells;
+	const int4 *tile_geo;
 	const int *tile_count;
 	const int *cover;
 	double *diag;                // D(k) = xi + eta * sum of the 4 neighbour covers

@@ -155,7 +155,7 @@ __device__ int tile_list(const GatherArgs &a, int tile, const DevBox &B, int x0,
 	if(!build) {
 		const int n = a.tile_count[tile];
 		for(int j = tid; j < n; j += 256)
-			unpack_geo(a.geo[a.tile_cells[tile * MAXT + j]], L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
+			unpack_geo(a.tile_geo[tile * MAXT + j], L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
 		__syncthreads();
 		return n;
 	}
@@ -180,6 +180,7 @@ __device__ int tile_list(const GatherArgs &a, int tile, const DevBox &B, int x0,
 			s_cx[pos] = cx; s_cy[pos] = cy; s_lx[pos] = lx; s_ly[pos] = ly;
 			s_ox[pos] = ox; s_oy[pos] = oy; s_start[pos] = st;
 			a.tile_cells[tile * MAXT + pos] = c;
+			a.tile_geo[tile * MAXT + pos] = a.geo[c];
 		}
 		__syncthreads();
 		if(tid == 0) *s_n += s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
@@ -204,7 +205,8 @@ __device__ inline int covering_row(int x, int y, int L, int cx, int cy, int LXj,
 	return st + qy * LXj + qx;
 }
 
-constexpr int GCHUNK = 8;   // covering patches examined per batch of independent loads
+constexpr int GCHUNK = 8;   // covering patches examined per batch of independent loads (CG gathers)
+constexpr int GCHUNK_PHI = 16;   // the same for the sum phi^2 gather (one batch at typical overlaps)
 
 // ---------------------------------------------------------------------------
 // k_box_gather: phi2[k] = sum over covering patches (cell order) of phi^2,
@@ -231,18 +233,18 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 	const int x = x0 + sx, y = y0 + sy;
 	double acc = 0.0;
 	int cnt = 0;
-	for(int base = 0; base < n; base += GCHUNK) {
-		int row[GCHUNK];
-		double v[GCHUNK];
+	for(int base = 0; base < n; base += GCHUNK_PHI) {
+		int row[GCHUNK_PHI];
+		double v[GCHUNK_PHI];
 #pragma unroll
-		for(int k = 0; k < GCHUNK; ++k) {
+		for(int k = 0; k < GCHUNK_PHI; ++k) {
 			const int j = base + k;
 			row[k] = j < n ? covering_row(x, y, L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]) : -1;
 		}
 #pragma unroll
-		for(int k = 0; k < GCHUNK; ++k) v[k] = row[k] >= 0 ? a.phi[row[k]] : 0.;
+		for(int k = 0; k < GCHUNK_PHI; ++k) v[k] = row[k] >= 0 ? a.phi[row[k]] : 0.;
 #pragma unroll
-		for(int k = 0; k < GCHUNK; ++k)
+		for(int k = 0; k < GCHUNK_PHI; ++k)
 			if(row[k] >= 0) {
 				acc += v[k] * v[k];
 				++cnt;
@@ -976,7 +978,7 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 	const int tw = min(TW, L - x0), th = min(TH, L - y0);
 	const int n = a.tile_count[tile];
 	for(int j = tid; j < n; j += 256)
-		unpack_geo(a.geo[a.tile_cells[tile * MAXT + j]], L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
+		unpack_geo(a.tile_geo[tile * MAXT + j], L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
 	__syncthreads();
 	const int sx = tid % TW, sy = tid / TW;
 	if(MODE == CG_GATHER_E) {
