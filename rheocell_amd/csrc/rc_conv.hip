@@ -52,7 +52,11 @@ constexpr int CONV_BW = 16 * CONV_TILES;           // block width in pixels
 constexpr int CONV_SW = CONV_BW + 2;               // staged columns
 constexpr int CONV_SH = CONV_ROWS + 2;             // staged rows
 
-__device__ inline unsigned pack2bf(float a, float b) { return (unsigned) f2bf(a) | ((unsigned) f2bf(b) << 16); }
+// two floats to packed bf16, rounded to nearest even by v_cvt_pk_bf16_f32
+__device__ inline unsigned pack2bf(float a, float b) {
+	return (unsigned) __builtin_bit_cast(unsigned short, (__bf16) a) |
+			((unsigned) __builtin_bit_cast(unsigned short, (__bf16) b) << 16);
+}
 // element offset of 16-byte chunk c (0..3) of LDS row q
 __device__ inline int swz(int q, int c) { return q * 32 + ((c ^ ((q >> 2) & 3)) << 3); }
 
@@ -68,8 +72,15 @@ __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restric
 	// stage: CONV_SH rows x CONV_SW columns x 4 chunks of 8 channels
 	for(int e = threadIdx.x; e < CONV_SH * CONV_SW * 4; e += 256) {
 		const int ch = e & 3, q = e >> 2, col = q % CONV_SW, r = q / CONV_SW;
-		// circular; columns past W of a partial block wrap too (staged, never read)
-		const int xs = (x0 + col - 1 + W) % W, ys = (y0 + r - 1 + H) % H;
+		// circular (operands lie in [-1, 2W) and [-1, 2H) as W >= 32 >= 16 CONV_TILES / 2 and
+		// H >= 3); columns past W of a partial block wrap too (staged, never read)
+		int xs = x0 + col - 1, ys = y0 + r - 1;
+		xs += xs < 0 ? W : 0;
+		xs -= xs >= W ? W : 0;
+		xs -= xs >= W ? W : 0;
+		ys += ys < 0 ? H : 0;
+		ys -= ys >= H ? H : 0;
+		ys -= ys >= H ? H : 0;
 		const uint4 v = *reinterpret_cast<const uint4 *>(in + (img + (size_t) ys * W + xs) * 32 + ch * 8);
 		*reinterpret_cast<uint4 *>(&s_in[swz(q, ch)]) = v;
 	}
@@ -140,7 +151,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restric
 
 hipError_t launch_conv3x3(const unsigned short *in, const unsigned short *w, const float *bias, unsigned short *out,
 		const unsigned short *mask, float *fout, int N, int H, int W, int mode, hipStream_t s) {
-	if(W % 16 || H < 1 || W < 1) return hipErrorInvalidValue;
+	if(W % 16 || W < 32 || H < 3) return hipErrorInvalidValue;   // the staging wraps assume these
 	const int blocks = ((W + CONV_BW - 1) / CONV_BW) * ((H + CONV_ROWS - 1) / CONV_ROWS);
 	hipLaunchKernelGGL(k_conv3x3, dim3(blocks, N), dim3(256), 0, s, in, w, bias, out, mask, fout, H, W, mode);
 	return hipGetLastError();
