@@ -50,7 +50,7 @@ struct rc_handle {
 	long long box_sites = 0;
 	int max_S = 0, max_W = 0;
 	size_t lds_stage = 0, lds_cg = 0;
-	bool wet = false, jq = false;
+	bool wet = false, jq = false, win = false;   // win: some omega != 0 (adhesion reads the box window)
 
 	// device memory
 	hipStream_t stream = nullptr;
@@ -86,7 +86,7 @@ struct rc_handle {
 		int key = 0;                             // index of the first fitting shape (the group's size class)
 		std::vector<int> cells;
 		int *d_cells = nullptr;
-		size_t lds = 0;
+		int mx = 0, my = 0;                      // largest patch sides of the group
 		std::vector<std::pair<int, int>> valid;   // shapes that fit every patch of the group
 		int threads = 256, rows = 7;             // shape in use
 		double tune_us[8] = {};                  // tuner: mean pure force-evaluation time per shape
@@ -233,6 +233,8 @@ static DevParams make_params(const rc_model_params &m, bool active) {
 	p.sqrt_dt = std::sqrt(m.dt);
 	p.anchoring = m.anchoring;
 	p.lambda_rcp = exact_rcp(m.lambda);
+	p.mu_a0 = p.mu / p.a0;              // the grouped quotients of WetPolarModel.cpp:278-291
+	p.rcoef = 4 * (p.kappa / p.lambda);
 	if(active) {
 		p.zetaQ_self = m.zetaQ_self;
 		p.zetaQ_inter = m.zetaQ_inter;
@@ -250,6 +252,7 @@ static DevParams make_params(const rc_model_params &m, bool active) {
 		p.beta = 0.;
 	}
 	p.friction_rcp = exact_rcp(p.friction);
+	p.pcoef = 4 * (p.beta / p.p0);
 	return p;
 }
 
@@ -294,6 +297,7 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		sites += (long long) d.L * d.L;
 		if(d.params.friction_cell != 0) h->wet = true;
 		if(d.params.J_Q != 0) h->jq = true;
+		if(d.params.omega != 0) h->win = true;
 	}
 	if(h->wet && nboxes != 1) {
 		*out = h;
@@ -392,7 +396,7 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 	std::vector<double> trig;
 	std::map<int, int> trig_off;
 	long long rows = 0;
-	int maxS = 0, maxW = 0, maxT = 0;
+	int maxS = 0, maxW = 0;
 	std::vector<std::pair<int, int>> dims;
 	for(int b = 0; b < h->nboxes; ++b) {
 		const DevBox &B = h->dboxes[b];
@@ -445,7 +449,6 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			rows += d.S;
 			maxS = std::max(maxS, d.S);
 			maxW = std::max(maxW, (d.LX + 2) * (d.LY + 2));
-			maxT = std::max(maxT, 2 * d.LY);
 			dims.emplace_back(d.LX, d.LY);
 		}
 	}
@@ -522,19 +525,17 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 		for(int k = 0; k < n; ++k) {
 			rc_handle::StageGroup &g = h->groups[k];
 			if(g.cells.empty()) continue;
-			int gS = 0, gW = 0, gT = 0;
 			for(int i : g.cells) {
-				gS = std::max(gS, dc[i].S);
-				gW = std::max(gW, (dc[i].LX + 2) * (dc[i].LY + 2));
-				gT = std::max(gT, 2 * dc[i].LY);
+				g.mx = std::max(g.mx, dc[i].LX);
+				g.my = std::max(g.my, dc[i].LY);
 			}
-			// patch, window / mu, y trig tables, Repulsion
-			g.lds = sizeof(double) * (size_t) (gS + gW + gT + (h->jq ? gS : 0));
 			for(int j = k; j < n; ++j) {
-				bool ok = true;
+				bool ok = stage_lds_bytes(g.mx, g.my, shapes[j][1], h->jq, h->win) <= STAGE_LDS_MAX;
 				for(int i : g.cells) ok = ok && stage_strip_rows(dc[i].LX, dc[i].LY, shapes[j][0]) <= shapes[j][1];
 				if(ok) g.valid.emplace_back(shapes[j][0], shapes[j][1]);
 			}
+			if(g.valid.empty())
+				return fail(h, RC_ERR_ARG, "a %dx%d patch needs more LDS than the stage kernel has", g.mx, g.my);
 			g.key = k;
 			g.threads = g.valid[0].first;
 			g.rows = g.valid[0].second;
@@ -562,10 +563,11 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 		h->stage_tuned = !tunable || h->stage_forced[0] != 0 || all_kept;
 		h->graph_valid = false;
 	}
-	// largest group LDS: patch, window / mu, y trig tables, Repulsion
-	h->lds_stage = sizeof(double) * (size_t) (maxS + maxW + maxT + (h->jq ? maxS : 0));
+	// largest stage LDS over the groups' valid shapes
+	h->lds_stage = 0;
+	for(const auto &g : h->groups)
+		for(const auto &v : g.valid) h->lds_stage = std::max(h->lds_stage, stage_lds_bytes(g.mx, g.my, v.second, h->jq, h->win));
 	h->lds_cg = sizeof(double2) * (size_t) maxW;
-	if(h->lds_stage > 160 * 1024) return fail(h, RC_ERR_ARG, "patch of %d sites needs %zu B of LDS", maxS, h->lds_stage);
 	HIPCHK(h, stage_set_lds_limit(std::max<size_t>(h->lds_stage, 64 * 1024)));
 	HIPCHK(h, hipStreamSynchronize(h->stream));
 	return RC_OK;
@@ -593,6 +595,7 @@ static StageArgs stage_args(rc_handle *h) {
 	a.frows = h->d_f;
 	a.rep = h->d_rep;
 	a.jq = h->jq ? 1 : 0;
+	a.win = h->win ? 1 : 0;
 	a.dphi_old = h->d_dphi;
 	a.geo = h->d_geo;
 	a.threads = 256;
@@ -633,7 +636,7 @@ static int launch_groups(rc_handle *h, int mode, const StageArgs &a0) {
 		a.threads = g.threads;
 		a.npt = g.rows;
 		a.cell_list = h->groups.size() > 1 ? g.d_cells : nullptr;
-		HIPCHK(h, launch_stage(mode, a, (int) g.cells.size(), g.lds, h->stream));
+		HIPCHK(h, launch_stage(mode, a, (int) g.cells.size(), stage_lds_bytes(g.mx, g.my, a.npt, h->jq, h->win), h->stream));
 	}
 	return RC_OK;
 }
@@ -1211,9 +1214,10 @@ static int autotune_stage(rc_handle *h) {
 			a.cell_list = h->groups.size() > 1 ? g.d_cells : nullptr;
 			const int n = (int) g.cells.size();
 			const int mode = h->wet ? MODE_FS : MODE_FUSED;
-			HIPCHK(h, launch_stage(mode, a, n, g.lds, h->stream));
+			const size_t lds = stage_lds_bytes(g.mx, g.my, a.npt, h->jq, h->win);
+			HIPCHK(h, launch_stage(mode, a, n, lds, h->stream));
 			HIPCHK(h, hipEventRecord(e0, h->stream));
-			for(int r = 0; r < 3; ++r) HIPCHK(h, launch_stage(mode, a, n, g.lds, h->stream));
+			for(int r = 0; r < 3; ++r) HIPCHK(h, launch_stage(mode, a, n, lds, h->stream));
 			HIPCHK(h, hipEventRecord(e1, h->stream));
 			HIPCHK(h, hipEventSynchronize(e1));
 			float ms = 0.f;

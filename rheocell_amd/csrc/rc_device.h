@@ -20,7 +20,6 @@ constexpr int MAXT = 128;        // cells intersecting one tile (bounded by 20 x
 constexpr int STORE_MAX = 20;    // WetPolarModel.cpp:51
 constexpr int BORDER = 4;        // BaseField.cpp:5
 constexpr int CELL_THREADS = 256;
-constexpr int NRED = 11;         // per-cell reductions of the field update
 
 // status flags written by kernels; every kernel returns at once when set
 constexpr unsigned ST_OVERLAP = 1u;
@@ -36,6 +35,7 @@ struct DevParams {
 	double gamma, lambda, omega, mu, kappa, a0, p0, beta, passive_alpha;
 	double zetaQ_self, zetaQ_inter, J_Q, friction, friction_cell, J0, dt, sqrt_dt;
 	double lambda_rcp, friction_rcp;   // exact 1/lambda, 1/friction when a power of two, else 0
+	double mu_a0, rcoef, pcoef;        // mu / a0, 4 (kappa / lambda), 4 (beta / p0), formed on the host
 	int anchoring, pad;
 };
 

@@ -35,6 +35,7 @@ struct StageArgs {
 	int threads, npt;            // stage shape: threads per cell, rows per thread strip (>= every patch's)
 	const int *cell_list;        // cell of each workgroup (null: workgroup i takes cell i)
 	int rcp;                     // lambda and every friction are powers of two (DevParams *_rcp)
+	int win;                     // LDS holds the box window of sum phi^2 (some omega != 0)
 	DevStatus *st;
 };
 
@@ -91,6 +92,18 @@ struct CGArgs {
 };
 
 hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s);
+// Dynamic LDS of the stage kernel for patches up to LX x LY and strips of npt
+// rows: {phi, mu} tile of LY + npt rows of LX + 2 columns (periodic columns on
+// both sides, periodic images of the first rows below), per-row trig and box
+// offsets, the Repulsion tile (jq) and the box window of sum phi^2 (win).
+constexpr size_t STAGE_LDS_MAX = 160 * 1024 - 4096;   // LDS per CU less the kernel's static arrays
+inline size_t stage_lds_bytes(int LX, int LY, int npt, bool jq, bool win) {
+	const size_t tr = (size_t) LY + npt, pw = (size_t) LX + 2;
+	size_t b = 16 * tr * pw + 16 * tr + 16 * tr;
+	if(jq) b += 8 * ((tr * pw + 1) & ~(size_t) 1);
+	if(win) b += 8 * pw * ((size_t) LY + 2);
+	return b;
+}
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds_bytes, hipStream_t s);
 hipError_t launch_cg_diag(const CGArgs &a, int nbox_sites, hipStream_t s);
 hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s);

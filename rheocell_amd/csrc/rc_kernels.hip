@@ -82,6 +82,7 @@ __device__ inline double readlane_f64(double v, int lane) {
 	return __builtin_bit_cast(double, ((unsigned long long) hi << 32) | lo);
 }
 constexpr int DPP_QUAD_1032 = 0xB1, DPP_QUAD_2301 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141, DPP_ROW_MIRROR = 0x140;
+constexpr int DPP_ROW_ROR4 = 0x124, DPP_ROW_ROR8 = 0x128;   // rotate right within a row of 16 lanes
 // sum over the 64 lanes, the same value in every lane
 __device__ inline double wave_sum(double v) {
 	v += dpp_f64<DPP_QUAD_1032>(v);
@@ -268,10 +269,12 @@ hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Column strips: a cell's LX x LY patch (ring-buffer coordinates, storage
 // q = qy * LX + qx) is covered by nb = CT / LX row blocks of R = ceil(LY / nb)
-// rows, CT the threads per cell; thread t takes column qx = t % LX of block t / LX.
-// Column-dependent quantities (E/W neighbour columns, the box-window column,
-// the x trig table entries) are then per-thread constants, the N/S neighbours
-// of a run of rows come from registers, and no index is divided per site.
+// rows, CT the threads per cell; thread t takes column qx = t % LX of block
+// t / LX.  Every thread of a used block evaluates R rows: the last block's rows
+// past LY are the periodic images of rows 0, 1, ... (read from their wrapped
+// storage rows, evaluated like any other row and then discarded), so no row of
+// the unrolled loops carries a per-thread guard.  Threads of unused blocks
+// evaluate rows 0 .. R-1 of their column and write nothing.
 __device__ __host__ inline int strip_rows(int LX, int LY, int CT) {
 	const int nb = CT / LX;
 	return nb > 0 ? (LY + nb - 1) / nb : 1 << 20;
@@ -282,9 +285,49 @@ __device__ __host__ inline int strip_rows(int LX, int LY, int CT) {
 template <bool RCP>
 __device__ inline double div_by(double x, double y, double rcp) { return RCP ? x * rcp : x / y; }
 
+// Sum of 8 per-lane values over a workgroup of NW waves.  A transposing
+// butterfly inside each row of 16 lanes (xor 1, xor 2, then rotations by 4
+// and 8; each exchange hands over the half of the values the partner keeps)
+// leaves in lane l the row's sum of value 4*b0 + 2*b1 + b2 (b = bits of l);
+// lanes 0-7 of every row store the row sums, and threads 0..7 add the 4*NW
+// row sums of their value in a fixed order.  Returns the total of value
+// threadIdx.x in threads 0..7.  Every lane must be active.
+template <int NW>
+__device__ inline double block_sum8(const double (&v)[8], double *s_part) {
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4;
+	double a4[4], a2[2];
+#pragma unroll
+	for(int i = 0; i < 4; ++i) {
+		const double send = b0 ? v[i] : v[i + 4], keep = b0 ? v[i + 4] : v[i];
+		a4[i] = keep + dpp_f64<DPP_QUAD_1032>(send);
+	}
+#pragma unroll
+	for(int i = 0; i < 2; ++i) {
+		const double send = b1 ? a4[i] : a4[i + 2], keep = b1 ? a4[i + 2] : a4[i];
+		a2[i] = keep + dpp_f64<DPP_QUAD_2301>(send);
+	}
+	double s;
+	{
+		const double send = b2 ? a2[0] : a2[1], keep = b2 ? a2[1] : a2[0];
+		s = keep + dpp_f64<DPP_ROW_ROR4>(send);
+	}
+	s += dpp_f64<DPP_ROW_ROR8>(s);
+	if((lane & 8) == 0) s_part[(wave * 4 + (lane >> 4)) * 8 + (b0 ? 4 : 0) + (b1 ? 2 : 0) + (b2 ? 1 : 0)] = s;
+	__syncthreads();
+	double t = 0.;
+	if(threadIdx.x < 8) {
+		t = s_part[threadIdx.x];
+#pragma unroll
+		for(int r = 1; r < 4 * NW; ++r) t += s_part[r * 8 + threadIdx.x];
+	}
+	return t;
+}
+
 // CT threads per cell; NPT >= the strip rows of every patch (compile time), so
 // the per-site registers are fully unrolled.  RCP: lambda and the friction of
 // every parameter set are powers of two (divisions become exact multiplications).
+// LDS layout: stage_lds_bytes (rc_kernels.h).
 template <int MODE, int NPT, bool RCP, int CT>
 __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 	// Any status flag halts the device, except that a growth request raised
@@ -302,42 +345,79 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		}
 	}
 	constexpr int NW = CT / 64;
-	extern __shared__ double smem[];
-	__shared__ double s_red[NW * NRED];
+	extern __shared__ double2 smem2[];
+	__shared__ double s_red[NW * 4 * 8];
 	__shared__ int s_redi[2 * NW];
 	__shared__ double s_bc;
 	const int ci = a.cell_list ? a.cell_list[blockIdx.x] : blockIdx.x, tid = threadIdx.x;
 	const DevCell C = a.cells[ci];
 	const DevBox B = a.boxes[C.box];
 	const DevParams P = a.params[2 * C.box + a.cf_set];
-	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L, WX = LX + 2, WY = LY + 2;
-	double *s_phi = smem;          // patch of phi (storage order)
-	double *s_win = smem + S;      // box window of sum phi^2, then mu in storage order (aliased)
-	double *s_mu = s_win;
-	double *s_trig = s_win + WX * WY;   // cos y, sin y (LY each)
-	double *s_rep = s_trig + 2 * LY;
+	const int LX = C.LX, LY = C.LY, L = B.L, PW = LX + 2, WX = LX + 2, WY = LY + 2;
+	const int TR = LY + NPT;                 // tile rows: LY storage rows and their periodic images
+	double2 *s_pm = smem2;                   // {phi, mu}: row qy at s_pm + qy * PW, column qx at + qx + 1
+	double2 *s_trow = s_pm + TR * PW;        // {cos y, sin y} of each tile row's box row
+	int4 *s_rowi = (int4 *) (s_trow + TR);   // {by * L, (ly + 1) * WX, ly, 0}
+	double *s_rep = (double *) (s_rowi + TR);               // Repulsion tile (jq), as s_pm
 	const bool do_rep = a.jq && MODE != MODE_FS && !a.pure;
 	const bool fs = MODE != MODE_CF;
+	const bool stage_win = a.win && P.omega != 0;
+	// box window of sum phi^2 (omega != 0); without it, an in-range stand-in
+	// for the discarded window load below
+	double *s_win = stage_win ? s_rep + (a.jq ? ((TR * PW + 1) & ~1) : 0) : (double *) smem2;
 
-	// this thread's strip: column qx, rows qy0 .. qy0 + nr - 1
+	// this thread's strip: column qx, rows qy0 .. qy0 + R - 1, the first nr of them its own
 	const int R = strip_rows(LX, LY, CT);
 	const int qx = tid % LX, blk = tid / LX;
-	const int qy0 = blk * R;
-	const int nr = max(0, min(R, LY - qy0));
-	const int xe = qx + 1 == LX ? 0 : qx + 1, xw = qx == 0 ? LX - 1 : qx - 1;
+	int qy0 = blk * R;
+	const bool idle = qy0 >= LY || blk >= CT / LX;
+	if(idle) qy0 = 0;
+	const int nr = idle ? 0 : min(R, LY - qy0);
 	int lx = qx + C.offx;                 // box-window column (ring offset applied)
 	if(lx >= LX) lx -= LX;
-	int ly0 = qy0 + C.offy;
-	while(ly0 >= LY) ly0 -= LY;
 	const int row0 = C.start + qy0 * LX + qx;
-
-	// stage the patch, the (LX+2)x(LY+2) box window of sum phi^2 and the y trig tables
-	const double *phic = a.phi_c + C.start;
-	for(int q = tid; q < S; q += CT) s_phi[q] = phic[q];
 	const int cx = C.corner % L, cy = C.corner / L;
-	// the adhesion term (omega != 0) reads the 5-point box Laplacian of sum phi^2:
-	// stage the window; otherwise each site reads its own sum phi^2 from the box
-	const bool stage_win = P.omega != 0;
+	double2 *pm = s_pm + qy0 * PW + qx + 1;   // this thread's site of strip row 0
+
+	// the strip's column of phi with one row above and below (periodic), read
+	// straight from the rows; pold / dold (corrector) and v (MODE_FS) likewise
+	const double *phic = a.phi_c + C.start + qx;
+	double col[NPT + 2];
+#pragma unroll
+	for(int k = 0; k < NPT + 2; ++k) {
+		col[k] = 0.;
+		if(k <= R + 1) {
+			int qy = qy0 + k - 1;
+			if(qy < 0) qy += LY;
+			if(qy >= LY) qy -= LY;
+			col[k] = phic[qy * LX];
+		}
+	}
+	double pold[NPT], dold[NPT];
+	double2 vv[NPT];
+#pragma unroll
+	for(int k = 0; k < NPT; ++k) {
+		pold[k] = dold[k] = 0.;
+		vv[k] = make_double2(0., 0.);
+		if(k < nr) {
+			if(fs && !a.store) {
+				pold[k] = a.phi_old[row0 + k * LX];
+				dold[k] = a.dphi_old[row0 + k * LX];
+			}
+			if(MODE == MODE_FS) vv[k] = a.vrows[row0 + k * LX];
+		}
+	}
+	// row tables, the box window (adhesion only) and the phi tile with its
+	// periodic columns
+	for(int t = tid; t < TR; t += CT) {
+		const int q = t % LY;
+		int ly = q + C.offy;
+		if(ly >= LY) ly -= LY;
+		int by = cy + ly;
+		if(by >= L) by -= L;
+		s_rowi[t] = make_int4(by * L, (ly + 1) * WX, ly, 0);
+		s_trow[t] = make_double2(a.trig[C.trig_y + ly], a.trig[C.trig_y + LY + ly]);
+	}
 	if(stage_win) {
 		int wy = tid / WX, wx = tid - wy * WX;
 		const int dy = CT / WX, dx = CT - dy * WX;
@@ -357,32 +437,16 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 			}
 		}
 	}
-	for(int t = tid; t < 2 * LY; t += CT) s_trig[t] = a.trig[C.trig_y + t];
-	double pold[NPT], dold[NPT];
-	double2 vv[NPT];
+	if(!idle) {
 #pragma unroll
-	for(int k = 0; k < NPT; ++k) {
-		pold[k] = dold[k] = 0.;
-		vv[k] = make_double2(0., 0.);
-		if(k < nr) {
-			if(fs && !a.store) {
-				pold[k] = a.phi_old[row0 + k * LX];
-				dold[k] = a.dphi_old[row0 + k * LX];
-			}
-			if(MODE == MODE_FS) vv[k] = a.vrows[row0 + k * LX];
+		for(int k = 0; k < NPT; ++k) {
+			if(k >= R) break;
+			pm[k * PW].x = col[k + 1];
+			if(qx == 0) pm[k * PW + LX].x = col[k + 1];
+			if(qx == LX - 1) pm[k * PW - LX].x = col[k + 1];
 		}
 	}
 	__syncthreads();
-
-	// the strip's column of phi with one row above and below (periodic)
-	double col[NPT + 2];
-#pragma unroll
-	for(int k = 0; k < NPT + 2; ++k) {
-		int qy = qy0 + k - 1;
-		if(qy < 0) qy += LY;
-		else if(qy >= LY) qy -= LY;
-		col[k] = nr > 0 && k <= nr + 1 ? s_phi[qy * LX + qx] : 0.;
-	}
 
 	// phase 1: perimeter = sum |grad phi|^2 (computeGlobalSums, WetPolarModel.cpp:243-246),
 	// needed before mu only by the perimeter term (beta != 0)
@@ -391,11 +455,10 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		double per[1] = { 0.0 };
 #pragma unroll
 		for(int k = 0; k < NPT; ++k) {
-			if(k >= nr) break;
-			const int o = (qy0 + k) * LX;
-			const double dx = .5 * (s_phi[o + xe] - s_phi[o + xw]);
+			if(k >= R) break;
+			const double dx = .5 * (pm[k * PW + 1].x - pm[k * PW - 1].x);
 			const double dy = .5 * (col[k + 2] - col[k]);
-			per[0] += dx * dx + dy * dy;
+			if(k < nr) per[0] += dx * dx + dy * dy;
 		}
 		block_sum<1, NW>(per, s_red);
 		if(tid == 0) s_bc = per[0];
@@ -405,75 +468,69 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 
 	// phase 2: chemical potential mu = dF/dphi (f_interaction, WetPolarModel.cpp:262-301).
 	// With omega == 0 (beta == 0) the adhesion (perimeter) term is exactly +-0 and is not evaluated.
-	// Loop-invariant factors are formed once, in the reference's evaluation order.
-	const double Acoef = -4 * (P.mu / P.a0) * (1 - C.area / P.a0);
-	const double Rcoef = 4 * (P.kappa / P.lambda);
+	// Loop-invariant factors are formed once, in the reference's evaluation order
+	// (mu / a0, 4 (kappa / lambda) and 4 (beta / p0) come formed from the host).
+	const double Acoef = -4 * P.mu_a0 * (1 - C.area / P.a0);
+	const double Rcoef = P.rcoef;
 	const double lam2 = 2 * P.lambda;
-	const double Pcoef = P.beta != 0 ? 4 * (P.beta / P.p0) * (1 - perimeter / P.p0) : 0.;
+	const double Pcoef = P.beta != 0 ? P.pcoef * (1 - perimeter / P.p0) : 0.;
+	const double *phi2b = a.phi2 + B.box_off + (cx + lx >= L ? cx + lx - L : cx + lx);
 	double mu[NPT];
 	double per2[1] = { 0.0 };
-	{
-		int w = (ly0 + 1) * WX + (lx + 1);
-		int ly = ly0;
-		// box site of the strip's first row: column bx fixed, row by advancing with ly
-		int bx = cx + lx;
-		if(bx >= L) bx -= L;
-		int by = cy + ly0;
-		if(by >= L) by -= L;
-		const double *phi2b = a.phi2 + B.box_off + bx;
 #pragma unroll
-		for(int k = 0; k < NPT; ++k) {
-			mu[k] = 0.;
-			if(k < nr) {
-				const int o = (qy0 + k) * LX;
-				const double ph = col[k + 1];
-				const double pE = s_phi[o + xe], pW = s_phi[o + xw], pN = col[k + 2], pS = col[k];
-				const double lap = pN + pS + pE + pW - 4 * ph;
-				const double dx = .5 * (pE - pW);
-				const double dy = .5 * (pN - pS);
-				if(MODE == MODE_CF && P.beta == 0) per2[0] += dx * dx + dy * dy;
-				// Both loads always issue and the value is selected: a select of the
-				// two pointers would become one generic-address (flat) load.  The
-				// window slot read without staging is in range and discarded.
-				const double p2g = phi2b[by * L];
-				const double p2w = s_win[w];
-				const double p2 = stage_win ? p2w : p2g;
-				const double CH = P.gamma * (div_by<RCP>(8 * ph * (1 - ph) * (1 - 2 * ph), P.lambda, P.lambda_rcp) - lam2 * lap);
-				const double A = Acoef * ph;
-				const double Rep = Rcoef * ph * (p2 - ph * ph);
-				double Adh = 0.;
-				if(P.omega != 0) {
-					const double laplacianSquare = s_win[w + 1] + s_win[w + WX] + s_win[w - 1] + s_win[w - WX] - 4. * p2;
-					const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
-					const double dd = laplacianSquare - lsquare;
-					const double suppress = dd / sqrt(1 + dd * dd);
-					Adh = -4 * P.lambda * P.omega * suppress * ph;
-				}
-				double Pp = 0.;
-				if(P.beta != 0) Pp = Pcoef * lap;
-				const double V = CH + A + Rep + Adh + Pp;
-				mu[k] = V;
-				const int row = row0 + k * LX;
-				if(MODE == MODE_CF && a.out_mu) {
-					a.out_mu[row] = V;
-					a.out_dx[row] = dx;
-					a.out_dy[row] = dy;
-					a.out_lap[row] = lap;
-				}
-				if(do_rep) {
-					const double rn = a.rep[row] + Rep;
-					a.rep[row] = rn;
-					s_rep[o + qx] = rn;
-				}
+	for(int k = 0; k < NPT; ++k) {
+		mu[k] = 0.;
+		if(k >= R) break;
+		const int4 ri = s_rowi[qy0 + k];
+		const double ph = col[k + 1];
+		const double pE = pm[k * PW + 1].x, pW = pm[k * PW - 1].x, pN = col[k + 2], pS = col[k];
+		const double lap = pN + pS + pE + pW - 4 * ph;
+		const double dx = .5 * (pE - pW);
+		const double dy = .5 * (pN - pS);
+		const bool valid = k < nr;
+		if(MODE == MODE_CF && P.beta == 0 && valid) per2[0] += dx * dx + dy * dy;
+		// Both loads always issue and the value is selected: a select of the
+		// two pointers would become one generic-address (flat) load.  The
+		// window slot read without staging is in range and discarded.
+		const int w = ri.y + lx + 1;
+		const double p2g = phi2b[ri.x];
+		const double p2w = s_win[stage_win ? w : 0];
+		const double p2 = stage_win ? p2w : p2g;
+		const double CH = P.gamma * (div_by<RCP>(8 * ph * (1 - ph) * (1 - 2 * ph), P.lambda, P.lambda_rcp) - lam2 * lap);
+		const double A = Acoef * ph;
+		const double Rep = Rcoef * ph * (p2 - ph * ph);
+		double Adh = 0.;
+		if(P.omega != 0) {
+			const double laplacianSquare = s_win[w + 1] + s_win[w + WX] + s_win[w - 1] + s_win[w - WX] - 4. * p2;
+			const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
+			const double dd = laplacianSquare - lsquare;
+			const double suppress = dd / sqrt(1 + dd * dd);
+			Adh = -4 * P.lambda * P.omega * suppress * ph;
+		}
+		double Pp = 0.;
+		if(P.beta != 0) Pp = Pcoef * lap;
+		const double V = CH + A + Rep + Adh + Pp;
+		mu[k] = V;
+		if(!idle) {
+			pm[k * PW].y = V;
+			if(qx == 0) pm[k * PW + LX].y = V;
+			if(qx == LX - 1) pm[k * PW - LX].y = V;
+		}
+		if(valid) {
+			const int row = row0 + k * LX;
+			if(MODE == MODE_CF && a.out_mu) {
+				a.out_mu[row] = V;
+				a.out_dx[row] = dx;
+				a.out_dy[row] = dy;
+				a.out_lap[row] = lap;
 			}
-			// next row of the window and of the box (ring order wraps at LY)
-			w += WX;
-			if(++by == L) by = 0;
-			if(++ly == LY) {
-				ly = 0;
-				w -= LY * WX;
-				by -= LY;
-				if(by < 0) by += L;
+			if(do_rep) {
+				const double rn = a.rep[row] + Rep;
+				a.rep[row] = rn;
+				double *rp = s_rep + (qy0 + k) * PW + qx + 1;
+				rp[0] = rn;
+				if(qx == 0) rp[LX] = rn;
+				if(qx == LX - 1) rp[-LX] = rn;
 			}
 		}
 	}
@@ -483,97 +540,95 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		__syncthreads();
 		perimeter = s_bc;
 	}
-	__syncthreads();  // every read of the sum phi^2 window is done: reuse it for mu
-#pragma unroll
-	for(int k = 0; k < NPT; ++k)
-		if(k < nr) s_mu[(qy0 + k) * LX + qx] = mu[k];
-	__syncthreads();
+	__syncthreads();   // mu of every site (and the Repulsion tile) is in LDS
 
 	// phase 3: forces (calc_internal_forces) and, unless MODE_CF, the Heun update.
 	// The per-cell sums (area, sumF, CoM trig sums, shape) are accumulated with
 	// FMA and the x trig sums as cos/sin(x) times the strip's sum of phi: their
 	// order already differs from the reference's sequential sum.
 	const DevParams Q = a.params[2 * C.box + a.fs_set];
-	double red[NRED];
+	double red[10];
 #pragma unroll
-	for(int j = 0; j < NRED; ++j) red[j] = 0.0;
+	for(int j = 0; j < 10; ++j) red[j] = 0.0;
 	int ymin = LY;
 	bool edge = false;
-	const double *cosy = s_trig, *siny = s_trig + LY;
 	const bool dry_cf = P.friction_cell == 0;
-	double muN = 0., muS = 0.;   // mu one row above / below the strip
-	if(MODE != MODE_FS && nr > 0) {
-		int qa = qy0 + nr;
+	double muN = 0., muS = 0.;   // mu one row above / below the strip (storage rows, periodic)
+	if(MODE != MODE_FS) {
+		int qa = qy0 + R;
 		if(qa >= LY) qa -= LY;
 		int qb = qy0 - 1;
 		if(qb < 0) qb += LY;
-		muN = s_mu[qa * LX + qx];
-		muS = s_mu[qb * LX + qx];
+		muN = s_pm[qa * PW + qx + 1].y;
+		muS = s_pm[qb * PW + qx + 1].y;
 	}
-	{
-		int ly = ly0;
 #pragma unroll
-		for(int k = 0; k < NPT; ++k) {
-			if(k >= nr) break;
-			const int o = (qy0 + k) * LX;
-			const int row = row0 + k * LX;
-			const double ph = col[k + 1];
-			const double dx = .5 * (s_phi[o + xe] - s_phi[o + xw]);
-			const double dy = .5 * (col[k + 2] - col[k]);
-			double vx, vy;
-			if(MODE == MODE_FS) {
-				vx = vv[k].x;
-				vy = vv[k].y;
+	for(int k = 0; k < NPT; ++k) {
+		if(k >= R) break;
+		const bool valid = k < nr;
+		const int row = row0 + k * LX;
+		const double ph = col[k + 1];
+		const double2 eE = pm[k * PW + 1], eW = pm[k * PW - 1];
+		const double dx = .5 * (eE.x - eW.x);
+		const double dy = .5 * (col[k + 2] - col[k]);
+		double vx, vy;
+		if(MODE == MODE_FS) {
+			vx = vv[k].x;
+			vy = vv[k].y;
+		}
+		else {
+			const double mN = k + 1 < R ? mu[k + 1] : muN;
+			const double mS = k > 0 ? mu[k - 1] : muS;
+			const double f_passive_x = (-1) * 0.5 * (eE.y - eW.y);
+			const double f_passive_y = (-1) * 0.5 * (mN - mS);
+			if(do_rep && valid) {
+				int qn = qy0 + k + 1, qs = qy0 + k - 1;
+				if(qn == LY) qn = 0;
+				if(qs < 0) qs = LY - 1;
+				const double *rp = s_rep + (qy0 + k) * PW + qx + 1;
+				red[8] += (-1) * 0.5 * (rp[1] - rp[-1]) * ph;
+				red[9] += (-1) * 0.5 * (s_rep[qn * PW + qx + 1] - s_rep[qs * PW + qx + 1]) * ph;
 			}
-			else {
-				const double mN = k + 1 < nr ? mu[k + 1] : muN;
-				const double mS = k > 0 ? mu[k - 1] : muS;
-				const double f_passive_x = (-1) * 0.5 * (s_mu[o + xe] - s_mu[o + xw]);
-				const double f_passive_y = (-1) * 0.5 * (mN - mS);
-				if(do_rep) {
-					int on = o + LX, os = o - LX;
-					if(qy0 + k + 1 == LY) on -= S;
-					if(qy0 + k == 0) os += S;
-					red[8] += (-1) * 0.5 * (s_rep[o + xe] - s_rep[o + xw]) * ph;
-					red[9] += (-1) * 0.5 * (s_rep[on + qx] - s_rep[os + qx]) * ph;
-				}
-				double Theta = (-dx * C.Q00) < 0 ? 0.0 : 1.0;
-				const double fQx = P.lambda * (-dx * C.Q00) * Theta * C.Q00;
-				Theta = (-dy * C.Q01) < 0 ? 0.0 : 1.0;
-				const double fQy = P.lambda * (-dy * C.Q01) * Theta * C.Q01;
-				const double fx = f_passive_x * P.passive_alpha + fQx * P.zetaQ_self;
-				const double fy = f_passive_y * P.passive_alpha + fQy * P.zetaQ_self;
-				vx = div_by<RCP>(fx, P.friction, P.friction_rcp);
-				vy = div_by<RCP>(fy, P.friction, P.friction_rcp);
-				if(MODE == MODE_CF) {
+			double Theta = (-dx * C.Q00) < 0 ? 0.0 : 1.0;
+			const double fQx = P.lambda * (-dx * C.Q00) * Theta * C.Q00;
+			Theta = (-dy * C.Q01) < 0 ? 0.0 : 1.0;
+			const double fQy = P.lambda * (-dy * C.Q01) * Theta * C.Q01;
+			const double fx = f_passive_x * P.passive_alpha + fQx * P.zetaQ_self;
+			const double fy = f_passive_y * P.passive_alpha + fQy * P.zetaQ_self;
+			vx = div_by<RCP>(fx, P.friction, P.friction_rcp);
+			vy = div_by<RCP>(fy, P.friction, P.friction_rcp);
+			if(MODE == MODE_CF) {
+				if(valid) {
 					if(dry_cf) a.vrows[row] = make_double2(vx, vy);
 					else a.frows[row] = make_double2(fx, fy);
-					continue;
 				}
-				if(a.write_v) a.vrows[row] = make_double2(vx, vy);
+				continue;
 			}
-			// FD_CPUBackend::first_step body (FD_CPUBackend.cpp:66-101)
-			const double dphi = -Q.J0 * mu[k] - vx * dx - vy * dy;
-			double phn;
-			if(a.store) {
-				a.dphi_old[row] = dphi;
-				phn = ph + Q.dt * .5 * (dphi + dphi);
-			}
-			else {
-				phn = pold[k] + Q.dt * .5 * (dphi + dold[k]);
-			}
+			if(a.write_v && valid) a.vrows[row] = make_double2(vx, vy);
+		}
+		// FD_CPUBackend::first_step body (FD_CPUBackend.cpp:66-101)
+		const double dphi = -Q.J0 * mu[k] - vx * dx - vy * dy;
+		double phn;
+		if(a.store) {
+			if(valid) a.dphi_old[row] = dphi;
+			phn = ph + Q.dt * .5 * (dphi + dphi);
+		}
+		else {
+			phn = pold[k] + Q.dt * .5 * (dphi + dold[k]);
+		}
+		if(valid) {
+			const double2 tr = s_trow[qy0 + k];
 			a.phi_out[row] = phn;
 			red[0] = fma(phn, phn, red[0]);
 			red[1] += phn;
-			red[4] = fma(phn, cosy[ly], red[4]);
-			red[5] = fma(phn, siny[ly], red[5]);
+			red[4] = fma(phn, tr.x, red[4]);
+			red[5] = fma(phn, tr.y, red[5]);
 			if(!a.store && phn < 0.09 && phn > 0.01) {
 				edge = true;
-				ymin = min(ymin, ly);
+				ymin = min(ymin, s_rowi[qy0 + k].z);
 			}
 			red[6] += -0.5 * (dx * dx - dy * dy);
 			red[7] += -dx * dy;
-			if(++ly == LY) ly = 0;
 		}
 	}
 	int xmin = edge ? lx : LX;
@@ -598,7 +653,17 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		return;
 	}
 
-	block_sum<NRED, NW>(red, s_red);
+	// per-cell sums: area, sumF, the four CoM trig sums, S00, S01 (thread j
+	// holds total j), then the CIL sums when they are accumulated
+	double trxy[2] = { 0., 0. };
+	if(do_rep) {
+		trxy[0] = red[8];
+		trxy[1] = red[9];
+		block_sum<2, NW>(trxy, s_red);
+		__syncthreads();
+	}
+	const double r8[8] = { red[0], red[1], red[2], red[3], red[4], red[5], red[6], red[7] };
+	const double tot = block_sum8<NW>(r8, s_red);
 	{
 		const int lane = tid & 63, wave = tid >> 6;
 		const int mx = wave_min(xmin), my = wave_min(ymin);
@@ -606,23 +671,25 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 			s_redi[wave] = mx;
 			s_redi[NW + wave] = my;
 		}
-		if(tid == 0) {
-#pragma unroll
-			for(int j = 0; j < NRED; ++j) s_red[j] = red[j];
-		}
 		__syncthreads();
 	}
+	if(tid >= 8) return;
+	// totals to lanes 0..3 of wave 0
+	const double area = __shfl(tot, 0, 64), sumF = __shfl(tot, 1, 64);
+	const double sxc = __shfl(tot, 2, 64), sxs = __shfl(tot, 3, 64);
+	const double syc = __shfl(tot, 4, 64), sys = __shfl(tot, 5, 64);
+	const double S00 = __shfl(tot, 6, 64), S01 = __shfl(tot, 7, 64);
+	const double trx0 = __shfl(trxy[0], 0, 64), try0 = __shfl(trxy[1], 0, 64);   // block_sum: thread 0
 	if(tid >= 4) return;
 
 	// ---- per-cell tail: lanes 0/1 take the two CoM atan2, lanes 2/3 the
 	// polarity angle and its cos / sin; lane 0 writes the cell ----
-	const double sumF = s_red[1];
-	const double trx = MODE == MODE_FUSED ? s_red[8] : C.trx;
-	const double try_ = MODE == MODE_FUSED ? s_red[9] : C.try_;
+	const double trx = MODE == MODE_FUSED ? (do_rep ? trx0 : 0.) : C.trx;
+	const double try_ = MODE == MODE_FUSED ? (do_rep ? try0 : 0.) : C.try_;
 	double val = 0.;
 	double th_old = C.thetaQ_old, th = 0.;
-	if(tid == 0) val = LX * (atan2(-s_red[3] / sumF, -s_red[2] / sumF) + PI_F) / (2 * PI_F);
-	else if(tid == 1) val = LY * (atan2(-s_red[5] / sumF, -s_red[4] / sumF) + PI_F) / (2 * PI_F);
+	if(tid == 0) val = LX * (atan2(-sxs / sumF, -sxc / sumF) + PI_F) / (2 * PI_F);
+	else if(tid == 1) val = LY * (atan2(-sys / sumF, -syc / sumF) + PI_F) / (2 * PI_F);
 	else {
 		// WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:365-396)
 		if(a.store) {
@@ -654,10 +721,10 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		G.area_old = C.area;
 		G.sumF_old = C.sumF;
 	}
-	G.area = s_red[0];
+	G.area = area;
 	G.sumF = sumF;
-	G.S00 = s_red[6];
-	G.S01 = s_red[7];
+	G.S00 = S00;
+	G.S01 = S01;
 	G.perimeter = 0.;
 	int xsl = LX, ysb = LY;
 	if(!a.store) {
