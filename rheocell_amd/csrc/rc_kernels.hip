@@ -327,9 +327,11 @@ __device__ inline double block_sum8(const double (&v)[8], double *s_part) {
 // CT threads per cell; NPT >= the strip rows of every patch (compile time), so
 // the per-site registers are fully unrolled.  RCP: lambda and the friction of
 // every parameter set are powers of two (divisions become exact multiplications).
-// LDS layout: stage_lds_bytes (rc_kernels.h).
-template <int MODE, int NPT, bool RCP, int CT>
+// ST: the predictor (first_step(true)) when set, else the corrector (a.store
+// equals ST; MODE_CF takes ST = 0).  LDS layout: stage_lds_bytes (rc_kernels.h).
+template <int MODE, int NPT, bool RCP, int CT, bool ST>
 __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
+	constexpr bool store = ST;
 	// Any status flag halts the device, except that a growth request raised
 	// by a corrector of this same step leaves the rest of that corrector stage
 	// to run (cells of other stage groups are launched after it); the host
@@ -339,7 +341,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 	{
 		const unsigned f = a.st->flags;
 		if(f) {
-			const bool same_step = MODE != MODE_CF && !a.store && (f & ST_KIND_MASK) == ST_GROWTH &&
+			const bool same_step = MODE != MODE_CF && !store && (f & ST_KIND_MASK) == ST_GROWTH &&
 					(f >> ST_STEP_SHIFT) == (unsigned) (step + 1);
 			if(!same_step) return;
 		}
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		pold[k] = dold[k] = 0.;
 		vv[k] = make_double2(0., 0.);
 		if(k < nr) {
-			if(fs && !a.store) {
+			if(fs && !store) {
 				pold[k] = a.phi_old[row0 + k * LX];
 				dold[k] = a.dphi_old[row0 + k * LX];
 			}
@@ -609,7 +611,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		// FD_CPUBackend::first_step body (FD_CPUBackend.cpp:66-101)
 		const double dphi = -Q.J0 * mu[k] - vx * dx - vy * dy;
 		double phn;
-		if(a.store) {
+		if(store) {
 			if(valid) a.dphi_old[row] = dphi;
 			phn = ph + Q.dt * .5 * (dphi + dphi);
 		}
@@ -623,7 +625,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 			red[1] += phn;
 			red[4] = fma(phn, tr.x, red[4]);
 			red[5] = fma(phn, tr.y, red[5]);
-			if(!a.store && phn < 0.09 && phn > 0.01) {
+			if(!store && phn < 0.09 && phn > 0.01) {
 				edge = true;
 				ymin = min(ymin, s_rowi[qy0 + k].z);
 			}
@@ -692,7 +694,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 	else if(tid == 1) val = LY * (atan2(-sys / sumF, -syc / sumF) + PI_F) / (2 * PI_F);
 	else {
 		// WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:365-396)
-		if(a.store) {
+		if(store) {
 			const double g = a.gauss ? a.gauss[ci] : 0.0;
 			th_old = C.thetaQ + Q.sqrt_dt * Q.zetaQ_inter * g;
 		}
@@ -700,7 +702,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		double arg = th;
 		if(Q.anchoring) {  // update_anchoring (WetPolarModel.cpp:417-430)
 			const double walls_length = 8;
-			const double com_old_y = a.store ? C.CoMy : C.CoM_oldy;
+			const double com_old_y = store ? C.CoMy : C.CoM_oldy;
 			const double dist1 = (com_old_y - walls_length);
 			const double dist2 = ((double) L - walls_length) - com_old_y;
 			if(dist1 < dist2) arg = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
@@ -715,7 +717,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 	if(tid != 0) return;
 	if(a.pure) return;   // timing run (shape tuner): no cell state is written
 	DevCell &G = a.cells[ci];
-	if(a.store) {
+	if(store) {
 		G.CoM_oldx = C.CoMx;
 		G.CoM_oldy = C.CoMy;
 		G.area_old = C.area;
@@ -727,7 +729,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 	G.S01 = S01;
 	G.perimeter = 0.;
 	int xsl = LX, ysb = LY;
-	if(!a.store) {
+	if(!store) {
 		for(int w = 0; w < NW; ++w) {
 			xsl = min(xsl, s_redi[w]);
 			ysb = min(ysb, s_redi[NW + w]);
@@ -735,7 +737,7 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 	}
 	G.x_sub_left = xsl;
 	G.y_sub_bottom = ysb;
-	if(!a.store) {
+	if(!store) {
 		// MultiPhaseField::set_positions (MultiPhaseField.cpp:223-247)
 		if(xsl >= BORDER && ysb >= BORDER) {
 			const int site = box_elem(L, C.corner, (int) CoMx, (int) CoMy);
@@ -787,28 +789,30 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 static const int STAGE_SHAPES[][2] = { { 256, 4 }, { 256, 7 }, { 512, 4 }, { 512, 7 }, { 512, 10 }, { 512, 16 } };
 constexpr int NSTAGE_SHAPES = 6;
 
-template <int MODE, bool RCP>
+template <int MODE, bool RCP, bool ST>
 static hipError_t set_lds_r(size_t bytes) {
 	hipError_t e;
 	const int b = (int) bytes;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 10, RCP, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	return hipFuncSetAttribute((const void *) k_stage<MODE, 16, RCP, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 256, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 256, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 10, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	return hipFuncSetAttribute((const void *) k_stage<MODE, 16, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
 }
-template <int MODE>
+template <int MODE, bool ST>
 static hipError_t set_lds(size_t bytes) {
-	hipError_t e = set_lds_r<MODE, false>(bytes);
-	return e != hipSuccess ? e : set_lds_r<MODE, true>(bytes);
+	hipError_t e = set_lds_r<MODE, false, ST>(bytes);
+	return e != hipSuccess ? e : set_lds_r<MODE, true, ST>(bytes);
 }
 
 hipError_t stage_set_lds_limit(size_t bytes) {
 	hipError_t e;
-	if((e = set_lds<MODE_FUSED>(bytes)) != hipSuccess) return e;
-	if((e = set_lds<MODE_CF>(bytes)) != hipSuccess) return e;
-	return set_lds<MODE_FS>(bytes);
+	if((e = set_lds<MODE_FUSED, true>(bytes)) != hipSuccess) return e;
+	if((e = set_lds<MODE_FUSED, false>(bytes)) != hipSuccess) return e;
+	if((e = set_lds<MODE_CF, false>(bytes)) != hipSuccess) return e;
+	if((e = set_lds<MODE_FS, true>(bytes)) != hipSuccess) return e;
+	return set_lds<MODE_FS, false>(bytes);
 }
 
 int stage_shapes(int (*shapes)[2]) {
@@ -820,25 +824,35 @@ int stage_shapes(int (*shapes)[2]) {
 }
 int stage_strip_rows(int LX, int LY, int threads) { return strip_rows(LX, LY, threads); }
 
-template <int MODE, bool RCP>
+template <int MODE, bool RCP, bool ST>
 static void launch_mode_r(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
 	const dim3 g(ncells);
 	if(a.threads == 256) {
-		if(a.npt == 4) RC_LAUNCH((k_stage<MODE, 4, RCP, 256>), g, dim3(256), lds, s, a);
-		else RC_LAUNCH((k_stage<MODE, 7, RCP, 256>), g, dim3(256), lds, s, a);
+		if(a.npt == 4) RC_LAUNCH((k_stage<MODE, 4, RCP, 256, ST>), g, dim3(256), lds, s, a);
+		else RC_LAUNCH((k_stage<MODE, 7, RCP, 256, ST>), g, dim3(256), lds, s, a);
 		return;
 	}
 	switch(a.npt) {
-	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP, 512>), g, dim3(512), lds, s, a); break;
-	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP, 512>), g, dim3(512), lds, s, a); break;
-	case 10: RC_LAUNCH((k_stage<MODE, 10, RCP, 512>), g, dim3(512), lds, s, a); break;
-	default: RC_LAUNCH((k_stage<MODE, 16, RCP, 512>), g, dim3(512), lds, s, a); break;
+	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
+	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
+	case 10: RC_LAUNCH((k_stage<MODE, 10, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
+	default: RC_LAUNCH((k_stage<MODE, 16, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
 	}
+}
+template <int MODE, bool ST>
+static void launch_mode_s(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
+	if(a.rcp) launch_mode_r<MODE, true, ST>(a, ncells, lds, s);
+	else launch_mode_r<MODE, false, ST>(a, ncells, lds, s);
 }
 template <int MODE>
 static void launch_mode(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
-	if(a.rcp) launch_mode_r<MODE, true>(a, ncells, lds, s);
-	else launch_mode_r<MODE, false>(a, ncells, lds, s);
+	if constexpr(MODE != MODE_CF) {
+		if(a.store) {
+			launch_mode_s<MODE, true>(a, ncells, lds, s);
+			return;
+		}
+	}
+	launch_mode_s<MODE, false>(a, ncells, lds, s);
 }
 
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
