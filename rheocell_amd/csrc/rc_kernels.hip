@@ -330,7 +330,7 @@ __device__ inline double block_sum8(const double (&v)[8], double *s_part) {
 // ST: the predictor (first_step(true)) when set, else the corrector (a.store
 // equals ST; MODE_CF takes ST = 0).  LDS layout: stage_lds_bytes (rc_kernels.h).
 template <int MODE, int NPT, bool RCP, int CT, bool ST>
-__global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
+__global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	constexpr bool store = ST;
 	// Any status flag halts the device, except that a growth request raised
 	// by a corrector of this same step leaves the rest of that corrector stage
@@ -393,20 +393,6 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 			if(qy < 0) qy += LY;
 			if(qy >= LY) qy -= LY;
 			col[k] = phic[qy * LX];
-		}
-	}
-	double pold[NPT], dold[NPT];
-	double2 vv[NPT];
-#pragma unroll
-	for(int k = 0; k < NPT; ++k) {
-		pold[k] = dold[k] = 0.;
-		vv[k] = make_double2(0., 0.);
-		if(k < nr) {
-			if(fs && !store) {
-				pold[k] = a.phi_old[row0 + k * LX];
-				dold[k] = a.dphi_old[row0 + k * LX];
-			}
-			if(MODE == MODE_FS) vv[k] = a.vrows[row0 + k * LX];
 		}
 	}
 	// row tables, the box window (adhesion only) and the phi tile with its
@@ -477,13 +463,17 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 	const double lam2 = 2 * P.lambda;
 	const double Pcoef = P.beta != 0 ? P.pcoef * (1 - perimeter / P.p0) : 0.;
 	const double *phi2b = a.phi2 + B.box_off + (cx + lx >= L ? cx + lx - L : cx + lx);
+	// the box sums of phi^2 at the strip's rows, all requested before the first
+	// is used (rows past the strip read valid table entries and are discarded)
+	double p2g[NPT];
+#pragma unroll
+	for(int k = 0; k < NPT; ++k) p2g[k] = phi2b[s_rowi[qy0 + k].x];
 	double mu[NPT];
 	double per2[1] = { 0.0 };
 #pragma unroll
 	for(int k = 0; k < NPT; ++k) {
 		mu[k] = 0.;
 		if(k >= R) break;
-		const int4 ri = s_rowi[qy0 + k];
 		const double ph = col[k + 1];
 		const double pE = pm[k * PW + 1].x, pW = pm[k * PW - 1].x, pN = col[k + 2], pS = col[k];
 		const double lap = pN + pS + pE + pW - 4 * ph;
@@ -494,10 +484,9 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 		// Both loads always issue and the value is selected: a select of the
 		// two pointers would become one generic-address (flat) load.  The
 		// window slot read without staging is in range and discarded.
-		const int w = ri.y + lx + 1;
-		const double p2g = phi2b[ri.x];
-		const double p2w = s_win[stage_win ? w : 0];
-		const double p2 = stage_win ? p2w : p2g;
+		const int w = stage_win ? s_rowi[qy0 + k].y + lx + 1 : 0;
+		const double p2w = s_win[w];
+		const double p2 = stage_win ? p2w : p2g[k];
 		const double CH = P.gamma * (div_by<RCP>(8 * ph * (1 - ph) * (1 - 2 * ph), P.lambda, P.lambda_rcp) - lam2 * lap);
 		const double A = Acoef * ph;
 		const double Rep = Rcoef * ph * (p2 - ph * ph);
@@ -534,6 +523,23 @@ __global__ __launch_bounds__(CT) void k_stage(StageArgs a) {
 				if(qx == 0) rp[LX] = rn;
 				if(qx == LX - 1) rp[-LX] = rn;
 			}
+		}
+	}
+	// the rows only the update reads (pold / dold of the corrector, v of
+	// MODE_FS) are requested after the box loads above, so that their
+	// transfer overlaps the evaluation of mu instead of preceding it
+	double pold[NPT], dold[NPT];
+	double2 vv[NPT];
+#pragma unroll
+	for(int k = 0; k < NPT; ++k) {
+		pold[k] = dold[k] = 0.;
+		vv[k] = make_double2(0., 0.);
+		if(k < nr) {
+			if(fs && !store) {
+				pold[k] = a.phi_old[row0 + k * LX];
+				dold[k] = a.dphi_old[row0 + k * LX];
+			}
+			if(MODE == MODE_FS) vv[k] = a.vrows[row0 + k * LX];
 		}
 	}
 	if(MODE == MODE_CF && P.beta == 0) {
