@@ -570,6 +570,10 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 		muN = s_pm[qa * PW + qx + 1].y;
 		muS = s_pm[qb * PW + qx + 1].y;
 	}
+	// every row load has landed (they were requested before mu was evaluated):
+	// one wait here, not a wait behind the previous row's stores in each row
+	// (the memory counter also counts stores)
+	__builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
 #pragma unroll
 	for(int k = 0; k < NPT; ++k) {
 		if(k >= R) break;
