@@ -48,7 +48,7 @@ struct GatherArgs {
 	double *phi2;
 	int *cover;
 	int *tile_cells;             // cells intersecting each tile, cell order
-	int4 *tile_geo;              // their geometry (as geo), written with the list
+	int4 *tile_geo;              // their entries relative to the tile (tile_entry), written with the list
 	int *tile_count;
 	int build_lists;             // 1: rebuild the lists from geo, 0: reuse them
 	DevStatus *st;

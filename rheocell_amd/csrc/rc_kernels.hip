@@ -124,6 +124,16 @@ __device__ inline void block_sum(double (&v)[NV], double *s_red) {
 	__syncthreads();
 }
 
+// Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8).  The
+// item (cell, tile, block of box sites) of workgroup b is chosen so that each
+// XCD takes one contiguous range of the n items: neighbouring cells and tiles
+// share box data, which then stays in one XCD's L2.
+constexpr int NXCD = 8;
+__device__ inline int xcd_item(int b, int n) {
+	const int q = n / NXCD, r = n % NXCD, x = b % NXCD, j = b / NXCD;
+	return x * q + min(x, r) + j;
+}
+
 __device__ inline bool circ_overlap(int a0, int alen, int b0, int blen, int L) {
 	// do [a0, a0+alen) and [b0, b0+blen) intersect on the circle of length L?
 	int d1 = b0 - a0;
@@ -139,37 +149,69 @@ __device__ inline bool circ_overlap(int a0, int alen, int b0, int blen, int L) {
 // box's cells; the corrector's gather reuses the predictor's lists (the
 // patches only move at the corrector's recentering).
 // ---------------------------------------------------------------------------
-__device__ inline void unpack_geo(const int4 g, int L, int &cx, int &cy, int &lx, int &ly, int &ox, int &oy, int &st) {
-	cx = g.x % L;
-	cy = g.x / L;
-	lx = g.y & 0xFFFF;
-	ly = g.y >> 16;
-	ox = g.z & 0xFFFF;
-	oy = g.z >> 16;
-	st = g.w;
+
+// A tile's list entry, relative to the tile's origin (x0, y0):
+// {lx0 | ly0 << 16, LX | LY << 16, offx | offy << 16, row start} with
+// lx0 = (x0 - cx) mod L, ly0 = (y0 - cy) mod L, so that box site
+// (x0 + sx, y0 + sy) lies at column (lx0 + sx) mod L, row (ly0 + sy) mod L of
+// the cell's patch.  Lists are padded to a multiple of 16 entries with cells
+// that cover nothing (LX = LY = 0): the site loops read whole batches.
+__device__ inline int4 tile_entry(const int4 g, int L, int x0, int y0) {
+	int lx0 = x0 - g.x % L;
+	if(lx0 < 0) lx0 += L;
+	int ly0 = y0 - g.x / L;
+	if(ly0 < 0) ly0 += L;
+	return make_int4(lx0 | (ly0 << 16), g.y, g.z, g.w);
 }
 
-// loads the tile's list into LDS (building it when build != 0); returns its length
-__device__ int tile_list(const GatherArgs &a, int tile, const DevBox &B, int x0, int y0, int tw, int th, int build,
-		int *s_cx, int *s_cy, int *s_lx, int *s_ly, int *s_ox, int *s_oy, int *s_start, int *s_wcnt, int *s_n) {
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, L = B.L;
-	if(!build) {
-		const int n = a.tile_count[tile];
-		for(int j = tid; j < n; j += 256)
-			unpack_geo(a.tile_geo[tile * MAXT + j], L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
-		__syncthreads();
-		return n;
-	}
-	if(tid == 0) *s_n = 0;
+// row of the patch site covering tile site (sx, sy), or -1 (sx < 32 <= L or sx < L)
+__device__ inline int entry_row(const int4 e, int sx, int sy, int L) {
+	int lx = (e.x & 0xFFFF) + sx;
+	if(lx >= L) lx -= L;
+	int ly = (e.x >> 16) + sy;
+	if(ly >= L) ly -= L;
+	const int LXj = e.y & 0xFFFF, LYj = e.y >> 16;
+	int qx = lx - (e.z & 0xFFFF);
+	if(qx < 0) qx += LXj;
+	int qy = ly - (e.z >> 16);
+	if(qy < 0) qy += LYj;
+	// branch-free, so that a batch's entries are all read before the first is used
+	return lx < LXj && ly < LYj ? e.w + qy * LXj + qx : -1;
+}
+
+__device__ inline void pad_list(int n, int4 *s_list) {
+	const int np = (n + 15) & ~15;
+	for(int j = n + (int) threadIdx.x; j < np; j += blockDim.x) s_list[j] = make_int4(0, 0, 0, 0);
+}
+
+// Loads the tile's list into LDS (reuse): the count and every entry are
+// requested at once, before the box description
+__device__ int tile_list_load(const int *tile_count, const int4 *tile_list, int tile, int4 *s_list) {
+	const int n = tile_count[tile];
+	for(int j = threadIdx.x; j < n; j += blockDim.x) s_list[j] = tile_list[tile * MAXT + j];
+	pad_list(min(n, MAXT), s_list);
 	__syncthreads();
-	for(int base = B.cell_begin; base < B.cell_end; base += 256) {
-		const int c = base + tid;
-		bool hit = false;
-		int cx = 0, cy = 0, lx = 0, ly = 0, ox = 0, oy = 0, st = 0;
-		if(c < B.cell_end) {
-			unpack_geo(a.geo[c], L, cx, cy, lx, ly, ox, oy, st);
-			hit = circ_overlap(cx, lx, x0, tw, L) && circ_overlap(cy, ly, y0, th, L);
-		}
+	return n;
+}
+
+// Builds the tile's list from the compact cell geometry (cell order) into LDS
+// and the global list; returns its length.  The geometry of the first
+// GB_PRE * 256 cells is requested at once.
+constexpr int GB_PRE = 8;
+__device__ int tile_list_build(const GatherArgs &a, int tile, const DevBox &B, int x0, int y0, int tw, int th,
+		int4 *s_list, int *s_wcnt, int *s_n) {
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, L = B.L;
+	if(tid == 0) *s_n = 0;
+	int4 gp[GB_PRE];
+#pragma unroll
+	for(int r = 0; r < GB_PRE; ++r) {
+		const int c = B.cell_begin + r * 256 + tid;
+		gp[r] = c < B.cell_end ? a.geo[c] : make_int4(0, 0, 0, 0);
+	}
+	__syncthreads();
+	auto round = [&](int c, const int4 g) {
+		const bool hit = c < B.cell_end && circ_overlap(g.x % L, g.y & 0xFFFF, x0, tw, L) &&
+				circ_overlap(g.x / L, g.y >> 16, y0, th, L);
 		const unsigned long long m = __ballot(hit);
 		const int pre = __popcll(m & ((1ull << lane) - 1ull));
 		if(lane == 0) s_wcnt[wave] = __popcll(m);
@@ -178,32 +220,30 @@ __device__ int tile_list(const GatherArgs &a, int tile, const DevBox &B, int x0,
 		for(int w = 0; w < wave; ++w) off += s_wcnt[w];
 		const int pos = off + pre;
 		if(hit && pos < MAXT) {
-			s_cx[pos] = cx; s_cy[pos] = cy; s_lx[pos] = lx; s_ly[pos] = ly;
-			s_ox[pos] = ox; s_oy[pos] = oy; s_start[pos] = st;
+			const int4 e = tile_entry(g, L, x0, y0);
+			s_list[pos] = e;
 			a.tile_cells[tile * MAXT + pos] = c;
-			a.tile_geo[tile * MAXT + pos] = a.geo[c];
+			a.tile_geo[tile * MAXT + pos] = e;
 		}
 		__syncthreads();
 		if(tid == 0) *s_n += s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
 		__syncthreads();
+	};
+#pragma unroll
+	for(int r = 0; r < GB_PRE; ++r) {
+		const int base = B.cell_begin + r * 256;
+		if(base >= B.cell_end) break;
+		round(base + tid, gp[r]);
+	}
+	for(int base = B.cell_begin + GB_PRE * 256; base < B.cell_end; base += 256) {
+		const int c = base + tid;
+		round(c, c < B.cell_end ? a.geo[c] : make_int4(0, 0, 0, 0));
 	}
 	const int n = *s_n;
 	if(tid == 0) a.tile_count[tile] = min(n, MAXT);
+	pad_list(min(n, MAXT), s_list);
+	__syncthreads();
 	return n;
-}
-
-// row of the patch site of cell j covering box site (x, y), or -1
-__device__ inline int covering_row(int x, int y, int L, int cx, int cy, int LXj, int LYj, int ox, int oy, int st) {
-	int lx = x - cx;
-	if(lx < 0) lx += L;
-	int ly = y - cy;
-	if(ly < 0) ly += L;
-	if(lx >= LXj || ly >= LYj) return -1;
-	int qx = lx - ox;
-	if(qx < 0) qx += LXj;
-	int qy = ly - oy;
-	if(qy < 0) qy += LYj;
-	return st + qy * LXj + qx;
 }
 
 constexpr int GCHUNK = 8;   // covering patches examined per batch of independent loads (CG gathers)
@@ -215,33 +255,29 @@ constexpr int GCHUNK_PHI = 16;   // the same for the sum phi^2 gather (one batch
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 	if(a.st->flags) return;
-	__shared__ int s_cx[MAXT], s_cy[MAXT], s_lx[MAXT], s_ly[MAXT], s_ox[MAXT], s_oy[MAXT], s_start[MAXT];
+	__shared__ int4 s_list[MAXT];
 	__shared__ int s_wcnt[4];
 	__shared__ int s_n;
-	const int tile = blockIdx.x, tid = threadIdx.x;
+	const int tile = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
 	const DevBox B = a.boxes[a.tile_box[tile]];
 	const int t = tile - B.tile_off, L = B.L;
 	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
 	const int tw = min(TW, L - x0), th = min(TH, L - y0);
-	const int n = tile_list(a, tile, B, x0, y0, tw, th, a.build_lists, s_cx, s_cy, s_lx, s_ly, s_ox, s_oy, s_start,
-			s_wcnt, &s_n);
+	const int n = a.build_lists ? tile_list_build(a, tile, B, x0, y0, tw, th, s_list, s_wcnt, &s_n)
+			: tile_list_load(a.tile_count, a.tile_geo, tile, s_list);
 	if(n > MAXT) {
 		if(tid == 0) atomicOr(&a.st->flags, ST_TILE);
 		return;
 	}
 	const int sx = tid % TW, sy = tid / TW;
 	if(sx >= tw || sy >= th) return;
-	const int x = x0 + sx, y = y0 + sy;
 	double acc = 0.0;
 	int cnt = 0;
 	for(int base = 0; base < n; base += GCHUNK_PHI) {
 		int row[GCHUNK_PHI];
 		double v[GCHUNK_PHI];
 #pragma unroll
-		for(int k = 0; k < GCHUNK_PHI; ++k) {
-			const int j = base + k;
-			row[k] = j < n ? covering_row(x, y, L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]) : -1;
-		}
+		for(int k = 0; k < GCHUNK_PHI; ++k) row[k] = entry_row(s_list[base + k], sx, sy, L);
 #pragma unroll
 		for(int k = 0; k < GCHUNK_PHI; ++k) v[k] = row[k] >= 0 ? a.phi[row[k]] : 0.;
 #pragma unroll
@@ -251,7 +287,7 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 				++cnt;
 			}
 	}
-	const int k = B.box_off + y * L + x;
+	const int k = B.box_off + (y0 + sy) * L + x0 + sx;
 	a.phi2[k] = acc;
 	a.cover[k] = cnt;
 	if(cnt > STORE_MAX) {
@@ -351,7 +387,8 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	__shared__ double s_red[NW * 4 * 8];
 	__shared__ int s_redi[2 * NW];
 	__shared__ double s_bc;
-	const int ci = a.cell_list ? a.cell_list[blockIdx.x] : blockIdx.x, tid = threadIdx.x;
+	const int slot = xcd_item(blockIdx.x, gridDim.x);
+	const int ci = a.cell_list ? a.cell_list[slot] : slot, tid = threadIdx.x;
 	const DevCell C = a.cells[ci];
 	const DevBox B = a.boxes[C.box];
 	const DevParams P = a.params[2 * C.box + a.cf_set];
@@ -1061,16 +1098,13 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 	if(a.st->flags) return;
 	const CGState *cg = a.cg;
 	if(MODE != CG_GATHER_X && !cg->active) return;
-	__shared__ int s_cx[MAXT], s_cy[MAXT], s_lx[MAXT], s_ly[MAXT], s_ox[MAXT], s_oy[MAXT], s_start[MAXT];
-	const int tile = blockIdx.x, tid = threadIdx.x;
+	__shared__ int4 s_list[MAXT];
+	const int tile = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
 	const DevBox B = a.boxes[a.tile_box[tile]];
+	const int n = tile_list_load(a.tile_count, a.tile_geo, tile, s_list);
 	const int t = tile - B.tile_off, L = B.L;
 	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
 	const int tw = min(TW, L - x0), th = min(TH, L - y0);
-	const int n = a.tile_count[tile];
-	for(int j = tid; j < n; j += 256)
-		unpack_geo(a.tile_geo[tile * MAXT + j], L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
-	__syncthreads();
 	const int sx = tid % TW, sy = tid / TW;
 	if(MODE == CG_GATHER_E) {
 		// e = (r + eta N4) / D per covering row (A e = r row by row); box sums of e, e^2; e.r -> partials
@@ -1082,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 			const double2 n4 = a.n4[k];
 			double2 acc = make_double2(0., 0.), acc2 = make_double2(0., 0.);
 			for(int j = 0; j < n; ++j) {
-				const int row = covering_row(x0 + sx, y0 + sy, L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]);
+				const int row = entry_row(s_list[j], sx, sy, L);
 				if(row < 0) continue;
 				const double2 r = a.r[row];
 				const double2 e = make_double2((r.x + eta * n4.x) / D, (r.y + eta * n4.y) / D);
@@ -1113,10 +1147,7 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 		int row[GCHUNK];
 		double2 rv[GCHUNK], pv[GCHUNK];
 #pragma unroll
-		for(int q = 0; q < GCHUNK; ++q) {
-			const int j = base + q;
-			row[q] = j < n ? covering_row(x, y, L, s_cx[j], s_cy[j], s_lx[j], s_ly[j], s_ox[j], s_oy[j], s_start[j]) : -1;
-		}
+		for(int q = 0; q < GCHUNK; ++q) row[q] = entry_row(s_list[base + q], sx, sy, L);
 #pragma unroll
 		for(int q = 0; q < GCHUNK; ++q) {
 			rv[q] = make_double2(0., 0.);
@@ -1130,6 +1161,9 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 				}
 			}
 		}
+		// one wait for the batch: the memory counter also counts the stores
+		// below, which would otherwise put a wait behind each of them
+		__builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
 #pragma unroll
 		for(int q = 0; q < GCHUNK; ++q) {
 			if(row[q] < 0) continue;
@@ -1186,7 +1220,7 @@ __global__ __launch_bounds__(256) void k_cg_box(CGArgs a, int nsites) {
 	CGState *cg = a.cg;
 	if(MODE != 0 && !cg->active) return;
 	__shared__ double s_red[4 * 2];
-	const int k = blockIdx.x * 256 + threadIdx.x;
+	const int k = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
 	const DevBox B = a.boxes[0];
 	const double eta = a.params[a.set].friction_cell;
 	double part[2] = { 0., 0. };
@@ -1239,7 +1273,7 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 	const bool rev1 = MODE == CG_ROW_REVERT && cg->rn2[1] > cg->rn2_start[1];
 	if(MODE == CG_ROW_REVERT && !rev0 && !rev1) return;
 	__shared__ double s_red[4 * 6];
-	const int ci = blockIdx.x, tid = threadIdx.x;
+	const int ci = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
 	const DevCell C = a.cells[ci];
 	const DevBox B = a.boxes[C.box];
 	const double eta = a.params[a.set].friction_cell;
