@@ -1295,7 +1295,48 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 		}
 	}
 	double red[6] = { 0., 0., 0., 0., 0., 0. };
-	for(int q = tid; q < S; q += 256) {
+	auto box_site = [&](int q) {
+		const int qy = q / LX, qx = q - qy * LX;
+		int bx = qx + C.offx;
+		if(bx >= LX) bx -= LX;
+		bx += cx;
+		if(bx >= L) bx -= L;
+		int by = qy + C.offy;
+		if(by >= LY) by -= LY;
+		by += cy;
+		if(by >= L) by -= L;
+		return B.box_off + by * L + bx;
+	};
+	if constexpr(MODE == CG_ROW_UPDATE) {
+		// x += alpha p, r -= alpha Ap on two rows per pass, every load of both
+		// issued before either row is updated (rows q, q + 256, ... in order)
+		auto upd = [&](double D, double2 n4, double2 vin, double2 xx, double2 rr, int row) {
+			double2 t;
+			t.x = D * vin.x - eta * n4.x;
+			t.y = D * vin.y - eta * n4.y;
+			const double inv = 1. / D;
+			if(!dn0) { xx.x += al0 * vin.x; rr.x -= al0 * t.x; }
+			if(!dn1) { xx.y += al1 * vin.y; rr.y -= al1 * t.y; }
+			a.x[row] = xx;
+			a.r[row] = rr;
+			red[0] += rr.x * rr.x; red[1] += rr.y * rr.y;
+			red[2] += rr.x * (inv * rr.x); red[3] += rr.y * (inv * rr.y);
+		};
+		for(int q0 = tid; q0 < S; q0 += 512) {
+			const bool two = q0 + 256 < S;
+			const int q1 = two ? q0 + 256 : q0;   // past the end: row q0 again, read and not written
+			const int k0 = box_site(q0), k1 = box_site(q1);
+			const int r0 = C.start + q0, r1 = C.start + q1;
+			const double D0 = a.diag[k0], D1 = a.diag[k1];
+			const double2 n40 = a.n4[k0], n41 = a.n4[k1];
+			const double2 p0 = a.p[r0], p1 = a.p[r1];
+			const double2 x0 = a.x[r0], x1 = a.x[r1];
+			const double2 rr0 = a.r[r0], rr1 = a.r[r1];
+			upd(D0, n40, p0, x0, rr0, r0);
+			if(two) upd(D1, n41, p1, x1, rr1, r1);
+		}
+	}
+	else for(int q = tid; q < S; q += 256) {
 		const int qy = q / LX, qx = q - qy * LX;
 		int bx = qx + C.offx;
 		if(bx >= LX) bx -= LX;
