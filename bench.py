@@ -54,10 +54,10 @@ CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update")
 # way, i.e. reads x 2.0, writes x 1.0.  Recorded in profiles/.
 READ_SCALE, WRITE_SCALE = 262144.0 / 131085.0, 1.0
 TRAFFIC_KIB = {
-    ("wet512_1600", "cg_row_update"): (88125.3, 85192.0),
-    ("wet512_1600", "cg_gather_p"): (59543.8, 52540.9),
-    ("dry512_1024", "stage_fused_pred"): (10338.6, 27136.4),
-    ("dry512_1024", "stage_fused_corr"): (23884.0, 13712.0),
+    ("wet512_1600", "cg_row_update"): (68279.8, 84825.6),
+    ("wet512_1600", "cg_gather_p"): (44948.1, 52416.3),
+    ("dry512_1024", "stage_fused_pred"): (8606.3, 27388.1),
+    ("dry512_1024", "stage_fused_corr"): (22059.6, 13695.6),
 }
 
 
@@ -76,6 +76,8 @@ def parse():
                     help="learned warm-start weights (wet workloads; skipped when absent)")
     ap.add_argument("--cg-steps", type=int, default=20, help="steps of the learned-vs-previous CG comparison")
     ap.add_argument("--no-graphs", action="store_true", help="launch dry steps one by one instead of HIP graph replays")
+    ap.add_argument("--stage-shape", default="", metavar="TxR",
+                    help="force the field-update kernel shape (threads x strip rows) instead of the autotuner's choice")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
                     help="only fill and copy a MIB-sized device buffer 3 times (counter calibration), then exit")
     return ap.parse_args()
@@ -200,6 +202,9 @@ def main():
     sim = Simulation(boxes)
     if args.no_graphs:
         sim.set_graphs(False)
+    if args.stage_shape:
+        t, r = (int(v) for v in args.stage_shape.lower().split("x"))
+        sim.set_stage_shape(t, r)
     rows = sim.total_rows()
     if len(boxes) > 1:
         log("[bench] %d boxes, sides %s" % (len(boxes), sorted(set(c.Lx for c, _ in boxes))))
