@@ -432,6 +432,21 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 			col[k] = phic[qy * LX];
 		}
 	}
+	// the box sums of phi^2 at the strip's rows, requested with the strip (rows
+	// past the strip read valid box sites and are discarded)
+	const double *phi2b = a.phi2 + B.box_off + (cx + lx >= L ? cx + lx - L : cx + lx);
+	double p2g[NPT];
+#pragma unroll
+	for(int k = 0; k < NPT; ++k) {
+		int qk = qy0 + k;
+		if(qk >= LY) qk -= LY;
+		if(qk >= LY) qk = 0;
+		int ly = qk + C.offy;
+		if(ly >= LY) ly -= LY;
+		int by = cy + ly;
+		if(by >= L) by -= L;
+		p2g[k] = phi2b[by * L];
+	}
 	// row tables, the box window (adhesion only) and the phi tile with its
 	// periodic columns
 	for(int t = tid; t < TR; t += CT) {
@@ -499,12 +514,6 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	const double Rcoef = P.rcoef;
 	const double lam2 = 2 * P.lambda;
 	const double Pcoef = P.beta != 0 ? P.pcoef * (1 - perimeter / P.p0) : 0.;
-	const double *phi2b = a.phi2 + B.box_off + (cx + lx >= L ? cx + lx - L : cx + lx);
-	// the box sums of phi^2 at the strip's rows, all requested before the first
-	// is used (rows past the strip read valid table entries and are discarded)
-	double p2g[NPT];
-#pragma unroll
-	for(int k = 0; k < NPT; ++k) p2g[k] = phi2b[s_rowi[qy0 + k].x];
 	double mu[NPT];
 	double per2[1] = { 0.0 };
 #pragma unroll
@@ -711,17 +720,16 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 		block_sum<2, NW>(trxy, s_red);
 		__syncthreads();
 	}
-	const double r8[8] = { red[0], red[1], red[2], red[3], red[4], red[5], red[6], red[7] };
-	const double tot = block_sum8<NW>(r8, s_red);
-	{
+	{   // the interface bounding box: per-wave minima, read after the barrier of block_sum8
 		const int lane = tid & 63, wave = tid >> 6;
 		const int mx = wave_min(xmin), my = wave_min(ymin);
 		if(lane == 0) {
 			s_redi[wave] = mx;
 			s_redi[NW + wave] = my;
 		}
-		__syncthreads();
 	}
+	const double r8[8] = { red[0], red[1], red[2], red[3], red[4], red[5], red[6], red[7] };
+	const double tot = block_sum8<NW>(r8, s_red);
 	if(tid >= 8) return;
 	// totals to lanes 0..3 of wave 0
 	const double area = __shfl(tot, 0, 64), sumF = __shfl(tot, 1, 64);
