@@ -184,14 +184,15 @@ __device__ inline void pad_list(int n, int4 *s_list) {
 	for(int j = n + (int) threadIdx.x; j < np; j += blockDim.x) s_list[j] = make_int4(0, 0, 0, 0);
 }
 
-// Loads the tile's list into LDS (reuse): the count and every entry are
-// requested at once, before the box description
-__device__ int tile_list_load(const int *tile_count, const int4 *tile_list, int tile, int4 *s_list) {
-	const int n = tile_count[tile];
-	for(int j = threadIdx.x; j < n; j += blockDim.x) s_list[j] = tile_list[tile * MAXT + j];
-	pad_list(min(n, MAXT), s_list);
-	__syncthreads();
-	return n;
+// A prebuilt list is read through the scalar cache: the count and the
+// entries are the same for every lane, so they arrive in SGPRs and no LDS
+// staging or barrier is needed.  (The lists are written by an earlier kernel;
+// the padding to a multiple of 16 is written with them.)
+using cint_p = const int __attribute__((address_space(4))) *;
+__device__ inline int list_count(const int *tile_count, int tile) { return ((cint_p) tile_count)[tile]; }
+__device__ inline int4 list_entry(const int4 *tile_list, int tile, int j) {
+	const cint_p e = (cint_p) (tile_list + tile * MAXT + j);
+	return make_int4(e[0], e[1], e[2], e[3]);
 }
 
 // Builds the tile's list from the compact cell geometry (cell order) into LDS
@@ -242,6 +243,8 @@ __device__ int tile_list_build(const GatherArgs &a, int tile, const DevBox &B, i
 	const int n = *s_n;
 	if(tid == 0) a.tile_count[tile] = min(n, MAXT);
 	pad_list(min(n, MAXT), s_list);
+	for(int j = min(n, MAXT) + tid; j < ((min(n, MAXT) + 15) & ~15); j += blockDim.x)
+		a.tile_geo[tile * MAXT + j] = make_int4(0, 0, 0, 0);
 	__syncthreads();
 	return n;
 }
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
 	const int tw = min(TW, L - x0), th = min(TH, L - y0);
 	const int n = a.build_lists ? tile_list_build(a, tile, B, x0, y0, tw, th, s_list, s_wcnt, &s_n)
-			: tile_list_load(a.tile_count, a.tile_geo, tile, s_list);
+			: list_count(a.tile_count, tile);
 	if(n > MAXT) {
 		if(tid == 0) atomicOr(&a.st->flags, ST_TILE);
 		return;
@@ -273,20 +276,24 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 	if(sx >= tw || sy >= th) return;
 	double acc = 0.0;
 	int cnt = 0;
-	for(int base = 0; base < n; base += GCHUNK_PHI) {
-		int row[GCHUNK_PHI];
-		double v[GCHUNK_PHI];
+	auto sites = [&](auto entry) {
+		for(int base = 0; base < n; base += GCHUNK_PHI) {
+			int row[GCHUNK_PHI];
+			double v[GCHUNK_PHI];
 #pragma unroll
-		for(int k = 0; k < GCHUNK_PHI; ++k) row[k] = entry_row(s_list[base + k], sx, sy, L);
+			for(int k = 0; k < GCHUNK_PHI; ++k) row[k] = entry_row(entry(base + k), sx, sy, L);
 #pragma unroll
-		for(int k = 0; k < GCHUNK_PHI; ++k) v[k] = row[k] >= 0 ? a.phi[row[k]] : 0.;
+			for(int k = 0; k < GCHUNK_PHI; ++k) v[k] = row[k] >= 0 ? a.phi[row[k]] : 0.;
 #pragma unroll
-		for(int k = 0; k < GCHUNK_PHI; ++k)
-			if(row[k] >= 0) {
-				acc += v[k] * v[k];
-				++cnt;
-			}
-	}
+			for(int k = 0; k < GCHUNK_PHI; ++k)
+				if(row[k] >= 0) {
+					acc += v[k] * v[k];
+					++cnt;
+				}
+		}
+	};
+	if(a.build_lists) sites([&](int j) { return s_list[j]; });
+	else sites([&](int j) { return list_entry(a.tile_geo, tile, j); });
 	const int k = B.box_off + (y0 + sy) * L + x0 + sx;
 	a.phi2[k] = acc;
 	a.cover[k] = cnt;
@@ -1106,10 +1113,9 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 	if(a.st->flags) return;
 	const CGState *cg = a.cg;
 	if(MODE != CG_GATHER_X && !cg->active) return;
-	__shared__ int4 s_list[MAXT];
 	const int tile = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
 	const DevBox B = a.boxes[a.tile_box[tile]];
-	const int n = tile_list_load(a.tile_count, a.tile_geo, tile, s_list);
+	const int n = list_count(a.tile_count, tile);
 	const int t = tile - B.tile_off, L = B.L;
 	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
 	const int tw = min(TW, L - x0), th = min(TH, L - y0);
@@ -1124,7 +1130,7 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 			const double2 n4 = a.n4[k];
 			double2 acc = make_double2(0., 0.), acc2 = make_double2(0., 0.);
 			for(int j = 0; j < n; ++j) {
-				const int row = entry_row(s_list[j], sx, sy, L);
+				const int row = entry_row(list_entry(a.tile_geo, tile, j), sx, sy, L);
 				if(row < 0) continue;
 				const double2 r = a.r[row];
 				const double2 e = make_double2((r.x + eta * n4.x) / D, (r.y + eta * n4.y) / D);
@@ -1155,7 +1161,7 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 		int row[GCHUNK];
 		double2 rv[GCHUNK], pv[GCHUNK];
 #pragma unroll
-		for(int q = 0; q < GCHUNK; ++q) row[q] = entry_row(s_list[base + q], sx, sy, L);
+		for(int q = 0; q < GCHUNK; ++q) row[q] = entry_row(list_entry(a.tile_geo, tile, base + q), sx, sy, L);
 #pragma unroll
 		for(int q = 0; q < GCHUNK; ++q) {
 			rv[q] = make_double2(0., 0.);
