@@ -1033,7 +1033,20 @@ __device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[
 	double acc[NV];
 #pragma unroll
 	for(int j = 0; j < NV; ++j) acc[j] = 0.;
-	for(int b = threadIdx.x; b < nblk; b += NT) {
+	// four partials per thread and value requested at once, then added in order
+	int b = threadIdx.x;
+	for(; b + 3 * NT < nblk; b += 4 * NT) {
+		double v[4][NV];
+#pragma unroll
+		for(int u = 0; u < 4; ++u)
+#pragma unroll
+			for(int j = 0; j < NV; ++j) v[u][j] = part[j * nblk + b + u * NT];
+#pragma unroll
+		for(int u = 0; u < 4; ++u)
+#pragma unroll
+			for(int j = 0; j < NV; ++j) acc[j] += v[u][j];
+	}
+	for(; b < nblk; b += NT) {
 #pragma unroll
 		for(int j = 0; j < NV; ++j) acc[j] += part[j * nblk + b];
 	}
