@@ -196,51 +196,53 @@ __device__ inline int4 list_entry(const int4 *tile_list, int tile, int j) {
 }
 
 // Builds the tile's list from the compact cell geometry (cell order) into LDS
-// and the global list; returns its length.  The geometry of the first
-// GB_PRE * 256 cells is requested at once.
-constexpr int GB_PRE = 8;
+// and the global list; returns its length.  Thread t tests cells 4t .. 4t+3
+// of each chunk of 1024 (their geometry requested at once); a ballot prefix
+// over the wave and the wave totals place the hits in cell order.
 __device__ int tile_list_build(const GatherArgs &a, int tile, const DevBox &B, int x0, int y0, int tw, int th,
-		int4 *s_list, int *s_wcnt, int *s_n) {
+		int4 *s_list, int *s_wcnt) {
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, L = B.L;
-	if(tid == 0) *s_n = 0;
-	int4 gp[GB_PRE];
+	const unsigned long long lt = (1ull << lane) - 1ull;
+	int n = 0;
+	for(int base = B.cell_begin; base < B.cell_end; base += 1024) {
+		int4 g[4];
+		bool hit[4];
 #pragma unroll
-	for(int r = 0; r < GB_PRE; ++r) {
-		const int c = B.cell_begin + r * 256 + tid;
-		gp[r] = c < B.cell_end ? a.geo[c] : make_int4(0, 0, 0, 0);
-	}
-	__syncthreads();
-	auto round = [&](int c, const int4 g) {
-		const bool hit = c < B.cell_end && circ_overlap(g.x % L, g.y & 0xFFFF, x0, tw, L) &&
-				circ_overlap(g.x / L, g.y >> 16, y0, th, L);
-		const unsigned long long m = __ballot(hit);
-		const int pre = __popcll(m & ((1ull << lane) - 1ull));
-		if(lane == 0) s_wcnt[wave] = __popcll(m);
-		__syncthreads();
-		int off = *s_n;
-		for(int w = 0; w < wave; ++w) off += s_wcnt[w];
-		const int pos = off + pre;
-		if(hit && pos < MAXT) {
-			const int4 e = tile_entry(g, L, x0, y0);
-			s_list[pos] = e;
-			a.tile_cells[tile * MAXT + pos] = c;
-			a.tile_geo[tile * MAXT + pos] = e;
+		for(int u = 0; u < 4; ++u) {
+			const int c = base + 4 * tid + u;
+			g[u] = c < B.cell_end ? a.geo[c] : make_int4(0, 0, 0, 0);
 		}
-		__syncthreads();
-		if(tid == 0) *s_n += s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
-		__syncthreads();
-	};
+		int pre = 0, tot = 0;
 #pragma unroll
-	for(int r = 0; r < GB_PRE; ++r) {
-		const int base = B.cell_begin + r * 256;
-		if(base >= B.cell_end) break;
-		round(base + tid, gp[r]);
+		for(int u = 0; u < 4; ++u) {
+			const int c = base + 4 * tid + u;
+			hit[u] = c < B.cell_end && circ_overlap(g[u].x % L, g[u].y & 0xFFFF, x0, tw, L) &&
+					circ_overlap(g[u].x / L, g[u].y >> 16, y0, th, L);
+			const unsigned long long m = __ballot(hit[u]);
+			pre += __popcll(m & lt);
+			tot += __popcll(m);
+		}
+		if(lane == 0) s_wcnt[wave] = tot;
+		__syncthreads();
+		int pos = n + pre, all = 0;
+		for(int w = 0; w < 4; ++w) {
+			if(w < wave) pos += s_wcnt[w];
+			all += s_wcnt[w];
+		}
+#pragma unroll
+		for(int u = 0; u < 4; ++u)
+			if(hit[u]) {
+				if(pos < MAXT) {
+					const int4 e = tile_entry(g[u], L, x0, y0);
+					s_list[pos] = e;
+					a.tile_cells[tile * MAXT + pos] = base + 4 * tid + u;
+					a.tile_geo[tile * MAXT + pos] = e;
+				}
+				++pos;
+			}
+		n += all;
+		__syncthreads();   // s_wcnt is rewritten by the next chunk
 	}
-	for(int base = B.cell_begin + GB_PRE * 256; base < B.cell_end; base += 256) {
-		const int c = base + tid;
-		round(c, c < B.cell_end ? a.geo[c] : make_int4(0, 0, 0, 0));
-	}
-	const int n = *s_n;
 	if(tid == 0) a.tile_count[tile] = min(n, MAXT);
 	pad_list(min(n, MAXT), s_list);
 	for(int j = min(n, MAXT) + tid; j < ((min(n, MAXT) + 15) & ~15); j += blockDim.x)
@@ -260,13 +262,12 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 	if(a.st->flags) return;
 	__shared__ int4 s_list[MAXT];
 	__shared__ int s_wcnt[4];
-	__shared__ int s_n;
 	const int tile = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
 	const DevBox B = a.boxes[a.tile_box[tile]];
 	const int t = tile - B.tile_off, L = B.L;
 	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
 	const int tw = min(TW, L - x0), th = min(TH, L - y0);
-	const int n = a.build_lists ? tile_list_build(a, tile, B, x0, y0, tw, th, s_list, s_wcnt, &s_n)
+	const int n = a.build_lists ? tile_list_build(a, tile, B, x0, y0, tw, th, s_list, s_wcnt)
 			: list_count(a.tile_count, tile);
 	if(n > MAXT) {
 		if(tid == 0) atomicOr(&a.st->flags, ST_TILE);
