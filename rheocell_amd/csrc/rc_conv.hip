@@ -60,9 +60,11 @@ __device__ inline unsigned pack2bf(float a, float b) {
 // element offset of 16-byte chunk c (0..3) of LDS row q
 __device__ inline int swz(int q, int c) { return q * 32 + ((c ^ ((q >> 2) & 3)) << 3); }
 
+// MODE and OUT (bf16 store wanted) are compile-time; fout may be null.
+template <int MODE, bool OUT>
 __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restrict__ in, const unsigned short *__restrict__ w,
 		const float *__restrict__ bias, unsigned short *__restrict__ out, const unsigned short *__restrict__ mask,
-		float *__restrict__ fout, int H, int W, int mode) {
+		float *__restrict__ fout, int H, int W) {
 	__shared__ __attribute__((aligned(16))) unsigned short s_in[CONV_SH * CONV_SW * 32];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int bx = (W + CONV_BW - 1) / CONV_BW;
@@ -100,6 +102,13 @@ __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restric
 	__syncthreads();
 	const int y = y0 + wave;
 	if(y >= H) return;
+	// LDS element offset of each tap's operand for tile 0: staged row wave + dy,
+	// column i + dx.  Tile t adds t * 16 rows of 32 elements: the swizzle depends
+	// on bits 2..3 of the row index, which a step of 16 leaves unchanged.
+	int tap_off[9];
+#pragma unroll
+	for(int tap = 0; tap < 9; ++tap) tap_off[tap] = swz((wave + tap / 3) * CONV_SW + i + tap % 3, kb);
+#pragma unroll
 	for(int t = 0; t < CONV_TILES; ++t) {
 		const int px = x0 + t * 16 + i;
 		if(x0 + t * 16 >= W) break;
@@ -107,8 +116,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restric
 		bf16x8 a[9];
 #pragma unroll
 		for(int tap = 0; tap < 9; ++tap) {
-			const int dy = tap / 3, dx = tap % 3;   // staged row wave + dy, column t*16 + i + dx
-			const u16x8 v = *reinterpret_cast<const u16x8 *>(&s_in[swz((wave + dy) * CONV_SW + t * 16 + i + dx, kb)]);
+			const u16x8 v = *reinterpret_cast<const u16x8 *>(&s_in[tap_off[tap] + t * 16 * 32]);
 			a[tap] = __builtin_bit_cast(bf16x8, v);
 		}
 #pragma unroll
@@ -125,15 +133,15 @@ __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restric
 			z1[r] = acc1[r] + bz1[r];
 		}
 		if(fout && kb == 0) *reinterpret_cast<float2 *>(fout + pix * 2) = make_float2(z0[0], z0[1]);
-		if(!out) continue;
-		if(mode == 0) {
+		if(!OUT) continue;
+		if(MODE == 0) {
 #pragma unroll
 			for(int r = 0; r < 4; ++r) {
 				z0[r] = fmaxf(z0[r], 0.f);
 				z1[r] = fmaxf(z1[r], 0.f);
 			}
 		}
-		else if(mode == 2) {
+		else if(MODE == 2) {
 			const uint2 m0 = *reinterpret_cast<const uint2 *>(mask + pix * 32 + 4 * kb);
 			const uint2 m1 = *reinterpret_cast<const uint2 *>(mask + pix * 32 + 16 + 4 * kb);
 			const unsigned mm0[4] = { m0.x & 0xFFFFu, m0.x >> 16, m0.y & 0xFFFFu, m0.y >> 16 };
@@ -153,7 +161,11 @@ hipError_t launch_conv3x3(const unsigned short *in, const unsigned short *w, con
 		const unsigned short *mask, float *fout, int N, int H, int W, int mode, hipStream_t s) {
 	if(W % 16 || W < 32 || H < 3) return hipErrorInvalidValue;   // the staging wraps assume these
 	const int blocks = ((W + CONV_BW - 1) / CONV_BW) * ((H + CONV_ROWS - 1) / CONV_ROWS);
-	hipLaunchKernelGGL(k_conv3x3, dim3(blocks, N), dim3(256), 0, s, in, w, bias, out, mask, fout, H, W, mode);
+	const dim3 g(blocks, N);
+	if(!out) hipLaunchKernelGGL((k_conv3x3<1, false>), g, dim3(256), 0, s, in, w, bias, out, mask, fout, H, W);
+	else if(mode == 0) hipLaunchKernelGGL((k_conv3x3<0, true>), g, dim3(256), 0, s, in, w, bias, out, mask, fout, H, W);
+	else if(mode == 2) hipLaunchKernelGGL((k_conv3x3<2, true>), g, dim3(256), 0, s, in, w, bias, out, mask, fout, H, W);
+	else hipLaunchKernelGGL((k_conv3x3<1, true>), g, dim3(256), 0, s, in, w, bias, out, mask, fout, H, W);
 	return hipGetLastError();
 }
 
