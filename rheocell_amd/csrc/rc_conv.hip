@@ -33,20 +33,20 @@ __device__ inline bf16x8 load8(const unsigned short *p) {
 // mode: 0 store bf16 ReLU(z); 1 store bf16 z; 2 store bf16 z masked by (mask > 0) (backprop through ReLU);
 // out may be null (no bf16 store); fout (optional): fp32 copy of channels 0..1 of z.
 //
-// A workgroup (4 waves) computes a CONV_ROWS x (16 CONV_TILES) pixel block:
-// wave w takes image row y0 + w and its CONV_TILES 16-pixel tiles, with the
-// layer's weights held in registers (9 taps x 2 output-channel halves = 18 A
-// operands, loaded from L2).  The block's input rows y0-1 .. y0+CONV_ROWS and
-// columns x0-1 .. x0+16 CONV_TILES (circular) are staged once in LDS, so every
-// input pixel leaves HBM/L2 about (CONV_ROWS+2)/CONV_ROWS times instead of 9
-// times.  (Staging the weights in LDS as well measured slower: the larger
-// LDS footprint cuts residency from 4 to 3 workgroups per CU.)
+// A workgroup (CONV_ROWS waves) computes a CONV_ROWS x (16 CONV_TILES) pixel
+// block: wave w takes image row y0 + w and its CONV_TILES 16-pixel tiles, with
+// the layer's weights held in registers (9 taps x 2 output-channel halves = 18
+// A operands).  The weights are staged in LDS once per workgroup and the
+// block's input rows y0-1 .. y0+CONV_ROWS and columns x0-1 .. x0+16 CONV_TILES
+// (circular) once as well, so every input pixel leaves HBM/L2 about
+// (CONV_ROWS+2)/CONV_ROWS times instead of 9 times.
 // LDS rows are 64 bytes (32 bf16) with the 16-byte chunk index XOR-swizzled by
 // bits 2..3 of the row index: the 16 lanes of a 128-bit read of 16
 // consecutive rows then hit distinct banks.  Weights are the A operand and
 // pixels the B operand, so D[co][px] leaves each lane 4 consecutive channels
 // of one pixel: bf16 results go out as 8-byte stores.
-constexpr int CONV_ROWS = 4;
+constexpr int CONV_ROWS = 8;                       // image rows per workgroup, one per wave
+constexpr int CONV_THREADS = 64 * CONV_ROWS;
 constexpr int CONV_TILES = 4;
 constexpr int CONV_BW = 16 * CONV_TILES;           // block width in pixels
 constexpr int CONV_SW = CONV_BW + 2;               // staged columns
@@ -62,17 +62,18 @@ __device__ inline int swz(int q, int c) { return q * 32 + ((c ^ ((q >> 2) & 3)) 
 
 // MODE and OUT (bf16 store wanted) are compile-time; fout may be null.
 template <int MODE, bool OUT>
-__global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restrict__ in, const unsigned short *__restrict__ w,
-		const float *__restrict__ bias, unsigned short *__restrict__ out, const unsigned short *__restrict__ mask,
-		float *__restrict__ fout, int H, int W) {
+__global__ __launch_bounds__(CONV_THREADS) void k_conv3x3(const unsigned short *__restrict__ in,
+		const unsigned short *__restrict__ w, const float *__restrict__ bias, unsigned short *__restrict__ out,
+		const unsigned short *__restrict__ mask, float *__restrict__ fout, int H, int W) {
 	__shared__ __attribute__((aligned(16))) unsigned short s_in[CONV_SH * CONV_SW * 32];
+	__shared__ __attribute__((aligned(16))) unsigned short s_w[9 * 32 * 32];   // rows (tap, co), swizzled like s_in
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int bx = (W + CONV_BW - 1) / CONV_BW;
 	const int x0 = (blockIdx.x % bx) * CONV_BW, y0 = (blockIdx.x / bx) * CONV_ROWS;
 	const int n = blockIdx.y;
 	const size_t img = (size_t) n * H * W;
 	// stage: CONV_SH rows x CONV_SW columns x 4 chunks of 8 channels
-	for(int e = threadIdx.x; e < CONV_SH * CONV_SW * 4; e += 256) {
+	for(int e = threadIdx.x; e < CONV_SH * CONV_SW * 4; e += CONV_THREADS) {
 		const int ch = e & 3, q = e >> 2, col = q % CONV_SW, r = q / CONV_SW;
 		// circular (operands lie in [-1, 2W) and [-1, 2H) as W >= 32 >= 16 CONV_TILES / 2 and
 		// H >= 3); columns past W of a partial block wrap too (staged, never read)
@@ -86,13 +87,10 @@ __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restric
 		const uint4 v = *reinterpret_cast<const uint4 *>(in + (img + (size_t) ys * W + xs) * 32 + ch * 8);
 		*reinterpret_cast<uint4 *>(&s_in[swz(q, ch)]) = v;
 	}
+	// the layer's weights, once per workgroup (every wave reads them from LDS)
+	for(int e = threadIdx.x; e < 9 * 32 * 4; e += CONV_THREADS)
+		*reinterpret_cast<uint4 *>(&s_w[swz(e >> 2, e & 3)]) = *reinterpret_cast<const uint4 *>(w + (size_t) e * 8);
 	const int i = lane & 15, kb = lane >> 4;
-	bf16x8 wa[9], wb[9];
-#pragma unroll
-	for(int tap = 0; tap < 9; ++tap) {
-		wa[tap] = load8(w + ((size_t) tap * 32 + i) * 32 + kb * 8);
-		wb[tap] = load8(w + ((size_t) tap * 32 + 16 + i) * 32 + kb * 8);
-	}
 	float bz0[4], bz1[4];
 #pragma unroll
 	for(int r = 0; r < 4; ++r) {
@@ -100,6 +98,12 @@ __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restric
 		bz1[r] = bias ? bias[16 + 4 * kb + r] : 0.f;
 	}
 	__syncthreads();
+	bf16x8 wa[9], wb[9];
+#pragma unroll
+	for(int tap = 0; tap < 9; ++tap) {
+		wa[tap] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&s_w[swz(tap * 32 + i, kb)]));
+		wb[tap] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&s_w[swz(tap * 32 + 16 + i, kb)]));
+	}
 	const int y = y0 + wave;
 	if(y >= H) return;
 	// LDS element offset of each tap's operand for tile 0: staged row wave + dy,
@@ -162,10 +166,11 @@ hipError_t launch_conv3x3(const unsigned short *in, const unsigned short *w, con
 	if(W % 16 || W < 32 || H < 3) return hipErrorInvalidValue;   // the staging wraps assume these
 	const int blocks = ((W + CONV_BW - 1) / CONV_BW) * ((H + CONV_ROWS - 1) / CONV_ROWS);
 	const dim3 g(blocks, N);
-	if(!out) hipLaunchKernelGGL((k_conv3x3<1, false>), g, dim3(256), 0, s, in, w, bias, out, mask, fout, H, W);
-	else if(mode == 0) hipLaunchKernelGGL((k_conv3x3<0, true>), g, dim3(256), 0, s, in, w, bias, out, mask, fout, H, W);
-	else if(mode == 2) hipLaunchKernelGGL((k_conv3x3<2, true>), g, dim3(256), 0, s, in, w, bias, out, mask, fout, H, W);
-	else hipLaunchKernelGGL((k_conv3x3<1, true>), g, dim3(256), 0, s, in, w, bias, out, mask, fout, H, W);
+	const dim3 b(CONV_THREADS);
+	if(!out) hipLaunchKernelGGL((k_conv3x3<1, false>), g, b, 0, s, in, w, bias, out, mask, fout, H, W);
+	else if(mode == 0) hipLaunchKernelGGL((k_conv3x3<0, true>), g, b, 0, s, in, w, bias, out, mask, fout, H, W);
+	else if(mode == 2) hipLaunchKernelGGL((k_conv3x3<2, true>), g, b, 0, s, in, w, bias, out, mask, fout, H, W);
+	else hipLaunchKernelGGL((k_conv3x3<1, true>), g, b, 0, s, in, w, bias, out, mask, fout, H, W);
 	return hipGetLastError();
 }
 
