@@ -1010,21 +1010,42 @@ __device__ void cg_after_pap(CGState *cg, const double *tot) {
 	for(int c = 0; c < 2; ++c)
 		if(!cg->done[c]) cg->alpha[c] = cg->absNew[c] / tot[c];
 }
-__device__ void cg_after_update(CGState *cg, const double *tot) {
+// the state the update recurrence reads, taken before the sums are ready
+struct CGUpdIn {
+	int done[2], iters[2], max_iter;
+	double thr[2], absNew[2];
+};
+__device__ inline CGUpdIn cg_update_inputs(const CGState *cg) {
+	CGUpdIn u;
+#pragma unroll
 	for(int c = 0; c < 2; ++c) {
-		if(cg->done[c]) continue;
-		cg->iters[c] += 1;
+		u.done[c] = cg->done[c];
+		u.iters[c] = cg->iters[c];
+		u.thr[c] = cg->thr[c];
+		u.absNew[c] = cg->absNew[c];
+	}
+	u.max_iter = cg->max_iter;
+	return u;
+}
+__device__ void cg_after_update(CGState *cg, const double *tot, const CGUpdIn &u) {
+	int done[2] = { u.done[0], u.done[1] };
+#pragma unroll
+	for(int c = 0; c < 2; ++c) {
+		if(done[c]) continue;
+		const int it = u.iters[c] + 1;
+		cg->iters[c] = it;
 		const double rn2 = tot[c];
 		cg->rn2[c] = rn2;
-		if(rn2 < cg->thr[c] || cg->iters[c] >= cg->max_iter) {
+		if(rn2 < u.thr[c] || it >= u.max_iter) {
+			done[c] = 1;
 			cg->done[c] = 1;
 			continue;
 		}
-		const double absOld = cg->absNew[c];
+		const double absOld = u.absNew[c];
 		cg->absNew[c] = tot[2 + c];
-		cg->beta[c] = cg->absNew[c] / absOld;
+		cg->beta[c] = tot[2 + c] / absOld;
 	}
-	cg->active = !(cg->done[0] && cg->done[1]);
+	cg->active = !(done[0] && done[1]);
 }
 
 // sums SoA partials of the previous kernel(s) over a block of NT threads in a
@@ -1034,22 +1055,26 @@ __device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[
 	double acc[NV];
 #pragma unroll
 	for(int j = 0; j < NV; ++j) acc[j] = 0.;
-	// four partials per thread and value requested at once, then added in order
-	int b = threadIdx.x;
-	for(; b + 3 * NT < nblk; b += 4 * NT) {
+	// Up to four partials per thread and value requested at once, then added
+	// in order; a slot past nblk reads the last partial and adds +0 instead
+	// (every sum unchanged).  32-bit byte offsets from the uniform base keep
+	// one address register per load.
+	const char *base = reinterpret_cast<const char *>(part);
+	for(int b = threadIdx.x; b < nblk; b += 4 * NT) {
 		double v[4][NV];
 #pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			const int bu = min(b + u * NT, nblk - 1);
+#pragma unroll
+			for(int j = 0; j < NV; ++j) {
+				const unsigned off = (unsigned) (j * nblk + bu) * 8u;
+				v[u][j] = *reinterpret_cast<const double *>(base + off);
+			}
+		}
+#pragma unroll
 		for(int u = 0; u < 4; ++u)
 #pragma unroll
-			for(int j = 0; j < NV; ++j) v[u][j] = part[j * nblk + b + u * NT];
-#pragma unroll
-		for(int u = 0; u < 4; ++u)
-#pragma unroll
-			for(int j = 0; j < NV; ++j) acc[j] += v[u][j];
-	}
-	for(; b < nblk; b += NT) {
-#pragma unroll
-		for(int j = 0; j < NV; ++j) acc[j] += part[j * nblk + b];
+			for(int j = 0; j < NV; ++j) acc[j] += b + u * NT < nblk ? v[u][j] : 0.;
 	}
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -1062,6 +1087,7 @@ __device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[
 #pragma unroll
 	for(int j = 0; j < NV; ++j) {
 		double t = 0.;
+#pragma unroll 4
 		for(int w = 0; w < NT / 64; ++w) t += s_red[w * NV + j];
 		tot[j] = t;
 	}
@@ -1070,13 +1096,21 @@ __device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[
 
 template <int WHAT>
 __global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
-	if(a.st->flags) return;
 	CGState *cg = a.cg;
-	if((WHAT == 1 || WHAT == 2) && !cg->active) return;
 	constexpr int NV = WHAT == 0 ? 6 : WHAT == 1 || WHAT == 3 ? 2 : 4;
 	__shared__ double s_red[16 * 6];
+	// The status word and the solve's flag only decide whether the sums are
+	// used: they, and the state the recurrence reads, are requested with the
+	// partials instead of ahead of them (partials a skipped launch left are
+	// read and discarded).
+	const unsigned flags = a.st->flags;
+	const int active = cg->active;
+	CGUpdIn upd;
+	if(WHAT == 2 && threadIdx.x == 0) upd = cg_update_inputs(cg);
 	double tot[NV];
 	reduce_parts<NV>(WHAT == 1 ? a.part_box : a.part, nblk, tot, s_red);
+	if(flags) return;
+	if((WHAT == 1 || WHAT == 2) && !active) return;
 	if(WHAT == 3) {
 		double eae[2];
 		reduce_parts<2>(a.part_box, a.nbox_blk, eae, s_red);
@@ -1101,7 +1135,7 @@ __global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
 		const double t6[6] = { cg->bb[0], cg->bb[1], tot[0], tot[1], tot[2], tot[3] };
 		cg_after_resid(cg, t6, a.tol, a.max_iter);
 	}
-	else cg_after_update(cg, tot);
+	else cg_after_update(cg, tot, upd);
 }
 
 hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s) {
