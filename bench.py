@@ -80,6 +80,9 @@ def parse():
                     help="force the field-update kernel shape (threads x strip rows) instead of the autotuner's choice")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
                     help="only fill and copy a MIB-sized device buffer 3 times (counter calibration), then exit")
+    ap.add_argument("--probe-bw", action="store_true",
+                    help="only time torch's own streaming kernels (copy, sum, add) at 1 GiB and at the wet "
+                         "row-vector size, the practical HBM ceiling of this part, then exit")
     return ap.parse_args()
 
 
@@ -174,10 +177,45 @@ def probe_copy(mib):
     print(json.dumps({"probe_copy_bytes_read": n * 8, "probe_copy_bytes_written": n * 8, "copies": 3}))
 
 
+def probe_bw():
+    """Achieved bandwidth of torch's streaming kernels (bytes moved / event-timed
+    duration): the practical ceiling next to the 8 TB/s published peak.  At
+    43 MB (one double2 row vector of the 512^2 / 1600-cell wet workload) the
+    arrays partly stay in the 256 MB last-level cache."""
+    import torch
+    out = {}
+
+    def rate(name, fn, nbytes, reps):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) * 1e-3 / reps
+        out[name] = {"GB/s": round(nbytes / t / 1e9, 1), "us": round(t * 1e6, 2), "bytes": nbytes}
+
+    for label, n, reps in (("1GiB", 128 << 20, 20), ("43MB", 2 * 2689600, 200)):
+        a = torch.rand(n, dtype=torch.float64, device="cuda")
+        c = torch.rand(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        rate("copy_" + label, lambda: b.copy_(a), 2 * 8 * n, reps)
+        rate("sum_" + label, lambda: a.sum(), 8 * n, reps)
+        rate("add_" + label, lambda: torch.add(a, c, out=b), 3 * 8 * n, reps)
+        del a, b, c
+    print(json.dumps({"probe_bw": out, "peak_GB/s": 8000.0}))
+
+
 def main():
     args = parse()
     if args.probe_copy:
         probe_copy(args.probe_copy)
+        return
+    if args.probe_bw:
+        probe_bw()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "

@@ -824,12 +824,17 @@ static int cg_solve(rc_handle *h, bool cold) {
 		int batch = std::max(2, h->last_iters - 1);
 		long long done_iters = 0;
 		for(;;) {
+			// an iteration's state update is written by the next iteration's
+			// k_cg_box<1> (its beta is derived in k_cg_gather), the batch's last
+			// by k_cg_reduce<2>
 			for(int it = 0; it < batch; ++it) {
+				c.apply_update = it > 0 ? 1 : 0;
 				{ TimeScope t(h, RC_T_CG_BOX_PAP); HIPCHK(h, launch_cg_box(1, c, (int) h->box_sites, h->stream)); }
 				{ TimeScope t(h, RC_T_CG_ROW_UPDATE); HIPCHK(h, launch_cg_row(CG_ROW_UPDATE, c, h->stream)); }
-				{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(2, c, h->ncells, h->stream)); }
 				{ TimeScope t(h, RC_T_CG_GATHER_P); HIPCHK(h, launch_cg_gather(CG_GATHER_P_UPDATE, c, h->ntiles, h->stream)); }
 			}
+			c.apply_update = 0;
+			{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(2, c, h->ncells, h->stream)); }
 			done_iters += batch;
 			HIPCHK(h, hipMemcpyAsync(h->h_cg, h->d_cg, sizeof(CGState), hipMemcpyDeviceToHost, h->stream));
 			HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));

@@ -933,8 +933,14 @@ hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hi
 //   k_cg_gather<P>  p = z + beta p (z = r / D), Sp(k), P2(k) = sum p^2 at k
 //   k_cg_box<1>     N4(k);  p.Ap = sum_k D(k) P2(k) - eta Sp(k) N4(k)  -> alpha
 //   k_cg_row<UPD>   x += alpha p, r -= alpha (D p - eta N4), r.r, r.z  -> beta
-// Reductions: per-workgroup partials, summed in a fixed order by a
-// one-workgroup k_cg_reduce launch that also runs the scalar recurrences.
+// Reductions: per-workgroup partials, summed in a fixed order.  alpha is
+// re-derived by every k_cg_row workgroup from the p.Ap partials; beta and the
+// convergence flags by every k_cg_gather<P_UPDATE> workgroup from the r.r,
+// r.z partials, which only read the state: workgroup 0 of the next
+// k_cg_box<1> writes that update (apply_update), after the gather has
+// finished, and a one-workgroup k_cg_reduce<2> at the end of a batch of
+// iterations.  All three sum in the same 256-thread order, so they agree bit
+// for bit.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cg_diag(CGArgs a, int nsites) {
 	const int k = blockIdx.x * 256 + threadIdx.x;
@@ -1047,23 +1053,42 @@ __device__ void cg_after_update(CGState *cg, const double *tot, const CGUpdIn &u
 	}
 	cg->active = !(done[0] && done[1]);
 }
+// beta and the convergence flags cg_after_update would set, without writing
+// the state (the same expressions, so the same bits)
+__device__ inline bool cg_update_peek(const CGUpdIn &u, const double *tot, bool (&done)[2], double (&beta)[2]) {
+#pragma unroll
+	for(int c = 0; c < 2; ++c) {
+		done[c] = u.done[c];
+		beta[c] = 0.;
+		if(done[c]) continue;
+		const int it = u.iters[c] + 1;
+		if(tot[c] < u.thr[c] || it >= u.max_iter) {
+			done[c] = true;
+			continue;
+		}
+		beta[c] = tot[2 + c] / u.absNew[c];
+	}
+	return !(done[0] && done[1]);
+}
 
 // sums SoA partials of the previous kernel(s) over a block of NT threads in a
-// fixed order (deterministic, the same in every workgroup); every thread gets tot
-template <int NV, int NT = 1024>
+// fixed order (deterministic, the same in every workgroup); every thread gets
+// tot.  U partials per thread and value are requested at once: U changes
+// only how the loads are grouped, not the order of the additions.
+template <int NV, int NT = 1024, int U = 4>
 __device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[NV], double *s_red) {
 	double acc[NV];
 #pragma unroll
 	for(int j = 0; j < NV; ++j) acc[j] = 0.;
-	// Up to four partials per thread and value requested at once, then added
-	// in order; a slot past nblk reads the last partial and adds +0 instead
+	// Up to U partials per thread and value requested at once, then added in
+	// order; a slot past nblk reads the last partial and adds +0 instead
 	// (every sum unchanged).  32-bit byte offsets from the uniform base keep
 	// one address register per load.
 	const char *base = reinterpret_cast<const char *>(part);
-	for(int b = threadIdx.x; b < nblk; b += 4 * NT) {
-		double v[4][NV];
+	for(int b = threadIdx.x; b < nblk; b += U * NT) {
+		double v[U][NV];
 #pragma unroll
-		for(int u = 0; u < 4; ++u) {
+		for(int u = 0; u < U; ++u) {
 			const int bu = min(b + u * NT, nblk - 1);
 #pragma unroll
 			for(int j = 0; j < NV; ++j) {
@@ -1072,7 +1097,7 @@ __device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[
 			}
 		}
 #pragma unroll
-		for(int u = 0; u < 4; ++u)
+		for(int u = 0; u < U; ++u)
 #pragma unroll
 			for(int j = 0; j < NV; ++j) acc[j] += b + u * NT < nblk ? v[u][j] : 0.;
 	}
@@ -1108,7 +1133,9 @@ __global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
 	CGUpdIn upd;
 	if(WHAT == 2 && threadIdx.x == 0) upd = cg_update_inputs(cg);
 	double tot[NV];
-	reduce_parts<NV>(WHAT == 1 ? a.part_box : a.part, nblk, tot, s_red);
+	// WHAT 2 runs on 256 threads: the order of k_cg_gather<P_UPDATE> and k_cg_box<1>
+	if(WHAT == 2) reduce_parts<NV, 256>(a.part, nblk, tot, s_red);
+	else reduce_parts<NV>(WHAT == 1 ? a.part_box : a.part, nblk, tot, s_red);
 	if(flags) return;
 	if((WHAT == 1 || WHAT == 2) && !active) return;
 	if(WHAT == 3) {
@@ -1145,7 +1172,7 @@ hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s) 
 	case 3: RC_LAUNCH(k_cg_reduce<3>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	case 4: RC_LAUNCH(k_cg_reduce<4>, dim3(1), dim3(1024), 0, s, a, nblk); break;
 	case 5: RC_LAUNCH(k_cg_reduce<5>, dim3(1), dim3(1024), 0, s, a, nblk); break;
-	default: RC_LAUNCH(k_cg_reduce<2>, dim3(1), dim3(1024), 0, s, a, nblk); break;
+	default: RC_LAUNCH(k_cg_reduce<2>, dim3(1), dim3(256), 0, s, a, nblk); break;
 	}
 	return hipGetLastError();
 }
@@ -1194,16 +1221,24 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 		block_partials<2>(er, a.part);
 		return;
 	}
+	bool dn[2] = { false, false };
+	double bt[2] = { 0., 0. };
+	if(MODE == CG_GATHER_P_UPDATE) {
+		// beta and the convergence flags of this iteration from the r.r, r.z
+		// partials of k_cg_row<UPDATE> (the state itself is written later)
+		__shared__ double s_upd[4 * 4];
+		const CGUpdIn u = cg_update_inputs(cg);
+		double tot[4];
+		reduce_parts<4, 256, 8>(a.part, a.ncells, tot, s_upd);
+		if(!cg_update_peek(u, tot, dn, bt)) return;
+	}
 	if(sx >= tw || sy >= th) return;
 	const int x = x0 + sx, y = y0 + sy;
 	const int k = B.box_off + y * L + x;
-	double inv = 0., b0 = 0., b1 = 0.;
-	bool d0 = false, d1 = false;
-	if(MODE != CG_GATHER_X) {
-		inv = 1. / a.diag[k];   // Jacobi: z = invdiag .* r
-		d0 = cg->done[0]; d1 = cg->done[1];
-		b0 = cg->beta[0]; b1 = cg->beta[1];
-	}
+	double inv = 0.;
+	const bool d0 = dn[0], d1 = dn[1];
+	const double b0 = bt[0], b1 = bt[1];
+	if(MODE != CG_GATHER_X) inv = 1. / a.diag[k];   // Jacobi: z = invdiag .* r
 	double2 acc = make_double2(0., 0.), acc2 = make_double2(0., 0.);
 	for(int base = 0; base < n; base += GCHUNK) {
 		int row[GCHUNK];
@@ -1282,6 +1317,15 @@ __global__ __launch_bounds__(256) void k_cg_box(CGArgs a, int nsites) {
 	CGState *cg = a.cg;
 	if(MODE != 0 && !cg->active) return;
 	__shared__ double s_red[4 * 2];
+	if(MODE == 1 && a.apply_update && blockIdx.x == 0) {
+		// the previous iteration's update of the CG state, whose beta and flags
+		// k_cg_gather<P_UPDATE> derived from the same partials
+		__shared__ double s_upd[4 * 4];
+		const CGUpdIn u = cg_update_inputs(cg);
+		double tot[4];
+		reduce_parts<4, 256, 8>(a.part, a.ncells, tot, s_upd);
+		if(threadIdx.x == 0) cg_after_update(cg, tot, u);
+	}
 	const int k = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
 	const DevBox B = a.boxes[0];
 	const double eta = a.params[a.set].friction_cell;
