@@ -849,7 +849,7 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 }
 
 // stage kernel shapes (threads per cell, rows per thread strip)
-static const int STAGE_SHAPES[][2] = { { 256, 4 }, { 256, 7 }, { 512, 4 }, { 512, 7 }, { 512, 10 }, { 512, 16 } };
+static const int STAGE_SHAPES[][2] = { { 256, 4 }, { 256, 7 }, { 512, 4 }, { 512, 7 }, { 512, 8 }, { 512, 16 } };
 constexpr int NSTAGE_SHAPES = 6;
 
 template <int MODE, bool RCP, bool ST>
@@ -860,7 +860,7 @@ static hipError_t set_lds_r(size_t bytes) {
 	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 256, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
 	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
 	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 10, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 8, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
 	return hipFuncSetAttribute((const void *) k_stage<MODE, 16, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
 }
 template <int MODE, bool ST>
@@ -898,7 +898,7 @@ static void launch_mode_r(const StageArgs &a, int ncells, size_t lds, hipStream_
 	switch(a.npt) {
 	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
 	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
-	case 10: RC_LAUNCH((k_stage<MODE, 10, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
+	case 8: RC_LAUNCH((k_stage<MODE, 8, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
 	default: RC_LAUNCH((k_stage<MODE, 16, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
 	}
 }
