@@ -110,7 +110,7 @@ struct rc_handle {
 	int restart_solver = 0;
 	long long size_rows = 0, size_rows_old = 0;
 	int warm_mode = RC_WARM_REFERENCE;
-	int last_iters = 8;
+	int last_iters[2] = { 8, 8 };   // iterations of the last warm / cold solve (first batch size)
 	rc_cg_stats_t cgs{};
 	std::string err;
 
@@ -821,7 +821,11 @@ static int cg_solve(rc_handle *h, bool cold) {
 			h->pending_learned_stat = true;
 		}
 		{ TimeScope t(h, RC_T_CG_GATHER_P_FIRST); HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream)); }
-		int batch = std::max(2, h->last_iters - 1);
+		// First batch: one iteration short of the last solve of the same kind
+		// (warm or cold: a cold solve needs several times the iterations);
+		// then batches of 2, 4, 8, ... so that a poor guess costs few host
+		// round trips.  Iterations past convergence return at entry.
+		int batch = std::max(2, h->last_iters[cold ? 1 : 0] - 1), next = 2;
 		long long done_iters = 0;
 		for(;;) {
 			// an iteration's state update is written by the next iteration's
@@ -842,7 +846,8 @@ static int cg_solve(rc_handle *h, bool cold) {
 			if(h->h_st->flags) break;
 			if(!h->h_cg->active) break;
 			if(done_iters >= c.max_iter + 2) break;
-			batch = 2;
+			batch = next;
+			next = std::min(2 * next, 64);
 		}
 		const CGState &s = *h->h_cg;
 		int mask = (s.zero_rhs[0] ? 1 : 0) | (s.zero_rhs[1] ? 2 : 0);
@@ -870,7 +875,7 @@ static int cg_solve(rc_handle *h, bool cold) {
 	h->cgs.last_cold = cold ? 1 : 0;
 	h->cgs.last_solve_ms = ms;
 	h->cgs.total_solve_ms += ms;
-	h->last_iters = std::max(s.iters[0], s.iters[1]);
+	h->last_iters[cold ? 1 : 0] = std::max(s.iters[0], s.iters[1]);
 	if(s.iters[0] >= c.max_iter || s.iters[1] >= c.max_iter) h->cgs.last_cold |= 2;
 	return RC_OK;
 }
