@@ -51,9 +51,13 @@ extern "C" {
 #define RC_ERR_HIP 2          /* HIP runtime error */
 #define RC_ERR_OVERLAP 3      /* > 20 patches cover one box site (WetPolarModel.cpp:250-255) */
 #define RC_ERR_GROWTH 4       /* a patch must grow (MultiPhaseField.cpp:248-310) */
-#define RC_ERR_CG 5           /* CG hit its iteration cap */
+#define RC_ERR_CG 5           /* reserved: a solve that reaches its iteration cap is reported in
+                                 rc_cg_stats_t.last_cold (bit 1) and the step goes on, as the
+                                 reference's unchecked Eigen solve does (WetPolarModel.cpp:223-232) */
 #define RC_ERR_STATE 6        /* call out of order (e.g. step before upload) */
 #define RC_ERR_TILE 7         /* too many patches intersect one box tile */
+#define RC_ERR_NAN 8          /* a cell's field or the wet solve's residual became NaN (no counterpart:
+                                 the reference carries NaN on); the device halts at the first one */
 
 /* Model parameters of one box, as read from the input file: the *_active
  * values of WetPolarModel::get_settings (WetPolarModel.cpp:28-46) plus the

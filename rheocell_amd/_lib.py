@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "librheocell.so")
 
 RC_OK = 0
 RC_ERR = {1: "RC_ERR_ARG", 2: "RC_ERR_HIP", 3: "RC_ERR_OVERLAP", 4: "RC_ERR_GROWTH", 5: "RC_ERR_CG",
-          6: "RC_ERR_STATE", 7: "RC_ERR_TILE"}
+          6: "RC_ERR_STATE", 7: "RC_ERR_TILE", 8: "RC_ERR_NAN"}
 RC_WARM_REFERENCE = 0
 RC_WARM_PREVIOUS = 1
 RC_WARM_LEARNED = 2

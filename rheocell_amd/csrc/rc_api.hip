@@ -914,6 +914,10 @@ static int check_status(rc_handle *h) {
 	const unsigned f = h->h_st->flags;
 	if(f & ST_OVERLAP) return fail(h, RC_ERR_OVERLAP, "Too many field patches overlap at box site %d", h->h_st->site);
 	if(f & ST_TILE) return fail(h, RC_ERR_TILE, "more than %d patches intersect one 32x8 box tile", MAXT);
+	if(f & ST_NAN) {
+		if(h->h_st->first_cell >= 0) return fail(h, RC_ERR_NAN, "the field of cell %d became NaN", h->h_st->first_cell);
+		return fail(h, RC_ERR_NAN, "the wet solve's right-hand side or residual became NaN");
+	}
 	if(f & ST_GROWTH)
 		return fail(h, RC_ERR_GROWTH, "patch of cell %d must grow (MultiPhaseField::set_positions growth branch)", h->h_st->first_cell);
 	return RC_OK;

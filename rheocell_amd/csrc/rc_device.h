@@ -25,6 +25,7 @@ constexpr int CELL_THREADS = 256;
 constexpr unsigned ST_OVERLAP = 1u;
 constexpr unsigned ST_GROWTH = 2u;
 constexpr unsigned ST_TILE = 4u;
+constexpr unsigned ST_NAN = 8u;
 // a growth request also ORs (step + 1) << ST_STEP_SHIFT into the flags: every
 // cell of one corrector stage writes the same bits, so one plain load tells a
 // later launch of that stage that the request is its own

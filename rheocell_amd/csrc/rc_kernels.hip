@@ -779,6 +779,12 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	th_old = __shfl(th_old, 2, 64);
 	if(tid != 0) return;
 	if(a.pure) return;   // timing run (shape tuner): no cell state is written
+	if(!(area == area) || !(sumF == sumF)) {
+		// a NaN field (no counterpart: the reference carries it on): halt
+		// before it reaches the cell's geometry; the first cell is named
+		if(!(atomicOr(&a.st->flags, ST_NAN) & ST_NAN)) a.st->first_cell = ci;
+		return;
+	}
 	DevCell &G = a.cells[ci];
 	if(store) {
 		G.CoM_oldx = C.CoMx;
@@ -978,11 +984,22 @@ __device__ inline void block_partials(const double (&v)[NV], double *part) {
 	}
 }
 
+// a NaN in the wet solve (no counterpart: Eigen would iterate to its cap):
+// halt the device; first_cell -1 names the solve rather than a cell
+__device__ inline void cg_flag_nan(DevStatus *st) {
+	if(!(atomicOr(&st->flags, ST_NAN) & ST_NAN)) st->first_cell = -1;
+}
+
 // scalar recurrences of Eigen's ConjugateGradient loop
-__device__ void cg_after_resid(CGState *cg, const double *tot, double tol, int max_iter) {
+__device__ void cg_after_resid(CGState *cg, const double *tot, double tol, int max_iter, DevStatus *st) {
 	cg->max_iter = max_iter;
 	for(int c = 0; c < 2; ++c) {
 		const double bb = tot[c], rr = tot[2 + c], rz = tot[4 + c];
+		if(!(bb == bb) || !(rr == rr) || !(rz == rz)) {
+			cg_flag_nan(st);
+			cg->done[c] = 1;
+			continue;
+		}
 		cg->bb[c] = bb;
 		cg->iters[c] = 0;
 		cg->beta[c] = 0.;
@@ -1033,7 +1050,7 @@ __device__ inline CGUpdIn cg_update_inputs(const CGState *cg) {
 	u.max_iter = cg->max_iter;
 	return u;
 }
-__device__ void cg_after_update(CGState *cg, const double *tot, const CGUpdIn &u) {
+__device__ void cg_after_update(CGState *cg, const double *tot, const CGUpdIn &u, DevStatus *st) {
 	int done[2] = { u.done[0], u.done[1] };
 #pragma unroll
 	for(int c = 0; c < 2; ++c) {
@@ -1042,6 +1059,7 @@ __device__ void cg_after_update(CGState *cg, const double *tot, const CGUpdIn &u
 		cg->iters[c] = it;
 		const double rn2 = tot[c];
 		cg->rn2[c] = rn2;
+		if(!(rn2 == rn2)) cg_flag_nan(st);   // later launches return at entry
 		if(rn2 < u.thr[c] || it >= u.max_iter) {
 			done[c] = 1;
 			cg->done[c] = 1;
@@ -1152,17 +1170,17 @@ __global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
 		cg->reverted[1] = r1;
 		if(!r0 && !r1) return;
 		const double t6[6] = { cg->bb[0], cg->bb[1], tot[0], tot[1], tot[2], tot[3] };
-		cg_after_resid(cg, t6, a.tol, a.max_iter);
+		cg_after_resid(cg, t6, a.tol, a.max_iter, a.st);
 		return;
 	}
-	if(WHAT == 0) cg_after_resid(cg, tot, a.tol, a.max_iter);
+	if(WHAT == 0) cg_after_resid(cg, tot, a.tol, a.max_iter, a.st);
 	else if(WHAT == 1) cg_after_pap(cg, tot);
 	else if(WHAT == 4) {
 		// residual of the new start; |b|^2 is that of the solve's first residual pass
 		const double t6[6] = { cg->bb[0], cg->bb[1], tot[0], tot[1], tot[2], tot[3] };
-		cg_after_resid(cg, t6, a.tol, a.max_iter);
+		cg_after_resid(cg, t6, a.tol, a.max_iter, a.st);
 	}
-	else cg_after_update(cg, tot, upd);
+	else cg_after_update(cg, tot, upd, a.st);
 }
 
 hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s) {
@@ -1324,7 +1342,7 @@ __global__ __launch_bounds__(256) void k_cg_box(CGArgs a, int nsites) {
 		const CGUpdIn u = cg_update_inputs(cg);
 		double tot[4];
 		reduce_parts<4, 256, 8>(a.part, a.ncells, tot, s_upd);
-		if(threadIdx.x == 0) cg_after_update(cg, tot, u);
+		if(threadIdx.x == 0) cg_after_update(cg, tot, u, a.st);
 	}
 	const int k = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
 	const DevBox B = a.boxes[0];

@@ -128,6 +128,35 @@ def test_wet_reference_tolerance():
     assert w < 1e-7
 
 
+def test_nan_field_halts():
+    # one NaN in cell 4's field: the first step names the first cell whose field
+    # it reached (through the box sum of phi^2 it reaches the cells overlapping
+    # that site in the same launch) and halts before recentering
+    conf = confgen.lattice_conf(64, 3, seed=7)
+    conf.cells[4].phi = conf.cells[4].phi.copy()
+    conf.cells[4].phi[100] = float("nan")
+    keys = dict(confgen.BASE_KEYS, seed=7)
+    with pytest.raises(RCexception, match=r"RC_ERR_NAN: the field of cell \d+ became NaN"):
+        sim = Simulation([(conf, keys)])
+        sim.step(2)
+    # a clean configuration still runs on a new handle
+    sim = Simulation([(confgen.lattice_conf(64, 3, seed=7), keys)])
+    sim.step(1)
+    sim.close()
+
+
+def test_nan_in_wet_solve_halts():
+    # a NaN cell friction: the first wet solve (the corrector's force
+    # evaluation) has r.z = NaN at its first residual and stops there instead
+    # of iterating to its cap (the cap is also set low, so the test cannot run
+    # long either way)
+    conf = confgen.lattice_conf(64, 3, seed=7)
+    keys = dict(confgen.BASE_KEYS, seed=7, friction_cell=float("nan"), CGtolerance=1e-8)
+    with pytest.raises(RCexception, match="RC_ERR_NAN: the wet solve's right-hand side or residual became NaN"):
+        sim = Simulation([(conf, keys)], cg_max_iter=200)
+        sim.step(1)
+
+
 def test_overlap_limit_raises():
     # 21 patches covering the same sites: the reference throws
     L = 64
