@@ -157,6 +157,46 @@ def test_nan_in_wet_solve_halts():
         sim.step(1)
 
 
+def test_api_argument_and_order_errors():
+    # the C ABI's own checks: a bad box, a call out of order, a patch that does
+    # not fit its box, an unknown warm-start mode
+    import ctypes as C
+    import numpy as np
+    from rheocell_amd import _lib
+    from rheocell_amd.sim import cell_from_conf, model_params
+    L = _lib.load()
+    conf = confgen.lattice_conf(64, 2, seed=3)
+    keys = dict(confgen.BASE_KEYS, seed=3)
+    desc = (_lib.rc_box_desc * 1)()
+    desc[0].L, desc[0].ncells, desc[0].params = 0, len(conf.cells), model_params(keys)
+    h = C.c_void_p()
+    assert L.rc_create(desc, 1, C.byref(h)) == 1          # RC_ERR_ARG: side 0
+    desc[0].L = 64
+    assert L.rc_create(desc, 1, C.byref(h)) == 0
+    try:
+        assert L.rc_step(h, 1, None) == 6                  # RC_ERR_STATE: nothing uploaded or initialised
+        assert L.rc_last_error(h) == b"rc_step before rc_init_forces"
+        assert L.rc_init_forces(h) == 6
+        assert L.rc_last_error(h) == b"rc_init_forces before rc_upload_cells"
+        cells = (_lib.rc_cell * len(conf.cells))()
+        for i, c in enumerate(conf.cells):
+            cells[i] = cell_from_conf(c)
+        cells[1].corner = 64 * 64 + 5                      # corner outside the box
+        phi = np.ascontiguousarray(np.concatenate([c.phi for c in conf.cells]))
+        ptr = phi.ctypes.data_as(C.POINTER(C.c_double))
+        assert L.rc_upload_cells(h, cells, ptr) == 1       # RC_ERR_ARG
+        assert b"corner" in L.rc_last_error(h)
+        cells[1] = cell_from_conf(conf.cells[1])
+        assert L.rc_upload_cells(h, cells, ptr) == 0
+        assert L.rc_set_warm_start(h, 7) == 1              # RC_ERR_ARG: unknown mode
+        assert L.rc_step(h, -1, None) in (1, 6)
+        assert L.rc_init_forces(h) == 0
+        assert L.rc_step(h, -1, None) == 1                 # RC_ERR_ARG: negative step count
+        assert L.rc_step(h, 1, None) == 0
+    finally:
+        L.rc_destroy(h)
+
+
 def test_overlap_limit_raises():
     # 21 patches covering the same sites: the reference throws
     L = 64
