@@ -80,6 +80,9 @@ def parse():
                     help="force the field-update kernel shape (threads x strip rows) instead of the autotuner's choice")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
                     help="only fill and copy a MIB-sized device buffer 3 times (counter calibration), then exit")
+    ap.add_argument("--box-field", type=int, default=0, metavar="SIZE_GRID",
+                    help="only time rc_box_field (VelocityField box fields with a SIZE_GRID coarse average) "
+                         "on the workload after --warmup steps, --steps calls, then exit")
     ap.add_argument("--probe-bw", action="store_true",
                     help="only time torch's own streaming kernels (copy, sum, add) at 1 GiB and at the wet "
                          "row-vector size, the practical HBM ceiling of this part, then exit")
@@ -243,6 +246,19 @@ def main():
     if args.stage_shape:
         t, r = (int(v) for v in args.stage_shape.lower().split("x"))
         sim.set_stage_shape(t, r)
+    if args.box_field:
+        # the VelocityField observable of box 0 on the device: first call after
+        # the warm-up steps untimed, then --steps calls (each includes the copy
+        # of the four L x L outputs to the host)
+        sim.step(max(args.warmup, 1), sim.gaussians(max(args.warmup, 1)))
+        sim.box_field(0, args.box_field)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sim.box_field(0, args.box_field)
+        ms = 1e3 * (time.perf_counter() - t0) / max(args.steps, 1)
+        print(json.dumps({"box_field_ms_per_call": round(ms, 4), "size_grid": args.box_field, "box": L,
+                          "cells": ncells, "workload": args.workload, "calls": args.steps}))
+        return
     rows = sim.total_rows()
     if len(boxes) > 1:
         log("[bench] %d boxes, sides %s" % (len(boxes), sorted(set(c.Lx for c, _ in boxes))))

@@ -132,6 +132,13 @@ int rc_get_rows(rc_handle *h, int which, double *out);
 /* which: 0 phi, 1 freeEnergy, 2 fieldDX, 3 fieldDY, 4 laplacianPhi -- all cells concatenated */
 int rc_get_site_array(rc_handle *h, int which, double *out);
 int rc_total_rows(rc_handle *h, long long *rows);
+/* Box fields of the observables VelocityField (which 0; VelocityField.cpp:69-128)
+ * and ShapeField (which 1; ShapeField.cpp:69-128) for box `box`, on the device
+ * from the current fields: f(k) = sum over the patches covering box site k (cell
+ * order) of w * phi, w = the velocity row (vx, vy) or the cell's (S00, S01);
+ * c = the size_grid x size_grid periodic average of f (f_field_coarse).  Each
+ * output is L*L doubles, site x + L y; any may be NULL. */
+int rc_box_field(rc_handle *h, int box, int which, int size_grid, double *fx, double *fy, double *cx, double *cy);
 int rc_cg_stats(rc_handle *h, rc_cg_stats_t *out);
 int rc_set_warm_start(rc_handle *h, int mode);
 /* Kernel timing with HIP events on the library's stream (enabled by

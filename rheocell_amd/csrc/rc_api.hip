@@ -1534,6 +1534,47 @@ extern "C" int rc_get_site_array(rc_handle *h, int which, double *out) {
 	return RC_OK;
 }
 
+// VelocityField / ShapeField box fields of one box, the raw sums and their
+// size_grid x size_grid coarse average (VelocityField.cpp:69-128,
+// ShapeField.cpp:69-128), computed on the device from the current fields
+extern "C" int rc_box_field(rc_handle *h, int box, int which, int size_grid, double *fx, double *fy, double *cx,
+		double *cy) {
+	if(!h) return RC_ERR_ARG;
+	if(box < 0 || box >= h->nboxes) return fail(h, RC_ERR_ARG, "box %d out of range", box);
+	if(which != 0 && which != 1) return fail(h, RC_ERR_ARG, "unknown box field %d (0 velocity, 1 shape)", which);
+	const DevBox &B = h->dboxes[box];
+	if(size_grid < 1 || size_grid > B.L) return fail(h, RC_ERR_ARG, "size_grid %d outside 1..%d", size_grid, B.L);
+	int rc;
+	// the velocity rows of the current fields (the dry model does not keep them)
+	if((rc = materialize(h, false))) return rc;
+	if(!h->lists_valid && (rc = run_gather(h, h->d_phiA))) return rc;
+	const size_t n = (size_t) B.L * B.L;
+	double *d = nullptr;
+	HIPCHK(h, dalloc(&d, 4 * n));
+	FieldArgs a{};
+	a.boxes = h->d_boxes;
+	a.tile_box = h->d_tile_box;
+	a.tile_geo = h->d_tile_geo;
+	a.tile_count = h->d_tile_count;
+	a.tile_cells = h->d_tile_cells;
+	a.cells = h->d_cells;
+	a.phi = h->d_phiA;
+	a.v = h->d_v;
+	a.which = which;
+	a.tile0 = B.tile_off;
+	a.fx = d;
+	a.fy = d + n;
+	hipError_t e = launch_box_field(a, B.ntx * B.nty, h->stream);
+	if(e == hipSuccess) e = launch_box_coarse(d, d + n, d + 2 * n, d + 3 * n, B.L, size_grid, h->stream);
+	double *outs[4] = { fx, fy, cx, cy };
+	for(int i = 0; i < 4 && e == hipSuccess; ++i)
+		if(outs[i]) e = hipMemcpyAsync(outs[i], d + i * n, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream);
+	if(e == hipSuccess) e = hipStreamSynchronize(h->stream);
+	dfree(d);
+	if(e != hipSuccess) return fail(h, RC_ERR_HIP, "box field: %s", hipGetErrorString(e));
+	return RC_OK;
+}
+
 extern "C" int rc_total_rows(rc_handle *h, long long *rows) {
 	if(!h || !rows) return RC_ERR_ARG;
 	*rows = h->rows;

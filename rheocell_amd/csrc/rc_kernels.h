@@ -93,6 +93,28 @@ struct CGArgs {
 };
 
 hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s);
+
+// Box fields of the VelocityField / ShapeField observables: per box site, the
+// sum over the covering patches (cell order) of w * phi, w = the velocity row
+// (which 0) or the cell's (S00, S01) (which 1), over the tiles of one box
+// (tile0 .. tile0 + ntiles - 1); the tile lists must describe the current patches.
+struct FieldArgs {
+	const DevBox *boxes;
+	const int *tile_box;
+	const int4 *tile_geo;
+	const int *tile_count;
+	const int *tile_cells;
+	const DevCell *cells;
+	const double *phi;
+	const double2 *v;
+	int which;
+	int tile0;
+	double *fx, *fy;             // L*L sums, box-local site index x + L y
+};
+hipError_t launch_box_field(const FieldArgs &a, int ntiles, hipStream_t s);
+// size_grid x size_grid periodic box average of fx, fy (L*L each) into cx, cy
+hipError_t launch_box_coarse(const double *fx, const double *fy, double *cx, double *cy, int L, int size_grid,
+		hipStream_t s);
 // Dynamic LDS of the stage kernel for patches up to LX x LY and strips of npt
 // rows: {phi, mu} tile of LY + npt rows of LX + 2 columns (periodic columns on
 // both sides, periodic images of the first rows below), per-row trig and box

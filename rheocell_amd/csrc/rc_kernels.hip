@@ -309,6 +309,128 @@ hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// Observable box fields (VelocityField::calc_field, VelocityField.cpp:69-78;
+// ShapeField::calc_field, ShapeField.cpp:69-78): f(k) += w * phi over the
+// fields in index order, so one sum per site in cell order reproduces the
+// reference's additions exactly.  Output only: no effect on the simulation.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_box_field(FieldArgs a) {
+	const int tile = a.tile0 + xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
+	const DevBox B = a.boxes[a.tile_box[tile]];
+	const int t = tile - B.tile_off, L = B.L;
+	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
+	const int tw = min(TW, L - x0), th = min(TH, L - y0);
+	const int n = min(list_count(a.tile_count, tile), MAXT);
+	const int sx = tid % TW, sy = tid / TW;
+	if(sx >= tw || sy >= th) return;
+	double fx = 0., fy = 0.;
+	// GCHUNK covering cells' rows requested at once, then added in cell order
+	for(int base = 0; base < n; base += GCHUNK) {
+		int row[GCHUNK];
+		double2 w[GCHUNK];
+		double ph[GCHUNK];
+#pragma unroll
+		for(int q = 0; q < GCHUNK; ++q) row[q] = entry_row(list_entry(a.tile_geo, tile, base + q), sx, sy, L);
+#pragma unroll
+		for(int q = 0; q < GCHUNK; ++q) {
+			w[q] = make_double2(0., 0.);
+			ph[q] = 0.;
+			if(row[q] < 0) continue;
+			if(a.which == 0) w[q] = a.v[row[q]];
+			else {
+				const int ci = a.tile_cells[tile * MAXT + base + q];
+				w[q] = make_double2(a.cells[ci].S00, a.cells[ci].S01);
+			}
+			ph[q] = a.phi[row[q]];
+		}
+#pragma unroll
+		for(int q = 0; q < GCHUNK; ++q) {
+			if(row[q] < 0) continue;
+			fx += w[q].x * ph[q];
+			fy += w[q].y * ph[q];
+		}
+	}
+	const int k = (y0 + sy) * L + x0 + sx;
+	a.fx[k] = fx;
+	a.fy[k] = fy;
+}
+
+hipError_t launch_box_field(const FieldArgs &a, int ntiles, hipStream_t s) {
+	if(ntiles <= 0) return hipSuccess;
+	RC_LAUNCH(k_box_field, dim3(ntiles), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+// the coarse field of VelocityField::f_field / ShapeField::f_field
+// (VelocityField.cpp:106-121): site (j, i) adds f(ss) / (g g) over the g x g
+// sites from (j - g/2, i - g/2), rows k outer, columns l inner, periodic.
+// Up to COARSE_LDS_G a 16 x 16 block of sites stages its (15 + g)^2 window
+// of both fields in LDS; larger windows read straight from the fields.
+constexpr int COARSE_B = 16;
+constexpr int COARSE_LDS_G = 33;
+__global__ __launch_bounds__(256) void k_box_coarse_lds(const double *__restrict__ fx, const double *__restrict__ fy,
+		double *__restrict__ cx, double *__restrict__ cy, int L, int g) {
+	extern __shared__ double s_w[];
+	const int W = COARSE_B + g - 1, h = g / 2;
+	const int bx0 = blockIdx.x * COARSE_B, by0 = blockIdx.y * COARSE_B;
+	for(int e = threadIdx.x; e < W * W; e += 256) {
+		const int wy = e / W, wx = e - wy * W;
+		int xx = (bx0 - h + wx) % L, yy = (by0 - h + wy) % L;
+		xx += xx < 0 ? L : 0;
+		yy += yy < 0 ? L : 0;
+		s_w[e] = fx[yy * L + xx];
+		s_w[W * W + e] = fy[yy * L + xx];
+	}
+	__syncthreads();
+	const int tx = threadIdx.x % COARSE_B, ty = threadIdx.x / COARSE_B;
+	const int j = bx0 + tx, i = by0 + ty;
+	if(j >= L || i >= L) return;
+	const double gg = (double) (g * g);
+	double sx = 0., sy = 0.;
+	for(int k = 0; k < g; ++k) {
+		const double *rx = s_w + (ty + k) * W + tx, *ry = rx + W * W;
+		for(int l = 0; l < g; ++l) {
+			sx += rx[l] / gg;
+			sy += ry[l] / gg;
+		}
+	}
+	cx[i * L + j] = sx;
+	cy[i * L + j] = sy;
+}
+
+__global__ __launch_bounds__(256) void k_box_coarse(const double *__restrict__ fx, const double *__restrict__ fy,
+		double *__restrict__ cx, double *__restrict__ cy, int L, int g) {
+	const int site = blockIdx.x * 256 + threadIdx.x;
+	if(site >= L * L) return;
+	const int i = site / L, j = site - i * L;
+	const double gg = (double) (g * g);
+	const int xs = (j - g / 2 + L) % L, ys = (i - g / 2 + L) % L;
+	double sx = 0., sy = 0.;
+	for(int k = 0; k < g; ++k) {
+		const int yy = (ys + k) % L;
+		for(int l = 0; l < g; ++l) {
+			const int ss = (xs + l) % L + yy * L;
+			sx += fx[ss] / gg;
+			sy += fy[ss] / gg;
+		}
+	}
+	cx[site] = sx;
+	cy[site] = sy;
+}
+
+hipError_t launch_box_coarse(const double *fx, const double *fy, double *cx, double *cy, int L, int size_grid,
+		hipStream_t s) {
+	if(L <= 0 || size_grid < 1 || size_grid > L) return hipErrorInvalidValue;
+	if(size_grid <= COARSE_LDS_G) {
+		const int W = COARSE_B + size_grid - 1, nb = (L + COARSE_B - 1) / COARSE_B;
+		RC_LAUNCH(k_box_coarse_lds, dim3(nb, nb), dim3(256), sizeof(double) * 2 * W * W, s, fx, fy, cx, cy, L,
+				size_grid);
+	}
+	else RC_LAUNCH(k_box_coarse, dim3((L * L + 255) / 256), dim3(256), 0, s, fx, fy, cx, cy, L, size_grid);
+	return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // k_stage<MODE>: force evaluation (+ field update) of one cell
 // ---------------------------------------------------------------------------
 // Column strips: a cell's LX x LY patch (ring-buffer coordinates, storage

@@ -127,6 +127,16 @@ class Simulation:
     def apply_changes_after_equilibration(self):
         self._check(self.L.rc_apply_changes_after_equilibration(self.h))
 
+    def box_field(self, which: int, size_grid: int, box: int = 0):
+        """VelocityField (which 0) / ShapeField (which 1) box fields of one box:
+        (fx, fy, coarse_x, coarse_y), each L x L indexed [y, x]
+        (src/Observables/VelocityField.cpp:69-128, ShapeField.cpp:69-128)."""
+        L = self.confs[box].Lx if 0 <= box < len(self.confs) else 1   # the library rejects a bad box
+        out = [np.zeros(L * L) for _ in range(4)]
+        ptrs = [o.ctypes.data_as(C.POINTER(C.c_double)) for o in out]
+        self._check(self.L.rc_box_field(self.h, box, which, size_grid, *ptrs))
+        return tuple(o.reshape(L, L) for o in out)
+
     def set_warm_start(self, mode: int):
         self._check(self.L.rc_set_warm_start(self.h, mode))
 
