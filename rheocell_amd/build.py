@@ -26,7 +26,7 @@ HOST_EXE = os.path.join(HERE, "rheoCell_hip")
 def build_host(verbose: bool = False) -> str:
     """The C++ drop-in (FD_HIPBackend + SimManager + main), linked to librheocell.so."""
     tmp = HOST_EXE + ".tmp"
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")] + HOST_SOURCES + [
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wno-unused-result", "-pthread", "-I" + os.path.join(ROOT, "include")] + HOST_SOURCES + [
         "-L" + HERE, "-lrheocell", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib",
         "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -250,6 +250,7 @@ rc_model_params WetPolarModel::to_params(number J0, number dt) const {
 // ---------------------------------------------------------------- backend
 FD_HIPBackend::FD_HIPBackend() {}
 FD_HIPBackend::~FD_HIPBackend() {
+	writer.reset();   // writes what is still queued
 	if(h) rc_destroy(h);
 }
 
@@ -287,6 +288,8 @@ void FD_HIPBackend::get_settings(input_file &inp) {
 	std::string T;
 	getInputString(&inp, "T", T, 1);
 	getInputString(&inp, "trajectory_file", trajectory_file, 1);
+	// configuration lines formatted and written by a writer thread (default), or inline
+	getInputBool(&inp, "async_conf_output", &async_conf_output, 0);
 	getInputString(&inp, "lastconf_file", lastconf_file, 0);
 	getInputString(&inp, "equilibratedconf_file", equilibratedconf_file, 0);
 	getInputNumber(&inp, "dt", &dt, 1);
@@ -482,31 +485,125 @@ void FD_HIPBackend::apply_simulation_data_changes() {
 }
 
 // Configuration::headers / field (src/Observables/Configuration.cpp:36-80)
-std::string FD_HIPBackend::conf_string(llint step) {
+std::string format_conf(const ConfSnapshot &s) {
 	std::stringstream conf;
 	conf.precision(15);
-	conf << "t = " << step << std::endl;
-	conf << "b = " << box.getXsize() << " " << box.getYsize() << std::endl;
-	for(int i = 0; i < N(); ++i) {
-		const MultiPhaseField *p = fields[i].get();
+	conf << "t = " << s.step << std::endl;
+	conf << "b = " << s.Lx << " " << s.Ly << std::endl;
+	for(size_t i = 0; i < s.cells.size(); ++i) {
+		const ConfSnapshot::Cell &p = s.cells[i];
 		if(i) conf << std::endl;
-		conf << p->LsubX << " " << p->LsubY << " " << p->CoM[0] << " " << p->CoM[1] << " " << p->offset[0] << " "
-			 << p->offset[1] << " " << p->sub_corner_bottom_left << " " << p->unrap_sub_corner_bottom_left_x << " "
-			 << p->unrap_sub_corner_bottom_left_y << " " << p->Q00 << " " << p->Q01 << " ";
-		for(int q = 0; q < p->subSize; q++) conf << p->map_sub_to_box[q] << " " << p->fieldScalar[q] << " ";
+		conf << p.LsubX << " " << p.LsubY << " " << p.CoM[0] << " " << p.CoM[1] << " " << p.offset[0] << " "
+			 << p.offset[1] << " " << p.corner << " " << p.unrap[0] << " " << p.unrap[1] << " " << p.Q00 << " "
+			 << p.Q01 << " ";
+		for(size_t q = 0; q < p.fieldScalar.size(); q++) conf << p.map_sub_to_box[q] << " " << p.fieldScalar[q] << " ";
 	}
 	return conf.str();
 }
 
+ConfSnapshot FD_HIPBackend::conf_snapshot(llint step) {
+	ConfSnapshot s;
+	s.step = step;
+	s.Lx = box.getXsize();
+	s.Ly = box.getYsize();
+	s.cells.resize(N());
+	for(int i = 0; i < N(); ++i) {
+		const MultiPhaseField *p = fields[i].get();
+		ConfSnapshot::Cell &c = s.cells[i];
+		c.LsubX = p->LsubX;
+		c.LsubY = p->LsubY;
+		c.offset[0] = p->offset[0];
+		c.offset[1] = p->offset[1];
+		c.corner = p->sub_corner_bottom_left;
+		c.CoM[0] = p->CoM[0];
+		c.CoM[1] = p->CoM[1];
+		c.unrap[0] = p->unrap_sub_corner_bottom_left_x;
+		c.unrap[1] = p->unrap_sub_corner_bottom_left_y;
+		c.Q00 = p->Q00;
+		c.Q01 = p->Q01;
+		c.map_sub_to_box.assign(p->map_sub_to_box.begin(), p->map_sub_to_box.begin() + p->subSize);
+		c.fieldScalar.assign(p->fieldScalar.begin(), p->fieldScalar.begin() + p->subSize);
+	}
+	return s;
+}
+
+std::string FD_HIPBackend::conf_string(llint step) { return format_conf(conf_snapshot(step)); }
+
+static void write_conf_files(const ConfSnapshot &s) {
+	const std::string text = format_conf(s) + "\n";
+	if(s.to_trajectory) {
+		std::ofstream t(s.trajectory_file, std::ios::app);
+		t << text;
+		if(!t) throw RCexception("Stream %s not writable", s.trajectory_file.c_str());
+	}
+	std::ofstream l(s.lastconf_file, std::ios::trunc);
+	l << text;
+	if(!l) throw RCexception("Stream %s not writable", s.lastconf_file.c_str());
+}
+
+ConfWriter::ConfWriter() : th([this] { run(); }) {}
+
+ConfWriter::~ConfWriter() {
+	{
+		std::lock_guard<std::mutex> g(m);
+		quit = true;
+	}
+	cv.notify_all();
+	th.join();
+}
+
+void ConfWriter::push(ConfSnapshot &&s) {
+	std::unique_lock<std::mutex> g(m);
+	cv.wait(g, [this] { return q.size() < 2 || !error.empty(); });
+	if(!error.empty()) throw RCexception("%s", error.c_str());
+	q.push_back(std::move(s));
+	cv.notify_all();
+}
+
+void ConfWriter::drain() {
+	std::unique_lock<std::mutex> g(m);
+	cv.wait(g, [this] { return (q.empty() && !busy) || !error.empty(); });
+	if(!error.empty()) throw RCexception("%s", error.c_str());
+}
+
+void ConfWriter::run() {
+	std::unique_lock<std::mutex> g(m);
+	for(;;) {
+		cv.wait(g, [this] { return quit || !q.empty(); });
+		if(q.empty()) return;   // quit with nothing left
+		ConfSnapshot s = std::move(q.front());
+		q.pop_front();
+		busy = true;
+		g.unlock();
+		std::string err;
+		try {
+			write_conf_files(s);
+		} catch(const std::exception &e) {
+			err = e.what();
+		}
+		g.lock();
+		busy = false;
+		if(!err.empty() && error.empty()) error = err;
+		cv.notify_all();
+	}
+}
+
 void FD_HIPBackend::print_conf(bool only_last) {
 	apply_simulation_data_changes();
-	const std::string s = conf_string(curr_step) + "\n";
-	if(!only_last) {
-		std::ofstream t(trajectory_file, std::ios::app);
-		t << s;
+	ConfSnapshot s = conf_snapshot(curr_step);
+	s.to_trajectory = !only_last;
+	s.trajectory_file = trajectory_file;
+	s.lastconf_file = lastconf_file;
+	if(!async_conf_output) {
+		write_conf_files(s);
+		return;
 	}
-	std::ofstream l(lastconf_file, std::ios::trunc);
-	l << s;
+	if(!writer) writer.reset(new ConfWriter());
+	writer->push(std::move(s));
+}
+
+void FD_HIPBackend::drain_output() {
+	if(writer) writer->drain();
 }
 
 void FD_HIPBackend::print_equilibration_info() {
@@ -578,4 +675,5 @@ void SimManager::run() {
 	}
 	if(b.current_step() == next && steps_run > 0) b.print_conf();
 	b.print_conf(true);
+	b.drain_output();
 }

@@ -14,12 +14,16 @@
 // layer reads inputs and confs, owns the random stream, and writes confs.
 #pragma once
 
+#include <condition_variable>
 #include <cstdarg>
+#include <deque>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rheocell.h"
@@ -109,6 +113,43 @@ public:
 	rc_model_params to_params(number J0, number dt) const;
 };
 
+// What Configuration::headers / field print of one step (Configuration.cpp:36-80),
+// copied off the live fields so that it can be formatted while steps go on.
+struct ConfSnapshot {
+	llint step = 0;
+	int Lx = 0, Ly = 0;
+	struct Cell {
+		int LsubX, LsubY, offset[2], corner;
+		number CoM[2], unrap[2], Q00, Q01;
+		std::vector<int> map_sub_to_box;
+		std::vector<number> fieldScalar;
+	};
+	std::vector<Cell> cells;
+	bool to_trajectory = true;       // append to the trajectory (not only the last conf)
+	std::string trajectory_file, lastconf_file;
+};
+std::string format_conf(const ConfSnapshot &s);   // the text Configuration writes, precision 15
+
+// Configuration output off the step loop (no counterpart in the reference,
+// which formats on its one thread): print_conf hands a snapshot to one writer
+// thread, which formats it and appends it in print order.  At most two
+// snapshots wait; a further print waits for the writer.
+class ConfWriter {
+public:
+	ConfWriter();
+	~ConfWriter();
+	void push(ConfSnapshot &&s);
+	void drain();   // every pushed snapshot written; rethrows a write error
+private:
+	void run();
+	std::mutex m;
+	std::condition_variable cv;
+	std::deque<ConfSnapshot> q;
+	bool quit = false, busy = false;
+	std::string error;
+	std::thread th;
+};
+
 class FD_HIPBackend {
 public:
 	FD_HIPBackend();
@@ -123,6 +164,8 @@ public:
 	void print_conf(bool only_last = false);
 	void print_equilibration_info();
 	std::string conf_string(llint step);
+	ConfSnapshot conf_snapshot(llint step);
+	void drain_output();   // waits for the configuration writer
 	llint current_step() const { return curr_step; }
 	void increment_current_step() { ++curr_step; }
 	int N() const { return (int) fields.size(); }
@@ -144,6 +187,8 @@ private:
 	bool restart_step_counter = false;
 	bool host_stale = true;
 	int warm_mode = RC_WARM_REFERENCE;
+	bool async_conf_output = true;   // input key async_conf_output (no counterpart in the reference)
+	std::unique_ptr<ConfWriter> writer;
 };
 
 class SimManager {

@@ -34,3 +34,17 @@ def test_executable_matches_oracle(tmp_path, fc):
         assert abs(c.CoM[0] - oc.CoM[0]) < 1e-9 and abs(c.Q00 - oc.Q00) < 1e-12
     traj = open(tmp_path / "trajectory.dat").read()
     assert traj.count("t = ") == 2  # steps 10 and 20
+
+
+def test_async_conf_output_matches_inline(tmp_path):
+    # the configuration writer thread produces the same files, byte for byte,
+    # as writing inline (async_conf_output = 0), at every print of the run
+    outs = {}
+    for mode in (1, 0):
+        d = tmp_path / ("async%d" % mode)
+        make_case(str(d), L=64, nside=3, seed=9, steps=12, print_every=2, async_conf_output=mode)
+        r = subprocess.run([EXE, "input"], cwd=str(d), capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs[mode] = (open(d / "trajectory.dat", "rb").read(), open(d / "last_conf.dat", "rb").read())
+    assert outs[1] == outs[0]
+    assert outs[1][0].count(b"t = ") == 6
