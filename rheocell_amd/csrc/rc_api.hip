@@ -98,6 +98,7 @@ struct rc_handle {
 	int graph_set = -1;
 	hipGraphExec_t graph_exec = nullptr;
 	double *d_ggauss = nullptr;     // the window's predictor noise [GRAPH_STEPS][ncells]
+	double *d_bf = nullptr;         // rc_box_field: four L x L arrays of the largest box, on first use
 	int *d_step_base = nullptr;     // first step of the window being replayed
 	int *h_step_base = nullptr;     // pinned, one slot per window of a chunk
 	long long graph_launches = 0, eager_steps = 0;
@@ -362,7 +363,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xprev); dfree(h->d_rprev);
 	for(auto &g : h->groups) dfree(g.d_cells);
 	if(h->graph_exec) (void) hipGraphExecDestroy(h->graph_exec);
-	dfree(h->d_ggauss); dfree(h->d_step_base);
+	dfree(h->d_ggauss); dfree(h->d_step_base); dfree(h->d_bf);
 	if(h->h_step_base) (void) hipHostFree(h->h_step_base); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
@@ -1549,8 +1550,12 @@ extern "C" int rc_box_field(rc_handle *h, int box, int which, int size_grid, dou
 	if((rc = materialize(h, false))) return rc;
 	if(!h->lists_valid && (rc = run_gather(h, h->d_phiA))) return rc;
 	const size_t n = (size_t) B.L * B.L;
-	double *d = nullptr;
-	HIPCHK(h, dalloc(&d, 4 * n));
+	if(!h->d_bf) {
+		size_t big = 0;
+		for(const DevBox &o : h->dboxes) big = std::max(big, (size_t) o.L * o.L);
+		HIPCHK(h, dalloc(&h->d_bf, 4 * big));
+	}
+	double *d = h->d_bf;
 	FieldArgs a{};
 	a.boxes = h->d_boxes;
 	a.tile_box = h->d_tile_box;
@@ -1570,7 +1575,6 @@ extern "C" int rc_box_field(rc_handle *h, int box, int which, int size_grid, dou
 	for(int i = 0; i < 4 && e == hipSuccess; ++i)
 		if(outs[i]) e = hipMemcpyAsync(outs[i], d + i * n, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream);
 	if(e == hipSuccess) e = hipStreamSynchronize(h->stream);
-	dfree(d);
 	if(e != hipSuccess) return fail(h, RC_ERR_HIP, "box field: %s", hipGetErrorString(e));
 	return RC_OK;
 }
