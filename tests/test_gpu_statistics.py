@@ -1,0 +1,80 @@
+"""Long-run statistics of the library against the oracle: the time-averaged
+cell area, deformation |S| and centre-of-mass speed agree within Z_MAX
+combined standard errors (block averages, tests/longrun.py).
+
+* dry, 140^2 box, 64 cells, 1000 steps: the library and the oracle on the
+  same seed, and the library on another seed against the oracle (independent
+  samples of the same dynamics);
+* wet, 64^2 box, 9 cells, CG to 1e-8, 400 steps: the library and the oracle
+  on the same seed.
+
+On the same seed the sampled series themselves also agree to 1e-10 relative
+(measured: 4e-14 after 1000 dry steps).
+
+test_longrun_oracle.py shows on the CPU that the criterion accepts two seeds
+of the same dynamics and rejects a 30% stronger active force and a dropped
+repulsion term.
+"""
+import numpy as np
+import pytest
+
+from oracle import OracleSim
+from rheocell_amd import confgen
+from rheocell_amd.sim import Simulation
+from tests.longrun import STATS, oracle_series, sim_series, z_scores
+
+pytestmark = pytest.mark.gpu
+
+Z_MAX = 5.0
+DT = 0.1
+
+
+def _report(name, a, b, z):
+    diff = np.abs(a.mean(axis=0) - b.mean(axis=0))
+    print(name, {s: "z=%.2f |dmean|=%.3g" % (zz, d) for s, zz, d in zip(STATS, z, diff)})
+
+
+def _deviation(a, b):
+    d = (np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)).max()
+    print("max relative deviation of the sampled series %.3g" % d)
+    return d
+
+
+def test_dry_long_run_statistics():
+    steps, every, skip = 1000, 20, 200
+    conf, keys = confgen.workload("dry140_64", seed=3)
+    orc = OracleSim(conf, keys)
+    ref = oracle_series(orc, len(conf.cells), conf.Lx, steps, every, skip, DT)
+    sim = Simulation([(conf, keys)])
+    got = sim_series(sim, conf.Lx, steps, every, skip, DT)
+    sim.close()
+    assert got.shape == ref.shape
+    z = z_scores(got, ref)
+    _report("dry same seed", got, ref, z)
+    assert (z < Z_MAX).all()
+    # the same seed keeps the trajectories together: 1000 steps within the
+    # 100-step parity bound
+    assert _deviation(got, ref) <= 1e-10
+    conf4, keys4 = confgen.workload("dry140_64", seed=4)
+    sim = Simulation([(conf4, keys4)])
+    other = sim_series(sim, conf4.Lx, steps, every, skip, DT)
+    sim.close()
+    z = z_scores(other, ref)
+    _report("dry seed 4 vs oracle seed 3", other, ref, z)
+    assert (z < Z_MAX).all()
+
+
+def test_wet_long_run_statistics():
+    steps, every, skip = 400, 10, 50
+    conf = confgen.lattice_conf(64, 3, 5)
+    keys = dict(confgen.BASE_KEYS, friction_cell=1, CGtolerance=1e-8, seed=5)
+    orc = OracleSim(conf, keys)
+    ref = oracle_series(orc, len(conf.cells), conf.Lx, steps, every, skip, DT)
+    sim = Simulation([(conf, keys)])
+    got = sim_series(sim, conf.Lx, steps, every, skip, DT)
+    sim.close()
+    assert got.shape == ref.shape
+    z = z_scores(got, ref)
+    _report("wet same seed", got, ref, z)
+    assert (z < Z_MAX).all()
+    assert _deviation(got, ref) <= 1e-10
