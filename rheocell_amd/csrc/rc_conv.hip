@@ -51,6 +51,10 @@ constexpr int CONV_TILES = 4;
 constexpr int CONV_BW = 16 * CONV_TILES;           // block width in pixels
 constexpr int CONV_SW = CONV_BW + 2;               // staged columns
 constexpr int CONV_SH = CONV_ROWS + 2;             // staged rows
+constexpr int CONV_NIN = CONV_SH * CONV_SW * 4;     // staged 16-byte input chunks
+constexpr int CONV_LIN = (CONV_NIN + CONV_THREADS - 1) / CONV_THREADS;   // per thread
+constexpr int CONV_NW = 9 * 32 * 4;                // 16-byte weight chunks of a layer
+constexpr int CONV_LW = (CONV_NW + CONV_THREADS - 1) / CONV_THREADS;
 
 // two floats to packed bf16, rounded to nearest even by v_cvt_pk_bf16_f32
 __device__ inline unsigned pack2bf(float a, float b) {
@@ -72,8 +76,15 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3x3(const unsigned short *
 	const int x0 = (blockIdx.x % bx) * CONV_BW, y0 = (blockIdx.x / bx) * CONV_ROWS;
 	const int n = blockIdx.y;
 	const size_t img = (size_t) n * H * W;
-	// stage: CONV_SH rows x CONV_SW columns x 4 chunks of 8 channels
-	for(int e = threadIdx.x; e < CONV_SH * CONV_SW * 4; e += CONV_THREADS) {
+	// stage: CONV_SH rows x CONV_SW columns x 4 chunks of 8 channels, and the
+	// layer's weights (once per workgroup; every wave reads them from LDS).
+	// Every load of a thread is issued before the first LDS write, so the
+	// staging costs one memory round trip, not one per chunk.
+	uint4 vin[CONV_LIN], vw[CONV_LW];
+#pragma unroll
+	for(int j = 0; j < CONV_LIN; ++j) {
+		// past the end (last round only): the last chunk again
+		const int e = min((int) threadIdx.x + j * CONV_THREADS, CONV_NIN - 1);
 		const int ch = e & 3, q = e >> 2, col = q % CONV_SW, r = q / CONV_SW;
 		// circular (operands lie in [-1, 2W) and [-1, 2H) as W >= 32 >= 16 CONV_TILES / 2 and
 		// H >= 3); columns past W of a partial block wrap too (staged, never read)
@@ -84,18 +95,31 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3x3(const unsigned short *
 		ys += ys < 0 ? H : 0;
 		ys -= ys >= H ? H : 0;
 		ys -= ys >= H ? H : 0;
-		const uint4 v = *reinterpret_cast<const uint4 *>(in + (img + (size_t) ys * W + xs) * 32 + ch * 8);
-		*reinterpret_cast<uint4 *>(&s_in[swz(q, ch)]) = v;
+		vin[j] = *reinterpret_cast<const uint4 *>(in + (img + (size_t) ys * W + xs) * 32 + ch * 8);
 	}
-	// the layer's weights, once per workgroup (every wave reads them from LDS)
-	for(int e = threadIdx.x; e < 9 * 32 * 4; e += CONV_THREADS)
-		*reinterpret_cast<uint4 *>(&s_w[swz(e >> 2, e & 3)]) = *reinterpret_cast<const uint4 *>(w + (size_t) e * 8);
+#pragma unroll
+	for(int j = 0; j < CONV_LW; ++j) {
+		const int e = min((int) threadIdx.x + j * CONV_THREADS, CONV_NW - 1);
+		vw[j] = *reinterpret_cast<const uint4 *>(w + (size_t) e * 8);
+	}
 	const int i = lane & 15, kb = lane >> 4;
 	float bz0[4], bz1[4];
 #pragma unroll
 	for(int r = 0; r < 4; ++r) {
 		bz0[r] = bias ? bias[4 * kb + r] : 0.f;
 		bz1[r] = bias ? bias[16 + 4 * kb + r] : 0.f;
+	}
+#pragma unroll
+	for(int j = 0; j < CONV_LIN; ++j) {
+		// unconditional: a thread past the end writes the last chunk's own value
+		// again (a guarded write would let the compiler sink its load behind it)
+		const int e = min((int) threadIdx.x + j * CONV_THREADS, CONV_NIN - 1);
+		*reinterpret_cast<uint4 *>(&s_in[swz(e >> 2, e & 3)]) = vin[j];
+	}
+#pragma unroll
+	for(int j = 0; j < CONV_LW; ++j) {
+		const int e = min((int) threadIdx.x + j * CONV_THREADS, CONV_NW - 1);
+		*reinterpret_cast<uint4 *>(&s_w[swz(e >> 2, e & 3)]) = vw[j];
 	}
 	__syncthreads();
 	bf16x8 wa[9], wb[9];

@@ -1344,15 +1344,26 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 			const double D = a.diag[k], eta = a.params[a.set].friction_cell;
 			const double2 n4 = a.n4[k];
 			double2 acc = make_double2(0., 0.), acc2 = make_double2(0., 0.);
-			for(int j = 0; j < n; ++j) {
-				const int row = entry_row(list_entry(a.tile_geo, tile, j), sx, sy, L);
-				if(row < 0) continue;
-				const double2 r = a.r[row];
-				const double2 e = make_double2((r.x + eta * n4.x) / D, (r.y + eta * n4.y) / D);
-				a.p[row] = e;
-				acc.x += e.x; acc.y += e.y;
-				acc2.x += e.x * e.x; acc2.y += e.y * e.y;
-				er[0] += e.x * r.x; er[1] += e.y * r.y;
+			// batches of GCHUNK covering rows (the lists are padded to 16 with
+			// entries that cover nothing), every load issued before the first use
+			for(int base = 0; base < n; base += GCHUNK) {
+				int row[GCHUNK];
+				double2 rv[GCHUNK];
+#pragma unroll
+				for(int q = 0; q < GCHUNK; ++q) row[q] = entry_row(list_entry(a.tile_geo, tile, base + q), sx, sy, L);
+#pragma unroll
+				for(int q = 0; q < GCHUNK; ++q) rv[q] = row[q] >= 0 ? a.r[row[q]] : make_double2(0., 0.);
+				__builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0), before the stores below
+#pragma unroll
+				for(int q = 0; q < GCHUNK; ++q) {
+					if(row[q] < 0) continue;
+					const double2 r = rv[q];
+					const double2 e = make_double2((r.x + eta * n4.x) / D, (r.y + eta * n4.y) / D);
+					a.p[row[q]] = e;
+					acc.x += e.x; acc.y += e.y;
+					acc2.x += e.x * e.x; acc2.y += e.y * e.y;
+					er[0] += e.x * r.x; er[1] += e.y * r.y;
+				}
 			}
 			a.sp[k] = acc;
 			a.p2[k] = acc2;
@@ -1553,7 +1564,12 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 		if(by >= L) by -= L;
 		return B.box_off + by * L + bx;
 	};
-	if constexpr(MODE == CG_ROW_UPDATE) {
+	if(MODE == CG_ROW_STEP) {
+		// the learned stage's step along e (held in p): alpha0 per component
+		al0 = cg->alpha0[0];
+		al1 = cg->alpha0[1];
+	}
+	if constexpr(MODE == CG_ROW_UPDATE || MODE == CG_ROW_STEP) {
 		// x += alpha p, r -= alpha Ap on two rows per pass, every load of both
 		// issued before either row is updated (rows q, q + 256, ... in order)
 		auto upd = [&](double D, double2 n4, double2 vin, double2 xx, double2 rr, int row) {
@@ -1602,21 +1618,6 @@ __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 			double2 xx = a.x[row], rn = a.r[row];
 			if(rev0) { xx.x = a.xprev[row].x; rn.x = a.rprev[row].x; }
 			if(rev1) { xx.y = a.xprev[row].y; rn.y = a.rprev[row].y; }
-			a.x[row] = xx;
-			a.r[row] = rn;
-			red[0] += rn.x * rn.x; red[1] += rn.y * rn.y;
-			red[2] += rn.x * (inv * rn.x); red[3] += rn.y * (inv * rn.y);
-			continue;
-		}
-		if(MODE == CG_ROW_STEP) {
-			// x += alpha0 e, r -= alpha0 A e (A e = D e - eta N4 of the box sums of e); r.r, r.z
-			const double inv = 1. / D;
-			const double a0 = cg->alpha0[0], a1 = cg->alpha0[1];
-			const double2 e = a.p[row];
-			double2 xx = a.x[row], rn = a.r[row];
-			xx.x += a0 * e.x; xx.y += a1 * e.y;
-			rn.x -= a0 * (D * e.x - eta * n4.x);
-			rn.y -= a1 * (D * e.y - eta * n4.y);
 			a.x[row] = xx;
 			a.r[row] = rn;
 			red[0] += rn.x * rn.x; red[1] += rn.y * rn.y;
