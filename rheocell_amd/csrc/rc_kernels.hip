@@ -1468,16 +1468,21 @@ __global__ __launch_bounds__(256) void k_cg_box(CGArgs a, int nsites) {
 	CGState *cg = a.cg;
 	if(MODE != 0 && !cg->active) return;
 	__shared__ double s_red[4 * 2];
-	if(MODE == 1 && a.apply_update && blockIdx.x == 0) {
-		// the previous iteration's update of the CG state, whose beta and flags
-		// k_cg_gather<P_UPDATE> derived from the same partials
+	const int nblk = (nsites + 255) / 256;
+	if(MODE == 1 && blockIdx.x == nblk) {
+		// one extra workgroup (launched with apply_update only): the previous
+		// iteration's update of the CG state, whose beta and flags
+		// k_cg_gather<P_UPDATE> derived from the same partials.  Beside the box
+		// workgroups rather than ahead of one of them, so the pass's duration is
+		// not the sum of the two chains.
 		__shared__ double s_upd[4 * 4];
 		const CGUpdIn u = cg_update_inputs(cg);
 		double tot[4];
 		reduce_parts<4, 256, 8>(a.part, a.ncells, tot, s_upd);
 		if(threadIdx.x == 0) cg_after_update(cg, tot, u, a.st);
+		return;
 	}
-	const int k = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+	const int k = xcd_item(blockIdx.x, nblk) * 256 + threadIdx.x;
 	const DevBox B = a.boxes[0];
 	const double eta = a.params[a.set].friction_cell;
 	double part[2] = { 0., 0. };
@@ -1509,12 +1514,17 @@ __global__ __launch_bounds__(256) void k_cg_box(CGArgs a, int nsites) {
 	}
 	if(MODE != 1) return;
 	block_sum<2>(part, s_red);
-	block_partials<2>(part, a.part_box);
+	// stride nblk: the update workgroup is not a box workgroup
+	if(threadIdx.x == 0) {
+		a.part_box[blockIdx.x] = part[0];
+		a.part_box[nblk + blockIdx.x] = part[1];
+	}
 }
 
 hipError_t launch_cg_box(int mode, const CGArgs &a, int nsites, hipStream_t s) {
-	const dim3 g((nsites + 255) / 256);
-	if(mode == 1) RC_LAUNCH(k_cg_box<1>, g, dim3(256), 0, s, a, nsites);
+	const int nblk = (nsites + 255) / 256;
+	const dim3 g(nblk);
+	if(mode == 1) RC_LAUNCH(k_cg_box<1>, dim3(nblk + (a.apply_update ? 1 : 0)), dim3(256), 0, s, a, nsites);
 	else if(mode == 2) RC_LAUNCH(k_cg_box<2>, g, dim3(256), 0, s, a, nsites);
 	else RC_LAUNCH(k_cg_box<0>, g, dim3(256), 0, s, a, nsites);
 	return hipGetLastError();
