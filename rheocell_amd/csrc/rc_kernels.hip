@@ -1064,7 +1064,7 @@ hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hi
 // Reductions: per-workgroup partials, summed in a fixed order.  alpha is
 // re-derived by every k_cg_row workgroup from the p.Ap partials; beta and the
 // convergence flags by every k_cg_gather<P_UPDATE> workgroup from the r.r,
-// r.z partials, which only read the state: workgroup 0 of the next
+// r.z partials, which only read the state: an extra workgroup of the next
 // k_cg_box<1> writes that update (apply_update), after the gather has
 // finished, and a one-workgroup k_cg_reduce<2> at the end of a batch of
 // iterations.  All three sum in the same 256-thread order, so they agree bit
