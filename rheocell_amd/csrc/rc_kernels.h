@@ -81,7 +81,7 @@ struct CGArgs {
 	float sf, sp2, sv;
 	double rows;
 	int nbox_blk;                // workgroups of a box-site pass (partials of k_cg_box<1>)
-	int apply_update;            // k_cg_box<1>: workgroup 0 applies the previous k_cg_row<UPDATE>'s state update
+	int apply_update;            // k_cg_box<1>: an extra workgroup applies the previous k_cg_row<UPDATE>'s state update
 	const double2 *xprev, *rprev;  // previous-solution start and its residual (learned fallback)
 	double *part;                // per-cell partial sums, 6 per cell
 	double *part_box;            // per-workgroup partials of the box pass, 2 each
