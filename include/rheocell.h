@@ -139,6 +139,21 @@ int rc_total_rows(rc_handle *h, long long *rows);
  * c = the size_grid x size_grid periodic average of f (f_field_coarse).  Each
  * output is L*L doubles, site x + L y; any may be NULL. */
 int rc_box_field(rc_handle *h, int box, int which, int size_grid, double *fx, double *fy, double *cx, double *cy);
+/* rc_box_field without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The four
+ * L x L staging fields live in a static device array of the library, one per
+ * GPU, held from the library's load to the end of the process
+ * (4 * RC_BOX_FIELD_STATIC_L^2 doubles = 8 MiB); boxes with a side above
+ * RC_BOX_FIELD_STATIC_L are refused with RC_ERR_ARG.  Every output that is
+ * not NULL receives L*L doubles and may be a device pointer or a host
+ * pointer (pinned host memory keeps the copy free of runtime staging).  Every
+ * handle on one GPU shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap on one GPU (several host threads, or handles
+ * on different streams) race.  Results equal rc_box_field's bit for bit. */
+#define RC_BOX_FIELD_STATIC_L 512
+int rc_box_field_noalloc(rc_handle *h, int box, int which, int size_grid, double *fx, double *fy, double *cx,
+		double *cy);
 int rc_cg_stats(rc_handle *h, rc_cg_stats_t *out);
 int rc_set_warm_start(rc_handle *h, int mode);
 /* Kernel timing with HIP events on the library's stream (enabled by

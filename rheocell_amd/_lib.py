@@ -35,7 +35,7 @@ EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_a
            "rc_net_train_step", "rc_net_last_error", "rc_net_destroy", "rc_attach_net", "rc_snapshot_box",
            "rc_record_solves", "rc_get_recorded", "rc_set_learned_stages", "rc_set_stage_shape",
            "rc_stage_groups", "rc_set_graphs", "rc_graph_stats",
-           "rc_growth_stats", "rc_box_field"]
+           "rc_growth_stats", "rc_box_field", "rc_box_field_noalloc"]
 
 
 class rc_model_params(C.Structure):
@@ -102,6 +102,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.rc_last_error.restype = C.c_char_p
     L.rc_destroy.argtypes = [vp]
     L.rc_box_field.argtypes = [vp, ip, ip, ip, dp, dp, dp, dp]
+    # outputs of the allocation-free variant may be device pointers: plain addresses
+    L.rc_box_field_noalloc.argtypes = [vp, ip, ip, ip, vp, vp, vp, vp]
     L.rc_destroy.restype = None
     fp = C.POINTER(C.c_float)
     L.rc_net_create.argtypes = [ip, ip, ip, C.POINTER(vp)]

@@ -24,6 +24,7 @@
 #include <cstddef>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,8 @@ struct rc_handle {
 	hipGraphExec_t graph_exec = nullptr;
 	double *d_ggauss = nullptr;     // the window's predictor noise [GRAPH_STEPS][ncells]
 	double *d_bf = nullptr;         // rc_box_field: four L x L arrays of the largest box, on first use
+	int device = 0;                 // device current at rc_create
+	double *scratch = nullptr;      // this device's static staging array (rc_box_field_noalloc)
 	int *d_step_base = nullptr;     // first step of the window being replayed
 	int *h_step_base = nullptr;     // pinned, one slot per window of a chunk
 	long long graph_launches = 0, eager_steps = 0;
@@ -258,6 +261,33 @@ static DevParams make_params(const rc_model_params &m, bool active) {
 }
 
 // ---------------------------------------------------------------------------
+// Static staging of rc_box_field_noalloc: the four L x L fields (f_x, f_y and
+// their coarse averages) of one box up to RC_BOX_FIELD_STATIC_L sides.  Each
+// GPU holds its own copy from the library's load to the end of the process;
+// host code reaches it only through the address hipGetSymbolAddress returns on
+// that device (the array's name is a host-side stand-in, not a device address).
+static __device__ __align__(256) double scratch[4 * RC_BOX_FIELD_STATIC_L * RC_BOX_FIELD_STATIC_L];
+constexpr int SCRATCH_DEVICES = 64;
+static double *scratch_ptr[SCRATCH_DEVICES];
+static std::mutex scratch_mu;
+
+// once per device, with that device current (called from rc_create)
+static int scratch_setup(rc_handle *h) {
+	HIPCHK(h, hipGetDevice(&h->device));
+	if(h->device < 0 || h->device >= SCRATCH_DEVICES) return RC_OK;   // rc_box_field_noalloc refuses such a device
+	std::lock_guard<std::mutex> lock(scratch_mu);
+	if(!scratch_ptr[h->device]) {
+		void *p = nullptr;
+		hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess || !p)
+			return fail(h, RC_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", h->device, hipGetErrorString(e));
+		scratch_ptr[h->device] = (double *) p;
+	}
+	h->scratch = scratch_ptr[h->device];
+	return RC_OK;
+}
+
+// ---------------------------------------------------------------------------
 extern "C" const char *rc_last_error(rc_handle *h) { return h ? h->err.c_str() : "null handle"; }
 
 extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) {
@@ -313,6 +343,8 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 	h->ntiles = tiles;
 	h->box_sites = sites;
 	*out = h;
+	int rc = scratch_setup(h);
+	if(rc) return rc;
 	HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
 	HIPCHK(h, dalloc(&h->d_boxes, nboxes));
 	HIPCHK(h, dalloc(&h->d_params, 2 * nboxes));
@@ -1537,25 +1569,20 @@ extern "C" int rc_get_site_array(rc_handle *h, int which, double *out) {
 
 // VelocityField / ShapeField box fields of one box, the raw sums and their
 // size_grid x size_grid coarse average (VelocityField.cpp:69-128,
-// ShapeField.cpp:69-128), computed on the device from the current fields
-extern "C" int rc_box_field(rc_handle *h, int box, int which, int size_grid, double *fx, double *fy, double *cx,
-		double *cy) {
-	if(!h) return RC_ERR_ARG;
-	if(box < 0 || box >= h->nboxes) return fail(h, RC_ERR_ARG, "box %d out of range", box);
-	if(which != 0 && which != 1) return fail(h, RC_ERR_ARG, "unknown box field %d (0 velocity, 1 shape)", which);
-	const DevBox &B = h->dboxes[box];
-	if(size_grid < 1 || size_grid > B.L) return fail(h, RC_ERR_ARG, "size_grid %d outside 1..%d", size_grid, B.L);
+// ShapeField.cpp:69-128), computed on the device from the current fields into
+// work (f_x, f_y, coarse_x, coarse_y: four L x L arrays), then the requested
+// outputs copied out of it
+static int box_field_into(rc_handle *h, int box, int which, int size_grid, double *work, double *const outs[4]) {
 	int rc;
 	// the velocity rows of the current fields (the dry model does not keep them)
 	if((rc = materialize(h, false))) return rc;
-	if(!h->lists_valid && (rc = run_gather(h, h->d_phiA))) return rc;
-	const size_t n = (size_t) B.L * B.L;
-	if(!h->d_bf) {
-		size_t big = 0;
-		for(const DevBox &o : h->dboxes) big = std::max(big, (size_t) o.L * o.L);
-		HIPCHK(h, dalloc(&h->d_bf, 4 * big));
+	if(!h->lists_valid) {
+		if((rc = run_gather(h, h->d_phiA))) return rc;
+		// a list overflow or an overlap raised by this gather leaves truncated lists
+		if((rc = check_status(h))) return rc;
 	}
-	double *d = h->d_bf;
+	const DevBox &B = h->dboxes[box];
+	const size_t n = (size_t) B.L * B.L;
 	FieldArgs a{};
 	a.boxes = h->d_boxes;
 	a.tile_box = h->d_tile_box;
@@ -1567,16 +1594,53 @@ extern "C" int rc_box_field(rc_handle *h, int box, int which, int size_grid, dou
 	a.v = h->d_v;
 	a.which = which;
 	a.tile0 = B.tile_off;
-	a.fx = d;
-	a.fy = d + n;
+	a.fx = work;
+	a.fy = work + n;
 	hipError_t e = launch_box_field(a, B.ntx * B.nty, h->stream);
-	if(e == hipSuccess) e = launch_box_coarse(d, d + n, d + 2 * n, d + 3 * n, B.L, size_grid, h->stream);
-	double *outs[4] = { fx, fy, cx, cy };
+	if(e == hipSuccess) e = launch_box_coarse(work, work + n, work + 2 * n, work + 3 * n, B.L, size_grid, h->stream);
 	for(int i = 0; i < 4 && e == hipSuccess; ++i)
-		if(outs[i]) e = hipMemcpyAsync(outs[i], d + i * n, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream);
+		if(outs[i]) e = hipMemcpyAsync(outs[i], work + i * n, sizeof(double) * n, hipMemcpyDefault, h->stream);
 	if(e == hipSuccess) e = hipStreamSynchronize(h->stream);
 	if(e != hipSuccess) return fail(h, RC_ERR_HIP, "box field: %s", hipGetErrorString(e));
 	return RC_OK;
+}
+
+static int box_field_check(rc_handle *h, int box, int which, int size_grid) {
+	if(box < 0 || box >= h->nboxes) return fail(h, RC_ERR_ARG, "box %d out of range", box);
+	if(which != 0 && which != 1) return fail(h, RC_ERR_ARG, "unknown box field %d (0 velocity, 1 shape)", which);
+	const int L = h->dboxes[box].L;
+	if(size_grid < 1 || size_grid > L) return fail(h, RC_ERR_ARG, "size_grid %d outside 1..%d", size_grid, L);
+	return RC_OK;
+}
+
+extern "C" int rc_box_field(rc_handle *h, int box, int which, int size_grid, double *fx, double *fy, double *cx,
+		double *cy) {
+	if(!h) return RC_ERR_ARG;
+	int rc;
+	if((rc = box_field_check(h, box, which, size_grid))) return rc;
+	if(!h->d_bf) {
+		size_t big = 0;
+		for(const DevBox &o : h->dboxes) big = std::max(big, (size_t) o.L * o.L);
+		HIPCHK(h, dalloc(&h->d_bf, 4 * big));
+	}
+	double *const outs[4] = { fx, fy, cx, cy };
+	return box_field_into(h, box, which, size_grid, h->d_bf, outs);
+}
+
+// rc_box_field staged in this device's static array: no allocation on any path
+extern "C" int rc_box_field_noalloc(rc_handle *h, int box, int which, int size_grid, double *fx, double *fy,
+		double *cx, double *cy) {
+	if(!h) return RC_ERR_ARG;
+	int rc;
+	if((rc = box_field_check(h, box, which, size_grid))) return rc;
+	const int L = h->dboxes[box].L;
+	if(L > RC_BOX_FIELD_STATIC_L)
+		return fail(h, RC_ERR_ARG, "box side %d exceeds the static staging limit RC_BOX_FIELD_STATIC_L = %d", L,
+				RC_BOX_FIELD_STATIC_L);
+	if(!h->scratch) return fail(h, RC_ERR_ARG, "device %d has no static staging array (devices 0..%d)", h->device,
+				SCRATCH_DEVICES - 1);
+	double *const outs[4] = { fx, fy, cx, cy };
+	return box_field_into(h, box, which, size_grid, h->scratch, outs);
 }
 
 extern "C" int rc_total_rows(rc_handle *h, long long *rows) {
