@@ -44,6 +44,10 @@ class OrcRng(C.Structure):
 
 
 def lib_path(flavour: str = "strict") -> str:
+    """liborc_<flavour>.so beside this file, or flavour itself when it is a
+    path to a build of the oracle (the planted-mistake builds of the tests)."""
+    if flavour.endswith(".so"):
+        return flavour
     return os.path.join(_HERE, "liborc_%s.so" % flavour)
 
 
@@ -86,6 +90,11 @@ def _lib(flavour: str = "strict") -> C.CDLL:
     L.orc_assemble_dense.argtypes = [vp, dp]
     L.orc_test_nbrs.argtypes = [ip, ip, C.POINTER(C.c_int)]
     L.orc_test_set_positions.argtypes = [ip, C.POINTER(C.c_int), dp, dp, ip]
+    L.orc_test_set_site.argtypes = [vp, ip, ip, dp]
+    L.orc_test_set_rows.argtypes = [vp, ip, dp]
+    L.orc_test_set_polarity.argtypes = [vp, ip, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.orc_test_first_step.argtypes = [vp, ip]
+    L.orc_test_forces.argtypes = [vp]
     _LIBS[flavour] = L
     return L
 
@@ -171,6 +180,29 @@ class OracleSim:
         st = OrcCgStats()
         self.L.orc_get_cg_stats(self.h, C.byref(st))
         return st
+
+    # --- hooks on hand-set state (closed-form pins) -------------------------
+    def set_site(self, i: int, which: int, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == self.cell(i).subSize
+        self._check(self.L.orc_test_set_site(self.h, i, which, _dp(v)))
+
+    def set_rows(self, which: int, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == self.L.orc_size_rows(self.h)
+        self._check(self.L.orc_test_set_rows(self.h, which, _dp(v)))
+
+    def set_polarity(self, i: int, theta: float, theta_old: float, tr=(0.0, 0.0)):
+        self._check(self.L.orc_test_set_polarity(self.h, i, theta, theta_old, tr[0], tr[1]))
+
+    def first_step(self, store: bool):
+        self._check(self.L.orc_test_first_step(self.h, 1 if store else 0))
+
+    def forces(self):
+        self._check(self.L.orc_test_forces(self.h))
+
+    def apply_changes_after_equilibration(self):
+        self.L.orc_apply_changes_after_equilibration(self.h)
 
     def apply_A(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -866,3 +866,43 @@ int orc_test_set_positions(int Lbox, int *ints, double *dbl, double *phi, int ph
 	cell_free(&p);
 	return rc;
 }
+
+int orc_test_set_site(orc_sim *s, int i, int which, const double *in) {
+	if(i < 0 || i >= s->N) return set_err(s, "cell index %d out of range", i);
+	ocell *p = &s->c[i];
+	double *dst = which == 0 ? p->phi : which == 1 ? p->mu : which == 2 ? p->dx : which == 3 ? p->dy
+			: which == 4 ? p->lap : which == 5 ? p->phi_old : which == 6 ? p->dphi_old : NULL;
+	if(!dst) return set_err(s, "bad site array %d", which);
+	memcpy(dst, in, sizeof(double) * (size_t) p->S);
+	return 0;
+}
+
+int orc_test_set_rows(orc_sim *s, int which, const double *in) {
+	double *dst = which == 0 ? s->vx : which == 1 ? s->vy : NULL;
+	if(!dst) return set_err(s, "bad row array %d", which);
+	memcpy(dst, in, sizeof(double) * (size_t) s->size_rows);
+	return 0;
+}
+
+int orc_test_set_polarity(orc_sim *s, int i, double thetaQ, double thetaQ_old, double tr0, double tr1) {
+	if(i < 0 || i >= s->N) return set_err(s, "cell index %d out of range", i);
+	ocell *p = &s->c[i];
+	p->thetaQ = thetaQ;
+	p->thetaQ_old = thetaQ_old;
+	p->Q00 = cos(thetaQ);
+	p->Q01 = sin(thetaQ);
+	p->tr[0] = tr0;
+	p->tr[1] = tr1;
+	return 0;
+}
+
+int orc_test_first_step(orc_sim *s, int store) { return first_step(s, store); }
+
+int orc_test_forces(orc_sim *s) {
+	for(int k = 0; k < s->L * s->L; k++) {
+		s->phi2[k] = 0;
+		s->sum_phi[k] = 0;
+	}
+	for(int i = 0; i < s->N; i++) s->c[i].perimeter = 0;
+	return compute_forces(s);
+}

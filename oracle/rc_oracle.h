@@ -108,6 +108,22 @@ void orc_test_nbrs(int LX, int LY, int *out);          /* 9*LX*LY */
  * values.  Returns 0 or -1. */
 int orc_test_set_positions(int Lbox, int *ints, double *dbl, double *phi, int phi_cap);
 
+/* Hooks that run the oracle's own first_step / compute_forces on hand-set
+ * state, so closed forms can pin them (tests only).
+ * orc_test_set_site: which as orc_get_site_array (0..6).
+ * orc_test_set_rows: which 0 vx, 1 vy (the rows first_step advects with).
+ * orc_test_set_polarity: thetaQ, thetaQ_old, Q = (cos, sin) of thetaQ, and the
+ *   CIL repulsion sums total_force_repulsion.
+ * orc_test_first_step: FD_CPUBackend::first_step(store).
+ * orc_test_forces: resetSums over the box and the perimeter reset of
+ *   set_properties_to_zero (as first_step does; area and the other sums are
+ *   kept), then begin_energy_computation(fields). */
+int orc_test_set_site(orc_sim *s, int i, int which, const double *in);
+int orc_test_set_rows(orc_sim *s, int which, const double *in);
+int orc_test_set_polarity(orc_sim *s, int i, double thetaQ, double thetaQ_old, double tr0, double tr1);
+int orc_test_first_step(orc_sim *s, int store);
+int orc_test_forces(orc_sim *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,9 @@
 //   nbrsub LX LY                    -> neighbors_sub table (periodic in the patch)
 //   setpos Lbox LX LY offx offy corner ux uy comx comy xsl ysb phi_0 ... phi_{S-1}
 //                                   -> MultiPhaseField::set_positions(box) result
+//   stencil Lbox LX LY offx offy corner phi_0 ... phi_{S-1}
+//                                   -> BaseInteraction::derivX / derivY / Laplacian
+//                                      (BaseInteraction.h:196-263) at every site
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -20,7 +23,20 @@
 
 #include "Boxes/SquareBox.h"
 #include "Fields/MultiPhaseField.h"
+#include "Interactions/BaseInteraction.h"
 #include "Utilities/Utils.h"
+
+// The reference's stencils are members of the abstract BaseInteraction; this
+// driver-side subclass only makes it instantiable and exposes them.
+class StencilProbe : public BaseInteraction {
+public:
+	void init() override {}
+	void allocate_fields(std::vector<BaseField *> &) override {}
+	void check_input_sanity(std::vector<BaseField *> &) override {}
+	double dx(BaseField *p, int q, int k) { return derivX(p, q, k); }
+	double dy(BaseField *p, int q, int k) { return derivY(p, q, k); }
+	double lap(BaseField *p, int q, int k) { return Laplacian(p, q, k); }
+};
 
 static void dump_vec(const char *name, const std::vector<double> &v) {
 	printf("\"%s\": [", name);
@@ -106,6 +122,32 @@ int main() {
 				   f.LsubX, f.LsubY, f.offset[0], f.offset[1], f.sub_corner_bottom_left, f.sub_corner_bottom_left_old,
 				   f.unrap_sub_corner_bottom_left_x, f.unrap_sub_corner_bottom_left_y, f.CoM[0], f.CoM[1], f.area);
 			dump_vec("phi", f.fieldScalar);
+			printf("}\n");
+		}
+		else if(cmd == "stencil") {
+			int Lb, LX, LY, ox, oy, corner;
+			in >> Lb >> LX >> LY >> ox >> oy >> corner;
+			SquareBox box;
+			box.init(Lb, Lb);
+			MultiPhaseField f;
+			f.init(LX, LY);
+			f.set_positions(ox, oy, corner, 0, 0, Lb);
+			for(int q = 0; q < f.subSize; ++q) in >> f.fieldScalar[q];
+			StencilProbe probe;
+			probe.set_box(&box);
+			std::vector<double> dx, dy, lap;
+			for(int q = 0; q < f.subSize; ++q) {
+				int k = f.GetSubIndex(q, &box);
+				dx.push_back(probe.dx(&f, q, k));
+				dy.push_back(probe.dy(&f, q, k));
+				lap.push_back(probe.lap(&f, q, k));
+			}
+			printf("{\"cmd\": \"stencil\", \"args\": [%d, %d, %d, %d, %d, %d], ", Lb, LX, LY, ox, oy, corner);
+			dump_vec("dx", dx);
+			printf(", ");
+			dump_vec("dy", dy);
+			printf(", ");
+			dump_vec("lap", lap);
 			printf("}\n");
 		}
 		fflush(stdout);

@@ -1,11 +1,12 @@
 #!/usr/bin/env python3
-"""Writes tests/golden/ref_fields.json from the reference's own code.
+"""Writes tests/golden/ref_fields.json and ref_stencils.json from the
+reference's own code.
 
 Runs oracle/_ref/ref_fields (oracle/ref_driver.cpp linked against the
-reference's MultiPhaseField, SquareBox and Utils sources, compiled from
-/root/reference by oracle/build_ref.sh) and, for the trig tables only, the
-same driver compiled with the reference's Release flags (ref_fields_fast).
-The fixture holds inputs and outputs only.
+reference's MultiPhaseField, SquareBox, BaseInteraction and Utils sources,
+compiled from /root/reference by oracle/build_ref.sh) and, for the trig
+tables only, the same driver compiled with the reference's Release flags
+(ref_fields_fast).  The fixtures hold inputs and outputs only.
 
 Usage: bash oracle/build_ref.sh && python tests/golden/make_golden.py
 """
@@ -89,6 +90,28 @@ def main():
     with open(os.path.join(HERE, "ref_fields.json"), "w") as f:
         json.dump(out, f, separators=(",", ":"))
     print("wrote", os.path.join(HERE, "ref_fields.json"))
+    stencils()
+
+
+def stencils():
+    """BaseInteraction::derivX / derivY / Laplacian (BaseInteraction.h:196-263)
+    of random patches in a periodic square box, ring offsets included."""
+    rng = np.random.default_rng(77)
+    cases = [(64, 9, 7, 0, 0, 0), (64, 9, 7, 4, 5, 63 * 64 + 60), (140, 13, 11, 12, 3, 70 * 140 + 139),
+             (128, 29, 31, 17, 30, 127), (96, 21, 21, 0, 20, 95 * 96)]
+    cmds, inputs = [], []
+    for (Lb, LX, LY, ox, oy, corner) in cases:
+        phi = np.round(rng.uniform(-0.2, 1.2, LX * LY), 12)
+        inputs.append([float(v) for v in phi])
+        cmds.append("stencil %d %d %d %d %d %d " % (Lb, LX, LY, ox, oy, corner) + " ".join("%.17g" % v for v in phi))
+    res = run(REF, cmds)
+    for r, phi in zip(res, inputs):
+        r["phi"] = phi
+    path = os.path.join(HERE, "ref_stencils.json")
+    with open(path, "w") as f:
+        json.dump({"source": "reference BaseInteraction stencils on MultiPhaseField/SquareBox (oracle/build_ref.sh)",
+                   "stencil": res}, f, separators=(",", ":"))
+    print("wrote", path)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,379 @@
+"""Pins of the oracle's per-site and per-cell arithmetic that the reference's
+own code cannot supply here (WetPolarModel needs Eigen to compile): closed
+forms the mathematics fixes, identities, and the reference's own stencils.
+
+Every pin is a function of the oracle build it checks (``flavour``), so that
+``test_planted_mistake_is_caught`` can run it against copies of the oracle
+with one plausible mistake planted (compiled on the CPU into a temporary
+directory) and show that it fails there.
+
+What each pin fixes (reference file:line):
+  heun_flat            FD_CPUBackend.cpp:66-84 (predictor-corrector), WetPolarModel.cpp:276-278
+  ramp_advection       FD_CPUBackend.cpp:66-84 (advection term, Heun weights)
+  com_cosine_moment    FD_CPUBackend.cpp:86-106 (trig sums, CoM formula, ring offset)
+  shape_tensor         FD_CPUBackend.cpp:98 + BaseInteraction.cpp:62-69, WetPolarModel.cpp:131-143
+  repulsion            WetPolarModel.cpp:281
+  free_energy          WetPolarModel.cpp:276-278, 291 and the perimeter sum of :246
+  passive_force        WetPolarModel.cpp:309-310 (and the dry v = f / xi, :194-198)
+  adhesion             WetPolarModel.cpp:268, 284-286
+  jq_theta             WetPolarModel.cpp:365-372
+  cil_sums             WetPolarModel.cpp:313-314 (summation by parts)
+  stencils             BaseInteraction.h:196-263 (golden vectors of the reference's code)
+"""
+import json
+import math
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import OracleSim
+from rheocell_amd import confgen
+from rheocell_amd.conf import CellConf, Conf
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+PI_F = float(np.float32(math.pi))
+
+
+def a0_of(R):
+    # a0 = PI * R * R with PI a float (defs.h:4, WetPolarModel.cpp:50)
+    return float(np.float32(np.float32(PI_F) * np.float32(R)) * np.float32(R))
+
+
+def _keys(**kw):
+    k = dict(confgen.BASE_KEYS)
+    k.update(dict(zetaQ_self=0.0, zetaQ_inter=0.0, J_Q=0.0, omega=0.0, beta=0.0))
+    k.update(kw)
+    return k
+
+
+def _cell(L, x0, y0, LX, LY, F, off=(0, 0), theta=0.3):
+    """A patch whose value at logical site (lx, ly) is F(lx, ly); storage
+    order follows the ring offset (MultiPhaseField.cpp:348-350)."""
+    q = np.arange(LX * LY)
+    lx = (q % LX + off[0]) % LX
+    ly = (q // LX + off[1]) % LY
+    phi = np.asarray(F(lx, ly), dtype=np.float64) * np.ones(LX * LY)
+    corner = (x0 % L) + (y0 % L) * L
+    return CellConf(LX, LY, ((x0 + LX / 2) % L, (y0 + LY / 2) % L), off, corner, (float(x0), float(y0)),
+                    math.cos(theta), math.sin(theta), np.zeros(LX * LY, dtype=np.int64), phi)
+
+
+def _logical(LX, LY, off=(0, 0)):
+    q = np.arange(LX * LY)
+    return (q % LX + off[0]) % LX, (q // LX + off[1]) % LY
+
+
+# --------------------------------------------------------------------------
+def pin_heun_flat(flavour="strict"):
+    """An isolated flat patch: grad = lap = 0, so mu = CH(c) + A(c, area) is
+    uniform, v = 0 and one Heun step is c -> c + dt/2 (d(phi*) + d(c)) with
+    phi* = c + dt d(c), d(x) = -J0 mu(x, S x^2)."""
+    L, LX, c = 64, 21, 0.3
+    keys = _keys(J0=1.3, dt=0.1, gamma=0.02, mu=45.0)
+    s = OracleSim(Conf(0, L, L, [_cell(L, 10, 10, LX, LX, lambda x, y: c)]), keys, flavour=flavour)
+    g, lam, m, J0, dt = keys["gamma"], keys["lambda"], keys["mu"], keys["J0"], keys["dt"]
+    a0 = a0_of(keys["R"])
+    S = LX * LX
+
+    def mu(x):
+        return g * 8 * x * (1 - x) * (1 - 2 * x) / lam - 4 * (m / a0) * (1 - S * x * x / a0) * x
+
+    d0 = -J0 * mu(c)
+    ps = c + dt * d0
+    expect = c + dt / 2 * (-J0 * mu(ps) + d0)
+    s.step(1)
+    phi = s.phi(0)
+    assert np.allclose(phi, expect, rtol=1e-13, atol=0), (phi[0], expect)
+    assert abs(expect - c) > 1e-4   # the step moved the field
+
+
+def pin_ramp_advection(flavour="strict"):
+    """A linear ramp phi = a + b x + c y with its exact central differences,
+    a uniform velocity (u, w) and a uniform mu = m: the predictor translates
+    the ramp by (u, w) dt and adds -J0 m dt; the corrector with a second
+    mu = m2 is the trapezoidal rule from phi_n."""
+    L, LX, LY = 64, 15, 13
+    a, b, c, u, w, m, m2 = 0.5, 0.013, -0.021, 0.7, -0.45, 0.31, -0.17   # phi > 0.09: no growth request
+    keys = _keys(J0=1.0, dt=0.1)
+    lx, ly = _logical(LX, LY)
+    ramp = a + b * lx + c * ly
+    s = OracleSim(Conf(0, L, L, [_cell(L, 20, 20, LX, LY, lambda x, y: a + b * x + c * y)]), keys, flavour=flavour)
+    dt, J0 = keys["dt"], keys["J0"]
+    n = LX * LY
+    s.set_site(0, 1, np.full(n, m))
+    s.set_site(0, 2, np.full(n, b))
+    s.set_site(0, 3, np.full(n, c))
+    s.set_rows(0, np.full(n, u))
+    s.set_rows(1, np.full(n, w))
+    s.first_step(True)
+    pred = a + b * (lx - u * dt) + c * (ly - w * dt) - J0 * m * dt
+    assert np.allclose(s.phi(0), pred, rtol=1e-14, atol=1e-15)
+    s.set_site(0, 1, np.full(n, m2))
+    s.first_step(False)
+    d1 = -J0 * m - u * b - w * c
+    d2 = -J0 * m2 - u * b - w * c
+    assert np.allclose(s.phi(0), ramp + dt / 2 * (d1 + d2), rtol=1e-14, atol=1e-15)
+
+
+def pin_com_cosine_moment(flavour="strict"):
+    """phi = 1 + e cos(2 pi (x - x0) / LX) + e cos(2 pi (y - y0) / LY) in
+    logical patch coordinates has circular mean (x0, y0): the CoM of
+    FD_CPUBackend.cpp:105-106 must return it (float-pi tables: ~1e-6)."""
+    L, LX, LY, off = 64, 21, 17, (5, 3)
+    x0, y0, e = 6.3, 11.7, 0.2
+    F = lambda x, y: 1 + e * np.cos(2 * np.pi * (x - x0) / LX) + e * np.cos(2 * np.pi * (y - y0) / LY)  # noqa: E731
+    s = OracleSim(Conf(0, L, L, [_cell(L, 9, 30, LX, LY, F, off=off)]), _keys(), flavour=flavour)
+    n = LX * LY
+    for which in (1, 2, 3):      # mu = grad phi = 0: the field stays put
+        s.set_site(0, which, np.zeros(n))
+    s.set_rows(0, np.zeros(n))
+    s.set_rows(1, np.zeros(n))
+    s.first_step(True)
+    com = s.cell(0).CoM
+    assert abs(com[0] - x0) < 2e-5 and abs(com[1] - y0) < 2e-5, (com[0], com[1])
+
+
+def pin_shape_tensor(flavour="strict"):
+    """S00 = -1/2 sum (dx^2 - dy^2), S01 = -sum dx dy of a plane wave
+    phi = c + e cos(kx x + ky y) (periodic in the patch), whose central
+    differences are -e sin(.) sin(kx) and -e sin(.) sin(ky): sum sin^2 = S / 2."""
+    L, LX, LY, c, e = 64, 20, 15, 0.5, 0.1
+    kx, ky = 2 * np.pi / LX, 2 * np.pi / LY
+    s = OracleSim(Conf(0, L, L, [_cell(L, 5, 5, LX, LY, lambda x, y: c + e * np.cos(kx * x + ky * y))]), _keys(),
+                  flavour=flavour)
+    S = LX * LY
+    info = s.cell(0)
+    s00 = -0.5 * e * e * (math.sin(kx) ** 2 - math.sin(ky) ** 2) * S / 2
+    s01 = -e * e * math.sin(kx) * math.sin(ky) * S / 2
+    assert abs(info.S00 - s00) < 1e-12 and abs(info.S01 - s01) < 1e-12, (info.S00, s00, info.S01, s01)
+
+
+def pin_repulsion(flavour="strict"):
+    """Two overlapping flat patches a, b: inside the overlap mu_a gains
+    4 (kappa / lambda) a b^2 over its value outside."""
+    L, LX, a, b = 64, 21, 0.6, 0.5
+    keys = _keys(kappa=0.3, gamma=0.0, mu=0.0)
+    cells = [_cell(L, 10, 10, LX, LX, lambda x, y: a), _cell(L, 20, 14, LX, LX, lambda x, y: b)]
+    s = OracleSim(Conf(0, L, L, cells), keys, flavour=flavour)
+    mu = s.site(0, 1).reshape(LX, LX)            # A: offset 0, [ly, lx] = box (10 + lx, 10 + ly)
+    rep = 4 * (keys["kappa"] / keys["lambda"]) * a * b * b
+    assert np.allclose(mu[8:, 12:], rep, rtol=1e-14, atol=0)    # box x >= 22, y >= 18: overlap
+    assert np.all(mu[:3, :] == 0) and np.all(mu[:, :8] == 0)    # no overlap
+    mub = s.site(1, 1).reshape(LX, LX)           # B sees a^2
+    assert np.allclose(mub[2:15, 2:9], 4 * (keys["kappa"] / keys["lambda"]) * b * a * a, rtol=1e-14, atol=0)
+
+
+def pin_free_energy(flavour="strict"):
+    """An isolated cell phi = c + e cos(kx x) + d cos(ky y): the discrete
+    Laplacian is 2 e (cos kx - 1) cos(kx x) + 2 d (cos ky - 1) cos(ky y), the
+    perimeter sum |grad phi|^2 = (LX LY / 2)(e^2 sin^2 kx + d^2 sin^2 ky), the
+    area sum phi^2 = LX LY (c^2 + e^2 / 2 + d^2 / 2); mu = CH + A + P."""
+    L, LX, LY, c, e, d = 64, 21, 19, 0.45, 0.2, 0.15
+    kx, ky = 2 * np.pi / LX, 2 * np.pi / LY
+    keys = _keys(gamma=0.05, mu=30.0, beta=0.4, kappa=0.0)
+    s = OracleSim(Conf(0, L, L, [_cell(L, 12, 8, LX, LY, lambda x, y: c + e * np.cos(kx * x) + d * np.cos(ky * y))]),
+                  keys, flavour=flavour)
+    s.apply_changes_after_equilibration()       # beta is 0 until then (WetPolarModel.cpp:4-21, 84-93)
+    s.forces()
+    lx, ly = _logical(LX, LY)
+    phi = c + e * np.cos(kx * lx) + d * np.cos(ky * ly)
+    lap = 2 * e * (math.cos(kx) - 1) * np.cos(kx * lx) + 2 * d * (math.cos(ky) - 1) * np.cos(ky * ly)
+    S = LX * LY
+    perim = S / 2 * (e * e * math.sin(kx) ** 2 + d * d * math.sin(ky) ** 2)
+    area = S * (c * c + e * e / 2 + d * d / 2)
+    g, lam, m, beta, a0, p0 = keys["gamma"], keys["lambda"], keys["mu"], keys["beta"], a0_of(keys["R"]), 12.0
+    CH = g * (8 * phi * (1 - phi) * (1 - 2 * phi) / lam - 2 * lam * lap)
+    A = -4 * (m / a0) * (1 - area / a0) * phi
+    P = 4 * (beta / p0) * (1 - perim / p0) * lap
+    assert np.allclose(s.site(0, 4), lap, rtol=0, atol=1e-13)
+    assert abs(s.cell(0).perimeter - perim) < 1e-11
+    assert np.allclose(s.site(0, 1), CH + A + P, rtol=1e-11, atol=1e-13)
+
+
+def pin_passive_force(flavour="strict"):
+    """gamma = kappa = omega = beta = 0 leaves mu = -4 (mu/a0)(1 - area/a0) phi,
+    linear in phi: on a ramp phi = a + b x the passive force -1/2 (mu(x+1) -
+    mu(x-1)) is 4 (mu/a0)(1 - area/a0) b away from the patch wrap, f_y = 0,
+    and the dry velocity is f / xi."""
+    L, LX, LY, a, b = 64, 21, 17, 0.2, 0.03
+    keys = _keys(gamma=0.0, kappa=0.0, mu=45.0, passive_alpha=1.0)
+    s = OracleSim(Conf(0, L, L, [_cell(L, 10, 10, LX, LY, lambda x, y: a + b * x)]), keys, flavour=flavour)
+    lx, _ = _logical(LX, LY)
+    area = float(np.sum((a + b * lx) ** 2))
+    a0 = a0_of(keys["R"])
+    fx = 4 * (keys["mu"] / a0) * (1 - area / a0) * b
+    vx, vy = s.rows(0), s.rows(1)      # friction 1 before equilibration
+    inner = (lx >= 1) & (lx <= LX - 2)
+    assert np.allclose(vx[inner], fx, rtol=1e-12, atol=0)
+    assert np.all(vy == 0)
+    assert abs(fx) > 1e-3
+
+
+def pin_adhesion(flavour="strict"):
+    """Adh = -4 lambda omega s / sqrt(1 + s^2) phi with s = lap_box(sum phi^2)
+    - (2 phi lap phi + 2 |grad phi|^2).
+    (1) two flat patches a, b: across B's left edge s = -b^2 (inside) and
+        +b^2 (outside); (2) one cosine cell phi = c + e cos(k x): sum phi^2 =
+        phi^2 and the discrete operators have closed forms."""
+    L, LX, a, b = 64, 21, 0.6, 0.5
+    keys = _keys(omega=0.05, gamma=0.0, mu=0.0, kappa=0.0)
+    lam, om = keys["lambda"], keys["omega"]
+    cells = [_cell(L, 10, 10, LX, LX, lambda x, y: a), _cell(L, 20, 5, LX, LX, lambda x, y: b)]
+    s = OracleSim(Conf(0, L, L, cells), keys, flavour=flavour)
+    mu = s.site(0, 1).reshape(LX, LX)       # [ly, lx] = box (10 + lx, 10 + ly); B covers x >= 20, y <= 25
+    adh = lambda sv, ph: -4 * lam * om * sv / math.sqrt(1 + sv * sv) * ph  # noqa: E731
+    assert np.allclose(mu[2:14, 10], adh(-b * b, a), rtol=1e-14, atol=0)   # box x = 20
+    assert np.allclose(mu[2:14, 9], adh(b * b, a), rtol=1e-14, atol=0)     # box x = 19
+    assert np.all(mu[2:14, 12:19] == 0) and np.all(mu[2:14, 1:8] == 0)
+    # (2)
+    LX, LY, c, e = 10, 21, 0.5, 0.3
+    k = 2 * np.pi / LX
+    s = OracleSim(Conf(0, L, L, [_cell(L, 20, 20, LX, LY, lambda x, y: c + e * np.cos(k * x))]), keys,
+                  flavour=flavour)
+    lx, ly = _logical(LX, LY)
+    phi = c + e * np.cos(k * lx)
+    phi2 = phi * phi                      # = c^2 + e^2/2 + 2 c e cos(kx) + e^2/2 cos(2kx)
+    lap_box = 2 * c * e * 2 * (math.cos(k) - 1) * np.cos(k * lx) + e * e / 2 * 2 * (math.cos(2 * k) - 1) * np.cos(2 * k * lx)
+    lap = e * 2 * (math.cos(k) - 1) * np.cos(k * lx)
+    dx = -e * np.sin(k * lx) * math.sin(k)
+    sv = lap_box - (2 * phi * lap + 2 * dx * dx)
+    expect = -4 * lam * om * sv / np.sqrt(1 + sv * sv) * phi
+    inner = (lx >= 1) & (lx <= LX - 2) & (ly >= 1) & (ly <= LY - 2)
+    assert np.allclose(s.site(0, 1)[inner], expect[inner], rtol=1e-9, atol=1e-14)
+    assert np.abs(expect[inner]).max() > 1e-4
+    assert np.allclose(s.box(0, L)[(20 + lx + (20 + ly) * L)], phi2, rtol=1e-14)
+
+
+def pin_jq_theta(flavour="strict"):
+    """theta = theta_old - dt J_Q atan2(F x p, F . p): the polarity turns
+    away from the CIL force F by dt J_Q times the signed angle from F to p,
+    taken in (-pi, pi]."""
+    L, LX = 64, 21
+    keys = _keys(J_Q=0.7, dt=0.1)
+    for th, th_old, ang_f in ((0.4, 0.9, -1.1), (2.5, 2.2, -2.0), (-0.3, 0.1, 1.9)):
+        s = OracleSim(Conf(0, L, L, [confgen.tanh_cell(L, 32.0, 32.0, 6.0, 2.0, LX, 0.0)]), keys, flavour=flavour)
+        s.apply_changes_after_equilibration()
+        n = LX * LX
+        for which in (1, 2, 3):
+            s.set_site(0, which, np.zeros(n))
+        s.set_rows(0, np.zeros(n))
+        s.set_rows(1, np.zeros(n))
+        F = 2.0 * np.array([math.cos(ang_f), math.sin(ang_f)])
+        s.set_polarity(0, th, th_old, tuple(F))
+        s.first_step(False)
+        rel = math.remainder(th - ang_f, 2 * math.pi)
+        expect = th_old - keys["dt"] * keys["J_Q"] * rel
+        info = s.cell(0)
+        assert abs(info.thetaQ - expect) < 1e-14, (info.thetaQ, expect)
+        assert abs(info.Q00 - math.cos(expect)) < 1e-14 and abs(info.Q01 - math.sin(expect)) < 1e-14
+
+
+def pin_cil_sums(flavour="strict"):
+    """Summation by parts on the periodic patch: sum_q -1/2 (R(q+E) - R(q+W))
+    phi(q) = sum_q R(q) dx(q) (likewise in y), with R the Repulsion rows."""
+    conf = confgen.lattice_conf(40, 2, seed=12)
+    s = OracleSim(conf, _keys(kappa=0.2), flavour=flavour)
+    worst = 0.0
+    for i in range(len(conf.cells)):
+        info = s.cell(i)
+        rep, dx, dy = s.site(i, 7), s.site(i, 2), s.site(i, 3)
+        tx, ty = float(np.dot(rep, dx)), float(np.dot(rep, dy))
+        scale = np.abs(rep).sum() * np.abs(dx).max()
+        assert scale > 1e-3
+        worst = max(worst, abs(info.total_force_repulsion[0] - tx) / scale, abs(info.total_force_repulsion[1] - ty) / scale)
+    assert worst < 1e-13, worst
+
+
+STENCILS = json.load(open(os.path.join(HERE, "golden", "ref_stencils.json")))["stencil"]
+
+
+def pin_stencils(flavour="strict"):
+    """derivX / derivY / Laplacian of the oracle equal the reference's own
+    BaseInteraction stencils bit for bit (tests/golden/make_golden.py)."""
+    for rec in STENCILS:
+        Lb, LX, LY, ox, oy, corner = rec["args"]
+        phi = np.array(rec["phi"])
+        S = LX * LY
+        c = CellConf(LX, LY, (LX / 2, LY / 2), (ox, oy), corner, (0.0, 0.0), 1.0, 0.0, np.zeros(S, dtype=np.int64), phi)
+        s = OracleSim(Conf(0, Lb, Lb, [c]), _keys(), flavour=flavour)
+        assert np.array_equal(s.site(0, 2), np.array(rec["dx"]))
+        assert np.array_equal(s.site(0, 3), np.array(rec["dy"]))
+        assert np.array_equal(s.site(0, 4), np.array(rec["lap"]))
+
+
+PINS = {name[4:]: fn for name, fn in list(globals().items()) if name.startswith("pin_")}
+
+
+@pytest.mark.parametrize("name", sorted(PINS))
+def test_pin(name):
+    PINS[name]()
+
+
+# --------------------------------------------------------------------------
+# Planted mistakes: (label, text in oracle/rc_oracle.c, replacement, pins that must fail)
+MUTANTS = [
+    ("heun_full_dt", "dt * .5 * (dphi + p->dphi_old[q])", "dt * (dphi + p->dphi_old[q])", ["heun_flat", "ramp_advection"]),
+    ("heun_from_phi", "double phi = p->phi_old[q] + dt", "double phi = p->phi[q] + dt", ["heun_flat", "ramp_advection"]),
+    ("heun_J0_sign", "double dphi = -J0 * p->mu[q]", "double dphi = J0 * p->mu[q]", ["heun_flat", "ramp_advection"]),
+    ("advection_sign", "- s->vx[row] * p->dx[q] - s->vy[row] * p->dy[q]",
+     "+ s->vx[row] * p->dx[q] - s->vy[row] * p->dy[q]", ["ramp_advection"]),
+    ("advection_swapped", "- s->vx[row] * p->dx[q] - s->vy[row] * p->dy[q]",
+     "- s->vx[row] * p->dy[q] - s->vy[row] * p->dx[q]", ["ramp_advection"]),
+    ("area_sum", "p->area += phi * phi;", "p->area += phi;", ["heun_flat"]),
+    ("com_no_pi", "p->CoM[0] = p->LX * (atan2(-temp[1] / p->sumF, -temp[0] / p->sumF) + PI_F)",
+     "p->CoM[0] = p->LX * (atan2(-temp[1] / p->sumF, -temp[0] / p->sumF))", ["com_cosine_moment"]),
+    ("com_swapped", "atan2(-temp[1] / p->sumF, -temp[0] / p->sumF)", "atan2(-temp[0] / p->sumF, -temp[1] / p->sumF)",
+     ["com_cosine_moment"]),
+    ("com_no_offset", "int x = (((q - (q / p->LX) * p->LX) + p->offset[0]) % p->LX);",
+     "int x = (((q - (q / p->LX) * p->LX)) % p->LX);", ["com_cosine_moment"]),
+    ("shape_sign", "p->S00 += -0.5 * (dx * dx - dy * dy);", "p->S00 += 0.5 * (dx * dx - dy * dy);", ["shape_tensor"]),
+    ("shape_cross", "p->S01 += -dx * dy;", "p->S01 += -dx * dx;", ["shape_tensor"]),
+    ("rep_no_self", "(phi2[k] - ph * ph)", "(phi2[k])", ["repulsion"]),
+    ("rep_coefficient", "4 * (kappa / lambda)", "4 * (kappa * lambda)", ["repulsion"]),
+    ("area_sign", "(1 - p->area / a0)", "(1 + p->area / a0)", ["heun_flat", "free_energy", "passive_force"]),
+    ("ch_lap", "- 2 * lambda * p->lap[q]", "- lambda * p->lap[q]", ["free_energy"]),
+    ("perimeter_sign", "(1 - p->perimeter / s->p0)", "(1 + p->perimeter / s->p0)", ["free_energy"]),
+    ("perimeter_x_only", "p->perimeter += p->dx[q] * p->dx[q] + p->dy[q] * p->dy[q];",
+     "p->perimeter += p->dx[q] * p->dx[q];", ["free_energy"]),
+    ("passive_no_half", "double f_passive_x = (-1) * 0.5 * (p->mu[nb[5]] - p->mu[nb[3]]);",
+     "double f_passive_x = (-1) * (p->mu[nb[5]] - p->mu[nb[3]]);", ["passive_force"]),
+    ("passive_sign", "double f_passive_x = (-1) * 0.5 * (p->mu[nb[5]] - p->mu[nb[3]]);",
+     "double f_passive_x = 0.5 * (p->mu[nb[5]] - p->mu[nb[3]]);", ["passive_force"]),
+    ("adhesion_sign", "double Adh = -4 * lambda * omega", "double Adh = 4 * lambda * omega", ["adhesion"]),
+    ("adhesion_lsquare", "double lsquare = 2 * ph * p->lap[q] + 2 *", "double lsquare = ph * p->lap[q] + 2 *",
+     ["adhesion"]),
+    ("adhesion_box_stencil", "phi2[bn[5]] + phi2[bn[7]] + phi2[bn[3]] + phi2[bn[1]] - 4. * phi2[k]",
+     "phi2[bn[5]] + phi2[bn[7]] + phi2[bn[3]] + phi2[bn[3]] - 4. * phi2[k]", ["adhesion"]),
+    ("jq_sign", "atan2(f0 * p->Q01 - f1 * p->Q00, f0 * p->Q00 + f1 * p->Q01)",
+     "atan2(f1 * p->Q00 - f0 * p->Q01, f0 * p->Q00 + f1 * p->Q01)", ["jq_theta"]),
+    ("jq_no_dt", "p->thetaQ = p->thetaQ_old - dt * s->J_Q", "p->thetaQ = p->thetaQ_old - s->J_Q", ["jq_theta"]),
+    ("cil_no_half", "p->tr[0] += (-1) * 0.5 * (p->rep[nb[5]] - p->rep[nb[3]]) * p->phi[q];",
+     "p->tr[0] += (-1) * (p->rep[nb[5]] - p->rep[nb[3]]) * p->phi[q];", ["cil_sums"]),
+    ("derivx_no_half", "return .5 * (p->phi[p->nbr[5 + q * 9]] - p->phi[p->nbr[3 + q * 9]]);",
+     "return (p->phi[p->nbr[5 + q * 9]] - p->phi[p->nbr[3 + q * 9]]);", ["stencils"]),
+    ("laplacian_centre", "+ p->phi[p->nbr[3 + q * 9]] - 4 * p->phi[q];", "+ p->phi[p->nbr[3 + q * 9]] - 2 * p->phi[q];",
+     ["stencils"]),
+]
+
+
+@pytest.fixture(scope="module")
+def oracle_source():
+    return open(os.path.join(ROOT, "oracle", "rc_oracle.c")).read()
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc is needed to build the planted-mistake oracles")
+@pytest.mark.parametrize("label,old,new,pins", MUTANTS, ids=[m[0] for m in MUTANTS])
+def test_planted_mistake_is_caught(tmp_path, oracle_source, label, old, new, pins):
+    assert oracle_source.count(old) == 1, "the text to mutate must occur once: %r" % old
+    src = tmp_path / "rc_oracle.c"
+    src.write_text(oracle_source.replace(old, new))
+    lib = str(tmp_path / ("liborc_%s.so" % label))
+    subprocess.run(["gcc", "-std=c11", "-O1", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+                    "-I" + os.path.join(ROOT, "oracle"), "-o", lib, str(src), "-lm"], check=True)
+    for name in pins:
+        with pytest.raises(AssertionError):
+            PINS[name](flavour=lib)
