@@ -107,6 +107,7 @@ typedef struct rc_cg_stats_t {
 	long long learned_used;          /* solves that kept the network's start */
 	long long learned_tried;         /* solves that ran the network stages (the rest fell
 	                                    back to the previous solution in some component) */
+	double worst_relres;             /* largest final |r| / |b| of any component of any solve */
 } rc_cg_stats_t;
 
 #define RC_WARM_REFERENCE 0   /* previous solution, cold restart every 11th solve (reference rule) */

@@ -69,7 +69,7 @@ class rc_cg_stats_t(C.Structure):
     _fields_ = [("solves", C.c_longlong), ("iters_total", C.c_longlong), ("last_iters", C.c_int * 2),
                 ("last_relres", C.c_double * 2), ("last_cold", C.c_int), ("last_solve_ms", C.c_double),
                 ("total_solve_ms", C.c_double), ("sweeps_total", C.c_longlong),
-                ("learned_used", C.c_longlong), ("learned_tried", C.c_longlong)]
+                ("learned_used", C.c_longlong), ("learned_tried", C.c_longlong), ("worst_relres", C.c_double)]
 
 
 _lib = None

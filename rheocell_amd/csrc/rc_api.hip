@@ -904,7 +904,10 @@ static int cg_solve(rc_handle *h, bool cold) {
 	h->cgs.sweeps_total += std::max(s.iters[0], s.iters[1]);
 	h->cgs.last_iters[0] = s.iters[0];
 	h->cgs.last_iters[1] = s.iters[1];
-	for(int k = 0; k < 2; ++k) h->cgs.last_relres[k] = s.bb[k] > 0 ? std::sqrt(s.rn2[k] / s.bb[k]) : 0.;
+	for(int k = 0; k < 2; ++k) {
+		h->cgs.last_relres[k] = s.bb[k] > 0 ? std::sqrt(s.rn2[k] / s.bb[k]) : 0.;
+		h->cgs.worst_relres = std::max(h->cgs.worst_relres, h->cgs.last_relres[k]);
+	}
 	h->cgs.last_cold = cold ? 1 : 0;
 	h->cgs.last_solve_ms = ms;
 	h->cgs.total_solve_ms += ms;
