@@ -17,8 +17,11 @@ class OrcParams(C.Structure):
         ("zetaQ_self", C.c_double), ("zetaQ_inter", C.c_double), ("J_Q", C.c_double),
         ("friction", C.c_double), ("friction_cell", C.c_double), ("CGtolerance", C.c_double),
         ("wall_slip", C.c_double), ("passive_alpha", C.c_double), ("beta", C.c_double),
-        ("J0", C.c_double), ("dt", C.c_double), ("cg_max_iter", C.c_int),
+        ("J0", C.c_double), ("dt", C.c_double), ("cg_max_iter", C.c_int), ("model", C.c_int),
     ]
+
+
+MODELS = {"wetpolarmodel": 0, "wetmodel": 1}
 
 
 class OrcCellInfo(C.Structure):
@@ -31,6 +34,7 @@ class OrcCellInfo(C.Structure):
         ("area", C.c_double), ("area_old", C.c_double), ("sumF", C.c_double), ("sumF_old", C.c_double),
         ("perimeter", C.c_double), ("S00", C.c_double), ("S01", C.c_double),
         ("total_force", C.c_double * 2), ("total_force_repulsion", C.c_double * 2),
+        ("nemQ", C.c_double * 2), ("nemQ_old", C.c_double * 2),
     ]
 
 
@@ -93,6 +97,7 @@ def _lib(flavour: str = "strict") -> C.CDLL:
     L.orc_test_set_site.argtypes = [vp, ip, ip, dp]
     L.orc_test_set_rows.argtypes = [vp, ip, dp]
     L.orc_test_set_polarity.argtypes = [vp, ip, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.orc_test_set_director.argtypes = [vp, ip, C.c_double, C.c_double, C.c_double, C.c_double]
     L.orc_test_first_step.argtypes = [vp, ip]
     L.orc_test_forces.argtypes = [vp]
     _LIBS[flavour] = L
@@ -110,7 +115,8 @@ def params_from_keys(keys: Dict[str, object], cg_max_iter: int = 0) -> OrcParams
                      zetaQ_self=g("zetaQ_self"), zetaQ_inter=g("zetaQ_inter"), J_Q=g("J_Q"),
                      friction=g("friction"), friction_cell=g("friction_cell"), CGtolerance=g("CGtolerance", 1e-4),
                      wall_slip=g("wall_slip", 0.5), passive_alpha=g("passive_alpha", 1.0), beta=g("beta"),
-                     J0=g("J0", 1.0), dt=g("dt"), cg_max_iter=cg_max_iter)
+                     J0=g("J0", 1.0), dt=g("dt"), cg_max_iter=cg_max_iter,
+                     model=MODELS[str(keys.get("interaction_type", "wetpolarmodel")).lower()])
 
 
 class OracleSim:
@@ -194,6 +200,9 @@ class OracleSim:
 
     def set_polarity(self, i: int, theta: float, theta_old: float, tr=(0.0, 0.0)):
         self._check(self.L.orc_test_set_polarity(self.h, i, theta, theta_old, tr[0], tr[1]))
+
+    def set_director(self, i: int, n, n_old):
+        self._check(self.L.orc_test_set_director(self.h, i, n[0], n[1], n_old[0], n_old[1]))
 
     def first_step(self, store: bool):
         self._check(self.L.orc_test_first_step(self.h, 1 if store else 0))

@@ -12,6 +12,11 @@
  *   MultiPhaseField ring buffer / recentering / growth
  *       (src/Fields/MultiPhaseField.cpp:27-359)
  *   SquareBox periodic map (src/Boxes/SquareBox.cpp:21-82, BaseBox.cpp:29-50)
+ *   WetModel, the nematic variant (interaction_type = wetmodel):
+ *       computeGlobalSums, f_interaction, calc_internal_forces,
+ *       updateDirectedActiveForces and the solver restart rule
+ *       (src/Interactions/WetModel.cpp:194-233, 423-747), and the director
+ *       set up by MultiPhaseField::init (src/Fields/MultiPhaseField.cpp:68-98)
  *   Utils::gaussian (src/Utilities/Utils.h:210-234) on POSIX drand48
  *   Eigen::ConjugateGradient<SparseMatrix, Upper> with the default Jacobi
  *       preconditioner (Eigen 3, un-vendored; restated from its published
@@ -56,6 +61,8 @@ typedef struct ocell {
 	double tf[2], tr[2];
 	int run_clock;
 	double *phi, *phi_old, *dphi_old, *mu, *dx, *dy, *lap, *rep, *pressure;
+	double *fd;                /* freeEnergyDensity (WetModel) */
+	double nem[2], nem_old[2]; /* nemQ, nemQ_old (WetModel) */
 	int *nbr;                  /* 9*S, periodic in the patch */
 	int *map;                  /* map_sub_to_box */
 	double *cosx, *sinx, *cosy, *siny;
@@ -70,6 +77,8 @@ struct orc_sim {
 	ocell *c;
 	int *bnbr;                 /* 9*L*L box neighbours */
 	double *phi2, *sum_phi;
+	double *sq00, *sq01;       /* sumQ00, sumQ01: box sums of phi Q (WetModel) */
+	int nematic;               /* interaction_type = wetmodel */
 	int *store_size, *store_idx;
 	int *start;
 	int size_rows, size_rows_old, restart_solver;
@@ -195,6 +204,7 @@ static void cell_resizing(ocell *p, int oldS, int newS) {
 	p->lap = xrealloc_keep(p->lap, oldS, newS, sizeof(double));
 	p->rep = xrealloc_keep(p->rep, oldS, newS, sizeof(double));
 	p->pressure = xrealloc_keep(p->pressure, oldS, newS, sizeof(double));
+	p->fd = xrealloc_keep(p->fd, oldS, newS, sizeof(double));
 	p->map = xrealloc_keep(p->map, oldS, newS, sizeof(int));
 	free(p->nbr);
 	p->nbr = malloc(sizeof(int) * 9 * (size_t) newS);
@@ -211,7 +221,7 @@ static void cell_resizing(ocell *p, int oldS, int newS) {
 
 static void cell_free(ocell *p) {
 	free(p->phi); free(p->phi_old); free(p->dphi_old); free(p->mu);
-	free(p->dx); free(p->dy); free(p->lap); free(p->rep); free(p->pressure);
+	free(p->dx); free(p->dy); free(p->lap); free(p->rep); free(p->pressure); free(p->fd);
 	free(p->map); free(p->nbr);
 	free(p->cosx); free(p->sinx); free(p->cosy); free(p->siny);
 }
@@ -335,15 +345,19 @@ orc_sim *orc_create(int L, int N, const orc_params *prm, long seed) {
 	s->tolerance = prm->CGtolerance > 0 ? prm->CGtolerance : DBL_EPSILON;
 	s->c = calloc((size_t) N, sizeof(ocell));
 	s->start = calloc((size_t) N, sizeof(int));
+	s->nematic = prm->model == ORC_WETNEMATIC;
 	orc_rng_seed(&s->rng, seed);
 	for(int i = 0; i < N; i++) {             /* read_topology, WetPolarModel.cpp:69-75 */
 		s->c[i].index = i;
-		s->c[i].run_clock = (int) (2 * orc_rng_drand48(&s->rng) * prm->J_Q);
+		/* WetModel::read_topology (WetModel.cpp:67-81) draws nothing */
+		if(!s->nematic) s->c[i].run_clock = (int) (2 * orc_rng_drand48(&s->rng) * prm->J_Q);
 	}
 	s->bnbr = malloc(sizeof(int) * 9 * (size_t) L * L);
 	periodic_nbrs(L, L, s->bnbr);
 	s->phi2 = calloc((size_t) L * L, sizeof(double));
 	s->sum_phi = calloc((size_t) L * L, sizeof(double));
+	s->sq00 = calloc((size_t) L * L, sizeof(double));
+	s->sq01 = calloc((size_t) L * L, sizeof(double));
 	s->store_size = calloc((size_t) L * L, sizeof(int));
 	s->store_idx = calloc((size_t) L * L * STORE_MAX, sizeof(int));
 	return s;
@@ -352,7 +366,7 @@ orc_sim *orc_create(int L, int N, const orc_params *prm, long seed) {
 void orc_destroy(orc_sim *s) {
 	if(!s) return;
 	for(int i = 0; i < s->N; i++) cell_free(&s->c[i]);
-	free(s->c); free(s->start); free(s->bnbr); free(s->phi2); free(s->sum_phi);
+	free(s->c); free(s->start); free(s->bnbr); free(s->phi2); free(s->sum_phi); free(s->sq00); free(s->sq01);
 	free(s->store_size); free(s->store_idx);
 	free(s->vx); free(s->vy); free(s->fx); free(s->fy);
 	free(s);
@@ -369,6 +383,17 @@ int orc_set_cell(orc_sim *s, int i, int LX, int LY, double comx, double comy, in
 	cell_resizing(p, oldS, p->S);
 	p->S00 = 0.; p->S01 = 0.;
 	p->thetaQ = p->thetaQ_old = PI_F * (1 - 2 * orc_rng_drand48(&s->rng));
+	{   /* the director of this draw (MultiPhaseField.cpp:80-95); the reader
+	     * below replaces Q00 / Q01 but not nemQ (SimBackend.cpp:283-286) */
+		double q01, q00;
+		sincos(2 * p->thetaQ, &q01, &q00);
+		double nemQ_mod = sqrt(q00 * q00 + q01 * q01);
+		double nx = sqrt((1 + q00 / nemQ_mod) / 2);
+		double sgn = q01 > 0 ? 1 : q01 < 0 ? -1 : 0;
+		double ny = sgn * sqrt((1 - q00 / nemQ_mod) / 2);
+		p->nem[0] = p->nem_old[0] = nx;
+		p->nem[1] = p->nem_old[1] = ny;
+	}
 	p->area = 0; p->perimeter = 0; p->sumF = 0;
 	periodic_nbrs(LX, LY, p->nbr);
 	/* SimBackend.cpp:274-296 */
@@ -427,7 +452,12 @@ static void init_field_properties(orc_sim *s) {
 	}
 }
 
-static double f_interaction(orc_sim *s, ocell *p, int q) {
+/* The terms WetPolarModel::f_interaction (WetPolarModel.cpp:262-301) and
+ * WetModel::f_interaction (WetModel.cpp:488-600) share: Cahn-Hilliard, area
+ * constraint, repulsion and adhesion */
+typedef struct fterms { double ph, p2, CH, A, Rep, Adh; } fterms;
+
+static fterms free_energy_terms(orc_sim *s, ocell *p, int q) {
 	int k = p->map[q];
 	const int *bn = s->bnbr + 9 * k;
 	double *phi2 = s->phi2;
@@ -435,17 +465,25 @@ static double f_interaction(orc_sim *s, ocell *p, int q) {
 	double ph = p->phi[q];
 	double gamma = s->act.gamma, lambda = s->act.lambda, mu = s->act.mu, kappa = s->act.kappa,
 			omega = s->act.omega, a0 = s->a0;
-	double CH = gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / lambda - 2 * lambda * p->lap[q]);
-	double A = -4 * (mu / a0) * (1 - p->area / a0) * ph;
-	double Rep = 4 * (kappa / lambda) * ph * (phi2[k] - ph * ph);
+	fterms t;
+	t.ph = ph;
+	t.p2 = phi2[k];
+	t.CH = gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / lambda - 2 * lambda * p->lap[q]);
+	t.A = -4 * (mu / a0) * (1 - p->area / a0) * ph;
+	t.Rep = 4 * (kappa / lambda) * ph * (phi2[k] - ph * ph);
 	double lsquare = 2 * ph * p->lap[q] + 2 * (p->dx[q] * p->dx[q] + p->dy[q] * p->dy[q]);
 	double suppress = (laplacianSquare - lsquare) / sqrt(1 + (laplacianSquare - lsquare) * (laplacianSquare - lsquare));
-	double Adh = -4 * lambda * omega * suppress * ph;
+	t.Adh = -4 * lambda * omega * suppress * ph;
+	return t;
+}
+
+static double f_interaction(orc_sim *s, ocell *p, int q) {
+	fterms t = free_energy_terms(s, p, q);
 	double P = 4 * (s->beta / s->p0) * (1 - p->perimeter / s->p0) * p->lap[q];
-	double V = CH + A + Rep + Adh + P;
+	double V = t.CH + t.A + t.Rep + t.Adh + P;
 	p->mu[q] += V;
-	p->pressure[q] = Rep - CH - A;
-	p->rep[q] += Rep;
+	p->pressure[q] = t.Rep - t.CH - t.A;
+	p->rep[q] += t.Rep;
 	return V;
 }
 
@@ -465,6 +503,52 @@ static void calc_internal_forces(orc_sim *s, ocell *p, int q) {
 	s->fy[row] = f_passive_y * s->act.passive_alpha + fQy * s->zetaQ_self;
 	p->tf[0] += s->fx[row] * p->phi[q];
 	p->tf[1] += s->fy[row] * p->phi[q];
+}
+
+/* WetModel::f_interaction (WetModel.cpp:488-600): no perimeter term; the
+ * free-energy density of each term is accumulated for the passive stress */
+static double f_interaction_nem(orc_sim *s, ocell *p, int q) {
+	fterms t = free_energy_terms(s, p, q);
+	double gamma = s->act.gamma, lambda = s->act.lambda, mu = s->act.mu, kappa = s->act.kappa, a0 = s->a0;
+	double ph = t.ph;
+	double CH_density = (gamma / lambda) * 4 * ph * ph * (1 - ph) * (1 - ph)
+			+ gamma * lambda * (p->dx[q] * p->dx[q] + p->dy[q] * p->dy[q]);
+	double A_density = -mu / a0 * ph * ph;
+	double Rep_density = (kappa / lambda) * ph * ph * (t.p2 - ph * ph);
+	double V = t.CH + t.A + t.Rep + t.Adh;
+	p->mu[q] += V;
+	p->pressure[q] = t.Rep - t.CH - t.A;
+	p->fd[q] += CH_density + A_density + Rep_density;
+	return V;
+}
+
+/* WetModel::calc_internal_forces (WetModel.cpp:603-641): the passive force is
+ * minus the divergence of the stress (f - mu phi) I + grad phi (x) df/d(grad phi)
+ * with df/d(grad phi) = 2 gamma lambda grad phi (freeEnergyDensityGradient,
+ * :597-598); the active forces are the cell's own nematic force and the one of
+ * its neighbours from the box sums of phi Q */
+static void calc_internal_forces_nem(orc_sim *s, ocell *p, int q) {
+	const int *nb = p->nbr + 9 * q;
+	const int *bn = s->bnbr + 9 * p->map[q];
+	const double gl = 2 * s->act.gamma * s->act.lambda;
+	const double *fd = p->fd, *mu = p->mu, *ph = p->phi, *dx = p->dx, *dy = p->dy;
+#define GX(n) (gl * dx[n])
+#define GY(n) (gl * dy[n])
+	double f_passive_x = -0.5 * ((fd[nb[5]] - mu[nb[5]] * ph[nb[5]] + dx[nb[5]] * GX(nb[5]))
+			- (fd[nb[3]] - mu[nb[3]] * ph[nb[3]] + dx[nb[3]] * GX(nb[3])))
+			- 0.5 * ((dx[nb[7]] * GY(nb[7])) - (dx[nb[1]] * GY(nb[1])));
+	double f_passive_y = -0.5 * ((fd[nb[7]] - mu[nb[7]] * ph[nb[7]] + dy[nb[7]] * GY(nb[7]))
+			- (fd[nb[1]] - mu[nb[1]] * ph[nb[1]] + dy[nb[1]] * GY(nb[1])))
+			- 0.5 * ((dy[nb[5]] * GX(nb[5])) - (dy[nb[3]] * GX(nb[3])));
+#undef GX
+#undef GY
+	double fQ_self_x = -(p->Q00 * dx[q] + p->Q01 * dy[q]);
+	double fQ_self_y = -(p->Q01 * dx[q] - p->Q00 * dy[q]);
+	double fQ_inter_x = -(0.5 * (s->sq00[bn[5]] - s->sq00[bn[3]]) + 0.5 * (s->sq01[bn[7]] - s->sq01[bn[1]])) - fQ_self_x;
+	double fQ_inter_y = -(0.5 * (s->sq01[bn[5]] - s->sq01[bn[3]]) - 0.5 * (s->sq00[bn[7]] - s->sq00[bn[1]])) - fQ_self_y;
+	int row = q + s->start[p->index];
+	s->fx[row] = f_passive_x * s->act.passive_alpha + fQ_self_x * s->zetaQ_self + fQ_inter_x * s->zetaQ_inter;
+	s->fy[row] = f_passive_y * s->act.passive_alpha + fQ_self_y * s->zetaQ_self + fQ_inter_y * s->zetaQ_inter;
 }
 
 /* matrix-free form of the assembled operator (see orc_assemble_dense):
@@ -563,7 +647,9 @@ done:
 static int compute_forces(orc_sim *s) {
 	int LL = s->L * s->L;
 	memset(s->store_size, 0, sizeof(int) * LL);
-	if(s->restart_solver < 10 || s->warm_mode == 1) s->size_rows_old = s->size_rows;
+	/* cold solve after 10 warm ones (WetPolarModel.cpp:152-157), 100 for
+	 * WetModel (WetModel.cpp:196-201) */
+	if(s->restart_solver < (s->nematic ? 100 : 10) || s->warm_mode == 1) s->size_rows_old = s->size_rows;
 	else {
 		s->size_rows_old = 0;
 		s->restart_solver = 0;
@@ -577,13 +663,18 @@ static int compute_forces(orc_sim *s) {
 			int k = sub_index(s, p, q);
 			s->phi2[k] += p->phi[q] * p->phi[q];
 			s->sum_phi[k] += p->phi[q];
+			if(s->nematic) {           /* WetModel::computeGlobalSums (WetModel.cpp:423-431) */
+				s->sq00[k] += p->phi[q] * p->Q00;
+				s->sq01[k] += p->phi[q] * p->Q01;
+			}
 			p->map[q] = k;
 			p->dx[q] = derivX(p, q);
 			p->dy[q] = derivY(p, q);
 			p->lap[q] = laplacian(p, q);
-			p->perimeter += p->dx[q] * p->dx[q] + p->dy[q] * p->dy[q];
+			if(!s->nematic) p->perimeter += p->dx[q] * p->dx[q] + p->dy[q] * p->dy[q];
 			p->tf[0] = p->tf[1] = p->tr[0] = p->tr[1] = 0.;
 			p->mu[q] = 0.;
+			p->fd[q] = 0.;
 			if(s->store_size[k] >= STORE_MAX)
 				return set_err(s, "Too many field patches overlap: %d, %d, %d, %d", p->index, k, q, p->corner);
 			s->store_idx[s->store_size[k] + k * STORE_MAX] = q + s->start[i];
@@ -599,12 +690,13 @@ static int compute_forces(orc_sim *s) {
 	s->U = 0;
 	for(int i = 0; i < s->N; i++) {
 		ocell *p = &s->c[i];
-		for(int q = 0; q < p->S; q++) s->U += f_interaction(s, p, q);
+		for(int q = 0; q < p->S; q++) s->U += s->nematic ? f_interaction_nem(s, p, q) : f_interaction(s, p, q);
 	}
 	for(int i = 0; i < s->N; i++) {
 		ocell *p = &s->c[i];
 		for(int q = 0; q < p->S; q++) {
-			calc_internal_forces(s, p, q);
+			if(s->nematic) calc_internal_forces_nem(s, p, q);
+			else calc_internal_forces(s, p, q);
 			if(s->friction_cell == 0) {
 				int row = q + s->start[i];
 				s->vx[row] = s->fx[row] / s->friction;
@@ -658,6 +750,42 @@ static void update_directed_active_forces(orc_sim *s, ocell *p, int store) {
 	}
 }
 
+/* WetModel::updateDirectedActiveForces (WetModel.cpp:644-700): the director
+ * relaxes towards the long axis of the shape tensor; update_anchoring
+ * (:703-718) */
+static void update_directed_nem(orc_sim *s, ocell *p, int store) {
+	double dt = s->act.dt;
+	double S_mod = sqrt(p->S00 * p->S00 + p->S01 * p->S01);
+	if(store) {
+		p->nem_old[0] = p->nem[0];
+		p->nem_old[1] = p->nem[1];
+	}
+	if(S_mod > 0.00000001) {
+		double t = 0.5 * atan2(p->S01, p->S00);
+		double d1, d0;
+		sincos(t, &d1, &d0);
+		double sgn = (d0 * p->nem[0] + d1 * p->nem[1] > 0.0) ? 1.0 : -1.0;
+		p->nem[0] = p->nem_old[0] + dt * s->J_Q * (sgn * d0 - p->nem[0]);
+		p->nem[1] = p->nem_old[1] + dt * s->J_Q * (sgn * d1 - p->nem[1]);
+		p->Q00 = 0.5 * (p->nem[0] * p->nem[0] - p->nem[1] * p->nem[1]);
+		p->Q01 = p->nem[0] * p->nem[1];
+	}
+	if(s->act.anchoring) {
+		double walls_length = 8;
+		double dist1 = ((double) p->CoM_old[1] - walls_length);
+		double dist2 = ((double) s->L - walls_length) - (double) p->CoM_old[1];
+		double theta;
+		if(dist1 < dist2) theta = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) s->L - 2 * walls_length) / 2);
+		else theta = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) s->L - 2 * walls_length) / 2));
+		double n1, n0;
+		sincos(theta, &n1, &n0);
+		p->nem[0] = n0;
+		p->nem[1] = n1;
+		p->Q00 = 0.5 * (p->nem[0] * p->nem[0] - p->nem[1] * p->nem[1]);
+		p->Q01 = p->nem[0] * p->nem[1];
+	}
+}
+
 /* FD_CPUBackend::first_step (FD_CPUBackend.cpp:44-120) */
 static int first_step(orc_sim *s, int store) {
 	double J0 = s->act.J0, dt = s->act.dt;
@@ -694,6 +822,8 @@ static int first_step(orc_sim *s, int store) {
 			if(!store && phi < 0.09 && phi > 0.01) check_borders(p, q);
 			s->phi2[k] = 0;                  /* resetSums */
 			s->sum_phi[k] = 0;
+			s->sq00[k] = 0;
+			s->sq01[k] = 0;
 			p->S00 += -0.5 * (p->dx[q] * p->dx[q] - p->dy[q] * p->dy[q]);
 			p->S01 += -p->dx[q] * p->dy[q];
 		}
@@ -701,7 +831,8 @@ static int first_step(orc_sim *s, int store) {
 		p->CoM[1] = p->LY * (atan2(-temp[3] / p->sumF, -temp[2] / p->sumF) + PI_F) / (2 * PI_F);
 		if(!store)
 			if(set_positions(s, p)) return -1;
-		update_directed_active_forces(s, p, store);
+		if(s->nematic) update_directed_nem(s, p, store);
+		else update_directed_active_forces(s, p, store);
 	}
 	return 0;
 }
@@ -710,6 +841,8 @@ int orc_init(orc_sim *s) {
 	for(int k = 0; k < s->L * s->L; k++) {
 		s->phi2[k] = 0;
 		s->sum_phi[k] = 0;
+		s->sq00[k] = 0;
+		s->sq01[k] = 0;
 		s->store_size[k] = 0;
 	}
 	s->restart_solver = 0;
@@ -759,6 +892,8 @@ int orc_get_cell(orc_sim *s, int i, orc_cell_info *o) {
 	o->perimeter = p->perimeter; o->S00 = p->S00; o->S01 = p->S01;
 	o->total_force[0] = p->tf[0]; o->total_force[1] = p->tf[1];
 	o->total_force_repulsion[0] = p->tr[0]; o->total_force_repulsion[1] = p->tr[1];
+	o->nemQ[0] = p->nem[0]; o->nemQ[1] = p->nem[1];
+	o->nemQ_old[0] = p->nem_old[0]; o->nemQ_old[1] = p->nem_old[1];
 	return 0;
 }
 
@@ -776,6 +911,7 @@ int orc_get_site_array(orc_sim *s, int i, int which, double *out) {
 		case 6: out[q] = p->dphi_old[q]; break;
 		case 7: out[q] = p->rep[q]; break;
 		case 8: out[q] = (double) p->map[q]; break;
+		case 9: out[q] = p->fd[q]; break;
 		default: return set_err(s, "bad site array %d", which);
 		}
 	}
@@ -795,6 +931,8 @@ int orc_get_box(orc_sim *s, int which, double *out) {
 		if(which == 0) out[k] = s->phi2[k];
 		else if(which == 1) out[k] = s->sum_phi[k];
 		else if(which == 2) out[k] = s->store_size[k];
+		else if(which == 3) out[k] = s->sq00[k];
+		else if(which == 4) out[k] = s->sq01[k];
 		else return set_err(s, "bad box array %d", which);
 	}
 	return 0;
@@ -896,12 +1034,24 @@ int orc_test_set_polarity(orc_sim *s, int i, double thetaQ, double thetaQ_old, d
 	return 0;
 }
 
+int orc_test_set_director(orc_sim *s, int i, double n0, double n1, double n0_old, double n1_old) {
+	if(i < 0 || i >= s->N) return set_err(s, "cell index %d out of range", i);
+	ocell *p = &s->c[i];
+	p->nem[0] = n0;
+	p->nem[1] = n1;
+	p->nem_old[0] = n0_old;
+	p->nem_old[1] = n1_old;
+	return 0;
+}
+
 int orc_test_first_step(orc_sim *s, int store) { return first_step(s, store); }
 
 int orc_test_forces(orc_sim *s) {
 	for(int k = 0; k < s->L * s->L; k++) {
 		s->phi2[k] = 0;
 		s->sum_phi[k] = 0;
+		s->sq00[k] = 0;
+		s->sq01[k] = 0;
 	}
 	for(int i = 0; i < s->N; i++) s->c[i].perimeter = 0;
 	return compute_forces(s);

@@ -29,7 +29,12 @@ typedef struct orc_params {
 	double friction, friction_cell, CGtolerance, wall_slip, passive_alpha, beta;
 	double J0, dt;
 	int cg_max_iter;     /* <=0: Eigen's default 2*rows */
+	int model;           /* ORC_WETPOLAR (interaction_type = wetpolarmodel) or
+	                        ORC_WETNEMATIC (interaction_type = wetmodel) */
 } orc_params;
+
+#define ORC_WETPOLAR 0
+#define ORC_WETNEMATIC 1
 
 /* Per-cell scalar state (BaseField.h:122-225). */
 typedef struct orc_cell_info {
@@ -43,6 +48,7 @@ typedef struct orc_cell_info {
 	double Q00, Q01, thetaQ, thetaQ_old;
 	double area, area_old, sumF, sumF_old, perimeter, S00, S01;
 	double total_force[2], total_force_repulsion[2];
+	double nemQ[2], nemQ_old[2];  /* nematic director (WetModel) */
 } orc_cell_info;
 
 typedef struct orc_cg_stats {
@@ -75,11 +81,13 @@ int orc_num_cells(orc_sim *s);
 int orc_size_rows(orc_sim *s);
 int orc_get_cell(orc_sim *s, int i, orc_cell_info *out);
 /* which: 0 phi, 1 freeEnergy (mu), 2 fieldDX, 3 fieldDY, 4 laplacianPhi,
- *        5 phi_old, 6 dphi_old, 7 Repulsion, 8 map_sub_to_box (as double) */
+ *        5 phi_old, 6 dphi_old, 7 Repulsion, 8 map_sub_to_box (as double),
+ *        9 freeEnergyDensity (WetModel) */
 int orc_get_site_array(orc_sim *s, int i, int which, double *out);
 /* velocity / force rows in field_start_index order. which: 0 vx,1 vy,2 fx,3 fy */
 int orc_get_rows(orc_sim *s, int which, double *out);
-int orc_get_box(orc_sim *s, int which, double *out); /* 0 phi2, 1 sum_phi, 2 cover count */
+/* 0 phi2, 1 sum_phi, 2 cover count, 3 sumQ00, 4 sumQ01 (WetModel) */
+int orc_get_box(orc_sim *s, int which, double *out);
 void orc_get_cg_stats(orc_sim *s, orc_cg_stats *out);
 /* 0: reference rule (cold every 11th solve / size change); 1: always warm */
 void orc_set_warm_mode(orc_sim *s, int mode);
@@ -121,6 +129,8 @@ int orc_test_set_positions(int Lbox, int *ints, double *dbl, double *phi, int ph
 int orc_test_set_site(orc_sim *s, int i, int which, const double *in);
 int orc_test_set_rows(orc_sim *s, int which, const double *in);
 int orc_test_set_polarity(orc_sim *s, int i, double thetaQ, double thetaQ_old, double tr0, double tr1);
+/* nematic director and its stored value (WetModel) */
+int orc_test_set_director(orc_sim *s, int i, double n0, double n1, double n0_old, double n1_old);
 int orc_test_first_step(orc_sim *s, int store);
 int orc_test_forces(orc_sim *s);
 

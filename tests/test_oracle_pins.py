@@ -332,7 +332,7 @@ MUTANTS = [
      "int x = (((q - (q / p->LX) * p->LX)) % p->LX);", ["com_cosine_moment"]),
     ("shape_sign", "p->S00 += -0.5 * (dx * dx - dy * dy);", "p->S00 += 0.5 * (dx * dx - dy * dy);", ["shape_tensor"]),
     ("shape_cross", "p->S01 += -dx * dy;", "p->S01 += -dx * dx;", ["shape_tensor"]),
-    ("rep_no_self", "(phi2[k] - ph * ph)", "(phi2[k])", ["repulsion"]),
+    ("rep_no_self", "ph * (phi2[k] - ph * ph)", "ph * (phi2[k])", ["repulsion"]),
     ("rep_coefficient", "4 * (kappa / lambda)", "4 * (kappa * lambda)", ["repulsion"]),
     ("area_sign", "(1 - p->area / a0)", "(1 + p->area / a0)", ["heun_flat", "free_energy", "passive_force"]),
     ("ch_lap", "- 2 * lambda * p->lap[q]", "- lambda * p->lap[q]", ["free_energy"]),
@@ -343,7 +343,7 @@ MUTANTS = [
      "double f_passive_x = (-1) * (p->mu[nb[5]] - p->mu[nb[3]]);", ["passive_force"]),
     ("passive_sign", "double f_passive_x = (-1) * 0.5 * (p->mu[nb[5]] - p->mu[nb[3]]);",
      "double f_passive_x = 0.5 * (p->mu[nb[5]] - p->mu[nb[3]]);", ["passive_force"]),
-    ("adhesion_sign", "double Adh = -4 * lambda * omega", "double Adh = 4 * lambda * omega", ["adhesion"]),
+    ("adhesion_sign", "t.Adh = -4 * lambda * omega", "t.Adh = 4 * lambda * omega", ["adhesion"]),
     ("adhesion_lsquare", "double lsquare = 2 * ph * p->lap[q] + 2 *", "double lsquare = ph * p->lap[q] + 2 *",
      ["adhesion"]),
     ("adhesion_box_stencil", "phi2[bn[5]] + phi2[bn[7]] + phi2[bn[3]] + phi2[bn[1]] - 4. * phi2[k]",
@@ -360,20 +360,23 @@ MUTANTS = [
 ]
 
 
-@pytest.fixture(scope="module")
-def oracle_source():
-    return open(os.path.join(ROOT, "oracle", "rc_oracle.c")).read()
+def build_planted(tmp_path, old, new, label):
+    """A copy of the oracle with `old` (which must occur once) replaced by
+    `new`, built on the CPU into tmp_path; returns the library path."""
+    source = open(os.path.join(ROOT, "oracle", "rc_oracle.c")).read()
+    assert source.count(old) == 1, "the text to mutate must occur once: %r" % old
+    src = tmp_path / "rc_oracle.c"
+    src.write_text(source.replace(old, new))
+    lib = str(tmp_path / ("liborc_%s.so" % label))
+    subprocess.run(["gcc", "-std=c11", "-O1", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+                    "-I" + os.path.join(ROOT, "oracle"), "-o", lib, str(src), "-lm"], check=True)
+    return lib
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc is needed to build the planted-mistake oracles")
 @pytest.mark.parametrize("label,old,new,pins", MUTANTS, ids=[m[0] for m in MUTANTS])
-def test_planted_mistake_is_caught(tmp_path, oracle_source, label, old, new, pins):
-    assert oracle_source.count(old) == 1, "the text to mutate must occur once: %r" % old
-    src = tmp_path / "rc_oracle.c"
-    src.write_text(oracle_source.replace(old, new))
-    lib = str(tmp_path / ("liborc_%s.so" % label))
-    subprocess.run(["gcc", "-std=c11", "-O1", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
-                    "-I" + os.path.join(ROOT, "oracle"), "-o", lib, str(src), "-lm"], check=True)
+def test_planted_mistake_is_caught(tmp_path, label, old, new, pins):
+    lib = build_planted(tmp_path, old, new, label)
     for name in pins:
         with pytest.raises(AssertionError):
             PINS[name](flavour=lib)
