@@ -70,7 +70,14 @@ typedef struct rc_model_params {
 	double friction, friction_cell, CGtolerance, wall_slip, passive_alpha, beta;
 	double J0, dt;
 	int cg_max_iter;      /* <= 0: Eigen's default of 2 * rows */
+	int interaction;      /* RC_INTERACTION_WETPOLAR (interaction_type = wetpolarmodel) or
+	                         RC_INTERACTION_WETNEMATIC (interaction_type = wetmodel,
+	                         src/Interactions/WetModel.cpp; beta is not one of its keys and
+	                         is ignored); every box of a handle has the same one */
 } rc_model_params;
+
+#define RC_INTERACTION_WETPOLAR 0
+#define RC_INTERACTION_WETNEMATIC 1
 
 /* One periodic square box (SquareBox, src/Boxes/SquareBox.cpp:21-29) with
  * its own cells and parameters.  Several boxes = independent replicas
@@ -92,6 +99,9 @@ typedef struct rc_cell {
 	double Q00, Q01, thetaQ, thetaQ_old;
 	double area, area_old, sumF, sumF_old, perimeter, S00, S01;
 	double total_force_repulsion[2];
+	double nemQ[2], nemQ_old[2];     /* nematic director and its stored value (WetModel); the
+	                                    reference sets them in MultiPhaseField::init from the
+	                                    cell's drand48 draw (MultiPhaseField.cpp:79-95) */
 } rc_cell;
 
 typedef struct rc_cg_stats_t {

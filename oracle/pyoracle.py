@@ -110,11 +110,15 @@ def _dp(a: np.ndarray):
 
 def params_from_keys(keys: Dict[str, object], cg_max_iter: int = 0) -> OrcParams:
     g = lambda k, d=None: float(keys[k]) if k in keys else d  # noqa: E731
-    return OrcParams(gamma=g("gamma", 0.06), lambda_=g("lambda", 2.5), omega=g("omega", 0.004), mu=g("mu", 20.0),
+    nem = str(keys.get("interaction_type", "wetpolarmodel")).lower() == "wetmodel"
+    # constructor defaults: WetPolarModel.cpp:4-21, WetModel.cpp:4-20
+    return OrcParams(gamma=g("gamma", 0.01 if nem else 0.06), lambda_=g("lambda", 2.5), omega=g("omega", 0.004),
+                     mu=g("mu", 3.0 if nem else 20.0),
                      kappa=g("kappa", 0.1), R=int(float(keys["R"])), anchoring=int(float(keys.get("anchoring", 0))),
                      zetaQ_self=g("zetaQ_self"), zetaQ_inter=g("zetaQ_inter"), J_Q=g("J_Q"),
                      friction=g("friction"), friction_cell=g("friction_cell"), CGtolerance=g("CGtolerance", 1e-4),
-                     wall_slip=g("wall_slip", 0.5), passive_alpha=g("passive_alpha", 1.0), beta=g("beta"),
+                     wall_slip=g("wall_slip", 0.5), passive_alpha=g("passive_alpha", 1.0),
+                     beta=g("beta", 0.0) if nem else g("beta"),
                      J0=g("J0", 1.0), dt=g("dt"), cg_max_iter=cg_max_iter,
                      model=MODELS[str(keys.get("interaction_type", "wetpolarmodel")).lower()])
 

@@ -45,8 +45,11 @@ class rc_model_params(C.Structure):
         ("zetaQ_self", C.c_double), ("zetaQ_inter", C.c_double), ("J_Q", C.c_double),
         ("friction", C.c_double), ("friction_cell", C.c_double), ("CGtolerance", C.c_double),
         ("wall_slip", C.c_double), ("passive_alpha", C.c_double), ("beta", C.c_double),
-        ("J0", C.c_double), ("dt", C.c_double), ("cg_max_iter", C.c_int),
+        ("J0", C.c_double), ("dt", C.c_double), ("cg_max_iter", C.c_int), ("interaction", C.c_int),
     ]
+
+
+RC_INTERACTION = {"wetpolarmodel": 0, "wetmodel": 1}
 
 
 class rc_box_desc(C.Structure):
@@ -61,7 +64,7 @@ class rc_cell(C.Structure):
         ("Q00", C.c_double), ("Q01", C.c_double), ("thetaQ", C.c_double), ("thetaQ_old", C.c_double),
         ("area", C.c_double), ("area_old", C.c_double), ("sumF", C.c_double), ("sumF_old", C.c_double),
         ("perimeter", C.c_double), ("S00", C.c_double), ("S01", C.c_double),
-        ("total_force_repulsion", C.c_double * 2),
+        ("total_force_repulsion", C.c_double * 2), ("nemQ", C.c_double * 2), ("nemQ_old", C.c_double * 2),
     ]
 
 

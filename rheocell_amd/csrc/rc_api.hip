@@ -52,6 +52,10 @@ struct rc_handle {
 	int max_S = 0, max_W = 0;
 	size_t lds_stage = 0, lds_cg = 0;
 	bool wet = false, jq = false, win = false;   // win: some omega != 0 (adhesion reads the box window)
+	bool nematic = false;         // interaction_type = wetmodel (WetModel.cpp)
+	// split schedule: the force evaluation is its own launch sequence and keeps
+	// the velocity rows (the wet model, and every nematic box)
+	bool split = false;
 
 	// device memory
 	hipStream_t stream = nullptr;
@@ -64,6 +68,7 @@ struct rc_handle {
 	double2 *d_v = nullptr, *d_f = nullptr, *d_r = nullptr, *d_p = nullptr;
 	double *d_trig = nullptr;
 	double *d_phi2 = nullptr, *d_diag = nullptr;
+	double *d_sq00 = nullptr, *d_sq01 = nullptr;   // nematic: box sums of phi Q00, phi Q01
 	int *d_cover = nullptr, *d_tile_cells = nullptr, *d_tile_count = nullptr;
 	int4 *d_tile_geo = nullptr;
 	double2 *d_sp = nullptr, *d_p2 = nullptr, *d_n4 = nullptr;
@@ -255,6 +260,8 @@ static DevParams make_params(const rc_model_params &m, bool active) {
 		p.friction_cell = 0.;
 		p.beta = 0.;
 	}
+	p.nematic = m.interaction == RC_INTERACTION_WETNEMATIC ? 1 : 0;
+	if(p.nematic) p.beta = 0.;   // WetModel has no perimeter term
 	p.friction_rcp = exact_rcp(p.friction);
 	p.pcoef = 4 * (p.beta / p.p0);
 	return p;
@@ -326,8 +333,17 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		cells += d.ncells;
 		tiles += B.ntx * B.nty;
 		sites += (long long) d.L * d.L;
+		if(d.params.interaction != RC_INTERACTION_WETPOLAR && d.params.interaction != RC_INTERACTION_WETNEMATIC) {
+			*out = h;
+			return fail(h, RC_ERR_ARG, "box %d: unknown interaction %d", b, d.params.interaction);
+		}
+		if(d.params.interaction != boxes[0].params.interaction) {
+			*out = h;
+			return fail(h, RC_ERR_ARG, "all boxes of a handle must share the interaction");
+		}
+		if(d.params.interaction == RC_INTERACTION_WETNEMATIC) h->nematic = true;
 		if(d.params.friction_cell != 0) h->wet = true;
-		if(d.params.J_Q != 0) h->jq = true;
+		if(d.params.J_Q != 0 && d.params.interaction == RC_INTERACTION_WETPOLAR) h->jq = true;
 		if(d.params.omega != 0) h->win = true;
 	}
 	if(h->wet && nboxes != 1) {
@@ -339,6 +355,7 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 			*out = h;
 			return fail(h, RC_ERR_ARG, "all boxes of a handle must share the dry/wet model");
 		}
+	h->split = h->wet || h->nematic;
 	h->ncells = cells;
 	h->ntiles = tiles;
 	h->box_sites = sites;
@@ -369,6 +386,10 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 	HIPCHK(h, hipMemcpy(h->d_params, h->dparams.data(), sizeof(DevParams) * 2 * nboxes, hipMemcpyHostToDevice));
 	HIPCHK(h, hipMemset(h->d_st, 0, sizeof(DevStatus)));
 	HIPCHK(h, hipMemset(h->d_phi2, 0, sizeof(double) * sites));
+	if(h->nematic) {
+		HIPCHK(h, dalloc(&h->d_sq00, sites));
+		HIPCHK(h, dalloc(&h->d_sq01, sites));
+	}
 	if(h->wet) {
 		HIPCHK(h, dalloc(&h->d_diag, sites));
 		HIPCHK(h, dalloc(&h->d_sp, sites));
@@ -389,7 +410,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_boxes); dfree(h->d_tile_box); dfree(h->d_cells); dfree(h->d_geo); dfree(h->d_params);
 	dfree(h->d_phiA); dfree(h->d_phiB); dfree(h->d_dphi); dfree(h->d_rep);
 	dfree(h->d_v); dfree(h->d_f); dfree(h->d_r); dfree(h->d_p); dfree(h->d_trig);
-	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_geo); dfree(h->d_tile_count);
+	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_sq00); dfree(h->d_sq01); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_geo); dfree(h->d_tile_count);
 	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
 	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xprev); dfree(h->d_rprev);
@@ -479,6 +500,10 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			d.S01 = c.S01;
 			d.trx = c.total_force_repulsion[0];
 			d.try_ = c.total_force_repulsion[1];
+			d.nem0 = c.nemQ[0];
+			d.nem1 = c.nemQ[1];
+			d.nem_old0 = c.nemQ_old[0];
+			d.nem_old1 = c.nemQ_old[1];
 			rows += d.S;
 			maxS = std::max(maxS, d.S);
 			maxW = std::max(maxW, (d.LX + 2) * (d.LY + 2));
@@ -499,9 +524,9 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 		if(h->jq) HIPCHK(h, dalloc(&h->d_rep, rows));
 		if(h->wet) {
 			HIPCHK(h, dalloc(&h->d_f, rows));
-			HIPCHK(h, dalloc(&h->d_r, rows));
 			HIPCHK(h, dalloc(&h->d_p, rows));
 		}
+		if(h->split) HIPCHK(h, dalloc(&h->d_r, rows));   // CG residual; scratch of materialize
 		h->rows = rows;
 		if(h->d_xprev) {
 			dfree(h->d_xprev);
@@ -536,6 +561,9 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 	h->max_W = maxW;
 	if(std::max(maxS, maxW) >= (1 << 16))
 		return fail(h, RC_ERR_ARG, "a patch of %d sites is too large", maxS);
+	if(h->nematic && nem_lds_bytes(maxS) > NEM_LDS_MAX)
+		return fail(h, RC_ERR_ARG, "a patch of %d sites is too large for the nematic force kernel (at most %d)", maxS,
+				(int) (NEM_LDS_MAX / nem_lds_bytes(1)));
 	{
 		// group the cells by the first stage shape (table order) that fits their patch
 		int shapes[16][2];
@@ -916,19 +944,63 @@ static int cg_solve(rc_handle *h, bool cold) {
 	return RC_OK;
 }
 
+// WetModel::begin_energy_computation(fields) (WetModel.cpp:194-233) up to the
+// solve: the box sums of phi Q (the gather of phi^2 has run), then the forces
+// of every cell (k_nem_force): the CG right-hand side, or the dry velocity
+static int nem_forces(rc_handle *h, const double *phi) {
+	FieldArgs f{};
+	f.boxes = h->d_boxes;
+	f.tile_box = h->d_tile_box;
+	f.tile_geo = h->d_tile_geo;
+	f.tile_count = h->d_tile_count;
+	f.tile_cells = h->d_tile_cells;
+	f.cells = h->d_cells;
+	f.phi = phi;
+	f.which = 2;
+	f.tile0 = 0;
+	f.global = 1;
+	f.fx = h->d_sq00;
+	f.fy = h->d_sq01;
+	HIPCHK(h, launch_box_field(f, h->ntiles, h->stream));
+	NemArgs n{};
+	n.boxes = h->d_boxes;
+	n.cells = h->d_cells;
+	n.params = h->d_params;
+	n.set = h->cur_set;
+	n.phi = phi;
+	n.phi2 = h->d_phi2;
+	n.sq00 = h->d_sq00;
+	n.sq01 = h->d_sq01;
+	n.frows = h->d_f;
+	n.vrows = h->d_v;
+	n.st = h->d_st;
+	TimeScope ts(h, RC_T_STAGE_CF);
+	HIPCHK(h, launch_nem_force(n, h->ncells, nem_lds_bytes(h->max_S), h->stream));
+	return RC_OK;
+}
+
 // WetPolarModel::begin_energy_computation(fields) on the wet path: box sums,
 // right-hand side (or dry velocity) and, for the wet solve, the CG.
+static int nem_forces(rc_handle *h, const double *phi);
+
 static int force_eval(rc_handle *h, double *phi, bool pure) {
 	int rc;
 	if((rc = run_gather(h, phi))) return rc;
-	StageArgs a = stage_args(h);
-	a.cf_set = h->cur_set;
-	a.fs_set = h->cur_set;
-	a.phi_c = phi;
-	a.pure = pure ? 1 : 0;
-	if((rc = run_stage(h, MODE_CF, a))) return rc;
-	// restart bookkeeping of WetPolarModel.cpp:151-157
-	if(h->restart_solver < 10 || h->warm_mode != RC_WARM_REFERENCE) h->size_rows_old = h->size_rows;
+	if(h->nematic) {
+		if((rc = nem_forces(h, phi))) return rc;
+	}
+	else {
+		StageArgs a = stage_args(h);
+		a.cf_set = h->cur_set;
+		a.fs_set = h->cur_set;
+		a.phi_c = phi;
+		a.pure = pure ? 1 : 0;
+		if((rc = run_stage(h, MODE_CF, a))) return rc;
+	}
+	// restart bookkeeping of WetPolarModel.cpp:151-157: a cold solve after 10
+	// warm ones, after 100 in WetModel (WetModel.cpp:196-201)
+	const int restart_after = h->nematic ? 100 : 10;
+	if(h->restart_solver < restart_after || h->warm_mode != RC_WARM_REFERENCE) h->size_rows_old = h->size_rows;
 	else {
 		h->size_rows_old = 0;
 		h->restart_solver = 0;
@@ -967,7 +1039,7 @@ extern "C" int rc_init_forces(rc_handle *h) {
 	h->restart_solver = 0;
 	h->size_rows = 0;
 	h->size_rows_old = 0;
-	if(h->wet) {
+	if(h->split) {
 		int rc = force_eval(h, h->d_phiA, false);
 		if(rc) return rc;
 	}
@@ -1016,8 +1088,10 @@ static int step_dry(rc_handle *h, const double *gauss_dev, int step) {
 	return RC_OK;
 }
 
-// returns RC_OK, an error, or -1 when the device halted (status flag set)
-static int step_wet(rc_handle *h, const double *gauss_dev, int step) {
+// the split schedule (the wet model, every nematic box): field update, then the
+// force evaluation, per stage.  Returns RC_OK, an error, or -1 when the device
+// halted (status flag set)
+static int step_split(rc_handle *h, const double *gauss_dev, int step) {
 	int rc;
 	StageArgs a = stage_args(h);
 	a.cf_set = h->cf_set;
@@ -1076,6 +1150,10 @@ static void to_rc_cell(const DevCell &d, rc_cell &c) {
 	c.S01 = d.S01;
 	c.total_force_repulsion[0] = d.trx;
 	c.total_force_repulsion[1] = d.try_;
+	c.nemQ[0] = d.nem0;
+	c.nemQ[1] = d.nem1;
+	c.nemQ_old[0] = d.nem_old0;
+	c.nemQ_old[1] = d.nem_old1;
 }
 
 static int download_raw(rc_handle *h, std::vector<rc_cell> &cells, std::vector<double> &phi, std::vector<double> *rep) {
@@ -1263,7 +1341,7 @@ static int autotune_stage(rc_handle *h) {
 			a.npt = g.valid[i].second;
 			a.cell_list = h->groups.size() > 1 ? g.d_cells : nullptr;
 			const int n = (int) g.cells.size();
-			const int mode = h->wet ? MODE_FS : MODE_FUSED;
+			const int mode = h->split ? MODE_FS : MODE_FUSED;
 			const size_t lds = stage_lds_bytes(g.mx, g.my, a.npt, h->jq, h->win);
 			HIPCHK(h, launch_stage(mode, a, n, lds, h->stream));
 			HIPCHK(h, hipEventRecord(e0, h->stream));
@@ -1430,7 +1508,7 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 			int window = 0;
 			for(; s < ns; ++s) {
 				// graph replay of a whole window (dry model, steady parameter set, no timing)
-				if(h->use_graphs && !h->wet && !h->timing && !grown && ns - s >= GRAPH_STEPS && h->cf_set == h->cur_set &&
+				if(h->use_graphs && !h->split && !h->timing && !grown && ns - s >= GRAPH_STEPS && h->cf_set == h->cur_set &&
 						window < 64) {
 					if(!h->graph_valid || h->graph_gauss != (gauss != nullptr) || h->graph_set != h->cur_set) {
 						HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1451,7 +1529,7 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 					continue;
 				}
 				const double *g = gauss ? h->d_gauss + (size_t) s * h->ncells : nullptr;
-				rc = h->wet ? step_wet(h, g, s) : step_dry(h, g, s);
+				rc = h->split ? step_split(h, g, s) : step_dry(h, g, s);
 				h->eager_steps += 1;
 				if(rc == -1) {
 					halted = true;
@@ -1471,7 +1549,7 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 					h->growths += 1;
 					h->growth_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
 				}
-				if(h->wet) {
+				if(h->split) {
 					if((rc = force_eval(h, h->d_phiA, false))) return rc;
 				}
 				else
@@ -1500,7 +1578,7 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 // the force evaluation of the current state without side effects (dry model)
 static int materialize(rc_handle *h, bool site_outputs) {
 	if(!h->initialized) return fail(h, RC_ERR_STATE, "no force evaluation before rc_init_forces");
-	if(h->wet && !site_outputs) return RC_OK;
+	if(h->split && !site_outputs) return RC_OK;
 	if(site_outputs) {
 		for(auto &p : h->d_out)
 			if(!p) HIPCHK(h, dalloc(&p, h->rows));
@@ -1512,8 +1590,10 @@ static int materialize(rc_handle *h, bool site_outputs) {
 	a.fs_set = h->cf_set;
 	a.phi_c = h->d_phiA;
 	a.pure = 1;
-	if(h->wet) {
-		// keep the solved velocity: divert the force rows to scratch
+	if(h->split) {
+		// keep the evaluated velocity: divert the force rows to scratch
+		// (mu, dx, dy and the Laplacian of the nematic model are the polar
+		// model's with beta = 0, which this evaluation takes)
 		a.frows = h->d_r;
 		a.vrows = h->d_r;
 	}
@@ -1531,7 +1611,7 @@ extern "C" int rc_download_cells(rc_handle *h, rc_cell *cells, double *phi) {
 	if(!h) return RC_ERR_ARG;
 	if(!h->uploaded) return fail(h, RC_ERR_STATE, "no cells uploaded");
 	int rc;
-	if(h->initialized && !h->wet)
+	if(h->initialized && !h->split)
 		if((rc = materialize(h, false))) return rc;
 	std::vector<rc_cell> c;
 	std::vector<double> p;
@@ -1546,7 +1626,7 @@ extern "C" int rc_get_rows(rc_handle *h, int which, double *out) {
 	if(!out || which < 0 || which > 3) return fail(h, RC_ERR_ARG, "bad rows request %d", which);
 	if(which >= 2 && !h->wet) return fail(h, RC_ERR_ARG, "force rows exist only for the wet model");
 	int rc;
-	if(!h->wet && (rc = materialize(h, false))) return rc;
+	if(!h->split && (rc = materialize(h, false))) return rc;
 	std::vector<double2> tmp(h->rows);
 	const double2 *src = which < 2 ? h->d_v : h->d_f;
 	HIPCHK(h, hipMemcpyAsync(tmp.data(), src, sizeof(double2) * h->rows, hipMemcpyDeviceToHost, h->stream));

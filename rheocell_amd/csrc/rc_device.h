@@ -37,7 +37,8 @@ struct DevParams {
 	double zetaQ_self, zetaQ_inter, J_Q, friction, friction_cell, J0, dt, sqrt_dt;
 	double lambda_rcp, friction_rcp;   // exact 1/lambda, 1/friction when a power of two, else 0
 	double mu_a0, rcoef, pcoef;        // mu / a0, 4 (kappa / lambda), 4 (beta / p0), formed on the host
-	int anchoring, pad;
+	int anchoring;
+	int nematic;                       // interaction_type = wetmodel (WetModel.cpp)
 };
 
 struct DevCell {
@@ -50,6 +51,7 @@ struct DevCell {
 	double Q00, Q01, thetaQ, thetaQ_old;
 	double area, area_old, sumF, sumF_old, perimeter, S00, S01;
 	double trx, try_;
+	double nem0, nem1, nem_old0, nem_old1;   // nematic director nemQ, nemQ_old (WetModel)
 };
 
 struct DevBox {

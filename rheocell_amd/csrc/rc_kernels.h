@@ -107,9 +107,10 @@ struct FieldArgs {
 	const DevCell *cells;
 	const double *phi;
 	const double2 *v;
-	int which;
+	int which;                   // 0 velocity rows, 1 the cell's (S00, S01), 2 the cell's (Q00, Q01)
 	int tile0;
-	double *fx, *fy;             // L*L sums, box-local site index x + L y
+	int global;                  // 1: outputs indexed by the box-array site (box_off + x + L y)
+	double *fx, *fy;             // L*L sums, box-local site index x + L y (global: every box)
 };
 hipError_t launch_box_field(const FieldArgs &a, int ntiles, hipStream_t s);
 // size_grid x size_grid periodic box average of fx, fy (L*L each) into cx, cy
@@ -128,6 +129,25 @@ inline size_t stage_lds_bytes(int LX, int LY, int npt, bool jq, bool win) {
 	return b;
 }
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds_bytes, hipStream_t s);
+
+// WetModel::f_interaction + calc_internal_forces (WetModel.cpp:488-641) of
+// every cell, one workgroup per cell: the stress-divergence passive force and
+// the self / inter-cell nematic forces from the box sums of phi Q; writes the
+// force rows (wet) or v = f / friction (dry).  LDS: nem_lds_bytes(max sites).
+struct NemArgs {
+	const DevBox *boxes;
+	const DevCell *cells;
+	const DevParams *params;     // params[2*box + set]
+	int set;
+	const double *phi;
+	const double *phi2, *sq00, *sq01;
+	double2 *frows;              // wet: the CG right-hand side
+	double2 *vrows;              // dry: the velocity
+	DevStatus *st;
+};
+constexpr size_t NEM_LDS_MAX = 160 * 1024 - 1024;
+inline size_t nem_lds_bytes(int max_sites) { return 32 * (size_t) max_sites; }
+hipError_t launch_nem_force(const NemArgs &a, int ncells, size_t lds_bytes, hipStream_t s);
 hipError_t launch_cg_diag(const CGArgs &a, int nbox_sites, hipStream_t s);
 hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s);
 // mode 0: N4 of sp; 1: N4 and the p.Ap partials; 2: N4 of the network output (learned stage)

@@ -312,7 +312,9 @@ hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s) {
 // Observable box fields (VelocityField::calc_field, VelocityField.cpp:69-78;
 // ShapeField::calc_field, ShapeField.cpp:69-78): f(k) += w * phi over the
 // fields in index order, so one sum per site in cell order reproduces the
-// reference's additions exactly.  Output only: no effect on the simulation.
+// reference's additions exactly.  With which = 2 the same gather gives the
+// nematic model's box sums sumQ00 / sumQ01 (WetModel::computeGlobalSums,
+// WetModel.cpp:423-431: phi Q added in cell order).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_box_field(FieldArgs a) {
 	const int tile = a.tile0 + xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
@@ -339,7 +341,8 @@ __global__ __launch_bounds__(256) void k_box_field(FieldArgs a) {
 			if(a.which == 0) w[q] = a.v[row[q]];
 			else {
 				const int ci = a.tile_cells[tile * MAXT + base + q];
-				w[q] = make_double2(a.cells[ci].S00, a.cells[ci].S01);
+				w[q] = a.which == 1 ? make_double2(a.cells[ci].S00, a.cells[ci].S01)
+						: make_double2(a.cells[ci].Q00, a.cells[ci].Q01);
 			}
 			ph[q] = a.phi[row[q]];
 		}
@@ -350,7 +353,7 @@ __global__ __launch_bounds__(256) void k_box_field(FieldArgs a) {
 			fy += w[q].y * ph[q];
 		}
 	}
-	const int k = (y0 + sy) * L + x0 + sx;
+	const int k = (a.global ? B.box_off : 0) + (y0 + sy) * L + x0 + sx;
 	a.fx[k] = fx;
 	a.fy[k] = fy;
 }
@@ -827,7 +830,8 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 
 	if(MODE == MODE_CF) {
 		if(a.pure) {
-			if(tid == 0) a.cells[ci].perimeter = perimeter;
+			// the nematic model keeps no perimeter (WetModel::computeGlobalSums)
+			if(tid == 0 && !P.nematic) a.cells[ci].perimeter = perimeter;
 			return;
 		}
 		double tr[2] = { red[8], red[9] };
@@ -875,8 +879,24 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	const double try_ = MODE == MODE_FUSED ? (do_rep ? try0 : 0.) : C.try_;
 	double val = 0.;
 	double th_old = C.thetaQ_old, th = 0.;
+	const bool nematic = MODE == MODE_FS && Q.nematic;
 	if(tid == 0) val = LX * (atan2(-sxs / sumF, -sxc / sumF) + PI_F) / (2 * PI_F);
 	else if(tid == 1) val = LY * (atan2(-sys / sumF, -syc / sumF) + PI_F) / (2 * PI_F);
+	else if(nematic) {
+		// WetModel::updateDirectedActiveForces (WetModel.cpp:657-670): the unit
+		// vector of the shape tensor's long axis, or the anchoring direction
+		// (update_anchoring, :703-716), which then replaces the director
+		double arg = 0.5 * atan2(S01, S00);
+		if(Q.anchoring) {
+			const double walls_length = 8;
+			const double com_old_y = store ? C.CoMy : C.CoM_oldy;
+			const double dist1 = (com_old_y - walls_length);
+			const double dist2 = ((double) L - walls_length) - com_old_y;
+			if(dist1 < dist2) arg = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
+			else arg = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) L - 2 * walls_length) / 2));
+		}
+		val = tid == 2 ? cos(arg) : sin(arg);
+	}
 	else {
 		// WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:365-396)
 		if(store) {
@@ -960,6 +980,30 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	}
 	G.CoMx = CoMx;
 	G.CoMy = CoMy;
+	if(nematic) {
+		const double n0_old = store ? C.nem0 : C.nem_old0, n1_old = store ? C.nem1 : C.nem_old1;
+		double n0 = C.nem0, n1 = C.nem1, q00 = C.Q00, q01 = C.Q01;
+		if(Q.anchoring) {
+			n0 = c2;
+			n1 = s3;
+			q00 = 0.5 * (n0 * n0 - n1 * n1);
+			q01 = n0 * n1;
+		}
+		else if(sqrt(S00 * S00 + S01 * S01) > 0.00000001) {
+			const double sgn = (c2 * n0 + s3 * n1 > 0.0) ? 1.0 : -1.0;
+			n0 = n0_old + Q.dt * Q.J_Q * (sgn * c2 - n0);
+			n1 = n1_old + Q.dt * Q.J_Q * (sgn * s3 - n1);
+			q00 = 0.5 * (n0 * n0 - n1 * n1);
+			q01 = n0 * n1;
+		}
+		G.nem_old0 = n0_old;
+		G.nem_old1 = n1_old;
+		G.nem0 = n0;
+		G.nem1 = n1;
+		G.Q00 = q00;
+		G.Q01 = q01;
+		return;
+	}
 	G.thetaQ_old = th_old;
 	G.thetaQ = th;
 	if(Q.anchoring) {
@@ -1051,6 +1095,117 @@ hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hi
 	if(mode == MODE_FUSED) launch_mode<MODE_FUSED>(a, ncells, lds, s);
 	else if(mode == MODE_CF) launch_mode<MODE_CF>(a, ncells, lds, s);
 	else launch_mode<MODE_FS>(a, ncells, lds, s);
+	return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// k_nem_force: the nematic model's force evaluation of one cell per
+// workgroup (WetModel::f_interaction and calc_internal_forces,
+// WetModel.cpp:488-641).  Pass 1 stages phi and evaluates per site the
+// chemical potential mu = CH + A + Rep + Adh, the free-energy density
+// f = CH_d + A_d + Rep_d and the central differences; it keeps f - mu phi,
+// dx, dy in LDS.  Pass 2 takes minus the discrete divergence of the stress
+// (f - mu phi) I + grad phi (x) 2 gamma lambda grad phi over the patch's
+// periodic neighbours, adds the cell's own nematic force and the inter-cell
+// one from the box sums of phi Q, and writes f (wet) or f / friction (dry).
+// Expressions keep the reference's operand order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_nem_force(NemArgs a) {
+	if(a.st->flags) return;
+	extern __shared__ double s_nem[];
+	const int ci = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
+	const DevCell C = a.cells[ci];
+	const DevBox B = a.boxes[C.box];
+	const DevParams P = a.params[2 * C.box + a.set];
+	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L;
+	const int cx = C.corner % L, cy = C.corner / L;
+	double *s_phi = s_nem, *s_s = s_nem + S, *s_dx = s_nem + 2 * S, *s_dy = s_nem + 3 * S;
+	for(int q = tid; q < S; q += 256) s_phi[q] = a.phi[C.start + q];
+	__syncthreads();
+	const double *phi2 = a.phi2 + B.box_off;
+	// box site of storage site (qx, qy) and its four box neighbours
+	auto box_site = [&](int qx, int qy, int &kE, int &kW, int &kN, int &kS) {
+		int lx = qx + C.offx;
+		if(lx >= LX) lx -= LX;
+		int ly = qy + C.offy;
+		if(ly >= LY) ly -= LY;
+		int bx = cx + lx;
+		if(bx >= L) bx -= L;
+		int by = cy + ly;
+		if(by >= L) by -= L;
+		const int bxe = bx + 1 == L ? 0 : bx + 1, bxw = bx == 0 ? L - 1 : bx - 1;
+		const int byn = by + 1 == L ? 0 : by + 1, bys = by == 0 ? L - 1 : by - 1;
+		kE = by * L + bxe;
+		kW = by * L + bxw;
+		kN = byn * L + bx;
+		kS = bys * L + bx;
+		return by * L + bx;
+	};
+	for(int q = tid; q < S; q += 256) {
+		const int qy = q / LX, qx = q - qy * LX;
+		const int qE = qx + 1 == LX ? q - qx : q + 1, qW = qx == 0 ? q + LX - 1 : q - 1;
+		const int qN = qy + 1 == LY ? qx : q + LX, qS = qy == 0 ? q + (LY - 1) * LX : q - LX;
+		const double ph = s_phi[q], pE = s_phi[qE], pW = s_phi[qW], pN = s_phi[qN], pS = s_phi[qS];
+		const double lap = pN + pS + pE + pW - 4 * ph;
+		const double dx = .5 * (pE - pW);
+		const double dy = .5 * (pN - pS);
+		int kE, kW, kN, kS;
+		const int k = box_site(qx, qy, kE, kW, kN, kS);
+		const double p2 = phi2[k];
+		const double CH = P.gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / P.lambda - 2 * P.lambda * lap);
+		const double A = -4 * P.mu_a0 * (1 - C.area / P.a0) * ph;
+		const double Rep = P.rcoef * ph * (p2 - ph * ph);
+		double Adh = 0.;
+		if(P.omega != 0) {
+			const double laplacianSquare = phi2[kE] + phi2[kN] + phi2[kW] + phi2[kS] - 4. * p2;
+			const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
+			const double dd = laplacianSquare - lsquare;
+			Adh = -4 * P.lambda * P.omega * (dd / sqrt(1 + dd * dd)) * ph;
+		}
+		const double mu = CH + A + Rep + Adh;
+		const double CH_d = (P.gamma / P.lambda) * 4 * ph * ph * (1 - ph) * (1 - ph) + P.gamma * P.lambda * (dx * dx + dy * dy);
+		const double A_d = -P.mu / P.a0 * ph * ph;
+		const double Rep_d = (P.kappa / P.lambda) * ph * ph * (p2 - ph * ph);
+		const double fd = CH_d + A_d + Rep_d;
+		s_s[q] = fd - mu * ph;
+		s_dx[q] = dx;
+		s_dy[q] = dy;
+	}
+	__syncthreads();
+	const double gl = 2 * P.gamma * P.lambda;
+	const double *sq00 = a.sq00 + B.box_off, *sq01 = a.sq01 + B.box_off;
+	const bool dry = P.friction_cell == 0;
+	for(int q = tid; q < S; q += 256) {
+		const int qy = q / LX, qx = q - qy * LX;
+		const int qE = qx + 1 == LX ? q - qx : q + 1, qW = qx == 0 ? q + LX - 1 : q - 1;
+		const int qN = qy + 1 == LY ? qx : q + LX, qS = qy == 0 ? q + (LY - 1) * LX : q - LX;
+		const double f_passive_x = -0.5 * ((s_s[qE] + s_dx[qE] * (gl * s_dx[qE])) - (s_s[qW] + s_dx[qW] * (gl * s_dx[qW])))
+				- 0.5 * ((s_dx[qN] * (gl * s_dy[qN])) - (s_dx[qS] * (gl * s_dy[qS])));
+		const double f_passive_y = -0.5 * ((s_s[qN] + s_dy[qN] * (gl * s_dy[qN])) - (s_s[qS] + s_dy[qS] * (gl * s_dy[qS])))
+				- 0.5 * ((s_dy[qE] * (gl * s_dx[qE])) - (s_dy[qW] * (gl * s_dx[qW])));
+		const double dx = s_dx[q], dy = s_dy[q];
+		const double fQ_self_x = -(C.Q00 * dx + C.Q01 * dy);
+		const double fQ_self_y = -(C.Q01 * dx - C.Q00 * dy);
+		int kE, kW, kN, kS;
+		(void) box_site(qx, qy, kE, kW, kN, kS);
+		const double fQ_inter_x = -(0.5 * (sq00[kE] - sq00[kW]) + 0.5 * (sq01[kN] - sq01[kS])) - fQ_self_x;
+		const double fQ_inter_y = -(0.5 * (sq01[kE] - sq01[kW]) - 0.5 * (sq00[kN] - sq00[kS])) - fQ_self_y;
+		const double fx = f_passive_x * P.passive_alpha + fQ_self_x * P.zetaQ_self + fQ_inter_x * P.zetaQ_inter;
+		const double fy = f_passive_y * P.passive_alpha + fQ_self_y * P.zetaQ_self + fQ_inter_y * P.zetaQ_inter;
+		if(dry) a.vrows[C.start + q] = make_double2(fx / P.friction, fy / P.friction);
+		else a.frows[C.start + q] = make_double2(fx, fy);
+	}
+}
+
+hipError_t launch_nem_force(const NemArgs &a, int ncells, size_t lds, hipStream_t s) {
+	if(ncells <= 0) return hipSuccess;
+	static size_t lds_set = 0;
+	if(lds > lds_set) {
+		hipError_t e = hipFuncSetAttribute((const void *) k_nem_force, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+		if(e != hipSuccess) return e;
+		lds_set = lds;
+	}
+	RC_LAUNCH(k_nem_force, dim3(ncells), dim3(256), lds, s, a);
 	return hipGetLastError();
 }
 

@@ -190,7 +190,13 @@ void WetPolarModel::get_settings(input_file &inp) {
 	std::string it = "wetpolarmodel";
 	getInputString(&inp, "interaction_type", it, 0);
 	std::transform(it.begin(), it.end(), it.begin(), ::tolower);
-	if(it != "wetpolarmodel") throw RCexception("interaction_type `%s' is not implemented by the HIP backend", it.c_str());
+	if(it != "wetpolarmodel" && it != "wetmodel")
+		throw RCexception("interaction_type `%s' is not implemented by the HIP backend", it.c_str());
+	nematic = it == "wetmodel";
+	if(nematic) {   // WetModel constructor values (WetModel.cpp:4-20)
+		gamma = 0.01;
+		mu = 3.;
+	}
 	getInputString(&inp, "topology", topology_filename, 1);
 	getInputInt(&inp, "R", &R, 1);
 	getInputNumber(&inp, "lambda", &lambda, 0);
@@ -209,7 +215,7 @@ void WetPolarModel::get_settings(input_file &inp) {
 	getInputNumber(&inp, "CGtolerance", &tolerance, 0);
 	getInputNumber(&inp, "wall_slip", &wall_slip, 0);
 	getInputNumber(&inp, "passive_alpha", &passive_alpha, 0);
-	getInputNumber(&inp, "beta", &beta_active, 1);
+	if(!nematic) getInputNumber(&inp, "beta", &beta_active, 1);   // WetModel has no beta
 	int ext = 0;
 	getInputBoolAsInt(&inp, "external_forces", &ext, 0);
 	if(ext) throw RCexception("external_forces are not implemented by the HIP backend");
@@ -244,6 +250,7 @@ rc_model_params WetPolarModel::to_params(number J0, number dt) const {
 	p.J0 = J0;
 	p.dt = dt;
 	p.cg_max_iter = 0;
+	p.interaction = nematic ? RC_INTERACTION_WETNEMATIC : RC_INTERACTION_WETPOLAR;
 	return p;
 }
 
@@ -325,6 +332,15 @@ void FD_HIPBackend::read_next_configuration() {
 		p.subSize = p.LsubX * p.LsubY;
 		if(v.size() < 11 + 2 * (size_t) p.subSize) throw RCexception("configuration line of cell %d is truncated", i);
 		p.thetaQ = p.thetaQ_old = PI * (1 - 2 * drand48());   // MultiPhaseField::init draw
+		{   // the director of that draw (MultiPhaseField.cpp:80-95), kept by the reader
+			const number q00 = cos(2 * p.thetaQ), q01 = sin(2 * p.thetaQ);
+			const number nemQ_mod = sqrt(q00 * q00 + q01 * q01);
+			const number nx = sqrt((1 + q00 / nemQ_mod) / 2);
+			const number sgn = q01 > 0 ? 1 : q01 < 0 ? -1 : 0;
+			const number ny = sgn * sqrt((1 - q00 / nemQ_mod) / 2);
+			p.nemQ = { nx, ny };
+			p.nemQ_old = { nx, ny };
+		}
 		p.x_sub_left = p.LsubX;
 		p.y_sub_bottom = p.LsubY;
 		p.area = p.perimeter = p.sumF = 0;
@@ -361,7 +377,8 @@ void FD_HIPBackend::init() {
 	for(int i = 0; i < n; ++i) {
 		fields.emplace_back(new MultiPhaseField());
 		fields.back()->index = i;
-		(void) (int) (2 * drand48() * interaction.J_Q_active);  // run_clock draw (WetPolarModel.cpp:72)
+		// run_clock draw (WetPolarModel.cpp:72); WetModel::read_topology draws nothing
+		if(!interaction.nematic) (void) (int) (2 * drand48() * interaction.J_Q_active);
 	}
 	read_next_configuration();
 	rc_box_desc d{};
@@ -416,6 +433,10 @@ void FD_HIPBackend::apply_changes_to_simulation_data() {
 		c.S01 = p.S01;
 		c.total_force_repulsion[0] = p.total_force_repulsion_x;
 		c.total_force_repulsion[1] = p.total_force_repulsion_y;
+		c.nemQ[0] = p.nemQ[0];
+		c.nemQ[1] = p.nemQ[1];
+		c.nemQ_old[0] = p.nemQ_old[0];
+		c.nemQ_old[1] = p.nemQ_old[1];
 		phi.insert(phi.end(), p.fieldScalar.begin(), p.fieldScalar.end());
 		interaction.field_start_index[i] = start;
 		start += p.subSize;
@@ -429,6 +450,11 @@ void FD_HIPBackend::sim_step() { sim_steps(1); }
 
 void FD_HIPBackend::sim_steps(int n) {
 	if(n <= 0) return;
+	if(interaction.nematic) {   // WetModel draws no noise
+		check(rc_step(h, n, nullptr));
+		host_stale = true;
+		return;
+	}
 	std::vector<double> g((size_t) n * N());
 	for(auto &x : g) x = Utils::gaussian();   // predictor draws, cell order (WetPolarModel.cpp:365)
 	check(rc_step(h, n, g.data()));
@@ -475,6 +501,8 @@ void FD_HIPBackend::apply_simulation_data_changes() {
 		p.S01 = c.S01;
 		p.total_force_repulsion_x = c.total_force_repulsion[0];
 		p.total_force_repulsion_y = c.total_force_repulsion[1];
+		p.nemQ = { c.nemQ[0], c.nemQ[1] };
+		p.nemQ_old = { c.nemQ_old[0], c.nemQ_old[1] };
 		p.fieldScalar.assign(phi.begin() + start, phi.begin() + start + p.subSize);
 		p.map_sub_to_box.resize(p.subSize);
 		for(int q = 0; q < p.subSize; ++q) p.map_sub_to_box[q] = p.GetSubIndex(q, box);

@@ -88,14 +88,18 @@ struct MultiPhaseField {
 	number Q00 = 0, Q01 = 0, thetaQ = 0, thetaQ_old = 0;
 	number area = 0, area_old = 0, sumF = 0, sumF_old = 0, perimeter = 0, S00 = 0, S01 = 0;
 	number total_force_repulsion_x = 0, total_force_repulsion_y = 0;
+	std::vector<number> nemQ{ 0., 0. }, nemQ_old{ 0., 0. };   // nematic director (WetModel)
 	std::vector<number> fieldScalar;
 	std::vector<int> map_sub_to_box;
 	int GetSubIndex(int q, const SquareBox &box) const;
 };
 
-// WetPolarModel parameters and the velocity rows (WetPolarModel.h:101-102)
+// The interaction's parameters and the velocity rows (WetPolarModel.h:101-102,
+// WetModel.h:99-100): interaction_type = wetpolarmodel, or the nematic
+// wetmodel (nematic set), whose keys are the same less beta
 class WetPolarModel {
 public:
+	bool nematic = false;
 	number gamma = 0.06, lambda = 2.5, omega = 0.004, mu = 20., kappa = 0.1;
 	int R = 8;
 	bool anchoring = false;

@@ -16,25 +16,37 @@ import numpy as np
 
 from . import _lib
 from .conf import CellConf, Conf, RCexception
-from .rng import ReferenceStream
+from .rng import ReferenceStream, nematic_directors
 
-_KEY_DEFAULTS = {  # WetPolarModel constructor / BaseInteraction defaults
-    "gamma": 0.06, "lambda": 2.5, "omega": 0.004, "mu": 20.0, "kappa": 0.1, "CGtolerance": 1e-4,
-    "wall_slip": 0.5, "passive_alpha": 1.0, "J0": 1.0, "anchoring": 0,
+# constructor / BaseInteraction defaults of each interaction
+_KEY_DEFAULTS = {
+    # WetPolarModel.cpp:4-21
+    "wetpolarmodel": {"gamma": 0.06, "lambda": 2.5, "omega": 0.004, "mu": 20.0, "kappa": 0.1, "CGtolerance": 1e-4,
+                      "wall_slip": 0.5, "passive_alpha": 1.0, "J0": 1.0, "anchoring": 0},
+    # WetModel.cpp:4-20 (it has no beta)
+    "wetmodel": {"gamma": 0.01, "lambda": 2.5, "omega": 0.004, "mu": 3.0, "kappa": 0.1, "CGtolerance": 1e-4,
+                 "wall_slip": 0.5, "passive_alpha": 1.0, "J0": 1.0, "anchoring": 0, "beta": 0.0},
 }
-_REQUIRED = ["R", "dt", "zetaQ_self", "zetaQ_inter", "J_Q", "friction", "friction_cell", "beta"]
+_REQUIRED = {"wetpolarmodel": ["R", "dt", "zetaQ_self", "zetaQ_inter", "J_Q", "friction", "friction_cell", "beta"],
+             "wetmodel": ["R", "dt", "zetaQ_self", "zetaQ_inter", "J_Q", "friction", "friction_cell"]}
+
+
+def interaction_of(keys: Dict[str, object]) -> str:
+    it = str(keys.get("interaction_type", "wetpolarmodel")).lower()
+    if it not in _KEY_DEFAULTS:
+        raise RCexception("interaction_type '%s' is not supported by the HIP backend" % keys.get("interaction_type"))
+    return it
 
 
 def model_params(keys: Dict[str, object], cg_max_iter: int = 0) -> _lib.rc_model_params:
     """Input keys -> rc_model_params.  The *_active keys are mandatory here:
     the reference leaves those members uninitialised when they are absent
-    (WetPolarModel.cpp:4-21 vs :37-46)."""
-    for k in _REQUIRED:
+    (WetPolarModel.cpp:4-21 vs :37-46, WetModel.cpp:4-20 vs :27-45)."""
+    it = interaction_of(keys)
+    for k in _REQUIRED[it]:
         if k not in keys:
             raise RCexception("key '%s' is required (the reference leaves it uninitialised)" % k)
-    g = lambda k: float(keys.get(k, _KEY_DEFAULTS.get(k, 0.0)))  # noqa: E731
-    if str(keys.get("interaction_type", "wetpolarmodel")).lower() != "wetpolarmodel":
-        raise RCexception("interaction_type '%s' is not supported by the HIP backend" % keys.get("interaction_type"))
+    g = lambda k: float(keys.get(k, _KEY_DEFAULTS[it].get(k, 0.0)))  # noqa: E731
     if str(keys.get("box", "square")).lower() != "square":
         raise RCexception("only box = square is supported")
     if int(float(keys.get("external_forces", 0))) != 0:
@@ -44,7 +56,8 @@ def model_params(keys: Dict[str, object], cg_max_iter: int = 0) -> _lib.rc_model
         R=int(float(keys["R"])), anchoring=int(float(keys.get("anchoring", 0))),
         zetaQ_self=g("zetaQ_self"), zetaQ_inter=g("zetaQ_inter"), J_Q=g("J_Q"), friction=g("friction"),
         friction_cell=g("friction_cell"), CGtolerance=g("CGtolerance"), wall_slip=g("wall_slip"),
-        passive_alpha=g("passive_alpha"), beta=g("beta"), J0=g("J0"), dt=g("dt"), cg_max_iter=cg_max_iter)
+        passive_alpha=g("passive_alpha"), beta=g("beta") if it == "wetpolarmodel" else 0.0, J0=g("J0"), dt=g("dt"),
+        cg_max_iter=cg_max_iter, interaction=_lib.RC_INTERACTION[it])
 
 
 def cell_from_conf(c: CellConf) -> _lib.rc_cell:
@@ -90,13 +103,18 @@ class Simulation:
         cells = [c for conf in self.confs for c in conf.cells]
         self.ncells = len(cells)
         self.counts = [len(conf.cells) for conf in self.confs]
+        sd = [int(k.get("seed", 1)) for k in self.keys] if seeds is None else list(seeds)
+        nematic = [interaction_of(k) == "wetmodel" for k in self.keys]
+        # the nematic director of each cell, from its drand48 draw at start-up
+        dirs = [d for s, n, nem in zip(sd, self.counts, nematic) for d in (nematic_directors(s, n) if nem else [(0., 0.)] * n)]
         arr = (_lib.rc_cell * self.ncells)()
         for i, c in enumerate(cells):
             arr[i] = cell_from_conf(c)
+            arr[i].nemQ[0], arr[i].nemQ[1] = dirs[i]
+            arr[i].nemQ_old[0], arr[i].nemQ_old[1] = dirs[i]
         phi = np.ascontiguousarray(np.concatenate([c.phi for c in cells]) if cells else np.zeros(1))
         self._check(self.L.rc_upload_cells(self.h, arr, phi.ctypes.data_as(C.POINTER(C.c_double))))
-        sd = [int(k.get("seed", 1)) for k in self.keys] if seeds is None else list(seeds)
-        self.streams = [ReferenceStream(s, n) for s, n in zip(sd, self.counts)]
+        self.streams = [ReferenceStream(s, n, nem) for s, n, nem in zip(sd, self.counts, nematic)]
         self.step_count = 0
         if init:
             self.init_forces()
