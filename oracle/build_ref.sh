@@ -19,7 +19,7 @@ if [ -x "$OUT/ref_fields" ] && [ -x "$OUT/ref_fields_fast" ] && [ "$OUT/ref_fiel
   exit 0
 fi
 SRCS="$REF/Fields/MultiPhaseField.cpp $REF/Fields/BaseField.cpp $REF/Boxes/SquareBox.cpp $REF/Boxes/BaseBox.cpp \
-  $REF/Interactions/BaseInteraction.cpp \
+  $REF/Interactions/BaseInteraction.cpp $REF/Boxes/SquareWalls.cpp $REF/Boxes/CircleWalls.cpp $REF/Boxes/TriangleWalls.cpp \
   $REF/Utilities/RCexception.cpp $REF/Utilities/Utils.cpp $REF/Utilities/Logger.cpp \
   $REF/Utilities/parse_input/parse_input.cpp $REF/Utilities/parse_input/getline.c"
 g++ -std=c++14 -O2 -fopenmp -w -I"$REF" -I"$REF/extern" "$HERE/ref_driver.cpp" $SRCS -o "$OUT/ref_fields"

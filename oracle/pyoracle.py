@@ -99,6 +99,8 @@ def _lib(flavour: str = "strict") -> C.CDLL:
     L.orc_test_set_polarity.argtypes = [vp, ip, C.c_double, C.c_double, C.c_double, C.c_double]
     L.orc_test_set_director.argtypes = [vp, ip, C.c_double, C.c_double, C.c_double, C.c_double]
     L.orc_test_first_step.argtypes = [vp, ip]
+    L.orc_set_walls.argtypes = [vp, dp]
+    L.orc_add_force.argtypes = [vp, ip, C.c_double, C.c_double, C.c_double]
     L.orc_test_forces.argtypes = [vp]
     _LIBS[flavour] = L
     return L
@@ -127,7 +129,9 @@ class OracleSim:
     """One reference process: box, cells, RNG, interaction state."""
 
     def __init__(self, conf, keys: Dict[str, object], seed: Optional[int] = None, flavour: str = "strict",
-                 cg_max_iter: int = 0, init: bool = True, equilibrate: bool = True):
+                 cg_max_iter: int = 0, init: bool = True, equilibrate: bool = True, walls=None, forces=()):
+        """walls: the box's getWalls field (L*L) or None; forces: dicts with
+        kind, kappa_wall, lambda, shear_rate (rheocell_amd.walls.read_forces)."""
         self.L = _lib(flavour)
         self.N = len(conf.cells)
         self.prm = params_from_keys(keys, cg_max_iter)
@@ -141,6 +145,12 @@ class OracleSim:
             rc = self.L.orc_set_cell(self.h, i, c.LsubX, c.LsubY, c.CoM[0], c.CoM[1], c.offset[0], c.offset[1],
                                      c.corner, int(c.unrap[0]), int(c.unrap[1]), c.Q00, c.Q01, _dp(phi))
             self._check(rc)
+        if walls is not None:
+            self._walls = np.ascontiguousarray(walls, dtype=np.float64)
+            self._check(self.L.orc_set_walls(self.h, _dp(self._walls)))
+        for f in forces:
+            self._check(self.L.orc_add_force(self.h, int(f["kind"]), float(f["kappa_wall"]), float(f["lambda"]),
+                                             float(f["shear_rate"])))
         if init:
             self._check(self.L.orc_init(self.h))
             if equilibrate:

@@ -17,6 +17,10 @@
  *       updateDirectedActiveForces and the solver restart rule
  *       (src/Interactions/WetModel.cpp:194-233, 423-747), and the director
  *       set up by MultiPhaseField::init (src/Fields/MultiPhaseField.cpp:68-98)
+ *   walls and external forces: BaseInteraction's wall-masked stencils
+ *       (src/Interactions/BaseInteraction.h:196-263), the wall branches of both
+ *       models, BaseField::set_F_ext / set_f_density_ext / set_V_ext
+ *       (src/Fields/BaseField.h:54-120) with the forces of src/Forces
  *   Utils::gaussian (src/Utilities/Utils.h:210-234) on POSIX drand48
  *   Eigen::ConjugateGradient<SparseMatrix, Upper> with the default Jacobi
  *       preconditioner (Eigen 3, un-vendored; restated from its published
@@ -78,6 +82,9 @@ struct orc_sim {
 	int *bnbr;                 /* 9*L*L box neighbours */
 	double *phi2, *sum_phi;
 	double *sq00, *sq01;       /* sumQ00, sumQ01: box sums of phi Q (WetModel) */
+	double *walls;             /* getWalls of the box (NULL: 0 everywhere) */
+	int nforce;
+	struct { int kind; double kappa_wall, lambda, rate_active, rate; } force[ORC_MAX_FORCES];
 	int nematic;               /* interaction_type = wetmodel */
 	int *store_size, *store_idx;
 	int *start;
@@ -367,6 +374,7 @@ void orc_destroy(orc_sim *s) {
 	if(!s) return;
 	for(int i = 0; i < s->N; i++) cell_free(&s->c[i]);
 	free(s->c); free(s->start); free(s->bnbr); free(s->phi2); free(s->sum_phi); free(s->sq00); free(s->sq01);
+	free(s->walls);
 	free(s->store_size); free(s->store_idx);
 	free(s->vx); free(s->vy); free(s->fx); free(s->fy);
 	free(s);
@@ -424,16 +432,133 @@ static void resize_rows(orc_sim *s, int rows) {
 	}
 }
 
-/* BaseInteraction::derivX/derivY/Laplacian in a periodic box (walls = 0) */
-static inline double derivX(const ocell *p, int q) {
-	return .5 * (p->phi[p->nbr[5 + q * 9]] - p->phi[p->nbr[3 + q * 9]]);
+/* ---------------- walls and external forces --------------------------- */
+int orc_set_walls(orc_sim *s, const double *walls) {
+	free(s->walls);
+	s->walls = NULL;
+	if(walls) {
+		s->walls = malloc(sizeof(double) * (size_t) s->L * s->L);
+		memcpy(s->walls, walls, sizeof(double) * (size_t) s->L * s->L);
+	}
+	return 0;
 }
-static inline double derivY(const ocell *p, int q) {
-	return .5 * (p->phi[p->nbr[7 + q * 9]] - p->phi[p->nbr[1 + q * 9]]);
+
+int orc_add_force(orc_sim *s, int kind, double kappa_wall, double lambda, double shear_rate) {
+	if(s->nforce >= ORC_MAX_FORCES) return set_err(s, "at most %d external forces", ORC_MAX_FORCES);
+	if(kind < ORC_FORCE_CHANNEL_WALLS || kind > ORC_FORCE_STRETCHING) return set_err(s, "unknown force kind %d", kind);
+	s->force[s->nforce].kind = kind;
+	s->force[s->nforce].kappa_wall = kappa_wall;
+	s->force[s->nforce].lambda = lambda;
+	s->force[s->nforce].rate_active = shear_rate;
+	s->force[s->nforce].rate = 0.;      /* shear_rate is 0 until equilibration ends */
+	s->nforce++;
+	return 0;
 }
-static inline double laplacian(const ocell *p, int q) {
-	return p->phi[p->nbr[7 + q * 9]] + p->phi[p->nbr[1 + q * 9]] + p->phi[p->nbr[5 + q * 9]]
-			+ p->phi[p->nbr[3 + q * 9]] - 4 * p->phi[q];
+
+static inline double get_walls(const orc_sim *s, int k) { return s->walls ? s->walls[k] : 0.; }
+static inline int open_site(const orc_sim *s, int k) { return get_walls(s, k) < s->act.wall_slip; }
+
+/* free_energy / free_energy_density of one force (ChannelWalls.cpp:26-31,
+ * ShearFlowChannel.cpp:33-39, PoiseuilleFlowChannel.cpp, Stretching.cpp) */
+static double force_free_energy(const orc_sim *s, int f, double phi, double walls) {
+	double kappa_wall = s->force[f].kappa_wall, lambda = s->force[f].lambda;
+	switch(s->force[f].kind) {
+	case ORC_FORCE_CHANNEL_WALLS: return 2.0 * kappa_wall / lambda * phi * walls * walls;
+	case ORC_FORCE_SHEAR_FLOW:
+	case ORC_FORCE_POIS_FLOW: return 4.0 * kappa_wall / lambda * phi * walls * walls;
+	default: return 0.;
+	}
+}
+static double force_free_energy_density(const orc_sim *s, int f, double phi, double walls) {
+	double kappa_wall = s->force[f].kappa_wall, lambda = s->force[f].lambda;
+	if(s->force[f].kind == ORC_FORCE_SHEAR_FLOW) return 2.0 * kappa_wall / lambda * phi * phi * walls * walls;
+	return 0.;
+}
+
+/* BaseField::set_F_ext (BaseField.h:54-66): the forces' free energy, also
+ * added to Repulsion; 0 without forces */
+static double set_F_ext(const orc_sim *s, ocell *p, int q, int k) {
+	double F_ext = 0.;
+	for(int f = 0; f < s->nforce; f++) {
+		double force_calc = force_free_energy(s, f, p->phi[q], get_walls(s, k));
+		F_ext += force_calc;
+		p->rep[q] += force_calc;
+	}
+	return F_ext;
+}
+static double set_f_density_ext(const orc_sim *s, ocell *p, int q, int k) {
+	double f_density_ext = 0.;
+	for(int f = 0; f < s->nforce; f++) f_density_ext += force_free_energy_density(s, f, p->phi[q], get_walls(s, k));
+	return f_density_ext;
+}
+
+/* BaseField::set_V_ext (BaseField.h:107-115): the last force's velocity
+ * profile at box site k (ShearFlowChannel.cpp:41-45,
+ * PoiseuilleFlowChannel.cpp, Stretching.cpp: of the cell index on a 10 x 10
+ * grid); ChannelWalls has none */
+static void set_V_ext(const orc_sim *s, const ocell *p, int k, double *V) {
+	for(int f = 0; f < s->nforce; f++) {
+		double lambda = s->force[f].lambda, shear_rate = s->force[f].rate;
+		int lx = s->L, ly = s->L;
+		V[0] = 0.;
+		V[1] = 0.;
+		if(s->force[f].kind == ORC_FORCE_SHEAR_FLOW) {
+			int y = (k / lx);
+			if(y > lambda && y < ly - lambda)
+				V[0] = ((y - (lambda + 1) + 0.5) - 0.5 * (ly - 2 * (lambda + 1))) * shear_rate / (ly - 2 * (lambda + 1));
+		}
+		else if(s->force[f].kind == ORC_FORCE_POIS_FLOW) {
+			int y = (k / lx);
+			if(y > lambda && y < ly - lambda)
+				V[0] = 4 * (y - (lambda + 1)) * shear_rate * (1 - (y - (lambda + 1)) / (ly - 2 * (lambda + 1))) / (ly - 2 * (lambda + 1));
+		}
+		else if(s->force[f].kind == ORC_FORCE_STRETCHING) {
+			int kk = p->index, y = (kk / 10), x = kk - y * 10;
+			if(x == 0) V[0] = -shear_rate;
+			else if(x == 10 - 1) V[0] = shear_rate;
+		}
+	}
+}
+
+/* BaseInteraction::derivX/derivY/Laplacian (BaseInteraction.h:196-263): a
+ * difference across a wall site (getWalls >= wall_slip) is dropped, a
+ * one-sided Laplacian doubles the open side */
+static inline double derivX(const orc_sim *s, const ocell *p, int q, int k) {
+	const int *bn = s->bnbr + 9 * k;
+	if(open_site(s, bn[5]) && open_site(s, bn[3]))
+		return .5 * (p->phi[p->nbr[5 + q * 9]] - p->phi[p->nbr[3 + q * 9]]);
+	return 0.;
+}
+static inline double derivY(const orc_sim *s, const ocell *p, int q, int k) {
+	const int *bn = s->bnbr + 9 * k;
+	if(open_site(s, bn[7]) && open_site(s, bn[1]))
+		return .5 * (p->phi[p->nbr[7 + q * 9]] - p->phi[p->nbr[1 + q * 9]]);
+	return 0.;
+}
+static inline double laplacian(const orc_sim *s, const ocell *p, int q, int k) {
+	const int *bn = s->bnbr + 9 * k;
+	const double *f = p->phi;
+	const int *nb = p->nbr + 9 * q;
+	const int N = open_site(s, bn[7]), S = open_site(s, bn[1]), E = open_site(s, bn[5]), W = open_site(s, bn[3]);
+	if(N && S) {
+		if(E && W) return f[nb[7]] + f[nb[1]] + f[nb[5]] + f[nb[3]] - 4 * f[q];
+		else if(E) return f[nb[7]] + f[nb[1]] + 2 * f[nb[5]] - 4 * f[q];
+		else if(W) return f[nb[7]] + f[nb[1]] + 2 * f[nb[3]] - 4 * f[q];
+		else return f[nb[7]] + f[nb[1]] - 2 * f[q];
+	}
+	else if(N) {
+		if(E && W) return 2 * f[nb[7]] + f[nb[5]] + f[nb[3]] - 4 * f[q];
+		else if(E) return 2 * f[nb[7]] + 2 * f[nb[5]] - 4 * f[q];
+		else if(W) return 2 * f[nb[7]] + 2 * f[nb[3]] - 4 * f[q];
+		else return 2 * f[nb[7]] - 2 * f[q];
+	}
+	else if(S) {
+		if(E && W) return 2 * f[nb[1]] + f[nb[5]] + f[nb[3]] - 4 * f[q];
+		else if(E) return 2 * f[nb[1]] + 2 * f[nb[5]] - 4 * f[q];
+		else if(W) return 2 * f[nb[1]] + 2 * f[nb[3]] - 4 * f[q];
+		else return 2 * f[nb[1]] - 2 * f[q];
+	}
+	return 0.;
 }
 
 /* begin_energy_computation() without arguments (WetPolarModel.cpp:113-147) */
@@ -442,8 +567,10 @@ static void init_field_properties(orc_sim *s) {
 		ocell *p = &s->c[i];
 		for(int q = 0; q < p->S; q++) {
 			p->tf[0] = p->tf[1] = p->tr[0] = p->tr[1] = 0.;
-			p->mu[q] = 0.;
-			double dx = derivX(p, q), dy = derivY(p, q);
+			double dx = derivX(s, p, q, p->map[q]), dy = derivY(s, p, q, p->map[q]);
+			/* BaseInteraction::updateFieldProperties (BaseInteraction.cpp:62-69) */
+			p->mu[q] = set_F_ext(s, p, q, p->map[q]);
+			p->fd[q] = set_f_density_ext(s, p, q, p->map[q]);
 			p->dx[q] = dx;
 			p->dy[q] = dy;
 			p->S00 += -0.5 * (dx * dx - dy * dy);
@@ -461,7 +588,9 @@ static fterms free_energy_terms(orc_sim *s, ocell *p, int q) {
 	int k = p->map[q];
 	const int *bn = s->bnbr + 9 * k;
 	double *phi2 = s->phi2;
-	double laplacianSquare = phi2[bn[5]] + phi2[bn[7]] + phi2[bn[3]] + phi2[bn[1]] - 4. * phi2[k];
+	double laplacianSquare;
+	if(open_site(s, k)) laplacianSquare = phi2[bn[5]] + phi2[bn[7]] + phi2[bn[3]] + phi2[bn[1]] - 4. * phi2[k];
+	else laplacianSquare = phi2[bn[5]] + phi2[bn[3]] - 2. * phi2[k];
 	double ph = p->phi[q];
 	double gamma = s->act.gamma, lambda = s->act.lambda, mu = s->act.mu, kappa = s->act.kappa,
 			omega = s->act.omega, a0 = s->a0;
@@ -499,10 +628,16 @@ static void calc_internal_forces(orc_sim *s, ocell *p, int q) {
 	Theta = (-p->dy[q] * p->Q01) < 0 ? 0.0 : 1.0;
 	double fQy = lambda * (-p->dy[q] * p->Q01) * Theta * p->Q01;
 	int row = q + s->start[p->index];
-	s->fx[row] = f_passive_x * s->act.passive_alpha + fQx * s->zetaQ_self;
-	s->fy[row] = f_passive_y * s->act.passive_alpha + fQy * s->zetaQ_self;
-	p->tf[0] += s->fx[row] * p->phi[q];
-	p->tf[1] += s->fy[row] * p->phi[q];
+	if(open_site(s, p->map[q])) {
+		s->fx[row] = f_passive_x * s->act.passive_alpha + fQx * s->zetaQ_self;
+		s->fy[row] = f_passive_y * s->act.passive_alpha + fQy * s->zetaQ_self;
+		p->tf[0] += s->fx[row] * p->phi[q];
+		p->tf[1] += s->fy[row] * p->phi[q];
+	}
+	else {
+		s->fx[row] = 0.;
+		s->fy[row] = 0.;
+	}
 }
 
 /* WetModel::f_interaction (WetModel.cpp:488-600): no perimeter term; the
@@ -547,8 +682,14 @@ static void calc_internal_forces_nem(orc_sim *s, ocell *p, int q) {
 	double fQ_inter_x = -(0.5 * (s->sq00[bn[5]] - s->sq00[bn[3]]) + 0.5 * (s->sq01[bn[7]] - s->sq01[bn[1]])) - fQ_self_x;
 	double fQ_inter_y = -(0.5 * (s->sq01[bn[5]] - s->sq01[bn[3]]) - 0.5 * (s->sq00[bn[7]] - s->sq00[bn[1]])) - fQ_self_y;
 	int row = q + s->start[p->index];
-	s->fx[row] = f_passive_x * s->act.passive_alpha + fQ_self_x * s->zetaQ_self + fQ_inter_x * s->zetaQ_inter;
-	s->fy[row] = f_passive_y * s->act.passive_alpha + fQ_self_y * s->zetaQ_self + fQ_inter_y * s->zetaQ_inter;
+	if(open_site(s, p->map[q])) {
+		s->fx[row] = f_passive_x * s->act.passive_alpha + fQ_self_x * s->zetaQ_self + fQ_inter_x * s->zetaQ_inter;
+		s->fy[row] = f_passive_y * s->act.passive_alpha + fQ_self_y * s->zetaQ_self + fQ_inter_y * s->zetaQ_inter;
+	}
+	else {
+		s->fx[row] = 0.;
+		s->fy[row] = 0.;
+	}
 }
 
 /* matrix-free form of the assembled operator (see orc_assemble_dense):
@@ -668,13 +809,13 @@ static int compute_forces(orc_sim *s) {
 				s->sq01[k] += p->phi[q] * p->Q01;
 			}
 			p->map[q] = k;
-			p->dx[q] = derivX(p, q);
-			p->dy[q] = derivY(p, q);
-			p->lap[q] = laplacian(p, q);
+			p->dx[q] = derivX(s, p, q, k);
+			p->dy[q] = derivY(s, p, q, k);
+			p->lap[q] = laplacian(s, p, q, k);
 			if(!s->nematic) p->perimeter += p->dx[q] * p->dx[q] + p->dy[q] * p->dy[q];
 			p->tf[0] = p->tf[1] = p->tr[0] = p->tr[1] = 0.;
-			p->mu[q] = 0.;
-			p->fd[q] = 0.;
+			p->mu[q] = set_F_ext(s, p, q, k);
+			p->fd[q] = set_f_density_ext(s, p, q, k);
 			if(s->store_size[k] >= STORE_MAX)
 				return set_err(s, "Too many field patches overlap: %d, %d, %d, %d", p->index, k, q, p->corner);
 			s->store_idx[s->store_size[k] + k * STORE_MAX] = q + s->start[i];
@@ -705,6 +846,7 @@ static int compute_forces(orc_sim *s) {
 		}
 	}
 	if(s->friction_cell != 0) {
+		if(s->walls) return set_err(s, "the wet solve with walls is not restated");
 		double *diag = malloc(sizeof(double) * s->size_rows);
 		compute_diag(s, diag);
 		int itx, ity;
@@ -802,9 +944,10 @@ static int first_step(orc_sim *s, int store) {
 		for(int q = 0; q < p->S; q++) {
 			int k = p->map[q];
 			int row = q + s->start[i];
-			double Vx = 0., Vy = 0.;   /* no external forces */
+			double V[2] = { 0., 0. };   /* V_ext, kept by the field between sites */
+			if(s->nforce > 0) set_V_ext(s, p, k, V);
 			double dphi = -J0 * p->mu[q] - s->vx[row] * p->dx[q] - s->vy[row] * p->dy[q]
-					- Vx * p->dx[q] - Vy * p->dy[q];
+					- V[0] * p->dx[q] - V[1] * p->dy[q];
 			if(store) {
 				p->phi_old[q] = p->phi[q];
 				p->dphi_old[q] = dphi;
@@ -852,6 +995,7 @@ int orc_init(orc_sim *s) {
 }
 
 void orc_apply_changes_after_equilibration(orc_sim *s) {
+	for(int f = 0; f < s->nforce; f++) s->force[f].rate = s->force[f].rate_active;   /* set_ext_properties */
 	s->zetaQ_self = s->act.zetaQ_self;
 	s->zetaQ_inter = s->act.zetaQ_inter;
 	s->J_Q = s->act.J_Q;

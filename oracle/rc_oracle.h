@@ -134,6 +134,21 @@ int orc_test_set_director(orc_sim *s, int i, double n0, double n1, double n0_old
 int orc_test_first_step(orc_sim *s, int store);
 int orc_test_forces(orc_sim *s);
 
+/* Walls and external forces (src/Boxes/SquareWalls.cpp and the other wall
+ * boxes, src/Forces/ChannelWalls.cpp, ShearFlowChannel.cpp,
+ * PoiseuilleFlowChannel.cpp, Stretching.cpp): the box's wall field
+ * (getWalls, L*L values; NULL: none) masks the stencils where it reaches
+ * wall_slip, and each force, acting on every cell in the order added, adds
+ * its free energy and (the last one) its advecting velocity profile.  The
+ * wet solve with walls is not restated (an error). */
+#define ORC_FORCE_CHANNEL_WALLS 0   /* types channel_walls, square_walls, circle_walls, triangle_walls, moving_walls */
+#define ORC_FORCE_SHEAR_FLOW 1      /* shear_flow_channel */
+#define ORC_FORCE_POIS_FLOW 2       /* pois_flow_channel */
+#define ORC_FORCE_STRETCHING 3      /* stretching */
+#define ORC_MAX_FORCES 4
+int orc_set_walls(orc_sim *s, const double *walls);
+int orc_add_force(orc_sim *s, int kind, double kappa_wall, double lambda, double shear_rate);
+
 #ifdef __cplusplus
 }
 #endif

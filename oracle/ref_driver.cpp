@@ -14,6 +14,10 @@
 //   stencil Lbox LX LY offx offy corner phi_0 ... phi_{S-1}
 //                                   -> BaseInteraction::derivX / derivY / Laplacian
 //                                      (BaseInteraction.h:196-263) at every site
+//   walls type Lbox lambda_wall     -> getWalls of every box site of the wall box
+//                                      type (square_walls, circle_walls, triangle_walls)
+//   stencilw type Lbox lambda_wall wall_slip LX LY offx offy corner phi_0 ... phi_{S-1}
+//                                   -> the stencils in that wall box
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -21,7 +25,11 @@
 #include <string>
 #include <vector>
 
+#include "Boxes/CircleWalls.h"
 #include "Boxes/SquareBox.h"
+#include "Boxes/SquareWalls.h"
+#include "Boxes/TriangleWalls.h"
+#include <memory>
 #include "Fields/MultiPhaseField.h"
 #include "Interactions/BaseInteraction.h"
 #include "Utilities/Utils.h"
@@ -33,10 +41,24 @@ public:
 	void init() override {}
 	void allocate_fields(std::vector<BaseField *> &) override {}
 	void check_input_sanity(std::vector<BaseField *> &) override {}
+	void set_wall_slip(double w) { wall_slip = w; }
 	double dx(BaseField *p, int q, int k) { return derivX(p, q, k); }
 	double dy(BaseField *p, int q, int k) { return derivY(p, q, k); }
 	double lap(BaseField *p, int q, int k) { return Laplacian(p, q, k); }
 };
+
+// a wall box of the reference, set up as BoxFactory + get_settings + init do
+static std::unique_ptr<BaseBox> wall_box(const std::string &type, int L, double lambda_wall) {
+	std::unique_ptr<BaseBox> b;
+	if(type == "square_walls") b.reset(new SquareWalls());
+	else if(type == "circle_walls") b.reset(new CircleWalls());
+	else b.reset(new TriangleWalls());
+	input_file inp;
+	inp.init_from_string("lambda_wall = " + std::to_string(lambda_wall));
+	b->get_settings(inp);
+	b->init(L, L);
+	return b;
+}
 
 static void dump_vec(const char *name, const std::vector<double> &v) {
 	printf("\"%s\": [", name);
@@ -124,17 +146,40 @@ int main() {
 			dump_vec("phi", f.fieldScalar);
 			printf("}\n");
 		}
-		else if(cmd == "stencil") {
+		else if(cmd == "walls") {
+			std::string type;
+			int Lb;
+			double lw;
+			in >> type >> Lb >> lw;
+			auto box = wall_box(type, Lb, lw);
+			std::vector<double> w;
+			for(int k = 0; k < Lb * Lb; ++k) w.push_back(box->getWalls(k));
+			printf("{\"cmd\": \"walls\", \"type\": \"%s\", \"L\": %d, \"lambda_wall\": %.17g, ", type.c_str(), Lb, lw);
+			dump_vec("walls", w);
+			printf("}\n");
+		}
+		else if(cmd == "stencil" || cmd == "stencilw") {
+			std::string type = "none";
+			double lw = 0, ws = 0.5;
+			if(cmd == "stencilw") in >> type;
 			int Lb, LX, LY, ox, oy, corner;
-			in >> Lb >> LX >> LY >> ox >> oy >> corner;
-			SquareBox box;
-			box.init(Lb, Lb);
+			in >> Lb;
+			if(cmd == "stencilw") in >> lw >> ws;
+			in >> LX >> LY >> ox >> oy >> corner;
+			std::unique_ptr<BaseBox> boxp;
+			if(cmd == "stencilw") boxp = wall_box(type, Lb, lw);
+			else {
+				boxp.reset(new SquareBox());
+				boxp->init(Lb, Lb);
+			}
+			BaseBox &box = *boxp;
 			MultiPhaseField f;
 			f.init(LX, LY);
 			f.set_positions(ox, oy, corner, 0, 0, Lb);
 			for(int q = 0; q < f.subSize; ++q) in >> f.fieldScalar[q];
 			StencilProbe probe;
 			probe.set_box(&box);
+			probe.set_wall_slip(ws);
 			std::vector<double> dx, dy, lap;
 			for(int q = 0; q < f.subSize; ++q) {
 				int k = f.GetSubIndex(q, &box);
@@ -142,7 +187,8 @@ int main() {
 				dy.push_back(probe.dy(&f, q, k));
 				lap.push_back(probe.lap(&f, q, k));
 			}
-			printf("{\"cmd\": \"stencil\", \"args\": [%d, %d, %d, %d, %d, %d], ", Lb, LX, LY, ox, oy, corner);
+			printf("{\"cmd\": \"%s\", \"type\": \"%s\", \"lambda_wall\": %.17g, \"wall_slip\": %.17g, "
+				   "\"args\": [%d, %d, %d, %d, %d, %d], ", cmd.c_str(), type.c_str(), lw, ws, Lb, LX, LY, ox, oy, corner);
 			dump_vec("dx", dx);
 			printf(", ");
 			dump_vec("dy", dy);

@@ -1,9 +1,10 @@
 #!/usr/bin/env python3
-"""Writes tests/golden/ref_fields.json and ref_stencils.json from the
-reference's own code.
+"""Writes tests/golden/ref_fields.json, ref_stencils.json and ref_walls.json
+from the reference's own code.
 
 Runs oracle/_ref/ref_fields (oracle/ref_driver.cpp linked against the
-reference's MultiPhaseField, SquareBox, BaseInteraction and Utils sources,
+reference's MultiPhaseField, SquareBox, the wall boxes, BaseInteraction and
+Utils sources,
 compiled from /root/reference by oracle/build_ref.sh) and, for the trig
 tables only, the same driver compiled with the reference's Release flags
 (ref_fields_fast).  The fixtures hold inputs and outputs only.
@@ -25,7 +26,8 @@ REF_FAST = os.path.join(ROOT, "oracle", "_ref", "ref_fields_fast")
 
 def run(binary, cmds):
     out = subprocess.run([binary], input="\n".join(cmds) + "\n", capture_output=True, text=True, check=True).stdout
-    return [json.loads(l) for l in out.splitlines() if l.strip()]
+    # the wall boxes also print a line of their own ("TESTING: Walls empty area")
+    return [json.loads(l) for l in out.splitlines() if l.startswith("{")]
 
 
 def main():
@@ -91,6 +93,7 @@ def main():
         json.dump(out, f, separators=(",", ":"))
     print("wrote", os.path.join(HERE, "ref_fields.json"))
     stencils()
+    walls()
 
 
 def stencils():
@@ -111,6 +114,34 @@ def stencils():
     with open(path, "w") as f:
         json.dump({"source": "reference BaseInteraction stencils on MultiPhaseField/SquareBox (oracle/build_ref.sh)",
                    "stencil": res}, f, separators=(",", ":"))
+    print("wrote", path)
+
+
+def walls():
+    """The wall fields of SquareWalls / CircleWalls / TriangleWalls and the
+    BaseInteraction stencils of patches that reach their walls."""
+    rng = np.random.default_rng(91)
+    boxes = [("square_walls", 48, 4.0), ("circle_walls", 48, 3.0), ("triangle_walls", 48, 2.0),
+             ("square_walls", 64, 5.0), ("circle_walls", 61, 4.0)]
+    cmds = ["walls %s %d %.17g" % b for b in boxes]
+    st_cases = [("square_walls", 48, 4.0, 0.5, 15, 13, 3, 2, 0), ("square_walls", 48, 4.0, 0.5, 17, 17, 0, 0, 40 * 48 + 38),
+                ("circle_walls", 48, 3.0, 0.5, 21, 21, 5, 9, 1 * 48 + 2), ("triangle_walls", 48, 2.0, 0.5, 19, 15, 0, 4, 30 * 48 + 14),
+                ("square_walls", 48, 4.0, 2.0, 15, 13, 3, 2, 0)]
+    inputs = []
+    for (t, Lb, lw, ws, LX, LY, ox, oy, corner) in st_cases:
+        phi = np.round(rng.uniform(-0.2, 1.2, LX * LY), 12)
+        inputs.append([float(v) for v in phi])
+        cmds.append("stencilw %s %d %.17g %.17g %d %d %d %d %d " % (t, Lb, lw, ws, LX, LY, ox, oy, corner)
+                    + " ".join("%.17g" % v for v in phi))
+    res = run(REF, cmds)
+    st = [r for r in res if r["cmd"] == "stencilw"]
+    for r, phi in zip(st, inputs):
+        r["phi"] = phi
+    path = os.path.join(HERE, "ref_walls.json")
+    with open(path, "w") as f:
+        json.dump({"source": "reference SquareWalls/CircleWalls/TriangleWalls and BaseInteraction stencils "
+                             "(oracle/build_ref.sh)", "walls": [r for r in res if r["cmd"] == "walls"], "stencilw": st},
+                  f, separators=(",", ":"))
     print("wrote", path)
 
 

@@ -355,8 +355,8 @@ MUTANTS = [
      "p->tr[0] += (-1) * (p->rep[nb[5]] - p->rep[nb[3]]) * p->phi[q];", ["cil_sums"]),
     ("derivx_no_half", "return .5 * (p->phi[p->nbr[5 + q * 9]] - p->phi[p->nbr[3 + q * 9]]);",
      "return (p->phi[p->nbr[5 + q * 9]] - p->phi[p->nbr[3 + q * 9]]);", ["stencils"]),
-    ("laplacian_centre", "+ p->phi[p->nbr[3 + q * 9]] - 4 * p->phi[q];", "+ p->phi[p->nbr[3 + q * 9]] - 2 * p->phi[q];",
-     ["stencils"]),
+    ("laplacian_centre", "if(E && W) return f[nb[7]] + f[nb[1]] + f[nb[5]] + f[nb[3]] - 4 * f[q];",
+     "if(E && W) return f[nb[7]] + f[nb[1]] + f[nb[5]] + f[nb[3]] - 2 * f[q];", ["stencils"]),
 ]
 
 
