@@ -79,13 +79,45 @@ typedef struct rc_model_params {
 #define RC_INTERACTION_WETPOLAR 0
 #define RC_INTERACTION_WETNEMATIC 1
 
-/* One periodic square box (SquareBox, src/Boxes/SquareBox.cpp:21-29) with
- * its own cells and parameters.  Several boxes = independent replicas
- * advanced together. */
+/* An external force acting on every cell (ForceFactory::add_force,
+ * src/Forces/ForceFactory.cpp:31-46, with field = -1).  kind:
+ *   RC_FORCE_CHANNEL_WALLS  types channel_walls, square_walls, circle_walls,
+ *                           triangle_walls, moving_walls (ChannelWalls.cpp):
+ *                           free energy 2 kappa_wall / lambda phi w^2
+ *   RC_FORCE_SHEAR_FLOW     shear_flow_channel (ShearFlowChannel.cpp): free energy
+ *                           4 kappa_wall / lambda phi w^2, density 2 kappa_wall /
+ *                           lambda phi^2 w^2, a linear shear profile between the walls
+ *   RC_FORCE_POIS_FLOW      pois_flow_channel (PoiseuilleFlowChannel.cpp): free
+ *                           energy as the shear flow, a parabolic profile
+ *   RC_FORCE_STRETCHING     stretching (Stretching.cpp): the cells of index 0 and
+ *                           9 mod 10 pulled apart along x
+ * w is the box's wall field; shear_rate acts after equilibration
+ * (rc_apply_changes_after_equilibration), 0 before. */
+#define RC_FORCE_CHANNEL_WALLS 0
+#define RC_FORCE_SHEAR_FLOW 1
+#define RC_FORCE_POIS_FLOW 2
+#define RC_FORCE_STRETCHING 3
+#define RC_MAX_FORCES 4
+typedef struct rc_ext_force {
+	int kind;
+	double kappa_wall, lambda, shear_rate;
+} rc_ext_force;
+
+/* One square box (SquareBox, src/Boxes/SquareBox.cpp:21-29) with its own
+ * cells and parameters.  Several boxes = independent replicas advanced
+ * together.  walls: the getWalls field of a wall box of the reference
+ * (SquareWalls / CircleWalls / TriangleWalls, L*L values, site x + L y),
+ * read by rc_create, or NULL for the periodic box; sites with walls >=
+ * wall_slip mask the stencils (BaseInteraction.h:196-263) and carry no force.
+ * Walls are supported on the dry path (friction_cell = 0); external forces
+ * on both, without J_Q. */
 typedef struct rc_box_desc {
 	int L;                /* box side */
 	int ncells;
 	rc_model_params params;
+	const double *walls;
+	int nforces;          /* 0 .. RC_MAX_FORCES, in the external_forces_file's order */
+	rc_ext_force forces[RC_MAX_FORCES];
 } rc_box_desc;
 
 /* Per-cell scalar state (public members of BaseField, src/Fields/BaseField.h:122-225). */

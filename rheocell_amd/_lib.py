@@ -52,8 +52,16 @@ class rc_model_params(C.Structure):
 RC_INTERACTION = {"wetpolarmodel": 0, "wetmodel": 1}
 
 
+class rc_ext_force(C.Structure):
+    _fields_ = [("kind", C.c_int), ("kappa_wall", C.c_double), ("lambda_", C.c_double), ("shear_rate", C.c_double)]
+
+
+RC_MAX_FORCES = 4
+
+
 class rc_box_desc(C.Structure):
-    _fields_ = [("L", C.c_int), ("ncells", C.c_int), ("params", rc_model_params)]
+    _fields_ = [("L", C.c_int), ("ncells", C.c_int), ("params", rc_model_params), ("walls", C.POINTER(C.c_double)),
+                ("nforces", C.c_int), ("forces", rc_ext_force * RC_MAX_FORCES)]
 
 
 class rc_cell(C.Structure):

@@ -53,6 +53,7 @@ struct rc_handle {
 	size_t lds_stage = 0, lds_cg = 0;
 	bool wet = false, jq = false, win = false;   // win: some omega != 0 (adhesion reads the box window)
 	bool nematic = false;         // interaction_type = wetmodel (WetModel.cpp)
+	bool ext = false;             // some box has walls or external forces
 	// split schedule: the force evaluation is its own launch sequence and keeps
 	// the velocity rows (the wet model, and every nematic box)
 	bool split = false;
@@ -69,6 +70,8 @@ struct rc_handle {
 	double *d_trig = nullptr;
 	double *d_phi2 = nullptr, *d_diag = nullptr;
 	double *d_sq00 = nullptr, *d_sq01 = nullptr;   // nematic: box sums of phi Q00, phi Q01
+	double *d_walls = nullptr;                     // ext: getWalls of every box site
+	unsigned char *d_wmask = nullptr;              // ext: open E/W/N/S/self bits (StageArgs::wmask)
 	int *d_cover = nullptr, *d_tile_cells = nullptr, *d_tile_count = nullptr;
 	int4 *d_tile_geo = nullptr;
 	double2 *d_sp = nullptr, *d_p2 = nullptr, *d_n4 = nullptr;
@@ -227,7 +230,8 @@ static double exact_rcp(double x) {
 	return 1. / x;
 }
 
-static DevParams make_params(const rc_model_params &m, bool active) {
+static DevParams make_params(const rc_box_desc &d, bool active) {
+	const rc_model_params &m = d.params;
 	DevParams p{};
 	p.gamma = m.gamma;
 	p.lambda = m.lambda;
@@ -262,6 +266,20 @@ static DevParams make_params(const rc_model_params &m, bool active) {
 	}
 	p.nematic = m.interaction == RC_INTERACTION_WETNEMATIC ? 1 : 0;
 	if(p.nematic) p.beta = 0.;   // WetModel has no perimeter term
+	// external forces: the coefficients of their free energy and density as
+	// the force classes form them (ChannelWalls.cpp:26-31, ShearFlowChannel.cpp:33-39),
+	// and the last force's velocity profile; the shear rate acts after
+	// equilibration (set_ext_properties)
+	p.nforce = d.nforces;
+	for(int f = 0; f < d.nforces; ++f) {
+		const rc_ext_force &F = d.forces[f];
+		p.fF[f] = F.kind == RC_FORCE_CHANNEL_WALLS ? 2.0 * F.kappa_wall / F.lambda
+				: (F.kind == RC_FORCE_SHEAR_FLOW || F.kind == RC_FORCE_POIS_FLOW) ? 4.0 * F.kappa_wall / F.lambda : 0.;
+		p.fD[f] = F.kind == RC_FORCE_SHEAR_FLOW ? 2.0 * F.kappa_wall / F.lambda : 0.;
+		p.vext_kind = F.kind == RC_FORCE_SHEAR_FLOW ? 1 : F.kind == RC_FORCE_POIS_FLOW ? 2 : F.kind == RC_FORCE_STRETCHING ? 3 : 0;
+		p.vext_lambda = F.lambda;
+		p.vext_rate = active ? F.shear_rate : 0.;
+	}
 	p.friction_rcp = exact_rcp(p.friction);
 	p.pcoef = 4 * (p.beta / p.p0);
 	return p;
@@ -328,8 +346,30 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		B.cell_begin = cells;
 		B.cell_end = cells + d.ncells;
 		h->dboxes.push_back(B);
-		h->dparams.push_back(make_params(d.params, false));
-		h->dparams.push_back(make_params(d.params, true));
+		if(d.nforces < 0 || d.nforces > RC_MAX_FORCES) {
+			*out = h;
+			return fail(h, RC_ERR_ARG, "box %d: %d external forces (0..%d)", b, d.nforces, RC_MAX_FORCES);
+		}
+		for(int f = 0; f < d.nforces; ++f)
+			if(d.forces[f].kind < RC_FORCE_CHANNEL_WALLS || d.forces[f].kind > RC_FORCE_STRETCHING ||
+					(d.forces[f].kind != RC_FORCE_STRETCHING && !(d.forces[f].lambda != 0))) {
+				*out = h;
+				return fail(h, RC_ERR_ARG, "box %d: external force %d has kind %d / lambda %g", b, f, d.forces[f].kind,
+						d.forces[f].lambda);
+			}
+		if(d.walls || d.nforces > 0) {
+			h->ext = true;
+			if(d.params.J_Q != 0 && d.params.interaction == RC_INTERACTION_WETPOLAR) {
+				*out = h;
+				return fail(h, RC_ERR_ARG, "box %d: J_Q with walls or external forces is not supported", b);
+			}
+			if(d.walls && d.params.friction_cell != 0) {
+				*out = h;
+				return fail(h, RC_ERR_ARG, "box %d: walls with the wet solve (friction_cell != 0) are not supported", b);
+			}
+		}
+		h->dparams.push_back(make_params(d, false));
+		h->dparams.push_back(make_params(d, true));
 		cells += d.ncells;
 		tiles += B.ntx * B.nty;
 		sites += (long long) d.L * d.L;
@@ -355,7 +395,11 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 			*out = h;
 			return fail(h, RC_ERR_ARG, "all boxes of a handle must share the dry/wet model");
 		}
-	h->split = h->wet || h->nematic;
+	h->split = h->wet || h->nematic || h->ext;
+	if(h->ext) {   // the stage kernel's walls variant has one shape
+		h->stage_forced[0] = 512;
+		h->stage_forced[1] = 8;
+	}
 	h->ncells = cells;
 	h->ntiles = tiles;
 	h->box_sites = sites;
@@ -390,6 +434,27 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, dalloc(&h->d_sq00, sites));
 		HIPCHK(h, dalloc(&h->d_sq01, sites));
 	}
+	if(h->ext) {
+		// the wall field and the open bits of every site: E 1, W 2, N 4, S 8,
+		// the site itself 16 (getWalls < wall_slip, BaseInteraction.h:196-263)
+		std::vector<double> w(sites, 0.);
+		std::vector<unsigned char> m(sites, 0);
+		for(int b = 0; b < nboxes; ++b) {
+			const int L = boxes[b].L, off = h->dboxes[b].box_off;
+			const double ws = boxes[b].params.wall_slip;
+			if(boxes[b].walls) std::copy(boxes[b].walls, boxes[b].walls + (size_t) L * L, w.begin() + off);
+			for(int y = 0; y < L; ++y)
+				for(int x = 0; x < L; ++x) {
+					auto open = [&](int xx, int yy) { return w[off + ((yy + L) % L) * L + (xx + L) % L] < ws; };
+					m[off + y * L + x] = (open(x + 1, y) ? 1 : 0) | (open(x - 1, y) ? 2 : 0) | (open(x, y + 1) ? 4 : 0) |
+							(open(x, y - 1) ? 8 : 0) | (open(x, y) ? 16 : 0);
+				}
+		}
+		HIPCHK(h, dalloc(&h->d_walls, sites));
+		HIPCHK(h, dalloc(&h->d_wmask, sites));
+		HIPCHK(h, hipMemcpy(h->d_walls, w.data(), sizeof(double) * sites, hipMemcpyHostToDevice));
+		HIPCHK(h, hipMemcpy(h->d_wmask, m.data(), sites, hipMemcpyHostToDevice));
+	}
 	if(h->wet) {
 		HIPCHK(h, dalloc(&h->d_diag, sites));
 		HIPCHK(h, dalloc(&h->d_sp, sites));
@@ -410,7 +475,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_boxes); dfree(h->d_tile_box); dfree(h->d_cells); dfree(h->d_geo); dfree(h->d_params);
 	dfree(h->d_phiA); dfree(h->d_phiB); dfree(h->d_dphi); dfree(h->d_rep);
 	dfree(h->d_v); dfree(h->d_f); dfree(h->d_r); dfree(h->d_p); dfree(h->d_trig);
-	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_sq00); dfree(h->d_sq01); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_geo); dfree(h->d_tile_count);
+	dfree(h->d_phi2); dfree(h->d_diag); dfree(h->d_sq00); dfree(h->d_sq01); dfree(h->d_walls); dfree(h->d_wmask); dfree(h->d_cover); dfree(h->d_tile_cells); dfree(h->d_tile_geo); dfree(h->d_tile_count);
 	dfree(h->d_sp); dfree(h->d_p2); dfree(h->d_n4); dfree(h->d_part_box); dfree(h->d_part); dfree(h->d_cg); dfree(h->d_st); dfree(h->d_gauss);
 	for(auto p : h->d_out) dfree(p);
 	dfree(h->d_snap); dfree(h->d_rec); dfree(h->d_xprev); dfree(h->d_rprev);
@@ -619,6 +684,11 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			h->groups[k].d_cells = nullptr;
 		}
 		h->groups = std::move(used);
+		if(h->ext)
+			for(const auto &g : h->groups)
+				if(g.threads != 512 || g.rows != 8)
+					return fail(h, RC_ERR_ARG, "a %dx%d patch does not fit the walls variant of the stage kernel (512 x 8)",
+							g.mx, g.my);
 		bool tunable = false;
 		for(const auto &g : h->groups) tunable = tunable || g.valid.size() > 1;
 		h->stage_tuned = !tunable || h->stage_forced[0] != 0 || all_kept;
@@ -665,6 +735,8 @@ static StageArgs stage_args(rc_handle *h) {
 	a.rcp = 1;
 	for(const DevParams &p : h->dparams)
 		if(p.lambda_rcp == 0 || p.friction_rcp == 0) a.rcp = 0;
+	a.wmask = h->d_wmask;
+	a.walls = h->d_walls;
 	a.st = h->d_st;
 	return a;
 }
@@ -973,6 +1045,8 @@ static int nem_forces(rc_handle *h, const double *phi) {
 	n.sq01 = h->d_sq01;
 	n.frows = h->d_f;
 	n.vrows = h->d_v;
+	n.wmask = h->d_wmask;
+	n.walls = h->d_walls;
 	n.st = h->d_st;
 	TimeScope ts(h, RC_T_STAGE_CF);
 	HIPCHK(h, launch_nem_force(n, h->ncells, nem_lds_bytes(h->max_S), h->stream));
@@ -1367,6 +1441,9 @@ static int autotune_stage(rc_handle *h) {
 
 extern "C" int rc_set_stage_shape(rc_handle *h, int threads, int rows) {
 	if(!h) return RC_ERR_ARG;
+	if(h->ext && !((threads == 0 && rows == 0) || (threads == 512 && rows == 8)))
+		return fail(h, RC_ERR_ARG, "walls and external forces run the stage kernel's 512 x 8 shape only");
+	if(h->ext) return RC_OK;
 	if(threads == 0 && rows == 0) {
 		h->stage_forced[0] = h->stage_forced[1] = 0;
 		h->stage_tuned = false;

@@ -39,6 +39,13 @@ struct DevParams {
 	double mu_a0, rcoef, pcoef;        // mu / a0, 4 (kappa / lambda), 4 (beta / p0), formed on the host
 	int anchoring;
 	int nematic;                       // interaction_type = wetmodel (WetModel.cpp)
+	// external forces (BaseField::set_F_ext / set_f_density_ext / set_V_ext):
+	// free energy sum_f ((fF[f] phi) w) w, density sum_f (((fD[f] phi) phi) w) w,
+	// and the last force's velocity profile (vext_kind: 0 none, 1 shear, 2
+	// Poiseuille, 3 stretching) with its lambda and current shear rate
+	int nforce, vext_kind;
+	double fF[4], fD[4];
+	double vext_lambda, vext_rate;
 };
 
 struct DevCell {

@@ -35,6 +35,11 @@ struct StageArgs {
 	int threads, npt;            // stage shape: threads per cell, rows per thread strip (>= every patch's)
 	const int *cell_list;        // cell of each workgroup (null: workgroup i takes cell i)
 	int rcp;                     // lambda and every friction are powers of two (DevParams *_rcp)
+	// walls and external forces (null: none): per box site, bit 1 / 2 / 4 / 8 =
+	// the E / W / N / S neighbour is open (getWalls < wall_slip), 16 = the site
+	// itself is; and the wall field getWalls
+	const unsigned char *wmask;
+	const double *walls;
 	int win;                     // LDS holds the box window of sum phi^2 (some omega != 0)
 	DevStatus *st;
 };
@@ -143,6 +148,8 @@ struct NemArgs {
 	const double *phi2, *sq00, *sq01;
 	double2 *frows;              // wet: the CG right-hand side
 	double2 *vrows;              // dry: the velocity
+	const unsigned char *wmask;  // walls and external forces (StageArgs), or null
+	const double *walls;
 	DevStatus *st;
 };
 constexpr size_t NEM_LDS_MAX = 160 * 1024 - 1024;

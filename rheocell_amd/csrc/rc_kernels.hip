@@ -493,12 +493,77 @@ __device__ inline double block_sum8(const double (&v)[8], double *s_part) {
 	return t;
 }
 
+// BaseInteraction::Laplacian with walls (BaseInteraction.h:218-262): bits of m
+// = open E 1, W 2, N 4, S 8; a closed side drops out and the opposite one
+// counts twice, in the reference's expressions
+__device__ inline double lap_masked(unsigned m, double pN, double pS, double pE, double pW, double ph) {
+	const bool E = m & 1, W = m & 2, N = m & 4, S = m & 8;
+	if(N && S) {
+		if(E && W) return pN + pS + pE + pW - 4 * ph;
+		else if(E) return pN + pS + 2 * pE - 4 * ph;
+		else if(W) return pN + pS + 2 * pW - 4 * ph;
+		return pN + pS - 2 * ph;
+	}
+	else if(N) {
+		if(E && W) return 2 * pN + pE + pW - 4 * ph;
+		else if(E) return 2 * pN + 2 * pE - 4 * ph;
+		else if(W) return 2 * pN + 2 * pW - 4 * ph;
+		return 2 * pN - 2 * ph;
+	}
+	else if(S) {
+		if(E && W) return 2 * pS + pE + pW - 4 * ph;
+		else if(E) return 2 * pS + 2 * pE - 4 * ph;
+		else if(W) return 2 * pS + 2 * pW - 4 * ph;
+		return 2 * pS - 2 * ph;
+	}
+	return 0.;
+}
+
+// BaseField::set_F_ext / set_f_density_ext (BaseField.h:54-79): the external
+// forces' free energy and its density at a site of wall field w
+__device__ inline double f_ext(const DevParams &P, double ph, double w) {
+	double F_ext = 0.;
+	for(int f = 0; f < P.nforce; ++f) F_ext += P.fF[f] * ph * w * w;
+	return F_ext;
+}
+__device__ inline double f_density_ext(const DevParams &P, double ph, double w) {
+	double f_density_ext = 0.;
+	for(int f = 0; f < P.nforce; ++f) f_density_ext += P.fD[f] * ph * ph * w * w;
+	return f_density_ext;
+}
+
+// BaseField::set_V_ext (BaseField.h:107-115): the x velocity of the last
+// force's profile at box row y (ShearFlowChannel.cpp:41-45,
+// PoiseuilleFlowChannel.cpp) or of the cell index (Stretching.cpp); the y
+// velocity is 0
+__device__ inline double v_ext(const DevParams &P, int y, int L, int index) {
+	const double lambda = P.vext_lambda, shear_rate = P.vext_rate;
+	const int ly = L;
+	if(P.vext_kind == 1) {
+		if(y > lambda && y < ly - lambda)
+			return ((y - (lambda + 1) + 0.5) - 0.5 * (ly - 2 * (lambda + 1))) * shear_rate / (ly - 2 * (lambda + 1));
+	}
+	else if(P.vext_kind == 2) {
+		if(y > lambda && y < ly - lambda)
+			return 4 * (y - (lambda + 1)) * shear_rate * (1 - (y - (lambda + 1)) / (ly - 2 * (lambda + 1))) / (ly - 2 * (lambda + 1));
+	}
+	else if(P.vext_kind == 3) {
+		const int yy = index / 10, x = index - yy * 10;
+		if(x == 0) return -shear_rate;
+		if(x == 10 - 1) return shear_rate;
+	}
+	return 0.;
+}
+
 // CT threads per cell; NPT >= the strip rows of every patch (compile time), so
 // the per-site registers are fully unrolled.  RCP: lambda and the friction of
 // every parameter set are powers of two (divisions become exact multiplications).
 // ST: the predictor (first_step(true)) when set, else the corrector (a.store
-// equals ST; MODE_CF takes ST = 0).  LDS layout: stage_lds_bytes (rc_kernels.h).
-template <int MODE, int NPT, bool RCP, int CT, bool ST>
+// equals ST; MODE_CF takes ST = 0).  WL: walls and external forces (a.wmask,
+// a.walls): masked stencils, the forces' free energy in mu, no force on wall
+// sites and the forces' advection in the update.  LDS layout: stage_lds_bytes
+// (rc_kernels.h).
+template <int MODE, int NPT, bool RCP, int CT, bool ST, bool WL = false>
 __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	constexpr bool store = ST;
 	// Any status flag halts the device, except that a growth request raised
@@ -567,8 +632,11 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	}
 	// the box sums of phi^2 at the strip's rows, requested with the strip (rows
 	// past the strip read valid box sites and are discarded)
-	const double *phi2b = a.phi2 + B.box_off + (cx + lx >= L ? cx + lx - L : cx + lx);
+	const int bcol = B.box_off + (cx + lx >= L ? cx + lx - L : cx + lx);
+	const double *phi2b = a.phi2 + bcol;
 	double p2g[NPT];
+	unsigned wm[NPT];    // WL: wall mask bits and wall field of the strip's sites
+	double wv[NPT];
 #pragma unroll
 	for(int k = 0; k < NPT; ++k) {
 		int qk = qy0 + k;
@@ -579,6 +647,12 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 		int by = cy + ly;
 		if(by >= L) by -= L;
 		p2g[k] = phi2b[by * L];
+		wm[k] = 31u;
+		wv[k] = 0.;
+		if constexpr(WL) {
+			wm[k] = a.wmask[bcol + by * L];
+			wv[k] = a.walls[bcol + by * L];
+		}
 	}
 	// row tables, the box window (adhesion only) and the phi tile with its
 	// periodic columns
@@ -629,8 +703,10 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 #pragma unroll
 		for(int k = 0; k < NPT; ++k) {
 			if(k >= R) break;
-			const double dx = .5 * (pm[k * PW + 1].x - pm[k * PW - 1].x);
-			const double dy = .5 * (col[k + 2] - col[k]);
+			double dx = .5 * (pm[k * PW + 1].x - pm[k * PW - 1].x);
+			double dy = .5 * (col[k + 2] - col[k]);
+			if(WL && (wm[k] & 3u) != 3u) dx = 0.;
+			if(WL && (wm[k] & 12u) != 12u) dy = 0.;
 			if(k < nr) per[0] += dx * dx + dy * dy;
 		}
 		block_sum<1, NW>(per, s_red);
@@ -655,9 +731,14 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 		if(k >= R) break;
 		const double ph = col[k + 1];
 		const double pE = pm[k * PW + 1].x, pW = pm[k * PW - 1].x, pN = col[k + 2], pS = col[k];
-		const double lap = pN + pS + pE + pW - 4 * ph;
-		const double dx = .5 * (pE - pW);
-		const double dy = .5 * (pN - pS);
+		double lap = pN + pS + pE + pW - 4 * ph;
+		double dx = .5 * (pE - pW);
+		double dy = .5 * (pN - pS);
+		if constexpr(WL) {
+			lap = lap_masked(wm[k], pN, pS, pE, pW, ph);
+			if((wm[k] & 3u) != 3u) dx = 0.;
+			if((wm[k] & 12u) != 12u) dy = 0.;
+		}
 		const bool valid = k < nr;
 		if(MODE == MODE_CF && P.beta == 0 && valid) per2[0] += dx * dx + dy * dy;
 		// Both loads always issue and the value is selected: a select of the
@@ -671,7 +752,8 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 		const double Rep = Rcoef * ph * (p2 - ph * ph);
 		double Adh = 0.;
 		if(P.omega != 0) {
-			const double laplacianSquare = s_win[w + 1] + s_win[w + WX] + s_win[w - 1] + s_win[w - WX] - 4. * p2;
+			double laplacianSquare = s_win[w + 1] + s_win[w + WX] + s_win[w - 1] + s_win[w - WX] - 4. * p2;
+			if(WL && !(wm[k] & 16u)) laplacianSquare = s_win[w + 1] + s_win[w - 1] - 2. * p2;   // a wall site
 			const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
 			const double dd = laplacianSquare - lsquare;
 			const double suppress = dd / sqrt(1 + dd * dd);
@@ -679,7 +761,8 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 		}
 		double Pp = 0.;
 		if(P.beta != 0) Pp = Pcoef * lap;
-		const double V = CH + A + Rep + Adh + Pp;
+		double V = CH + A + Rep + Adh + Pp;
+		if constexpr(WL) V = f_ext(P, ph, wv[k]) + V;   // freeEnergy starts at F_ext
 		mu[k] = V;
 		if(!idle) {
 			pm[k * PW].y = V;
@@ -760,8 +843,10 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 		const int row = row0 + k * LX;
 		const double ph = col[k + 1];
 		const double2 eE = pm[k * PW + 1], eW = pm[k * PW - 1];
-		const double dx = .5 * (eE.x - eW.x);
-		const double dy = .5 * (col[k + 2] - col[k]);
+		double dx = .5 * (eE.x - eW.x);
+		double dy = .5 * (col[k + 2] - col[k]);
+		if(WL && (wm[k] & 3u) != 3u) dx = 0.;
+		if(WL && (wm[k] & 12u) != 12u) dy = 0.;
 		double vx, vy;
 		if(MODE == MODE_FS) {
 			vx = vv[k].x;
@@ -784,8 +869,9 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 			const double fQx = P.lambda * (-dx * C.Q00) * Theta * C.Q00;
 			Theta = (-dy * C.Q01) < 0 ? 0.0 : 1.0;
 			const double fQy = P.lambda * (-dy * C.Q01) * Theta * C.Q01;
-			const double fx = f_passive_x * P.passive_alpha + fQx * P.zetaQ_self;
-			const double fy = f_passive_y * P.passive_alpha + fQy * P.zetaQ_self;
+			double fx = f_passive_x * P.passive_alpha + fQx * P.zetaQ_self;
+			double fy = f_passive_y * P.passive_alpha + fQy * P.zetaQ_self;
+			if(WL && !(wm[k] & 16u)) fx = fy = 0.;   // no force on a wall site
 			vx = div_by<RCP>(fx, P.friction, P.friction_rcp);
 			vy = div_by<RCP>(fy, P.friction, P.friction_rcp);
 			if(MODE == MODE_CF) {
@@ -798,7 +884,10 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 			if(a.write_v && valid) a.vrows[row] = make_double2(vx, vy);
 		}
 		// FD_CPUBackend::first_step body (FD_CPUBackend.cpp:66-101)
-		const double dphi = -Q.J0 * mu[k] - vx * dx - vy * dy;
+		double dphi = -Q.J0 * mu[k] - vx * dx - vy * dy;
+		if constexpr(WL) {   // external advection (the y velocity is 0)
+			if(Q.vext_kind) dphi = dphi - v_ext(Q, s_rowi[qy0 + k].x / L, L, ci - B.cell_begin) * dx;
+		}
 		double phn;
 		if(store) {
 			if(valid) a.dphi_old[row] = dphi;
@@ -1043,6 +1132,13 @@ static hipError_t set_lds(size_t bytes) {
 
 hipError_t stage_set_lds_limit(size_t bytes) {
 	hipError_t e;
+	const int b = (int) bytes;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_CF, 8, false, 512, false, true>,
+			hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_FS, 8, false, 512, false, true>,
+			hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_FS, 8, false, 512, true, true>,
+			hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
 	if((e = set_lds<MODE_FUSED, true>(bytes)) != hipSuccess) return e;
 	if((e = set_lds<MODE_FUSED, false>(bytes)) != hipSuccess) return e;
 	if((e = set_lds<MODE_CF, false>(bytes)) != hipSuccess) return e;
@@ -1062,6 +1158,13 @@ int stage_strip_rows(int LX, int LY, int threads) { return strip_rows(LX, LY, th
 template <int MODE, bool RCP, bool ST>
 static void launch_mode_r(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
 	const dim3 g(ncells);
+	if constexpr(MODE != MODE_FUSED && !RCP) {
+		// walls and external forces: one shape (512 threads, 8 rows)
+		if(a.wmask) {
+			RC_LAUNCH((k_stage<MODE, 8, false, 512, ST, true>), g, dim3(512), lds, s, a);
+			return;
+		}
+	}
 	if(a.threads == 256) {
 		if(a.npt == 4) RC_LAUNCH((k_stage<MODE, 4, RCP, 256, ST>), g, dim3(256), lds, s, a);
 		else RC_LAUNCH((k_stage<MODE, 7, RCP, 256, ST>), g, dim3(256), lds, s, a);
@@ -1076,7 +1179,7 @@ static void launch_mode_r(const StageArgs &a, int ncells, size_t lds, hipStream_
 }
 template <int MODE, bool ST>
 static void launch_mode_s(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
-	if(a.rcp) launch_mode_r<MODE, true, ST>(a, ncells, lds, s);
+	if(a.rcp && !a.wmask) launch_mode_r<MODE, true, ST>(a, ncells, lds, s);
 	else launch_mode_r<MODE, false, ST>(a, ncells, lds, s);
 }
 template <int MODE>
@@ -1146,27 +1249,32 @@ __global__ __launch_bounds__(256) void k_nem_force(NemArgs a) {
 		const int qE = qx + 1 == LX ? q - qx : q + 1, qW = qx == 0 ? q + LX - 1 : q - 1;
 		const int qN = qy + 1 == LY ? qx : q + LX, qS = qy == 0 ? q + (LY - 1) * LX : q - LX;
 		const double ph = s_phi[q], pE = s_phi[qE], pW = s_phi[qW], pN = s_phi[qN], pS = s_phi[qS];
-		const double lap = pN + pS + pE + pW - 4 * ph;
-		const double dx = .5 * (pE - pW);
-		const double dy = .5 * (pN - pS);
 		int kE, kW, kN, kS;
 		const int k = box_site(qx, qy, kE, kW, kN, kS);
+		const unsigned m = a.wmask ? a.wmask[B.box_off + k] : 31u;
+		const double w = a.wmask ? a.walls[B.box_off + k] : 0.;
+		const double lap = a.wmask ? lap_masked(m, pN, pS, pE, pW, ph) : pN + pS + pE + pW - 4 * ph;
+		const double dx = (m & 3u) == 3u ? .5 * (pE - pW) : 0.;
+		const double dy = (m & 12u) == 12u ? .5 * (pN - pS) : 0.;
 		const double p2 = phi2[k];
 		const double CH = P.gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / P.lambda - 2 * P.lambda * lap);
 		const double A = -4 * P.mu_a0 * (1 - C.area / P.a0) * ph;
 		const double Rep = P.rcoef * ph * (p2 - ph * ph);
 		double Adh = 0.;
 		if(P.omega != 0) {
-			const double laplacianSquare = phi2[kE] + phi2[kN] + phi2[kW] + phi2[kS] - 4. * p2;
+			const double laplacianSquare = (m & 16u) ? phi2[kE] + phi2[kN] + phi2[kW] + phi2[kS] - 4. * p2
+					: phi2[kE] + phi2[kW] - 2. * p2;   // a wall site
 			const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
 			const double dd = laplacianSquare - lsquare;
 			Adh = -4 * P.lambda * P.omega * (dd / sqrt(1 + dd * dd)) * ph;
 		}
-		const double mu = CH + A + Rep + Adh;
+		double mu = CH + A + Rep + Adh;
+		if(a.wmask) mu = f_ext(P, ph, w) + mu;   // freeEnergy starts at F_ext
 		const double CH_d = (P.gamma / P.lambda) * 4 * ph * ph * (1 - ph) * (1 - ph) + P.gamma * P.lambda * (dx * dx + dy * dy);
 		const double A_d = -P.mu / P.a0 * ph * ph;
 		const double Rep_d = (P.kappa / P.lambda) * ph * ph * (p2 - ph * ph);
-		const double fd = CH_d + A_d + Rep_d;
+		double fd = CH_d + A_d + Rep_d;
+		if(a.wmask) fd = f_density_ext(P, ph, w) + fd;
 		s_s[q] = fd - mu * ph;
 		s_dx[q] = dx;
 		s_dy[q] = dy;
@@ -1190,8 +1298,9 @@ __global__ __launch_bounds__(256) void k_nem_force(NemArgs a) {
 		(void) box_site(qx, qy, kE, kW, kN, kS);
 		const double fQ_inter_x = -(0.5 * (sq00[kE] - sq00[kW]) + 0.5 * (sq01[kN] - sq01[kS])) - fQ_self_x;
 		const double fQ_inter_y = -(0.5 * (sq01[kE] - sq01[kW]) - 0.5 * (sq00[kN] - sq00[kS])) - fQ_self_y;
-		const double fx = f_passive_x * P.passive_alpha + fQ_self_x * P.zetaQ_self + fQ_inter_x * P.zetaQ_inter;
-		const double fy = f_passive_y * P.passive_alpha + fQ_self_y * P.zetaQ_self + fQ_inter_y * P.zetaQ_inter;
+		double fx = f_passive_x * P.passive_alpha + fQ_self_x * P.zetaQ_self + fQ_inter_x * P.zetaQ_inter;
+		double fy = f_passive_y * P.passive_alpha + fQ_self_y * P.zetaQ_self + fQ_inter_y * P.zetaQ_inter;
+		if(a.wmask && !(a.wmask[B.box_off + box_site(qx, qy, kE, kW, kN, kS)] & 16u)) fx = fy = 0.;   // wall site
 		if(dry) a.vrows[C.start + q] = make_double2(fx / P.friction, fy / P.friction);
 		else a.frows[C.start + q] = make_double2(fx, fy);
 	}

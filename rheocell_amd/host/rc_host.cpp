@@ -216,9 +216,97 @@ void WetPolarModel::get_settings(input_file &inp) {
 	getInputNumber(&inp, "wall_slip", &wall_slip, 0);
 	getInputNumber(&inp, "passive_alpha", &passive_alpha, 0);
 	if(!nematic) getInputNumber(&inp, "beta", &beta_active, 1);   // WetModel has no beta
-	int ext = 0;
-	getInputBoolAsInt(&inp, "external_forces", &ext, 0);
-	if(ext) throw RCexception("external_forces are not implemented by the HIP backend");
+}
+
+// getWalls of the reference's square wall boxes as their init builds it
+// (SquareWalls.cpp:19-40, CircleWalls.cpp:19-37, TriangleWalls.cpp:19-50)
+static std::vector<double> wall_field(const std::string &type, int L, number lambda_wall) {
+	std::vector<double> w((size_t) L * L, 0.);
+	if(type == "square_walls") {
+		for(int y = 0; y < L; y++)
+			for(int x = 0; x < L; x++)
+				if(x < lambda_wall - 1 || x > L - lambda_wall || y < lambda_wall || y > L - lambda_wall) w[x + y * L] = 1.;
+	}
+	else if(type == "circle_walls") {
+		for(int y = 0; y < L; y++)
+			for(int x = 0; x < L; x++) {
+				double dist = L / 2 - sqrt(((L / 2) - x) * ((L / 2) - x) + ((L / 2) - y) * ((L / 2) - y));
+				w[x + y * L] = (dist <= 0 || dist < lambda_wall) ? 1. : 0.;
+			}
+	}
+	else if(type == "triangle_walls") {
+		std::fill(w.begin(), w.end(), 1.);
+		int side = L - lambda_wall * 2;
+		int height = int(round(side * sqrt(3.0) / 2.0));
+		int center = L / 2, top = L / 2 - height / 2;
+		for(int row = 0; row < height; ++row) {
+			int half = int((side / 2.0) * (1.0 - (double) row / height));
+			int y = top + row;
+			if(y >= 0 && y < L)
+				for(int x = center - half; x <= center + half && x < L; ++x)
+					if(x >= 0) w[x + y * L] = 0.;
+		}
+	}
+	return w;
+}
+
+// ForceFactory::make_forces (ForceFactory.cpp:61-150): the { key = value }
+// blocks of external_forces_file, each a force on the fields it names (here
+// every field, field = -1)
+void FD_HIPBackend::read_external_forces(input_file &inp) {
+	bool ext = false;
+	getInputBool(&inp, "external_forces", &ext, 0);
+	if(!ext) return;
+	std::string file;
+	getInputString(&inp, "external_forces_file", file, 1);
+	bool json = false;
+	getInputBool(&inp, "external_forces_as_JSON", &json, 0);
+	if(json) throw RCexception("external_forces_as_JSON is not implemented by the HIP backend");
+	std::ifstream in(file);
+	if(!in.good()) throw RCexception("Can't read external_forces_file '%s'", file.c_str());
+	std::string text, line;
+	while(std::getline(in, line)) {
+		const size_t c = line.find('#');
+		text += (c == std::string::npos ? line : line.substr(0, c)) + "\n";
+	}
+	int depth = 0;
+	size_t open = 0;
+	for(size_t i = 0; i < text.size(); ++i) {
+		if(text[i] == '{') {
+			if(++depth > 1) throw RCexception("Syntax error in '%s': parentheses do not match", file.c_str());
+			open = i + 1;
+		}
+		else if(text[i] == '}') {
+			if(--depth < 0) throw RCexception("Syntax error in '%s': parentheses do not match", file.c_str());
+			input_file f;
+			f.init_from_string(text.substr(open, i - open));
+			std::string type, field;
+			getInputString(&f, "type", type, 1);
+			getInputString(&f, "field", field, 1);
+			if(field != "-1") throw RCexception("external force `%s': only field = -1 is implemented by the HIP backend", type.c_str());
+			rc_ext_force F{};
+			F.lambda = 2.;
+			if(type == "channel_walls" || type == "moving_walls" || type == "circle_walls" || type == "triangle_walls" ||
+					type == "square_walls") {
+				F.kind = RC_FORCE_CHANNEL_WALLS;
+				getInputNumber(&f, "kappa_wall", &F.kappa_wall, 0);
+				getInputNumber(&f, "lambda", &F.lambda, 0);
+			}
+			else if(type == "shear_flow_channel" || type == "pois_flow_channel") {
+				F.kind = type == "shear_flow_channel" ? RC_FORCE_SHEAR_FLOW : RC_FORCE_POIS_FLOW;
+				getInputNumber(&f, "kappa_wall", &F.kappa_wall, 1);
+				getInputNumber(&f, "lambda", &F.lambda, 1);
+				getInputNumber(&f, "shear_rate", &F.shear_rate, 1);
+			}
+			else if(type == "stretching") {
+				F.kind = RC_FORCE_STRETCHING;
+				getInputNumber(&f, "shear_rate", &F.shear_rate, 1);
+			}
+			else throw RCexception("Invalid force type `%s'", type.c_str());
+			if(forces.size() == RC_MAX_FORCES) throw RCexception("at most %d external forces", RC_MAX_FORCES);
+			forces.push_back(F);
+		}
+	}
 }
 
 int WetPolarModel::get_N_from_topology() const {
@@ -276,9 +364,15 @@ void FD_HIPBackend::get_settings(input_file &inp) {
 	std::string prec = "double";
 	getInputString(&inp, "backend_precision", prec, 0);
 	if(prec != "double") throw RCexception("the HIP backend computes in double precision only");
-	std::string box_type = "square";
-	getInputString(&inp, "box", box_type, 0);
-	if(box_type != "square") throw RCexception("box `%s' is not implemented by the HIP backend", box_type.c_str());
+	std::string box = "square";
+	getInputString(&inp, "box", box, 0);
+	if(box != "square") throw RCexception("box `%s' is not implemented by the HIP backend", box.c_str());
+	// BoxFactory::make_box (BoxFactory.cpp:16-26): square, or one of its wall boxes
+	getInputString(&inp, "type", box_type, 0);
+	if(box_type == "square_walls" || box_type == "circle_walls" || box_type == "triangle_walls")
+		getInputNumber(&inp, "lambda_wall", &lambda_wall, 1);
+	else box_type = "none";
+	read_external_forces(inp);
 	interaction.get_settings(inp);
 	getInputBool(&inp, "restart_step_counter", &restart_step_counter, 0);
 	std::string reload;
@@ -385,6 +479,12 @@ void FD_HIPBackend::init() {
 	d.L = box.side;
 	d.ncells = n;
 	d.params = interaction.to_params(J0, dt);
+	if(box_type != "none") {
+		walls = wall_field(box_type, box.side, lambda_wall);
+		d.walls = walls.data();
+	}
+	d.nforces = (int) forces.size();
+	for(size_t f = 0; f < forces.size(); ++f) d.forces[f] = forces[f];
 	int rc = rc_create(&d, 1, &h);
 	if(rc != RC_OK) {
 		std::string msg = h ? rc_last_error(h) : "rc_create failed";

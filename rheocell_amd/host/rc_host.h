@@ -5,6 +5,8 @@
 //   RCexception                 src/Utilities/RCexception.h
 //   Utils::gaussian             src/Utilities/Utils.h:210-234 (glibc drand48)
 //   SquareBox                   src/Boxes/SquareBox.cpp:21-82
+//   wall boxes / external forces src/Boxes/SquareWalls.cpp, CircleWalls.cpp, TriangleWalls.cpp,
+//                               src/Forces/ForceFactory.cpp:61-150
 //   MultiPhaseField (host copy) src/Fields/BaseField.h:122-225, MultiPhaseField.cpp:348-359
 //   WetPolarModel (parameters)  src/Interactions/WetPolarModel.cpp:4-93
 //   FD_HIPBackend               src/Backends/SimBackend.h:86-158, FD_CPUBackend.cpp:13-160
@@ -184,6 +186,7 @@ public:
 
 private:
 	void read_next_configuration();
+	void read_external_forces(input_file &inp);
 	void check(int rc);
 	rc_handle *h = nullptr;
 	std::string conf_filename, trajectory_file, lastconf_file = "last_conf.dat",
@@ -192,6 +195,10 @@ private:
 	bool host_stale = true;
 	int warm_mode = RC_WARM_REFERENCE;
 	bool async_conf_output = true;   // input key async_conf_output (no counterpart in the reference)
+	std::string box_type = "none";   // type: none or a square wall box
+	number lambda_wall = 0;
+	std::vector<double> walls;        // getWalls of the box (empty: none)
+	std::vector<rc_ext_force> forces; // external forces (external_forces_file)
 	std::unique_ptr<ConfWriter> writer;
 };
 

@@ -16,6 +16,7 @@ import numpy as np
 
 from . import _lib
 from .conf import CellConf, Conf, RCexception
+from . import walls
 from .rng import ReferenceStream, nematic_directors
 
 # constructor / BaseInteraction defaults of each interaction
@@ -48,9 +49,7 @@ def model_params(keys: Dict[str, object], cg_max_iter: int = 0) -> _lib.rc_model
             raise RCexception("key '%s' is required (the reference leaves it uninitialised)" % k)
     g = lambda k: float(keys.get(k, _KEY_DEFAULTS[it].get(k, 0.0)))  # noqa: E731
     if str(keys.get("box", "square")).lower() != "square":
-        raise RCexception("only box = square is supported")
-    if int(float(keys.get("external_forces", 0))) != 0:
-        raise RCexception("external_forces are not supported by the HIP backend")
+        raise RCexception("only box = square is supported (with type = none or %s)" % ", ".join(walls.WALL_BOXES))
     return _lib.rc_model_params(
         gamma=g("gamma"), lambda_=g("lambda"), omega=g("omega"), mu=g("mu"), kappa=g("kappa"),
         R=int(float(keys["R"])), anchoring=int(float(keys.get("anchoring", 0))),
@@ -88,6 +87,7 @@ class Simulation:
     def __init__(self, boxes: Sequence[Tuple[Conf, Dict[str, object]]], seeds: Optional[Sequence[int]] = None,
                  cg_max_iter: int = 0, init: bool = True, equilibrate: bool = True):
         self.L = _lib.load()
+        self._keep = []    # arrays rc_create reads
         self.confs = [b[0] for b in boxes]
         self.keys = [b[1] for b in boxes]
         descs = (_lib.rc_box_desc * len(boxes))()
@@ -97,6 +97,7 @@ class Simulation:
             descs[i].L = conf.Lx
             descs[i].ncells = len(conf.cells)
             descs[i].params = model_params(keys, cg_max_iter)
+            self._set_walls_and_forces(descs[i], keys, conf.Lx)
         self.h = C.c_void_p()
         rc = self.L.rc_create(descs, len(boxes), C.byref(self.h))
         self._check(rc)
@@ -120,6 +121,29 @@ class Simulation:
             self.init_forces()
             if equilibrate:
                 self.apply_changes_after_equilibration()
+
+    def _set_walls_and_forces(self, desc, keys, L):
+        """The box's wall field (BoxFactory: box = square with type = a wall
+        box, src/Boxes/BoxFactory.cpp:16-26) and the external forces of
+        external_forces_file (ForceFactory::make_forces, ForceFactory.cpp:61-150)."""
+        btype = str(keys.get("type", "none")).lower()
+        if btype in walls.WALL_BOXES:
+            if "lambda_wall" not in keys:
+                raise RCexception("box type '%s' needs lambda_wall" % btype)
+            w = np.ascontiguousarray(walls.wall_field(btype, L, float(keys["lambda_wall"])))
+            self._keep.append(w)
+            desc.walls = w.ctypes.data_as(C.POINTER(C.c_double))
+        if str(keys.get("external_forces", "0")).lower() in ("1", "true", "yes"):
+            if "external_forces_file" not in keys:
+                raise RCexception("external_forces = 1 needs external_forces_file")
+            with open(str(keys["external_forces_file"])) as f:
+                forces = walls.read_forces(f.read())
+            if len(forces) > _lib.RC_MAX_FORCES:
+                raise RCexception("at most %d external forces" % _lib.RC_MAX_FORCES)
+            desc.nforces = len(forces)
+            for j, fo in enumerate(forces):
+                desc.forces[j] = _lib.rc_ext_force(kind=fo["kind"], kappa_wall=fo["kappa_wall"], lambda_=fo["lambda"],
+                                                   shear_rate=fo["shear_rate"])
 
     # --- errors -----------------------------------------------------------
     def _check(self, rc: int):

@@ -36,6 +36,28 @@ def test_executable_matches_oracle(tmp_path, fc):
     assert traj.count("t = ") == 2  # steps 10 and 20
 
 
+@pytest.mark.parametrize("interaction", ["wetpolarmodel", "wetmodel"])
+def test_executable_with_walls_and_external_forces(tmp_path, interaction):
+    # box = square with type = square_walls, and an external_forces_file of the
+    # reference's format (brace blocks, # comments)
+    from rheocell_amd.walls import read_forces, wall_field
+    forces = ("# walls and a shear flow\n{\ntype = square_walls\nfield = -1\nkappa_wall = 0.2\n}\n"
+              "{\ntype = shear_flow_channel\nfield = -1\nkappa_wall = 0.1\nlambda = 3.0\nshear_rate = 0.05\n}\n")
+    (tmp_path / "external.conf").write_text(forces)
+    extra = dict(interaction_type="wetmodel", J_Q=0.1, zetaQ_self=0.3) if interaction == "wetmodel" else {}
+    conf, keys = make_case(str(tmp_path), L=96, nside=4, seed=6, steps=12, print_every=12, type="square_walls",
+                           lambda_wall=4, external_forces=1, external_forces_file="external.conf", **extra)
+    r = subprocess.run([EXE, "input"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    last = read_conf(str(tmp_path / "last_conf.dat"), len(conf.cells))
+    orc = OracleSim(conf, keys, walls=wall_field("square_walls", 96, 4.0), forces=read_forces(forces))
+    orc.step(12)
+    for i, c in enumerate(last.cells):
+        oc = orc.cell(i)
+        assert (c.LsubX, c.LsubY, c.offset, c.corner) == (oc.LsubX, oc.LsubY, tuple(oc.offset), oc.corner)
+        assert np.abs(c.phi - orc.phi(i)).max() < 1e-10
+
+
 def test_async_conf_output_matches_inline(tmp_path):
     # the configuration writer thread produces the same files, byte for byte,
     # as writing inline (async_conf_output = 0), at every print of the run
