@@ -10,9 +10,14 @@ inputs resident in HBM.
 
 Prints ONE JSON line.  Also measures, outside the timed region:
   * roofline: per-kernel device time with HIP events on the library stream;
-    achieved bandwidth of the dominant kernel from its algorithmic bytes;
+    achieved bandwidth of the dominant kernel from its algorithmic bytes, and
+    of the field-update kernels (field_update);
   * cpu_baseline: the C oracle (reference-faithful port, -O3 -ffast-math,
-    1 core) on a bounded sample of the same workload.
+    1 core) on a bounded sample of the same workload: one untimed step (it
+    holds the cold solve of a wet run), then at least 3 timed steps, in a
+    child process pinned to one core that runs beside the GPU measurement;
+  * cg_warm_start: CG time per step with the previous-solution start and the
+    learned start over 500 steps each (the 512^2 / 1600-cell comparison).
 """
 from __future__ import annotations
 
@@ -64,17 +69,19 @@ TRAFFIC_KIB = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20, help="untimed steps (also captures the dry-step graph)")
     ap.add_argument("--workload", default="wet512_1600")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--timing-steps", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU-baseline steps (after one untimed step)")
+    ap.add_argument("--cpu-baseline-only", action="store_true",
+                    help="only run the CPU baseline of the workload and print its JSON object (the child process)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
     ap.add_argument("--weights", default=os.path.join(ROOT, "rheocell_amd", "weights", "warmstart_wet256.npz"),
                     help="learned warm-start weights (wet workloads; skipped when absent)")
-    ap.add_argument("--cg-steps", type=int, default=20, help="steps of the learned-vs-previous CG comparison")
+    ap.add_argument("--cg-steps", type=int, default=500, help="steps of the learned-vs-previous CG comparison")
     ap.add_argument("--no-graphs", action="store_true", help="launch dry steps one by one instead of HIP graph replays")
     ap.add_argument("--stage-shape", default="", metavar="TxR",
                     help="force the field-update kernel shape (threads x strip rows) instead of the autotuner's choice")
@@ -93,15 +100,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(conf, keys, steps, budget_s=15.0):
-    """At least `steps` Heun steps, more while the first one leaves room in a
-    ~budget_s window (a bounded sample of the same workload)."""
+def cpu_baseline(conf, keys, steps, budget_s=12.0):
+    """One untimed Heun step (a wet run's cold solve), then at least `steps`
+    timed steps, more while they fit a ~budget_s window: a bounded sample of
+    the same workload on one core."""
     from oracle import OracleSim  # the checker, timed here as the CPU port
     try:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except Exception:
         pass
     s = OracleSim(conf, keys, flavour="fast")
+    s.step(1)
+    s0 = s.cg_stats()
     t0 = time.perf_counter()
     s.step(1)
     t1 = time.perf_counter() - t0
@@ -120,15 +130,28 @@ def cpu_baseline(conf, keys, steps, budget_s=15.0):
     except OSError:
         pass
     st = s.cg_stats()
-    return {
+    solves = st.solves - s0.solves
+    out = {
         "value": len(conf.cells) * steps / dt,
         "unit": "cell-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": "%s, first %d Heun step(s) after init, oracle/liborc_fast.so (-O3 -ffast-math), 1 thread on %s; "
-                  "%.1f s, CG iterations %d over %d solves" % (keys.get("_name", "workload"), steps, cpu, dt,
-                                                               st.iters_total, st.solves),
+        "sample": "%s: %d Heun steps timed after 1 untimed step, oracle/liborc_fast.so (-O3 -ffast-math), "
+                  "1 thread on %s; %.1f s" % (keys.get("_name", "workload"), steps, cpu, dt),
     }
+    if solves:
+        out["cg_iters_per_solve"] = round((st.iters_total - s0.iters_total) / solves, 2)
+        out["sample"] += "; %.1f CG iterations per solve over %d solves" % (out["cg_iters_per_solve"], solves)
+    return out
+
+
+def cpu_baseline_child(args):
+    """The CPU baseline in a child process (started before this process
+    touches the GPU), so that it runs beside the GPU measurement."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload", args.workload,
+           "--seed", str(args.seed), "--cpu-steps", str(args.cpu_steps)]
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, text=True)
 
 
 def cg_compare(conf, keys, weights, warmup, steps):
@@ -220,6 +243,15 @@ def main():
     if args.probe_bw:
         probe_bw()
         return
+    if args.cpu_baseline_only:
+        from rheocell_amd import confgen
+        boxes = confgen.replica_ensemble(32, seed=args.seed) if args.workload == "ensemble32" else \
+            [confgen.workload(args.workload, seed=args.seed)]
+        conf, keys = boxes[0]
+        keys["_name"] = args.workload if len(boxes) == 1 else "%s replica 0 (%d cells, %d^2 box)" % (
+            args.workload, len(conf.cells), conf.Lx)
+        print(json.dumps(cpu_baseline(conf, keys, args.cpu_steps)), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -227,6 +259,11 @@ def main():
     import numpy as np
     from rheocell_amd import confgen, _lib
     from rheocell_amd.sim import Simulation
+
+    cpu_child = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline: 1 + %d step(s) of the oracle on 1 core, in a child process" % args.cpu_steps)
+        cpu_child = cpu_baseline_child(args)
 
     log("[bench] generating %s (seed %d)" % (args.workload, args.seed))
     if args.workload == "ensemble32":
@@ -292,6 +329,7 @@ def main():
 
     roof = None
     kernels = {}
+    field_update = {}
     if not args.no_timing and args.timing_steps > 0:
         sim.enable_timing(True)
         st_a = sim.cg_stats()
@@ -320,12 +358,27 @@ def main():
             traffic = round(1024 * (tr[0] * READ_SCALE + tr[1] * WRITE_SCALE)) if tr else None
             roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                    "traffic_source": ("recorded: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and workload "
+                                       "(profiles/), x2.0 reads / x1.0 writes from a plain copy; not measured in "
+                                       "this run") if tr else None,
                     "kernel": _lib.T_SYMBOL.get(best, best), "kind": best,
                     "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_us, 3),
                     "launches": d["launches"], "useful_launches": useful,
                     "bytes_model": "%d B/row x %d rows + %d B/site x %d box sites" % (per_row, rows, per_site,
                                                                                       box_sites)}
         log("[bench] kernels: " + json.dumps({k: (round(v["ms_total"], 3), v["launches"]) for k, v in kernels.items()}))
+        # the field-update kernels (force evaluation and Heun update), each
+        # against the HBM peak with its algorithmic bytes
+        for name in ("stage_fused_pred", "stage_fused_corr", "stage_cf", "stage_fs_pred", "stage_fs_corr", "box_gather"):
+            d = kernels.get(name)
+            if not d:
+                continue
+            per_row, per_site = BYTES[name]
+            b = per_row * rows + per_site * box_sites
+            us = 1e3 * d["ms_total"] / d["launches"]
+            field_update[_lib.T_SYMBOL.get(name, name) + " " + name] = {
+                "avg_launch_us": round(us, 3), "bytes_per_launch": b,
+                "achieved_GB/s": round(b / (us * 1e-6) / 1e9, 1), "frac": round(b / (us * 1e-6) / 1e9 / PEAK_HBM_GBS, 4)}
 
     cgcmp = None
     if wet and args.weights and os.path.exists(args.weights) and args.cg_steps > 0:
@@ -334,9 +387,12 @@ def main():
         log("[bench] " + json.dumps(cgcmp))
 
     cpu = None
-    if not args.no_cpu_baseline:
-        log("[bench] CPU baseline: %d step(s) of the oracle on 1 core" % args.cpu_steps)
-        cpu = cpu_baseline(conf, keys, args.cpu_steps)
+    if cpu_child is not None:
+        log("[bench] waiting for the CPU baseline")
+        out_txt, _ = cpu_child.communicate(timeout=900)
+        if cpu_child.returncode != 0:
+            raise RuntimeError("CPU baseline child failed (%d)" % cpu_child.returncode)
+        cpu = json.loads(out_txt.strip().splitlines()[-1])
         log("[bench] CPU %.4g cell-steps/s" % cpu["value"])
 
     solves = st1.solves - st0.solves
@@ -374,6 +430,8 @@ def main():
         out["cg_iters_per_solve"] = round(iters / max(solves, 1), 2)
     if cgcmp:
         out["cg_warm_start"] = cgcmp
+    if field_update:
+        out["field_update_roofline"] = field_update
     if kernels:
         out["kernels_ms"] = {k: round(v["ms_total"], 4) for k, v in kernels.items()}
     print(json.dumps(out), flush=True)
