@@ -393,6 +393,13 @@ void FD_HIPBackend::get_settings(input_file &inp) {
 	getInputBool(&inp, "async_conf_output", &async_conf_output, 0);
 	getInputString(&inp, "lastconf_file", lastconf_file, 0);
 	getInputString(&inp, "equilibratedconf_file", equilibratedconf_file, 0);
+	// ObservableOutput prefixes every output file name with output_prefix
+	// (ObservableOutput.cpp:20-27)
+	std::string prefix;
+	getInputString(&inp, "output_prefix", prefix, 0);
+	trajectory_file = prefix + trajectory_file;
+	lastconf_file = prefix + lastconf_file;
+	equilibratedconf_file = prefix + equilibratedconf_file;
 	getInputNumber(&inp, "dt", &dt, 1);
 	getInputNumber(&inp, "J0", &J0, 0);
 	std::string wm;
@@ -466,6 +473,12 @@ void FD_HIPBackend::read_next_configuration() {
 }
 
 void FD_HIPBackend::init() {
+	// the trajectory is opened at start-up, appended to unless
+	// restart_step_counter is set, which starts it afresh (ObservableOutput.cpp:31-33, 94-99)
+	if(restart_step_counter) {
+		std::ofstream t(trajectory_file, std::ios::trunc);
+		if(!t) throw RCexception("Stream %s not writable", trajectory_file.c_str());
+	}
 	const int n = interaction.get_N_from_topology();
 	fields.clear();
 	for(int i = 0; i < n; ++i) {
@@ -678,6 +691,8 @@ ConfWriter::~ConfWriter() {
 	}
 	cv.notify_all();
 	th.join();
+	// a write error nobody collected (the run left through another exception)
+	if(!error.empty()) fprintf(stderr, "ERROR: configuration output: %s\n", error.c_str());
 }
 
 void ConfWriter::push(ConfSnapshot &&s) {
@@ -698,7 +713,12 @@ void ConfWriter::run() {
 	std::unique_lock<std::mutex> g(m);
 	for(;;) {
 		cv.wait(g, [this] { return quit || !q.empty(); });
-		if(q.empty()) return;   // quit with nothing left
+		if(!error.empty()) q.clear();   // after a write error nothing more is written (no gaps)
+		if(q.empty()) {
+			if(quit) return;   // quit with nothing left
+			cv.notify_all();
+			continue;
+		}
 		ConfSnapshot s = std::move(q.front());
 		q.pop_front();
 		busy = true;

@@ -58,6 +58,22 @@ def test_executable_with_walls_and_external_forces(tmp_path, interaction):
         assert np.abs(c.phi - orc.phi(i)).max() < 1e-10
 
 
+@pytest.mark.parametrize("restart", [1, 0])
+def test_trajectory_restart_and_output_prefix(tmp_path, restart):
+    # restart_step_counter = 1 starts the trajectory afresh, 0 appends to it
+    # (ObservableOutput.cpp:31-33, 94-99); output_prefix prefixes every name
+    make_case(str(tmp_path), L=64, nside=2, seed=3, steps=4, print_every=2, output_prefix="out_",
+              restart_step_counter=restart)
+    stale = "t = 999\nstale frame\n"
+    (tmp_path / "out_trajectory.dat").write_text(stale)
+    r = subprocess.run([EXE, "input"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    traj = (tmp_path / "out_trajectory.dat").read_text()
+    assert traj.startswith(stale) == (restart == 0)
+    assert traj.count("t = ") == 2 + (restart == 0)
+    assert (tmp_path / "out_last_conf.dat").exists() and not (tmp_path / "trajectory.dat").exists()
+
+
 def test_async_conf_output_matches_inline(tmp_path):
     # the configuration writer thread produces the same files, byte for byte,
     # as writing inline (async_conf_output = 0), at every print of the run
