@@ -94,6 +94,7 @@ def main():
     print("wrote", os.path.join(HERE, "ref_fields.json"))
     stencils()
     walls()
+    example_input()
 
 
 def stencils():
@@ -142,6 +143,18 @@ def walls():
         json.dump({"source": "reference SquareWalls/CircleWalls/TriangleWalls and BaseInteraction stencils "
                              "(oracle/build_ref.sh)", "walls": [r for r in res if r["cmd"] == "walls"], "stencilw": st},
                   f, separators=(",", ":"))
+    print("wrote", path)
+
+
+def example_input():
+    """The key = value pairs of the reference's examples/input (data: the
+    parameters its example run uses), parsed as the reference's reader does."""
+    sys.path.insert(0, ROOT)
+    from rheocell_amd.conf import read_input
+    keys = read_input("/root/reference/examples/input")
+    path = os.path.join(HERE, "example_input.json")
+    with open(path, "w") as f:
+        json.dump({"source": "key = value pairs of examples/input of the reference", "keys": keys}, f, indent=0)
     print("wrote", path)
 
 

@@ -330,3 +330,22 @@ def test_graph_replays_match_eager_steps(grow):
     # steps queued after a growth request halt and are launched again once it is served
     assert gb == 0 and (eb == 69 if not grow else eb >= 69)
     assert ga >= 1 and ga * 16 + ea >= 69
+
+
+def test_tile_list_overflow_raises():
+    # 136 patches of 3 x 3 sites (three covering each site, far below the
+    # overlap limit of 20) all intersect the first 32 x 8 box tile: more than
+    # its list holds (128), which the gather reports as RC_ERR_TILE
+    from rheocell_amd.conf import CellConf, Conf
+    L = 64
+    cells = []
+    for x0 in range(-2, 32):
+        for y0 in (-2, 1, 4, 7):
+            corner = (x0 % L) + (y0 % L) * L
+            cells.append(CellConf(3, 3, ((x0 + 1.5) % L, (y0 + 1.5) % L), (0, 0), corner, (float(x0), float(y0)),
+                                  1.0, 0.0, np.zeros(9, dtype=np.int64), np.full(9, 0.5)))
+    assert len(cells) == 136
+    sim = Simulation([(Conf(0, L, L, cells), dict(confgen.BASE_KEYS, seed=2))])
+    with pytest.raises(RCexception, match="RC_ERR_TILE"):
+        sim.step(1)
+    sim.close()

@@ -58,6 +58,29 @@ def test_executable_with_walls_and_external_forces(tmp_path, interaction):
         assert np.abs(c.phi - orc.phi(i)).max() < 1e-10
 
 
+def test_executable_runs_the_reference_example_input(tmp_path):
+    # examples/input of the reference (wetmodel, 100 cells at 140^2), with a
+    # conf of 100 cells, no equilibration, 4 steps and a tight solver
+    # tolerance so that the oracle's solve is the same to 1e-10
+    import json
+    ex = json.load(open(os.path.join(ROOT, "tests", "golden", "example_input.json")))["keys"]
+    ex = dict(ex, backend="HIP", equilibration_steps=0, steps=4, print_conf_interval=4, CGtolerance=1e-12)
+    from rheocell_amd import confgen
+    from rheocell_amd.conf import write_conf, write_input, write_topology
+    conf = confgen.lattice_conf(140, 10, seed=1)
+    with open(tmp_path / "start_conf.dat", "w") as f:
+        write_conf(conf, f)
+    write_topology(str(tmp_path / "topology.top"), 100)
+    write_input(str(tmp_path / "input"), ex)
+    r = subprocess.run([EXE, "input"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    last = read_conf(str(tmp_path / "last_conf.dat"), 100)
+    orc = OracleSim(conf, ex)
+    orc.step(4)
+    for i, c in enumerate(last.cells):
+        assert np.abs(c.phi - orc.phi(i)).max() < 1e-10
+
+
 @pytest.mark.parametrize("restart", [1, 0])
 def test_trajectory_restart_and_output_prefix(tmp_path, restart):
     # restart_step_counter = 1 starts the trajectory afresh, 0 appends to it

@@ -89,3 +89,33 @@ def test_conf_round_trip(tmp_path):
     with open(tmp_path / "again.dat", "w") as f:
         write_conf(back, f)
     assert open(tmp_path / "again.dat").read() == open(tmp_path / "start_conf.dat").read()
+
+
+REF_SRC = "/root/reference/src"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="the reference sources are not present")
+def test_fd_hip_backend_compiles_against_the_reference_headers(tmp_path):
+    # the in-tree binding (FD_HIPBackend : FDBackend) against the reference's
+    # own SimBackend / FDBackend / BaseField / BaseInteraction headers; the
+    # instance proves every pure virtual is implemented
+    src = tmp_path / "check.cpp"
+    src.write_text('#include "FD_HIPBackend.h"\nvoid make_backend() { FD_HIPBackend b; (void) b; }\n')
+    r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-iquote", os.path.join(REF_SRC, "Backends"),
+                        "-I" + REF_SRC, "-I" + os.path.join(REF_SRC, "extern"), "-I" + os.path.join(ROOT, "include"),
+                        "-I" + os.path.join(ROOT, "rheocell_amd", "host"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_reference_example_input_is_accepted():
+    # the key = value pairs of the reference's examples/input (a fixture of
+    # tests/golden/make_golden.py) map to the nematic model's parameters
+    import json
+    from rheocell_amd import _lib
+    from rheocell_amd.sim import model_params
+    keys = json.load(open(os.path.join(ROOT, "tests", "golden", "example_input.json")))["keys"]
+    p = model_params(keys)
+    assert p.interaction == _lib.RC_INTERACTION["wetmodel"]
+    assert (p.gamma, p.mu, p.lambda_, p.kappa, p.wall_slip, p.friction_cell, p.CGtolerance) == \
+        (0.06, 20.0, 2.0, 0.5, 2.0, 1.0, 1e-5)
+    assert (p.J_Q, p.zetaQ_self, p.zetaQ_inter, p.beta) == (0.1, 0.3, 0.0, 0.0)
