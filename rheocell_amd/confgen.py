@@ -80,6 +80,7 @@ WORKLOADS = {
     "dry512_1024": (512, 32, 2000, 0),   # dry, scaled up
     "wet256_400": (256, 20, 500, 1),     # wet, CG to 1e-8
     "wet512_1600": (512, 40, 500, 1),    # wet, the headline workload
+    "wet1024_6400": (1024, 80, 500, 1),  # wet, solver vectors (172 MB each) larger than the 256 MB last-level cache
 }
 
 

@@ -28,7 +28,7 @@ def test_forward_matches_oracle():
     _, ref = conv_ref.forward(x.astype(np.float64), w.astype(np.float64), b.astype(np.float64), bf=True)
     err = np.abs(got - ref).max() / np.abs(ref).max()
     print("forward rel err", err)
-    assert err < 2e-2
+    assert err < 8e-3          # measured 1.6e-3 (BF16 operands, FP32 accumulation)
 
 
 def test_gradients_match_oracle():
@@ -40,12 +40,12 @@ def test_gradients_match_oracle():
     rl, rw, rb = conv_ref.loss_and_grads(x.astype(np.float64), y.astype(np.float64), w.astype(np.float64),
                                          b.astype(np.float64), bf=True)
     print("loss", loss, rl)
-    assert abs(loss - rl) <= 1e-3 * rl
+    assert abs(loss - rl) <= 1e-4 * rl      # measured 2.4e-6
     for l in range(6):
         e = np.linalg.norm(gw[l] - rw[l]) / np.linalg.norm(rw[l])
         eb = np.linalg.norm(gb[l] - rb[l]) / max(np.linalg.norm(rb[l]), 1e-30)
         print("layer", l, "rel err w", e, "b", eb)
-        assert e < 3e-2 and eb < 3e-2
+        assert e < 3e-3 and eb < 3e-3     # measured <= 5.8e-4 (layer 0)
     mw, mb = conv_ref.masks()
     assert np.all(gw[mw == 0] == 0) and np.all(gb[mb == 0] == 0)
 
