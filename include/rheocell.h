@@ -246,6 +246,14 @@ int rc_set_stage_shape(rc_handle *h, int threads, int rows);
  * valid shape in microseconds (0 before tuning).  Any output may be null. */
 int rc_stage_groups(rc_handle *h, int max_groups, int *ngroups, int (*shape)[3], int (*valid)[8][2],
 		double (*tune_us)[8]);
+/* The dry fused stage (no J_Q, omega, beta; periodic boxes; patch sides of at
+ * most 3 strip rows at 1024 threads) can instead run as one persistent launch
+ * over every cell of every box, one 1024-thread workgroup per CU with the
+ * next cell's loads in flight.  The tuner times it against the groups and
+ * keeps the faster; rc_set_stage_shape(h, 1024, 0) forces it, any other shape
+ * the groups.  Reports whether the layout allows it, whether it is used, and
+ * the tuner's times of both (predictor stage, us; 0 if not timed). */
+int rc_stage_persistent(rc_handle *h, int *eligible, int *used, double *tune_us_persistent, double *tune_us_groups);
 
 /* ---- HIP graphs (no counterpart in the reference) -----------------------
  * Dry-model steps are replayed from a HIP graph of 16 steps (captured once per

@@ -34,7 +34,7 @@ EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_a
            "rc_net_create", "rc_net_set_params", "rc_net_get_params", "rc_net_forward", "rc_net_grads",
            "rc_net_train_step", "rc_net_last_error", "rc_net_destroy", "rc_attach_net", "rc_snapshot_box",
            "rc_record_solves", "rc_get_recorded", "rc_set_learned_stages", "rc_set_stage_shape",
-           "rc_stage_groups", "rc_set_graphs", "rc_graph_stats",
+           "rc_stage_groups", "rc_stage_persistent", "rc_set_graphs", "rc_graph_stats",
            "rc_growth_stats", "rc_box_field", "rc_box_field_noalloc"]
 
 
@@ -138,6 +138,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.rc_graph_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.rc_stage_groups.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]
+    L.rc_stage_persistent.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L

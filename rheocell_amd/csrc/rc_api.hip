@@ -117,6 +117,13 @@ struct rc_handle {
 	double growth_ms = 0.;          // host time spent serving them
 	int stage_forced[2] = { 0, 0 };     // rc_set_stage_shape; {0, 0}: autotune
 	bool stage_tuned = false;
+	// the dry fused stage as one persistent launch over every cell (k_stage_p)
+	// instead of one launch per group: eligible layouts, the choice, and the
+	// tuner's times of both (predictor stage, us)
+	bool persist_ok = false, persist = false;
+	int p_npt = 0, p_tile = 0, p_rows = 0, p_sites = 0;
+	int ncu = 0;                  // compute units of the device (k_stage_p grid)
+	double persist_us = 0., groups_us = 0.;
 	int cur_set = 0;              // parameter set currently applied
 	int cf_set = 0;               // set of the force evaluation preceding the next field update
 	int restart_solver = 0;
@@ -299,6 +306,7 @@ static std::mutex scratch_mu;
 // once per device, with that device current (called from rc_create)
 static int scratch_setup(rc_handle *h) {
 	HIPCHK(h, hipGetDevice(&h->device));
+	HIPCHK(h, hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device));
 	if(h->device < 0 || h->device >= SCRATCH_DEVICES) return RC_OK;   // rc_box_field_noalloc refuses such a device
 	std::lock_guard<std::mutex> lock(scratch_mu);
 	if(!scratch_ptr[h->device]) {
@@ -689,8 +697,36 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 				if(g.threads != 512 || g.rows != 8)
 					return fail(h, RC_ERR_ARG, "a %dx%d patch does not fit the walls variant of the stage kernel (512 x 8)",
 							g.mx, g.my);
-		bool tunable = false;
+		// the persistent form (k_stage_p): dry fused stages in periodic boxes
+		// without J_Q, omega and beta, patches of at most 3 strip rows at 1024
+		// threads, and LDS for the cells of one workgroup
+		const bool was_persist = h->persist;
+		h->persist_ok = !h->split && !h->jq && !h->win && !h->ext && h->ncu > 0 && h->nboxes <= 4096;
+		for(const DevParams &p : h->dparams) h->persist_ok = h->persist_ok && p.beta == 0 && p.omega == 0;
+		h->p_npt = 2;
+		h->p_tile = h->p_rows = h->p_sites = 0;
+		for(int i = 0; i < h->ncells && h->persist_ok; ++i) h->p_npt = std::max(h->p_npt, p_rows_of(dc[i].LX, dc[i].LY));
+		if(h->p_npt > P_STRIP_ROWS) h->persist_ok = false;
+		for(int i = 0; i < h->ncells && h->persist_ok; ++i) {
+			h->p_tile = std::max(h->p_tile, p_tile_of(dc[i].LX, dc[i].LY, h->p_npt));
+			h->p_rows = std::max(h->p_rows, dc[i].LY + h->p_npt);
+			h->p_sites = std::max(h->p_sites, dc[i].LX * dc[i].LY);
+		}
+		if(h->persist_ok) {
+			StageArgs pa{};
+			pa.p_tile = h->p_tile;
+			pa.p_rows = h->p_rows;
+			pa.p_sites = h->p_sites;
+			pa.p_nbox = h->nboxes;
+			pa.store = 1;
+			const int grid = std::min(h->ncells, h->ncu);
+			if(stage_p_lds_bytes(pa, (h->ncells + grid - 1) / grid) > STAGE_LDS_MAX - 512) h->persist_ok = false;   // + k_stage_p's static LDS
+		}
+		bool tunable = h->persist_ok && h->ncells >= P_MIN_CELLS_PER_CU * h->ncu;
 		for(const auto &g : h->groups) tunable = tunable || g.valid.size() > 1;
+		if(h->stage_forced[0] == P_THREADS) h->persist = h->persist_ok;   // forced; the groups after a growth it no longer fits
+		else if(h->stage_forced[0] != 0) h->persist = false;
+		else h->persist = h->persist_ok && all_kept && was_persist;
 		h->stage_tuned = !tunable || h->stage_forced[0] != 0 || all_kept;
 		h->graph_valid = false;
 	}
@@ -774,11 +810,27 @@ static int launch_groups(rc_handle *h, int mode, const StageArgs &a0) {
 	return RC_OK;
 }
 
+// StageArgs of k_stage_p: the layout's tile, row table and patch bounds
+static StageArgs persist_args(rc_handle *h, const StageArgs &a0) {
+	StageArgs a = a0;
+	a.npt = h->p_npt;
+	a.p_tile = h->p_tile;
+	a.p_rows = h->p_rows;
+	a.p_sites = h->p_sites;
+	a.p_nbox = h->nboxes;
+	a.cell_list = nullptr;
+	return a;
+}
+
 static int run_stage(rc_handle *h, int mode, const StageArgs &a) {
 	const int kind = mode == MODE_CF ? RC_T_STAGE_CF
 			: mode == MODE_FUSED ? (a.store ? RC_T_STAGE_FUSED_PRED : RC_T_STAGE_FUSED_CORR)
 			: (a.store ? RC_T_STAGE_FS_PRED : RC_T_STAGE_FS_CORR);
-	{
+	if(mode == MODE_FUSED && h->persist) {
+		TimeScope ts(h, kind);
+		HIPCHK(h, launch_stage_p(persist_args(h, a), h->ncells, h->ncu, h->stream));
+	}
+	else {
 		TimeScope ts(h, kind, h->groups.size() > 1);   // several launches are timed as one
 		int rc = launch_groups(h, mode, a);
 		if(rc) return rc;
@@ -1434,6 +1486,39 @@ static int autotune_stage(rc_handle *h) {
 		g.threads = pick.first;
 		g.rows = pick.second;
 	}
+	// the persistent form against the groups with their chosen shapes
+	h->persist = false;
+	h->persist_us = h->groups_us = 0.;
+	if(h->persist_ok && h->ncells >= P_MIN_CELLS_PER_CU * h->ncu) {
+		StageArgs a = stage_args(h);
+		a.cf_set = h->cf_set;
+		a.fs_set = h->cur_set;
+		a.store = 1;
+		a.pure = 1;
+		a.write_v = 0;
+		a.phi_c = h->d_phiA;
+		a.phi_out = h->d_phiB;
+		a.gauss = nullptr;
+		for(int form = 0; form < 2; ++form) {
+			auto launch = [&]() -> int {
+				if(form == 1) {
+					HIPCHK(h, launch_stage_p(persist_args(h, a), h->ncells, h->ncu, h->stream));
+					return RC_OK;
+				}
+				return launch_groups(h, MODE_FUSED, a);
+			};
+			if((rc = launch())) return rc;
+			HIPCHK(h, hipEventRecord(e0, h->stream));
+			for(int r = 0; r < 3; ++r)
+				if((rc = launch())) return rc;
+			HIPCHK(h, hipEventRecord(e1, h->stream));
+			HIPCHK(h, hipEventSynchronize(e1));
+			float ms = 0.f;
+			HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+			(form == 1 ? h->persist_us : h->groups_us) = 1e3 * ms / 3;
+		}
+		h->persist = h->ncells >= P_MIN_CELLS_PER_CU * h->ncu && h->persist_us < h->groups_us;
+	}
 	h->timing = timing;
 	h->graph_valid = false;
 	return check_status(h);
@@ -1450,6 +1535,16 @@ extern "C" int rc_set_stage_shape(rc_handle *h, int threads, int rows) {
 		h->graph_valid = false;
 		return RC_OK;
 	}
+	if(threads == P_THREADS) {
+		if(!h->persist_ok || (rows != 0 && rows != h->p_npt))
+			return fail(h, RC_ERR_ARG, "the persistent stage kernel does not fit this layout or its parameters");
+		h->persist = true;
+		h->stage_forced[0] = threads;
+		h->stage_forced[1] = rows;
+		h->stage_tuned = true;
+		h->graph_valid = false;
+		return RC_OK;
+	}
 	bool any = false;
 	for(auto &g : h->groups)
 		for(const auto &v : g.valid)
@@ -1459,6 +1554,7 @@ extern "C" int rc_set_stage_shape(rc_handle *h, int threads, int rows) {
 				any = true;
 			}
 	if(!any) return fail(h, RC_ERR_ARG, "stage shape %d threads x %d rows fits no patch group", threads, rows);
+	h->persist = false;
 	h->stage_forced[0] = threads;
 	h->stage_forced[1] = rows;
 	h->stage_tuned = true;
@@ -1486,6 +1582,16 @@ extern "C" int rc_stage_groups(rc_handle *h, int max_groups, int *ngroups, int (
 			if(tune_us) tune_us[k][i] = in ? g.tune_us[i] : 0.;
 		}
 	}
+	return RC_OK;
+}
+
+extern "C" int rc_stage_persistent(rc_handle *h, int *eligible, int *used, double *tune_us_persistent,
+		double *tune_us_groups) {
+	if(!h) return RC_ERR_ARG;
+	if(eligible) *eligible = h->persist_ok ? 1 : 0;
+	if(used) *used = h->persist ? 1 : 0;
+	if(tune_us_persistent) *tune_us_persistent = h->persist_us;
+	if(tune_us_groups) *tune_us_groups = h->groups_us;
 	return RC_OK;
 }
 

@@ -41,6 +41,8 @@ struct StageArgs {
 	const unsigned char *wmask;
 	const double *walls;
 	int win;                     // LDS holds the box window of sum phi^2 (some omega != 0)
+	int p_tile, p_rows, p_sites; // k_stage_p: {phi, mu} tile capacity (double2), row-table length, largest patch
+	int p_nbox;                  // k_stage_p: boxes (their parameters are staged in LDS)
 	DevStatus *st;
 };
 
@@ -134,6 +136,26 @@ inline size_t stage_lds_bytes(int LX, int LY, int npt, bool jq, bool win) {
 	return b;
 }
 hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds_bytes, hipStream_t s);
+
+// The dry fused stage as one persistent launch over every cell (k_stage_p):
+// grid workgroups of 1024 threads, one per CU, each walking a contiguous range
+// of cells with the next cell's loads in flight.  For parameter sets without
+// J_Q, omega and beta, periodic boxes, and patches whose strips of 1024
+// threads have at most P_STRIP_ROWS rows (p_rows_of); StageArgs::npt is the
+// launch's strip rows (2 or 3), at least every cell's.
+constexpr int P_THREADS = 1024, P_STRIP_ROWS = 3;
+// the tuner considers it only with at least this many cells per CU (its
+// overlap needs several cells per workgroup); below, results do not depend
+// on timing: the groups run
+constexpr int P_MIN_CELLS_PER_CU = 2;
+inline int p_rows_of(int LX, int LY) { return LX >= 1 && LX <= P_THREADS ? (LY + P_THREADS / LX - 1) / (P_THREADS / LX) : 1 << 20; }
+// tile capacity (double2) of one patch at npt strip rows; LDS of a launch (StageArgs p_*, store)
+inline int p_tile_of(int LX, int LY, int npt) { return (LY + npt) * (LX + 2); }
+inline size_t stage_p_lds_bytes(const StageArgs &a, int cells_per_wg) {
+	return 16 * (size_t) a.p_tile + 16 * (size_t) a.p_rows + 8 * (size_t) a.p_sites * (a.store ? 2 : 1) +
+			8 * 12 * 2 * (size_t) a.p_nbox + 8 * (size_t) (a.p_nbox + 1) + 64 * (size_t) cells_per_wg + 8 * (size_t) cells_per_wg;
+}
+hipError_t launch_stage_p(const StageArgs &a, int ncells, int grid, hipStream_t s);
 
 // WetModel::f_interaction + calc_internal_forces (WetModel.cpp:488-641) of
 // every cell, one workgroup per cell: the stress-divergence passive force and

@@ -1202,6 +1202,573 @@ hipError_t launch_stage(int mode, const StageArgs &a, int ncells, size_t lds, hi
 }
 
 // ---------------------------------------------------------------------------
+// k_stage_p<ST, RCP>: the dry fused stage (force evaluation and Heun update of
+// k_stage<MODE_FUSED>, for parameter sets without the Repulsion/CIL sums,
+// adhesion or perimeter terms, in periodic boxes) as one persistent launch
+// over every cell of every replica.  One 1024-thread workgroup per CU walks a
+// contiguous range of cells (the ranges of one XCD are contiguous, as with
+// xcd_item), software-pipelined over its cells:
+//   * the next cell's strips (phi column with the rows above and below, the
+//     box sums of phi^2, phi_old and dphi_old, the trig values of its rows
+//     and of the thread's column) are loaded into a second register set while
+//     the current cell is evaluated, and the geometry of the cell after that
+//     one a step earlier still;
+//   * the updated field (and dphi) is kept in LDS and written to the rows at
+//     the start of the next cell, coalesced, after that cell's loads have been
+//     issued, so that no wait for the stores stands before them;
+//   * the per-cell tails (CoM atan2, polarity, recentering or the growth
+//     request) of all the workgroup's cells run at the end, four lanes per
+//     cell, from the per-cell sums kept in LDS.
+// The only wait on memory in a cell's evaluation is the one for its own
+// prefetched values, at the top.  Per-site expressions, operand order and the
+// per-cell sums are those of k_stage<MODE_FUSED>; the parameters of every box
+// are staged in LDS once.  Strips as in k_stage with CT = 1024 threads per
+// cell and at most PNPT rows per thread (p_fits, checked by the host).
+// ---------------------------------------------------------------------------
+constexpr int PCT = P_THREADS;
+constexpr int PPAR = 12;   // parameters staged per box (PPar order)
+enum PPar { PP_GAMMA, PP_LAMBDA, PP_LAMBDA_RCP, PP_MU_A0, PP_A0, PP_RCOEF, PP_ALPHA, PP_ZETA_SELF, PP_FRICTION,
+	PP_FRICTION_RCP, PP_J0, PP_DT };
+
+struct PGeoRaw {   // a cell's geometry and the values its evaluation reads, as loaded (vector registers)
+	int4 g0;       // corner, LX | LY << 16, offx | offy << 16, start
+	int4 g1;       // box, trig_x, trig_y, pad
+	double area, Q00, Q01;
+};
+struct PGeo {      // the same, uniform
+	int LX, LY, start, offx, offy, corner, box, trig_x, trig_y, L, box_off;
+	double area, Q00, Q01;
+};
+
+// vector loads of one address: kept out of the scalar cache, whose counter is
+// shared with LDS and would make the next LDS wait also wait for these loads
+__device__ inline PGeoRaw p_load_geo(const StageArgs &a, int ci) {
+	int vi = ci;
+	asm volatile("" : "+v"(vi));
+	PGeoRaw r;
+	r.g0 = a.geo[vi];
+	r.g1 = *(const int4 *) ((const int *) (a.cells + vi) + offsetof(DevCell, box) / 4);
+	const double *dp = (const double *) (a.cells + vi);
+	r.area = dp[offsetof(DevCell, area) / 8];
+	r.Q00 = dp[offsetof(DevCell, Q00) / 8];
+	r.Q01 = dp[offsetof(DevCell, Q01) / 8];
+	return r;
+}
+__device__ inline double rfl_f64(double v) {
+	const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+	const unsigned lo = (unsigned) __builtin_amdgcn_readfirstlane((int) (unsigned) u);
+	const unsigned hi = (unsigned) __builtin_amdgcn_readfirstlane((int) (unsigned) (u >> 32));
+	return __builtin_bit_cast(double, ((unsigned long long) hi << 32) | lo);
+}
+__device__ inline PGeo p_geo(const PGeoRaw &r, const int2 *s_box) {
+	PGeo g;
+	g.corner = __builtin_amdgcn_readfirstlane(r.g0.x);
+	const int lxy = __builtin_amdgcn_readfirstlane(r.g0.y), off = __builtin_amdgcn_readfirstlane(r.g0.z);
+	g.LX = lxy & 0xFFFF;
+	g.LY = lxy >> 16;
+	g.offx = off & 0xFFFF;
+	g.offy = off >> 16;
+	g.start = __builtin_amdgcn_readfirstlane(r.g0.w);
+	g.box = __builtin_amdgcn_readfirstlane(r.g1.x);
+	g.trig_x = __builtin_amdgcn_readfirstlane(r.g1.y);
+	g.trig_y = __builtin_amdgcn_readfirstlane(r.g1.z);
+	const int2 bb = s_box[g.box];
+	g.L = __builtin_amdgcn_readfirstlane(bb.x);
+	g.box_off = __builtin_amdgcn_readfirstlane(bb.y);
+	g.area = rfl_f64(r.area);
+	g.Q00 = rfl_f64(r.Q00);
+	g.Q01 = rfl_f64(r.Q01);
+	return g;
+}
+
+// this thread's strip of a cell (k_stage's column strips, CT = PCT)
+struct PStrip {
+	int R, qx, qy0, nr, lx;
+	bool idle;
+};
+// n / d for 0 <= n < 2048 and 2 <= d <= 1024 with a multiply-high: m =
+// floor((2^32 - 1) / d) + 1 exceeds 2^32 / d by e in (0, 1], so n m / 2^32 =
+// n / d + n e / 2^32 with n e / 2^32 < 2^-21 < 1 / d: the floor is exact
+__device__ inline unsigned p_magic(unsigned d) { return 0xFFFFFFFFu / d + 1u; }
+__device__ inline int p_div(int n, unsigned m) { return (int) __umulhi((unsigned) n, m); }
+
+__device__ inline PStrip p_strip(int LX, int LY, int offx) {
+	PStrip t;
+	const int tid = threadIdx.x;
+	const unsigned mx = p_magic((unsigned) LX);   // uniform
+	t.R = strip_rows(LX, LY, PCT);
+	const int blk = p_div(tid, mx);
+	t.qx = tid - blk * LX;
+	t.qy0 = blk * t.R;
+	t.idle = t.qy0 >= LY || blk >= PCT / LX;
+	if(t.idle) t.qy0 = 0;
+	t.nr = t.idle ? 0 : min(t.R, LY - t.qy0);
+	t.lx = t.qx + offx;
+	if(t.lx >= LX) t.lx -= LX;
+	return t;
+}
+
+// the values a cell's evaluation reads, per thread
+template <int PNPT>
+struct PData {
+	double col[PNPT + 2], p2[PNPT], pold[PNPT], dold[PNPT];
+	double2 trow;     // {cos y, sin y} of tile row threadIdx.x (threads < LY + PNPT)
+	double2 trx;      // {cos x, sin x} of this thread's column
+};
+
+template <bool ST, int PNPT>
+__device__ inline void p_load(const StageArgs &a, const PGeo &g, PData<PNPT> &d) {
+	const int tid = threadIdx.x, LX = g.LX, LY = g.LY, L = g.L;
+	const PStrip t = p_strip(LX, LY, g.offx);
+	const int cx = g.corner % L, cy = g.corner / L;
+	const double *phic = a.phi_c + g.start + t.qx;
+#pragma unroll
+	for(int k = 0; k < PNPT + 2; ++k) {
+		int qy = t.qy0 + k - 1;
+		if(qy < 0) qy += LY;
+		if(qy >= LY) qy -= LY;
+		if(qy >= LY) qy -= LY;
+		d.col[k] = phic[qy * LX];
+	}
+	const int bcol = g.box_off + (cx + t.lx >= L ? cx + t.lx - L : cx + t.lx);
+	const double *phi2b = a.phi2 + bcol;
+#pragma unroll
+	for(int k = 0; k < PNPT; ++k) {
+		int qk = t.qy0 + k;
+		if(qk >= LY) qk -= LY;
+		if(qk >= LY) qk = 0;
+		int ly = qk + g.offy;
+		if(ly >= LY) ly -= LY;
+		int by = cy + ly;
+		if(by >= L) by -= L;
+		d.p2[k] = phi2b[by * L];
+	}
+	if(!ST) {
+		// rows past the strip's own (k >= nr) read row qy0 instead and are discarded
+		const int row0 = g.start + t.qy0 * LX + t.qx;
+#pragma unroll
+		for(int k = 0; k < PNPT; ++k) {
+			const int rk = k < t.nr ? row0 + k * LX : row0;
+			d.pold[k] = a.phi_old[rk];
+			d.dold[k] = a.dphi_old[rk];
+		}
+	}
+	else {
+#pragma unroll
+		for(int k = 0; k < PNPT; ++k) d.pold[k] = d.dold[k] = 0.;
+	}
+	int ly = tid - p_div(tid, p_magic((unsigned) LY)) * LY + g.offy;
+	if(ly >= LY) ly -= LY;
+	d.trow = make_double2(a.trig[g.trig_y + ly], a.trig[g.trig_y + LY + ly]);
+	d.trx = make_double2(a.trig[g.trig_x + t.lx], a.trig[g.trig_x + LX + t.lx]);
+}
+
+// LDS regions of k_stage_p
+struct PLds {
+	double2 *pm, *trow;
+	double *phn, *dph, *par, *sum, *red;
+	int2 *box;
+	int *min, *redi;
+};
+
+// one cell of k_stage_p: evaluates cell ci (geometry g, values cur) and, once
+// its tile is in LDS, requests the values of cell cn (into gn / nxt, from the
+// geometry rn loaded one cell earlier) and the geometry of cell cn2 (into rn),
+// then writes the update staged by the previous cell and stages its own
+template <bool ST, bool RCP, int PNPT>
+__device__ __forceinline__ void p_cell(const StageArgs &a, const PLds &L_, int ci, int j, int cn, int cn2,
+		const PGeo &g, const PData<PNPT> &cur, PGeo &gn, PData<PNPT> &nxt, PGeoRaw &rn, int &prev_start, int &prev_S) {
+	constexpr bool store = ST;
+	constexpr int NW = PCT / 64;
+	const int tid = threadIdx.x;
+	const int LX = g.LX, LY = g.LY, PW = LX + 2;
+	const PStrip t = p_strip(LX, LY, g.offx);
+	const int R = t.R, qx = t.qx, qy0 = t.qy0, nr = t.nr, lx = t.lx;
+	const bool idle = t.idle;
+	double2 *const s_pm = L_.pm;
+	// the tile: rows 0 .. LY - 1 and, past them, images of rows 0 .. PNPT - 1
+	double2 *pm = s_pm + qy0 * PW + qx + 1;
+	if(tid < LY + PNPT) L_.trow[tid] = cur.trow;
+	if(!idle) {
+#pragma unroll
+		for(int k = 0; k < PNPT; ++k) {
+			if(k >= R) break;
+			const double v = cur.col[k + 1];
+			double2 *e = pm + k * PW;
+			e->x = v;
+			if(qx == 0) e[LX].x = v;
+			if(qx == LX - 1) e[-LX].x = v;
+			if(qy0 + k < PNPT) {
+				double2 *im = e + LY * PW;
+				im->x = v;
+				if(qx == 0) im[LX].x = v;
+				if(qx == LX - 1) im[-LX].x = v;
+			}
+		}
+	}
+	__syncthreads();
+	// the next cell's loads, then the previous cell's staged update
+	gn = p_geo(rn, L_.box);
+	rn = p_load_geo(a, cn2);
+	p_load<ST, PNPT>(a, gn, nxt);
+	for(int qq = tid; qq < prev_S; qq += PCT) {
+		a.phi_out[prev_start + qq] = L_.phn[qq];
+		if(ST) a.dphi_old[prev_start + qq] = L_.dph[qq];
+	}
+	(void) cn;
+
+	// chemical potential (f_interaction, WetPolarModel.cpp:262-301) without
+	// the adhesion and perimeter terms (omega = beta = 0: both exactly +0)
+	const double *P = L_.par + PPAR * g.box;
+	const double Acoef = -4 * P[PP_MU_A0] * (1 - g.area / P[PP_A0]);
+	const double Rcoef = P[PP_RCOEF];
+	const double lambda = P[PP_LAMBDA];
+	const double lam2 = 2 * lambda;
+	double mu[PNPT];
+#pragma unroll
+	for(int k = 0; k < PNPT; ++k) {
+		mu[k] = 0.;
+		if(k >= R) break;
+		const double2 *e = pm + k * PW;
+		const double ph = e->x;
+		const double pE = e[1].x, pW = e[-1].x, pN = e[PW].x;
+		const double pS = qy0 + k == 0 ? s_pm[(LY - 1) * PW + qx + 1].x : e[-PW].x;
+		const double lap = pN + pS + pE + pW - 4 * ph;
+		const double p2 = cur.p2[k];
+		const double CH = P[PP_GAMMA] * (div_by<RCP>(8 * ph * (1 - ph) * (1 - 2 * ph), lambda, P[PP_LAMBDA_RCP]) - lam2 * lap);
+		const double A = Acoef * ph;
+		const double Rep = Rcoef * ph * (p2 - ph * ph);
+		const double Adh = 0., Pp = 0.;
+		const double V = CH + A + Rep + Adh + Pp;
+		mu[k] = V;
+		if(!idle) {
+			double2 *w = pm + k * PW;
+			w->y = V;
+			if(qx == 0) w[LX].y = V;
+			if(qx == LX - 1) w[-LX].y = V;
+			if(qy0 + k < PNPT) {
+				double2 *im = w + LY * PW;
+				im->y = V;
+				if(qx == 0) im[LX].y = V;
+				if(qx == LX - 1) im[-LX].y = V;
+			}
+		}
+	}
+	__syncthreads();
+
+	// forces (calc_internal_forces) and the Heun update (first_step); the
+	// update is staged in LDS (storage order) and written with the next cell
+	const double alpha = P[PP_ALPHA], zself = P[PP_ZETA_SELF], fric = P[PP_FRICTION], frcp = P[PP_FRICTION_RCP];
+	const double J0 = P[PP_J0], dt = P[PP_DT];
+	double red[8];
+#pragma unroll
+	for(int jj = 0; jj < 8; ++jj) red[jj] = 0.0;
+	int ymin = LY;
+	bool edge = false;
+	double muN, muS;
+	{
+		int qa = qy0 + R;
+		if(qa >= LY) qa -= LY;
+		if(qa >= LY) qa -= LY;
+		int qb = qy0 - 1;
+		if(qb < 0) qb += LY;
+		muN = s_pm[qa * PW + qx + 1].y;
+		muS = s_pm[qb * PW + qx + 1].y;
+	}
+#pragma unroll
+	for(int k = 0; k < PNPT; ++k) {
+		if(k >= R) break;
+		const bool valid = k < nr;
+		const double2 *e = pm + k * PW;
+		const double ph = e->x;
+		const double pN = e[PW].x;
+		const double pS = qy0 + k == 0 ? s_pm[(LY - 1) * PW + qx + 1].x : e[-PW].x;
+		const double2 eE = e[1], eW = e[-1];
+		const double dx = .5 * (eE.x - eW.x);
+		const double dy = .5 * (pN - pS);
+		const double mN = k + 1 < R ? mu[k + 1] : muN;
+		const double mS = k > 0 ? mu[k - 1] : muS;
+		const double f_passive_x = (-1) * 0.5 * (eE.y - eW.y);
+		const double f_passive_y = (-1) * 0.5 * (mN - mS);
+		double Theta = (-dx * g.Q00) < 0 ? 0.0 : 1.0;
+		const double fQx = lambda * (-dx * g.Q00) * Theta * g.Q00;
+		Theta = (-dy * g.Q01) < 0 ? 0.0 : 1.0;
+		const double fQy = lambda * (-dy * g.Q01) * Theta * g.Q01;
+		const double fx = f_passive_x * alpha + fQx * zself;
+		const double fy = f_passive_y * alpha + fQy * zself;
+		const double vx = div_by<RCP>(fx, fric, frcp);
+		const double vy = div_by<RCP>(fy, fric, frcp);
+		const double dphi = -J0 * mu[k] - vx * dx - vy * dy;
+		double phn;
+		if(store) phn = ph + dt * .5 * (dphi + dphi);
+		else phn = cur.pold[k] + dt * .5 * (dphi + cur.dold[k]);
+		if(valid) {
+			const int qs = (qy0 + k) * LX + qx;
+			L_.phn[qs] = phn;
+			if(ST) L_.dph[qs] = dphi;
+			const double2 tr = L_.trow[qy0 + k];
+			red[0] = fma(phn, phn, red[0]);
+			red[1] += phn;
+			red[4] = fma(phn, tr.x, red[4]);
+			red[5] = fma(phn, tr.y, red[5]);
+			if(!store && phn < 0.09 && phn > 0.01) {
+				edge = true;
+				int ly = qy0 + k + g.offy;
+				if(ly >= LY) ly -= LY;
+				ymin = min(ymin, ly);
+			}
+			red[6] += -0.5 * (dx * dx - dy * dy);
+			red[7] += -dx * dy;
+		}
+	}
+	const int xmin = edge ? lx : LX;
+	if(nr > 0) {
+		red[2] = red[1] * cur.trx.x;
+		red[3] = red[1] * cur.trx.y;
+	}
+	{
+		const int lane = tid & 63, wave = tid >> 6;
+		const int mx = wave_min(xmin), my = wave_min(ymin);
+		if(lane == 0) {
+			L_.redi[wave] = mx;
+			L_.redi[NW + wave] = my;
+		}
+	}
+	const double tot = block_sum8<NW>(red, L_.red);
+	if(tid < 8) L_.sum[8 * j + tid] = tot;
+	if(tid == 0) {
+		int xsl = LX, ysb = LY;
+		for(int w = 0; w < NW; ++w) {
+			xsl = min(xsl, L_.redi[w]);
+			ysb = min(ysb, L_.redi[NW + w]);
+		}
+		L_.min[2 * j] = xsl;
+		L_.min[2 * j + 1] = ysb;
+	}
+	prev_start = g.start;
+	prev_S = LX * LY;
+	(void) ci;
+}
+
+template <bool ST, bool RCP, int PNPT>
+__global__ __launch_bounds__(PCT, 1) void k_stage_p(StageArgs a, int ncells) {
+	constexpr bool store = ST;
+	const int step = a.step + (a.step_base ? *a.step_base : 0);
+	{   // as k_stage: a growth request of this same corrector does not halt it
+		const unsigned f = a.st->flags;
+		if(f) {
+			const bool same_step = !store && (f & ST_KIND_MASK) == ST_GROWTH && (f >> ST_STEP_SHIFT) == (unsigned) (step + 1);
+			if(!same_step) return;
+		}
+	}
+	constexpr int NW = PCT / 64;
+	extern __shared__ double2 smem2[];
+	__shared__ double s_red[NW * 4 * 8];
+	__shared__ int s_redi[2 * NW];
+	const int tid = threadIdx.x;
+	const int q = xcd_item(blockIdx.x, gridDim.x);
+	const int c0 = (int) ((long long) q * ncells / gridDim.x), c1 = (int) ((long long) (q + 1) * ncells / gridDim.x);
+	if(c0 >= c1) return;
+	// LDS: the {phi, mu} tile (a.p_tile double2), the row table (a.p_rows),
+	// the staged field update and dphi (a.p_sites each), the boxes'
+	// parameters and {L, box_off}, then per cell of the range 8 sums and the
+	// interface box minima
+	PLds L_;
+	L_.pm = smem2;
+	L_.trow = L_.pm + a.p_tile;
+	L_.phn = (double *) (L_.trow + a.p_rows);
+	L_.dph = L_.phn + a.p_sites;
+	L_.par = L_.dph + (ST ? a.p_sites : 0);
+	L_.box = (int2 *) (L_.par + PPAR * 2 * a.p_nbox);
+	L_.sum = (double *) (L_.box + a.p_nbox + (a.p_nbox & 1));
+	L_.min = (int *) (L_.sum + 8 * (c1 - c0));
+	L_.red = s_red;
+	L_.redi = s_redi;
+	double *s_sum = L_.sum;
+	int *s_min = L_.min;
+	for(int i = tid; i < a.p_nbox; i += PCT) {
+		const DevBox B = a.boxes[i];
+		L_.box[i] = make_int2(B.L, B.box_off);
+		const DevParams &P = a.params[2 * i + a.cf_set];
+		const DevParams &Q = a.params[2 * i + a.fs_set];
+		double *w = L_.par + PPAR * i;
+		w[PP_GAMMA] = P.gamma;
+		w[PP_LAMBDA] = P.lambda;
+		w[PP_LAMBDA_RCP] = P.lambda_rcp;
+		w[PP_MU_A0] = P.mu_a0;
+		w[PP_A0] = P.a0;
+		w[PP_RCOEF] = P.rcoef;
+		w[PP_ALPHA] = P.passive_alpha;
+		w[PP_ZETA_SELF] = P.zetaQ_self;
+		w[PP_FRICTION] = P.friction;
+		w[PP_FRICTION_RCP] = P.friction_rcp;
+		w[PP_J0] = Q.J0;
+		w[PP_DT] = Q.dt;
+	}
+	__syncthreads();
+
+	// two register sets, A and B, alternate between the cell being evaluated
+	// and the next one being loaded; cells past the range reload the last one
+	const int last = c1 - 1;
+	PGeoRaw rn = p_load_geo(a, c0);
+	PGeo gA = p_geo(rn, L_.box), gB;
+	rn = p_load_geo(a, min(c0 + 1, last));
+	PData<PNPT> dA, dB;
+	p_load<ST, PNPT>(a, gA, dA);
+	int prev_start = 0, prev_S = 0;   // the cell whose update is staged in LDS
+	for(int ci = c0; ci < c1; ci += 2) {
+		p_cell<ST, RCP, PNPT>(a, L_, ci, ci - c0, min(ci + 1, last), min(ci + 2, last), gA, dA, gB, dB, rn, prev_start, prev_S);
+		if(ci + 1 >= c1) break;
+		p_cell<ST, RCP, PNPT>(a, L_, ci + 1, ci + 1 - c0, min(ci + 2, last), min(ci + 3, last), gB, dB, gA, dA, rn, prev_start,
+				prev_S);
+	}
+	__syncthreads();
+	for(int qq = tid; qq < prev_S; qq += PCT) {
+		a.phi_out[prev_start + qq] = L_.phn[qq];
+		if(ST) a.dphi_old[prev_start + qq] = L_.dph[qq];
+	}
+	if(a.pure) return;   // timing run (shape tuner): no cell state is written
+
+	// ---- per-cell tails, four lanes per cell (k_stage's tail) ----
+	const int sub = tid & 3, base = (tid & 63) & ~3;
+	for(int j0 = 0; j0 < c1 - c0; j0 += PCT / 4) {
+		const int j = j0 + (tid >> 2);
+		const bool on = j < c1 - c0;
+		const int ci = on ? c0 + j : c0;
+		const DevCell C = a.cells[ci];
+		const DevBox B = a.boxes[C.box];
+		const DevParams &Q = a.params[2 * C.box + a.fs_set];
+		const int LX = C.LX, LY = C.LY, L = B.L;
+		const double *sm = s_sum + 8 * (ci - c0);
+		const double area = sm[0], sumF = sm[1], sxc = sm[2], sxs = sm[3];
+		const double syc = sm[4], sys = sm[5], S00 = sm[6], S01 = sm[7];
+		const double trx = 0., try_ = 0.;
+		double val = 0.;
+		double th_old = C.thetaQ_old, th = 0.;
+		if(sub == 0) val = LX * (atan2(-sxs / sumF, -sxc / sumF) + PI_F) / (2 * PI_F);
+		else if(sub == 1) val = LY * (atan2(-sys / sumF, -syc / sumF) + PI_F) / (2 * PI_F);
+		else {
+			// WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:365-396)
+			if(store) {
+				const double gs = a.gauss ? a.gauss[ci] : 0.0;
+				th_old = C.thetaQ + Q.sqrt_dt * Q.zetaQ_inter * gs;
+			}
+			th = th_old - Q.dt * Q.J_Q * (atan2(trx * C.Q01 - try_ * C.Q00, trx * C.Q00 + try_ * C.Q01));
+			double arg = th;
+			if(Q.anchoring) {  // update_anchoring (WetPolarModel.cpp:417-430)
+				const double walls_length = 8;
+				const double com_old_y = store ? C.CoMy : C.CoM_oldy;
+				const double dist1 = (com_old_y - walls_length);
+				const double dist2 = ((double) L - walls_length) - com_old_y;
+				if(dist1 < dist2) arg = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
+				else arg = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) L - 2 * walls_length) / 2));
+			}
+			val = sub == 2 ? cos(arg) : sin(arg);
+		}
+		double CoMx = __shfl(val, base + 0, 64), CoMy = __shfl(val, base + 1, 64);
+		const double c2 = __shfl(val, base + 2, 64), s3 = __shfl(val, base + 3, 64);
+		th = __shfl(th, base + 2, 64);
+		th_old = __shfl(th_old, base + 2, 64);
+		if(!on || sub != 0) continue;
+		if(!(area == area) || !(sumF == sumF)) {
+			// a NaN field (no counterpart: the reference carries it on): halt
+			// before it reaches the cell's geometry; the first cell is named
+			if(!(atomicOr(&a.st->flags, ST_NAN) & ST_NAN)) a.st->first_cell = ci;
+			continue;
+		}
+		DevCell &G = a.cells[ci];
+		if(store) {
+			G.CoM_oldx = C.CoMx;
+			G.CoM_oldy = C.CoMy;
+			G.area_old = C.area;
+			G.sumF_old = C.sumF;
+		}
+		G.area = area;
+		G.sumF = sumF;
+		G.S00 = S00;
+		G.S01 = S01;
+		G.perimeter = 0.;
+		const int xsl = store ? LX : s_min[2 * j], ysb = store ? LY : s_min[2 * j + 1];
+		G.x_sub_left = xsl;
+		G.y_sub_bottom = ysb;
+		if(!store) {
+			// MultiPhaseField::set_positions (MultiPhaseField.cpp:223-247)
+			if(xsl >= BORDER && ysb >= BORDER) {
+				const int site = box_elem(L, C.corner, (int) CoMx, (int) CoMy);
+				CoMy = (int) (site / L) + (CoMy - (int) CoMy);
+				CoMx = (int) (site - (int) (site / L) * L) + (CoMx - (int) CoMx);
+				const double co0 = box_x(L, C.corner, 0), co1 = box_y(L, C.corner, 0);
+				const int corner = box_elem(L, site, (int) -LX / 2, (int) -LY / 2);
+				const double cn0 = box_x(L, corner, 0), cn1 = box_y(L, corner, 0);
+				const double d0 = cn0 - co0 - rint((cn0 - co0) / L) * L;
+				const double d1 = cn1 - co1 - rint((cn1 - co1) / L) * L;
+				const int offx = (int) (C.offx + LX - d0) % LX;
+				const int offy = (int) (C.offy + LY - d1) % LY;
+				G.corner_old = C.corner;
+				G.corner = corner;
+				G.unrapx = C.unrapx + d0;
+				G.unrapy = C.unrapy + d1;
+				G.offx = offx;
+				G.offy = offy;
+				a.geo[ci] = make_int4(corner, LX | (LY << 16), offx | (offy << 16), C.start);
+			}
+			else {
+				// patch growth (MultiPhaseField.cpp:248-310) is done by the host:
+				// flag it, with the step, and halt
+				if(atomicOr(&a.st->flags, ST_GROWTH | ((unsigned) (step + 1) << ST_STEP_SHIFT)) == 0u) {
+					a.st->first_cell = ci;
+					a.st->step = step;
+				}
+			}
+		}
+		G.CoMx = CoMx;
+		G.CoMy = CoMy;
+		G.thetaQ_old = th_old;
+		G.thetaQ = th;
+		if(Q.anchoring) {
+			G.Q00 = 0.5 * (c2 * c2 - s3 * s3);
+			G.Q01 = c2 * s3;
+		}
+		else {
+			G.Q00 = c2;
+			G.Q01 = s3;
+		}
+		G.trx = trx;
+		G.try_ = try_;
+	}
+}
+
+template <int PNPT>
+static hipError_t launch_stage_p_n(const StageArgs &a0, int ncells, int grid, size_t lds, hipStream_t s) {
+	static size_t lds_set[4] = { 0, 0, 0, 0 };
+	const int v = (a0.store ? 1 : 0) + (a0.rcp ? 2 : 0);
+	const void *fn = v == 0 ? (const void *) k_stage_p<false, false, PNPT> : v == 1 ? (const void *) k_stage_p<true, false, PNPT>
+			: v == 2 ? (const void *) k_stage_p<false, true, PNPT> : (const void *) k_stage_p<true, true, PNPT>;
+	if(lds > lds_set[v]) {
+		hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+		if(e != hipSuccess) return e;
+		lds_set[v] = lds;
+	}
+	switch(v) {
+	case 0: RC_LAUNCH((k_stage_p<false, false, PNPT>), dim3(grid), dim3(PCT), lds, s, a0, ncells); break;
+	case 1: RC_LAUNCH((k_stage_p<true, false, PNPT>), dim3(grid), dim3(PCT), lds, s, a0, ncells); break;
+	case 2: RC_LAUNCH((k_stage_p<false, true, PNPT>), dim3(grid), dim3(PCT), lds, s, a0, ncells); break;
+	default: RC_LAUNCH((k_stage_p<true, true, PNPT>), dim3(grid), dim3(PCT), lds, s, a0, ncells); break;
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_stage_p(const StageArgs &a0, int ncells, int grid, hipStream_t s) {
+	// a0.npt: strip rows of the launch (2 or 3, at least every cell's)
+	if(ncells <= 0) return hipSuccess;
+	if(a0.npt != 2 && a0.npt != 3) return hipErrorInvalidValue;
+	if(grid > ncells) grid = ncells;
+	const int per = (ncells + grid - 1) / grid;
+	const size_t lds = stage_p_lds_bytes(a0, per);
+	return a0.npt == 2 ? launch_stage_p_n<2>(a0, ncells, grid, lds, s) : launch_stage_p_n<3>(a0, ncells, grid, lds, s);
+}
+
+// ---------------------------------------------------------------------------
 // k_nem_force: the nematic model's force evaluation of one cell per
 // workgroup (WetModel::f_interaction and calc_internal_forces,
 // WetModel.cpp:488-641).  Pass 1 stages phi and evaluates per site the

@@ -217,7 +217,8 @@ class Simulation:
 
     def set_stage_shape(self, threads: int = 0, rows: int = 0):
         """Force the field-update kernel's shape (threads per cell, rows per
-        thread strip); 0, 0 lets the autotuner choose."""
+        thread strip); 1024, 0 forces the persistent dry stage (one launch
+        over every cell); 0, 0 lets the autotuner choose."""
         self._check(self.L.rc_set_stage_shape(self.h, threads, rows))
 
     def stage_groups(self) -> List[Dict]:
@@ -234,6 +235,15 @@ class Simulation:
             out.append({"threads": shape[3 * g], "rows": shape[3 * g + 1], "cells": shape[3 * g + 2], "valid": v,
                         "tune_us": {"%dx%d" % sv: round(us[8 * g + i], 2) for i, sv in enumerate(v)}})
         return out
+
+    def stage_persistent(self) -> Dict:
+        """The persistent form of the dry fused stage (one launch over every
+        cell): whether the layout allows it, whether it is used, and the
+        tuner's predictor times of it and of the groups (us)."""
+        el, used, tp, tg = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+        self._check(self.L.rc_stage_persistent(self.h, C.byref(el), C.byref(used), C.byref(tp), C.byref(tg)))
+        return {"eligible": bool(el.value), "used": bool(used.value), "tune_us_persistent": round(tp.value, 2),
+                "tune_us_groups": round(tg.value, 2)}
 
     def set_graphs(self, on: bool = True):
         self._check(self.L.rc_set_graphs(self.h, 1 if on else 0))
