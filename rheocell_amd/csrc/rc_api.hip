@@ -908,7 +908,9 @@ static int learned_stage(rc_handle *h, const CGArgs &c0) {
 	c.rows = (double) h->rows;
 	c.nbox_blk = (ns + 255) / 256;
 	HIPCHK(h, launch_cg_gather(CG_GATHER_FEAT, c, h->ntiles, h->stream));
-	if(net_forward_packed(h->net, h->stream))
+	// the solve's active flag: stages after convergence skip the network
+	const int *run = reinterpret_cast<const int *>(reinterpret_cast<const char *>(h->d_cg) + offsetof(CGState, active));
+	if(net_forward_packed(h->net, h->stream, run))
 		return fail(h, RC_ERR_HIP, "network forward pass failed: %s", rc_net_last_error(h->net));
 	HIPCHK(h, launch_cg_box(2, c, ns, h->stream));
 	HIPCHK(h, launch_cg_gather(CG_GATHER_E, c, h->ntiles, h->stream));

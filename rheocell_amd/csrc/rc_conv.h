@@ -17,8 +17,9 @@ __device__ __host__ inline unsigned short f2bf(float f) {  // round to nearest e
 }
 __device__ __host__ inline float bf2f(unsigned short h) { return __builtin_bit_cast(float, ((unsigned) h) << 16); }
 
+// run (optional): the launch does nothing when *run == 0 on the device
 hipError_t launch_conv3x3(const unsigned short *in, const unsigned short *w, const float *bias, unsigned short *out,
-		const unsigned short *mask, float *fout, int N, int H, int W, int mode, hipStream_t s);
+		const unsigned short *mask, float *fout, int N, int H, int W, int mode, hipStream_t s, const int *run = nullptr);
 hipError_t launch_conv_wgrad(const unsigned short *in, const unsigned short *g, float *part, int N, int H, int W, hipStream_t s);
 int conv_wgrad_blocks(int N, int H);
 hipError_t launch_sum_parts(const float *part, float *grad, int nblk, int len, hipStream_t s);

@@ -65,10 +65,13 @@ __device__ inline unsigned pack2bf(float a, float b) {
 __device__ inline int swz(int q, int c) { return q * 32 + ((c ^ ((q >> 2) & 3)) << 3); }
 
 // MODE and OUT (bf16 store wanted) are compile-time; fout may be null.
+// run: if set and *run == 0, the launch does nothing (a learned CG stage
+// after the solve has converged)
 template <int MODE, bool OUT>
 __global__ __launch_bounds__(CONV_THREADS) void k_conv3x3(const unsigned short *__restrict__ in,
 		const unsigned short *__restrict__ w, const float *__restrict__ bias, unsigned short *__restrict__ out,
-		const unsigned short *__restrict__ mask, float *__restrict__ fout, int H, int W) {
+		const unsigned short *__restrict__ mask, float *__restrict__ fout, int H, int W, const int *run) {
+	if(run && *run == 0) return;
 	__shared__ __attribute__((aligned(16))) unsigned short s_in[CONV_SH * CONV_SW * 32];
 	__shared__ __attribute__((aligned(16))) unsigned short s_w[9 * 32 * 32];   // rows (tap, co), swizzled like s_in
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -186,15 +189,15 @@ __global__ __launch_bounds__(CONV_THREADS) void k_conv3x3(const unsigned short *
 }
 
 hipError_t launch_conv3x3(const unsigned short *in, const unsigned short *w, const float *bias, unsigned short *out,
-		const unsigned short *mask, float *fout, int N, int H, int W, int mode, hipStream_t s) {
+		const unsigned short *mask, float *fout, int N, int H, int W, int mode, hipStream_t s, const int *run) {
 	if(W % 16 || W < 32 || H < 3) return hipErrorInvalidValue;   // the staging wraps assume these
 	const int blocks = ((W + CONV_BW - 1) / CONV_BW) * ((H + CONV_ROWS - 1) / CONV_ROWS);
 	const dim3 g(blocks, N);
 	const dim3 b(CONV_THREADS);
-	if(!out) hipLaunchKernelGGL((k_conv3x3<1, false>), g, b, 0, s, in, w, bias, out, mask, fout, H, W);
-	else if(mode == 0) hipLaunchKernelGGL((k_conv3x3<0, true>), g, b, 0, s, in, w, bias, out, mask, fout, H, W);
-	else if(mode == 2) hipLaunchKernelGGL((k_conv3x3<2, true>), g, b, 0, s, in, w, bias, out, mask, fout, H, W);
-	else hipLaunchKernelGGL((k_conv3x3<1, true>), g, b, 0, s, in, w, bias, out, mask, fout, H, W);
+	if(!out) hipLaunchKernelGGL((k_conv3x3<1, false>), g, b, 0, s, in, w, bias, out, mask, fout, H, W, run);
+	else if(mode == 0) hipLaunchKernelGGL((k_conv3x3<0, true>), g, b, 0, s, in, w, bias, out, mask, fout, H, W, run);
+	else if(mode == 2) hipLaunchKernelGGL((k_conv3x3<2, true>), g, b, 0, s, in, w, bias, out, mask, fout, H, W, run);
+	else hipLaunchKernelGGL((k_conv3x3<1, true>), g, b, 0, s, in, w, bias, out, mask, fout, H, W, run);
 	return hipGetLastError();
 }
 

@@ -2108,7 +2108,8 @@ __global__ __launch_bounds__(1024) void k_cg_reduce(CGArgs a, int nblk) {
 	if(WHAT == 2) reduce_parts<NV, 256>(a.part, nblk, tot, s_red);
 	else reduce_parts<NV>(WHAT == 1 ? a.part_box : a.part, nblk, tot, s_red);
 	if(flags) return;
-	if((WHAT == 1 || WHAT == 2) && !active) return;
+	// (a learned stage after convergence leaves the state as it is: WHAT 3, 4)
+	if((WHAT == 1 || WHAT == 2 || WHAT == 3 || WHAT == 4) && !active) return;
 	if(WHAT == 3) {
 		double eae[2];
 		reduce_parts<2>(a.part_box, a.nbox_blk, eae, s_red);
@@ -2365,7 +2366,8 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_cg_row(CGArgs a) {
 	if(a.st->flags) return;
 	CGState *cg = a.cg;
-	if(MODE == CG_ROW_UPDATE && !cg->active) return;
+	// a learned stage after the solve has converged is a no-op (alpha0 = 0): skipped
+	if((MODE == CG_ROW_UPDATE || MODE == CG_ROW_STEP) && !cg->active) return;
 	// fallback: a component whose learned start has a larger residual than the previous solution
 	const bool rev0 = MODE == CG_ROW_REVERT && cg->rn2[0] > cg->rn2_start[0];
 	const bool rev1 = MODE == CG_ROW_REVERT && cg->rn2[1] > cg->rn2_start[1];

@@ -144,16 +144,16 @@ extern "C" int rc_net_get_params(rc_net *n, float *w, float *b) {
 
 // forward pass on n->act[0] already packed; result in n->out (fp32, 2 channels);
 // the last layer stores no bf16 tensor (backward reads act[0..5] only)
-static int forward_packed(rc_net *n, hipStream_t s) {
+static int forward_packed(rc_net *n, hipStream_t s, const int *run = nullptr) {
 	for(int l = 0; l < NET_LAYERS; ++l) {
 		const bool last = l == NET_LAYERS - 1;
 		NCHK(n, launch_conv3x3(n->act[l], n->w + (size_t) l * NET_WLAYER, n->bf + l * NET_C, last ? nullptr : n->act[l + 1],
-				nullptr, last ? n->out : nullptr, n->N, n->H, n->W, last ? 1 : 0, s));
+				nullptr, last ? n->out : nullptr, n->N, n->H, n->W, last ? 1 : 0, s, run));
 	}
 	return RC_OK;
 }
 unsigned short *net_input(rc_net *n) { return n->act[0]; }
-int net_forward_packed(rc_net *n, hipStream_t s) { return forward_packed(n, s); }
+int net_forward_packed(rc_net *n, hipStream_t s, const int *run) { return forward_packed(n, s, run); }
 
 int net_forward_device(rc_net *n, const float *d_x, hipStream_t s) {
 	NCHK(n, launch_pack_features(d_x, n->act[0], n->npix, s));
