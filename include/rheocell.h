@@ -262,8 +262,9 @@ int rc_stage_persistent(rc_handle *h, int *eligible, int *used, double *tune_us_
  * either way.  Default: enabled. */
 int rc_set_graphs(rc_handle *h, int enable);
 int rc_graph_stats(rc_handle *h, long long *graph_launches, long long *eager_steps);
-/* patch growths served inside rc_step (MultiPhaseField.cpp:248-310 on the
- * host: download, re-cut, re-layout) and the host time they took */
+/* patch growths served inside rc_step (MultiPhaseField.cpp:248-310: geometry
+ * and re-layout on the host, the re-cut of the fields on the device) and the
+ * host time they took */
 int rc_growth_stats(rc_handle *h, long long *growths, double *host_ms);
 
 /* ---- Learned CG warm start (no counterpart in the reference) ----------

@@ -1299,9 +1299,12 @@ static int download_raw(rc_handle *h, std::vector<rc_cell> &cells, std::vector<d
 	return RC_OK;
 }
 
-// Host part of MultiPhaseField::set_positions (MultiPhaseField.cpp:248-310):
-// the patch is re-cut around the interface bounding box, border 4 sites.
-static int grow_patch(rc_handle *h, rc_cell &c, const double *phi, std::vector<double> &out, int side) {
+// MultiPhaseField::set_positions, growth branch (MultiPhaseField.cpp:248-310):
+// the geometry of the re-cut around the interface bounding box (border 4
+// sites), on the host from the cell record alone; the field is re-cut on the
+// device (k_grow_rows) from the entry this fills.  The area (sum phi^2 of the
+// copied sites) comes back from the device.
+static int grow_geometry(rc_handle *h, rc_cell &c, int side, long long old_start, long long new_start, int4 *ent) {
 	auto bx = [side](int site, int d) {
 		int x = (site - ((site / side) * side)) + d;
 		while(x < 0) x += side;
@@ -1319,40 +1322,22 @@ static int grow_patch(rc_handle *h, rc_cell &c, const double *phi, std::vector<d
 	const int nLY = LY - (2 * c.y_sub_bottom) + (2 * BORDER);
 	if(nLX < 3 || nLY < 3 || nLX > side || nLY > side)
 		return fail(h, RC_ERR_GROWTH, "patch growth to %dx%d does not fit box %d", nLX, nLY, side);
-	const int nS = nLX * nLY;
+	if(sizeof(int) * (size_t) nLX * nLY > GROW_LDS_MAX)
+		return fail(h, RC_ERR_GROWTH, "patch growth to %dx%d exceeds the device re-cut (%d sites)", nLX, nLY,
+				(int) (GROW_LDS_MAX / sizeof(int)));
 	const int ncorner = bx(c.corner, -(-(2 * c.x_sub_left) + (2 * BORDER)) / 2)
 			+ by(c.corner, -(-(2 * c.y_sub_bottom) + (2 * BORDER)) / 2) * side;
-	out.assign(nS, 0.);
-	double area = 0;
 	int sy, sx, dimy, dimx;
 	if(nLY < LY) { sy = (LY - nLY) / 2; dimy = nLY; } else { sy = 0; dimy = LY; }
 	if(nLX < LX) { sx = (LX - nLX) / 2; dimx = nLX; } else { sx = 0; dimx = LX; }
-	for(int i = sy; i < dimy; i++) {
-		const int yy = by(c.corner, i);
-		int off_y = i - c.offset[1];
-		if(off_y < 0) off_y += LY;
-		for(int j = sx; j < dimx; j++) {
-			const int xx = bx(c.corner, j);
-			int dx = (ncorner - (ncorner / side) * side) - xx;
-			if(dx > side / 2) dx -= side;
-			else if(dx < -side / 2) dx += side;
-			int dy = (ncorner / side) - yy;
-			if(dy > side / 2) dy -= side;
-			else if(dy < -side / 2) dy += side;
-			int off_x = j - c.offset[0];
-			if(off_x < 0) off_x += LX;
-			const int dst = std::abs(dx) + std::abs(dy) * nLX;
-			if(dst < 0 || dst >= nS) return fail(h, RC_ERR_GROWTH, "patch growth writes outside the new patch");
-			const double v = phi[off_x + off_y * LX];
-			out[dst] = v;
-			area += v * v;
-		}
-	}
+	ent[0] = make_int4((int) old_start, (int) new_start, LX, LY);
+	ent[1] = make_int4(c.offset[0], c.offset[1], c.corner, side);
+	ent[2] = make_int4(nLX, nLY, ncorner, sx);
+	ent[3] = make_int4(sy, dimx, dimy, 0);
 	const double c0x = bx(c.corner, 0), c0y = by(c.corner, 0);
 	const double c1x = bx(ncorner, 0), c1y = by(ncorner, 0);
 	c.unrap[0] += c1x - c0x - std::rint((c1x - c0x) / side) * side;
 	c.unrap[1] += c1y - c0y - std::rint((c1y - c0y) / side) * side;
-	c.area = area;
 	c.LsubX = nLX;
 	c.LsubY = nLY;
 	c.corner_old = c.corner;
@@ -1365,24 +1350,21 @@ static int grow_patch(rc_handle *h, rc_cell &c, const double *phi, std::vector<d
 	return RC_OK;
 }
 
-// serves a growth request raised by the device: re-cuts the flagged patches,
-// re-lays out the rows and clears the halt
+// serves a growth request raised by the device: re-cuts the flagged patches
+// on the device, re-lays out the rows and clears the halt
 static int serve_growth(rc_handle *h) {
-	// cell state; the rows of the cells that grow; every other cell's rows are
-	// moved to their new offsets on the device (k_repack_rows)
+	// cell state (geometry only crosses to the host); every other cell's rows
+	// move to their new offsets (k_repack_rows), the growing ones are re-cut
+	// (k_grow_rows)
 	std::vector<DevCell> dc(h->ncells);
 	HIPCHK(h, hipMemcpyAsync(dc.data(), h->d_cells, sizeof(DevCell) * h->ncells, hipMemcpyDeviceToHost, h->stream));
 	HIPCHK(h, hipStreamSynchronize(h->stream));
 	std::vector<rc_cell> cells(h->ncells);
 	for(int i = 0; i < h->ncells; ++i) to_rc_cell(dc[i], cells[i]);
-	std::vector<int4> map;
-	struct Up {
-		long long start;
-		std::vector<double> phi, rep;
-	};
-	std::vector<Up> ups;
+	std::vector<int4> map, ent;
+	std::vector<int> grown;
 	long long nrows = 0;
-	int rc;
+	int max_sites = 0, rc;
 	for(int b = 0; b < h->nboxes; ++b) {
 		const DevBox &B = h->dboxes[b];
 		for(int i = B.cell_begin; i < B.cell_end; ++i) {
@@ -1394,45 +1376,49 @@ static int serve_growth(rc_handle *h) {
 				nrows += S;
 				continue;
 			}
-			std::vector<double> ophi(S), orep;
-			HIPCHK(h, hipMemcpy(ophi.data(), h->d_phiA + old, sizeof(double) * S, hipMemcpyDeviceToHost));
-			if(h->jq) {
-				orep.resize(S);
-				HIPCHK(h, hipMemcpy(orep.data(), h->d_rep + old, sizeof(double) * S, hipMemcpyDeviceToHost));
-			}
-			Up u;
-			u.start = nrows;
-			if((rc = grow_patch(h, c, ophi.data(), u.phi, B.L))) return rc;
+			int4 e[4];
+			if((rc = grow_geometry(h, c, B.L, old, nrows, e))) return rc;
+			ent.insert(ent.end(), e, e + 4);
+			grown.push_back(i);
 			const int nS = c.LsubX * c.LsubY;
-			if(h->jq) {  // std::vector::resize keeps the first min(old, new) entries
-				u.rep.assign(nS, 0.);
-				for(int q = 0; q < std::min(S, nS); ++q) u.rep[q] = orep[q];
-			}
+			max_sites = std::max(max_sites, nS);
 			nrows += nS;
-			ups.push_back(std::move(u));
 		}
 	}
-	double *d_nphi = nullptr, *d_nrep = nullptr;
-	int4 *d_map = nullptr;
+	double *d_nphi = nullptr, *d_nrep = nullptr, *d_area = nullptr;
+	int4 *d_map = nullptr, *d_ent = nullptr;
+	const int ng = (int) grown.size();
+	std::vector<double> area(ng);
 	HIPCHK(h, dalloc(&d_nphi, nrows));
 	if(h->jq) HIPCHK(h, dalloc(&d_nrep, nrows));
-	HIPCHK(h, dalloc(&d_map, map.size()));
-	HIPCHK(h, hipMemcpyAsync(d_map, map.data(), sizeof(int4) * map.size(), hipMemcpyHostToDevice, h->stream));
+	HIPCHK(h, dalloc(&d_map, std::max<size_t>(map.size(), 1)));
+	HIPCHK(h, dalloc(&d_ent, std::max<size_t>(ent.size(), 1)));
+	HIPCHK(h, dalloc(&d_area, std::max(ng, 1)));
+	if(!map.empty())
+		HIPCHK(h, hipMemcpyAsync(d_map, map.data(), sizeof(int4) * map.size(), hipMemcpyHostToDevice, h->stream));
+	if(!ent.empty())
+		HIPCHK(h, hipMemcpyAsync(d_ent, ent.data(), sizeof(int4) * ent.size(), hipMemcpyHostToDevice, h->stream));
 	HIPCHK(h, launch_repack_rows(h->d_phiA, d_nphi, h->jq ? h->d_rep : nullptr, d_nrep, d_map, (int) map.size(),
 			h->stream));
-	for(const Up &u : ups) {
-		HIPCHK(h, hipMemcpyAsync(d_nphi + u.start, u.phi.data(), sizeof(double) * u.phi.size(), hipMemcpyHostToDevice,
-				h->stream));
-		if(h->jq)
-			HIPCHK(h, hipMemcpyAsync(d_nrep + u.start, u.rep.data(), sizeof(double) * u.rep.size(),
-					hipMemcpyHostToDevice, h->stream));
-	}
+	HIPCHK(h, launch_grow_rows(h->d_phiA, d_nphi, h->jq ? h->d_rep : nullptr, d_nrep, d_ent, ng, max_sites, d_area,
+			h->stream));
+	if(ng) HIPCHK(h, hipMemcpyAsync(area.data(), d_area, sizeof(double) * ng, hipMemcpyDeviceToHost, h->stream));
 	HIPCHK(h, hipStreamSynchronize(h->stream));
-	rc = upload_layout(h, cells.data(), d_nphi, d_nrep, true);
+	rc = RC_OK;
+	for(int k = 0; k < ng; ++k) {
+		if(!(area[k] == area[k])) {
+			rc = fail(h, RC_ERR_GROWTH, "patch growth of cell %d writes outside the new patch", grown[k]);
+			break;
+		}
+		cells[grown[k]].area = area[k];
+	}
+	if(!rc) rc = upload_layout(h, cells.data(), d_nphi, d_nrep, true);
 	(void) hipStreamSynchronize(h->stream);
 	dfree(d_nphi);
 	dfree(d_nrep);
 	dfree(d_map);
+	dfree(d_ent);
+	dfree(d_area);
 	if(rc) return rc;
 	h->h_st->flags = 0;
 	return RC_OK;
@@ -1693,7 +1679,9 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 			int window = 0;
 			for(; s < ns; ++s) {
 				// graph replay of a whole window (dry model, steady parameter set, no timing)
-				if(h->use_graphs && !h->split && !h->timing && !grown && ns - s >= GRAPH_STEPS && h->cf_set == h->cur_set &&
+				// after a growth (new layout: the graph is captured again) only when
+				// at least two windows remain, so that the capture pays
+				if(h->use_graphs && !h->split && !h->timing && ns - s >= (grown ? 2 : 1) * GRAPH_STEPS && h->cf_set == h->cur_set &&
 						window < 64) {
 					if(!h->graph_valid || h->graph_gauss != (gauss != nullptr) || h->graph_set != h->cur_set) {
 						HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1740,7 +1728,7 @@ extern "C" int rc_step(rc_handle *h, int nsteps, const double *gauss) {
 				else
 					h->cf_set = h->cur_set;
 				s = st.step + 1;
-				grown = true;   // the rest of the chunk runs eagerly (its noise windows are misaligned)
+				grown = true;
 				continue;
 			}
 			if(st.flags) break;

@@ -193,6 +193,11 @@ hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap,
 		int nsites, hipStream_t s);
 hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
 // map[j] = {old row start, new row start, rows}: dst[new + q] = src[old + q] (and src2 -> dst2 if set)
+// growth re-cut of listed patches (k_grow_rows; 4 int4 per entry), at most
+// GROW_LDS_MAX / 4 sites per new patch
+constexpr size_t GROW_LDS_MAX = 128 * 1024;
+hipError_t launch_grow_rows(const double *src, double *dst, const double *src2, double *dst2, const int4 *ent, int n,
+		int max_sites, double *area_out, hipStream_t s);
 hipError_t launch_repack_rows(const double *src, double *dst, const double *src2, double *dst2, const int4 *map, int n,
 		hipStream_t s);
 hipError_t stage_set_lds_limit(size_t bytes);

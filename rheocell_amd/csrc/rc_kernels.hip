@@ -2559,6 +2559,87 @@ hipError_t launch_repack_rows(const double *src, double *dst, const double *src2
 	return hipGetLastError();
 }
 
+// The re-cut of a growing patch (MultiPhaseField::set_positions, growth
+// branch, MultiPhaseField.cpp:248-310) on the device, one workgroup per
+// growing cell.  ent[4 j .. 4 j + 3] = {old start, new start, LX, LY}, {offx,
+// offy, corner, box side}, {new LX, new LY, new corner, sx}, {sy, dimx, dimy,
+// 0}: the host's geometry of the re-cut.  Every old site (i, j) of rows sy ..
+// dimy - 1 and columns sx .. dimx - 1 goes to new site |dx| + |dy| new LX
+// (dx, dy: the old site's offset from the new corner, nearest image); where
+// the reference's loop writes one new site twice the later (i, j) in its
+// order (rows, then columns) wins: the largest key i LX + j, taken with an
+// LDS atomicMax.  Sites nothing maps to are 0.  area_out[j] = sum of phi^2
+// over the copied sites (the reference adds every copy, overwritten or not),
+// in a fixed order, or NaN if a site maps outside the new patch.  The
+// Repulsion rows (src2, when set) keep their first min(old, new) entries
+// (std::vector::resize).
+__global__ __launch_bounds__(256) void k_grow_rows(const double *__restrict__ src, double *__restrict__ dst,
+		const double *__restrict__ src2, double *__restrict__ dst2, const int4 *__restrict__ ent, double *area_out) {
+	extern __shared__ int s_key[];
+	__shared__ double s_part[4];
+	__shared__ int s_bad;
+	const int tid = threadIdx.x;
+	const int4 e0 = ent[4 * blockIdx.x], e1 = ent[4 * blockIdx.x + 1], e2 = ent[4 * blockIdx.x + 2], e3 = ent[4 * blockIdx.x + 3];
+	const int ostart = e0.x, nstart = e0.y, LX = e0.z, LY = e0.w;
+	const int offx = e1.x, offy = e1.y, corner = e1.z, side = e1.w;
+	const int nLX = e2.x, nLY = e2.y, ncorner = e2.z, sx = e2.w;
+	const int sy = e3.x, dimx = e3.y, dimy = e3.z;
+	const int nS = nLX * nLY, S = LX * LY;
+	for(int q = tid; q < nS; q += 256) s_key[q] = -1;
+	if(tid == 0) s_bad = 0;
+	__syncthreads();
+	const int ncx = ncorner - (ncorner / side) * side, ncy = ncorner / side;
+	const int w = dimx - sx, hg = dimy - sy;
+	const int cnt = w > 0 && hg > 0 ? w * hg : 0;
+	auto source = [&](int i, int j) {
+		int off_y = i - offy;
+		if(off_y < 0) off_y += LY;
+		int off_x = j - offx;
+		if(off_x < 0) off_x += LX;
+		return ostart + off_x + off_y * LX;
+	};
+	double acc = 0.;
+	for(int t = tid; t < cnt; t += 256) {
+		const int i = sy + t / w, j = sx + t % w;
+		const int yy = box_y(side, corner, i), xx = box_x(side, corner, j);
+		int dx = ncx - xx;
+		if(dx > side / 2) dx -= side;
+		else if(dx < -side / 2) dx += side;
+		int dy = ncy - yy;
+		if(dy > side / 2) dy -= side;
+		else if(dy < -side / 2) dy += side;
+		const int d = abs(dx) + abs(dy) * nLX;
+		if(d < 0 || d >= nS) s_bad = 1;
+		else atomicMax(&s_key[d], i * LX + j);
+		const double v = src[source(i, j)];
+		acc += v * v;
+	}
+	__syncthreads();
+	for(int q = tid; q < nS; q += 256) {
+		const int k = s_key[q];
+		dst[nstart + q] = k >= 0 ? src[source(k / LX, k % LX)] : 0.;
+		if(src2) dst2[nstart + q] = q < S ? src2[ostart + q] : 0.;
+	}
+	const double ws = wave_sum(acc);
+	if((tid & 63) == 0) s_part[tid >> 6] = ws;
+	__syncthreads();
+	if(tid == 0) area_out[blockIdx.x] = s_bad ? NAN : (s_part[0] + s_part[1]) + (s_part[2] + s_part[3]);
+}
+hipError_t launch_grow_rows(const double *src, double *dst, const double *src2, double *dst2, const int4 *ent, int n,
+		int max_sites, double *area_out, hipStream_t s) {
+	if(n <= 0) return hipSuccess;
+	const size_t lds = sizeof(int) * (size_t) max_sites;
+	if(lds > GROW_LDS_MAX) return hipErrorInvalidValue;
+	static size_t lds_set = 0;
+	if(lds > lds_set) {
+		hipError_t e = hipFuncSetAttribute((const void *) k_grow_rows, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+		if(e != hipSuccess) return e;
+		lds_set = lds;
+	}
+	RC_LAUNCH(k_grow_rows, dim3(n), dim3(256), lds, s, src, dst, src2, dst2, ent, area_out);
+	return hipGetLastError();
+}
+
 __global__ void k_zero_rows(double2 *x, long long rows, int mask) {
 	const long long i = (long long) blockIdx.x * 256 + threadIdx.x;
 	if(i >= rows) return;

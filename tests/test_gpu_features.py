@@ -332,6 +332,25 @@ def test_graph_replays_match_eager_steps(grow):
     assert ga >= 1 and ga * 16 + ea >= 69
 
 
+def test_graph_replays_resume_after_a_growth():
+    # a patch grows in the first step; with 79 steps left the re-laid-out steps
+    # are captured again and replayed: identical fields to ordinary launches
+    conf = _growth_conf()
+    keys = dict(confgen.BASE_KEYS, seed=1)
+    a = Simulation([(conf, keys)])
+    b = Simulation([(conf, keys)])
+    b.set_graphs(False)
+    a.step(80)
+    b.step(80)
+    ca, pa = a.download()
+    cb, pb = b.download()
+    assert pa.shape == pb.shape and np.array_equal(pa, pb)
+    assert [(c.corner, c.LsubX, c.LsubY) for c in ca] == [(c.corner, c.LsubX, c.LsubY) for c in cb]
+    (ga, ea), (n, _) = a.graph_stats(), a.growth_stats()
+    print("growths", n, "graph replays", ga, "eager steps", ea)
+    assert n >= 1 and ga >= 3
+
+
 def test_tile_list_overflow_raises():
     # 136 patches of 3 x 3 sites (three covering each site, far below the
     # overlap limit of 20) all intersect the first 32 x 8 box tile: more than
