@@ -49,6 +49,8 @@ BYTES = {
     "cg_box_pap": (0, 56),          # read box sums, P2, diag; write N4
     "cg_row_update": (80, 24),      # read x, p, r; write x, r; read N4 and diag
     "cg_reduce": (0, 0),
+    "nem_sums": (8, 16),            # read phi; write the box sums of phi Q00, phi Q01
+    "nem_force": (24, 24),          # read phi; write f (2 doubles); read sum phi^2, sum phi Q00, sum phi Q01
 }
 CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update")
 
@@ -109,7 +111,11 @@ def cpu_baseline(conf, keys, steps, budget_s=12.0):
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except Exception:
         pass
-    s = OracleSim(conf, keys, flavour="fast")
+    wall = None
+    if str(keys.get("type", "none")) != "none":   # a wall box (the library takes the same field)
+        from rheocell_amd.walls import wall_field
+        wall = wall_field(str(keys["type"]), conf.Lx, float(keys["lambda_wall"]))
+    s = OracleSim(conf, keys, flavour="fast", walls=wall)
     s.step(1)
     s0 = s.cg_stats()
     t0 = time.perf_counter()
@@ -318,8 +324,10 @@ def main():
         log("[bench] stage group: %d cells, shape %dx%d, tuner us %s" % (g["cells"], g["threads"], g["rows"],
                                                                          g["tune_us"]))
     step_roof = None
-    if not wet:
-        # whole dry step: read phi_n, write phi*, dphi; read phi*, phi_n, dphi, write phi_n+1;
+    fused = not wet and str(keys.get("interaction_type", "wetpolarmodel")) == "wetpolarmodel" and \
+        str(keys.get("type", "none")) == "none"
+    if fused:
+        # whole fused dry step: read phi_n, write phi*, dphi; read phi*, phi_n, dphi, write phi_n+1;
         # the box sum phi^2 written and read twice (72 B per patch site + 32 B per box site)
         step_bytes = 72 * rows + 32 * box_sites
         step_roof = {"bytes_per_step": step_bytes, "achieved": round(step_bytes / (secs / args.steps) / 1e9, 1),
@@ -369,7 +377,8 @@ def main():
         log("[bench] kernels: " + json.dumps({k: (round(v["ms_total"], 3), v["launches"]) for k, v in kernels.items()}))
         # the field-update kernels (force evaluation and Heun update), each
         # against the HBM peak with its algorithmic bytes
-        for name in ("stage_fused_pred", "stage_fused_corr", "stage_cf", "stage_fs_pred", "stage_fs_corr", "box_gather"):
+        for name in ("stage_fused_pred", "stage_fused_corr", "stage_cf", "stage_fs_pred", "stage_fs_corr", "box_gather",
+                     "nem_sums", "nem_force"):
             d = kernels.get(name)
             if not d:
                 continue
@@ -381,7 +390,8 @@ def main():
                 "achieved_GB/s": round(b / (us * 1e-6) / 1e9, 1), "frac": round(b / (us * 1e-6) / 1e9 / PEAK_HBM_GBS, 4)}
 
     cgcmp = None
-    if wet and args.weights and os.path.exists(args.weights) and args.cg_steps > 0:
+    polar = str(keys.get("interaction_type", "wetpolarmodel")) == "wetpolarmodel"
+    if wet and polar and args.weights and os.path.exists(args.weights) and args.cg_steps > 0:
         log("[bench] CG warm starts: previous solution vs learned, %d steps each" % args.cg_steps)
         cgcmp = cg_compare(conf, keys, args.weights, args.warmup, args.cg_steps)
         log("[bench] " + json.dumps(cgcmp))
@@ -413,9 +423,11 @@ def main():
         "config": {"workload": args.workload, "box": L if len(boxes) == 1 else sorted(c.Lx for c, _ in boxes),
                    "boxes": len(boxes), "cells": ncells,
                    "patch": "41x41" if len(boxes) == 1 else "29..49 (uneven)", "rows": rows,
-                   "model": "wetpolarmodel", "friction_cell": float(keys["friction_cell"]),
+                   "model": str(keys.get("interaction_type", "wetpolarmodel")),
+                   "box_type": str(keys.get("type", "none")), "friction_cell": float(keys["friction_cell"]),
                    "cg_tolerance": float(keys["CGtolerance"]) if wet else None,
-                   "warm_start": "previous solution, cold every 11th solve (reference rule)" if wet else None},
+                   "warm_start": ("previous solution, cold every %s solve (reference rule)" %
+                                  ("101st" if keys.get("interaction_type") == "wetmodel" else "11th")) if wet else None},
         "roofline": roof,
         "cpu_baseline": cpu,
         "graphs": (lambda g: {"replays": g[0], "steps_launched_singly": g[1]})(sim.graph_stats()),

@@ -223,7 +223,9 @@ int rc_set_warm_start(rc_handle *h, int mode);
 #define RC_T_CG_GATHER_P_FIRST 15  /* p = z fused with the box gather of p */
 #define RC_T_CG_REDUCE 16          /* one-workgroup reduction of CG partial sums */
 #define RC_T_CG_LEARNED 17         /* the learned refinement stages of one solve (group) */
-#define RC_NKINDS 18
+#define RC_T_NEM_SUMS 19           /* k_box_field: nematic box sums of phi Q00, phi Q01 */
+#define RC_T_NEM_FORCE 18          /* k_nem_force: the nematic model's force evaluation */
+#define RC_NKINDS 20
 int rc_timing(rc_handle *h, int kind, double *ms, long long *launches);
 int rc_enable_timing(rc_handle *h, int on);
 const char *rc_last_error(rc_handle *h);

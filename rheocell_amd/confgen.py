@@ -108,7 +108,30 @@ def replica_ensemble(nrep: int = 32, seed: int = 1, coverage: float = 0.9):
     return out
 
 
+# The reference's own example model (examples/input: interaction_type =
+# wetmodel and its parameters) on the 512^2 / 1600-cell lattice; and the dry
+# polar model in a square wall box (SquareWalls, lambda_wall = 4) at 512^2 /
+# 1024 cells
+EXAMPLE_NEMATIC_KEYS: Dict[str, object] = {
+    "interaction_type": "wetmodel", "gamma": 0.06, "mu": 20, "lambda": 2.0, "kappa": 0.5, "omega": 0,
+    "wall_slip": 2, "passive_alpha": 1, "beta": 0, "friction": 1, "friction_cell": 1, "J_Q": 0.1, "anchoring": 0,
+    "zetaQ_self": 0.3, "zetaQ_inter": 0, "CGtolerance": 1e-5, "dt": 0.1, "R": 8, "J0": 1,
+}
+EXTRA_WORKLOADS = {
+    "nematic512_1600": (512, 40, 500, EXAMPLE_NEMATIC_KEYS),
+    "walls512_1024": (512, 32, 2000, {"type": "square_walls", "lambda_wall": 4.0}),
+}
+
+
 def workload(name: str, seed: int = 1):
+    if name in EXTRA_WORKLOADS:
+        L, nside, steps, extra = EXTRA_WORKLOADS[name]
+        conf = lattice_conf(L, nside, seed)
+        keys = dict(BASE_KEYS)
+        keys.update(extra)
+        keys["steps"] = steps
+        keys["seed"] = seed
+        return conf, keys
     L, nside, steps, fc = WORKLOADS[name]
     conf = lattice_conf(L, nside, seed)
     keys = dict(BASE_KEYS)

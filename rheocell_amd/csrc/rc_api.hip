@@ -403,11 +403,7 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 			*out = h;
 			return fail(h, RC_ERR_ARG, "all boxes of a handle must share the dry/wet model");
 		}
-	h->split = h->wet || h->nematic || h->ext;
-	if(h->ext) {   // the stage kernel's walls variant has one shape
-		h->stage_forced[0] = 512;
-		h->stage_forced[1] = 8;
-	}
+	h->split = h->wet || h->nematic;
 	h->ncells = cells;
 	h->ntiles = tiles;
 	h->box_sites = sites;
@@ -692,11 +688,6 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 			h->groups[k].d_cells = nullptr;
 		}
 		h->groups = std::move(used);
-		if(h->ext)
-			for(const auto &g : h->groups)
-				if(g.threads != 512 || g.rows != 8)
-					return fail(h, RC_ERR_ARG, "a %dx%d patch does not fit the walls variant of the stage kernel (512 x 8)",
-							g.mx, g.my);
 		// the persistent form (k_stage_p): dry fused stages in periodic boxes
 		// without J_Q, omega and beta, patches of at most 3 strip rows at 1024
 		// threads, and LDS for the cells of one workgroup
@@ -1087,7 +1078,10 @@ static int nem_forces(rc_handle *h, const double *phi) {
 	f.global = 1;
 	f.fx = h->d_sq00;
 	f.fy = h->d_sq01;
-	HIPCHK(h, launch_box_field(f, h->ntiles, h->stream));
+	{
+		TimeScope ts(h, RC_T_NEM_SUMS);
+		HIPCHK(h, launch_box_field(f, h->ntiles, h->stream));
+	}
 	NemArgs n{};
 	n.boxes = h->d_boxes;
 	n.cells = h->d_cells;
@@ -1102,7 +1096,7 @@ static int nem_forces(rc_handle *h, const double *phi) {
 	n.wmask = h->d_wmask;
 	n.walls = h->d_walls;
 	n.st = h->d_st;
-	TimeScope ts(h, RC_T_STAGE_CF);
+	TimeScope ts(h, RC_T_NEM_FORCE);
 	HIPCHK(h, launch_nem_force(n, h->ncells, nem_lds_bytes(h->max_S), h->stream));
 	return RC_OK;
 }
@@ -1514,9 +1508,6 @@ static int autotune_stage(rc_handle *h) {
 
 extern "C" int rc_set_stage_shape(rc_handle *h, int threads, int rows) {
 	if(!h) return RC_ERR_ARG;
-	if(h->ext && !((threads == 0 && rows == 0) || (threads == 512 && rows == 8)))
-		return fail(h, RC_ERR_ARG, "walls and external forces run the stage kernel's 512 x 8 shape only");
-	if(h->ext) return RC_OK;
 	if(threads == 0 && rows == 0) {
 		h->stage_forced[0] = h->stage_forced[1] = 0;
 		h->stage_tuned = false;

@@ -521,14 +521,20 @@ __device__ inline double lap_masked(unsigned m, double pN, double pS, double pE,
 
 // BaseField::set_F_ext / set_f_density_ext (BaseField.h:54-79): the external
 // forces' free energy and its density at a site of wall field w
+// (the loops run over the fixed maximum with a guard: constant indices keep
+// the parameter copy in scalar registers instead of private memory)
 __device__ inline double f_ext(const DevParams &P, double ph, double w) {
 	double F_ext = 0.;
-	for(int f = 0; f < P.nforce; ++f) F_ext += P.fF[f] * ph * w * w;
+#pragma unroll
+	for(int f = 0; f < 4; ++f)
+		if(f < P.nforce) F_ext += P.fF[f] * ph * w * w;
 	return F_ext;
 }
 __device__ inline double f_density_ext(const DevParams &P, double ph, double w) {
 	double f_density_ext = 0.;
-	for(int f = 0; f < P.nforce; ++f) f_density_ext += P.fD[f] * ph * ph * w * w;
+#pragma unroll
+	for(int f = 0; f < 4; ++f)
+		if(f < P.nforce) f_density_ext += P.fD[f] * ph * ph * w * w;
 	return f_density_ext;
 }
 
@@ -1113,32 +1119,28 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 static const int STAGE_SHAPES[][2] = { { 256, 4 }, { 256, 7 }, { 512, 4 }, { 512, 7 }, { 512, 8 }, { 512, 16 } };
 constexpr int NSTAGE_SHAPES = 6;
 
-template <int MODE, bool RCP, bool ST>
+template <int MODE, bool RCP, bool ST, bool WL>
 static hipError_t set_lds_r(size_t bytes) {
 	hipError_t e;
 	const int b = (int) bytes;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 256, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 256, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 8, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	return hipFuncSetAttribute((const void *) k_stage<MODE, 16, RCP, 512, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 256, ST, WL>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 256, ST, WL>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 4, RCP, 512, ST, WL>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 7, RCP, 512, ST, WL>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	if((e = hipFuncSetAttribute((const void *) k_stage<MODE, 8, RCP, 512, ST, WL>, hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
+	return hipFuncSetAttribute((const void *) k_stage<MODE, 16, RCP, 512, ST, WL>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
 }
 template <int MODE, bool ST>
 static hipError_t set_lds(size_t bytes) {
-	hipError_t e = set_lds_r<MODE, false, ST>(bytes);
-	return e != hipSuccess ? e : set_lds_r<MODE, true, ST>(bytes);
+	hipError_t e;
+	if((e = set_lds_r<MODE, false, ST, false>(bytes)) != hipSuccess) return e;
+	if((e = set_lds_r<MODE, true, ST, false>(bytes)) != hipSuccess) return e;
+	if((e = set_lds_r<MODE, false, ST, true>(bytes)) != hipSuccess) return e;
+	return set_lds_r<MODE, true, ST, true>(bytes);
 }
 
 hipError_t stage_set_lds_limit(size_t bytes) {
 	hipError_t e;
-	const int b = (int) bytes;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_CF, 8, false, 512, false, true>,
-			hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_FS, 8, false, 512, false, true>,
-			hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
-	if((e = hipFuncSetAttribute((const void *) k_stage<MODE_FS, 8, false, 512, true, true>,
-			hipFuncAttributeMaxDynamicSharedMemorySize, b)) != hipSuccess) return e;
 	if((e = set_lds<MODE_FUSED, true>(bytes)) != hipSuccess) return e;
 	if((e = set_lds<MODE_FUSED, false>(bytes)) != hipSuccess) return e;
 	if((e = set_lds<MODE_CF, false>(bytes)) != hipSuccess) return e;
@@ -1155,32 +1157,30 @@ int stage_shapes(int (*shapes)[2]) {
 }
 int stage_strip_rows(int LX, int LY, int threads) { return strip_rows(LX, LY, threads); }
 
-template <int MODE, bool RCP, bool ST>
+// WL: walls and external forces (a.wmask set)
+template <int MODE, bool RCP, bool ST, bool WL>
 static void launch_mode_r(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
 	const dim3 g(ncells);
-	if constexpr(MODE != MODE_FUSED && !RCP) {
-		// walls and external forces: one shape (512 threads, 8 rows)
-		if(a.wmask) {
-			RC_LAUNCH((k_stage<MODE, 8, false, 512, ST, true>), g, dim3(512), lds, s, a);
-			return;
-		}
-	}
 	if(a.threads == 256) {
-		if(a.npt == 4) RC_LAUNCH((k_stage<MODE, 4, RCP, 256, ST>), g, dim3(256), lds, s, a);
-		else RC_LAUNCH((k_stage<MODE, 7, RCP, 256, ST>), g, dim3(256), lds, s, a);
+		if(a.npt == 4) RC_LAUNCH((k_stage<MODE, 4, RCP, 256, ST, WL>), g, dim3(256), lds, s, a);
+		else RC_LAUNCH((k_stage<MODE, 7, RCP, 256, ST, WL>), g, dim3(256), lds, s, a);
 		return;
 	}
 	switch(a.npt) {
-	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
-	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
-	case 8: RC_LAUNCH((k_stage<MODE, 8, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
-	default: RC_LAUNCH((k_stage<MODE, 16, RCP, 512, ST>), g, dim3(512), lds, s, a); break;
+	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP, 512, ST, WL>), g, dim3(512), lds, s, a); break;
+	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP, 512, ST, WL>), g, dim3(512), lds, s, a); break;
+	case 8: RC_LAUNCH((k_stage<MODE, 8, RCP, 512, ST, WL>), g, dim3(512), lds, s, a); break;
+	default: RC_LAUNCH((k_stage<MODE, 16, RCP, 512, ST, WL>), g, dim3(512), lds, s, a); break;
 	}
 }
 template <int MODE, bool ST>
 static void launch_mode_s(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
-	if(a.rcp && !a.wmask) launch_mode_r<MODE, true, ST>(a, ncells, lds, s);
-	else launch_mode_r<MODE, false, ST>(a, ncells, lds, s);
+	if(a.wmask) {
+		if(a.rcp) launch_mode_r<MODE, true, ST, true>(a, ncells, lds, s);
+		else launch_mode_r<MODE, false, ST, true>(a, ncells, lds, s);
+	}
+	else if(a.rcp) launch_mode_r<MODE, true, ST, false>(a, ncells, lds, s);
+	else launch_mode_r<MODE, false, ST, false>(a, ncells, lds, s);
 }
 template <int MODE>
 static void launch_mode(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
