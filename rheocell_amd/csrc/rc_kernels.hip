@@ -1811,65 +1811,104 @@ __global__ __launch_bounds__(256) void k_nem_force(NemArgs a) {
 		kS = bys * L + bx;
 		return by * L + bx;
 	};
-	for(int q = tid; q < S; q += 256) {
-		const int qy = q / LX, qx = q - qy * LX;
-		const int qE = qx + 1 == LX ? q - qx : q + 1, qW = qx == 0 ? q + LX - 1 : q - 1;
-		const int qN = qy + 1 == LY ? qx : q + LX, qS = qy == 0 ? q + (LY - 1) * LX : q - LX;
-		const double ph = s_phi[q], pE = s_phi[qE], pW = s_phi[qW], pN = s_phi[qN], pS = s_phi[qS];
-		int kE, kW, kN, kS;
-		const int k = box_site(qx, qy, kE, kW, kN, kS);
-		const unsigned m = a.wmask ? a.wmask[B.box_off + k] : 31u;
-		const double w = a.wmask ? a.walls[B.box_off + k] : 0.;
-		const double lap = a.wmask ? lap_masked(m, pN, pS, pE, pW, ph) : pN + pS + pE + pW - 4 * ph;
-		const double dx = (m & 3u) == 3u ? .5 * (pE - pW) : 0.;
-		const double dy = (m & 12u) == 12u ? .5 * (pN - pS) : 0.;
-		const double p2 = phi2[k];
-		const double CH = P.gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / P.lambda - 2 * P.lambda * lap);
-		const double A = -4 * P.mu_a0 * (1 - C.area / P.a0) * ph;
-		const double Rep = P.rcoef * ph * (p2 - ph * ph);
-		double Adh = 0.;
-		if(P.omega != 0) {
-			const double laplacianSquare = (m & 16u) ? phi2[kE] + phi2[kN] + phi2[kW] + phi2[kS] - 4. * p2
-					: phi2[kE] + phi2[kW] - 2. * p2;   // a wall site
-			const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
-			const double dd = laplacianSquare - lsquare;
-			Adh = -4 * P.lambda * P.omega * (dd / sqrt(1 + dd * dd)) * ph;
+	// sites in batches of NB per thread: the batch's box loads are issued
+	// together, then evaluated (q = qy LX + qx by a multiply-high division,
+	// exact for q, LX < 2^16)
+	constexpr int NB = 4;
+	const unsigned mlx = 0xFFFFFFFFu / (unsigned) LX + 1u;
+	const unsigned char *wmb = a.wmask ? a.wmask + B.box_off : nullptr;
+	const double *wlb = a.wmask ? a.walls + B.box_off : nullptr;
+	for(int q0 = tid; q0 < S; q0 += NB * 256) {
+		int kk[NB], kE[NB], kW[NB], kN[NB], kS[NB];
+		double p2[NB], w[NB];
+		unsigned m[NB];
+#pragma unroll
+		for(int u = 0; u < NB; ++u) {
+			const int q = min(q0 + u * 256, S - 1);
+			const int qy = (int) __umulhi((unsigned) q, mlx), qx = q - qy * LX;
+			kk[u] = box_site(qx, qy, kE[u], kW[u], kN[u], kS[u]);
 		}
-		double mu = CH + A + Rep + Adh;
-		if(a.wmask) mu = f_ext(P, ph, w) + mu;   // freeEnergy starts at F_ext
-		const double CH_d = (P.gamma / P.lambda) * 4 * ph * ph * (1 - ph) * (1 - ph) + P.gamma * P.lambda * (dx * dx + dy * dy);
-		const double A_d = -P.mu / P.a0 * ph * ph;
-		const double Rep_d = (P.kappa / P.lambda) * ph * ph * (p2 - ph * ph);
-		double fd = CH_d + A_d + Rep_d;
-		if(a.wmask) fd = f_density_ext(P, ph, w) + fd;
-		s_s[q] = fd - mu * ph;
-		s_dx[q] = dx;
-		s_dy[q] = dy;
+#pragma unroll
+		for(int u = 0; u < NB; ++u) {
+			p2[u] = phi2[kk[u]];
+			m[u] = wmb ? wmb[kk[u]] : 31u;
+			w[u] = wlb ? wlb[kk[u]] : 0.;
+		}
+#pragma unroll
+		for(int u = 0; u < NB; ++u) {
+			const int q = q0 + u * 256;
+			if(q >= S) break;
+			const int qy = (int) __umulhi((unsigned) q, mlx), qx = q - qy * LX;
+			const int qE = qx + 1 == LX ? q - qx : q + 1, qW = qx == 0 ? q + LX - 1 : q - 1;
+			const int qN = qy + 1 == LY ? qx : q + LX, qS = qy == 0 ? q + (LY - 1) * LX : q - LX;
+			const double ph = s_phi[q], pE = s_phi[qE], pW = s_phi[qW], pN = s_phi[qN], pS = s_phi[qS];
+			const unsigned mm = m[u];
+			const double lap = a.wmask ? lap_masked(mm, pN, pS, pE, pW, ph) : pN + pS + pE + pW - 4 * ph;
+			const double dx = (mm & 3u) == 3u ? .5 * (pE - pW) : 0.;
+			const double dy = (mm & 12u) == 12u ? .5 * (pN - pS) : 0.;
+			const double p2q = p2[u];
+			const double CH = P.gamma * (8 * ph * (1 - ph) * (1 - 2 * ph) / P.lambda - 2 * P.lambda * lap);
+			const double A = -4 * P.mu_a0 * (1 - C.area / P.a0) * ph;
+			const double Rep = P.rcoef * ph * (p2q - ph * ph);
+			double Adh = 0.;
+			if(P.omega != 0) {
+				const double laplacianSquare = (mm & 16u) ? phi2[kE[u]] + phi2[kN[u]] + phi2[kW[u]] + phi2[kS[u]] - 4. * p2q
+						: phi2[kE[u]] + phi2[kW[u]] - 2. * p2q;   // a wall site
+				const double lsquare = 2 * ph * lap + 2 * (dx * dx + dy * dy);
+				const double dd = laplacianSquare - lsquare;
+				Adh = -4 * P.lambda * P.omega * (dd / sqrt(1 + dd * dd)) * ph;
+			}
+			double mu = CH + A + Rep + Adh;
+			if(a.wmask) mu = f_ext(P, ph, w[u]) + mu;   // freeEnergy starts at F_ext
+			const double CH_d = (P.gamma / P.lambda) * 4 * ph * ph * (1 - ph) * (1 - ph) + P.gamma * P.lambda * (dx * dx + dy * dy);
+			const double A_d = -P.mu / P.a0 * ph * ph;
+			const double Rep_d = (P.kappa / P.lambda) * ph * ph * (p2q - ph * ph);
+			double fd = CH_d + A_d + Rep_d;
+			if(a.wmask) fd = f_density_ext(P, ph, w[u]) + fd;
+			s_s[q] = fd - mu * ph;
+			s_dx[q] = dx;
+			s_dy[q] = dy;
+		}
 	}
 	__syncthreads();
 	const double gl = 2 * P.gamma * P.lambda;
 	const double *sq00 = a.sq00 + B.box_off, *sq01 = a.sq01 + B.box_off;
 	const bool dry = P.friction_cell == 0;
-	for(int q = tid; q < S; q += 256) {
-		const int qy = q / LX, qx = q - qy * LX;
-		const int qE = qx + 1 == LX ? q - qx : q + 1, qW = qx == 0 ? q + LX - 1 : q - 1;
-		const int qN = qy + 1 == LY ? qx : q + LX, qS = qy == 0 ? q + (LY - 1) * LX : q - LX;
-		const double f_passive_x = -0.5 * ((s_s[qE] + s_dx[qE] * (gl * s_dx[qE])) - (s_s[qW] + s_dx[qW] * (gl * s_dx[qW])))
-				- 0.5 * ((s_dx[qN] * (gl * s_dy[qN])) - (s_dx[qS] * (gl * s_dy[qS])));
-		const double f_passive_y = -0.5 * ((s_s[qN] + s_dy[qN] * (gl * s_dy[qN])) - (s_s[qS] + s_dy[qS] * (gl * s_dy[qS])))
-				- 0.5 * ((s_dy[qE] * (gl * s_dx[qE])) - (s_dy[qW] * (gl * s_dx[qW])));
-		const double dx = s_dx[q], dy = s_dy[q];
-		const double fQ_self_x = -(C.Q00 * dx + C.Q01 * dy);
-		const double fQ_self_y = -(C.Q01 * dx - C.Q00 * dy);
-		int kE, kW, kN, kS;
-		(void) box_site(qx, qy, kE, kW, kN, kS);
-		const double fQ_inter_x = -(0.5 * (sq00[kE] - sq00[kW]) + 0.5 * (sq01[kN] - sq01[kS])) - fQ_self_x;
-		const double fQ_inter_y = -(0.5 * (sq01[kE] - sq01[kW]) - 0.5 * (sq00[kN] - sq00[kS])) - fQ_self_y;
-		double fx = f_passive_x * P.passive_alpha + fQ_self_x * P.zetaQ_self + fQ_inter_x * P.zetaQ_inter;
-		double fy = f_passive_y * P.passive_alpha + fQ_self_y * P.zetaQ_self + fQ_inter_y * P.zetaQ_inter;
-		if(a.wmask && !(a.wmask[B.box_off + box_site(qx, qy, kE, kW, kN, kS)] & 16u)) fx = fy = 0.;   // wall site
-		if(dry) a.vrows[C.start + q] = make_double2(fx / P.friction, fy / P.friction);
-		else a.frows[C.start + q] = make_double2(fx, fy);
+	for(int q0 = tid; q0 < S; q0 += NB * 256) {
+		double e00[NB], w00[NB], n01[NB], s01[NB], e01[NB], w01[NB], n00[NB], s00[NB];
+		unsigned m[NB];
+#pragma unroll
+		for(int u = 0; u < NB; ++u) {
+			const int q = min(q0 + u * 256, S - 1);
+			const int qy = (int) __umulhi((unsigned) q, mlx), qx = q - qy * LX;
+			int kE, kW, kN, kS;
+			const int k = box_site(qx, qy, kE, kW, kN, kS);
+			e00[u] = sq00[kE]; w00[u] = sq00[kW]; n00[u] = sq00[kN]; s00[u] = sq00[kS];
+			e01[u] = sq01[kE]; w01[u] = sq01[kW]; n01[u] = sq01[kN]; s01[u] = sq01[kS];
+			m[u] = wmb ? wmb[k] : 31u;
+		}
+#pragma unroll
+		for(int u = 0; u < NB; ++u) {
+			const int q = q0 + u * 256;
+			if(q >= S) break;
+			const int qy = (int) __umulhi((unsigned) q, mlx), qx = q - qy * LX;
+			const int qE = qx + 1 == LX ? q - qx : q + 1, qW = qx == 0 ? q + LX - 1 : q - 1;
+			const int qN = qy + 1 == LY ? qx : q + LX, qS = qy == 0 ? q + (LY - 1) * LX : q - LX;
+			const double f_passive_x = -0.5 * ((s_s[qE] + s_dx[qE] * (gl * s_dx[qE])) - (s_s[qW] + s_dx[qW] * (gl * s_dx[qW])))
+					- 0.5 * ((s_dx[qN] * (gl * s_dy[qN])) - (s_dx[qS] * (gl * s_dy[qS])));
+			const double f_passive_y = -0.5 * ((s_s[qN] + s_dy[qN] * (gl * s_dy[qN])) - (s_s[qS] + s_dy[qS] * (gl * s_dy[qS])))
+					- 0.5 * ((s_dy[qE] * (gl * s_dx[qE])) - (s_dy[qW] * (gl * s_dx[qW])));
+			const double dx = s_dx[q], dy = s_dy[q];
+			const double fQ_self_x = -(C.Q00 * dx + C.Q01 * dy);
+			const double fQ_self_y = -(C.Q01 * dx - C.Q00 * dy);
+			const double fQ_inter_x = -(0.5 * (e00[u] - w00[u]) + 0.5 * (n01[u] - s01[u])) - fQ_self_x;
+			const double fQ_inter_y = -(0.5 * (e01[u] - w01[u]) - 0.5 * (n00[u] - s00[u])) - fQ_self_y;
+			double fx = f_passive_x * P.passive_alpha + fQ_self_x * P.zetaQ_self + fQ_inter_x * P.zetaQ_inter;
+			double fy = f_passive_y * P.passive_alpha + fQ_self_y * P.zetaQ_self + fQ_inter_y * P.zetaQ_inter;
+			if(a.wmask && !(m[u] & 16u)) fx = fy = 0.;   // wall site
+			if(dry) a.vrows[C.start + q] = make_double2(fx / P.friction, fy / P.friction);
+			else a.frows[C.start + q] = make_double2(fx, fy);
+		}
 	}
 }
 
