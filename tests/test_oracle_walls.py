@@ -10,6 +10,8 @@ Each oracle pin is also run against oracle builds with one planted mistake.
   shear_profile      ShearFlowChannel.cpp:41-45, FD_CPUBackend.cpp:62-72, BaseField.h:107-115
   pois_profile       PoiseuilleFlowChannel.cpp (velocity_profile)
   stretching         Stretching.cpp (velocity_profile of the cell index)
+  nem_wall_density   WetModel.cpp:540-610 with BaseField.h:67-79, ShearFlowChannel.cpp:33-46
+                     (the forces' free-energy density in the nematic stress)
 """
 import json
 import math
@@ -23,6 +25,7 @@ from oracle import OracleSim
 from rheocell_amd import confgen
 from rheocell_amd.conf import CellConf, Conf
 from rheocell_amd.walls import read_forces, wall_field
+from tests.test_oracle_nematic import _nkeys
 from tests.test_oracle_pins import _cell, _keys, _logical, build_planted
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -159,6 +162,30 @@ def pin_stretching(flavour="strict"):
         assert np.allclose(v, expect, rtol=1e-12, atol=1e-15), (i, v[:3], expect)
 
 
+def pin_nem_wall_density(flavour="strict"):
+    """The nematic model's stress is f - mu phi with the forces' density in f
+    and their free energy in mu: on a flat patch c across the bottom wall of a
+    square_walls box (gradients and Laplacians 0, repulsion 0 for one cell),
+    the stress jumps by (fD - fF) c^2 from an open site to a wall site, where
+    a ShearFlowChannel force has fF = 4 kw / lf and fD = 2 kw / lf.  An open
+    site whose south neighbour is a wall feels f_y = -1/2 (s_N - s_S) =
+    1/2 (fD - fF) c^2 (dry: v = f / xi, xi = 1); the open sites above it
+    feel nothing."""
+    L, LX, c, kw, lf = 48, 15, 0.4, 0.5, 3.0
+    w = wall_field("square_walls", L, 4.0)
+    forces = [dict(kind=1, kappa_wall=kw, **{"lambda": lf}, shear_rate=0.0)]
+    keys = _nkeys(friction_cell=0, friction=1.0, passive_alpha=1.0)
+    s = OracleSim(Conf(0, L, L, [_cell(L, 16, 0, LX, LX, lambda x, y: c)]), keys, flavour=flavour, walls=w,
+                  forces=forces)
+    lx, ly = _logical(LX, LX)
+    vy = s.rows(1)
+    inner = (lx >= 2) & (lx <= 12)
+    expect = 0.5 * (2.0 * kw / lf - 4.0 * kw / lf) * c * c
+    assert np.allclose(vy[inner & (ly == 4)], expect, rtol=1e-12, atol=0)
+    assert np.all(vy[inner & (ly >= 5) & (ly <= 10)] == 0)
+    assert np.all(vy[inner & (ly <= 3)] == 0)   # wall sites
+
+
 PINS = {name[4:]: fn for name, fn in list(globals().items()) if name.startswith("pin_")}
 
 
@@ -185,6 +212,8 @@ MUTANTS = [
     ("first_force_wins", "		V[0] = 0.;\n		V[1] = 0.;\n", "		if(f > 0) break;\n		V[0] = 0.;\n		V[1] = 0.;\n",
      ["shear_profile"]),
     ("rate_before_equilibration", "s->force[s->nforce].rate = 0.;", "s->force[s->nforce].rate = shear_rate;", ["shear_profile"]),
+    ("flow_density", "return 2.0 * kappa_wall / lambda * phi * phi * walls * walls;",
+     "return 4.0 * kappa_wall / lambda * phi * phi * walls * walls;", ["nem_wall_density"]),
 ]
 
 
