@@ -959,61 +959,76 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	}
 	const double r8[8] = { red[0], red[1], red[2], red[3], red[4], red[5], red[6], red[7] };
 	const double tot = block_sum8<NW>(r8, s_red);
-	if(tid >= 8) return;
-	// totals to lanes 0..3 of wave 0
-	const double area = __shfl(tot, 0, 64), sumF = __shfl(tot, 1, 64);
-	const double sxc = __shfl(tot, 2, 64), sxs = __shfl(tot, 3, 64);
-	const double syc = __shfl(tot, 4, 64), sys = __shfl(tot, 5, 64);
-	const double S00 = __shfl(tot, 6, 64), S01 = __shfl(tot, 7, 64);
-	const double trx0 = __shfl(trxy[0], 0, 64), try0 = __shfl(trxy[1], 0, 64);   // block_sum: thread 0
-	if(tid >= 4) return;
+	// the totals (threads 0..7) and the CIL sums (block_sum: thread 0) to LDS
+	__shared__ double s_tail[16];
+	if(tid < 8) s_tail[tid] = tot;
+	if(tid == 0) {
+		s_tail[8] = trxy[0];
+		s_tail[9] = trxy[1];
+	}
+	__syncthreads();
+	const double area = s_tail[0], sumF = s_tail[1];
+	const double sxc = s_tail[2], sxs = s_tail[3];
+	const double syc = s_tail[4], sys = s_tail[5];
+	const double S00 = s_tail[6], S01 = s_tail[7];
+	const double trx0 = s_tail[8], try0 = s_tail[9];
 
-	// ---- per-cell tail: lanes 0/1 take the two CoM atan2, lanes 2/3 the
-	// polarity angle and its cos / sin; lane 0 writes the cell ----
+	// ---- per-cell tail: lane 0 of waves 0 / 1 takes the two CoM atan2, of
+	// waves 2 / 3 the polarity angle and its cos / sin (one function per wave,
+	// so the four run side by side on four SIMDs instead of one after another
+	// in one wave); thread 0 writes the cell ----
 	const double trx = MODE == MODE_FUSED ? (do_rep ? trx0 : 0.) : C.trx;
 	const double try_ = MODE == MODE_FUSED ? (do_rep ? try0 : 0.) : C.try_;
-	double val = 0.;
-	double th_old = C.thetaQ_old, th = 0.;
+	const int twave = tid >> 6;
 	const bool nematic = MODE == MODE_FS && Q.nematic;
-	if(tid == 0) val = LX * (atan2(-sxs / sumF, -sxc / sumF) + PI_F) / (2 * PI_F);
-	else if(tid == 1) val = LY * (atan2(-sys / sumF, -syc / sumF) + PI_F) / (2 * PI_F);
-	else if(nematic) {
-		// WetModel::updateDirectedActiveForces (WetModel.cpp:657-670): the unit
-		// vector of the shape tensor's long axis, or the anchoring direction
-		// (update_anchoring, :703-716), which then replaces the director
-		double arg = 0.5 * atan2(S01, S00);
-		if(Q.anchoring) {
-			const double walls_length = 8;
-			const double com_old_y = store ? C.CoMy : C.CoM_oldy;
-			const double dist1 = (com_old_y - walls_length);
-			const double dist2 = ((double) L - walls_length) - com_old_y;
-			if(dist1 < dist2) arg = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
-			else arg = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) L - 2 * walls_length) / 2));
+	if((tid & 63) == 0 && twave < 4) {
+		double val = 0.;
+		double th_old = C.thetaQ_old, th = 0.;
+		if(twave == 0) val = LX * (atan2(-sxs / sumF, -sxc / sumF) + PI_F) / (2 * PI_F);
+		else if(twave == 1) val = LY * (atan2(-sys / sumF, -syc / sumF) + PI_F) / (2 * PI_F);
+		else if(nematic) {
+			// WetModel::updateDirectedActiveForces (WetModel.cpp:657-670): the unit
+			// vector of the shape tensor's long axis, or the anchoring direction
+			// (update_anchoring, :703-716), which then replaces the director
+			double arg = 0.5 * atan2(S01, S00);
+			if(Q.anchoring) {
+				const double walls_length = 8;
+				const double com_old_y = store ? C.CoMy : C.CoM_oldy;
+				const double dist1 = (com_old_y - walls_length);
+				const double dist2 = ((double) L - walls_length) - com_old_y;
+				if(dist1 < dist2) arg = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
+				else arg = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) L - 2 * walls_length) / 2));
+			}
+			val = twave == 2 ? cos(arg) : sin(arg);
 		}
-		val = tid == 2 ? cos(arg) : sin(arg);
+		else {
+			// WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:365-396)
+			if(store) {
+				const double g = a.gauss ? a.gauss[ci] : 0.0;
+				th_old = C.thetaQ + Q.sqrt_dt * Q.zetaQ_inter * g;
+			}
+			th = th_old - Q.dt * Q.J_Q * (atan2(trx * C.Q01 - try_ * C.Q00, trx * C.Q00 + try_ * C.Q01));
+			double arg = th;
+			if(Q.anchoring) {  // update_anchoring (WetPolarModel.cpp:417-430)
+				const double walls_length = 8;
+				const double com_old_y = store ? C.CoMy : C.CoM_oldy;
+				const double dist1 = (com_old_y - walls_length);
+				const double dist2 = ((double) L - walls_length) - com_old_y;
+				if(dist1 < dist2) arg = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
+				else arg = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) L - 2 * walls_length) / 2));
+			}
+			val = twave == 2 ? cos(arg) : sin(arg);
+		}
+		s_tail[10 + twave] = val;
+		if(twave == 2) {
+			s_tail[14] = th;
+			s_tail[15] = th_old;
+		}
 	}
-	else {
-		// WetPolarModel::updateDirectedActiveForces (WetPolarModel.cpp:365-396)
-		if(store) {
-			const double g = a.gauss ? a.gauss[ci] : 0.0;
-			th_old = C.thetaQ + Q.sqrt_dt * Q.zetaQ_inter * g;
-		}
-		th = th_old - Q.dt * Q.J_Q * (atan2(trx * C.Q01 - try_ * C.Q00, trx * C.Q00 + try_ * C.Q01));
-		double arg = th;
-		if(Q.anchoring) {  // update_anchoring (WetPolarModel.cpp:417-430)
-			const double walls_length = 8;
-			const double com_old_y = store ? C.CoMy : C.CoM_oldy;
-			const double dist1 = (com_old_y - walls_length);
-			const double dist2 = ((double) L - walls_length) - com_old_y;
-			if(dist1 < dist2) arg = (PI_F / 2) - (PI_F / 2) * dist1 / (((double) L - 2 * walls_length) / 2);
-			else arg = (PI_F / 2) + ((PI_F / 2) * dist2 / (((double) L - 2 * walls_length) / 2));
-		}
-		val = tid == 2 ? cos(arg) : sin(arg);
-	}
-	double CoMx = __shfl(val, 0, 64), CoMy = __shfl(val, 1, 64);
-	const double c2 = __shfl(val, 2, 64), s3 = __shfl(val, 3, 64);
-	th = __shfl(th, 2, 64);
-	th_old = __shfl(th_old, 2, 64);
+	__syncthreads();
+	double CoMx = s_tail[10], CoMy = s_tail[11];
+	const double c2 = s_tail[12], s3 = s_tail[13];
+	double th = s_tail[14], th_old = s_tail[15];
 	if(tid != 0) return;
 	if(a.pure) return;   // timing run (shape tuner): no cell state is written
 	if(!(area == area) || !(sumF == sumF)) {
