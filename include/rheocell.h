@@ -199,6 +199,19 @@ int rc_box_field_noalloc(rc_handle *h, int box, int which, int size_grid, double
 		double *cy);
 int rc_cg_stats(rc_handle *h, rc_cg_stats_t *out);
 int rc_set_warm_start(rc_handle *h, int mode);
+/* Form of the wet solve's Jacobi-PCG (no counterpart in the reference, which
+ * runs Eigen's ConjugateGradient on the assembled rows, WetPolarModel.cpp:200-233).
+ * Both forms run Eigen's iteration from the same start with the same stop
+ * test; they differ in rounding only.  RC_CG_SITES (default): the iterates
+ * are carried as scalar multiples of the first residual's part that sums to
+ * zero at every box site plus L x L box fields, since every row's diagonal
+ * depends on its box site only; an iteration moves O(L^2) data.
+ * RC_CG_ROWS: every iterate as rows; an iteration moves O(rows) data.  The
+ * learned start's refinement stages and rc_record_solves run on rows in
+ * either form. */
+#define RC_CG_SITES 0
+#define RC_CG_ROWS 1
+int rc_set_cg_form(rc_handle *h, int form);
 /* Kernel timing with HIP events on the library's stream (enabled by
  * rc_enable_timing).  Single-kernel kinds record their events at kernel
  * start and end (hipExtLaunchKernel), so they time the kernel alone;
@@ -225,7 +238,14 @@ int rc_set_warm_start(rc_handle *h, int mode);
 #define RC_T_CG_LEARNED 17         /* the learned refinement stages of one solve (group) */
 #define RC_T_NEM_SUMS 19           /* k_box_field: nematic box sums of phi Q00, phi Q01 */
 #define RC_T_NEM_FORCE 18          /* k_nem_force: the nematic model's force evaluation */
-#define RC_NKINDS 20
+#define RC_T_SCG_GATHER 20         /* site form: box sums of b and of the start x0 */
+#define RC_T_SCG_INIT 21           /* site form: per-site set-up */
+#define RC_T_SCG_ROW_INIT 22       /* site form: ker-P residual sums over the rows */
+#define RC_T_SCG_DIR 23            /* site form: beta, p's site part q, its A-image w, q.m w */
+#define RC_T_SCG_UPD 24            /* site form: alpha, x's and r's site parts */
+#define RC_T_SCG_FINAL 25          /* site form: the solution rows */
+#define RC_T_SCG_REDUCE 26         /* site form: set-up reduction and the stop test after a batch */
+#define RC_NKINDS 27
 int rc_timing(rc_handle *h, int kind, double *ms, long long *launches);
 int rc_enable_timing(rc_handle *h, int on);
 const char *rc_last_error(rc_handle *h);

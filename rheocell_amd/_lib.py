@@ -16,18 +16,23 @@ RC_ERR = {1: "RC_ERR_ARG", 2: "RC_ERR_HIP", 3: "RC_ERR_OVERLAP", 4: "RC_ERR_GROW
 RC_WARM_REFERENCE = 0
 RC_WARM_PREVIOUS = 1
 RC_WARM_LEARNED = 2
+RC_CG_SITES = 0
+RC_CG_ROWS = 1
 
 # timing kinds (rheocell.h RC_T_*)
 T_KINDS = ["box_gather", "stage_fused_pred", "stage_fused_corr", "stage_cf", "stage_fs_pred", "stage_fs_corr",
            "cg_diag", "cg_gather_x", "cg_gather_p", "cg_row_resid", "cg_box_pap", "cg_row_update", "cg_box_n4",
-           "cg_solve", "step", "cg_gather_p_first", "cg_reduce", "cg_learned", "nem_force", "nem_sums"]
+           "cg_solve", "step", "cg_gather_p_first", "cg_reduce", "cg_learned", "nem_force", "nem_sums",
+           "scg_gather", "scg_init", "scg_row_init", "scg_dir", "scg_upd", "scg_final", "scg_reduce"]
 # kernel symbol of each timing kind (as rocprofv3 names it)
 T_SYMBOL = {"box_gather": "k_box_gather", "stage_fused_pred": "k_stage<0>", "stage_fused_corr": "k_stage<0>",
             "stage_cf": "k_stage<1>", "stage_fs_pred": "k_stage<2>", "stage_fs_corr": "k_stage<2>",
             "cg_diag": "k_cg_diag", "cg_gather_x": "k_cg_gather<0>", "cg_gather_p_first": "k_cg_gather<1>",
             "cg_gather_p": "k_cg_gather<2>", "cg_row_resid": "k_cg_row<0>", "cg_box_pap": "k_cg_box<1>",
             "cg_row_update": "k_cg_row<2>", "cg_box_n4": "k_cg_box<0>", "cg_reduce": "k_cg_reduce",
-            "nem_force": "k_nem_force", "nem_sums": "k_box_field"}
+            "nem_force": "k_nem_force", "nem_sums": "k_box_field", "scg_gather": "k_cg_gather<5>",
+            "scg_init": "k_scg_init", "scg_row_init": "k_scg_row_init", "scg_dir": "k_scg_dir",
+            "scg_upd": "k_scg_upd", "scg_final": "k_scg_final", "scg_reduce": "k_scg_"}
 
 EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_after_equilibration", "rc_step",
            "rc_download_cells", "rc_get_rows", "rc_get_site_array", "rc_total_rows", "rc_cg_stats",
@@ -36,7 +41,7 @@ EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_a
            "rc_net_train_step", "rc_net_last_error", "rc_net_destroy", "rc_attach_net", "rc_snapshot_box",
            "rc_record_solves", "rc_get_recorded", "rc_set_learned_stages", "rc_set_stage_shape",
            "rc_stage_groups", "rc_stage_persistent", "rc_set_graphs", "rc_graph_stats",
-           "rc_growth_stats", "rc_box_field", "rc_box_field_noalloc"]
+           "rc_growth_stats", "rc_box_field", "rc_box_field_noalloc", "rc_set_cg_form"]
 
 
 class rc_model_params(C.Structure):
@@ -108,6 +113,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.rc_total_rows.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.rc_cg_stats.argtypes = [vp, C.POINTER(rc_cg_stats_t)]
     L.rc_set_warm_start.argtypes = [vp, ip]
+    L.rc_set_cg_form.argtypes = [vp, ip]
     L.rc_timing.argtypes = [vp, ip, dp, C.POINTER(C.c_longlong)]
     L.rc_enable_timing.argtypes = [vp, ip]
     L.rc_last_error.argtypes = [vp]

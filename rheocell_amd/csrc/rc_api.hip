@@ -83,6 +83,12 @@ struct rc_handle {
 	long long gauss_cap = 0;
 	double *d_out[4] = { nullptr, nullptr, nullptr, nullptr };
 	CGState *h_cg = nullptr;      // pinned
+	// site-space form of the CG (SCGState)
+	int cg_form = RC_CG_SITES;
+	double2 *d_sc = nullptr, *d_sy = nullptr, *d_ss = nullptr, *d_sw = nullptr, *d_sq0 = nullptr, *d_sq1 = nullptr;
+	double *d_part_p = nullptr, *d_part_r0 = nullptr, *d_part_r1 = nullptr;
+	SCGState *d_scg = nullptr;    // [2]
+	SCGState *h_scg = nullptr;    // pinned
 	hipEvent_t cg_e0 = nullptr, cg_e1 = nullptr;
 	DevStatus *h_st = nullptr;    // pinned
 
@@ -469,6 +475,14 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, dalloc(&h->d_part, (size_t) std::max(6 * std::max(cells, 1), 2 * tiles)));
 		HIPCHK(h, dalloc(&h->d_cg, 1));
 		HIPCHK(h, hipMemset(h->d_cg, 0, sizeof(CGState)));
+		const size_t nsb = (sites + 255) / 256;
+		for(double2 **f : { &h->d_sc, &h->d_sy, &h->d_ss, &h->d_sw, &h->d_sq0, &h->d_sq1 }) HIPCHK(h, dalloc(f, sites));
+		HIPCHK(h, dalloc(&h->d_part_p, 2 * nsb));
+		HIPCHK(h, dalloc(&h->d_part_r0, 4 * nsb));
+		HIPCHK(h, dalloc(&h->d_part_r1, 4 * nsb));
+		HIPCHK(h, dalloc(&h->d_scg, 2));
+		HIPCHK(h, hipMemset(h->d_scg, 0, 2 * sizeof(SCGState)));
+		HIPCHK(h, hipHostMalloc((void **) &h->h_scg, sizeof(SCGState)));
 	}
 	return RC_OK;
 }
@@ -488,6 +502,9 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_ggauss); dfree(h->d_step_base); dfree(h->d_bf);
 	if(h->h_step_base) (void) hipHostFree(h->h_step_base); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
+	dfree(h->d_sc); dfree(h->d_sy); dfree(h->d_ss); dfree(h->d_sw); dfree(h->d_sq0); dfree(h->d_sq1);
+	dfree(h->d_part_p); dfree(h->d_part_r0); dfree(h->d_part_r1); dfree(h->d_scg);
+	if(h->h_scg) (void) hipHostFree(h->h_scg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
 	for(auto e : h->pool) (void) hipEventDestroy(e);
 	if(h->cg_e0) (void) hipEventDestroy(h->cg_e0);
@@ -862,6 +879,38 @@ static CGArgs cg_args(rc_handle *h) {
 	return c;
 }
 
+static SCGArgs scg_args(rc_handle *h, const CGArgs &c) {
+	SCGArgs a{};
+	a.boxes = h->d_boxes;
+	a.cells = h->d_cells;
+	a.params = h->d_params;
+	a.set = h->cur_set;
+	a.cover = h->d_cover;
+	a.diag = h->d_diag;
+	a.sb = h->d_sp;
+	a.sx = h->d_p2;
+	a.c = h->d_sc;
+	a.y = h->d_sy;
+	a.s = h->d_ss;
+	a.w = h->d_sw;
+	a.q0 = h->d_sq0;
+	a.q1 = h->d_sq1;
+	a.b = h->d_f;
+	a.x = h->d_v;
+	a.part = h->d_part;
+	a.part_p = h->d_part_p;
+	a.part_r0 = h->d_part_r0;
+	a.part_r1 = h->d_part_r1;
+	a.s2 = h->d_scg;
+	a.tol = c.tol;
+	a.max_iter = c.max_iter;
+	a.ncells = h->ncells;
+	a.nsites = (int) h->box_sites;
+	a.nsb = (int) ((h->box_sites + 255) / 256);
+	a.st = h->d_st;
+	return a;
+}
+
 // initial residual |b - A x|^2 (x + y components) of the rows in c.x
 static int resid_norm(rc_handle *h, const CGArgs &c, double *rn2) {
 	HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream));
@@ -954,21 +1003,94 @@ static int record_end(rc_handle *h) {
 	return RC_OK;
 }
 
+// Site form of the iteration (SCGState), from the start rows in d_v: the box
+// sums of b and x0, the set-up, then iterations in host batches, each ended
+// by a stop test the host reads (batch sizes as in the row form); the
+// solution rows replace the start in d_v and the last state is in h_scg.
+static int scg_iterations(rc_handle *h, const CGArgs &c, bool cold) {
+	const SCGArgs a = scg_args(h, c);
+	{ TimeScope t(h, RC_T_SCG_GATHER); HIPCHK(h, launch_cg_gather(CG_GATHER_BX, c, h->ntiles, h->stream)); }
+	{ TimeScope t(h, RC_T_SCG_INIT); HIPCHK(h, launch_scg(SCG_INIT, a, 0, h->stream)); }
+	{ TimeScope t(h, RC_T_SCG_ROW_INIT); HIPCHK(h, launch_scg(SCG_ROW_INIT, a, 0, h->stream)); }
+	{ TimeScope t(h, RC_T_SCG_REDUCE); HIPCHK(h, launch_scg(SCG_SETUP_REDUCE, a, 0, h->stream)); }
+	int batch = std::max(2, h->last_iters[cold ? 1 : 0] - 1), next = 2;
+	int j = 0;
+	for(;;) {
+		for(int it = 0; it < batch; ++it, ++j) {
+			{ TimeScope t(h, RC_T_SCG_DIR); HIPCHK(h, launch_scg(SCG_DIR, a, j, h->stream)); }
+			{ TimeScope t(h, RC_T_SCG_UPD); HIPCHK(h, launch_scg(SCG_UPD, a, j, h->stream)); }
+		}
+		{ TimeScope t(h, RC_T_SCG_REDUCE); HIPCHK(h, launch_scg(SCG_CHECK, a, j, h->stream)); }
+		HIPCHK(h, hipMemcpyAsync(h->h_scg, h->d_scg + (j & 1), sizeof(SCGState), hipMemcpyDeviceToHost, h->stream));
+		HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
+		HIPCHK(h, hipStreamSynchronize(h->stream));
+		if(h->h_st->flags) break;
+		if(!h->h_scg->active_now) break;
+		if(j >= c.max_iter + 2) break;
+		batch = next;
+		next = std::min(2 * next, 64);
+	}
+	{ TimeScope t(h, RC_T_SCG_FINAL); HIPCHK(h, launch_scg(SCG_FINAL, a, j, h->stream)); }
+	return RC_OK;
+}
+
+// Row form of the iteration (CGState): p = z fused with the box sums of p,
+// then iterations in host batches; the solution rows in d_v.
+static int cg_row_iterations(rc_handle *h, CGArgs c, bool cold) {
+	{ TimeScope t(h, RC_T_CG_GATHER_P_FIRST); HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream)); }
+	// First batch: one iteration short of the last solve of the same kind
+	// (warm or cold: a cold solve needs several times the iterations);
+	// then batches of 2, 4, 8, ... so that a poor guess costs few host
+	// round trips.  Iterations past convergence return at entry.
+	int batch = std::max(2, h->last_iters[cold ? 1 : 0] - 1), next = 2;
+	long long done_iters = 0;
+	for(;;) {
+		// an iteration's state update is written by the next iteration's
+		// k_cg_box<1> (its beta is derived in k_cg_gather), the batch's last
+		// by k_cg_reduce<2>
+		for(int it = 0; it < batch; ++it) {
+			c.apply_update = it > 0 ? 1 : 0;
+			{ TimeScope t(h, RC_T_CG_BOX_PAP); HIPCHK(h, launch_cg_box(1, c, (int) h->box_sites, h->stream)); }
+			{ TimeScope t(h, RC_T_CG_ROW_UPDATE); HIPCHK(h, launch_cg_row(CG_ROW_UPDATE, c, h->stream)); }
+			{ TimeScope t(h, RC_T_CG_GATHER_P); HIPCHK(h, launch_cg_gather(CG_GATHER_P_UPDATE, c, h->ntiles, h->stream)); }
+		}
+		c.apply_update = 0;
+		{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(2, c, h->ncells, h->stream)); }
+		done_iters += batch;
+		HIPCHK(h, hipMemcpyAsync(h->h_cg, h->d_cg, sizeof(CGState), hipMemcpyDeviceToHost, h->stream));
+		HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
+		HIPCHK(h, hipStreamSynchronize(h->stream));
+		if(h->h_st->flags) break;
+		if(!h->h_cg->active) break;
+		if(done_iters >= c.max_iter + 2) break;
+		batch = next;
+		next = std::min(2 * next, 64);
+	}
+	const CGState &s = *h->h_cg;
+	int mask = (s.zero_rhs[0] ? 1 : 0) | (s.zero_rhs[1] ? 2 : 0);
+	if(mask) HIPCHK(h, launch_zero_rows(h->d_v, h->rows, mask, h->stream));
+	return RC_OK;
+}
+
 // Eigen ConjugateGradient solve (solve() when cold, solveWithGuess() otherwise)
 static int cg_solve(rc_handle *h, bool cold) {
 	CGArgs c = cg_args(h);
+	const bool sites = h->cg_form == RC_CG_SITES;
 	hipEvent_t e0 = h->cg_e0, e1 = h->cg_e1;
 	HIPCHK(h, hipEventRecord(e0, h->stream));
 	{
 		TimeScope ts(h, RC_T_CG_SOLVE);
 		if(cold) HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * h->rows, h->stream));
-		const int nbox_blk = (int) ((h->box_sites + 255) / 256);
 		{ TimeScope t(h, RC_T_CG_DIAG); HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }
-		{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream)); }
 		const bool learned = h->warm_mode == RC_WARM_LEARNED && h->net;
+		// the row form's start (residual rows and their sums): the row
+		// iteration, the learned stages and the training records work on it
+		if(!sites || learned || h->d_rec) {
+			{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
+			{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
+			{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }
+			{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(0, c, h->ncells, h->stream)); }
+		}
 		if(h->d_rec) {
 			int rc = record_stage(h, c);
 			if(rc) return rc;
@@ -998,66 +1120,51 @@ static int cg_solve(rc_handle *h, bool cold) {
 			h->cgs.learned_tried += 1;
 			h->pending_learned_stat = true;
 		}
-		{ TimeScope t(h, RC_T_CG_GATHER_P_FIRST); HIPCHK(h, launch_cg_gather(CG_GATHER_P_FIRST, c, h->ntiles, h->stream)); }
-		// First batch: one iteration short of the last solve of the same kind
-		// (warm or cold: a cold solve needs several times the iterations);
-		// then batches of 2, 4, 8, ... so that a poor guess costs few host
-		// round trips.  Iterations past convergence return at entry.
-		int batch = std::max(2, h->last_iters[cold ? 1 : 0] - 1), next = 2;
-		long long done_iters = 0;
-		for(;;) {
-			// an iteration's state update is written by the next iteration's
-			// k_cg_box<1> (its beta is derived in k_cg_gather), the batch's last
-			// by k_cg_reduce<2>
-			for(int it = 0; it < batch; ++it) {
-				c.apply_update = it > 0 ? 1 : 0;
-				{ TimeScope t(h, RC_T_CG_BOX_PAP); HIPCHK(h, launch_cg_box(1, c, (int) h->box_sites, h->stream)); }
-				{ TimeScope t(h, RC_T_CG_ROW_UPDATE); HIPCHK(h, launch_cg_row(CG_ROW_UPDATE, c, h->stream)); }
-				{ TimeScope t(h, RC_T_CG_GATHER_P); HIPCHK(h, launch_cg_gather(CG_GATHER_P_UPDATE, c, h->ntiles, h->stream)); }
-			}
-			c.apply_update = 0;
-			{ TimeScope t(h, RC_T_CG_REDUCE); HIPCHK(h, launch_cg_reduce(2, c, h->ncells, h->stream)); }
-			done_iters += batch;
-			HIPCHK(h, hipMemcpyAsync(h->h_cg, h->d_cg, sizeof(CGState), hipMemcpyDeviceToHost, h->stream));
-			HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
-			HIPCHK(h, hipStreamSynchronize(h->stream));
-			if(h->h_st->flags) break;
-			if(!h->h_cg->active) break;
-			if(done_iters >= c.max_iter + 2) break;
-			batch = next;
-			next = std::min(2 * next, 64);
+		if(sites) {
+			int rc = scg_iterations(h, c, cold);
+			if(rc) return rc;
+			// the learned start's fallback flags, for the statistics below
+			if(h->pending_learned_stat) HIPCHK(h, hipMemcpyAsync(h->h_cg, h->d_cg, sizeof(CGState), hipMemcpyDeviceToHost, h->stream));
 		}
-		const CGState &s = *h->h_cg;
-		int mask = (s.zero_rhs[0] ? 1 : 0) | (s.zero_rhs[1] ? 2 : 0);
-		if(mask) HIPCHK(h, launch_zero_rows(h->d_v, h->rows, mask, h->stream));
+		else {
+			int rc = cg_row_iterations(h, c, cold);
+			if(rc) return rc;
+		}
 	}
 	HIPCHK(h, hipEventRecord(e1, h->stream));
 	HIPCHK(h, hipEventSynchronize(e1));
 	float ms = 0.f;
 	(void) hipEventElapsedTime(&ms, e0, e1);
-	const CGState &s = *h->h_cg;
 	if(h->pending_learned_stat) {
-		if(!s.reverted[0] && !s.reverted[1]) h->cgs.learned_used += 1;
+		if(!h->h_cg->reverted[0] && !h->h_cg->reverted[1]) h->cgs.learned_used += 1;
 		h->pending_learned_stat = false;
 	}
 	if(h->rec_open) {
 		int rc = record_end(h);
 		if(rc) return rc;
 	}
-	h->cgs.solves += 2;
-	h->cgs.iters_total += s.iters[0] + s.iters[1];
-	h->cgs.sweeps_total += std::max(s.iters[0], s.iters[1]);
-	h->cgs.last_iters[0] = s.iters[0];
-	h->cgs.last_iters[1] = s.iters[1];
+	// iterations, final |r|^2 and |b|^2 of each component, from either form's state
+	int iters[2];
+	double rn2[2], bb[2];
 	for(int k = 0; k < 2; ++k) {
-		h->cgs.last_relres[k] = s.bb[k] > 0 ? std::sqrt(s.rn2[k] / s.bb[k]) : 0.;
+		iters[k] = sites ? h->h_scg->iters[k] : h->h_cg->iters[k];
+		rn2[k] = sites ? h->h_scg->rn2_now[k] : h->h_cg->rn2[k];
+		bb[k] = sites ? h->h_scg->bb[k] : h->h_cg->bb[k];
+	}
+	h->cgs.solves += 2;
+	h->cgs.iters_total += iters[0] + iters[1];
+	h->cgs.sweeps_total += std::max(iters[0], iters[1]);
+	h->cgs.last_iters[0] = iters[0];
+	h->cgs.last_iters[1] = iters[1];
+	for(int k = 0; k < 2; ++k) {
+		h->cgs.last_relres[k] = bb[k] > 0 ? std::sqrt(rn2[k] / bb[k]) : 0.;
 		h->cgs.worst_relres = std::max(h->cgs.worst_relres, h->cgs.last_relres[k]);
 	}
 	h->cgs.last_cold = cold ? 1 : 0;
 	h->cgs.last_solve_ms = ms;
 	h->cgs.total_solve_ms += ms;
-	h->last_iters[cold ? 1 : 0] = std::max(s.iters[0], s.iters[1]);
-	if(s.iters[0] >= c.max_iter || s.iters[1] >= c.max_iter) h->cgs.last_cold |= 2;
+	h->last_iters[cold ? 1 : 0] = std::max(iters[0], iters[1]);
+	if(iters[0] >= c.max_iter || iters[1] >= c.max_iter) h->cgs.last_cold |= 2;
 	return RC_OK;
 }
 
@@ -1172,6 +1279,13 @@ extern "C" int rc_init_forces(rc_handle *h) {
 extern "C" int rc_apply_changes_after_equilibration(rc_handle *h) {
 	if(!h) return RC_ERR_ARG;
 	h->cur_set = 1;
+	return RC_OK;
+}
+
+extern "C" int rc_set_cg_form(rc_handle *h, int form) {
+	if(!h) return RC_ERR_ARG;
+	if(form != RC_CG_SITES && form != RC_CG_ROWS) return fail(h, RC_ERR_ARG, "unknown CG form %d", form);
+	h->cg_form = form;
 	return RC_OK;
 }
 

@@ -81,4 +81,26 @@ struct CGState {
 	int reverted[2];    // learned start dropped for the previous solution
 };
 
+// Site-space form of the same CG.  Every row sits at one box site k_i, and the
+// diagonal D(k_i) depends on the site only, so A = D - eta P^T Nb P (P: sum of
+// the rows at each site, Nb: the four box neighbours) maps the rows that sum
+// to zero at every site (ker P) onto themselves (as D) and the site-constant
+// rows P^T y onto themselves (as P^T (D y - eta Nb(m y)), m = rows per site);
+// the Jacobi preconditioner D keeps both too.  Eigen's iterates are therefore
+//   r = rho r_perp + P^T y,   p = pi z_perp + P^T q,   x = x0 + xi z_perp + P^T s
+// with r_perp the ker-P part of the first residual and z_perp = r_perp / D:
+// scalars rho, pi, xi per component and L x L fields y, q, s.  The inner
+// products split the same way (r_perp . P^T y = 0, P^T a . P^T b = a . m b).
+// Two copies, read from [j & 1] and written to [(j + 1) & 1] by iteration j.
+struct SCGState {
+	double bb[2], thr[2], rn2[2];
+	double rperp2[2], rzperp[2];   // |r_perp|^2, r_perp . (r_perp / D)
+	double rho[2], pi[2], xi[2];
+	double absNew[2], alpha[2], beta[2];
+	int done[2], iters[2], zero_rhs[2], max_iter, active;
+	// stop test of the last update, written by k_scg_check after a batch
+	double rn2_now[2];
+	int done_now[2], active_now, pad;
+};
+
 }  // namespace rc

@@ -61,7 +61,9 @@ struct GatherArgs {
 	DevStatus *st;
 };
 
-enum CGGatherMode { CG_GATHER_X = 0, CG_GATHER_P_FIRST = 1, CG_GATHER_P_UPDATE = 2, CG_GATHER_FEAT = 3, CG_GATHER_E = 4 };
+// CG_GATHER_BX: box sums of b (into sp) and of x (into p2) in one pass
+enum CGGatherMode { CG_GATHER_X = 0, CG_GATHER_P_FIRST = 1, CG_GATHER_P_UPDATE = 2, CG_GATHER_FEAT = 3, CG_GATHER_E = 4,
+	CG_GATHER_BX = 5 };
 enum CGRowMode { CG_ROW_RESID = 0, CG_ROW_UPDATE = 2, CG_ROW_STEP = 5, CG_ROW_REVERT = 6 };
 
 struct CGArgs {
@@ -192,6 +194,41 @@ hipError_t launch_cg_reduce(int what, const CGArgs &a, int nblk, hipStream_t s);
 hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap, int chan, const double *rn2, double rows,
 		int nsites, hipStream_t s);
 hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
+
+// Site-space CG (SCGState): one wet box of nsites = L*L sites, box sums of b
+// and x0 in sb and sx (k_cg_gather<CG_GATHER_BX>), D in diag, rows per site in
+// cover.  Site passes run nsb = ceil(nsites / 256) workgroups of 256 sites.
+struct SCGArgs {
+	const DevBox *boxes;
+	const DevCell *cells;
+	const DevParams *params;
+	int set;
+	const int *cover;
+	const double *diag;
+	const double2 *sb, *sx;      // box sums of b and of x0
+	double2 *c;                  // b_bar - D x0_bar per site (site means of b, x0)
+	double2 *y, *s, *w;          // site parts of r, of x - x0, of A p
+	double2 *q0, *q1;            // site part of p: iteration j writes q_j to (j & 1 ? q1 : q0)
+	const double2 *b;            // right-hand side rows
+	double2 *x;                  // start rows in, solution rows out
+	double *part;                // row-pass partials, 6 per cell
+	double *part_p;              // site partials of q . m w, [2][nsb]
+	double *part_r0, *part_r1;   // site partials of y . m y, y . m y / D, [4][nsb]; iteration j writes (j & 1 ? r1 : r0)
+	SCGState *s2;                // [2]
+	double tol;
+	int max_iter;
+	int ncells, nsb, nsites;
+	DevStatus *st;
+};
+// launches of the site-space CG, in solve order:
+//   SCG_INIT          per site: c, y = c + eta N4(sx), s = 0 and their partials
+//   SCG_ROW_INIT      per cell: |b|^2 and the ker-P residual sums over the rows
+//   SCG_SETUP_REDUCE  one workgroup: Eigen's start into SCGState [0]
+//   SCG_DIR, SCG_UPD  iteration j: beta, q_j, w and q.m w; then alpha, s, y and SCGState [(j + 1) & 1]
+//   SCG_CHECK         the stop test of the state entering iteration j (after a batch)
+//   SCG_FINAL         x = x0 + xi z_perp + P^T s from the state entering iteration j
+enum SCGLaunch { SCG_INIT = 0, SCG_ROW_INIT, SCG_SETUP_REDUCE, SCG_DIR, SCG_UPD, SCG_CHECK, SCG_FINAL };
+hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s);
 // map[j] = {old row start, new row start, rows}: dst[new + q] = src[old + q] (and src2 -> dst2 if set)
 // growth re-cut of listed patches (k_grow_rows; 4 int4 per entry), at most
 // GROW_LDS_MAX / 4 sites per new patch

@@ -2213,7 +2213,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 	if(a.st->flags) return;
 	const CGState *cg = a.cg;
-	if(MODE != CG_GATHER_X && !cg->active) return;
+	if(MODE != CG_GATHER_X && MODE != CG_GATHER_BX && !cg->active) return;
 	const int tile = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
 	const DevBox B = a.boxes[a.tile_box[tile]];
 	const int n = list_count(a.tile_count, tile);
@@ -2275,7 +2275,7 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 	double inv = 0.;
 	const bool d0 = dn[0], d1 = dn[1];
 	const double b0 = bt[0], b1 = bt[1];
-	if(MODE != CG_GATHER_X) inv = 1. / a.diag[k];   // Jacobi: z = invdiag .* r
+	if(MODE != CG_GATHER_X && MODE != CG_GATHER_BX) inv = 1. / a.diag[k];   // Jacobi: z = invdiag .* r
 	double2 acc = make_double2(0., 0.), acc2 = make_double2(0., 0.);
 	for(int base = 0; base < n; base += GCHUNK) {
 		int row[GCHUNK];
@@ -2289,6 +2289,10 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 			if(row[q] >= 0) {
 				if(MODE == CG_GATHER_X) rv[q] = a.x[row[q]];
 				else if(MODE == CG_GATHER_FEAT) rv[q] = a.r[row[q]];
+				else if(MODE == CG_GATHER_BX) {
+					rv[q] = a.b[row[q]];
+					pv[q] = a.x[row[q]];
+				}
 				else {
 					rv[q] = a.r[row[q]];
 					if(MODE == CG_GATHER_P_UPDATE) pv[q] = a.p[row[q]];
@@ -2302,7 +2306,12 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 		for(int q = 0; q < GCHUNK; ++q) {
 			if(row[q] < 0) continue;
 			double2 v;
-			if(MODE == CG_GATHER_X || MODE == CG_GATHER_FEAT) v = rv[q];
+			if(MODE == CG_GATHER_BX) {
+				v = rv[q];
+				acc2.x += pv[q].x;
+				acc2.y += pv[q].y;
+			}
+			else if(MODE == CG_GATHER_X || MODE == CG_GATHER_FEAT) v = rv[q];
 			else {
 				if(MODE == CG_GATHER_P_FIRST) v = make_double2(inv * rv[q].x, inv * rv[q].y);
 				else {
@@ -2331,7 +2340,7 @@ __global__ __launch_bounds__(256) void k_cg_gather(CGArgs a) {
 		return;
 	}
 	a.sp[k] = acc;
-	if(MODE != CG_GATHER_X) a.p2[k] = acc2;
+	if(MODE != CG_GATHER_X) a.p2[k] = acc2;   // BX: the box sums of x
 }
 
 hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s) {
@@ -2340,6 +2349,7 @@ hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s
 	case CG_GATHER_P_FIRST: RC_LAUNCH(k_cg_gather<CG_GATHER_P_FIRST>, dim3(ntiles), dim3(256), 0, s, a); break;
 	case CG_GATHER_FEAT: RC_LAUNCH(k_cg_gather<CG_GATHER_FEAT>, dim3(ntiles), dim3(256), 0, s, a); break;
 	case CG_GATHER_E: RC_LAUNCH(k_cg_gather<CG_GATHER_E>, dim3(ntiles), dim3(256), 0, s, a); break;
+	case CG_GATHER_BX: RC_LAUNCH(k_cg_gather<CG_GATHER_BX>, dim3(ntiles), dim3(256), 0, s, a); break;
 	default: RC_LAUNCH(k_cg_gather<CG_GATHER_P_UPDATE>, dim3(ntiles), dim3(256), 0, s, a); break;
 	}
 	return hipGetLastError();
@@ -2706,6 +2716,421 @@ __global__ void k_zero_rows(double2 *x, long long rows, int mask) {
 hipError_t launch_zero_rows(double2 *x, long long rows, int mask, hipStream_t s) {
 	if(rows <= 0) return hipSuccess;
 	RC_LAUNCH(k_zero_rows, dim3((unsigned) ((rows + 255) / 256)), dim3(256), 0, s, x, rows, mask);
+	return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// Site-space CG (SCGState in rc_device.h): Eigen's Jacobi-PCG iterates of the
+// wet solve carried as scalars times the ker-P parts r_perp, z_perp = r_perp /
+// D of the first residual plus L x L fields y, q, s.  The iterates are the
+// reference's in exact arithmetic; an iteration reads and writes O(L^2) data
+// instead of O(rows), about a tenth at the patch overlaps of the benchmarks.
+// ---------------------------------------------------------------------------
+
+// the four box neighbours of site k, in the triplet order of
+// WetPolarModel.cpp:206-215 (neighbours 1, 3, 5, 7 = S, W, E, N)
+__device__ inline void site_nbrs(int k, int L, int *nb) {
+	const int y = k / L, x = k - y * L;
+	const int xe = x + 1 == L ? 0 : x + 1, xw = x == 0 ? L - 1 : x - 1;
+	const int yn = y + 1 == L ? 0 : y + 1, ys = y == 0 ? L - 1 : y - 1;
+	nb[0] = ys * L + x;
+	nb[1] = y * L + xw;
+	nb[2] = y * L + xe;
+	nb[3] = yn * L + x;
+}
+
+struct SCGPeek {
+	bool done[2];
+	double rn2[2], rz[2], beta[2], pi[2];
+};
+// Eigen's stop test and beta for the update that entered iteration j, from that
+// update's site sums tot = {y.m y (x, y), y.m (y / D) (x, y)}:
+//   |r|^2 = rho^2 |r_perp|^2 + y.m y,   r.z = rho^2 r_perp.z_perp + y.m (y / D)
+// (a NaN residual is not "done": the writer raises ST_NAN, which halts the device)
+__device__ inline SCGPeek scg_peek(const SCGState &S, const double (&tot)[4]) {
+	SCGPeek k;
+#pragma unroll
+	for(int c = 0; c < 2; ++c) {
+		k.done[c] = S.done[c];
+		k.rn2[c] = S.rn2[c];
+		k.rz[c] = S.absNew[c];
+		k.beta[c] = 0.;
+		k.pi[c] = S.pi[c];
+		if(k.done[c]) continue;
+		const double r2 = S.rho[c] * S.rho[c];
+		const double rn2 = r2 * S.rperp2[c] + tot[c];
+		k.rn2[c] = rn2;
+		if(rn2 < S.thr[c] || S.iters[c] >= S.max_iter) {
+			k.done[c] = true;
+			continue;
+		}
+		const double rz = r2 * S.rzperp[c] + tot[2 + c];
+		k.rz[c] = rz;
+		k.beta[c] = rz / S.absNew[c];
+		k.pi[c] = S.rho[c] + k.beta[c] * S.pi[c];
+	}
+	return k;
+}
+
+// Set-up, per site: site means b_bar = P b / m, x_bar = P x0 / m;
+// c = b_bar - D x_bar; y = (P r0) / m = c + eta N4(P x0), the site part of the
+// first residual r0 = b - A x0; s = 0.  Partials y.m y, y.m (y / D) -> part_r1.
+__global__ __launch_bounds__(256) void k_scg_init(SCGArgs a) {
+	if(a.st->flags) return;
+	__shared__ double s_red[4 * 4];
+	const DevBox B = a.boxes[0];
+	const int o = B.box_off, L = B.L;
+	const double eta = a.params[a.set].friction_cell;
+	const int k = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+	double part[4] = { 0., 0., 0., 0. };
+	if(k < a.nsites) {
+		const int m = a.cover[o + k];
+		double2 c = make_double2(0., 0.), y = make_double2(0., 0.);
+		if(m > 0) {
+			int nb[4];
+			site_nbrs(k, L, nb);
+			const double D = a.diag[o + k], inv = 1. / D, fm = (double) m;
+			const double2 sb = a.sb[o + k], sx = a.sx[o + k];
+			const double2 gS = a.sx[o + nb[0]], gW = a.sx[o + nb[1]], gE = a.sx[o + nb[2]], gN = a.sx[o + nb[3]];
+			c.x = sb.x / fm - D * (sx.x / fm);
+			c.y = sb.y / fm - D * (sx.y / fm);
+			y.x = c.x + eta * (((gS.x + gW.x) + gE.x) + gN.x);
+			y.y = c.y + eta * (((gS.y + gW.y) + gE.y) + gN.y);
+			part[0] = fm * (y.x * y.x);
+			part[1] = fm * (y.y * y.y);
+			part[2] = fm * (y.x * (inv * y.x));
+			part[3] = fm * (y.y * (inv * y.y));
+		}
+		a.c[k] = c;
+		a.y[k] = y;
+		a.s[k] = make_double2(0., 0.);
+	}
+	block_sum<4>(part, s_red);
+	block_partials<4>(part, a.part_r1);
+}
+
+// box site of storage row q of cell C (MultiPhaseField ring buffer, SquareBox wrap)
+__device__ inline int cell_site(const DevCell &C, int L, int q) {
+	const int qy = q / C.LX, qx = q - qy * C.LX;
+	int bx = qx + C.offx;
+	if(bx >= C.LX) bx -= C.LX;
+	bx += C.corner % L;
+	if(bx >= L) bx -= L;
+	int by = qy + C.offy;
+	if(by >= C.LY) by -= C.LY;
+	by += C.corner / L;
+	if(by >= L) by -= L;
+	return by * L + bx;
+}
+
+// Set-up, per cell: the ker-P part of the first residual, r_perp_i = (b_i -
+// D x0_i) - c(k_i), summed as |b|^2, |r_perp|^2, r_perp.(r_perp / D) -> part
+// (6 per cell: b.b x, y; r_perp.r_perp x, y; r_perp.z_perp x, y)
+__global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
+	if(a.st->flags) return;
+	__shared__ double s_red[4 * 6];
+	const int ci = xcd_item(blockIdx.x, gridDim.x);
+	const DevCell C = a.cells[ci];
+	const DevBox B = a.boxes[C.box];
+	double red[6] = { 0., 0., 0., 0., 0., 0. };
+	for(int q = threadIdx.x; q < C.S; q += 256) {
+		const int k = cell_site(C, B.L, q), row = C.start + q;
+		const double D = a.diag[B.box_off + k], inv = 1. / D;
+		const double2 c = a.c[k], bb = a.b[row], x0 = a.x[row];
+		const double rx = (bb.x - D * x0.x) - c.x, ry = (bb.y - D * x0.y) - c.y;
+		red[0] += bb.x * bb.x; red[1] += bb.y * bb.y;
+		red[2] += rx * rx; red[3] += ry * ry;
+		red[4] += rx * (inv * rx); red[5] += ry * (inv * ry);
+	}
+	block_sum<6>(red, s_red);
+	block_partials<6>(red, a.part);
+}
+
+// Set-up, one workgroup: Eigen's start (ConjugateGradient.h: |b|^2, the
+// threshold max(tol^2 |b|^2, DBL_MIN), the first residual's |r|^2 and r.z)
+// into SCGState [0], with rho = pi = 1, xi = 0
+__global__ __launch_bounds__(1024) void k_scg_setup_reduce(SCGArgs a) {
+	__shared__ double s_red[16 * 6];
+	const unsigned flags = a.st->flags;
+	double t6[6], t4[4];
+	reduce_parts<6>(a.part, a.ncells, t6, s_red);
+	reduce_parts<4>(a.part_r1, a.nsb, t4, s_red);
+	if(flags || threadIdx.x != 0) return;
+	SCGState S{};
+	S.max_iter = a.max_iter;
+	for(int c = 0; c < 2; ++c) {
+		const double bb = t6[c], rp2 = t6[2 + c], rzp = t6[4 + c];
+		const double rn2 = rp2 + t4[c], rz = rzp + t4[2 + c];
+		S.rho[c] = 1.;
+		S.pi[c] = 1.;
+		S.bb[c] = bb;
+		S.rperp2[c] = rp2;
+		S.rzperp[c] = rzp;
+		if(!(bb == bb) || !(rn2 == rn2) || !(rz == rz)) {
+			cg_flag_nan(a.st);
+			S.done[c] = 1;
+			continue;
+		}
+		if(bb == 0) {
+			S.zero_rhs[c] = 1;
+			S.done[c] = 1;
+			continue;
+		}
+		double thr = a.tol * a.tol * bb;
+		if(thr < DBL_MIN) thr = DBL_MIN;
+		S.thr[c] = thr;
+		S.rn2[c] = rn2;
+		if(rn2 < thr) {
+			S.done[c] = 1;
+			continue;
+		}
+		S.absNew[c] = rz;
+	}
+	S.active = !(S.done[0] && S.done[1]);
+	for(int c = 0; c < 2; ++c) {
+		S.rn2_now[c] = S.rn2[c];
+		S.done_now[c] = S.done[c];
+	}
+	S.active_now = S.active;
+	a.s2[0] = S;
+}
+
+// Iteration j, first pass (per site): beta from the last update's sums (every
+// workgroup, the same fixed order), q_j = y / D + beta q_(j-1) at the site and
+// its four neighbours, w = D q_j - eta N4(m q_j) (the site part of A p), the
+// partials q_j.m w -> part_p.  q is double-buffered by the parity of j.
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_scg_dir(SCGArgs a, int j) {
+	if(a.st->flags) return;
+	const SCGState *Sp = a.s2 + (j & 1);
+	if(!Sp->active) return;
+	__shared__ double s_red[4 * 4];
+	const DevBox B = a.boxes[0];
+	const int o = B.box_off, L = B.L;
+	const double eta = a.params[a.set].friction_cell;
+	const int k0 = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+	const bool in = k0 < a.nsites;
+	const int k = in ? k0 : a.nsites - 1;
+	int nb[5];
+	nb[0] = k;
+	site_nbrs(k, L, nb + 1);
+	const double2 *qprev = (j & 1) ? a.q0 : a.q1;
+	double2 *qout = (j & 1) ? a.q1 : a.q0;
+	// the site data first: it does not depend on the sums
+	int m[5];
+	double D[5];
+	double2 yv[5], qp[5];
+#pragma unroll
+	for(int i = 0; i < 5; ++i) {
+		m[i] = a.cover[o + nb[i]];
+		D[i] = a.diag[o + nb[i]];
+		yv[i] = a.y[nb[i]];
+		qp[i] = FIRST ? make_double2(0., 0.) : qprev[nb[i]];
+	}
+	double b0 = 0., b1 = 0.;
+	if(!FIRST) {
+		double tot[4];
+		reduce_parts<4, 256>((j - 1) & 1 ? a.part_r1 : a.part_r0, gridDim.x, tot, s_red);
+		const SCGPeek pk = scg_peek(*Sp, tot);
+		if(pk.done[0] && pk.done[1]) return;
+		b0 = pk.beta[0];
+		b1 = pk.beta[1];
+	}
+	double2 q[5];
+#pragma unroll
+	for(int i = 0; i < 5; ++i) {
+		const double inv = 1. / D[i];
+		q[i] = make_double2(0., 0.);
+		if(m[i] > 0) {
+			q[i].x = FIRST ? inv * yv[i].x : inv * yv[i].x + b0 * qp[i].x;
+			q[i].y = FIRST ? inv * yv[i].y : inv * yv[i].y + b1 * qp[i].y;
+		}
+	}
+	double2 w = make_double2(0., 0.);
+	if(m[0] > 0) {
+		const double m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4];
+		w.x = D[0] * q[0].x - eta * (((m1 * q[1].x + m2 * q[2].x) + m3 * q[3].x) + m4 * q[4].x);
+		w.y = D[0] * q[0].y - eta * (((m1 * q[1].y + m2 * q[2].y) + m3 * q[3].y) + m4 * q[4].y);
+	}
+	double part[2] = { 0., 0. };
+	if(in) {
+		qout[k] = q[0];
+		a.w[k] = w;
+		const double fm = m[0];
+		part[0] = fm * (q[0].x * w.x);
+		part[1] = fm * (q[0].y * w.y);
+	}
+	block_sum<2>(part, s_red);
+	block_partials<2>(part, a.part_p);
+}
+
+// Iteration j, second pass (per site): alpha = r.z / p.Ap with
+// p.Ap = pi^2 r_perp.z_perp + q.m w (every workgroup, the same order); s += alpha q,
+// y -= alpha w; partials y.m y, y.m (y / D) -> part_r[j & 1].  Workgroup 0
+// writes SCGState [(j + 1) & 1]: rho -= alpha pi, xi += alpha pi and the
+// recurrence scalars (nobody reads that copy during iteration j).
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_scg_upd(SCGArgs a, int j) {
+	if(a.st->flags) return;
+	const SCGState *Sp = a.s2 + (j & 1);
+	SCGState *Sn = a.s2 + ((j + 1) & 1);
+	const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+	if(!Sp->active) {
+		if(writer) *Sn = *Sp;
+		return;
+	}
+	__shared__ double s_red[4 * 4];
+	const DevBox B = a.boxes[0];
+	const int o = B.box_off;
+	const int k0 = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+	const bool in = k0 < a.nsites;
+	const int k = in ? k0 : a.nsites - 1;
+	const double2 *qj = (j & 1) ? a.q1 : a.q0;
+	const int m = a.cover[o + k];
+	const double D = a.diag[o + k];
+	const double2 q = qj[k], w = a.w[k];
+	double2 s = a.s[k], y = a.y[k];
+	SCGPeek pk;
+	if(FIRST) {
+#pragma unroll
+		for(int c = 0; c < 2; ++c) {
+			pk.done[c] = Sp->done[c];
+			pk.rn2[c] = Sp->rn2[c];
+			pk.rz[c] = Sp->absNew[c];
+			pk.beta[c] = 0.;
+			pk.pi[c] = Sp->pi[c];
+		}
+	}
+	else {
+		double tot[4];
+		reduce_parts<4, 256>((j - 1) & 1 ? a.part_r1 : a.part_r0, gridDim.x, tot, s_red);
+		pk = scg_peek(*Sp, tot);
+	}
+	if(pk.done[0] && pk.done[1]) {
+		if(writer) {
+			SCGState S = *Sp;
+			for(int c = 0; c < 2; ++c) {
+				S.rn2[c] = pk.rn2[c];
+				S.done[c] = 1;
+			}
+			S.active = 0;
+			*Sn = S;
+		}
+		return;
+	}
+	double mqw[2];
+	reduce_parts<2, 256>(a.part_p, gridDim.x, mqw, s_red);
+	double al[2];
+#pragma unroll
+	for(int c = 0; c < 2; ++c) al[c] = pk.done[c] ? 0. : pk.rz[c] / (pk.pi[c] * pk.pi[c] * Sp->rzperp[c] + mqw[c]);
+	if(!pk.done[0]) {
+		s.x += al[0] * q.x;
+		y.x -= al[0] * w.x;
+	}
+	if(!pk.done[1]) {
+		s.y += al[1] * q.y;
+		y.y -= al[1] * w.y;
+	}
+	double part[4] = { 0., 0., 0., 0. };
+	if(in) {
+		a.s[k] = s;
+		a.y[k] = y;
+		const double fm = m, inv = 1. / D;
+		part[0] = fm * (y.x * y.x);
+		part[1] = fm * (y.y * y.y);
+		part[2] = fm * (y.x * (inv * y.x));
+		part[3] = fm * (y.y * (inv * y.y));
+	}
+	block_sum<4>(part, s_red);
+	block_partials<4>(part, (j & 1) ? a.part_r1 : a.part_r0);
+	if(writer) {
+		SCGState S = *Sp;
+		for(int c = 0; c < 2; ++c) {
+			S.rn2[c] = pk.rn2[c];
+			if(!(pk.rn2[c] == pk.rn2[c])) cg_flag_nan(a.st);
+			if(pk.done[c]) {
+				S.done[c] = 1;
+				continue;
+			}
+			S.absNew[c] = pk.rz[c];
+			S.beta[c] = pk.beta[c];
+			S.pi[c] = pk.pi[c];
+			S.alpha[c] = al[c];
+			S.rho[c] = Sp->rho[c] - al[c] * pk.pi[c];
+			S.xi[c] = Sp->xi[c] + al[c] * pk.pi[c];
+			S.iters[c] = Sp->iters[c] + 1;
+		}
+		S.active = 1;
+		*Sn = S;
+	}
+}
+
+// The stop test of the state entering iteration j (after a batch), written
+// into SCGState [j & 1] as rn2_now, done_now, active_now for the host
+__global__ __launch_bounds__(256) void k_scg_check(SCGArgs a, int j) {
+	__shared__ double s_red[4 * 4];
+	SCGState *S = a.s2 + (j & 1);
+	const int active = S->active;
+	double tot[4] = { 0., 0., 0., 0. };
+	if(active && j > 0) reduce_parts<4, 256>((j - 1) & 1 ? a.part_r1 : a.part_r0, a.nsb, tot, s_red);
+	if(threadIdx.x != 0) return;
+	if(!active || j == 0) {
+		for(int c = 0; c < 2; ++c) {
+			S->rn2_now[c] = S->rn2[c];
+			S->done_now[c] = S->done[c];
+		}
+		S->active_now = active;
+		return;
+	}
+	const SCGPeek pk = scg_peek(*S, tot);
+	for(int c = 0; c < 2; ++c) {
+		S->rn2_now[c] = pk.rn2[c];
+		S->done_now[c] = pk.done[c];
+		if(!(pk.rn2[c] == pk.rn2[c])) cg_flag_nan(a.st);
+	}
+	S->active_now = !(pk.done[0] && pk.done[1]);
+}
+
+// x_i = x0_i + xi (r_perp_i / D) + s(k_i), r_perp as in k_scg_row_init; a
+// component with a zero right-hand side is 0 (Eigen's solve of b = 0)
+__global__ __launch_bounds__(256) void k_scg_final(SCGArgs a, int j) {
+	if(a.st->flags) return;
+	const SCGState *S = a.s2 + (j & 1);
+	const double xi0 = S->xi[0], xi1 = S->xi[1];
+	const bool z0 = S->zero_rhs[0], z1 = S->zero_rhs[1];
+	const int ci = xcd_item(blockIdx.x, gridDim.x);
+	const DevCell C = a.cells[ci];
+	const DevBox B = a.boxes[C.box];
+	for(int q = threadIdx.x; q < C.S; q += 256) {
+		const int k = cell_site(C, B.L, q), row = C.start + q;
+		const double D = a.diag[B.box_off + k], inv = 1. / D;
+		const double2 c = a.c[k], sv = a.s[k], bb = a.b[row], x0 = a.x[row];
+		const double rx = (bb.x - D * x0.x) - c.x, ry = (bb.y - D * x0.y) - c.y;
+		double2 x;
+		x.x = z0 ? 0. : (x0.x + xi0 * (inv * rx)) + sv.x;
+		x.y = z1 ? 0. : (x0.y + xi1 * (inv * ry)) + sv.y;
+		a.x[row] = x;
+	}
+}
+
+hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s) {
+	switch(which) {
+	case SCG_INIT: RC_LAUNCH(k_scg_init, dim3(a.nsb), dim3(256), 0, s, a); break;
+	case SCG_ROW_INIT: RC_LAUNCH(k_scg_row_init, dim3(a.ncells), dim3(256), 0, s, a); break;
+	case SCG_SETUP_REDUCE: RC_LAUNCH(k_scg_setup_reduce, dim3(1), dim3(1024), 0, s, a); break;
+	case SCG_DIR:
+		if(j == 0) RC_LAUNCH(k_scg_dir<true>, dim3(a.nsb), dim3(256), 0, s, a, j);
+		else RC_LAUNCH(k_scg_dir<false>, dim3(a.nsb), dim3(256), 0, s, a, j);
+		break;
+	case SCG_UPD:
+		if(j == 0) RC_LAUNCH(k_scg_upd<true>, dim3(a.nsb), dim3(256), 0, s, a, j);
+		else RC_LAUNCH(k_scg_upd<false>, dim3(a.nsb), dim3(256), 0, s, a, j);
+		break;
+	case SCG_CHECK: RC_LAUNCH(k_scg_check, dim3(1), dim3(256), 0, s, a, j); break;
+	case SCG_FINAL: RC_LAUNCH(k_scg_final, dim3(a.ncells), dim3(256), 0, s, a, j); break;
+	default: return hipErrorInvalidValue;
+	}
 	return hipGetLastError();
 }
 

@@ -85,7 +85,8 @@ class Simulation:
     """One handle: one or more independent boxes advanced together."""
 
     def __init__(self, boxes: Sequence[Tuple[Conf, Dict[str, object]]], seeds: Optional[Sequence[int]] = None,
-                 cg_max_iter: int = 0, init: bool = True, equilibrate: bool = True):
+                 cg_max_iter: int = 0, init: bool = True, equilibrate: bool = True,
+                 cg_form: Optional[int] = None):
         self.L = _lib.load()
         self._keep = []    # arrays rc_create reads
         self.confs = [b[0] for b in boxes]
@@ -101,6 +102,8 @@ class Simulation:
         self.h = C.c_void_p()
         rc = self.L.rc_create(descs, len(boxes), C.byref(self.h))
         self._check(rc)
+        if cg_form is not None:
+            self.set_cg_form(cg_form)
         cells = [c for conf in self.confs for c in conf.cells]
         self.ncells = len(cells)
         self.counts = [len(conf.cells) for conf in self.confs]
@@ -181,6 +184,11 @@ class Simulation:
 
     def set_warm_start(self, mode: int):
         self._check(self.L.rc_set_warm_start(self.h, mode))
+
+    def set_cg_form(self, form: int):
+        """_lib.RC_CG_SITES (default) or _lib.RC_CG_ROWS: the wet solve's iterates
+        as box fields or as rows (the same iteration, rounding apart)."""
+        self._check(self.L.rc_set_cg_form(self.h, form))
 
     def gaussians(self, nsteps: int) -> np.ndarray:
         parts = [s.gaussians(nsteps) for s in self.streams]
