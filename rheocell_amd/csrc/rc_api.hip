@@ -85,8 +85,9 @@ struct rc_handle {
 	CGState *h_cg = nullptr;      // pinned
 	// site-space form of the CG (SCGState)
 	int cg_form = RC_CG_SITES;
-	double2 *d_sc = nullptr, *d_sy = nullptr, *d_ss = nullptr, *d_sw = nullptr, *d_sq0 = nullptr, *d_sq1 = nullptr;
-	double *d_part_p = nullptr, *d_part_r0 = nullptr, *d_part_r1 = nullptr;
+	double2 *d_sc = nullptr, *d_sq = nullptr, *d_ss = nullptr;
+	double2 *d_sy[2] = {}, *d_sw[2] = {}, *d_su[2] = {};
+	double *d_psum[2] = {};
 	SCGState *d_scg = nullptr;    // [2]
 	SCGState *h_scg = nullptr;    // pinned
 	hipEvent_t cg_e0 = nullptr, cg_e1 = nullptr;
@@ -475,11 +476,11 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, dalloc(&h->d_part, (size_t) std::max(6 * std::max(cells, 1), 2 * tiles)));
 		HIPCHK(h, dalloc(&h->d_cg, 1));
 		HIPCHK(h, hipMemset(h->d_cg, 0, sizeof(CGState)));
-		const size_t nsb = (sites + 255) / 256;
-		for(double2 **f : { &h->d_sc, &h->d_sy, &h->d_ss, &h->d_sw, &h->d_sq0, &h->d_sq1 }) HIPCHK(h, dalloc(f, sites));
-		HIPCHK(h, dalloc(&h->d_part_p, 2 * nsb));
-		HIPCHK(h, dalloc(&h->d_part_r0, 4 * nsb));
-		HIPCHK(h, dalloc(&h->d_part_r1, 4 * nsb));
+		const size_t nsb = (sites + SCG_THREADS - 1) / SCG_THREADS;
+		for(double2 **f : { &h->d_sc, &h->d_sq, &h->d_ss, &h->d_sy[0], &h->d_sy[1], &h->d_sw[0], &h->d_sw[1], &h->d_su[0],
+				&h->d_su[1] })
+			HIPCHK(h, dalloc(f, sites));
+		for(int i = 0; i < 2; ++i) HIPCHK(h, dalloc(&h->d_psum[i], 6 * nsb));
 		HIPCHK(h, dalloc(&h->d_scg, 2));
 		HIPCHK(h, hipMemset(h->d_scg, 0, 2 * sizeof(SCGState)));
 		HIPCHK(h, hipHostMalloc((void **) &h->h_scg, sizeof(SCGState)));
@@ -502,8 +503,10 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_ggauss); dfree(h->d_step_base); dfree(h->d_bf);
 	if(h->h_step_base) (void) hipHostFree(h->h_step_base); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
-	dfree(h->d_sc); dfree(h->d_sy); dfree(h->d_ss); dfree(h->d_sw); dfree(h->d_sq0); dfree(h->d_sq1);
-	dfree(h->d_part_p); dfree(h->d_part_r0); dfree(h->d_part_r1); dfree(h->d_scg);
+	dfree(h->d_sc); dfree(h->d_sq); dfree(h->d_ss); dfree(h->d_scg);
+	for(int i = 0; i < 2; ++i) {
+		dfree(h->d_sy[i]); dfree(h->d_sw[i]); dfree(h->d_su[i]); dfree(h->d_psum[i]);
+	}
 	if(h->h_scg) (void) hipHostFree(h->h_scg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
 	for(auto e : h->pool) (void) hipEventDestroy(e);
@@ -890,23 +893,23 @@ static SCGArgs scg_args(rc_handle *h, const CGArgs &c) {
 	a.sb = h->d_sp;
 	a.sx = h->d_p2;
 	a.c = h->d_sc;
-	a.y = h->d_sy;
+	a.q = h->d_sq;
 	a.s = h->d_ss;
-	a.w = h->d_sw;
-	a.q0 = h->d_sq0;
-	a.q1 = h->d_sq1;
+	for(int i = 0; i < 2; ++i) {
+		a.y[i] = h->d_sy[i];
+		a.w[i] = h->d_sw[i];
+		a.u[i] = h->d_su[i];
+		a.psum[i] = h->d_psum[i];
+	}
 	a.b = h->d_f;
 	a.x = h->d_v;
 	a.part = h->d_part;
-	a.part_p = h->d_part_p;
-	a.part_r0 = h->d_part_r0;
-	a.part_r1 = h->d_part_r1;
 	a.s2 = h->d_scg;
 	a.tol = c.tol;
 	a.max_iter = c.max_iter;
 	a.ncells = h->ncells;
 	a.nsites = (int) h->box_sites;
-	a.nsb = (int) ((h->box_sites + 255) / 256);
+	a.nsb = (int) ((h->box_sites + SCG_THREADS - 1) / SCG_THREADS);
 	a.st = h->d_st;
 	return a;
 }
@@ -1011,14 +1014,15 @@ static int scg_iterations(rc_handle *h, const CGArgs &c, bool cold) {
 	const SCGArgs a = scg_args(h, c);
 	{ TimeScope t(h, RC_T_SCG_GATHER); HIPCHK(h, launch_cg_gather(CG_GATHER_BX, c, h->ntiles, h->stream)); }
 	{ TimeScope t(h, RC_T_SCG_INIT); HIPCHK(h, launch_scg(SCG_INIT, a, 0, h->stream)); }
+	{ TimeScope t(h, RC_T_SCG_INIT); HIPCHK(h, launch_scg(SCG_INIT_W, a, 0, h->stream)); }
 	{ TimeScope t(h, RC_T_SCG_ROW_INIT); HIPCHK(h, launch_scg(SCG_ROW_INIT, a, 0, h->stream)); }
 	{ TimeScope t(h, RC_T_SCG_REDUCE); HIPCHK(h, launch_scg(SCG_SETUP_REDUCE, a, 0, h->stream)); }
 	int batch = std::max(2, h->last_iters[cold ? 1 : 0] - 1), next = 2;
 	int j = 0;
 	for(;;) {
 		for(int it = 0; it < batch; ++it, ++j) {
-			{ TimeScope t(h, RC_T_SCG_DIR); HIPCHK(h, launch_scg(SCG_DIR, a, j, h->stream)); }
-			{ TimeScope t(h, RC_T_SCG_UPD); HIPCHK(h, launch_scg(SCG_UPD, a, j, h->stream)); }
+			TimeScope t(h, RC_T_SCG_ITER);
+			HIPCHK(h, launch_scg(SCG_ITER, a, j, h->stream));
 		}
 		{ TimeScope t(h, RC_T_SCG_REDUCE); HIPCHK(h, launch_scg(SCG_CHECK, a, j, h->stream)); }
 		HIPCHK(h, hipMemcpyAsync(h->h_scg, h->d_scg + (j & 1), sizeof(SCGState), hipMemcpyDeviceToHost, h->stream));

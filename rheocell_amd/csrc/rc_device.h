@@ -91,12 +91,16 @@ struct CGState {
 // with r_perp the ker-P part of the first residual and z_perp = r_perp / D:
 // scalars rho, pi, xi per component and L x L fields y, q, s.  The inner
 // products split the same way (r_perp . P^T y = 0, P^T a . P^T b = a . m b).
-// Two copies, read from [j & 1] and written to [(j + 1) & 1] by iteration j.
+// The recurrences are Chronopoulos and Gear's arrangement of the same PCG
+// (gamma = r.z and delta = z.Az reduced together, alpha = gamma / (delta -
+// beta gamma / alpha_prev), A p carried as w + beta A p_prev): one reduction,
+// one pass per iteration.  Two copies, read from [j & 1] and written to
+// [(j + 1) & 1] by iteration j.
 struct SCGState {
 	double bb[2], thr[2], rn2[2];
 	double rperp2[2], rzperp[2];   // |r_perp|^2, r_perp . (r_perp / D)
 	double rho[2], pi[2], xi[2];
-	double absNew[2], alpha[2], beta[2];
+	double gamma[2], alpha[2], beta[2];   // r.z, alpha and beta of the last iteration
 	int done[2], iters[2], zero_rhs[2], max_iter, active;
 	// stop test of the last update, written by k_scg_check after a batch
 	double rn2_now[2];

@@ -197,7 +197,11 @@ hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream
 
 // Site-space CG (SCGState): one wet box of nsites = L*L sites, box sums of b
 // and x0 in sb and sx (k_cg_gather<CG_GATHER_BX>), D in diag, rows per site in
-// cover.  Site passes run nsb = ceil(nsites / 256) workgroups of 256 sites.
+// cover.  Site passes run nsb = ceil(nsites / SCG_THREADS) workgroups, one site
+// per thread.  Iteration j reads the fields of parity j & 1 and writes those of
+// the other parity (its neighbours' old values stay readable), and the partial
+// sums likewise.
+constexpr int SCG_THREADS = 512;
 struct SCGArgs {
 	const DevBox *boxes;
 	const DevCell *cells;
@@ -207,13 +211,14 @@ struct SCGArgs {
 	const double *diag;
 	const double2 *sb, *sx;      // box sums of b and of x0
 	double2 *c;                  // b_bar - D x0_bar per site (site means of b, x0)
-	double2 *y, *s, *w;          // site parts of r, of x - x0, of A p
-	double2 *q0, *q1;            // site part of p: iteration j writes q_j to (j & 1 ? q1 : q0)
+	double2 *q, *s;              // site parts of p and of x - x0
+	double2 *y[2];               // site part of r
+	double2 *w[2];               // site part of A z, z = D^-1 r
+	double2 *u[2];               // site part of A p
 	const double2 *b;            // right-hand side rows
 	double2 *x;                  // start rows in, solution rows out
 	double *part;                // row-pass partials, 6 per cell
-	double *part_p;              // site partials of q . m w, [2][nsb]
-	double *part_r0, *part_r1;   // site partials of y . m y, y . m y / D, [4][nsb]; iteration j writes (j & 1 ? r1 : r0)
+	double *psum[2];             // site partials y.m y, y.m (y / D), (y / D).m w: [6][nsb] each
 	SCGState *s2;                // [2]
 	double tol;
 	int max_iter;
@@ -221,13 +226,15 @@ struct SCGArgs {
 	DevStatus *st;
 };
 // launches of the site-space CG, in solve order:
-//   SCG_INIT          per site: c, y = c + eta N4(sx), s = 0 and their partials
+//   SCG_INIT          per site: c, y = c + eta N4(sx), s = q = 0, A p = 0
+//   SCG_INIT_W        per site: w = the site part of A z and the partials of parity 0
 //   SCG_ROW_INIT      per cell: |b|^2 and the ker-P residual sums over the rows
 //   SCG_SETUP_REDUCE  one workgroup: Eigen's start into SCGState [0]
-//   SCG_DIR, SCG_UPD  iteration j: beta, q_j, w and q.m w; then alpha, s, y and SCGState [(j + 1) & 1]
+//   SCG_ITER          iteration j: gamma, delta, the stop test, alpha, beta; p, A p,
+//                     x, r updated; w of the new r and the partials; SCGState [(j + 1) & 1]
 //   SCG_CHECK         the stop test of the state entering iteration j (after a batch)
 //   SCG_FINAL         x = x0 + xi z_perp + P^T s from the state entering iteration j
-enum SCGLaunch { SCG_INIT = 0, SCG_ROW_INIT, SCG_SETUP_REDUCE, SCG_DIR, SCG_UPD, SCG_CHECK, SCG_FINAL };
+enum SCGLaunch { SCG_INIT = 0, SCG_INIT_W, SCG_ROW_INIT, SCG_SETUP_REDUCE, SCG_ITER, SCG_CHECK, SCG_FINAL };
 hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s);
 // map[j] = {old row start, new row start, rows}: dst[new + q] = src[old + q] (and src2 -> dst2 if set)
 // growth re-cut of listed patches (k_grow_rows; 4 int4 per entry), at most

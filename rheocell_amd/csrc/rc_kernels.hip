@@ -2723,13 +2723,20 @@ hipError_t launch_zero_rows(double2 *x, long long rows, int mask, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Site-space CG (SCGState in rc_device.h): Eigen's Jacobi-PCG iterates of the
 // wet solve carried as scalars times the ker-P parts r_perp, z_perp = r_perp /
-// D of the first residual plus L x L fields y, q, s.  The iterates are the
-// reference's in exact arithmetic; an iteration reads and writes O(L^2) data
-// instead of O(rows), about a tenth at the patch overlaps of the benchmarks.
+// D of the first residual plus L x L fields, in Chronopoulos and Gear's
+// arrangement (one reduction per iteration).  The iterates are the
+// reference's in exact arithmetic; an iteration is one pass over the box
+// sites instead of passes over every row.
 // ---------------------------------------------------------------------------
 
 // the four box neighbours of site k, in the triplet order of
 // WetPolarModel.cpp:206-215 (neighbours 1, 3, 5, 7 = S, W, E, N)
+__device__ inline int4 site_nbrs4(int k, int L) {
+	const int y = k / L, x = k - y * L;
+	const int xe = x + 1 == L ? 0 : x + 1, xw = x == 0 ? L - 1 : x - 1;
+	const int yn = y + 1 == L ? 0 : y + 1, ys = y == 0 ? L - 1 : y - 1;
+	return make_int4(ys * L + x, y * L + xw, y * L + xe, yn * L + x);
+}
 __device__ inline void site_nbrs(int k, int L, int *nb) {
 	const int y = k / L, x = k - y * L;
 	const int xe = x + 1 == L ? 0 : x + 1, xw = x == 0 ? L - 1 : x - 1;
@@ -2740,74 +2747,103 @@ __device__ inline void site_nbrs(int k, int L, int *nb) {
 	nb[3] = yn * L + x;
 }
 
-struct SCGPeek {
-	bool done[2];
-	double rn2[2], rz[2], beta[2], pi[2];
-};
-// Eigen's stop test and beta for the update that entered iteration j, from that
-// update's site sums tot = {y.m y (x, y), y.m (y / D) (x, y)}:
-//   |r|^2 = rho^2 |r_perp|^2 + y.m y,   r.z = rho^2 r_perp.z_perp + y.m (y / D)
-// (a NaN residual is not "done": the writer raises ST_NAN, which halts the device)
-__device__ inline SCGPeek scg_peek(const SCGState &S, const double (&tot)[4]) {
-	SCGPeek k;
+// site of this thread (SCG_THREADS per workgroup, XCD-contiguous blocks) and
+// whether it is a box site (threads past the last site take the last one and
+// write nothing)
+__device__ inline int scg_site(const SCGArgs &a, bool &in) {
+	const int k = xcd_item(blockIdx.x, gridDim.x) * SCG_THREADS + threadIdx.x;
+	in = k < a.nsites;
+	return in ? k : a.nsites - 1;
+}
+
+// w = the site part of A z for z = D^-1 (site part of r): D z(k) - eta N4(m z)
+// from the five sites' y (own first, then S, W, E, N), D and m
+__device__ inline double2 scg_w(const double2 (&y)[5], const double (&D)[5], const int (&m)[5], double eta) {
+	double2 z[5];
 #pragma unroll
-	for(int c = 0; c < 2; ++c) {
-		k.done[c] = S.done[c];
-		k.rn2[c] = S.rn2[c];
-		k.rz[c] = S.absNew[c];
-		k.beta[c] = 0.;
-		k.pi[c] = S.pi[c];
-		if(k.done[c]) continue;
-		const double r2 = S.rho[c] * S.rho[c];
-		const double rn2 = r2 * S.rperp2[c] + tot[c];
-		k.rn2[c] = rn2;
-		if(rn2 < S.thr[c] || S.iters[c] >= S.max_iter) {
-			k.done[c] = true;
-			continue;
-		}
-		const double rz = r2 * S.rzperp[c] + tot[2 + c];
-		k.rz[c] = rz;
-		k.beta[c] = rz / S.absNew[c];
-		k.pi[c] = S.rho[c] + k.beta[c] * S.pi[c];
+	for(int i = 0; i < 5; ++i) {
+		const double inv = 1. / D[i];
+		z[i] = m[i] > 0 ? make_double2(inv * y[i].x, inv * y[i].y) : make_double2(0., 0.);
 	}
-	return k;
+	if(m[0] == 0) return make_double2(0., 0.);
+	const double m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4];
+	return make_double2(D[0] * z[0].x - eta * (((m1 * z[1].x + m2 * z[2].x) + m3 * z[3].x) + m4 * z[4].x),
+			D[0] * z[0].y - eta * (((m1 * z[1].y + m2 * z[2].y) + m3 * z[3].y) + m4 * z[4].y));
+}
+
+// the partial sums of one site: y.m y, y.m (y / D), (y / D).m w (x, y components)
+__device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, double2 w) {
+	const double fm = m, inv = 1. / D;
+	const double zx = inv * y.x, zy = inv * y.y;
+	part[0] = fm * (y.x * y.x);
+	part[1] = fm * (y.y * y.y);
+	part[2] = fm * (y.x * zx);
+	part[3] = fm * (y.y * zy);
+	part[4] = fm * (zx * w.x);
+	part[5] = fm * (zy * w.y);
 }
 
 // Set-up, per site: site means b_bar = P b / m, x_bar = P x0 / m;
 // c = b_bar - D x_bar; y = (P r0) / m = c + eta N4(P x0), the site part of the
-// first residual r0 = b - A x0; s = 0.  Partials y.m y, y.m (y / D) -> part_r1.
-__global__ __launch_bounds__(256) void k_scg_init(SCGArgs a) {
+// first residual r0 = b - A x0; s = q = 0 and A p = 0 (p_-1 = 0).
+__global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	if(a.st->flags) return;
-	__shared__ double s_red[4 * 4];
 	const DevBox B = a.boxes[0];
-	const int o = B.box_off, L = B.L;
+	const int o = B.box_off;
 	const double eta = a.params[a.set].friction_cell;
-	const int k = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-	double part[4] = { 0., 0., 0., 0. };
-	if(k < a.nsites) {
-		const int m = a.cover[o + k];
-		double2 c = make_double2(0., 0.), y = make_double2(0., 0.);
-		if(m > 0) {
-			int nb[4];
-			site_nbrs(k, L, nb);
-			const double D = a.diag[o + k], inv = 1. / D, fm = (double) m;
-			const double2 sb = a.sb[o + k], sx = a.sx[o + k];
-			const double2 gS = a.sx[o + nb[0]], gW = a.sx[o + nb[1]], gE = a.sx[o + nb[2]], gN = a.sx[o + nb[3]];
-			c.x = sb.x / fm - D * (sx.x / fm);
-			c.y = sb.y / fm - D * (sx.y / fm);
-			y.x = c.x + eta * (((gS.x + gW.x) + gE.x) + gN.x);
-			y.y = c.y + eta * (((gS.y + gW.y) + gE.y) + gN.y);
-			part[0] = fm * (y.x * y.x);
-			part[1] = fm * (y.y * y.y);
-			part[2] = fm * (y.x * (inv * y.x));
-			part[3] = fm * (y.y * (inv * y.y));
-		}
-		a.c[k] = c;
-		a.y[k] = y;
-		a.s[k] = make_double2(0., 0.);
+	bool in;
+	const int k = scg_site(a, in);
+	if(!in) return;
+	const int m = a.cover[o + k];
+	double2 c = make_double2(0., 0.), y = make_double2(0., 0.);
+	if(m > 0) {
+		int nb[4];
+		site_nbrs(k, B.L, nb);
+		const double D = a.diag[o + k], fm = (double) m;
+		const double2 sb = a.sb[o + k], sx = a.sx[o + k];
+		const double2 gS = a.sx[o + nb[0]], gW = a.sx[o + nb[1]], gE = a.sx[o + nb[2]], gN = a.sx[o + nb[3]];
+		c.x = sb.x / fm - D * (sx.x / fm);
+		c.y = sb.y / fm - D * (sx.y / fm);
+		y.x = c.x + eta * (((gS.x + gW.x) + gE.x) + gN.x);
+		y.y = c.y + eta * (((gS.y + gW.y) + gE.y) + gN.y);
 	}
-	block_sum<4>(part, s_red);
-	block_partials<4>(part, a.part_r1);
+	const double2 zero = make_double2(0., 0.);
+	a.c[k] = c;
+	a.y[0][k] = y;
+	a.s[k] = zero;
+	a.q[k] = zero;
+	a.u[0][k] = zero;
+}
+
+// Set-up, per site: w of the first residual and the partials of parity 0
+__global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
+	if(a.st->flags) return;
+	__shared__ double s_red[(SCG_THREADS / 64) * 6];
+	const DevBox B = a.boxes[0];
+	const int o = B.box_off;
+	const double eta = a.params[a.set].friction_cell;
+	bool in;
+	const int k = scg_site(a, in);
+	int nb[5];
+	nb[0] = k;
+	site_nbrs(k, B.L, nb + 1);
+	int m[5];
+	double D[5];
+	double2 y[5];
+#pragma unroll
+	for(int i = 0; i < 5; ++i) {
+		m[i] = a.cover[o + nb[i]];
+		D[i] = a.diag[o + nb[i]];
+		y[i] = a.y[0][nb[i]];
+	}
+	const double2 w = scg_w(y, D, m, eta);
+	double part[6] = { 0., 0., 0., 0., 0., 0. };
+	if(in) {
+		a.w[0][k] = w;
+		scg_part(part, m[0], D[0], y[0], w);
+	}
+	block_sum<6, SCG_THREADS / 64>(part, s_red);
+	block_partials<6>(part, a.psum[0]);
 }
 
 // box site of storage row q of cell C (MultiPhaseField ring buffer, SquareBox wrap)
@@ -2848,26 +2884,26 @@ __global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
 }
 
 // Set-up, one workgroup: Eigen's start (ConjugateGradient.h: |b|^2, the
-// threshold max(tol^2 |b|^2, DBL_MIN), the first residual's |r|^2 and r.z)
-// into SCGState [0], with rho = pi = 1, xi = 0
+// threshold max(tol^2 |b|^2, DBL_MIN), the first residual's |r|^2 and its test)
+// into SCGState [0], with rho = 1, pi = xi = 0
 __global__ __launch_bounds__(1024) void k_scg_setup_reduce(SCGArgs a) {
 	__shared__ double s_red[16 * 6];
 	const unsigned flags = a.st->flags;
-	double t6[6], t4[4];
+	double t6[6], ts[6];
 	reduce_parts<6>(a.part, a.ncells, t6, s_red);
-	reduce_parts<4>(a.part_r1, a.nsb, t4, s_red);
+	reduce_parts<6>(a.psum[0], a.nsb, ts, s_red);
 	if(flags || threadIdx.x != 0) return;
 	SCGState S{};
 	S.max_iter = a.max_iter;
+#pragma unroll
 	for(int c = 0; c < 2; ++c) {
 		const double bb = t6[c], rp2 = t6[2 + c], rzp = t6[4 + c];
-		const double rn2 = rp2 + t4[c], rz = rzp + t4[2 + c];
+		const double rn2 = rp2 + ts[c];
 		S.rho[c] = 1.;
-		S.pi[c] = 1.;
 		S.bb[c] = bb;
 		S.rperp2[c] = rp2;
 		S.rzperp[c] = rzp;
-		if(!(bb == bb) || !(rn2 == rn2) || !(rz == rz)) {
+		if(!(bb == bb) || !(rn2 == rn2) || !(ts[2 + c] == ts[2 + c]) || !(ts[4 + c] == ts[4 + c])) {
 			cg_flag_nan(a.st);
 			S.done[c] = 1;
 			continue;
@@ -2881,13 +2917,10 @@ __global__ __launch_bounds__(1024) void k_scg_setup_reduce(SCGArgs a) {
 		if(thr < DBL_MIN) thr = DBL_MIN;
 		S.thr[c] = thr;
 		S.rn2[c] = rn2;
-		if(rn2 < thr) {
-			S.done[c] = 1;
-			continue;
-		}
-		S.absNew[c] = rz;
+		if(rn2 < thr) S.done[c] = 1;
 	}
 	S.active = !(S.done[0] && S.done[1]);
+#pragma unroll
 	for(int c = 0; c < 2; ++c) {
 		S.rn2_now[c] = S.rn2[c];
 		S.done_now[c] = S.done[c];
@@ -2896,186 +2929,194 @@ __global__ __launch_bounds__(1024) void k_scg_setup_reduce(SCGArgs a) {
 	a.s2[0] = S;
 }
 
-// Iteration j, first pass (per site): beta from the last update's sums (every
-// workgroup, the same fixed order), q_j = y / D + beta q_(j-1) at the site and
-// its four neighbours, w = D q_j - eta N4(m q_j) (the site part of A p), the
-// partials q_j.m w -> part_p.  q is double-buffered by the parity of j.
-template <bool FIRST>
-__global__ __launch_bounds__(256) void k_scg_dir(SCGArgs a, int j) {
-	if(a.st->flags) return;
-	const SCGState *Sp = a.s2 + (j & 1);
-	if(!Sp->active) return;
-	__shared__ double s_red[4 * 4];
-	const DevBox B = a.boxes[0];
-	const int o = B.box_off, L = B.L;
-	const double eta = a.params[a.set].friction_cell;
-	const int k0 = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-	const bool in = k0 < a.nsites;
-	const int k = in ? k0 : a.nsites - 1;
-	int nb[5];
-	nb[0] = k;
-	site_nbrs(k, L, nb + 1);
-	const double2 *qprev = (j & 1) ? a.q0 : a.q1;
-	double2 *qout = (j & 1) ? a.q1 : a.q0;
-	// the site data first: it does not depend on the sums
-	int m[5];
-	double D[5];
-	double2 yv[5], qp[5];
+struct SCGCoef {
+	bool done[2];
+	double rn2[2], gamma[2], alpha[2], beta[2], pi[2];
+};
+// Iteration j's scalars from the sums tot of the residual entering it:
+//   |r|^2 = rho^2 |r_perp|^2 + y.m y  (Eigen's stop test of the last update; j > 0)
+//   gamma = r.z  = rho^2 r_perp.z_perp + y.m (y / D)
+//   delta = z.Az = rho^2 r_perp.z_perp + (y / D).m w      (A z_perp = r_perp)
+//   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
+//   (j = 0: beta = 0, alpha = gamma / delta), pi = rho + beta pi_prev
+// (a NaN residual is not "done": the writer raises ST_NAN, which halts the device)
+__device__ inline SCGCoef scg_coef(const SCGState &S, const double (&tot)[6], bool first) {
+	SCGCoef k;
 #pragma unroll
-	for(int i = 0; i < 5; ++i) {
-		m[i] = a.cover[o + nb[i]];
-		D[i] = a.diag[o + nb[i]];
-		yv[i] = a.y[nb[i]];
-		qp[i] = FIRST ? make_double2(0., 0.) : qprev[nb[i]];
-	}
-	double b0 = 0., b1 = 0.;
-	if(!FIRST) {
-		double tot[4];
-		reduce_parts<4, 256>((j - 1) & 1 ? a.part_r1 : a.part_r0, gridDim.x, tot, s_red);
-		const SCGPeek pk = scg_peek(*Sp, tot);
-		if(pk.done[0] && pk.done[1]) return;
-		b0 = pk.beta[0];
-		b1 = pk.beta[1];
-	}
-	double2 q[5];
-#pragma unroll
-	for(int i = 0; i < 5; ++i) {
-		const double inv = 1. / D[i];
-		q[i] = make_double2(0., 0.);
-		if(m[i] > 0) {
-			q[i].x = FIRST ? inv * yv[i].x : inv * yv[i].x + b0 * qp[i].x;
-			q[i].y = FIRST ? inv * yv[i].y : inv * yv[i].y + b1 * qp[i].y;
+	for(int c = 0; c < 2; ++c) {
+		k.done[c] = S.done[c];
+		k.rn2[c] = S.rn2[c];
+		k.gamma[c] = S.gamma[c];
+		k.alpha[c] = 0.;
+		k.beta[c] = 0.;
+		k.pi[c] = S.pi[c];
+		if(k.done[c]) continue;
+		const double r2 = S.rho[c] * S.rho[c];
+		if(!first) {
+			const double rn2 = r2 * S.rperp2[c] + tot[c];
+			k.rn2[c] = rn2;
+			if(rn2 < S.thr[c] || S.iters[c] >= S.max_iter) {
+				k.done[c] = true;
+				continue;
+			}
 		}
+		const double gamma = r2 * S.rzperp[c] + tot[2 + c];
+		const double delta = r2 * S.rzperp[c] + tot[4 + c];
+		k.gamma[c] = gamma;
+		if(first) k.alpha[c] = gamma / delta;
+		else {
+			k.beta[c] = gamma / S.gamma[c];
+			k.alpha[c] = gamma / (delta - k.beta[c] * gamma / S.alpha[c]);
+		}
+		k.pi[c] = S.rho[c] + k.beta[c] * S.pi[c];
 	}
-	double2 w = make_double2(0., 0.);
-	if(m[0] > 0) {
-		const double m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4];
-		w.x = D[0] * q[0].x - eta * (((m1 * q[1].x + m2 * q[2].x) + m3 * q[3].x) + m4 * q[4].x);
-		w.y = D[0] * q[0].y - eta * (((m1 * q[1].y + m2 * q[2].y) + m3 * q[3].y) + m4 * q[4].y);
-	}
-	double part[2] = { 0., 0. };
-	if(in) {
-		qout[k] = q[0];
-		a.w[k] = w;
-		const double fm = m[0];
-		part[0] = fm * (q[0].x * w.x);
-		part[1] = fm * (q[0].y * w.y);
-	}
-	block_sum<2>(part, s_red);
-	block_partials<2>(part, a.part_p);
+	return k;
 }
 
-// Iteration j, second pass (per site): alpha = r.z / p.Ap with
-// p.Ap = pi^2 r_perp.z_perp + q.m w (every workgroup, the same order); s += alpha q,
-// y -= alpha w; partials y.m y, y.m (y / D) -> part_r[j & 1].  Workgroup 0
-// writes SCGState [(j + 1) & 1]: rho -= alpha pi, xi += alpha pi and the
-// recurrence scalars (nobody reads that copy during iteration j).
+// one site's data entering an iteration: r's, A z's and A p's site parts, D, m
+struct SCGSite {
+	double2 y, w, u;
+	double D;
+	int m;
+};
+__device__ inline SCGSite scg_load(const SCGArgs &a, int o, int pin, int k) {
+	SCGSite v;
+	v.m = a.cover[o + k];
+	v.D = a.diag[o + k];
+	v.y = a.y[pin][k];
+	v.w = a.w[pin][k];
+	v.u = a.u[pin][k];
+	return v;
+}
+
+// Iteration j (per site, one launch): the scalars (every workgroup, the same
+// fixed order of the partials), then
+//   q = y / D + beta q,  u = w + beta u  (p and A p),  s += alpha q,  y -= alpha u,
+// the new y also at the four neighbours (from their old y, w, u), the new w and
+// the new partials.  Workgroup 0 writes SCGState [(j + 1) & 1]:
+// rho -= alpha pi, xi += alpha pi and the recurrence scalars (nobody reads that
+// copy during iteration j).
 template <bool FIRST>
-__global__ __launch_bounds__(256) void k_scg_upd(SCGArgs a, int j) {
+__global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	if(a.st->flags) return;
-	const SCGState *Sp = a.s2 + (j & 1);
-	SCGState *Sn = a.s2 + ((j + 1) & 1);
+	const int pin = j & 1, pout = pin ^ 1;
+	const SCGState *Sp = a.s2 + pin;
+	SCGState *Sn = a.s2 + pout;
 	const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
 	if(!Sp->active) {
 		if(writer) *Sn = *Sp;
 		return;
 	}
-	__shared__ double s_red[4 * 4];
+	__shared__ double s_red[(SCG_THREADS / 64) * 6];
 	const DevBox B = a.boxes[0];
 	const int o = B.box_off;
-	const int k0 = xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-	const bool in = k0 < a.nsites;
-	const int k = in ? k0 : a.nsites - 1;
-	const double2 *qj = (j & 1) ? a.q1 : a.q0;
-	const int m = a.cover[o + k];
-	const double D = a.diag[o + k];
-	const double2 q = qj[k], w = a.w[k];
-	double2 s = a.s[k], y = a.y[k];
-	SCGPeek pk;
-	if(FIRST) {
-#pragma unroll
-		for(int c = 0; c < 2; ++c) {
-			pk.done[c] = Sp->done[c];
-			pk.rn2[c] = Sp->rn2[c];
-			pk.rz[c] = Sp->absNew[c];
-			pk.beta[c] = 0.;
-			pk.pi[c] = Sp->pi[c];
-		}
-	}
-	else {
-		double tot[4];
-		reduce_parts<4, 256>((j - 1) & 1 ? a.part_r1 : a.part_r0, gridDim.x, tot, s_red);
-		pk = scg_peek(*Sp, tot);
-	}
-	if(pk.done[0] && pk.done[1]) {
+	const double eta = a.params[a.set].friction_cell;
+	bool in;
+	const int k = scg_site(a, in);
+	const int4 nb = site_nbrs4(k, B.L);
+	// the site data first: it does not depend on the sums
+	const SCGSite c0 = scg_load(a, o, pin, k), cS = scg_load(a, o, pin, nb.x), cW = scg_load(a, o, pin, nb.y),
+			cE = scg_load(a, o, pin, nb.z), cN = scg_load(a, o, pin, nb.w);
+	const double2 q = a.q[k];
+	double2 s = a.s[k];
+	double tot[6];
+	reduce_parts<6, SCG_THREADS, 1>(a.psum[pin], gridDim.x, tot, s_red);
+	const SCGCoef cf = scg_coef(*Sp, tot, FIRST);
+	if(cf.done[0] && cf.done[1]) {
 		if(writer) {
-			SCGState S = *Sp;
+			*Sn = *Sp;
+#pragma unroll
 			for(int c = 0; c < 2; ++c) {
-				S.rn2[c] = pk.rn2[c];
-				S.done[c] = 1;
+				Sn->rn2[c] = cf.rn2[c];
+				Sn->done[c] = 1;
 			}
-			S.active = 0;
-			*Sn = S;
+			Sn->active = 0;
 		}
 		return;
 	}
-	double mqw[2];
-	reduce_parts<2, 256>(a.part_p, gridDim.x, mqw, s_red);
-	double al[2];
-#pragma unroll
-	for(int c = 0; c < 2; ++c) al[c] = pk.done[c] ? 0. : pk.rz[c] / (pk.pi[c] * pk.pi[c] * Sp->rzperp[c] + mqw[c]);
-	if(!pk.done[0]) {
-		s.x += al[0] * q.x;
-		y.x -= al[0] * w.x;
+	const bool d0 = cf.done[0], d1 = cf.done[1];
+	const double al0 = cf.alpha[0], al1 = cf.alpha[1], b0 = cf.beta[0], b1 = cf.beta[1];
+	// p and s at the site, from the old r
+	double2 qn = q;
+	{
+		const double inv = 1. / c0.D;
+		if(!d0) {
+			qn.x = inv * c0.y.x + b0 * q.x;
+			s.x += al0 * qn.x;
+		}
+		if(!d1) {
+			qn.y = inv * c0.y.y + b1 * q.y;
+			s.y += al1 * qn.y;
+		}
 	}
-	if(!pk.done[1]) {
-		s.y += al[1] * q.y;
-		y.y -= al[1] * w.y;
+	// A p and the new r of a site; its new z = y / D (0 where no row lives)
+	auto step = [&](const SCGSite &v, double2 &un, double2 &yn) {
+		un = v.u;
+		yn = v.y;
+		if(!d0) {
+			un.x = v.w.x + b0 * v.u.x;
+			yn.x = v.y.x - al0 * un.x;
+		}
+		if(!d1) {
+			un.y = v.w.y + b1 * v.u.y;
+			yn.y = v.y.y - al1 * un.y;
+		}
+		const double inv = 1. / v.D;
+		return v.m > 0 ? make_double2(inv * yn.x, inv * yn.y) : make_double2(0., 0.);
+	};
+	double2 un0, yn0, un, yn;
+	const double2 z0 = step(c0, un0, yn0);
+	const double2 zS = step(cS, un, yn), zW = step(cW, un, yn), zE = step(cE, un, yn), zN = step(cN, un, yn);
+	double2 wn = make_double2(0., 0.);
+	if(c0.m > 0) {
+		const double m1 = cS.m, m2 = cW.m, m3 = cE.m, m4 = cN.m;
+		wn.x = c0.D * z0.x - eta * (((m1 * zS.x + m2 * zW.x) + m3 * zE.x) + m4 * zN.x);
+		wn.y = c0.D * z0.y - eta * (((m1 * zS.y + m2 * zW.y) + m3 * zE.y) + m4 * zN.y);
 	}
-	double part[4] = { 0., 0., 0., 0. };
+	double part[6] = { 0., 0., 0., 0., 0., 0. };
 	if(in) {
+		a.q[k] = qn;
 		a.s[k] = s;
-		a.y[k] = y;
-		const double fm = m, inv = 1. / D;
-		part[0] = fm * (y.x * y.x);
-		part[1] = fm * (y.y * y.y);
-		part[2] = fm * (y.x * (inv * y.x));
-		part[3] = fm * (y.y * (inv * y.y));
+		a.u[pout][k] = un0;
+		a.y[pout][k] = yn0;
+		a.w[pout][k] = wn;
+		scg_part(part, c0.m, c0.D, yn0, wn);
 	}
-	block_sum<4>(part, s_red);
-	block_partials<4>(part, (j & 1) ? a.part_r1 : a.part_r0);
+	block_sum<6, SCG_THREADS / 64>(part, s_red);
+	block_partials<6>(part, a.psum[pout]);
 	if(writer) {
-		SCGState S = *Sp;
+		*Sn = *Sp;
+#pragma unroll
 		for(int c = 0; c < 2; ++c) {
-			S.rn2[c] = pk.rn2[c];
-			if(!(pk.rn2[c] == pk.rn2[c])) cg_flag_nan(a.st);
-			if(pk.done[c]) {
-				S.done[c] = 1;
+			Sn->rn2[c] = cf.rn2[c];
+			if(!(cf.rn2[c] == cf.rn2[c])) cg_flag_nan(a.st);
+			if(cf.done[c]) {
+				Sn->done[c] = 1;
 				continue;
 			}
-			S.absNew[c] = pk.rz[c];
-			S.beta[c] = pk.beta[c];
-			S.pi[c] = pk.pi[c];
-			S.alpha[c] = al[c];
-			S.rho[c] = Sp->rho[c] - al[c] * pk.pi[c];
-			S.xi[c] = Sp->xi[c] + al[c] * pk.pi[c];
-			S.iters[c] = Sp->iters[c] + 1;
+			Sn->gamma[c] = cf.gamma[c];
+			Sn->alpha[c] = cf.alpha[c];
+			Sn->beta[c] = cf.beta[c];
+			Sn->pi[c] = cf.pi[c];
+			Sn->rho[c] = Sp->rho[c] - cf.alpha[c] * cf.pi[c];
+			Sn->xi[c] = Sp->xi[c] + cf.alpha[c] * cf.pi[c];
+			Sn->iters[c] = Sp->iters[c] + 1;
 		}
-		S.active = 1;
-		*Sn = S;
+		Sn->active = 1;
 	}
 }
 
 // The stop test of the state entering iteration j (after a batch), written
 // into SCGState [j & 1] as rn2_now, done_now, active_now for the host
 __global__ __launch_bounds__(256) void k_scg_check(SCGArgs a, int j) {
-	__shared__ double s_red[4 * 4];
+	__shared__ double s_red[4 * 6];
 	SCGState *S = a.s2 + (j & 1);
 	const int active = S->active;
-	double tot[4] = { 0., 0., 0., 0. };
-	if(active && j > 0) reduce_parts<4, 256>((j - 1) & 1 ? a.part_r1 : a.part_r0, a.nsb, tot, s_red);
+	double tot[6] = { 0., 0., 0., 0., 0., 0. };
+	if(active && j > 0) reduce_parts<6, 256>(a.psum[j & 1], a.nsb, tot, s_red);
 	if(threadIdx.x != 0) return;
 	if(!active || j == 0) {
+#pragma unroll
 		for(int c = 0; c < 2; ++c) {
 			S->rn2_now[c] = S->rn2[c];
 			S->done_now[c] = S->done[c];
@@ -3083,13 +3124,14 @@ __global__ __launch_bounds__(256) void k_scg_check(SCGArgs a, int j) {
 		S->active_now = active;
 		return;
 	}
-	const SCGPeek pk = scg_peek(*S, tot);
+	const SCGCoef cf = scg_coef(*S, tot, false);
+#pragma unroll
 	for(int c = 0; c < 2; ++c) {
-		S->rn2_now[c] = pk.rn2[c];
-		S->done_now[c] = pk.done[c];
-		if(!(pk.rn2[c] == pk.rn2[c])) cg_flag_nan(a.st);
+		S->rn2_now[c] = cf.rn2[c];
+		S->done_now[c] = cf.done[c];
+		if(!(cf.rn2[c] == cf.rn2[c])) cg_flag_nan(a.st);
 	}
-	S->active_now = !(pk.done[0] && pk.done[1]);
+	S->active_now = !(cf.done[0] && cf.done[1]);
 }
 
 // x_i = x0_i + xi (r_perp_i / D) + s(k_i), r_perp as in k_scg_row_init; a
@@ -3116,16 +3158,13 @@ __global__ __launch_bounds__(256) void k_scg_final(SCGArgs a, int j) {
 
 hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s) {
 	switch(which) {
-	case SCG_INIT: RC_LAUNCH(k_scg_init, dim3(a.nsb), dim3(256), 0, s, a); break;
+	case SCG_INIT: RC_LAUNCH(k_scg_init, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a); break;
+	case SCG_INIT_W: RC_LAUNCH(k_scg_init_w, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a); break;
 	case SCG_ROW_INIT: RC_LAUNCH(k_scg_row_init, dim3(a.ncells), dim3(256), 0, s, a); break;
 	case SCG_SETUP_REDUCE: RC_LAUNCH(k_scg_setup_reduce, dim3(1), dim3(1024), 0, s, a); break;
-	case SCG_DIR:
-		if(j == 0) RC_LAUNCH(k_scg_dir<true>, dim3(a.nsb), dim3(256), 0, s, a, j);
-		else RC_LAUNCH(k_scg_dir<false>, dim3(a.nsb), dim3(256), 0, s, a, j);
-		break;
-	case SCG_UPD:
-		if(j == 0) RC_LAUNCH(k_scg_upd<true>, dim3(a.nsb), dim3(256), 0, s, a, j);
-		else RC_LAUNCH(k_scg_upd<false>, dim3(a.nsb), dim3(256), 0, s, a, j);
+	case SCG_ITER:
+		if(j == 0) RC_LAUNCH(k_scg_iter<true>, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a, j);
+		else RC_LAUNCH(k_scg_iter<false>, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a, j);
 		break;
 	case SCG_CHECK: RC_LAUNCH(k_scg_check, dim3(1), dim3(256), 0, s, a, j); break;
 	case SCG_FINAL: RC_LAUNCH(k_scg_final, dim3(a.ncells), dim3(256), 0, s, a, j); break;
