@@ -17,7 +17,13 @@ Prints ONE JSON line.  Also measures, outside the timed region:
     holds the cold solve of a wet run), then at least 3 timed steps, in a
     child process pinned to one core that runs beside the GPU measurement;
   * cg_warm_start: CG time per step with the previous-solution start and the
-    learned start over 500 steps each (the 512^2 / 1600-cell comparison).
+    learned start over 500 steps each (the 512^2 / 1600-cell comparison), with
+    the CG in its default site form and in the row form.
+
+The wet solve is the reference's Jacobi-PCG (Eigen's ConjugateGradient) in
+site form: its iterates carried as scalars times the first residual's
+per-site zero-sum part plus box-site fields, one pass over the box sites per
+iteration (rc_set_cg_form; include/rheocell.h).
 """
 from __future__ import annotations
 
@@ -51,8 +57,15 @@ BYTES = {
     "cg_reduce": (0, 0),
     "nem_sums": (8, 16),            # read phi; write the box sums of phi Q00, phi Q01
     "nem_force": (24, 24),          # read phi; write f (2 doubles); read sum phi^2, sum phi Q00, sum phi Q01
+    # site form of the CG (both components, double2 fields)
+    "scg_gather": (32, 32),         # read b, x0 rows; write their box sums
+    "scg_init": (0, 124),           # read cover, D, sum b, sum x0; write c, y, s, q, A p
+    "scg_init_w": (0, 44),          # read cover, D, y; write w
+    "scg_row_init": (32, 24),       # read b, x0 rows; read c, D
+    "scg_iter": (0, 172),           # read y, w, A p, q, s, cover, D; write y, w, A p, q, s
+    "scg_final": (48, 40),          # read b, x0 rows, write x; read c, s, D
 }
-CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update")
+CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update", "scg_iter")
 
 # HBM traffic per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
 # dispatch, mean over the launches that did work), scaled by what the same
@@ -61,6 +74,7 @@ CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update")
 # way, i.e. reads x 2.0, writes x 1.0.  Recorded in profiles/.
 READ_SCALE, WRITE_SCALE = 262144.0 / 131085.0, 1.0
 TRAFFIC_KIB = {
+    ("wet512_1600", "scg_iter"): (12191.8, 20576.2),
     ("wet512_1600", "cg_row_update"): (68309.3, 84801.2),
     ("wet512_1600", "cg_gather_p"): (44435.5, 52247.6),
     ("dry512_1024", "stage_fused_pred"): (8607.8, 27347.2),
@@ -160,16 +174,18 @@ def cpu_baseline_child(args):
     return subprocess.Popen(cmd, stdout=subprocess.PIPE, text=True)
 
 
-def cg_compare(conf, keys, weights, warmup, steps):
+def cg_compare(conf, keys, weights, warmup, steps, form):
     """CG wall time per step with the previous-solution warm start and with the
-    learned warm start, from the same initial state (same random stream)."""
+    learned warm start, from the same initial state (same random stream), with
+    the CG iterating in the given form (_lib.RC_CG_SITES or RC_CG_ROWS; the
+    learned refinement stages run on rows in both)."""
     from rheocell_amd import _lib, learned
     from rheocell_amd.sim import Simulation
     w, b, scale = learned.load(weights)
     stages = learned.trained_stages(weights)
     out = {}
     for mode in ("previous", "learned"):
-        sim = Simulation([(conf, keys)])
+        sim = Simulation([(conf, keys)], cg_form=form)
         g = sim.gaussians(warmup + steps)
         if mode == "learned":
             learned.attach(sim, w, b, scale, stages=stages)
@@ -392,8 +408,11 @@ def main():
     cgcmp = None
     polar = str(keys.get("interaction_type", "wetpolarmodel")) == "wetpolarmodel"
     if wet and polar and args.weights and os.path.exists(args.weights) and args.cg_steps > 0:
-        log("[bench] CG warm starts: previous solution vs learned, %d steps each" % args.cg_steps)
-        cgcmp = cg_compare(conf, keys, args.weights, args.warmup, args.cg_steps)
+        log("[bench] CG warm starts: previous solution vs learned, %d steps each, site and row forms" % args.cg_steps)
+        cgcmp = {"site_form": cg_compare(conf, keys, args.weights, args.warmup, args.cg_steps, _lib.RC_CG_SITES),
+                 "row_form": cg_compare(conf, keys, args.weights, args.warmup, args.cg_steps, _lib.RC_CG_ROWS),
+                 "note": "site form: the default CG (value above); row form: the same iteration over every row "
+                         "(Eigen's form); the learned start's network stages run on rows in both"}
         log("[bench] " + json.dumps(cgcmp))
 
     cpu = None

@@ -239,12 +239,13 @@ int rc_set_cg_form(rc_handle *h, int form);
 #define RC_T_NEM_SUMS 19           /* k_box_field: nematic box sums of phi Q00, phi Q01 */
 #define RC_T_NEM_FORCE 18          /* k_nem_force: the nematic model's force evaluation */
 #define RC_T_SCG_GATHER 20         /* site form: box sums of b and of the start x0 */
-#define RC_T_SCG_INIT 21           /* site form: per-site set-up (two passes) */
+#define RC_T_SCG_INIT 21           /* site form: per-site set-up (site means, first residual's site part) */
 #define RC_T_SCG_ROW_INIT 22       /* site form: ker-P residual sums over the rows */
 #define RC_T_SCG_ITER 23           /* site form: one iteration (one pass over the box sites) */
 #define RC_T_SCG_FINAL 24          /* site form: the solution rows */
 #define RC_T_SCG_REDUCE 25         /* site form: set-up reduction and the stop test after a batch */
-#define RC_NKINDS 26
+#define RC_T_SCG_INIT_W 26         /* site form: set-up, the first residual's A z and the sums */
+#define RC_NKINDS 27
 int rc_timing(rc_handle *h, int kind, double *ms, long long *launches);
 int rc_enable_timing(rc_handle *h, int on);
 const char *rc_last_error(rc_handle *h);

@@ -23,7 +23,7 @@ RC_CG_ROWS = 1
 T_KINDS = ["box_gather", "stage_fused_pred", "stage_fused_corr", "stage_cf", "stage_fs_pred", "stage_fs_corr",
            "cg_diag", "cg_gather_x", "cg_gather_p", "cg_row_resid", "cg_box_pap", "cg_row_update", "cg_box_n4",
            "cg_solve", "step", "cg_gather_p_first", "cg_reduce", "cg_learned", "nem_force", "nem_sums",
-           "scg_gather", "scg_init", "scg_row_init", "scg_iter", "scg_final", "scg_reduce"]
+           "scg_gather", "scg_init", "scg_row_init", "scg_iter", "scg_final", "scg_reduce", "scg_init_w"]
 # kernel symbol of each timing kind (as rocprofv3 names it)
 T_SYMBOL = {"box_gather": "k_box_gather", "stage_fused_pred": "k_stage<0>", "stage_fused_corr": "k_stage<0>",
             "stage_cf": "k_stage<1>", "stage_fs_pred": "k_stage<2>", "stage_fs_corr": "k_stage<2>",
@@ -32,7 +32,7 @@ T_SYMBOL = {"box_gather": "k_box_gather", "stage_fused_pred": "k_stage<0>", "sta
             "cg_row_update": "k_cg_row<2>", "cg_box_n4": "k_cg_box<0>", "cg_reduce": "k_cg_reduce",
             "nem_force": "k_nem_force", "nem_sums": "k_box_field", "scg_gather": "k_cg_gather<5>",
             "scg_init": "k_scg_init", "scg_row_init": "k_scg_row_init", "scg_iter": "k_scg_iter",
-            "scg_final": "k_scg_final", "scg_reduce": "k_scg_"}
+            "scg_final": "k_scg_final", "scg_reduce": "k_scg_", "scg_init_w": "k_scg_init_w"}
 
 EXPORTS = ["rc_create", "rc_upload_cells", "rc_init_forces", "rc_apply_changes_after_equilibration", "rc_step",
            "rc_download_cells", "rc_get_rows", "rc_get_site_array", "rc_total_rows", "rc_cg_stats",

@@ -476,7 +476,7 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, dalloc(&h->d_part, (size_t) std::max(6 * std::max(cells, 1), 2 * tiles)));
 		HIPCHK(h, dalloc(&h->d_cg, 1));
 		HIPCHK(h, hipMemset(h->d_cg, 0, sizeof(CGState)));
-		const size_t nsb = (sites + SCG_THREADS - 1) / SCG_THREADS;
+		const size_t nsb = (size_t) ((h->dboxes[0].L + SCG_TX - 1) / SCG_TX) * ((h->dboxes[0].L + SCG_TY - 1) / SCG_TY);
 		for(double2 **f : { &h->d_sc, &h->d_sq, &h->d_ss, &h->d_sy[0], &h->d_sy[1], &h->d_sw[0], &h->d_sw[1], &h->d_su[0],
 				&h->d_su[1] })
 			HIPCHK(h, dalloc(f, sites));
@@ -909,7 +909,9 @@ static SCGArgs scg_args(rc_handle *h, const CGArgs &c) {
 	a.max_iter = c.max_iter;
 	a.ncells = h->ncells;
 	a.nsites = (int) h->box_sites;
-	a.nsb = (int) ((h->box_sites + SCG_THREADS - 1) / SCG_THREADS);
+	a.L = h->dboxes[0].L;
+	a.ntx = (a.L + SCG_TX - 1) / SCG_TX;
+	a.nsb = a.ntx * ((a.L + SCG_TY - 1) / SCG_TY);
 	a.st = h->d_st;
 	return a;
 }
@@ -1008,16 +1010,18 @@ static int record_end(rc_handle *h) {
 
 // Site form of the iteration (SCGState), from the start rows in d_v: the box
 // sums of b and x0, the set-up, then iterations in host batches, each ended
-// by a stop test the host reads (batch sizes as in the row form); the
-// solution rows replace the start in d_v and the last state is in h_scg.
+// by a stop test the host reads; the solution rows replace the start in d_v
+// and the last state is in h_scg.  The first batch is two iterations longer
+// than the last solve of the same kind: a launch after convergence returns at
+// entry (about 1 us), far less than another host round trip.
 static int scg_iterations(rc_handle *h, const CGArgs &c, bool cold) {
 	const SCGArgs a = scg_args(h, c);
 	{ TimeScope t(h, RC_T_SCG_GATHER); HIPCHK(h, launch_cg_gather(CG_GATHER_BX, c, h->ntiles, h->stream)); }
 	{ TimeScope t(h, RC_T_SCG_INIT); HIPCHK(h, launch_scg(SCG_INIT, a, 0, h->stream)); }
-	{ TimeScope t(h, RC_T_SCG_INIT); HIPCHK(h, launch_scg(SCG_INIT_W, a, 0, h->stream)); }
+	{ TimeScope t(h, RC_T_SCG_INIT_W); HIPCHK(h, launch_scg(SCG_INIT_W, a, 0, h->stream)); }
 	{ TimeScope t(h, RC_T_SCG_ROW_INIT); HIPCHK(h, launch_scg(SCG_ROW_INIT, a, 0, h->stream)); }
 	{ TimeScope t(h, RC_T_SCG_REDUCE); HIPCHK(h, launch_scg(SCG_SETUP_REDUCE, a, 0, h->stream)); }
-	int batch = std::max(2, h->last_iters[cold ? 1 : 0] - 1), next = 2;
+	int batch = std::max(2, h->last_iters[cold ? 1 : 0] + 2), next = 4;
 	int j = 0;
 	for(;;) {
 		for(int it = 0; it < batch; ++it, ++j) {

@@ -197,11 +197,11 @@ hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream
 
 // Site-space CG (SCGState): one wet box of nsites = L*L sites, box sums of b
 // and x0 in sb and sx (k_cg_gather<CG_GATHER_BX>), D in diag, rows per site in
-// cover.  Site passes run nsb = ceil(nsites / SCG_THREADS) workgroups, one site
-// per thread.  Iteration j reads the fields of parity j & 1 and writes those of
+// cover.  Site passes run one workgroup per SCG_TX x SCG_TY tile of the box
+// (nsb tiles, ntx per tile row), one site per thread.  Iteration j reads the fields of parity j & 1 and writes those of
 // the other parity (its neighbours' old values stay readable), and the partial
 // sums likewise.
-constexpr int SCG_THREADS = 512;
+constexpr int SCG_TX = 64, SCG_TY = 8, SCG_THREADS = SCG_TX * SCG_TY;   // iteration tile: SCG_TX x SCG_TY sites
 struct SCGArgs {
 	const DevBox *boxes;
 	const DevCell *cells;
@@ -222,7 +222,8 @@ struct SCGArgs {
 	SCGState *s2;                // [2]
 	double tol;
 	int max_iter;
-	int ncells, nsb, nsites;
+	int ncells, nsites;
+	int L, ntx, nsb;             // box side; tiles per tile row; tiles (workgroups of a site pass)
 	DevStatus *st;
 };
 // launches of the site-space CG, in solve order:

@@ -2731,12 +2731,6 @@ hipError_t launch_zero_rows(double2 *x, long long rows, int mask, hipStream_t s)
 
 // the four box neighbours of site k, in the triplet order of
 // WetPolarModel.cpp:206-215 (neighbours 1, 3, 5, 7 = S, W, E, N)
-__device__ inline int4 site_nbrs4(int k, int L) {
-	const int y = k / L, x = k - y * L;
-	const int xe = x + 1 == L ? 0 : x + 1, xw = x == 0 ? L - 1 : x - 1;
-	const int yn = y + 1 == L ? 0 : y + 1, ys = y == 0 ? L - 1 : y - 1;
-	return make_int4(ys * L + x, y * L + xw, y * L + xe, yn * L + x);
-}
 __device__ inline void site_nbrs(int k, int L, int *nb) {
 	const int y = k / L, x = k - y * L;
 	const int xe = x + 1 == L ? 0 : x + 1, xw = x == 0 ? L - 1 : x - 1;
@@ -2756,21 +2750,6 @@ __device__ inline int scg_site(const SCGArgs &a, bool &in) {
 	return in ? k : a.nsites - 1;
 }
 
-// w = the site part of A z for z = D^-1 (site part of r): D z(k) - eta N4(m z)
-// from the five sites' y (own first, then S, W, E, N), D and m
-__device__ inline double2 scg_w(const double2 (&y)[5], const double (&D)[5], const int (&m)[5], double eta) {
-	double2 z[5];
-#pragma unroll
-	for(int i = 0; i < 5; ++i) {
-		const double inv = 1. / D[i];
-		z[i] = m[i] > 0 ? make_double2(inv * y[i].x, inv * y[i].y) : make_double2(0., 0.);
-	}
-	if(m[0] == 0) return make_double2(0., 0.);
-	const double m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4];
-	return make_double2(D[0] * z[0].x - eta * (((m1 * z[1].x + m2 * z[2].x) + m3 * z[3].x) + m4 * z[4].x),
-			D[0] * z[0].y - eta * (((m1 * z[1].y + m2 * z[2].y) + m3 * z[3].y) + m4 * z[4].y));
-}
-
 // the partial sums of one site: y.m y, y.m (y / D), (y / D).m w (x, y components)
 __device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, double2 w) {
 	const double fm = m, inv = 1. / D;
@@ -2787,7 +2766,7 @@ __device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, d
 // c = b_bar - D x_bar; y = (P r0) / m = c + eta N4(P x0), the site part of the
 // first residual r0 = b - A x0; s = q = 0 and A p = 0 (p_-1 = 0).
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
-	if(a.st->flags) return;
+	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
 	const DevBox B = a.boxes[0];
 	const int o = B.box_off;
 	const double eta = a.params[a.set].friction_cell;
@@ -2808,6 +2787,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 		y.y = c.y + eta * (((gS.y + gW.y) + gE.y) + gN.y);
 	}
 	const double2 zero = make_double2(0., 0.);
+	if(flags) return;
 	a.c[k] = c;
 	a.y[0][k] = y;
 	a.s[k] = zero;
@@ -2815,32 +2795,91 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	a.u[0][k] = zero;
 }
 
-// Set-up, per site: w of the first residual and the partials of parity 0
-__global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
-	if(a.st->flags) return;
-	__shared__ double s_red[(SCG_THREADS / 64) * 6];
-	const DevBox B = a.boxes[0];
-	const int o = B.box_off;
-	const double eta = a.params[a.set].friction_cell;
-	bool in;
-	const int k = scg_site(a, in);
-	int nb[5];
-	nb[0] = k;
-	site_nbrs(k, B.L, nb + 1);
-	int m[5];
-	double D[5];
-	double2 y[5];
-#pragma unroll
-	for(int i = 0; i < 5; ++i) {
-		m[i] = a.cover[o + nb[i]];
-		D[i] = a.diag[o + nb[i]];
-		y[i] = a.y[0][nb[i]];
+// The tile of a site pass: workgroup b takes tile xcd_item(b) of SCG_TX x SCG_TY
+// sites.  Every slot of the tile and of the one-site ring around it, at tile
+// coordinates (hx, hy) in -1 .. SCG_TX x -1 .. SCG_TY, stands for the box site
+// ((x0 + hx) mod L, (y0 + hy) mod L), so a stencil over the slots is the
+// periodic stencil also in a tile the box edge cuts (slots past the edge hold
+// wrapped sites and are not written).  Thread t owns slot (t mod SCG_TX,
+// t / SCG_TX); threads 0 .. SCG_RING - 1 also take one ring slot.
+constexpr int SCG_HW = SCG_TX + 2, SCG_RING = 2 * SCG_HW + 2 * SCG_TY;
+struct SCGTile {
+	int x0, y0;
+	int k;        // own slot's site
+	bool in;      // the own slot is a box site of this tile
+	int slot;     // own slot's LDS index
+	int hk, hslot;   // ring slot's site and LDS index (hslot < 0: none)
+};
+__device__ inline SCGTile scg_tile(const SCGArgs &a) {
+	SCGTile t;
+	const int tile = xcd_item(blockIdx.x, gridDim.x), L = a.L;
+	t.x0 = (tile % a.ntx) * SCG_TX;
+	t.y0 = (tile / a.ntx) * SCG_TY;
+	const int lx = threadIdx.x % SCG_TX, ly = threadIdx.x / SCG_TX;
+	int gx = t.x0 + lx, gy = t.y0 + ly;
+	t.in = gx < L && gy < L;
+	gx %= L;
+	gy %= L;
+	t.k = gy * L + gx;
+	t.slot = (ly + 1) * SCG_HW + lx + 1;
+	t.hslot = -1;
+	t.hk = 0;
+	const int i = threadIdx.x;
+	if(i < SCG_RING) {
+		int hx, hy;
+		if(i < SCG_HW) { hx = i - 1; hy = -1; }
+		else if(i < 2 * SCG_HW) { hx = i - SCG_HW - 1; hy = SCG_TY; }
+		else if(i < 2 * SCG_HW + SCG_TY) { hx = -1; hy = i - 2 * SCG_HW; }
+		else { hx = SCG_TX; hy = i - 2 * SCG_HW - SCG_TY; }
+		int hxg = t.x0 + hx, hyg = t.y0 + hy;
+		hxg = hxg < 0 ? hxg + L : hxg % L;
+		hyg = hyg < 0 ? hyg + L : hyg % L;
+		t.hk = hyg * L + hxg;
+		t.hslot = (hy + 1) * SCG_HW + hx + 1;
 	}
-	const double2 w = scg_w(y, D, m, eta);
+	return t;
+}
+// w = D z - eta N4(m z) at the own slot, from the slots' m z in LDS (S, W, E, N
+// in the triplet order of WetPolarModel.cpp:206-215)
+__device__ inline double2 scg_w_lds(const double2 *s_mz, int slot, double D, double2 z, int m, double eta) {
+	if(m == 0) return make_double2(0., 0.);
+	const double2 mS = s_mz[slot - SCG_HW], mW = s_mz[slot - 1], mE = s_mz[slot + 1], mN = s_mz[slot + SCG_HW];
+	return make_double2(D * z.x - eta * (((mS.x + mW.x) + mE.x) + mN.x), D * z.y - eta * (((mS.y + mW.y) + mE.y) + mN.y));
+}
+// z = y / D where a row lives (0 elsewhere) and its product with m
+__device__ inline double2 scg_z(double2 y, double D, int m) {
+	const double inv = 1. / D;
+	return m > 0 ? make_double2(inv * y.x, inv * y.y) : make_double2(0., 0.);
+}
+__device__ inline double2 scg_mz(double2 z, int m) {
+	const double fm = m;
+	return make_double2(fm * z.x, fm * z.y);
+}
+
+// Set-up, per tile: w of the first residual and the partials of parity 0
+__global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
+	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
+	__shared__ double s_red[(SCG_THREADS / 64) * 6];
+	__shared__ double2 s_mz[SCG_HW * (SCG_TY + 2)];
+	const int o = a.boxes[0].box_off;
+	const double eta = a.params[a.set].friction_cell;
+	const SCGTile t = scg_tile(a);
+	const int m = a.cover[o + t.k];
+	const double D = a.diag[o + t.k];
+	const double2 y = a.y[0][t.k];
+	const double2 z = scg_z(y, D, m);
+	if(flags) return;
+	s_mz[t.slot] = scg_mz(z, m);
+	if(t.hslot >= 0) {
+		const int hm = a.cover[o + t.hk];
+		s_mz[t.hslot] = scg_mz(scg_z(a.y[0][t.hk], a.diag[o + t.hk], hm), hm);
+	}
+	__syncthreads();
+	const double2 w = scg_w_lds(s_mz, t.slot, D, z, m, eta);
 	double part[6] = { 0., 0., 0., 0., 0., 0. };
-	if(in) {
-		a.w[0][k] = w;
-		scg_part(part, m[0], D[0], y[0], w);
+	if(t.in) {
+		a.w[0][t.k] = w;
+		scg_part(part, m, D, y, w);
 	}
 	block_sum<6, SCG_THREADS / 64>(part, s_red);
 	block_partials<6>(part, a.psum[0]);
@@ -2864,7 +2903,7 @@ __device__ inline int cell_site(const DevCell &C, int L, int q) {
 // D x0_i) - c(k_i), summed as |b|^2, |r_perp|^2, r_perp.(r_perp / D) -> part
 // (6 per cell: b.b x, y; r_perp.r_perp x, y; r_perp.z_perp x, y)
 __global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
-	if(a.st->flags) return;
+	const unsigned flags = a.st->flags;   // tested before the partials are written, not ahead of the loads
 	__shared__ double s_red[4 * 6];
 	const int ci = xcd_item(blockIdx.x, gridDim.x);
 	const DevCell C = a.cells[ci];
@@ -2879,6 +2918,7 @@ __global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
 		red[2] += rx * rx; red[3] += ry * ry;
 		red[4] += rx * (inv * rx); red[5] += ry * (inv * ry);
 	}
+	if(flags) return;
 	block_sum<6>(red, s_red);
 	block_partials<6>(red, a.part);
 }
@@ -2989,38 +3029,41 @@ __device__ inline SCGSite scg_load(const SCGArgs &a, int o, int pin, int k) {
 	return v;
 }
 
-// Iteration j (per site, one launch): the scalars (every workgroup, the same
-// fixed order of the partials), then
+// Iteration j (one launch, one workgroup per tile): the scalars (every
+// workgroup, the same fixed order of the partials), then per site
 //   q = y / D + beta q,  u = w + beta u  (p and A p),  s += alpha q,  y -= alpha u,
-// the new y also at the four neighbours (from their old y, w, u), the new w and
-// the new partials.  Workgroup 0 writes SCGState [(j + 1) & 1]:
+// the new y and m z of every tile and ring slot into LDS, the new w from them
+// and the new partials.  Workgroup 0 writes SCGState [(j + 1) & 1]:
 // rho -= alpha pi, xi += alpha pi and the recurrence scalars (nobody reads that
 // copy during iteration j).
 template <bool FIRST>
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
-	if(a.st->flags) return;
 	const int pin = j & 1, pout = pin ^ 1;
 	const SCGState *Sp = a.s2 + pin;
 	SCGState *Sn = a.s2 + pout;
 	const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
-	if(!Sp->active) {
+	__shared__ double s_red[(SCG_THREADS / 64) * 6];
+	__shared__ double2 s_mz[SCG_HW * (SCG_TY + 2)];
+	const int o = a.boxes[0].box_off;
+	const double eta = a.params[a.set].friction_cell;
+	const SCGTile t = scg_tile(a);
+	// Every load is requested before any is used: the site data, the partials,
+	// the status word and the state (all addresses are valid whatever they
+	// hold), so the launch waits for one memory round trip, not three.
+	const SCGSite c0 = scg_load(a, o, pin, t.k);
+	SCGSite ch{};
+	if(t.hslot >= 0) ch = scg_load(a, o, pin, t.hk);
+	const double2 q = a.q[t.k];
+	double2 s = a.s[t.k];
+	const unsigned flags = a.st->flags;
+	const int active = Sp->active;
+	double tot[6];
+	reduce_parts<6, SCG_THREADS, 1>(a.psum[pin], gridDim.x, tot, s_red);
+	if(flags) return;
+	if(!active) {
 		if(writer) *Sn = *Sp;
 		return;
 	}
-	__shared__ double s_red[(SCG_THREADS / 64) * 6];
-	const DevBox B = a.boxes[0];
-	const int o = B.box_off;
-	const double eta = a.params[a.set].friction_cell;
-	bool in;
-	const int k = scg_site(a, in);
-	const int4 nb = site_nbrs4(k, B.L);
-	// the site data first: it does not depend on the sums
-	const SCGSite c0 = scg_load(a, o, pin, k), cS = scg_load(a, o, pin, nb.x), cW = scg_load(a, o, pin, nb.y),
-			cE = scg_load(a, o, pin, nb.z), cN = scg_load(a, o, pin, nb.w);
-	const double2 q = a.q[k];
-	double2 s = a.s[k];
-	double tot[6];
-	reduce_parts<6, SCG_THREADS, 1>(a.psum[pin], gridDim.x, tot, s_red);
 	const SCGCoef cf = scg_coef(*Sp, tot, FIRST);
 	if(cf.done[0] && cf.done[1]) {
 		if(writer) {
@@ -3049,7 +3092,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 			s.y += al1 * qn.y;
 		}
 	}
-	// A p and the new r of a site; its new z = y / D (0 where no row lives)
+	// A p and the new r of a site
 	auto step = [&](const SCGSite &v, double2 &un, double2 &yn) {
 		un = v.u;
 		yn = v.y;
@@ -3061,25 +3104,25 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 			un.y = v.w.y + b1 * v.u.y;
 			yn.y = v.y.y - al1 * un.y;
 		}
-		const double inv = 1. / v.D;
-		return v.m > 0 ? make_double2(inv * yn.x, inv * yn.y) : make_double2(0., 0.);
 	};
-	double2 un0, yn0, un, yn;
-	const double2 z0 = step(c0, un0, yn0);
-	const double2 zS = step(cS, un, yn), zW = step(cW, un, yn), zE = step(cE, un, yn), zN = step(cN, un, yn);
-	double2 wn = make_double2(0., 0.);
-	if(c0.m > 0) {
-		const double m1 = cS.m, m2 = cW.m, m3 = cE.m, m4 = cN.m;
-		wn.x = c0.D * z0.x - eta * (((m1 * zS.x + m2 * zW.x) + m3 * zE.x) + m4 * zN.x);
-		wn.y = c0.D * z0.y - eta * (((m1 * zS.y + m2 * zW.y) + m3 * zE.y) + m4 * zN.y);
+	double2 un0, yn0;
+	step(c0, un0, yn0);
+	const double2 z0 = scg_z(yn0, c0.D, c0.m);
+	s_mz[t.slot] = scg_mz(z0, c0.m);
+	if(t.hslot >= 0) {
+		double2 unh, ynh;
+		step(ch, unh, ynh);
+		s_mz[t.hslot] = scg_mz(scg_z(ynh, ch.D, ch.m), ch.m);
 	}
+	__syncthreads();
+	const double2 wn = scg_w_lds(s_mz, t.slot, c0.D, z0, c0.m, eta);
 	double part[6] = { 0., 0., 0., 0., 0., 0. };
-	if(in) {
-		a.q[k] = qn;
-		a.s[k] = s;
-		a.u[pout][k] = un0;
-		a.y[pout][k] = yn0;
-		a.w[pout][k] = wn;
+	if(t.in) {
+		a.q[t.k] = qn;
+		a.s[t.k] = s;
+		a.u[pout][t.k] = un0;
+		a.y[pout][t.k] = yn0;
+		a.w[pout][t.k] = wn;
 		scg_part(part, c0.m, c0.D, yn0, wn);
 	}
 	block_sum<6, SCG_THREADS / 64>(part, s_red);
@@ -3137,7 +3180,7 @@ __global__ __launch_bounds__(256) void k_scg_check(SCGArgs a, int j) {
 // x_i = x0_i + xi (r_perp_i / D) + s(k_i), r_perp as in k_scg_row_init; a
 // component with a zero right-hand side is 0 (Eigen's solve of b = 0)
 __global__ __launch_bounds__(256) void k_scg_final(SCGArgs a, int j) {
-	if(a.st->flags) return;
+	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
 	const SCGState *S = a.s2 + (j & 1);
 	const double xi0 = S->xi[0], xi1 = S->xi[1];
 	const bool z0 = S->zero_rhs[0], z1 = S->zero_rhs[1];
@@ -3152,6 +3195,7 @@ __global__ __launch_bounds__(256) void k_scg_final(SCGArgs a, int j) {
 		double2 x;
 		x.x = z0 ? 0. : (x0.x + xi0 * (inv * rx)) + sv.x;
 		x.y = z1 ? 0. : (x0.y + xi1 * (inv * ry)) + sv.y;
+		if(flags) return;
 		a.x[row] = x;
 	}
 }
