@@ -85,8 +85,8 @@ struct rc_handle {
 	CGState *h_cg = nullptr;      // pinned
 	// site-space form of the CG (SCGState)
 	int cg_form = RC_CG_SITES;
-	double2 *d_sc = nullptr, *d_sq = nullptr, *d_ss = nullptr;
-	double2 *d_sy[2] = {}, *d_sw[2] = {}, *d_su[2] = {};
+	double2 *d_sc = nullptr, *d_ss = nullptr;
+	double2 *d_sy[2] = {}, *d_sq[2] = {};
 	double *d_psum[2] = {};
 	SCGState *d_scg = nullptr;    // [2]
 	SCGState *h_scg = nullptr;    // pinned
@@ -477,8 +477,7 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, dalloc(&h->d_cg, 1));
 		HIPCHK(h, hipMemset(h->d_cg, 0, sizeof(CGState)));
 		const size_t nsb = (size_t) ((h->dboxes[0].L + SCG_TX - 1) / SCG_TX) * ((h->dboxes[0].L + SCG_TY - 1) / SCG_TY);
-		for(double2 **f : { &h->d_sc, &h->d_sq, &h->d_ss, &h->d_sy[0], &h->d_sy[1], &h->d_sw[0], &h->d_sw[1], &h->d_su[0],
-				&h->d_su[1] })
+		for(double2 **f : { &h->d_sc, &h->d_ss, &h->d_sy[0], &h->d_sy[1], &h->d_sq[0], &h->d_sq[1] })
 			HIPCHK(h, dalloc(f, sites));
 		for(int i = 0; i < 2; ++i) HIPCHK(h, dalloc(&h->d_psum[i], 6 * nsb));
 		HIPCHK(h, dalloc(&h->d_scg, 2));
@@ -503,9 +502,9 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_ggauss); dfree(h->d_step_base); dfree(h->d_bf);
 	if(h->h_step_base) (void) hipHostFree(h->h_step_base); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
-	dfree(h->d_sc); dfree(h->d_sq); dfree(h->d_ss); dfree(h->d_scg);
+	dfree(h->d_sc); dfree(h->d_ss); dfree(h->d_scg);
 	for(int i = 0; i < 2; ++i) {
-		dfree(h->d_sy[i]); dfree(h->d_sw[i]); dfree(h->d_su[i]); dfree(h->d_psum[i]);
+		dfree(h->d_sy[i]); dfree(h->d_sq[i]); dfree(h->d_psum[i]);
 	}
 	if(h->h_scg) (void) hipHostFree(h->h_scg);
 	if(h->h_st) (void) hipHostFree(h->h_st);
@@ -893,12 +892,10 @@ static SCGArgs scg_args(rc_handle *h, const CGArgs &c) {
 	a.sb = h->d_sp;
 	a.sx = h->d_p2;
 	a.c = h->d_sc;
-	a.q = h->d_sq;
 	a.s = h->d_ss;
 	for(int i = 0; i < 2; ++i) {
 		a.y[i] = h->d_sy[i];
-		a.w[i] = h->d_sw[i];
-		a.u[i] = h->d_su[i];
+		a.q[i] = h->d_sq[i];
 		a.psum[i] = h->d_psum[i];
 	}
 	a.b = h->d_f;

@@ -211,10 +211,9 @@ struct SCGArgs {
 	const double *diag;
 	const double2 *sb, *sx;      // box sums of b and of x0
 	double2 *c;                  // b_bar - D x0_bar per site (site means of b, x0)
-	double2 *q, *s;              // site parts of p and of x - x0
+	double2 *s;                  // site part of x - x0
 	double2 *y[2];               // site part of r
-	double2 *w[2];               // site part of A z, z = D^-1 r
-	double2 *u[2];               // site part of A p
+	double2 *q[2];               // site part of p
 	const double2 *b;            // right-hand side rows
 	double2 *x;                  // start rows in, solution rows out
 	double *part;                // row-pass partials, 6 per cell
@@ -227,12 +226,12 @@ struct SCGArgs {
 	DevStatus *st;
 };
 // launches of the site-space CG, in solve order:
-//   SCG_INIT          per site: c, y = c + eta N4(sx), s = q = 0, A p = 0
-//   SCG_INIT_W        per site: w = the site part of A z and the partials of parity 0
+//   SCG_INIT          per site: c, y = c + eta N4(sx), s = 0
+//   SCG_INIT_W        per site: the partials of parity 0 (with w = the site part of A z)
 //   SCG_ROW_INIT      per cell: |b|^2 and the ker-P residual sums over the rows
 //   SCG_SETUP_REDUCE  one workgroup: Eigen's start into SCGState [0]
-//   SCG_ITER          iteration j: gamma, delta, the stop test, alpha, beta; p, A p,
-//                     x, r updated; w of the new r and the partials; SCGState [(j + 1) & 1]
+//   SCG_ITER          iteration j: gamma, delta, the stop test, alpha, beta; p and A p
+//                     from it, x and r updated; the partials of the new r; SCGState [(j + 1) & 1]
 //   SCG_CHECK         the stop test of the state entering iteration j (after a batch)
 //   SCG_FINAL         x = x0 + xi z_perp + P^T s from the state entering iteration j
 enum SCGLaunch { SCG_INIT = 0, SCG_INIT_W, SCG_ROW_INIT, SCG_SETUP_REDUCE, SCG_ITER, SCG_CHECK, SCG_FINAL };

@@ -2096,6 +2096,28 @@ __device__ inline bool cg_update_peek(const CGUpdIn &u, const double *tot, bool 
 	return !(done[0] && done[1]);
 }
 
+// the per-thread sums acc of a block of NT threads added up: each wave's lanes
+// (wave_sum), then the waves in order; every thread gets tot
+template <int NV, int NT>
+__device__ inline void reduce_tail(double (&acc)[NV], double (&tot)[NV], double *s_red) {
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+	for(int j = 0; j < NV; ++j) acc[j] = wave_sum(acc[j]);
+	if(lane == 0) {
+#pragma unroll
+		for(int j = 0; j < NV; ++j) s_red[wave * NV + j] = acc[j];
+	}
+	__syncthreads();
+#pragma unroll
+	for(int j = 0; j < NV; ++j) {
+		double t = 0.;
+#pragma unroll 4
+		for(int w = 0; w < NT / 64; ++w) t += s_red[w * NV + j];
+		tot[j] = t;
+	}
+	__syncthreads();
+}
+
 // sums SoA partials of the previous kernel(s) over a block of NT threads in a
 // fixed order (deterministic, the same in every workgroup); every thread gets
 // tot.  U partials per thread and value are requested at once: U changes
@@ -2126,22 +2148,7 @@ __device__ inline void reduce_parts(const double *part, int nblk, double (&tot)[
 #pragma unroll
 			for(int j = 0; j < NV; ++j) acc[j] += b + u * NT < nblk ? v[u][j] : 0.;
 	}
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-	for(int j = 0; j < NV; ++j) acc[j] = wave_sum(acc[j]);
-	if(lane == 0) {
-#pragma unroll
-		for(int j = 0; j < NV; ++j) s_red[wave * NV + j] = acc[j];
-	}
-	__syncthreads();
-#pragma unroll
-	for(int j = 0; j < NV; ++j) {
-		double t = 0.;
-#pragma unroll 4
-		for(int w = 0; w < NT / 64; ++w) t += s_red[w * NV + j];
-		tot[j] = t;
-	}
-	__syncthreads();
+	reduce_tail<NV, NT>(acc, tot, s_red);
 }
 
 template <int WHAT>
@@ -2762,9 +2769,33 @@ __device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, d
 	part[5] = fm * (zy * w.y);
 }
 
+// The sums of the previous site pass's partials (6 values per tile), split so
+// that a launch requests this thread's first partials with its other loads
+// and tests the status word before using any: scg_parts_load, then
+// scg_parts_sum, in the order of reduce_parts<6, SCG_THREADS, 1>.
+__device__ inline void scg_parts_load(const double *part, int nblk, double (&v)[6]) {
+	const int b = min((int) threadIdx.x, nblk - 1);
+#pragma unroll
+	for(int j = 0; j < 6; ++j) v[j] = part[j * nblk + b];
+}
+__device__ inline void scg_parts_sum(const double *part, int nblk, const double (&v)[6], double (&tot)[6],
+		double *s_red) {
+	double acc[6];
+	const bool own = (int) threadIdx.x < nblk;
+#pragma unroll
+	for(int j = 0; j < 6; ++j) {
+		acc[j] = 0.;
+		acc[j] += own ? v[j] : 0.;
+	}
+	for(int b = threadIdx.x + SCG_THREADS; b < nblk; b += SCG_THREADS)
+#pragma unroll
+		for(int j = 0; j < 6; ++j) acc[j] += part[j * nblk + b];
+	reduce_tail<6, SCG_THREADS>(acc, tot, s_red);
+}
+
 // Set-up, per site: site means b_bar = P b / m, x_bar = P x0 / m;
 // c = b_bar - D x_bar; y = (P r0) / m = c + eta N4(P x0), the site part of the
-// first residual r0 = b - A x0; s = q = 0 and A p = 0 (p_-1 = 0).
+// first residual r0 = b - A x0; s = 0.
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
 	const DevBox B = a.boxes[0];
@@ -2791,8 +2822,6 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	a.c[k] = c;
 	a.y[0][k] = y;
 	a.s[k] = zero;
-	a.q[k] = zero;
-	a.u[0][k] = zero;
 }
 
 // The tile of a site pass: workgroup b takes tile xcd_item(b) of SCG_TX x SCG_TY
@@ -2856,7 +2885,7 @@ __device__ inline double2 scg_mz(double2 z, int m) {
 	return make_double2(fm * z.x, fm * z.y);
 }
 
-// Set-up, per tile: w of the first residual and the partials of parity 0
+// Set-up, per tile: the partials of parity 0, with w = A z of the first residual
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
 	__shared__ double s_red[(SCG_THREADS / 64) * 6];
@@ -2877,10 +2906,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
 	__syncthreads();
 	const double2 w = scg_w_lds(s_mz, t.slot, D, z, m, eta);
 	double part[6] = { 0., 0., 0., 0., 0., 0. };
-	if(t.in) {
-		a.w[0][t.k] = w;
-		scg_part(part, m, D, y, w);
-	}
+	if(t.in) scg_part(part, m, D, y, w);
 	block_sum<6, SCG_THREADS / 64>(part, s_red);
 	block_partials<6>(part, a.psum[0]);
 }
@@ -3013,27 +3039,93 @@ __device__ inline SCGCoef scg_coef(const SCGState &S, const double (&tot)[6], bo
 	return k;
 }
 
-// one site's data entering an iteration: r's, A z's and A p's site parts, D, m
+// The iteration's tile with a two-site ring: slots at tile coordinates (hx, hy)
+// in -2 .. SCG_TX + 1 x -2 .. SCG_TY + 1, grid index (hy + 2) SCG_G2W + hx + 2,
+// each standing for the box site ((x0 + hx) mod L, (y0 + hy) mod L).  Thread t
+// owns slot (t mod SCG_TX, t / SCG_TX) and threads 0 .. SCG_RING2 - 1 one ring
+// slot each: the two rows below, the two above, then the two columns on
+// either side, row by row.
+constexpr int SCG_G2W = SCG_TX + 4, SCG_G2H = SCG_TY + 4, SCG_RING2 = SCG_G2W * SCG_G2H - SCG_THREADS;
+static_assert(SCG_RING2 <= SCG_THREADS, "one ring slot per thread");
+struct SCGTile2 {
+	int k;         // own slot's site
+	bool in;       // the own slot is a box site of this tile
+	int g;         // own slot's grid index
+	int hk, hg;    // ring slot's site and grid index (hg < 0: none)
+	bool hin;      // the ring slot lies in the one-site ring
+};
+__device__ inline SCGTile2 scg_tile2(const SCGArgs &a) {
+	SCGTile2 t;
+	const int tile = xcd_item(blockIdx.x, gridDim.x), L = a.L;
+	const int x0 = (tile % a.ntx) * SCG_TX, y0 = (tile / a.ntx) * SCG_TY;
+	const int lx = threadIdx.x % SCG_TX, ly = threadIdx.x / SCG_TX;
+	int gx = x0 + lx, gy = y0 + ly;
+	t.in = gx < L && gy < L;
+	gx %= L;
+	gy %= L;
+	t.k = gy * L + gx;
+	t.g = (ly + 2) * SCG_G2W + lx + 2;
+	t.hg = -1;
+	t.hk = 0;
+	t.hin = false;
+	const int i = threadIdx.x;
+	if(i < SCG_RING2) {
+		int hx, hy;
+		if(i < 2 * SCG_G2W) {
+			hy = i / SCG_G2W - 2;
+			hx = i % SCG_G2W - 2;
+		}
+		else if(i < 4 * SCG_G2W) {
+			const int r = i - 2 * SCG_G2W;
+			hy = SCG_TY + r / SCG_G2W;
+			hx = r % SCG_G2W - 2;
+		}
+		else {
+			const int r = i - 4 * SCG_G2W, c = r & 3;
+			hy = r >> 2;
+			hx = c < 2 ? c - 2 : SCG_TX + c - 2;
+		}
+		int hxg = x0 + hx, hyg = y0 + hy;
+		hxg = hxg < 0 ? hxg + L : hxg % L;
+		hyg = hyg < 0 ? hyg + L : hyg % L;
+		t.hk = hyg * L + hxg;
+		t.hg = (hy + 2) * SCG_G2W + hx + 2;
+		t.hin = hx >= -1 && hx <= SCG_TX && hy >= -1 && hy <= SCG_TY;
+	}
+	return t;
+}
+
+// one slot's data entering an iteration: r's and p's site parts, D, m
 struct SCGSite {
-	double2 y, w, u;
+	double2 y, q;
 	double D;
 	int m;
 };
-__device__ inline SCGSite scg_load(const SCGArgs &a, int o, int pin, int k) {
+__device__ inline SCGSite scg_load(const SCGArgs &a, int o, int pin, int k, bool first) {
 	SCGSite v;
 	v.m = a.cover[o + k];
 	v.D = a.diag[o + k];
 	v.y = a.y[pin][k];
-	v.w = a.w[pin][k];
-	v.u = a.u[pin][k];
+	v.q = first ? make_double2(0., 0.) : a.q[pin][k];
 	return v;
+}
+// the m-weighted four-neighbour sum of a grid of m v values (S, W, E, N in the
+// triplet order of WetPolarModel.cpp:206-215) and D v - eta times it: the site
+// part of A applied to v (0 where no row lives)
+__device__ inline double2 scg_apply(const double2 *s_mv, int g, double D, double2 v, int m, double eta) {
+	if(m == 0) return make_double2(0., 0.);
+	const double2 mS = s_mv[g - SCG_G2W], mW = s_mv[g - 1], mE = s_mv[g + 1], mN = s_mv[g + SCG_G2W];
+	return make_double2(D * v.x - eta * (((mS.x + mW.x) + mE.x) + mN.x), D * v.y - eta * (((mS.y + mW.y) + mE.y) + mN.y));
 }
 
 // Iteration j (one launch, one workgroup per tile): the scalars (every
-// workgroup, the same fixed order of the partials), then per site
-//   q = y / D + beta q,  u = w + beta u  (p and A p),  s += alpha q,  y -= alpha u,
-// the new y and m z of every tile and ring slot into LDS, the new w from them
-// and the new partials.  Workgroup 0 writes SCGState [(j + 1) & 1]:
+// workgroup, the same fixed order of the partials), then
+//   q = z + beta q (p),  A p from q over the one-site ring,  s += alpha q,
+//   y -= alpha A p,  z = y / D,  w = A z (for the sums only),
+// q over the tile and its two-site ring, the new y over the one-site ring,
+// each exchanged through LDS as m q and m z.  Only y, q (both parities) and s
+// are stored: A p and w are formed from them, so an iteration reads and
+// writes 108 B per box site.  Workgroup 0 writes SCGState [(j + 1) & 1]:
 // rho -= alpha pi, xi += alpha pi and the recurrence scalars (nobody reads that
 // copy during iteration j).
 template <bool FIRST>
@@ -3043,27 +3135,30 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	SCGState *Sn = a.s2 + pout;
 	const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
 	__shared__ double s_red[(SCG_THREADS / 64) * 6];
-	__shared__ double2 s_mz[SCG_HW * (SCG_TY + 2)];
+	__shared__ double2 s_mq[SCG_G2W * SCG_G2H];
+	__shared__ double2 s_mz[SCG_G2W * SCG_G2H];
 	const int o = a.boxes[0].box_off;
 	const double eta = a.params[a.set].friction_cell;
-	const SCGTile t = scg_tile(a);
-	// Every load is requested before any is used: the site data, the partials,
-	// the status word and the state (all addresses are valid whatever they
-	// hold), so the launch waits for one memory round trip, not three.
-	const SCGSite c0 = scg_load(a, o, pin, t.k);
+	const SCGTile2 t = scg_tile2(a);
+	// Every load is requested before any is used: the site data, this
+	// thread's partials, the status word and the state (all addresses are
+	// valid whatever they hold), so the launch waits for one memory round trip,
+	// not three, and a launch after convergence returns after that one.
+	const SCGSite c0 = scg_load(a, o, pin, t.k, FIRST);
 	SCGSite ch{};
-	if(t.hslot >= 0) ch = scg_load(a, o, pin, t.hk);
-	const double2 q = a.q[t.k];
+	if(t.hg >= 0) ch = scg_load(a, o, pin, t.hk, FIRST);
 	double2 s = a.s[t.k];
+	double pre[6];
+	scg_parts_load(a.psum[pin], gridDim.x, pre);
 	const unsigned flags = a.st->flags;
 	const int active = Sp->active;
-	double tot[6];
-	reduce_parts<6, SCG_THREADS, 1>(a.psum[pin], gridDim.x, tot, s_red);
 	if(flags) return;
 	if(!active) {
 		if(writer) *Sn = *Sp;
 		return;
 	}
+	double tot[6];
+	scg_parts_sum(a.psum[pin], gridDim.x, pre, tot, s_red);
 	const SCGCoef cf = scg_coef(*Sp, tot, FIRST);
 	if(cf.done[0] && cf.done[1]) {
 		if(writer) {
@@ -3079,50 +3174,44 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	}
 	const bool d0 = cf.done[0], d1 = cf.done[1];
 	const double al0 = cf.alpha[0], al1 = cf.alpha[1], b0 = cf.beta[0], b1 = cf.beta[1];
-	// p and s at the site, from the old r
-	double2 qn = q;
-	{
-		const double inv = 1. / c0.D;
-		if(!d0) {
-			qn.x = inv * c0.y.x + b0 * q.x;
-			s.x += al0 * qn.x;
-		}
-		if(!d1) {
-			qn.y = inv * c0.y.y + b1 * q.y;
-			s.y += al1 * qn.y;
-		}
-	}
-	// A p and the new r of a site
-	auto step = [&](const SCGSite &v, double2 &un, double2 &yn) {
-		un = v.u;
-		yn = v.y;
-		if(!d0) {
-			un.x = v.w.x + b0 * v.u.x;
-			yn.x = v.y.x - al0 * un.x;
-		}
-		if(!d1) {
-			un.y = v.w.y + b1 * v.u.y;
-			yn.y = v.y.y - al1 * un.y;
-		}
+	// p = z + beta p_prev at a slot (z = y / D where a row lives); a converged
+	// component keeps its p
+	auto pdir = [&](const SCGSite &v) {
+		const double2 z = scg_z(v.y, v.D, v.m);
+		double2 qn = v.q;
+		if(!d0) qn.x = z.x + b0 * v.q.x;
+		if(!d1) qn.y = z.y + b1 * v.q.y;
+		return qn;
 	};
-	double2 un0, yn0;
-	step(c0, un0, yn0);
-	const double2 z0 = scg_z(yn0, c0.D, c0.m);
-	s_mz[t.slot] = scg_mz(z0, c0.m);
-	if(t.hslot >= 0) {
-		double2 unh, ynh;
-		step(ch, unh, ynh);
-		s_mz[t.hslot] = scg_mz(scg_z(ynh, ch.D, ch.m), ch.m);
+	const double2 q0 = pdir(c0);
+	s_mq[t.g] = scg_mz(q0, c0.m);
+	double2 qh = make_double2(0., 0.);
+	if(t.hg >= 0) {
+		qh = pdir(ch);
+		s_mq[t.hg] = scg_mz(qh, ch.m);
 	}
 	__syncthreads();
-	const double2 wn = scg_w_lds(s_mz, t.slot, c0.D, z0, c0.m, eta);
+	// A p and the new r over the one-site ring; m z of the new r into LDS
+	auto rstep = [&](const SCGSite &v, double2 qn, int g) {
+		const double2 ap = scg_apply(s_mq, g, v.D, qn, v.m, eta);
+		double2 yn = v.y;
+		if(!d0) yn.x = v.y.x - al0 * ap.x;
+		if(!d1) yn.y = v.y.y - al1 * ap.y;
+		return yn;
+	};
+	const double2 yn0 = rstep(c0, q0, t.g);
+	const double2 z0 = scg_z(yn0, c0.D, c0.m);
+	s_mz[t.g] = scg_mz(z0, c0.m);
+	if(t.hin) s_mz[t.hg] = scg_mz(scg_z(rstep(ch, qh, t.hg), ch.D, ch.m), ch.m);
+	__syncthreads();
+	const double2 wn = scg_apply(s_mz, t.g, c0.D, z0, c0.m, eta);
+	if(!d0) s.x += al0 * q0.x;
+	if(!d1) s.y += al1 * q0.y;
 	double part[6] = { 0., 0., 0., 0., 0., 0. };
 	if(t.in) {
-		a.q[t.k] = qn;
 		a.s[t.k] = s;
-		a.u[pout][t.k] = un0;
+		a.q[pout][t.k] = q0;
 		a.y[pout][t.k] = yn0;
-		a.w[pout][t.k] = wn;
 		scg_part(part, c0.m, c0.D, yn0, wn);
 	}
 	block_sum<6, SCG_THREADS / 64>(part, s_red);
@@ -3151,12 +3240,17 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 
 // The stop test of the state entering iteration j (after a batch), written
 // into SCGState [j & 1] as rn2_now, done_now, active_now for the host
-__global__ __launch_bounds__(256) void k_scg_check(SCGArgs a, int j) {
-	__shared__ double s_red[4 * 6];
+__global__ __launch_bounds__(SCG_THREADS) void k_scg_check(SCGArgs a, int j) {
+	__shared__ double s_red[(SCG_THREADS / 64) * 6];
 	SCGState *S = a.s2 + (j & 1);
 	const int active = S->active;
 	double tot[6] = { 0., 0., 0., 0., 0., 0. };
-	if(active && j > 0) reduce_parts<6, 256>(a.psum[j & 1], a.nsb, tot, s_red);
+	if(active && j > 0) {
+		// the sums in the order of the iteration that would use them
+		double pre[6];
+		scg_parts_load(a.psum[j & 1], a.nsb, pre);
+		scg_parts_sum(a.psum[j & 1], a.nsb, pre, tot, s_red);
+	}
 	if(threadIdx.x != 0) return;
 	if(!active || j == 0) {
 #pragma unroll
@@ -3210,7 +3304,7 @@ hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s) {
 		if(j == 0) RC_LAUNCH(k_scg_iter<true>, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a, j);
 		else RC_LAUNCH(k_scg_iter<false>, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a, j);
 		break;
-	case SCG_CHECK: RC_LAUNCH(k_scg_check, dim3(1), dim3(256), 0, s, a, j); break;
+	case SCG_CHECK: RC_LAUNCH(k_scg_check, dim3(1), dim3(SCG_THREADS), 0, s, a, j); break;
 	case SCG_FINAL: RC_LAUNCH(k_scg_final, dim3(a.ncells), dim3(256), 0, s, a, j); break;
 	default: return hipErrorInvalidValue;
 	}
