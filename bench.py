@@ -165,6 +165,17 @@ def cpu_baseline(conf, keys, steps, budget_s=12.0):
     return out
 
 
+def ensemble_boxes(workload, seed):
+    """The replica ensembles: 32 dry replicas (ensemble32) or 8 wet ones
+    (wet_ensemble8), uneven boxes and patches, one handle; None otherwise."""
+    from rheocell_amd import confgen
+    if workload == "ensemble32":
+        return confgen.replica_ensemble(32, seed=seed)
+    if workload == "wet_ensemble8":
+        return confgen.wet_replica_ensemble(8, seed=seed)
+    return None
+
+
 def cpu_baseline_child(args):
     """The CPU baseline in a child process (started before this process
     touches the GPU), so that it runs beside the GPU measurement."""
@@ -267,8 +278,7 @@ def main():
         return
     if args.cpu_baseline_only:
         from rheocell_amd import confgen
-        boxes = confgen.replica_ensemble(32, seed=args.seed) if args.workload == "ensemble32" else \
-            [confgen.workload(args.workload, seed=args.seed)]
+        boxes = ensemble_boxes(args.workload, args.seed) or [confgen.workload(args.workload, seed=args.seed)]
         conf, keys = boxes[0]
         keys["_name"] = args.workload if len(boxes) == 1 else "%s replica 0 (%d cells, %d^2 box)" % (
             args.workload, len(conf.cells), conf.Lx)
@@ -288,10 +298,7 @@ def main():
         cpu_child = cpu_baseline_child(args)
 
     log("[bench] generating %s (seed %d)" % (args.workload, args.seed))
-    if args.workload == "ensemble32":
-        boxes = confgen.replica_ensemble(32, seed=args.seed)
-    else:
-        boxes = [confgen.workload(args.workload, seed=args.seed)]
+    boxes = ensemble_boxes(args.workload, args.seed) or [confgen.workload(args.workload, seed=args.seed)]
     conf, keys = boxes[0]
     keys["_name"] = args.workload if len(boxes) == 1 else "%s replica 0 (%d cells, %d^2 box)" % (
         args.workload, len(conf.cells), conf.Lx)
@@ -407,7 +414,7 @@ def main():
 
     cgcmp = None
     polar = str(keys.get("interaction_type", "wetpolarmodel")) == "wetpolarmodel"
-    if wet and polar and args.weights and os.path.exists(args.weights) and args.cg_steps > 0:
+    if wet and polar and len(boxes) == 1 and args.weights and os.path.exists(args.weights) and args.cg_steps > 0:
         log("[bench] CG warm starts: previous solution vs learned, %d steps each, site and row forms" % args.cg_steps)
         cgcmp = {"site_form": cg_compare(conf, keys, args.weights, args.warmup, args.cg_steps, _lib.RC_CG_SITES),
                  "row_form": cg_compare(conf, keys, args.weights, args.warmup, args.cg_steps, _lib.RC_CG_ROWS),

@@ -108,6 +108,19 @@ def replica_ensemble(nrep: int = 32, seed: int = 1, coverage: float = 0.9):
     return out
 
 
+def wet_replica_ensemble(nrep: int = 8, seed: int = 1, coverage: float = 0.9, tol: float = 1e-8):
+    """Independent wet replicas (friction_cell = 1, CG to ``tol``): the boxes,
+    uneven radii and patches, seeds and zetaQ_self of ``replica_ensemble``."""
+    out = []
+    for conf, keys in replica_ensemble(nrep, seed, coverage):
+        keys = dict(keys)
+        keys["friction_cell"] = 1.0
+        keys["CGtolerance"] = tol
+        keys["steps"] = 500
+        out.append((conf, keys))
+    return out
+
+
 # The reference's own example model (examples/input: interaction_type =
 # wetmodel and its parameters) on the 512^2 / 1600-cell lattice; and the dry
 # polar model in a square wall box (SquareWalls, lambda_wall = 4) at 512^2 /

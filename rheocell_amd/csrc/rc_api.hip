@@ -88,8 +88,13 @@ struct rc_handle {
 	double2 *d_sc = nullptr, *d_ss = nullptr;
 	double2 *d_sy[2] = {}, *d_sq[2] = {};
 	double *d_psum[2] = {};
-	SCGState *d_scg = nullptr;    // [2]
-	SCGState *h_scg = nullptr;    // pinned
+	SCGState *d_scg = nullptr;    // [2 * nboxes]
+	SCGState *h_scg = nullptr;    // pinned, [2 * nboxes]
+	SCGBox *d_cgb = nullptr;      // [nboxes]
+	std::vector<SCGBox> cgb;      // host copy (its iteration caps follow the layout)
+	int *d_cg_tile_box = nullptr; // [cg_ntiles]
+	int cg_ntiles = 0;
+	int scg_parity = 0;           // h_scg[2 b + scg_parity]: each box's state after the last solve
 	hipEvent_t cg_e0 = nullptr, cg_e1 = nullptr;
 	DevStatus *h_st = nullptr;    // pinned
 
@@ -133,8 +138,12 @@ struct rc_handle {
 	double persist_us = 0., groups_us = 0.;
 	int cur_set = 0;              // parameter set currently applied
 	int cf_set = 0;               // set of the force evaluation preceding the next field update
-	int restart_solver = 0;
-	long long size_rows = 0, size_rows_old = 0;
+	// the reference's solver restart bookkeeping (WetPolarModel.cpp:151-157), one
+	// simulation per box: rows of each box now and at its last solve, warm
+	// solves since its last cold one; rows of box b are box_rows[b] .. box_rows[b + 1] - 1
+	std::vector<int> restart_solver;
+	std::vector<long long> size_rows, size_rows_old, box_rows;
+	std::vector<char> cold_box;   // boxes whose current solve starts from zero
 	int warm_mode = RC_WARM_REFERENCE;
 	int last_iters[2] = { 8, 8 };   // iterations of the last warm / cold solve (first batch size)
 	rc_cg_stats_t cgs{};
@@ -401,10 +410,6 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		if(d.params.J_Q != 0 && d.params.interaction == RC_INTERACTION_WETPOLAR) h->jq = true;
 		if(d.params.omega != 0) h->win = true;
 	}
-	if(h->wet && nboxes != 1) {
-		*out = h;
-		return fail(h, RC_ERR_ARG, "the wet model (friction_cell != 0) runs one box per handle");
-	}
 	for(int b = 1; b < nboxes; ++b)
 		if((boxes[b].params.friction_cell != 0) != h->wet) {
 			*out = h;
@@ -476,13 +481,36 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, dalloc(&h->d_part, (size_t) std::max(6 * std::max(cells, 1), 2 * tiles)));
 		HIPCHK(h, dalloc(&h->d_cg, 1));
 		HIPCHK(h, hipMemset(h->d_cg, 0, sizeof(CGState)));
-		const size_t nsb = (size_t) ((h->dboxes[0].L + SCG_TX - 1) / SCG_TX) * ((h->dboxes[0].L + SCG_TY - 1) / SCG_TY);
+		// the site-form tiles of every box (box-major) and each box's solver parameters
+		std::vector<SCGBox> &cgb = h->cgb;
+		cgb.assign(nboxes, SCGBox{});
+		std::vector<int> cg_tile_box;
+		for(int b = 0; b < nboxes; ++b) {
+			const DevBox &B = h->dboxes[b];
+			SCGBox &g = cgb[b];
+			g.L = B.L;
+			g.box_off = B.box_off;
+			g.ntx = (B.L + SCG_TX - 1) / SCG_TX;
+			g.tile_off = (int) cg_tile_box.size();
+			g.ntiles = g.ntx * ((B.L + SCG_TY - 1) / SCG_TY);
+			g.cell_begin = B.cell_begin;
+			g.cell_end = B.cell_end;
+			const rc_model_params &mp = boxes[b].params;
+			g.tol = mp.CGtolerance > 0 ? mp.CGtolerance : 2.220446049250313e-16;
+			g.max_iter = mp.cg_max_iter > 0 ? mp.cg_max_iter : 1 << 30;
+			cg_tile_box.insert(cg_tile_box.end(), g.ntiles, b);
+		}
+		h->cg_ntiles = (int) cg_tile_box.size();
 		for(double2 **f : { &h->d_sc, &h->d_ss, &h->d_sy[0], &h->d_sy[1], &h->d_sq[0], &h->d_sq[1] })
 			HIPCHK(h, dalloc(f, sites));
-		for(int i = 0; i < 2; ++i) HIPCHK(h, dalloc(&h->d_psum[i], 6 * nsb));
-		HIPCHK(h, dalloc(&h->d_scg, 2));
-		HIPCHK(h, hipMemset(h->d_scg, 0, 2 * sizeof(SCGState)));
-		HIPCHK(h, hipHostMalloc((void **) &h->h_scg, sizeof(SCGState)));
+		for(int i = 0; i < 2; ++i) HIPCHK(h, dalloc(&h->d_psum[i], 6 * (size_t) h->cg_ntiles));
+		HIPCHK(h, dalloc(&h->d_scg, 2 * (size_t) nboxes));
+		HIPCHK(h, hipMemset(h->d_scg, 0, 2 * nboxes * sizeof(SCGState)));
+		HIPCHK(h, hipHostMalloc((void **) &h->h_scg, 2 * nboxes * sizeof(SCGState)));
+		HIPCHK(h, dalloc(&h->d_cgb, nboxes));
+		HIPCHK(h, hipMemcpy(h->d_cgb, cgb.data(), sizeof(SCGBox) * nboxes, hipMemcpyHostToDevice));
+		HIPCHK(h, dalloc(&h->d_cg_tile_box, h->cg_ntiles));
+		HIPCHK(h, hipMemcpy(h->d_cg_tile_box, cg_tile_box.data(), sizeof(int) * h->cg_ntiles, hipMemcpyHostToDevice));
 	}
 	return RC_OK;
 }
@@ -502,7 +530,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_ggauss); dfree(h->d_step_base); dfree(h->d_bf);
 	if(h->h_step_base) (void) hipHostFree(h->h_step_base); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
-	dfree(h->d_sc); dfree(h->d_ss); dfree(h->d_scg);
+	dfree(h->d_sc); dfree(h->d_ss); dfree(h->d_scg); dfree(h->d_cgb); dfree(h->d_cg_tile_box);
 	for(int i = 0; i < 2; ++i) {
 		dfree(h->d_sy[i]); dfree(h->d_sq[i]); dfree(h->d_psum[i]);
 	}
@@ -540,8 +568,10 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 	long long rows = 0;
 	int maxS = 0, maxW = 0;
 	std::vector<std::pair<int, int>> dims;
+	std::vector<long long> box_rows(h->nboxes + 1, 0);
 	for(int b = 0; b < h->nboxes; ++b) {
 		const DevBox &B = h->dboxes[b];
+		box_rows[b] = rows;
 		for(int i = B.cell_begin; i < B.cell_end; ++i) {
 			const rc_cell &c = cells[i];
 			DevCell &d = dc[i];
@@ -599,6 +629,15 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 		}
 	}
 	if(rows >= (1ll << 31)) return fail(h, RC_ERR_ARG, "too many patch sites (%lld)", rows);
+	box_rows[h->nboxes] = rows;
+	h->box_rows = box_rows;
+	if(h->wet) {
+		// Eigen's default cap, 2 x the matrix size, per box (its own solve)
+		for(int b = 0; b < h->nboxes; ++b)
+			if(h->boxes[b].params.cg_max_iter <= 0)
+				h->cgb[b].max_iter = (int) std::min<long long>(2 * (box_rows[b + 1] - box_rows[b]), 1ll << 30);
+		HIPCHK(h, hipMemcpy(h->d_cgb, h->cgb.data(), sizeof(SCGBox) * h->nboxes, hipMemcpyHostToDevice));
+	}
 	if(rows != h->rows) {
 		dfree(h->d_phiA); dfree(h->d_phiB); dfree(h->d_dphi); dfree(h->d_rep); dfree(h->d_v);
 		dfree(h->d_f); dfree(h->d_r); dfree(h->d_p);
@@ -876,6 +915,7 @@ static CGArgs cg_args(rc_handle *h) {
 	const int mi = h->boxes[0].params.cg_max_iter;
 	c.max_iter = mi > 0 ? mi : (int) std::min<long long>(2 * h->rows, 1ll << 30);
 	c.ncells = h->ncells;
+	c.nboxes = h->nboxes;
 	c.nbox_blk = (int) ((h->box_sites + 255) / 256);
 	c.st = h->d_st;
 	return c;
@@ -902,13 +942,11 @@ static SCGArgs scg_args(rc_handle *h, const CGArgs &c) {
 	a.x = h->d_v;
 	a.part = h->d_part;
 	a.s2 = h->d_scg;
-	a.tol = c.tol;
-	a.max_iter = c.max_iter;
+	a.cgb = h->d_cgb;
+	a.tile_box = h->d_cg_tile_box;
 	a.ncells = h->ncells;
-	a.nsites = (int) h->box_sites;
-	a.L = h->dboxes[0].L;
-	a.ntx = (a.L + SCG_TX - 1) / SCG_TX;
-	a.nsb = a.ntx * ((a.L + SCG_TY - 1) / SCG_TY);
+	a.ntiles = h->cg_ntiles;
+	a.nboxes = h->nboxes;
 	a.st = h->d_st;
 	return a;
 }
@@ -1019,22 +1057,26 @@ static int scg_iterations(rc_handle *h, const CGArgs &c, bool cold) {
 	{ TimeScope t(h, RC_T_SCG_ROW_INIT); HIPCHK(h, launch_scg(SCG_ROW_INIT, a, 0, h->stream)); }
 	{ TimeScope t(h, RC_T_SCG_REDUCE); HIPCHK(h, launch_scg(SCG_SETUP_REDUCE, a, 0, h->stream)); }
 	int batch = std::max(2, h->last_iters[cold ? 1 : 0] + 2), next = 4;
-	int j = 0;
+	int j = 0, cap = 0;
+	for(const SCGBox &g : h->cgb) cap = std::max(cap, g.max_iter);
 	for(;;) {
 		for(int it = 0; it < batch; ++it, ++j) {
 			TimeScope t(h, RC_T_SCG_ITER);
 			HIPCHK(h, launch_scg(SCG_ITER, a, j, h->stream));
 		}
 		{ TimeScope t(h, RC_T_SCG_REDUCE); HIPCHK(h, launch_scg(SCG_CHECK, a, j, h->stream)); }
-		HIPCHK(h, hipMemcpyAsync(h->h_scg, h->d_scg + (j & 1), sizeof(SCGState), hipMemcpyDeviceToHost, h->stream));
+		HIPCHK(h, hipMemcpyAsync(h->h_scg, h->d_scg, 2 * h->nboxes * sizeof(SCGState), hipMemcpyDeviceToHost, h->stream));
 		HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
 		HIPCHK(h, hipStreamSynchronize(h->stream));
 		if(h->h_st->flags) break;
-		if(!h->h_scg->active_now) break;
-		if(j >= c.max_iter + 2) break;
+		bool active = false;
+		for(int b = 0; b < h->nboxes; ++b) active = active || h->h_scg[2 * b + (j & 1)].active_now;
+		if(!active) break;
+		if(j >= cap + 2) break;
 		batch = next;
 		next = std::min(2 * next, 64);
 	}
+	h->scg_parity = j & 1;
 	{ TimeScope t(h, RC_T_SCG_FINAL); HIPCHK(h, launch_scg(SCG_FINAL, a, j, h->stream)); }
 	return RC_OK;
 }
@@ -1085,12 +1127,17 @@ static int cg_solve(rc_handle *h, bool cold) {
 	HIPCHK(h, hipEventRecord(e0, h->stream));
 	{
 		TimeScope ts(h, RC_T_CG_SOLVE);
-		if(cold) HIPCHK(h, hipMemsetAsync(h->d_v, 0, sizeof(double2) * h->rows, h->stream));
+		// solve() of Eigen (x0 = 0) in the boxes restarting cold, solveWithGuess() in the others
+		for(int b = 0; b < h->nboxes; ++b)
+			if(h->cold_box.empty() ? cold : h->cold_box[b])
+				HIPCHK(h, hipMemsetAsync(h->d_v + h->box_rows[b], 0, sizeof(double2) * (h->box_rows[b + 1] - h->box_rows[b]),
+						h->stream));
 		{ TimeScope t(h, RC_T_CG_DIAG); HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream)); }
 		const bool learned = h->warm_mode == RC_WARM_LEARNED && h->net;
 		// the row form's start (residual rows and their sums): the row
 		// iteration, the learned stages and the training records work on it
 		if(!sites || learned || h->d_rec) {
+			if(h->nboxes != 1) return fail(h, RC_ERR_ARG, "the row form of the CG runs one box per handle");
 			{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
 			{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
 			{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }
@@ -1148,28 +1195,36 @@ static int cg_solve(rc_handle *h, bool cold) {
 		int rc = record_end(h);
 		if(rc) return rc;
 	}
-	// iterations, final |r|^2 and |b|^2 of each component, from either form's state
-	int iters[2];
-	double rn2[2], bb[2];
-	for(int k = 0; k < 2; ++k) {
-		iters[k] = sites ? h->h_scg->iters[k] : h->h_cg->iters[k];
-		rn2[k] = sites ? h->h_scg->rn2_now[k] : h->h_cg->rn2[k];
-		bb[k] = sites ? h->h_scg->bb[k] : h->h_cg->bb[k];
+	// iterations, final |r|^2 and |b|^2 of each component of each box, from
+	// either form's state: sums over the boxes' solves, the largest of the
+	// last iteration counts and relative residuals
+	int last_it[2] = { 0, 0 }, sweeps = 0;
+	bool capped = false;
+	for(int k = 0; k < 2; ++k) h->cgs.last_relres[k] = 0.;
+	for(int b = 0; b < (sites ? h->nboxes : 1); ++b) {
+		const SCGState *S = sites ? h->h_scg + 2 * b + h->scg_parity : nullptr;
+		const int cap = sites ? S->max_iter : c.max_iter;
+		for(int k = 0; k < 2; ++k) {
+			const int it = sites ? S->iters[k] : h->h_cg->iters[k];
+			const double rn2 = sites ? S->rn2_now[k] : h->h_cg->rn2[k], bb = sites ? S->bb[k] : h->h_cg->bb[k];
+			const double rel = bb > 0 ? std::sqrt(rn2 / bb) : 0.;
+			h->cgs.solves += 1;
+			h->cgs.iters_total += it;
+			last_it[k] = std::max(last_it[k], it);
+			h->cgs.last_relres[k] = std::max(h->cgs.last_relres[k], rel);
+			h->cgs.worst_relres = std::max(h->cgs.worst_relres, rel);
+			capped = capped || it >= cap;
+		}
+		sweeps = std::max(sweeps, sites ? std::max(S->iters[0], S->iters[1]) : std::max(h->h_cg->iters[0], h->h_cg->iters[1]));
 	}
-	h->cgs.solves += 2;
-	h->cgs.iters_total += iters[0] + iters[1];
-	h->cgs.sweeps_total += std::max(iters[0], iters[1]);
-	h->cgs.last_iters[0] = iters[0];
-	h->cgs.last_iters[1] = iters[1];
-	for(int k = 0; k < 2; ++k) {
-		h->cgs.last_relres[k] = bb[k] > 0 ? std::sqrt(rn2[k] / bb[k]) : 0.;
-		h->cgs.worst_relres = std::max(h->cgs.worst_relres, h->cgs.last_relres[k]);
-	}
+	h->cgs.sweeps_total += sweeps;
+	h->cgs.last_iters[0] = last_it[0];
+	h->cgs.last_iters[1] = last_it[1];
 	h->cgs.last_cold = cold ? 1 : 0;
 	h->cgs.last_solve_ms = ms;
 	h->cgs.total_solve_ms += ms;
-	h->last_iters[cold ? 1 : 0] = std::max(iters[0], iters[1]);
-	if(iters[0] >= c.max_iter || iters[1] >= c.max_iter) h->cgs.last_cold |= 2;
+	h->last_iters[cold ? 1 : 0] = std::max(last_it[0], last_it[1]);
+	if(capped) h->cgs.last_cold |= 2;
 	return RC_OK;
 }
 
@@ -1233,18 +1288,32 @@ static int force_eval(rc_handle *h, double *phi, bool pure) {
 	}
 	// restart bookkeeping of WetPolarModel.cpp:151-157: a cold solve after 10
 	// warm ones, after 100 in WetModel (WetModel.cpp:196-201)
+	// (each box is a simulation of its own: its own rows and counter)
 	const int restart_after = h->nematic ? 100 : 10;
-	if(h->restart_solver < restart_after || h->warm_mode != RC_WARM_REFERENCE) h->size_rows_old = h->size_rows;
-	else {
-		h->size_rows_old = 0;
-		h->restart_solver = 0;
+	const int nb = h->nboxes;
+	if((int) h->restart_solver.size() != nb) {
+		h->restart_solver.assign(nb, 0);
+		h->size_rows.assign(nb, 0);
+		h->size_rows_old.assign(nb, 0);
 	}
-	h->size_rows = h->rows;
-	const bool cold = h->size_rows != h->size_rows_old;
+	h->cold_box.assign(nb, 0);
+	bool any_cold = false;
+	for(int b = 0; b < nb; ++b) {
+		if(h->restart_solver[b] < restart_after || h->warm_mode != RC_WARM_REFERENCE) h->size_rows_old[b] = h->size_rows[b];
+		else {
+			h->size_rows_old[b] = 0;
+			h->restart_solver[b] = 0;
+		}
+		h->size_rows[b] = h->box_rows[b + 1] - h->box_rows[b];
+		h->cold_box[b] = h->size_rows[b] != h->size_rows_old[b];
+		any_cold = any_cold || h->cold_box[b];
+	}
 	if(h->dparams[2 * 0 + h->cur_set].friction_cell != 0) {
-		if(cold) h->restart_solver = 0;
-		else h->restart_solver += 1;
-		if((rc = cg_solve(h, cold))) return rc;
+		for(int b = 0; b < nb; ++b) {
+			if(h->cold_box[b]) h->restart_solver[b] = 0;
+			else h->restart_solver[b] += 1;
+		}
+		if((rc = cg_solve(h, any_cold))) return rc;
 	}
 	h->cf_set = h->cur_set;
 	return RC_OK;
@@ -1270,9 +1339,9 @@ extern "C" int rc_init_forces(rc_handle *h) {
 	if(!h->uploaded) return fail(h, RC_ERR_STATE, "rc_init_forces before rc_upload_cells");
 	h->cur_set = 0;
 	h->cf_set = 0;
-	h->restart_solver = 0;
-	h->size_rows = 0;
-	h->size_rows_old = 0;
+	h->restart_solver.assign(h->nboxes, 0);
+	h->size_rows.assign(h->nboxes, 0);
+	h->size_rows_old.assign(h->nboxes, 0);
 	if(h->split) {
 		int rc = force_eval(h, h->d_phiA, false);
 		if(rc) return rc;
@@ -1290,6 +1359,7 @@ extern "C" int rc_apply_changes_after_equilibration(rc_handle *h) {
 extern "C" int rc_set_cg_form(rc_handle *h, int form) {
 	if(!h) return RC_ERR_ARG;
 	if(form != RC_CG_SITES && form != RC_CG_ROWS) return fail(h, RC_ERR_ARG, "unknown CG form %d", form);
+	if(form == RC_CG_ROWS && h->nboxes != 1) return fail(h, RC_ERR_ARG, "the row form of the CG runs one box per handle");
 	h->cg_form = form;
 	return RC_OK;
 }

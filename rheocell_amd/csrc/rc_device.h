@@ -96,6 +96,14 @@ struct CGState {
 // beta gamma / alpha_prev), A p carried as w + beta A p_prev): one reduction,
 // one pass per iteration.  Two copies, read from [j & 1] and written to
 // [(j + 1) & 1] by iteration j.
+// One box of the site-space CG: side, first box site, tiles (SCG_TX x SCG_TY,
+// ntx per tile row, the box's first tile in the handle's tile order), cells,
+// and the solver tolerance and iteration cap of the box's parameters.
+struct SCGBox {
+	int L, box_off, ntx, tile_off, ntiles, cell_begin, cell_end, max_iter;
+	double tol;
+};
+
 struct SCGState {
 	double bb[2], thr[2], rn2[2];
 	double rperp2[2], rzperp[2];   // |r_perp|^2, r_perp . (r_perp / D)

@@ -98,6 +98,7 @@ struct CGArgs {
 	double tol;
 	int max_iter;
 	int ncells;
+	int nboxes;                  // boxes of the handle (k_cg_diag; the other row-form kernels take box 0)
 	DevStatus *st;
 };
 
@@ -195,12 +196,15 @@ hipError_t launch_box_snapshot(const CGArgs &a, const double *phi2, float *snap,
 		int nsites, hipStream_t s);
 hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream_t s);
 
-// Site-space CG (SCGState): one wet box of nsites = L*L sites, box sums of b
-// and x0 in sb and sx (k_cg_gather<CG_GATHER_BX>), D in diag, rows per site in
-// cover.  Site passes run one workgroup per SCG_TX x SCG_TY tile of the box
-// (nsb tiles, ntx per tile row), one site per thread.  Iteration j reads the fields of parity j & 1 and writes those of
-// the other parity (its neighbours' old values stay readable), and the partial
-// sums likewise.
+// Site-space CG (SCGState): every wet box of the handle (cgb[box]), box sums
+// of b and x0 in sb and sx (k_cg_gather<CG_GATHER_BX>), D in diag, rows per
+// site in cover; the site fields are indexed like the box arrays (box_off +
+// site).  Site passes run one workgroup per SCG_TX x SCG_TY tile of a box
+// (ntiles over all boxes, box-major; tile_box[tile] = its box), one site per
+// thread.  Each box solves on its own: its own scalars (s2[2 box + parity]) and
+// partial sums (its tiles' range of psum).  Iteration j reads the fields of
+// parity j & 1 and writes those of the other parity (its neighbours' old
+// values stay readable), and the partial sums likewise.
 constexpr int SCG_TX = 64, SCG_TY = 8, SCG_THREADS = SCG_TX * SCG_TY;   // iteration tile: SCG_TX x SCG_TY sites
 struct SCGArgs {
 	const DevBox *boxes;
@@ -217,22 +221,21 @@ struct SCGArgs {
 	const double2 *b;            // right-hand side rows
 	double2 *x;                  // start rows in, solution rows out
 	double *part;                // row-pass partials, 6 per cell
-	double *psum[2];             // site partials y.m y, y.m (y / D), (y / D).m w: [6][nsb] each
-	SCGState *s2;                // [2]
-	double tol;
-	int max_iter;
-	int ncells, nsites;
-	int L, ntx, nsb;             // box side; tiles per tile row; tiles (workgroups of a site pass)
+	double *psum[2];             // site partials y.m y, y.m (y / D), (y / D).m w: [6][ntiles] each
+	SCGState *s2;                // [2 * nboxes]
+	const SCGBox *cgb;           // [nboxes]
+	const int *tile_box;         // [ntiles]
+	int ncells, ntiles, nboxes;
 	DevStatus *st;
 };
 // launches of the site-space CG, in solve order:
 //   SCG_INIT          per site: c, y = c + eta N4(sx), s = 0
 //   SCG_INIT_W        per site: the partials of parity 0 (with w = the site part of A z)
 //   SCG_ROW_INIT      per cell: |b|^2 and the ker-P residual sums over the rows
-//   SCG_SETUP_REDUCE  one workgroup: Eigen's start into SCGState [0]
+//   SCG_SETUP_REDUCE  one workgroup per box: Eigen's start into SCGState [2 box]
 //   SCG_ITER          iteration j: gamma, delta, the stop test, alpha, beta; p and A p
 //                     from it, x and r updated; the partials of the new r; SCGState [(j + 1) & 1]
-//   SCG_CHECK         the stop test of the state entering iteration j (after a batch)
+//   SCG_CHECK         one workgroup per box: the stop test of the state entering iteration j
 //   SCG_FINAL         x = x0 + xi z_perp + P^T s from the state entering iteration j
 enum SCGLaunch { SCG_INIT = 0, SCG_INIT_W, SCG_ROW_INIT, SCG_SETUP_REDUCE, SCG_ITER, SCG_CHECK, SCG_FINAL };
 hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s);

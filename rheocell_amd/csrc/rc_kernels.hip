@@ -1955,11 +1955,15 @@ hipError_t launch_nem_force(const NemArgs &a, int ncells, size_t lds, hipStream_
 // iterations.  All three sum in the same 256-thread order, so they agree bit
 // for bit.
 // ---------------------------------------------------------------------------
+// D(k) of every box site of every box (the site's box found by its offset)
 __global__ __launch_bounds__(256) void k_cg_diag(CGArgs a, int nsites) {
-	const int k = blockIdx.x * 256 + threadIdx.x;
-	if(k >= nsites) return;
-	const DevBox B = a.boxes[0];
-	const DevParams P = a.params[a.set];
+	const int kg = blockIdx.x * 256 + threadIdx.x;
+	if(kg >= nsites) return;
+	int b = 0;
+	while(b + 1 < a.nboxes && a.boxes[b + 1].box_off <= kg) ++b;
+	const DevBox B = a.boxes[b];
+	const DevParams P = a.params[2 * b + a.set];
+	const int k = kg - B.box_off;
 	const int L = B.L, y = k / L, x = k - y * L;
 	const int xe = x + 1 == L ? 0 : x + 1, xw = x == 0 ? L - 1 : x - 1;
 	const int yn = y + 1 == L ? 0 : y + 1, ys = y == 0 ? L - 1 : y - 1;
@@ -2731,9 +2735,10 @@ hipError_t launch_zero_rows(double2 *x, long long rows, int mask, hipStream_t s)
 // Site-space CG (SCGState in rc_device.h): Eigen's Jacobi-PCG iterates of the
 // wet solve carried as scalars times the ker-P parts r_perp, z_perp = r_perp /
 // D of the first residual plus L x L fields, in Chronopoulos and Gear's
-// arrangement (one reduction per iteration).  The iterates are the
-// reference's in exact arithmetic; an iteration is one pass over the box
-// sites instead of passes over every row.
+// arrangement (one reduction per iteration), for every wet box of a handle at
+// once (each box with its own scalars).  The iterates are the reference's in
+// exact arithmetic; an iteration is one pass over the box sites instead of
+// passes over every row.
 // ---------------------------------------------------------------------------
 
 // the four box neighbours of site k, in the triplet order of
@@ -2748,15 +2753,6 @@ __device__ inline void site_nbrs(int k, int L, int *nb) {
 	nb[3] = yn * L + x;
 }
 
-// site of this thread (SCG_THREADS per workgroup, XCD-contiguous blocks) and
-// whether it is a box site (threads past the last site take the last one and
-// write nothing)
-__device__ inline int scg_site(const SCGArgs &a, bool &in) {
-	const int k = xcd_item(blockIdx.x, gridDim.x) * SCG_THREADS + threadIdx.x;
-	in = k < a.nsites;
-	return in ? k : a.nsites - 1;
-}
-
 // the partial sums of one site: y.m y, y.m (y / D), (y / D).m w (x, y components)
 __device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, double2 w) {
 	const double fm = m, inv = 1. / D;
@@ -2768,29 +2764,68 @@ __device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, d
 	part[4] = fm * (zx * w.x);
 	part[5] = fm * (zy * w.y);
 }
-
-// The sums of the previous site pass's partials (6 values per tile), split so
-// that a launch requests this thread's first partials with its other loads
-// and tests the status word before using any: scg_parts_load, then
-// scg_parts_sum, in the order of reduce_parts<6, SCG_THREADS, 1>.
-__device__ inline void scg_parts_load(const double *part, int nblk, double (&v)[6]) {
-	const int b = min((int) threadIdx.x, nblk - 1);
+// a tile's partials (valid in thread 0) -> part[j * ntiles + tile]
+__device__ inline void scg_partials(const double (&v)[6], double *part, int ntiles, int tile) {
+	if(threadIdx.x == 0) {
 #pragma unroll
-	for(int j = 0; j < 6; ++j) v[j] = part[j * nblk + b];
+		for(int j = 0; j < 6; ++j) part[j * ntiles + tile] = v[j];
+	}
 }
-__device__ inline void scg_parts_sum(const double *part, int nblk, const double (&v)[6], double (&tot)[6],
-		double *s_red) {
+
+// The sums of one box's partials (6 values per item, items begin .. begin +
+// count - 1 of a [6][stride] array) over NT threads in one fixed order, split
+// so that a launch requests this thread's first partials with its other loads
+// and tests the status word before using any: parts_load, then parts_sum.
+template <int NT>
+__device__ inline void parts_load(const double *part, int stride, int begin, int count, double (&v)[6]) {
+	const int b = begin + min((int) threadIdx.x, count - 1);
+#pragma unroll
+	for(int j = 0; j < 6; ++j) v[j] = part[j * stride + b];
+}
+template <int NT>
+__device__ inline void parts_sum(const double *part, int stride, int begin, int count, const double (&v)[6],
+		double (&tot)[6], double *s_red) {
 	double acc[6];
-	const bool own = (int) threadIdx.x < nblk;
+	const bool own = (int) threadIdx.x < count;
 #pragma unroll
 	for(int j = 0; j < 6; ++j) {
 		acc[j] = 0.;
 		acc[j] += own ? v[j] : 0.;
 	}
-	for(int b = threadIdx.x + SCG_THREADS; b < nblk; b += SCG_THREADS)
+	for(int b = threadIdx.x + NT; b < count; b += NT)
 #pragma unroll
-		for(int j = 0; j < 6; ++j) acc[j] += part[j * nblk + b];
-	reduce_tail<6, SCG_THREADS>(acc, tot, s_red);
+		for(int j = 0; j < 6; ++j) acc[j] += part[j * stride + begin + b];
+	reduce_tail<6, NT>(acc, tot, s_red);
+}
+
+// The tile of a site pass: workgroup b takes tile xcd_item(b) of the handle's
+// tiles (box-major), a tile of SCG_TX x SCG_TY sites of its box.  Slots at tile
+// coordinates (hx, hy) stand for the box site ((x0 + hx) mod L, (y0 + hy) mod
+// L), so a stencil over the slots is the periodic stencil also in a tile the
+// box edge cuts (slots past the edge hold wrapped sites and are not written).
+// Sites are returned as box-array indices (box_off + site).
+struct SCGTileBox {
+	int tile, box, o, L, x0, y0;
+	bool lead;    // the box's first tile (its workgroup writes the box's state)
+};
+__device__ inline SCGTileBox scg_tile_box(const SCGArgs &a) {
+	SCGTileBox t;
+	t.tile = xcd_item(blockIdx.x, gridDim.x);
+	t.box = a.tile_box[t.tile];
+	const SCGBox Bx = a.cgb[t.box];
+	const int lt = t.tile - Bx.tile_off;
+	t.o = Bx.box_off;
+	t.L = Bx.L;
+	t.x0 = (lt % Bx.ntx) * SCG_TX;
+	t.y0 = (lt / Bx.ntx) * SCG_TY;
+	t.lead = lt == 0;
+	return t;
+}
+__device__ inline int scg_slot_site(const SCGTileBox &T, int hx, int hy) {
+	int gx = T.x0 + hx, gy = T.y0 + hy;
+	gx = gx < 0 ? gx + T.L : gx % T.L;
+	gy = gy < 0 ? gy + T.L : gy % T.L;
+	return T.o + gy * T.L + gx;
 }
 
 // Set-up, per site: site means b_bar = P b / m, x_bar = P x0 / m;
@@ -2798,20 +2833,19 @@ __device__ inline void scg_parts_sum(const double *part, int nblk, const double 
 // first residual r0 = b - A x0; s = 0.
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
-	const DevBox B = a.boxes[0];
-	const int o = B.box_off;
-	const double eta = a.params[a.set].friction_cell;
-	bool in;
-	const int k = scg_site(a, in);
-	if(!in) return;
-	const int m = a.cover[o + k];
+	const SCGTileBox T = scg_tile_box(a);
+	const int lx = threadIdx.x % SCG_TX, ly = threadIdx.x / SCG_TX;
+	if(T.x0 + lx >= T.L || T.y0 + ly >= T.L) return;
+	const int kl = (T.y0 + ly) * T.L + T.x0 + lx, k = T.o + kl;
+	const double eta = a.params[2 * T.box + a.set].friction_cell;
+	const int m = a.cover[k];
 	double2 c = make_double2(0., 0.), y = make_double2(0., 0.);
 	if(m > 0) {
 		int nb[4];
-		site_nbrs(k, B.L, nb);
-		const double D = a.diag[o + k], fm = (double) m;
-		const double2 sb = a.sb[o + k], sx = a.sx[o + k];
-		const double2 gS = a.sx[o + nb[0]], gW = a.sx[o + nb[1]], gE = a.sx[o + nb[2]], gN = a.sx[o + nb[3]];
+		site_nbrs(kl, T.L, nb);
+		const double D = a.diag[k], fm = (double) m;
+		const double2 sb = a.sb[k], sx = a.sx[k];
+		const double2 gS = a.sx[T.o + nb[0]], gW = a.sx[T.o + nb[1]], gE = a.sx[T.o + nb[2]], gN = a.sx[T.o + nb[3]];
 		c.x = sb.x / fm - D * (sx.x / fm);
 		c.y = sb.y / fm - D * (sx.y / fm);
 		y.x = c.x + eta * (((gS.x + gW.x) + gE.x) + gN.x);
@@ -2824,32 +2858,21 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	a.s[k] = zero;
 }
 
-// The tile of a site pass: workgroup b takes tile xcd_item(b) of SCG_TX x SCG_TY
-// sites.  Every slot of the tile and of the one-site ring around it, at tile
-// coordinates (hx, hy) in -1 .. SCG_TX x -1 .. SCG_TY, stands for the box site
-// ((x0 + hx) mod L, (y0 + hy) mod L), so a stencil over the slots is the
-// periodic stencil also in a tile the box edge cuts (slots past the edge hold
-// wrapped sites and are not written).  Thread t owns slot (t mod SCG_TX,
-// t / SCG_TX); threads 0 .. SCG_RING - 1 also take one ring slot.
+// The one-site ring of a tile (init_w): slots at (hx, hy) in -1 .. SCG_TX x
+// -1 .. SCG_TY, LDS index (hy + 1) SCG_HW + hx + 1.  Thread t owns slot
+// (t mod SCG_TX, t / SCG_TX); threads 0 .. SCG_RING - 1 also take one ring slot.
 constexpr int SCG_HW = SCG_TX + 2, SCG_RING = 2 * SCG_HW + 2 * SCG_TY;
 struct SCGTile {
-	int x0, y0;
 	int k;        // own slot's site
 	bool in;      // the own slot is a box site of this tile
 	int slot;     // own slot's LDS index
 	int hk, hslot;   // ring slot's site and LDS index (hslot < 0: none)
 };
-__device__ inline SCGTile scg_tile(const SCGArgs &a) {
+__device__ inline SCGTile scg_tile(const SCGTileBox &T) {
 	SCGTile t;
-	const int tile = xcd_item(blockIdx.x, gridDim.x), L = a.L;
-	t.x0 = (tile % a.ntx) * SCG_TX;
-	t.y0 = (tile / a.ntx) * SCG_TY;
 	const int lx = threadIdx.x % SCG_TX, ly = threadIdx.x / SCG_TX;
-	int gx = t.x0 + lx, gy = t.y0 + ly;
-	t.in = gx < L && gy < L;
-	gx %= L;
-	gy %= L;
-	t.k = gy * L + gx;
+	t.in = T.x0 + lx < T.L && T.y0 + ly < T.L;
+	t.k = scg_slot_site(T, lx, ly);
 	t.slot = (ly + 1) * SCG_HW + lx + 1;
 	t.hslot = -1;
 	t.hk = 0;
@@ -2860,10 +2883,7 @@ __device__ inline SCGTile scg_tile(const SCGArgs &a) {
 		else if(i < 2 * SCG_HW) { hx = i - SCG_HW - 1; hy = SCG_TY; }
 		else if(i < 2 * SCG_HW + SCG_TY) { hx = -1; hy = i - 2 * SCG_HW; }
 		else { hx = SCG_TX; hy = i - 2 * SCG_HW - SCG_TY; }
-		int hxg = t.x0 + hx, hyg = t.y0 + hy;
-		hxg = hxg < 0 ? hxg + L : hxg % L;
-		hyg = hyg < 0 ? hyg + L : hyg % L;
-		t.hk = hyg * L + hxg;
+		t.hk = scg_slot_site(T, hx, hy);
 		t.hslot = (hy + 1) * SCG_HW + hx + 1;
 	}
 	return t;
@@ -2890,25 +2910,25 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
 	__shared__ double s_red[(SCG_THREADS / 64) * 6];
 	__shared__ double2 s_mz[SCG_HW * (SCG_TY + 2)];
-	const int o = a.boxes[0].box_off;
-	const double eta = a.params[a.set].friction_cell;
-	const SCGTile t = scg_tile(a);
-	const int m = a.cover[o + t.k];
-	const double D = a.diag[o + t.k];
+	const SCGTileBox T = scg_tile_box(a);
+	const double eta = a.params[2 * T.box + a.set].friction_cell;
+	const SCGTile t = scg_tile(T);
+	const int m = a.cover[t.k];
+	const double D = a.diag[t.k];
 	const double2 y = a.y[0][t.k];
 	const double2 z = scg_z(y, D, m);
 	if(flags) return;
 	s_mz[t.slot] = scg_mz(z, m);
 	if(t.hslot >= 0) {
-		const int hm = a.cover[o + t.hk];
-		s_mz[t.hslot] = scg_mz(scg_z(a.y[0][t.hk], a.diag[o + t.hk], hm), hm);
+		const int hm = a.cover[t.hk];
+		s_mz[t.hslot] = scg_mz(scg_z(a.y[0][t.hk], a.diag[t.hk], hm), hm);
 	}
 	__syncthreads();
 	const double2 w = scg_w_lds(s_mz, t.slot, D, z, m, eta);
 	double part[6] = { 0., 0., 0., 0., 0., 0. };
 	if(t.in) scg_part(part, m, D, y, w);
 	block_sum<6, SCG_THREADS / 64>(part, s_red);
-	block_partials<6>(part, a.psum[0]);
+	scg_partials(part, a.psum[0], a.ntiles, T.tile);
 }
 
 // box site of storage row q of cell C (MultiPhaseField ring buffer, SquareBox wrap)
@@ -2927,7 +2947,7 @@ __device__ inline int cell_site(const DevCell &C, int L, int q) {
 
 // Set-up, per cell: the ker-P part of the first residual, r_perp_i = (b_i -
 // D x0_i) - c(k_i), summed as |b|^2, |r_perp|^2, r_perp.(r_perp / D) -> part
-// (6 per cell: b.b x, y; r_perp.r_perp x, y; r_perp.z_perp x, y)
+// (6 per cell, [6][ncells]: b.b x, y; r_perp.r_perp x, y; r_perp.z_perp x, y)
 __global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the partials are written, not ahead of the loads
 	__shared__ double s_red[4 * 6];
@@ -2936,8 +2956,8 @@ __global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
 	const DevBox B = a.boxes[C.box];
 	double red[6] = { 0., 0., 0., 0., 0., 0. };
 	for(int q = threadIdx.x; q < C.S; q += 256) {
-		const int k = cell_site(C, B.L, q), row = C.start + q;
-		const double D = a.diag[B.box_off + k], inv = 1. / D;
+		const int k = B.box_off + cell_site(C, B.L, q), row = C.start + q;
+		const double D = a.diag[k], inv = 1. / D;
 		const double2 c = a.c[k], bb = a.b[row], x0 = a.x[row];
 		const double rx = (bb.x - D * x0.x) - c.x, ry = (bb.y - D * x0.y) - c.y;
 		red[0] += bb.x * bb.x; red[1] += bb.y * bb.y;
@@ -2946,22 +2966,30 @@ __global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
 	}
 	if(flags) return;
 	block_sum<6>(red, s_red);
-	block_partials<6>(red, a.part);
+	// indexed by cell, so that each box's cells are one range
+	if(threadIdx.x == 0) {
+#pragma unroll
+		for(int j = 0; j < 6; ++j) a.part[j * a.ncells + ci] = red[j];
+	}
 }
 
-// Set-up, one workgroup: Eigen's start (ConjugateGradient.h: |b|^2, the
+// Set-up, one workgroup per box: Eigen's start (ConjugateGradient.h: |b|^2, the
 // threshold max(tol^2 |b|^2, DBL_MIN), the first residual's |r|^2 and its test)
-// into SCGState [0], with rho = 1, pi = xi = 0
+// into SCGState [2 box], with rho = 1, pi = xi = 0; the box's cells' row sums
+// and its tiles' site sums
 __global__ __launch_bounds__(1024) void k_scg_setup_reduce(SCGArgs a) {
 	__shared__ double s_red[16 * 6];
 	const unsigned flags = a.st->flags;
-	double t6[6], ts[6];
-	reduce_parts<6>(a.part, a.ncells, t6, s_red);
-	reduce_parts<6>(a.psum[0], a.nsb, ts, s_red);
+	const int box = blockIdx.x;
+	const SCGBox Bx = a.cgb[box];
+	double t6[6], ts[6], v[6];
+	parts_load<1024>(a.part, a.ncells, Bx.cell_begin, Bx.cell_end - Bx.cell_begin, v);
+	parts_sum<1024>(a.part, a.ncells, Bx.cell_begin, Bx.cell_end - Bx.cell_begin, v, t6, s_red);
+	parts_load<1024>(a.psum[0], a.ntiles, Bx.tile_off, Bx.ntiles, v);
+	parts_sum<1024>(a.psum[0], a.ntiles, Bx.tile_off, Bx.ntiles, v, ts, s_red);
 	if(flags || threadIdx.x != 0) return;
 	SCGState S{};
-	S.max_iter = a.max_iter;
-#pragma unroll
+	S.max_iter = Bx.max_iter;
 	for(int c = 0; c < 2; ++c) {
 		const double bb = t6[c], rp2 = t6[2 + c], rzp = t6[4 + c];
 		const double rn2 = rp2 + ts[c];
@@ -2979,20 +3007,19 @@ __global__ __launch_bounds__(1024) void k_scg_setup_reduce(SCGArgs a) {
 			S.done[c] = 1;
 			continue;
 		}
-		double thr = a.tol * a.tol * bb;
+		double thr = Bx.tol * Bx.tol * bb;
 		if(thr < DBL_MIN) thr = DBL_MIN;
 		S.thr[c] = thr;
 		S.rn2[c] = rn2;
 		if(rn2 < thr) S.done[c] = 1;
 	}
 	S.active = !(S.done[0] && S.done[1]);
-#pragma unroll
 	for(int c = 0; c < 2; ++c) {
 		S.rn2_now[c] = S.rn2[c];
 		S.done_now[c] = S.done[c];
 	}
 	S.active_now = S.active;
-	a.s2[0] = S;
+	a.s2[2 * box] = S;
 }
 
 struct SCGCoef {
@@ -3040,11 +3067,10 @@ __device__ inline SCGCoef scg_coef(const SCGState &S, const double (&tot)[6], bo
 }
 
 // The iteration's tile with a two-site ring: slots at tile coordinates (hx, hy)
-// in -2 .. SCG_TX + 1 x -2 .. SCG_TY + 1, grid index (hy + 2) SCG_G2W + hx + 2,
-// each standing for the box site ((x0 + hx) mod L, (y0 + hy) mod L).  Thread t
-// owns slot (t mod SCG_TX, t / SCG_TX) and threads 0 .. SCG_RING2 - 1 one ring
-// slot each: the two rows below, the two above, then the two columns on
-// either side, row by row.
+// in -2 .. SCG_TX + 1 x -2 .. SCG_TY + 1, grid index (hy + 2) SCG_G2W + hx + 2.
+// Thread t owns slot (t mod SCG_TX, t / SCG_TX) and threads 0 .. SCG_RING2 - 1
+// one ring slot each: the two rows below, the two above, then the two columns
+// on either side, row by row.
 constexpr int SCG_G2W = SCG_TX + 4, SCG_G2H = SCG_TY + 4, SCG_RING2 = SCG_G2W * SCG_G2H - SCG_THREADS;
 static_assert(SCG_RING2 <= SCG_THREADS, "one ring slot per thread");
 struct SCGTile2 {
@@ -3054,16 +3080,11 @@ struct SCGTile2 {
 	int hk, hg;    // ring slot's site and grid index (hg < 0: none)
 	bool hin;      // the ring slot lies in the one-site ring
 };
-__device__ inline SCGTile2 scg_tile2(const SCGArgs &a) {
+__device__ inline SCGTile2 scg_tile2(const SCGTileBox &T) {
 	SCGTile2 t;
-	const int tile = xcd_item(blockIdx.x, gridDim.x), L = a.L;
-	const int x0 = (tile % a.ntx) * SCG_TX, y0 = (tile / a.ntx) * SCG_TY;
 	const int lx = threadIdx.x % SCG_TX, ly = threadIdx.x / SCG_TX;
-	int gx = x0 + lx, gy = y0 + ly;
-	t.in = gx < L && gy < L;
-	gx %= L;
-	gy %= L;
-	t.k = gy * L + gx;
+	t.in = T.x0 + lx < T.L && T.y0 + ly < T.L;
+	t.k = scg_slot_site(T, lx, ly);
 	t.g = (ly + 2) * SCG_G2W + lx + 2;
 	t.hg = -1;
 	t.hk = 0;
@@ -3085,10 +3106,7 @@ __device__ inline SCGTile2 scg_tile2(const SCGArgs &a) {
 			hy = r >> 2;
 			hx = c < 2 ? c - 2 : SCG_TX + c - 2;
 		}
-		int hxg = x0 + hx, hyg = y0 + hy;
-		hxg = hxg < 0 ? hxg + L : hxg % L;
-		hyg = hyg < 0 ? hyg + L : hyg % L;
-		t.hk = hyg * L + hxg;
+		t.hk = scg_slot_site(T, hx, hy);
 		t.hg = (hy + 2) * SCG_G2W + hx + 2;
 		t.hin = hx >= -1 && hx <= SCG_TX && hy >= -1 && hy <= SCG_TY;
 	}
@@ -3101,10 +3119,10 @@ struct SCGSite {
 	double D;
 	int m;
 };
-__device__ inline SCGSite scg_load(const SCGArgs &a, int o, int pin, int k, bool first) {
+__device__ inline SCGSite scg_load(const SCGArgs &a, int pin, int k, bool first) {
 	SCGSite v;
-	v.m = a.cover[o + k];
-	v.D = a.diag[o + k];
+	v.m = a.cover[k];
+	v.D = a.diag[k];
 	v.y = a.y[pin][k];
 	v.q = first ? make_double2(0., 0.) : a.q[pin][k];
 	return v;
@@ -3118,38 +3136,40 @@ __device__ inline double2 scg_apply(const double2 *s_mv, int g, double D, double
 	return make_double2(D * v.x - eta * (((mS.x + mW.x) + mE.x) + mN.x), D * v.y - eta * (((mS.y + mW.y) + mE.y) + mN.y));
 }
 
-// Iteration j (one launch, one workgroup per tile): the scalars (every
-// workgroup, the same fixed order of the partials), then
+// Iteration j (one launch, one workgroup per tile of every box): the box's
+// scalars (every workgroup of the box, the same fixed order of its tiles'
+// partials), then
 //   q = z + beta q (p),  A p from q over the one-site ring,  s += alpha q,
 //   y -= alpha A p,  z = y / D,  w = A z (for the sums only),
 // q over the tile and its two-site ring, the new y over the one-site ring,
 // each exchanged through LDS as m q and m z.  Only y, q (both parities) and s
 // are stored: A p and w are formed from them, so an iteration reads and
-// writes 108 B per box site.  Workgroup 0 writes SCGState [(j + 1) & 1]:
-// rho -= alpha pi, xi += alpha pi and the recurrence scalars (nobody reads that
-// copy during iteration j).
+// writes 108 B per box site.  The workgroup of the box's first tile writes
+// SCGState [2 box + ((j + 1) & 1)]: rho -= alpha pi, xi += alpha pi and the
+// recurrence scalars (nobody reads that copy during iteration j).
 template <bool FIRST>
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	const int pin = j & 1, pout = pin ^ 1;
-	const SCGState *Sp = a.s2 + pin;
-	SCGState *Sn = a.s2 + pout;
-	const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
 	__shared__ double s_red[(SCG_THREADS / 64) * 6];
 	__shared__ double2 s_mq[SCG_G2W * SCG_G2H];
 	__shared__ double2 s_mz[SCG_G2W * SCG_G2H];
-	const int o = a.boxes[0].box_off;
-	const double eta = a.params[a.set].friction_cell;
-	const SCGTile2 t = scg_tile2(a);
+	const SCGTileBox T = scg_tile_box(a);
+	const SCGBox Bx = a.cgb[T.box];
+	const SCGState *Sp = a.s2 + 2 * T.box + pin;
+	SCGState *Sn = a.s2 + 2 * T.box + pout;
+	const bool writer = T.lead && threadIdx.x == 0;
+	const double eta = a.params[2 * T.box + a.set].friction_cell;
+	const SCGTile2 t = scg_tile2(T);
 	// Every load is requested before any is used: the site data, this
 	// thread's partials, the status word and the state (all addresses are
 	// valid whatever they hold), so the launch waits for one memory round trip,
 	// not three, and a launch after convergence returns after that one.
-	const SCGSite c0 = scg_load(a, o, pin, t.k, FIRST);
+	const SCGSite c0 = scg_load(a, pin, t.k, FIRST);
 	SCGSite ch{};
-	if(t.hg >= 0) ch = scg_load(a, o, pin, t.hk, FIRST);
+	if(t.hg >= 0) ch = scg_load(a, pin, t.hk, FIRST);
 	double2 s = a.s[t.k];
 	double pre[6];
-	scg_parts_load(a.psum[pin], gridDim.x, pre);
+	parts_load<SCG_THREADS>(a.psum[pin], a.ntiles, Bx.tile_off, Bx.ntiles, pre);
 	const unsigned flags = a.st->flags;
 	const int active = Sp->active;
 	if(flags) return;
@@ -3158,7 +3178,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 		return;
 	}
 	double tot[6];
-	scg_parts_sum(a.psum[pin], gridDim.x, pre, tot, s_red);
+	parts_sum<SCG_THREADS>(a.psum[pin], a.ntiles, Bx.tile_off, Bx.ntiles, pre, tot, s_red);
 	const SCGCoef cf = scg_coef(*Sp, tot, FIRST);
 	if(cf.done[0] && cf.done[1]) {
 		if(writer) {
@@ -3215,7 +3235,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 		scg_part(part, c0.m, c0.D, yn0, wn);
 	}
 	block_sum<6, SCG_THREADS / 64>(part, s_red);
-	block_partials<6>(part, a.psum[pout]);
+	scg_partials(part, a.psum[pout], a.ntiles, T.tile);
 	if(writer) {
 		*Sn = *Sp;
 #pragma unroll
@@ -3238,22 +3258,24 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	}
 }
 
-// The stop test of the state entering iteration j (after a batch), written
-// into SCGState [j & 1] as rn2_now, done_now, active_now for the host
+// The stop test of each box's state entering iteration j (after a batch),
+// one workgroup per box, written into SCGState [2 box + (j & 1)] as rn2_now,
+// done_now, active_now for the host; the sums in the order of the iteration
+// that would use them
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_check(SCGArgs a, int j) {
 	__shared__ double s_red[(SCG_THREADS / 64) * 6];
-	SCGState *S = a.s2 + (j & 1);
+	const int box = blockIdx.x;
+	const SCGBox Bx = a.cgb[box];
+	SCGState *S = a.s2 + 2 * box + (j & 1);
 	const int active = S->active;
 	double tot[6] = { 0., 0., 0., 0., 0., 0. };
 	if(active && j > 0) {
-		// the sums in the order of the iteration that would use them
 		double pre[6];
-		scg_parts_load(a.psum[j & 1], a.nsb, pre);
-		scg_parts_sum(a.psum[j & 1], a.nsb, pre, tot, s_red);
+		parts_load<SCG_THREADS>(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre);
+		parts_sum<SCG_THREADS>(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre, tot, s_red);
 	}
 	if(threadIdx.x != 0) return;
 	if(!active || j == 0) {
-#pragma unroll
 		for(int c = 0; c < 2; ++c) {
 			S->rn2_now[c] = S->rn2[c];
 			S->done_now[c] = S->done[c];
@@ -3262,7 +3284,6 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_check(SCGArgs a, int j) {
 		return;
 	}
 	const SCGCoef cf = scg_coef(*S, tot, false);
-#pragma unroll
 	for(int c = 0; c < 2; ++c) {
 		S->rn2_now[c] = cf.rn2[c];
 		S->done_now[c] = cf.done[c];
@@ -3271,19 +3292,20 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_check(SCGArgs a, int j) {
 	S->active_now = !(cf.done[0] && cf.done[1]);
 }
 
-// x_i = x0_i + xi (r_perp_i / D) + s(k_i), r_perp as in k_scg_row_init; a
-// component with a zero right-hand side is 0 (Eigen's solve of b = 0)
+// x_i = x0_i + xi (r_perp_i / D) + s(k_i), r_perp as in k_scg_row_init, with
+// the scalars of the cell's box; a component with a zero right-hand side is 0
+// (Eigen's solve of b = 0)
 __global__ __launch_bounds__(256) void k_scg_final(SCGArgs a, int j) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
-	const SCGState *S = a.s2 + (j & 1);
-	const double xi0 = S->xi[0], xi1 = S->xi[1];
-	const bool z0 = S->zero_rhs[0], z1 = S->zero_rhs[1];
 	const int ci = xcd_item(blockIdx.x, gridDim.x);
 	const DevCell C = a.cells[ci];
 	const DevBox B = a.boxes[C.box];
+	const SCGState *S = a.s2 + 2 * C.box + (j & 1);
+	const double xi0 = S->xi[0], xi1 = S->xi[1];
+	const bool z0 = S->zero_rhs[0], z1 = S->zero_rhs[1];
 	for(int q = threadIdx.x; q < C.S; q += 256) {
-		const int k = cell_site(C, B.L, q), row = C.start + q;
-		const double D = a.diag[B.box_off + k], inv = 1. / D;
+		const int k = B.box_off + cell_site(C, B.L, q), row = C.start + q;
+		const double D = a.diag[k], inv = 1. / D;
 		const double2 c = a.c[k], sv = a.s[k], bb = a.b[row], x0 = a.x[row];
 		const double rx = (bb.x - D * x0.x) - c.x, ry = (bb.y - D * x0.y) - c.y;
 		double2 x;
@@ -3296,15 +3318,15 @@ __global__ __launch_bounds__(256) void k_scg_final(SCGArgs a, int j) {
 
 hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s) {
 	switch(which) {
-	case SCG_INIT: RC_LAUNCH(k_scg_init, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a); break;
-	case SCG_INIT_W: RC_LAUNCH(k_scg_init_w, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a); break;
+	case SCG_INIT: RC_LAUNCH(k_scg_init, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
+	case SCG_INIT_W: RC_LAUNCH(k_scg_init_w, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
 	case SCG_ROW_INIT: RC_LAUNCH(k_scg_row_init, dim3(a.ncells), dim3(256), 0, s, a); break;
-	case SCG_SETUP_REDUCE: RC_LAUNCH(k_scg_setup_reduce, dim3(1), dim3(1024), 0, s, a); break;
+	case SCG_SETUP_REDUCE: RC_LAUNCH(k_scg_setup_reduce, dim3(a.nboxes), dim3(1024), 0, s, a); break;
 	case SCG_ITER:
-		if(j == 0) RC_LAUNCH(k_scg_iter<true>, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a, j);
-		else RC_LAUNCH(k_scg_iter<false>, dim3(a.nsb), dim3(SCG_THREADS), 0, s, a, j);
+		if(j == 0) RC_LAUNCH(k_scg_iter<true>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+		else RC_LAUNCH(k_scg_iter<false>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
 		break;
-	case SCG_CHECK: RC_LAUNCH(k_scg_check, dim3(1), dim3(SCG_THREADS), 0, s, a, j); break;
+	case SCG_CHECK: RC_LAUNCH(k_scg_check, dim3(a.nboxes), dim3(SCG_THREADS), 0, s, a, j); break;
 	case SCG_FINAL: RC_LAUNCH(k_scg_final, dim3(a.ncells), dim3(256), 0, s, a, j); break;
 	default: return hipErrorInvalidValue;
 	}
