@@ -1053,9 +1053,8 @@ static int scg_iterations(rc_handle *h, const CGArgs &c, bool cold) {
 	const SCGArgs a = scg_args(h, c);
 	{ TimeScope t(h, RC_T_SCG_GATHER); HIPCHK(h, launch_cg_gather(CG_GATHER_BX, c, h->ntiles, h->stream)); }
 	{ TimeScope t(h, RC_T_SCG_INIT); HIPCHK(h, launch_scg(SCG_INIT, a, 0, h->stream)); }
-	{ TimeScope t(h, RC_T_SCG_INIT_W); HIPCHK(h, launch_scg(SCG_INIT_W, a, 0, h->stream)); }
 	{ TimeScope t(h, RC_T_SCG_ROW_INIT); HIPCHK(h, launch_scg(SCG_ROW_INIT, a, 0, h->stream)); }
-	{ TimeScope t(h, RC_T_SCG_REDUCE); HIPCHK(h, launch_scg(SCG_SETUP_REDUCE, a, 0, h->stream)); }
+	{ TimeScope t(h, RC_T_SCG_INIT_W); HIPCHK(h, launch_scg(SCG_INIT_W, a, 0, h->stream)); }
 	int batch = std::max(2, h->last_iters[cold ? 1 : 0] + 2), next = 4;
 	int j = 0, cap = 0;
 	for(const SCGBox &g : h->cgb) cap = std::max(cap, g.max_iter);
@@ -1132,12 +1131,13 @@ static int cg_solve(rc_handle *h, bool cold) {
 			if(h->cold_box.empty() ? cold : h->cold_box[b])
 				HIPCHK(h, hipMemsetAsync(h->d_v + h->box_rows[b], 0, sizeof(double2) * (h->box_rows[b + 1] - h->box_rows[b]),
 						h->stream));
-		{ TimeScope t(h, RC_T_CG_DIAG); HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream)); }
 		const bool learned = h->warm_mode == RC_WARM_LEARNED && h->net;
 		// the row form's start (residual rows and their sums): the row
 		// iteration, the learned stages and the training records work on it
+		// (the site form's set-up computes D itself)
 		if(!sites || learned || h->d_rec) {
 			if(h->nboxes != 1) return fail(h, RC_ERR_ARG, "the row form of the CG runs one box per handle");
+			{ TimeScope t(h, RC_T_CG_DIAG); HIPCHK(h, launch_cg_diag(c, (int) h->box_sites, h->stream)); }
 			{ TimeScope t(h, RC_T_CG_GATHER_X); HIPCHK(h, launch_cg_gather(CG_GATHER_X, c, h->ntiles, h->stream)); }
 			{ TimeScope t(h, RC_T_CG_BOX_N4); HIPCHK(h, launch_cg_box(0, c, (int) h->box_sites, h->stream)); }
 			{ TimeScope t(h, RC_T_CG_ROW_RESID); HIPCHK(h, launch_cg_row(CG_ROW_RESID, c, h->stream)); }

@@ -212,7 +212,7 @@ struct SCGArgs {
 	const DevParams *params;
 	int set;
 	const int *cover;
-	const double *diag;
+	double *diag;                // D(k), written by SCG_INIT
 	const double2 *sb, *sx;      // box sums of b and of x0
 	double2 *c;                  // b_bar - D x0_bar per site (site means of b, x0)
 	double2 *s;                  // site part of x - x0
@@ -229,15 +229,15 @@ struct SCGArgs {
 	DevStatus *st;
 };
 // launches of the site-space CG, in solve order:
-//   SCG_INIT          per site: c, y = c + eta N4(sx), s = 0
-//   SCG_INIT_W        per site: the partials of parity 0 (with w = the site part of A z)
+//   SCG_INIT          per site: D, c, y = c + eta N4(sx), s = 0
 //   SCG_ROW_INIT      per cell: |b|^2 and the ker-P residual sums over the rows
-//   SCG_SETUP_REDUCE  one workgroup per box: Eigen's start into SCGState [2 box]
+//   SCG_INIT_W        per site: the partials of parity 0 (with w = the site part of A z);
+//                     per box: Eigen's start into SCGState [2 box]
 //   SCG_ITER          iteration j: gamma, delta, the stop test, alpha, beta; p and A p
 //                     from it, x and r updated; the partials of the new r; SCGState [(j + 1) & 1]
 //   SCG_CHECK         one workgroup per box: the stop test of the state entering iteration j
 //   SCG_FINAL         x = x0 + xi z_perp + P^T s from the state entering iteration j
-enum SCGLaunch { SCG_INIT = 0, SCG_INIT_W, SCG_ROW_INIT, SCG_SETUP_REDUCE, SCG_ITER, SCG_CHECK, SCG_FINAL };
+enum SCGLaunch { SCG_INIT = 0, SCG_ROW_INIT, SCG_INIT_W, SCG_ITER, SCG_CHECK, SCG_FINAL };
 hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s);
 // map[j] = {old row start, new row start, rows}: dst[new + q] = src[old + q] (and src2 -> dst2 if set)
 // growth re-cut of listed patches (k_grow_rows; 4 int4 per entry), at most

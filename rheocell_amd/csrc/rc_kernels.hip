@@ -2828,22 +2828,29 @@ __device__ inline int scg_slot_site(const SCGTileBox &T, int hx, int hy) {
 	return T.o + gy * T.L + gx;
 }
 
-// Set-up, per site: site means b_bar = P b / m, x_bar = P x0 / m;
-// c = b_bar - D x_bar; y = (P r0) / m = c + eta N4(P x0), the site part of the
-// first residual r0 = b - A x0; s = 0.
+// Set-up, per site: D(k) = xi + eta (the rows of the four neighbours), added
+// one triplet at a time in the order of WetPolarModel.cpp:206-215 (S, W, E, N);
+// site means b_bar = P b / m, x_bar = P x0 / m; c = b_bar - D x_bar;
+// y = (P r0) / m = c + eta N4(P x0), the site part of the first residual
+// r0 = b - A x0; s = 0.
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
 	const SCGTileBox T = scg_tile_box(a);
 	const int lx = threadIdx.x % SCG_TX, ly = threadIdx.x / SCG_TX;
 	if(T.x0 + lx >= T.L || T.y0 + ly >= T.L) return;
 	const int kl = (T.y0 + ly) * T.L + T.x0 + lx, k = T.o + kl;
-	const double eta = a.params[2 * T.box + a.set].friction_cell;
+	const DevParams &P = a.params[2 * T.box + a.set];
+	const double eta = P.friction_cell;
+	int nb[4];
+	site_nbrs(kl, T.L, nb);
 	const int m = a.cover[k];
+	const int mn[4] = { a.cover[T.o + nb[0]], a.cover[T.o + nb[1]], a.cover[T.o + nb[2]], a.cover[T.o + nb[3]] };
+	double D = P.friction;
+	for(int d = 0; d < 4; ++d)
+		for(int l = 0; l < mn[d]; ++l) D += P.friction_cell * 1.0;
 	double2 c = make_double2(0., 0.), y = make_double2(0., 0.);
 	if(m > 0) {
-		int nb[4];
-		site_nbrs(kl, T.L, nb);
-		const double D = a.diag[k], fm = (double) m;
+		const double fm = (double) m;
 		const double2 sb = a.sb[k], sx = a.sx[k];
 		const double2 gS = a.sx[T.o + nb[0]], gW = a.sx[T.o + nb[1]], gE = a.sx[T.o + nb[2]], gN = a.sx[T.o + nb[3]];
 		c.x = sb.x / fm - D * (sx.x / fm);
@@ -2853,6 +2860,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	}
 	const double2 zero = make_double2(0., 0.);
 	if(flags) return;
+	a.diag[k] = D;
 	a.c[k] = c;
 	a.y[0][k] = y;
 	a.s[k] = zero;
@@ -2905,12 +2913,50 @@ __device__ inline double2 scg_mz(double2 z, int m) {
 	return make_double2(fm * z.x, fm * z.y);
 }
 
-// Set-up, per tile: the partials of parity 0, with w = A z of the first residual
+// Set-up, per tile: the partials of parity 0, with w = A z of the first
+// residual.  The workgroup of each box's first tile also sums the box's row
+// partials (k_scg_row_init) and writes Eigen's start (ConjugateGradient.h:
+// |b|^2 and the threshold max(tol^2 |b|^2, DBL_MIN)) into SCGState [2 box],
+// with rho = 1, pi = xi = 0; the first stop test is iteration 0's.
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
 	__shared__ double s_red[(SCG_THREADS / 64) * 6];
 	__shared__ double2 s_mz[SCG_HW * (SCG_TY + 2)];
 	const SCGTileBox T = scg_tile_box(a);
+	if(T.lead) {
+		const SCGBox Bx = a.cgb[T.box];
+		double t6[6], v[6];
+		parts_load<SCG_THREADS>(a.part, a.ncells, Bx.cell_begin, Bx.cell_end - Bx.cell_begin, v);
+		parts_sum<SCG_THREADS>(a.part, a.ncells, Bx.cell_begin, Bx.cell_end - Bx.cell_begin, v, t6, s_red);
+		if(!flags && threadIdx.x == 0) {
+			SCGState S{};
+			S.max_iter = Bx.max_iter;
+			for(int c = 0; c < 2; ++c) {
+				const double bb = t6[c];
+				S.rho[c] = 1.;
+				S.bb[c] = bb;
+				S.rperp2[c] = t6[2 + c];
+				S.rzperp[c] = t6[4 + c];
+				if(!(bb == bb) || !(t6[2 + c] == t6[2 + c]) || !(t6[4 + c] == t6[4 + c])) {
+					cg_flag_nan(a.st);
+					S.done[c] = 1;
+					continue;
+				}
+				if(bb == 0) {
+					S.zero_rhs[c] = 1;
+					S.done[c] = 1;
+					continue;
+				}
+				double thr = Bx.tol * Bx.tol * bb;
+				if(thr < DBL_MIN) thr = DBL_MIN;
+				S.thr[c] = thr;
+			}
+			S.active = !(S.done[0] && S.done[1]);
+			for(int c = 0; c < 2; ++c) S.done_now[c] = S.done[c];
+			S.active_now = S.active;
+			a.s2[2 * T.box] = S;
+		}
+	}
 	const double eta = a.params[2 * T.box + a.set].friction_cell;
 	const SCGTile t = scg_tile(T);
 	const int m = a.cover[t.k];
@@ -2973,61 +3019,13 @@ __global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
 	}
 }
 
-// Set-up, one workgroup per box: Eigen's start (ConjugateGradient.h: |b|^2, the
-// threshold max(tol^2 |b|^2, DBL_MIN), the first residual's |r|^2 and its test)
-// into SCGState [2 box], with rho = 1, pi = xi = 0; the box's cells' row sums
-// and its tiles' site sums
-__global__ __launch_bounds__(1024) void k_scg_setup_reduce(SCGArgs a) {
-	__shared__ double s_red[16 * 6];
-	const unsigned flags = a.st->flags;
-	const int box = blockIdx.x;
-	const SCGBox Bx = a.cgb[box];
-	double t6[6], ts[6], v[6];
-	parts_load<1024>(a.part, a.ncells, Bx.cell_begin, Bx.cell_end - Bx.cell_begin, v);
-	parts_sum<1024>(a.part, a.ncells, Bx.cell_begin, Bx.cell_end - Bx.cell_begin, v, t6, s_red);
-	parts_load<1024>(a.psum[0], a.ntiles, Bx.tile_off, Bx.ntiles, v);
-	parts_sum<1024>(a.psum[0], a.ntiles, Bx.tile_off, Bx.ntiles, v, ts, s_red);
-	if(flags || threadIdx.x != 0) return;
-	SCGState S{};
-	S.max_iter = Bx.max_iter;
-	for(int c = 0; c < 2; ++c) {
-		const double bb = t6[c], rp2 = t6[2 + c], rzp = t6[4 + c];
-		const double rn2 = rp2 + ts[c];
-		S.rho[c] = 1.;
-		S.bb[c] = bb;
-		S.rperp2[c] = rp2;
-		S.rzperp[c] = rzp;
-		if(!(bb == bb) || !(rn2 == rn2) || !(ts[2 + c] == ts[2 + c]) || !(ts[4 + c] == ts[4 + c])) {
-			cg_flag_nan(a.st);
-			S.done[c] = 1;
-			continue;
-		}
-		if(bb == 0) {
-			S.zero_rhs[c] = 1;
-			S.done[c] = 1;
-			continue;
-		}
-		double thr = Bx.tol * Bx.tol * bb;
-		if(thr < DBL_MIN) thr = DBL_MIN;
-		S.thr[c] = thr;
-		S.rn2[c] = rn2;
-		if(rn2 < thr) S.done[c] = 1;
-	}
-	S.active = !(S.done[0] && S.done[1]);
-	for(int c = 0; c < 2; ++c) {
-		S.rn2_now[c] = S.rn2[c];
-		S.done_now[c] = S.done[c];
-	}
-	S.active_now = S.active;
-	a.s2[2 * box] = S;
-}
-
 struct SCGCoef {
 	bool done[2];
 	double rn2[2], gamma[2], alpha[2], beta[2], pi[2];
 };
 // Iteration j's scalars from the sums tot of the residual entering it:
-//   |r|^2 = rho^2 |r_perp|^2 + y.m y  (Eigen's stop test of the last update; j > 0)
+//   |r|^2 = rho^2 |r_perp|^2 + y.m y  (Eigen's stop test: of the first residual
+//   at j = 0, of the last update after that)
 //   gamma = r.z  = rho^2 r_perp.z_perp + y.m (y / D)
 //   delta = z.Az = rho^2 r_perp.z_perp + (y / D).m w      (A z_perp = r_perp)
 //   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
@@ -3045,13 +3043,11 @@ __device__ inline SCGCoef scg_coef(const SCGState &S, const double (&tot)[6], bo
 		k.pi[c] = S.pi[c];
 		if(k.done[c]) continue;
 		const double r2 = S.rho[c] * S.rho[c];
-		if(!first) {
-			const double rn2 = r2 * S.rperp2[c] + tot[c];
-			k.rn2[c] = rn2;
-			if(rn2 < S.thr[c] || S.iters[c] >= S.max_iter) {
-				k.done[c] = true;
-				continue;
-			}
+		const double rn2 = r2 * S.rperp2[c] + tot[c];
+		k.rn2[c] = rn2;
+		if(rn2 < S.thr[c] || S.iters[c] >= S.max_iter) {
+			k.done[c] = true;
+			continue;
 		}
 		const double gamma = r2 * S.rzperp[c] + tot[2 + c];
 		const double delta = r2 * S.rzperp[c] + tot[4 + c];
@@ -3321,7 +3317,6 @@ hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s) {
 	case SCG_INIT: RC_LAUNCH(k_scg_init, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
 	case SCG_INIT_W: RC_LAUNCH(k_scg_init_w, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
 	case SCG_ROW_INIT: RC_LAUNCH(k_scg_row_init, dim3(a.ncells), dim3(256), 0, s, a); break;
-	case SCG_SETUP_REDUCE: RC_LAUNCH(k_scg_setup_reduce, dim3(a.nboxes), dim3(1024), 0, s, a); break;
 	case SCG_ITER:
 		if(j == 0) RC_LAUNCH(k_scg_iter<true>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
 		else RC_LAUNCH(k_scg_iter<false>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
