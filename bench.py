@@ -99,6 +99,8 @@ def parse():
                     help="learned warm-start weights (wet workloads; skipped when absent)")
     ap.add_argument("--cg-steps", type=int, default=500, help="steps of the learned-vs-previous CG comparison")
     ap.add_argument("--no-graphs", action="store_true", help="launch dry steps one by one instead of HIP graph replays")
+    ap.add_argument("--cg-form", choices=("sites", "rows"), default="sites",
+                    help="form of the wet solve's CG: box-site fields (default) or every row (rc_set_cg_form)")
     ap.add_argument("--stage-shape", default="", metavar="TxR",
                     help="force the field-update kernel shape (threads x strip rows) instead of the autotuner's choice")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
@@ -306,7 +308,7 @@ def main():
     L = conf.Lx
     box_sites = sum(c.Lx * c.Lx for c, _ in boxes)
     wet = float(keys["friction_cell"]) != 0
-    sim = Simulation(boxes)
+    sim = Simulation(boxes, cg_form=_lib.RC_CG_ROWS if args.cg_form == "rows" else _lib.RC_CG_SITES)
     if args.no_graphs:
         sim.set_graphs(False)
     if args.stage_shape:
@@ -452,6 +454,7 @@ def main():
                    "model": str(keys.get("interaction_type", "wetpolarmodel")),
                    "box_type": str(keys.get("type", "none")), "friction_cell": float(keys["friction_cell"]),
                    "cg_tolerance": float(keys["CGtolerance"]) if wet else None,
+                   "cg_form": args.cg_form if wet else None,
                    "warm_start": ("previous solution, cold every %s solve (reference rule)" %
                                   ("101st" if keys.get("interaction_type") == "wetmodel" else "11th")) if wet else None},
         "roofline": roof,
