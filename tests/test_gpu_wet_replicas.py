@@ -84,6 +84,34 @@ def test_wet_replicas_equal_one_handle_per_box():
     assert st.iters_total == its and st.worst_relres < 1e-8
 
 
+def test_nematic_wet_replicas_match_the_oracle():
+    """The reference's example model (interaction_type = wetmodel) in two
+    boxes of one handle, each against its own oracle."""
+    boxes = []
+    for L, n, seed in ((64, 3, 31), (96, 4, 32)):
+        conf = confgen.lattice_conf(L, n, seed)
+        keys = dict(confgen.BASE_KEYS)
+        keys.update(confgen.EXAMPLE_NEMATIC_KEYS)
+        keys.update(CGtolerance=1e-12, seed=seed)
+        boxes.append((conf, keys))
+    sim = Simulation(boxes)
+    orcs = [OracleSim(c, k) for c, k in boxes]
+    sim.step(3)
+    for o in orcs:
+        o.step(3)
+    _, phi = sim.download()
+    vx = sim.rows(0)
+    start, worst, worst_v = 0, 0.0, 0.0
+    for o in orcs:
+        ref = np.concatenate([o.phi(j) for j in range(o.N)])
+        n = len(ref)
+        worst = max(worst, float(np.abs(phi[start:start + n] - ref).max()))
+        worst_v = max(worst_v, float(np.abs(vx[start:start + n] - o.rows(0)).max()))
+        start += n
+    print("2 nematic wet replicas, 3 steps: max|dphi| %.3g max|dvx| %.3g" % (worst, worst_v), flush=True)
+    assert worst <= TOL and worst_v <= TOL
+
+
 def test_row_form_and_learned_start_need_one_box():
     sim = Simulation(_boxes(1e-8), init=False)
     with pytest.raises(RCexception, match="RC_ERR_ARG"):
