@@ -91,6 +91,7 @@ struct rc_handle {
 	SCGState *d_scg = nullptr;    // [2 * nboxes]
 	SCGState *h_scg = nullptr;    // pinned, [2 * nboxes]
 	SCGBox *d_cgb = nullptr;      // [nboxes]
+	SCGTileGeo *d_cg_geo = nullptr;   // [cg_ntiles]
 	std::vector<SCGBox> cgb;      // host copy (its iteration caps follow the layout)
 	int *d_cg_tile_box = nullptr; // [cg_ntiles]
 	int cg_ntiles = 0;
@@ -485,6 +486,7 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		std::vector<SCGBox> &cgb = h->cgb;
 		cgb.assign(nboxes, SCGBox{});
 		std::vector<int> cg_tile_box;
+		std::vector<SCGTileGeo> cg_geo;
 		for(int b = 0; b < nboxes; ++b) {
 			const DevBox &B = h->dboxes[b];
 			SCGBox &g = cgb[b];
@@ -499,7 +501,23 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 			g.tol = mp.CGtolerance > 0 ? mp.CGtolerance : 2.220446049250313e-16;
 			g.max_iter = mp.cg_max_iter > 0 ? mp.cg_max_iter : 1 << 30;
 			cg_tile_box.insert(cg_tile_box.end(), g.ntiles, b);
+			for(int lt = 0; lt < g.ntiles; ++lt) {
+				SCGTileGeo G{};
+				G.box = b;
+				G.o = g.box_off;
+				G.L = g.L;
+				G.tile_off = g.tile_off;
+				G.ntiles = g.ntiles;
+				G.x0 = (lt % g.ntx) * SCG_TX;
+				G.y0 = (lt / g.ntx) * SCG_TY;
+				G.lead = lt == 0;
+				G.eta[0] = h->dparams[2 * b].friction_cell;
+				G.eta[1] = h->dparams[2 * b + 1].friction_cell;
+				cg_geo.push_back(G);
+			}
 		}
+		// the one-box iteration takes box 0 from its launch arguments at site 0
+		if(cgb[0].box_off != 0) return fail(h, RC_ERR_STATE, "the first box does not start at site 0");
 		h->cg_ntiles = (int) cg_tile_box.size();
 		for(double2 **f : { &h->d_sc, &h->d_ss, &h->d_sy[0], &h->d_sy[1], &h->d_sq[0], &h->d_sq[1] })
 			HIPCHK(h, dalloc(f, sites));
@@ -511,6 +529,8 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, hipMemcpy(h->d_cgb, cgb.data(), sizeof(SCGBox) * nboxes, hipMemcpyHostToDevice));
 		HIPCHK(h, dalloc(&h->d_cg_tile_box, h->cg_ntiles));
 		HIPCHK(h, hipMemcpy(h->d_cg_tile_box, cg_tile_box.data(), sizeof(int) * h->cg_ntiles, hipMemcpyHostToDevice));
+		HIPCHK(h, dalloc(&h->d_cg_geo, h->cg_ntiles));
+		HIPCHK(h, hipMemcpy(h->d_cg_geo, cg_geo.data(), sizeof(SCGTileGeo) * h->cg_ntiles, hipMemcpyHostToDevice));
 	}
 	return RC_OK;
 }
@@ -530,7 +550,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_ggauss); dfree(h->d_step_base); dfree(h->d_bf);
 	if(h->h_step_base) (void) hipHostFree(h->h_step_base); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
-	dfree(h->d_sc); dfree(h->d_ss); dfree(h->d_scg); dfree(h->d_cgb); dfree(h->d_cg_tile_box);
+	dfree(h->d_sc); dfree(h->d_ss); dfree(h->d_scg); dfree(h->d_cgb); dfree(h->d_cg_tile_box); dfree(h->d_cg_geo);
 	for(int i = 0; i < 2; ++i) {
 		dfree(h->d_sy[i]); dfree(h->d_sq[i]); dfree(h->d_psum[i]);
 	}
@@ -944,6 +964,10 @@ static SCGArgs scg_args(rc_handle *h, const CGArgs &c) {
 	a.s2 = h->d_scg;
 	a.cgb = h->d_cgb;
 	a.tile_box = h->d_cg_tile_box;
+	a.geo = h->d_cg_geo;
+	a.one_L = h->cgb[0].L;
+	a.one_ntx = h->cgb[0].ntx;
+	a.one_eta = h->dparams[2 * 0 + h->cur_set].friction_cell;
 	a.ncells = h->ncells;
 	a.ntiles = h->cg_ntiles;
 	a.nboxes = h->nboxes;

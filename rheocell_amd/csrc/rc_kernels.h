@@ -206,6 +206,15 @@ hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream
 // parity j & 1 and writes those of the other parity (its neighbours' old
 // values stay readable), and the partial sums likewise.
 constexpr int SCG_TX = 64, SCG_TY = 8, SCG_THREADS = SCG_TX * SCG_TY;   // iteration tile: SCG_TX x SCG_TY sites
+// One tile of the iteration: its box, the box's first site, side, first tile
+// and tile count, the tile's corner, whether it is the box's first tile, and
+// the box's viscosity (friction_cell) in both parameter sets.  The iteration
+// reads it in one load (with one box, it builds it from the launch arguments
+// instead and needs no load before the site data).
+struct SCGTileGeo {
+	int box, o, L, tile_off, ntiles, x0, y0, lead;
+	double eta[2];
+};
 struct SCGArgs {
 	const DevBox *boxes;
 	const DevCell *cells;
@@ -225,6 +234,9 @@ struct SCGArgs {
 	SCGState *s2;                // [2 * nboxes]
 	const SCGBox *cgb;           // [nboxes]
 	const int *tile_box;         // [ntiles]
+	const SCGTileGeo *geo;       // [ntiles]
+	int one_L, one_ntx;          // box 0's side and tiles per row (read when nboxes == 1)
+	double one_eta;              // box 0's friction_cell in the current parameter set
 	int ncells, ntiles, nboxes;
 	DevStatus *st;
 };
@@ -232,7 +244,7 @@ struct SCGArgs {
 //   SCG_INIT          per site: D, c, y = c + eta N4(sx), s = 0
 //   SCG_ROW_INIT      per cell: |b|^2 and the ker-P residual sums over the rows
 //   SCG_INIT_W        per site: the partials of parity 0 (with w = the site part of A z);
-//                     per box: Eigen's start into SCGState [2 box]
+//                     per box: Eigen's start into SCGState [2 box] and [2 box + 1]
 //   SCG_ITER          iteration j: gamma, delta, the stop test, alpha, beta; p and A p
 //                     from it, x and r updated; the partials of the new r; SCGState [(j + 1) & 1]
 //   SCG_CHECK         one workgroup per box: the stop test of the state entering iteration j

@@ -2798,6 +2798,57 @@ __device__ inline void parts_sum(const double *part, int stride, int begin, int 
 	reduce_tail<6, NT>(acc, tot, s_red);
 }
 
+// The site form's sums, in one fixed order wherever they are formed:
+// a tile's six partials (scg_tile_partials) and a box's sums of its tiles'
+// partials (scg_sums_load, scg_sums_add; the same in every workgroup of the box
+// and in k_scg_check).  Wave w < 6 of the workgroup adds value w: each lane its
+// items lane, lane + 64, ... in order, then the lanes (wave_sum); the other
+// waves idle.  One value per wave keeps the cross-lane work to six wave sums
+// per workgroup.
+constexpr int SCG_PU = SCG_THREADS / 64;   // items per lane requested with a launch's first loads
+static_assert(SCG_THREADS >= 6 * 64, "one wave per value");
+// a tile's partials from each thread's six values (s_part: [6][SCG_THREADS]
+// doubles of LDS) -> part[j * ntiles + tile]
+__device__ inline void scg_tile_partials(const double (&v)[6], double *s_part, double *part, int ntiles, int tile) {
+#pragma unroll
+	for(int j = 0; j < 6; ++j) s_part[j * SCG_THREADS + threadIdx.x] = v[j];
+	__syncthreads();
+	const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	if(w < 6) {
+		double acc = 0.;
+#pragma unroll
+		for(int i = 0; i < SCG_THREADS / 64; ++i) acc += s_part[w * SCG_THREADS + lane + 64 * i];
+		acc = wave_sum(acc);
+		if(lane == 0) part[w * ntiles + tile] = acc;
+	}
+}
+// a box's partials: items begin .. begin + count - 1 of a [6][stride] array.
+// The first SCG_PU items of the lane are requested by scg_sums_load (all
+// addresses valid: waves 6 and 7 and items past the end read a valid item
+// that is not added), the rest by scg_sums_add, which gives every thread the
+// six sums (s_red: 6 doubles of LDS).
+__device__ inline void scg_sums_load(const double *part, int stride, int begin, int count, double (&v)[SCG_PU]) {
+	const int w = min((int) threadIdx.x >> 6, 5), lane = threadIdx.x & 63;
+#pragma unroll
+	for(int i = 0; i < SCG_PU; ++i) v[i] = part[w * stride + begin + min(lane + 64 * i, count - 1)];
+}
+__device__ inline void scg_sums_add(const double *part, int stride, int begin, int count, const double (&v)[SCG_PU],
+		double (&tot)[6], double *s_red) {
+	const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	if(w < 6) {
+		double acc = 0.;
+#pragma unroll
+		for(int i = 0; i < SCG_PU; ++i)
+			if(lane + 64 * i < count) acc += v[i];
+		for(int it = lane + 64 * SCG_PU; it < count; it += 64) acc += part[w * stride + begin + it];
+		acc = wave_sum(acc);
+		if(lane == 0) s_red[w] = acc;
+	}
+	__syncthreads();
+#pragma unroll
+	for(int j = 0; j < 6; ++j) tot[j] = s_red[j];
+}
+
 // The tile of a site pass: workgroup b takes tile xcd_item(b) of the handle's
 // tiles (box-major), a tile of SCG_TX x SCG_TY sites of its box.  Slots at tile
 // coordinates (hx, hy) stand for the box site ((x0 + hx) mod L, (y0 + hy) mod
@@ -2821,11 +2872,55 @@ __device__ inline SCGTileBox scg_tile_box(const SCGArgs &a) {
 	t.lead = lt == 0;
 	return t;
 }
+// the same for the iteration, with the box's tile range and viscosity: from
+// the launch arguments with one box (ONE), else one load of the tile's
+// SCGTileGeo
+struct SCGIterTile {
+	SCGTileBox t;
+	int tile_off, ntiles;
+	double eta;
+};
+template <bool ONE>
+__device__ inline SCGIterTile scg_iter_tile(const SCGArgs &a) {
+	SCGIterTile g;
+	const int tile = xcd_item(blockIdx.x, gridDim.x);
+	g.t.tile = tile;
+	if(ONE) {
+		g.t.box = 0;
+		g.t.o = 0;
+		g.t.L = a.one_L;
+		g.t.x0 = (tile % a.one_ntx) * SCG_TX;
+		g.t.y0 = (tile / a.one_ntx) * SCG_TY;
+		g.t.lead = tile == 0;
+		g.tile_off = 0;
+		g.ntiles = a.ntiles;
+		g.eta = a.one_eta;
+	}
+	else {
+		const SCGTileGeo G = a.geo[tile];
+		g.t.box = G.box;
+		g.t.o = G.o;
+		g.t.L = G.L;
+		g.t.x0 = G.x0;
+		g.t.y0 = G.y0;
+		g.t.lead = G.lead != 0;
+		g.tile_off = G.tile_off;
+		g.ntiles = G.ntiles;
+		g.eta = a.set ? G.eta[1] : G.eta[0];   // a select, not an indexed copy in scratch
+	}
+	return g;
+}
+// Slot coordinates lie in -2 .. L + SCG_TX (a tile starts inside its box),
+// so for L > SCG_TX / 2 they wrap by selects (at most two subtractions of L);
+// smaller boxes take the remainder (a branch on L, the same for the whole
+// workgroup).
+__device__ inline int scg_wrap(int g, int L) {
+	return g < 0 ? g + L : (g < L ? g : (g < 2 * L ? g - L : g - 2 * L));
+}
 __device__ inline int scg_slot_site(const SCGTileBox &T, int hx, int hy) {
-	int gx = T.x0 + hx, gy = T.y0 + hy;
-	gx = gx < 0 ? gx + T.L : gx % T.L;
-	gy = gy < 0 ? gy + T.L : gy % T.L;
-	return T.o + gy * T.L + gx;
+	const int gx = T.x0 + hx, gy = T.y0 + hy;
+	if(T.L <= SCG_TX / 2) return T.o + (gy < 0 ? gy + T.L : gy % T.L) * T.L + (gx < 0 ? gx + T.L : gx % T.L);
+	return T.o + scg_wrap(gy, T.L) * T.L + scg_wrap(gx, T.L);
 }
 
 // Set-up, per site: D(k) = xi + eta (the rows of the four neighbours), added
@@ -2922,6 +3017,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
 	__shared__ double s_red[(SCG_THREADS / 64) * 6];
 	__shared__ double2 s_mz[SCG_HW * (SCG_TY + 2)];
+	__shared__ double s_part[6 * SCG_THREADS];
 	const SCGTileBox T = scg_tile_box(a);
 	if(T.lead) {
 		const SCGBox Bx = a.cgb[T.box];
@@ -2954,7 +3050,9 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
 			S.active = !(S.done[0] && S.done[1]);
 			for(int c = 0; c < 2; ++c) S.done_now[c] = S.done[c];
 			S.active_now = S.active;
+			// both copies: an iteration rewrites only the scalars that change
 			a.s2[2 * T.box] = S;
+			a.s2[2 * T.box + 1] = S;
 		}
 	}
 	const double eta = a.params[2 * T.box + a.set].friction_cell;
@@ -2973,8 +3071,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
 	const double2 w = scg_w_lds(s_mz, t.slot, D, z, m, eta);
 	double part[6] = { 0., 0., 0., 0., 0., 0. };
 	if(t.in) scg_part(part, m, D, y, w);
-	block_sum<6, SCG_THREADS / 64>(part, s_red);
-	scg_partials(part, a.psum[0], a.ntiles, T.tile);
+	scg_tile_partials(part, s_part, a.psum[0], a.ntiles, T.tile);
 }
 
 // box site of storage row q of cell C (MultiPhaseField ring buffer, SquareBox wrap)
@@ -3019,6 +3116,51 @@ __global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
 	}
 }
 
+// The scalars of SCGState an iteration reads, requested with its first loads,
+// and the ones it rewrites (scg_put; the others are the set-up's, the same in
+// both copies).
+struct SCGVar {
+	double rn2[2], gamma[2], alpha[2], beta[2], pi[2], rho[2], xi[2];
+	double thr[2], rperp2[2], rzperp[2];
+	int done[2], iters[2], max_iter, active;
+};
+__device__ inline SCGVar scg_var(const SCGState *S) {
+	SCGVar v;
+#pragma unroll
+	for(int c = 0; c < 2; ++c) {
+		v.rn2[c] = S->rn2[c];
+		v.gamma[c] = S->gamma[c];
+		v.alpha[c] = S->alpha[c];
+		v.beta[c] = S->beta[c];
+		v.pi[c] = S->pi[c];
+		v.rho[c] = S->rho[c];
+		v.xi[c] = S->xi[c];
+		v.thr[c] = S->thr[c];
+		v.rperp2[c] = S->rperp2[c];
+		v.rzperp[c] = S->rzperp[c];
+		v.done[c] = S->done[c];
+		v.iters[c] = S->iters[c];
+	}
+	v.max_iter = S->max_iter;
+	v.active = S->active;
+	return v;
+}
+__device__ inline void scg_put(SCGState *S, const SCGVar &v) {
+#pragma unroll
+	for(int c = 0; c < 2; ++c) {
+		S->rn2[c] = v.rn2[c];
+		S->gamma[c] = v.gamma[c];
+		S->alpha[c] = v.alpha[c];
+		S->beta[c] = v.beta[c];
+		S->pi[c] = v.pi[c];
+		S->rho[c] = v.rho[c];
+		S->xi[c] = v.xi[c];
+		S->done[c] = v.done[c];
+		S->iters[c] = v.iters[c];
+	}
+	S->active = v.active;
+}
+
 struct SCGCoef {
 	bool done[2];
 	double rn2[2], gamma[2], alpha[2], beta[2], pi[2];
@@ -3031,7 +3173,8 @@ struct SCGCoef {
 //   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
 //   (j = 0: beta = 0, alpha = gamma / delta), pi = rho + beta pi_prev
 // (a NaN residual is not "done": the writer raises ST_NAN, which halts the device)
-__device__ inline SCGCoef scg_coef(const SCGState &S, const double (&tot)[6], bool first) {
+template <class State>
+__device__ inline SCGCoef scg_coef(const State &S, const double (&tot)[6], bool first) {
 	SCGCoef k;
 #pragma unroll
 	for(int c = 0; c < 2; ++c) {
@@ -3143,48 +3286,52 @@ __device__ inline double2 scg_apply(const double2 *s_mv, int g, double D, double
 // writes 108 B per box site.  The workgroup of the box's first tile writes
 // SCGState [2 box + ((j + 1) & 1)]: rho -= alpha pi, xi += alpha pi and the
 // recurrence scalars (nobody reads that copy during iteration j).
-template <bool FIRST>
+template <bool FIRST, bool ONE>
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	const int pin = j & 1, pout = pin ^ 1;
-	__shared__ double s_red[(SCG_THREADS / 64) * 6];
+	__shared__ double s_red[6];
 	__shared__ double2 s_mq[SCG_G2W * SCG_G2H];
 	__shared__ double2 s_mz[SCG_G2W * SCG_G2H];
-	const SCGTileBox T = scg_tile_box(a);
-	const SCGBox Bx = a.cgb[T.box];
+	__shared__ double s_part[6 * SCG_THREADS];
+	const SCGIterTile G = scg_iter_tile<ONE>(a);
+	const SCGTileBox &T = G.t;
 	const SCGState *Sp = a.s2 + 2 * T.box + pin;
 	SCGState *Sn = a.s2 + 2 * T.box + pout;
 	const bool writer = T.lead && threadIdx.x == 0;
-	const double eta = a.params[2 * T.box + a.set].friction_cell;
+	const double eta = G.eta;
 	const SCGTile2 t = scg_tile2(T);
-	// Every load is requested before any is used: the site data, this
-	// thread's partials, the status word and the state (all addresses are
-	// valid whatever they hold), so the launch waits for one memory round trip,
-	// not three, and a launch after convergence returns after that one.
+	// Every load is requested before any is used: the site data, this lane's
+	// partials, the status word and the state's scalars (all addresses are
+	// valid whatever they hold), so the launch waits for one memory round
+	// trip, and a launch after convergence returns after that one.
 	const SCGSite c0 = scg_load(a, pin, t.k, FIRST);
 	SCGSite ch{};
 	if(t.hg >= 0) ch = scg_load(a, pin, t.hk, FIRST);
 	double2 s = a.s[t.k];
-	double pre[6];
-	parts_load<SCG_THREADS>(a.psum[pin], a.ntiles, Bx.tile_off, Bx.ntiles, pre);
+	double pre[SCG_PU];
+	scg_sums_load(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre);
 	const unsigned flags = a.st->flags;
-	const int active = Sp->active;
+	SCGVar V = scg_var(Sp);
+	// an ordering point: the compiler keeps the loads above it, where they
+	// were requested, instead of moving them under the branches below
+	asm volatile("" ::: "memory");
 	if(flags) return;
-	if(!active) {
-		if(writer) *Sn = *Sp;
+	if(!V.active) {
+		if(writer) scg_put(Sn, V);
 		return;
 	}
 	double tot[6];
-	parts_sum<SCG_THREADS>(a.psum[pin], a.ntiles, Bx.tile_off, Bx.ntiles, pre, tot, s_red);
-	const SCGCoef cf = scg_coef(*Sp, tot, FIRST);
+	scg_sums_add(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre, tot, s_red);
+	const SCGCoef cf = scg_coef(V, tot, FIRST);
 	if(cf.done[0] && cf.done[1]) {
 		if(writer) {
-			*Sn = *Sp;
 #pragma unroll
 			for(int c = 0; c < 2; ++c) {
-				Sn->rn2[c] = cf.rn2[c];
-				Sn->done[c] = 1;
+				V.rn2[c] = cf.rn2[c];
+				V.done[c] = 1;
 			}
-			Sn->active = 0;
+			V.active = 0;
+			scg_put(Sn, V);
 		}
 		return;
 	}
@@ -3230,28 +3377,27 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 		a.y[pout][t.k] = yn0;
 		scg_part(part, c0.m, c0.D, yn0, wn);
 	}
-	block_sum<6, SCG_THREADS / 64>(part, s_red);
-	scg_partials(part, a.psum[pout], a.ntiles, T.tile);
 	if(writer) {
-		*Sn = *Sp;
 #pragma unroll
 		for(int c = 0; c < 2; ++c) {
-			Sn->rn2[c] = cf.rn2[c];
+			V.rn2[c] = cf.rn2[c];
 			if(!(cf.rn2[c] == cf.rn2[c])) cg_flag_nan(a.st);
 			if(cf.done[c]) {
-				Sn->done[c] = 1;
+				V.done[c] = 1;
 				continue;
 			}
-			Sn->gamma[c] = cf.gamma[c];
-			Sn->alpha[c] = cf.alpha[c];
-			Sn->beta[c] = cf.beta[c];
-			Sn->pi[c] = cf.pi[c];
-			Sn->rho[c] = Sp->rho[c] - cf.alpha[c] * cf.pi[c];
-			Sn->xi[c] = Sp->xi[c] + cf.alpha[c] * cf.pi[c];
-			Sn->iters[c] = Sp->iters[c] + 1;
+			V.gamma[c] = cf.gamma[c];
+			V.alpha[c] = cf.alpha[c];
+			V.beta[c] = cf.beta[c];
+			V.rho[c] = V.rho[c] - cf.alpha[c] * cf.pi[c];
+			V.xi[c] = V.xi[c] + cf.alpha[c] * cf.pi[c];
+			V.pi[c] = cf.pi[c];
+			V.iters[c] = V.iters[c] + 1;
 		}
-		Sn->active = 1;
+		V.active = 1;
+		scg_put(Sn, V);
 	}
+	scg_tile_partials(part, s_part, a.psum[pout], a.ntiles, T.tile);
 }
 
 // The stop test of each box's state entering iteration j (after a batch),
@@ -3266,9 +3412,9 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_check(SCGArgs a, int j) {
 	const int active = S->active;
 	double tot[6] = { 0., 0., 0., 0., 0., 0. };
 	if(active && j > 0) {
-		double pre[6];
-		parts_load<SCG_THREADS>(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre);
-		parts_sum<SCG_THREADS>(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre, tot, s_red);
+		double pre[SCG_PU];
+		scg_sums_load(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre);
+		scg_sums_add(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre, tot, s_red);
 	}
 	if(threadIdx.x != 0) return;
 	if(!active || j == 0) {
@@ -3318,8 +3464,14 @@ hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s) {
 	case SCG_INIT_W: RC_LAUNCH(k_scg_init_w, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
 	case SCG_ROW_INIT: RC_LAUNCH(k_scg_row_init, dim3(a.ncells), dim3(256), 0, s, a); break;
 	case SCG_ITER:
-		if(j == 0) RC_LAUNCH(k_scg_iter<true>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
-		else RC_LAUNCH(k_scg_iter<false>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+		if(a.nboxes == 1) {
+			if(j == 0) RC_LAUNCH((k_scg_iter<true, true>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+			else RC_LAUNCH((k_scg_iter<false, true>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+		}
+		else {
+			if(j == 0) RC_LAUNCH((k_scg_iter<true, false>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+			else RC_LAUNCH((k_scg_iter<false, false>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+		}
 		break;
 	case SCG_CHECK: RC_LAUNCH(k_scg_check, dim3(a.nboxes), dim3(SCG_THREADS), 0, s, a, j); break;
 	case SCG_FINAL: RC_LAUNCH(k_scg_final, dim3(a.ncells), dim3(256), 0, s, a, j); break;
