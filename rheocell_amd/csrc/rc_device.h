@@ -110,6 +110,10 @@ struct SCGState {
 	double rho[2], pi[2], xi[2];
 	double gamma[2], alpha[2], beta[2];   // r.z, alpha and beta of the last iteration
 	int done[2], iters[2], zero_rhs[2], max_iter, active;
+	// s lacks the step alpha q of the last iteration (an even one: even
+	// iterations leave their step of s to the next, which reads q anyway;
+	// that q is in the parity 1 copy)
+	int pend[2];
 	// stop test of the last update, written by k_scg_check after a batch
 	double rn2_now[2];
 	int done_now[2], active_now, pad;

@@ -3122,7 +3122,7 @@ __global__ __launch_bounds__(256) void k_scg_row_init(SCGArgs a) {
 struct SCGVar {
 	double rn2[2], gamma[2], alpha[2], beta[2], pi[2], rho[2], xi[2];
 	double thr[2], rperp2[2], rzperp[2];
-	int done[2], iters[2], max_iter, active;
+	int done[2], iters[2], pend[2], max_iter, active;
 };
 __device__ inline SCGVar scg_var(const SCGState *S) {
 	SCGVar v;
@@ -3140,6 +3140,7 @@ __device__ inline SCGVar scg_var(const SCGState *S) {
 		v.rzperp[c] = S->rzperp[c];
 		v.done[c] = S->done[c];
 		v.iters[c] = S->iters[c];
+		v.pend[c] = S->pend[c];
 	}
 	v.max_iter = S->max_iter;
 	v.active = S->active;
@@ -3157,6 +3158,7 @@ __device__ inline void scg_put(SCGState *S, const SCGVar &v) {
 		S->xi[c] = v.xi[c];
 		S->done[c] = v.done[c];
 		S->iters[c] = v.iters[c];
+		S->pend[c] = v.pend[c];
 	}
 	S->active = v.active;
 }
@@ -3307,7 +3309,12 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	const SCGSite c0 = scg_load(a, pin, t.k, FIRST);
 	SCGSite ch{};
 	if(t.hg >= 0) ch = scg_load(a, pin, t.hk, FIRST);
-	double2 s = a.s[t.k];
+	// s moves at odd iterations only, by the previous iteration's pending
+	// step and its own: (s + alpha_prev q_prev) + alpha q, the order of a
+	// step per iteration, with q_prev the q this iteration reads anyway
+	const bool s_pass = (j & 1) != 0;
+	double2 s = make_double2(0., 0.);
+	if(s_pass) s = a.s[t.k];
 	double pre[SCG_PU];
 	scg_sums_load(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre);
 	const unsigned flags = a.st->flags;
@@ -3368,11 +3375,15 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	if(t.hin) s_mz[t.hg] = scg_mz(scg_z(rstep(ch, qh, t.hg), ch.D, ch.m), ch.m);
 	__syncthreads();
 	const double2 wn = scg_apply(s_mz, t.g, c0.D, z0, c0.m, eta);
-	if(!d0) s.x += al0 * q0.x;
-	if(!d1) s.y += al1 * q0.y;
+	if(s_pass) {
+		if(V.pend[0]) s.x += V.alpha[0] * c0.q.x;
+		if(V.pend[1]) s.y += V.alpha[1] * c0.q.y;
+		if(!d0) s.x += al0 * q0.x;
+		if(!d1) s.y += al1 * q0.y;
+	}
 	double part[6] = { 0., 0., 0., 0., 0., 0. };
 	if(t.in) {
-		a.s[t.k] = s;
+		if(s_pass) a.s[t.k] = s;
 		a.q[pout][t.k] = q0;
 		a.y[pout][t.k] = yn0;
 		scg_part(part, c0.m, c0.D, yn0, wn);
@@ -3382,6 +3393,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 		for(int c = 0; c < 2; ++c) {
 			V.rn2[c] = cf.rn2[c];
 			if(!(cf.rn2[c] == cf.rn2[c])) cg_flag_nan(a.st);
+			V.pend[c] = !s_pass && !cf.done[c];
 			if(cf.done[c]) {
 				V.done[c] = 1;
 				continue;
@@ -3445,10 +3457,19 @@ __global__ __launch_bounds__(256) void k_scg_final(SCGArgs a, int j) {
 	const SCGState *S = a.s2 + 2 * C.box + (j & 1);
 	const double xi0 = S->xi[0], xi1 = S->xi[1];
 	const bool z0 = S->zero_rhs[0], z1 = S->zero_rhs[1];
+	// the step an even last iteration left pending (its q in the parity 1 copy)
+	const bool pe0 = S->pend[0], pe1 = S->pend[1];
+	const double pa0 = S->alpha[0], pa1 = S->alpha[1];
 	for(int q = threadIdx.x; q < C.S; q += 256) {
 		const int k = B.box_off + cell_site(C, B.L, q), row = C.start + q;
 		const double D = a.diag[k], inv = 1. / D;
-		const double2 c = a.c[k], sv = a.s[k], bb = a.b[row], x0 = a.x[row];
+		const double2 c = a.c[k], bb = a.b[row], x0 = a.x[row];
+		double2 sv = a.s[k];
+		if(pe0 || pe1) {
+			const double2 qp = a.q[1][k];
+			if(pe0) sv.x += pa0 * qp.x;
+			if(pe1) sv.y += pa1 * qp.y;
+		}
 		const double rx = (bb.x - D * x0.x) - c.x, ry = (bb.y - D * x0.y) - c.y;
 		double2 x;
 		x.x = z0 ? 0. : (x0.x + xi0 * (inv * rx)) + sv.x;
