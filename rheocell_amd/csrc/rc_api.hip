@@ -92,6 +92,8 @@ struct rc_handle {
 	SCGState *h_scg = nullptr;    // pinned, [2 * nboxes]
 	SCGBox *d_cgb = nullptr;      // [nboxes]
 	SCGTileGeo *d_cg_geo = nullptr;   // [cg_ntiles]
+	int *d_mn = nullptr;          // [sites] rows here and at the four neighbours (site form)
+	double *d_dtab = nullptr;     // [nboxes][2][2][SCG_NTAB] D and 1 / D by neighbour rows
 	std::vector<SCGBox> cgb;      // host copy (its iteration caps follow the layout)
 	int *d_cg_tile_box = nullptr; // [cg_ntiles]
 	int cg_ntiles = 0;
@@ -518,6 +520,23 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		}
 		// the one-box iteration takes box 0 from its launch arguments at site 0
 		if(cgb[0].box_off != 0) return fail(h, RC_ERR_STATE, "the first box does not start at site 0");
+		// D by the four neighbours' row count n (friction, then friction_cell
+		// added once per row, as k_scg_init adds them) and its reciprocal
+		std::vector<double> dtab((size_t) nboxes * 2 * 2 * SCG_NTAB);
+		for(int b = 0; b < nboxes; ++b)
+			for(int set = 0; set < 2; ++set) {
+				const DevParams &P = h->dparams[2 * b + set];
+				double *t = dtab.data() + (size_t) (2 * b + set) * 2 * SCG_NTAB;
+				double D = P.friction;
+				for(int n = 0; n < SCG_NTAB; ++n) {
+					if(n > 0) D += P.friction_cell * 1.0;
+					t[n] = D;
+					t[SCG_NTAB + n] = 1. / D;
+				}
+			}
+		HIPCHK(h, dalloc(&h->d_dtab, dtab.size()));
+		HIPCHK(h, hipMemcpy(h->d_dtab, dtab.data(), sizeof(double) * dtab.size(), hipMemcpyHostToDevice));
+		HIPCHK(h, dalloc(&h->d_mn, sites));
 		h->cg_ntiles = (int) cg_tile_box.size();
 		for(double2 **f : { &h->d_sc, &h->d_ss, &h->d_sy[0], &h->d_sy[1], &h->d_sq[0], &h->d_sq[1] })
 			HIPCHK(h, dalloc(f, sites));
@@ -551,6 +570,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	if(h->h_step_base) (void) hipHostFree(h->h_step_base); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
 	dfree(h->d_sc); dfree(h->d_ss); dfree(h->d_scg); dfree(h->d_cgb); dfree(h->d_cg_tile_box); dfree(h->d_cg_geo);
+	dfree(h->d_mn); dfree(h->d_dtab);
 	for(int i = 0; i < 2; ++i) {
 		dfree(h->d_sy[i]); dfree(h->d_sq[i]); dfree(h->d_psum[i]);
 	}
@@ -949,6 +969,8 @@ static SCGArgs scg_args(rc_handle *h, const CGArgs &c) {
 	a.set = h->cur_set;
 	a.cover = h->d_cover;
 	a.diag = h->d_diag;
+	a.mn = h->d_mn;
+	a.dtab = h->d_dtab;
 	a.sb = h->d_sp;
 	a.sx = h->d_p2;
 	a.c = h->d_sc;

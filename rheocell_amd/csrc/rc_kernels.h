@@ -215,6 +215,11 @@ struct SCGTileGeo {
 	int box, o, L, tile_off, ntiles, x0, y0, lead;
 	double eta[2];
 };
+// D(k) = friction + friction_cell added once per row of the four neighbours,
+// so it depends only on the neighbours' row count n <= 4 STORE_MAX: the
+// iteration takes D and 1 / D from a table by n (the same additions and the
+// same division as per site) and reads m and n packed in one int per site.
+constexpr int SCG_NTAB = 4 * STORE_MAX + 1;
 struct SCGArgs {
 	const DevBox *boxes;
 	const DevCell *cells;
@@ -222,6 +227,8 @@ struct SCGArgs {
 	int set;
 	const int *cover;
 	double *diag;                // D(k), written by SCG_INIT
+	int *mn;                     // m | n << 8 per site (rows here, rows of the four neighbours), written by SCG_INIT
+	const double *dtab;          // [nboxes][2 parameter sets][D, 1 / D][SCG_NTAB] by n
 	const double2 *sb, *sx;      // box sums of b and of x0
 	double2 *c;                  // b_bar - D x0_bar per site (site means of b, x0)
 	double2 *s;                  // site part of x - x0

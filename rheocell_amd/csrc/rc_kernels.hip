@@ -2754,8 +2754,8 @@ __device__ inline void site_nbrs(int k, int L, int *nb) {
 }
 
 // the partial sums of one site: y.m y, y.m (y / D), (y / D).m w (x, y components)
-__device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, double2 w) {
-	const double fm = m, inv = 1. / D;
+__device__ inline void scg_part_i(double (&part)[6], int m, double inv, double2 y, double2 w) {
+	const double fm = m;
 	const double zx = inv * y.x, zy = inv * y.y;
 	part[0] = fm * (y.x * y.x);
 	part[1] = fm * (y.y * y.y);
@@ -2763,6 +2763,9 @@ __device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, d
 	part[3] = fm * (y.y * zy);
 	part[4] = fm * (zx * w.x);
 	part[5] = fm * (zy * w.y);
+}
+__device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, double2 w) {
+	scg_part_i(part, m, 1. / D, y, w);
 }
 // a tile's partials (valid in thread 0) -> part[j * ntiles + tile]
 __device__ inline void scg_partials(const double (&v)[6], double *part, int ntiles, int tile) {
@@ -2943,6 +2946,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	double D = P.friction;
 	for(int d = 0; d < 4; ++d)
 		for(int l = 0; l < mn[d]; ++l) D += P.friction_cell * 1.0;
+	const int n = (mn[0] + mn[1]) + (mn[2] + mn[3]);
 	double2 c = make_double2(0., 0.), y = make_double2(0., 0.);
 	if(m > 0) {
 		const double fm = (double) m;
@@ -2956,6 +2960,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	const double2 zero = make_double2(0., 0.);
 	if(flags) return;
 	a.diag[k] = D;
+	a.mn[k] = m | n << 8;
 	a.c[k] = c;
 	a.y[0][k] = y;
 	a.s[k] = zero;
@@ -3254,19 +3259,29 @@ __device__ inline SCGTile2 scg_tile2(const SCGTileBox &T) {
 	return t;
 }
 
-// one slot's data entering an iteration: r's and p's site parts, D, m
+// one slot's data entering an iteration: r's and p's site parts, m and n
+// (D and 1 / D follow from n by the box's table, scg_diag)
 struct SCGSite {
 	double2 y, q;
-	double D;
-	int m;
+	double D, inv;
+	int m, n;
 };
 __device__ inline SCGSite scg_load(const SCGArgs &a, int pin, int k, bool first) {
 	SCGSite v;
-	v.m = a.cover[k];
-	v.D = a.diag[k];
+	const int mn = a.mn[k];
+	v.m = mn & 255;
+	v.n = min(mn >> 8, SCG_NTAB - 1);
 	v.y = a.y[pin][k];
 	v.q = first ? make_double2(0., 0.) : a.q[pin][k];
 	return v;
+}
+__device__ inline void scg_diag(SCGSite &v, const double *s_tab) {
+	v.D = s_tab[v.n];
+	v.inv = s_tab[SCG_NTAB + v.n];
+}
+// z = (1 / D) y where a row lives, as scg_z, from the tabulated 1 / D
+__device__ inline double2 scg_zi(double2 y, double inv, int m) {
+	return m > 0 ? make_double2(inv * y.x, inv * y.y) : make_double2(0., 0.);
 }
 // the m-weighted four-neighbour sum of a grid of m v values (S, W, E, N in the
 // triplet order of WetPolarModel.cpp:206-215) and D v - eta times it: the site
@@ -3295,6 +3310,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	__shared__ double2 s_mq[SCG_G2W * SCG_G2H];
 	__shared__ double2 s_mz[SCG_G2W * SCG_G2H];
 	__shared__ double s_part[6 * SCG_THREADS];
+	__shared__ double s_tab[2 * SCG_NTAB];
 	const SCGIterTile G = scg_iter_tile<ONE>(a);
 	const SCGTileBox &T = G.t;
 	const SCGState *Sp = a.s2 + 2 * T.box + pin;
@@ -3306,7 +3322,9 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	// partials, the status word and the state's scalars (all addresses are
 	// valid whatever they hold), so the launch waits for one memory round
 	// trip, and a launch after convergence returns after that one.
-	const SCGSite c0 = scg_load(a, pin, t.k, FIRST);
+	const int tab_i = min((int) threadIdx.x, 2 * SCG_NTAB - 1);
+	const double tab = a.dtab[(2 * T.box + a.set) * 2 * SCG_NTAB + tab_i];
+	SCGSite c0 = scg_load(a, pin, t.k, FIRST);
 	SCGSite ch{};
 	if(t.hg >= 0) ch = scg_load(a, pin, t.hk, FIRST);
 	// s moves at odd iterations only, by the previous iteration's pending
@@ -3327,8 +3345,12 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 		if(writer) scg_put(Sn, V);
 		return;
 	}
+	if(threadIdx.x < 2 * SCG_NTAB) s_tab[threadIdx.x] = tab;
 	double tot[6];
 	scg_sums_add(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre, tot, s_red);
+	// (the table is in LDS: scg_sums_add ends with a barrier)
+	scg_diag(c0, s_tab);
+	if(t.hg >= 0) scg_diag(ch, s_tab);
 	const SCGCoef cf = scg_coef(V, tot, FIRST);
 	if(cf.done[0] && cf.done[1]) {
 		if(writer) {
@@ -3347,7 +3369,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	// p = z + beta p_prev at a slot (z = y / D where a row lives); a converged
 	// component keeps its p
 	auto pdir = [&](const SCGSite &v) {
-		const double2 z = scg_z(v.y, v.D, v.m);
+		const double2 z = scg_zi(v.y, v.inv, v.m);
 		double2 qn = v.q;
 		if(!d0) qn.x = z.x + b0 * v.q.x;
 		if(!d1) qn.y = z.y + b1 * v.q.y;
@@ -3370,9 +3392,9 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 		return yn;
 	};
 	const double2 yn0 = rstep(c0, q0, t.g);
-	const double2 z0 = scg_z(yn0, c0.D, c0.m);
+	const double2 z0 = scg_zi(yn0, c0.inv, c0.m);
 	s_mz[t.g] = scg_mz(z0, c0.m);
-	if(t.hin) s_mz[t.hg] = scg_mz(scg_z(rstep(ch, qh, t.hg), ch.D, ch.m), ch.m);
+	if(t.hin) s_mz[t.hg] = scg_mz(scg_zi(rstep(ch, qh, t.hg), ch.inv, ch.m), ch.m);
 	__syncthreads();
 	const double2 wn = scg_apply(s_mz, t.g, c0.D, z0, c0.m, eta);
 	if(s_pass) {
@@ -3386,7 +3408,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 		if(s_pass) a.s[t.k] = s;
 		a.q[pout][t.k] = q0;
 		a.y[pout][t.k] = yn0;
-		scg_part(part, c0.m, c0.D, yn0, wn);
+		scg_part_i(part, c0.m, c0.inv, yn0, wn);
 	}
 	if(writer) {
 #pragma unroll
