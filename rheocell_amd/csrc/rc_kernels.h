@@ -205,7 +205,7 @@ hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream
 // partial sums (its tiles' range of psum).  Iteration j reads the fields of
 // parity j & 1 and writes those of the other parity (its neighbours' old
 // values stay readable), and the partial sums likewise.
-constexpr int SCG_TX = 64, SCG_TY = 8, SCG_THREADS = SCG_TX * SCG_TY;   // iteration tile: SCG_TX x SCG_TY sites
+constexpr int SCG_TX = 64, SCG_TY = 16, SCG_THREADS = SCG_TX * SCG_TY;   // iteration tile: SCG_TX x SCG_TY sites
 // One tile of the iteration: its box, the box's first site, side, first tile
 // and tile count, the tile's corner, whether it is the box's first tile, and
 // the box's viscosity (friction_cell) in both parameter sets.  The iteration
