@@ -59,10 +59,12 @@ BYTES = {
     "nem_force": (24, 24),          # read phi; write f (2 doubles); read sum phi^2, sum phi Q00, sum phi Q01
     # site form of the CG (both components, double2 fields)
     "scg_gather": (32, 32),         # read b, x0 rows; write their box sums
-    "scg_init": (0, 92),            # read cover, D, sum b, sum x0; write c, y, s
+    "scg_init": (0, 96),            # read cover, sum b, sum x0; write D, m|n, c, y, s
     "scg_init_w": (0, 28),          # read cover, D, y (the first partial sums)
     "scg_row_init": (32, 24),       # read b, x0 rows; read c, D
-    "scg_iter": (0, 108),           # read y, q, s, cover, D; write y, q, s
+    # read m|n, y, q, write y, q (68 B); odd iterations also read and write s
+    # (100 B): 84 B per site averaged over an even and an odd iteration
+    "scg_iter": (0, 84),
     "scg_final": (48, 40),          # read b, x0 rows, write x; read c, s, D
 }
 CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update", "scg_iter")
@@ -74,7 +76,7 @@ CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update", "scg_iter")
 # way, i.e. reads x 2.0, writes x 1.0.  Recorded in profiles/.
 READ_SCALE, WRITE_SCALE = 262144.0 / 131085.0, 1.0
 TRAFFIC_KIB = {
-    ("wet512_1600", "scg_iter"): (7722.9, 12384.3),
+    ("wet512_1600", "scg_iter"): (6038.0, 10252.2),
     ("wet512_1600", "cg_row_update"): (68309.3, 84801.2),
     ("wet512_1600", "cg_gather_p"): (44435.5, 52247.6),
     ("dry512_1024", "stage_fused_pred"): (8607.8, 27347.2),
