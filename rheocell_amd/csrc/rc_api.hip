@@ -880,6 +880,10 @@ static int run_gather(rc_handle *h, const double *phi) {
 	g.tile_geo = h->d_tile_geo;
 	g.tile_count = h->d_tile_count;
 	g.build_lists = h->lists_valid ? 0 : 1;
+	g.nboxes = h->nboxes;
+	g.one_L = h->dboxes[0].L;
+	g.one_ntx = h->dboxes[0].ntx;
+	g.one_cells = h->dboxes[0].cell_end;
 	g.st = h->d_st;
 	TimeScope ts(h, RC_T_BOX_GATHER);
 	HIPCHK(h, launch_gather(g, h->ntiles, h->stream));

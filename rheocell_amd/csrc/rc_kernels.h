@@ -58,6 +58,8 @@ struct GatherArgs {
 	int4 *tile_geo;              // their entries relative to the tile (tile_entry), written with the list
 	int *tile_count;
 	int build_lists;             // 1: rebuild the lists from geo, 0: reuse them
+	int nboxes;                  // 1: box 0's geometry from the fields below, no table load
+	int one_L, one_ntx, one_cells;
 	DevStatus *st;
 };
 

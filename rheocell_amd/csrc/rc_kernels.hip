@@ -253,48 +253,80 @@ __device__ int tile_list_build(const GatherArgs &a, int tile, const DevBox &B, i
 
 constexpr int GCHUNK = 8;   // covering patches examined per batch of independent loads (CG gathers)
 constexpr int GCHUNK_PHI = 16;   // the same for the sum phi^2 gather (one batch at typical overlaps)
+constexpr int GCHUNK_LIST = 32;  // list entries a reusing gather requests before it knows the count
 
 // ---------------------------------------------------------------------------
 // k_box_gather: phi2[k] = sum over covering patches (cell order) of phi^2,
 // cover[k] = number of covering patches (the reference's overlap list size)
 // ---------------------------------------------------------------------------
+// The tile's box: box 0 from the launch arguments (ONE), else the tables.
+template <bool ONE>
+__device__ inline DevBox gather_box(const GatherArgs &a, int tile) {
+	if(ONE) {
+		DevBox B{};
+		B.L = a.one_L;
+		B.ntx = a.one_ntx;
+		B.cell_end = a.one_cells;
+		return B;
+	}
+	return a.boxes[a.tile_box[tile]];
+}
+template <bool ONE>
 __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
-	if(a.st->flags) return;
 	__shared__ int4 s_list[MAXT];
 	__shared__ int s_wcnt[4];
 	const int tile = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
-	const DevBox B = a.boxes[a.tile_box[tile]];
+	const DevBox B = gather_box<ONE>(a, tile);
 	const int t = tile - B.tile_off, L = B.L;
 	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
 	const int tw = min(TW, L - x0), th = min(TH, L - y0);
-	const int n = a.build_lists ? tile_list_build(a, tile, B, x0, y0, tw, th, s_list, s_wcnt)
-			: list_count(a.tile_count, tile);
+	const int sx = tid % TW, sy = tid / TW;
+	double acc = 0.0;
+	int cnt = 0;
+	auto add = [&](const int (&row)[GCHUNK_PHI]) {
+		double v[GCHUNK_PHI];
+#pragma unroll
+		for(int k = 0; k < GCHUNK_PHI; ++k) v[k] = row[k] >= 0 ? a.phi[row[k]] : 0.;
+#pragma unroll
+		for(int k = 0; k < GCHUNK_PHI; ++k)
+			if(row[k] >= 0) {
+				acc += v[k] * v[k];
+				++cnt;
+			}
+	};
+	int n;
+	if(a.build_lists) {
+		if(a.st->flags) return;
+		n = tile_list_build(a, tile, B, x0, y0, tw, th, s_list, s_wcnt);
+	}
+	else {
+		// the status word, the count and the list's first GCHUNK_LIST entries
+		// requested at once (one memory round trip; a list is padded with
+		// empty entries to a multiple of 16, and only entries below the padded
+		// count are read)
+		const unsigned flags = a.st->flags;
+		n = list_count(a.tile_count, tile);
+		const int4 *src = a.tile_geo + (size_t) tile * MAXT;
+		const int4 e = src[min(tid, GCHUNK_LIST - 1)];
+		// all three in hand here: the compiler issues the loads together
+		asm volatile("" :: "v"(e.x), "v"(e.y), "v"(e.z), "v"(e.w), "s"(flags), "s"(n));
+		if(tid < GCHUNK_LIST) s_list[tid] = e;
+		if(flags) return;
+		const int np = min((n + 15) & ~15, MAXT);
+		for(int j = GCHUNK_LIST + tid; j < np; j += blockDim.x) s_list[j] = src[j];
+		__syncthreads();
+	}
 	if(n > MAXT) {
 		if(tid == 0) atomicOr(&a.st->flags, ST_TILE);
 		return;
 	}
-	const int sx = tid % TW, sy = tid / TW;
 	if(sx >= tw || sy >= th) return;
-	double acc = 0.0;
-	int cnt = 0;
-	auto sites = [&](auto entry) {
-		for(int base = 0; base < n; base += GCHUNK_PHI) {
-			int row[GCHUNK_PHI];
-			double v[GCHUNK_PHI];
+	for(int base = 0; base < n; base += GCHUNK_PHI) {
+		int row[GCHUNK_PHI];
 #pragma unroll
-			for(int k = 0; k < GCHUNK_PHI; ++k) row[k] = entry_row(entry(base + k), sx, sy, L);
-#pragma unroll
-			for(int k = 0; k < GCHUNK_PHI; ++k) v[k] = row[k] >= 0 ? a.phi[row[k]] : 0.;
-#pragma unroll
-			for(int k = 0; k < GCHUNK_PHI; ++k)
-				if(row[k] >= 0) {
-					acc += v[k] * v[k];
-					++cnt;
-				}
-		}
-	};
-	if(a.build_lists) sites([&](int j) { return s_list[j]; });
-	else sites([&](int j) { return list_entry(a.tile_geo, tile, j); });
+		for(int k = 0; k < GCHUNK_PHI; ++k) row[k] = entry_row(s_list[base + k], sx, sy, L);
+		add(row);
+	}
 	const int k = B.box_off + (y0 + sy) * L + x0 + sx;
 	a.phi2[k] = acc;
 	a.cover[k] = cnt;
@@ -304,7 +336,8 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 }
 
 hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s) {
-	RC_LAUNCH(k_box_gather, dim3(ntiles), dim3(256), 0, s, a);
+	if(a.nboxes == 1) RC_LAUNCH(k_box_gather<true>, dim3(ntiles), dim3(256), 0, s, a);
+	else RC_LAUNCH(k_box_gather<false>, dim3(ntiles), dim3(256), 0, s, a);
 	return hipGetLastError();
 }
 
