@@ -2800,14 +2800,6 @@ __device__ inline void scg_part_i(double (&part)[6], int m, double inv, double2 
 __device__ inline void scg_part(double (&part)[6], int m, double D, double2 y, double2 w) {
 	scg_part_i(part, m, 1. / D, y, w);
 }
-// a tile's partials (valid in thread 0) -> part[j * ntiles + tile]
-__device__ inline void scg_partials(const double (&v)[6], double *part, int ntiles, int tile) {
-	if(threadIdx.x == 0) {
-#pragma unroll
-		for(int j = 0; j < 6; ++j) part[j * ntiles + tile] = v[j];
-	}
-}
-
 // The sums of one box's partials (6 values per item, items begin .. begin +
 // count - 1 of a [6][stride] array) over NT threads in one fixed order, split
 // so that a launch requests this thread's first partials with its other loads
