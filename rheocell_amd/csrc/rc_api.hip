@@ -728,9 +728,9 @@ static int upload_layout(rc_handle *h, const rc_cell *cells, const double *phi, 
 	h->max_W = maxW;
 	if(std::max(maxS, maxW) >= (1 << 16))
 		return fail(h, RC_ERR_ARG, "a patch of %d sites is too large", maxS);
-	if(h->nematic && nem_lds_bytes(maxS) > NEM_LDS_MAX)
+	if(h->nematic && nem_lds_bytes(maxS, h->d_wmask != nullptr) > NEM_LDS_MAX)
 		return fail(h, RC_ERR_ARG, "a patch of %d sites is too large for the nematic force kernel (at most %d)", maxS,
-				(int) (NEM_LDS_MAX / nem_lds_bytes(1)));
+				(int) (NEM_LDS_MAX / nem_lds_bytes(1, h->d_wmask != nullptr)));
 	{
 		// group the cells by the first stage shape (table order) that fits their patch
 		int shapes[16][2];
@@ -1314,7 +1314,7 @@ static int nem_forces(rc_handle *h, const double *phi) {
 	n.walls = h->d_walls;
 	n.st = h->d_st;
 	TimeScope ts(h, RC_T_NEM_FORCE);
-	HIPCHK(h, launch_nem_force(n, h->ncells, nem_lds_bytes(h->max_S), h->stream));
+	HIPCHK(h, launch_nem_force(n, h->ncells, nem_lds_bytes(h->max_S, h->d_wmask != nullptr), h->stream));
 	return RC_OK;
 }
 

@@ -180,7 +180,8 @@ struct NemArgs {
 	DevStatus *st;
 };
 constexpr size_t NEM_LDS_MAX = 160 * 1024 - 1024;
-inline size_t nem_lds_bytes(int max_sites) { return 32 * (size_t) max_sites; }
+// phi and f - mu phi per site; in a wall box also the two masked differences
+inline size_t nem_lds_bytes(int max_sites, bool walls) { return (walls ? 32 : 16) * (size_t) max_sites; }
 hipError_t launch_nem_force(const NemArgs &a, int ncells, size_t lds_bytes, hipStream_t s);
 hipError_t launch_cg_diag(const CGArgs &a, int nbox_sites, hipStream_t s);
 hipError_t launch_cg_gather(int mode, const CGArgs &a, int ntiles, hipStream_t s);

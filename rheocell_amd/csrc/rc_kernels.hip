@@ -252,7 +252,7 @@ __device__ int tile_list_build(const GatherArgs &a, int tile, const DevBox &B, i
 }
 
 constexpr int GCHUNK = 8;   // covering patches examined per batch of independent loads (CG gathers)
-constexpr int GCHUNK_PHI = 16;   // the same for the sum phi^2 gather (one batch at typical overlaps)
+constexpr int GCHUNK_PHI = 24;   // the same for the sum phi^2 gather (one batch at typical overlaps; entries past the count are masked)
 constexpr int GCHUNK_LIST = 32;  // list entries a reusing gather requests before it knows the count
 
 // ---------------------------------------------------------------------------
@@ -324,7 +324,10 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 	for(int base = 0; base < n; base += GCHUNK_PHI) {
 		int row[GCHUNK_PHI];
 #pragma unroll
-		for(int k = 0; k < GCHUNK_PHI; ++k) row[k] = entry_row(s_list[base + k], sx, sy, L);
+		for(int k = 0; k < GCHUNK_PHI; ++k) {
+			const int j = base + k;
+			row[k] = j < n ? entry_row(s_list[min(j, MAXT - 1)], sx, sy, L) : -1;
+		}
 		add(row);
 	}
 	const int k = B.box_off + (y0 + sy) * L + x0 + sx;
@@ -1837,6 +1840,10 @@ __global__ __launch_bounds__(256) void k_nem_force(NemArgs a) {
 	const DevParams P = a.params[2 * C.box + a.set];
 	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L;
 	const int cx = C.corner % L, cy = C.corner / L;
+	// LDS: phi and f - mu phi of the patch; in a wall box also the masked
+	// central differences (elsewhere they are formed again from phi where the
+	// force needs them: half the LDS, twice the workgroups per CU)
+	const bool walls = a.wmask != nullptr;
 	double *s_phi = s_nem, *s_s = s_nem + S, *s_dx = s_nem + 2 * S, *s_dy = s_nem + 3 * S;
 	for(int q = tid; q < S; q += 256) s_phi[q] = a.phi[C.start + q];
 	__syncthreads();
@@ -1914,8 +1921,10 @@ __global__ __launch_bounds__(256) void k_nem_force(NemArgs a) {
 			double fd = CH_d + A_d + Rep_d;
 			if(a.wmask) fd = f_density_ext(P, ph, w[u]) + fd;
 			s_s[q] = fd - mu * ph;
-			s_dx[q] = dx;
-			s_dy[q] = dy;
+			if(walls) {
+				s_dx[q] = dx;
+				s_dy[q] = dy;
+			}
 		}
 	}
 	__syncthreads();
@@ -1940,13 +1949,39 @@ __global__ __launch_bounds__(256) void k_nem_force(NemArgs a) {
 			const int q = q0 + u * 256;
 			if(q >= S) break;
 			const int qy = (int) __umulhi((unsigned) q, mlx), qx = q - qy * LX;
-			const int qE = qx + 1 == LX ? q - qx : q + 1, qW = qx == 0 ? q + LX - 1 : q - 1;
-			const int qN = qy + 1 == LY ? qx : q + LX, qS = qy == 0 ? q + (LY - 1) * LX : q - LX;
-			const double f_passive_x = -0.5 * ((s_s[qE] + s_dx[qE] * (gl * s_dx[qE])) - (s_s[qW] + s_dx[qW] * (gl * s_dx[qW])))
-					- 0.5 * ((s_dx[qN] * (gl * s_dy[qN])) - (s_dx[qS] * (gl * s_dy[qS])));
-			const double f_passive_y = -0.5 * ((s_s[qN] + s_dy[qN] * (gl * s_dy[qN])) - (s_s[qS] + s_dy[qS] * (gl * s_dy[qS])))
-					- 0.5 * ((s_dy[qE] * (gl * s_dx[qE])) - (s_dy[qW] * (gl * s_dx[qW])));
-			const double dx = s_dx[q], dy = s_dy[q];
+			const int xE = qx + 1 == LX ? 0 : qx + 1, xW = qx == 0 ? LX - 1 : qx - 1;
+			const int yN = qy + 1 == LY ? 0 : qy + 1, yS = qy == 0 ? LY - 1 : qy - 1;
+			const int qE = qy * LX + xE, qW = qy * LX + xW, qN = yN * LX + qx, qS = yS * LX + qx;
+			// the central differences at q and its four neighbours: from LDS in a
+			// wall box, else .5 (phi_E - phi_W), .5 (phi_N - phi_S) as in phase 1
+			double dxq, dyq, dxE, dyE, dxW, dyW, dxN, dyN, dxS, dyS;
+			if(walls) {
+				dxq = s_dx[q]; dyq = s_dy[q];
+				dxE = s_dx[qE]; dyE = s_dy[qE];
+				dxW = s_dx[qW]; dyW = s_dy[qW];
+				dxN = s_dx[qN]; dyN = s_dy[qN];
+				dxS = s_dx[qS]; dyS = s_dy[qS];
+			}
+			else {
+				const int xEE = xE + 1 == LX ? 0 : xE + 1, xWW = xW == 0 ? LX - 1 : xW - 1;
+				const int yNN = yN + 1 == LY ? 0 : yN + 1, ySS = yS == 0 ? LY - 1 : yS - 1;
+				auto d = [&](int x, int y) { return s_phi[y * LX + x]; };
+				dxq = .5 * (d(xE, qy) - d(xW, qy));
+				dyq = .5 * (d(qx, yN) - d(qx, yS));
+				dxE = .5 * (d(xEE, qy) - d(qx, qy));
+				dyE = .5 * (d(xE, yN) - d(xE, yS));
+				dxW = .5 * (d(qx, qy) - d(xWW, qy));
+				dyW = .5 * (d(xW, yN) - d(xW, yS));
+				dxN = .5 * (d(xE, yN) - d(xW, yN));
+				dyN = .5 * (d(qx, yNN) - d(qx, qy));
+				dxS = .5 * (d(xE, yS) - d(xW, yS));
+				dyS = .5 * (d(qx, qy) - d(qx, ySS));
+			}
+			const double f_passive_x = -0.5 * ((s_s[qE] + dxE * (gl * dxE)) - (s_s[qW] + dxW * (gl * dxW)))
+					- 0.5 * ((dxN * (gl * dyN)) - (dxS * (gl * dyS)));
+			const double f_passive_y = -0.5 * ((s_s[qN] + dyN * (gl * dyN)) - (s_s[qS] + dyS * (gl * dyS)))
+					- 0.5 * ((dyE * (gl * dxE)) - (dyW * (gl * dxW)));
+			const double dx = dxq, dy = dyq;
 			const double fQ_self_x = -(C.Q00 * dx + C.Q01 * dy);
 			const double fQ_self_y = -(C.Q01 * dx - C.Q00 * dy);
 			const double fQ_inter_x = -(0.5 * (e00[u] - w00[u]) + 0.5 * (n01[u] - s01[u])) - fQ_self_x;
