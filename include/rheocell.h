@@ -236,7 +236,7 @@ int rc_set_cg_form(rc_handle *h, int form);
 #define RC_T_CG_GATHER_P_FIRST 15  /* p = z fused with the box gather of p */
 #define RC_T_CG_REDUCE 16          /* one-workgroup reduction of CG partial sums */
 #define RC_T_CG_LEARNED 17         /* the learned refinement stages of one solve (group) */
-#define RC_T_NEM_SUMS 19           /* k_box_field: nematic box sums of phi Q00, phi Q01 */
+#define RC_T_NEM_SUMS 19           /* (unused: the nematic box sums of phi Q00, phi Q01 come from k_box_gather) */
 #define RC_T_NEM_FORCE 18          /* k_nem_force: the nematic model's force evaluation */
 #define RC_T_SCG_GATHER 20         /* site form: box sums of b and of the start x0 */
 #define RC_T_SCG_INIT 21           /* site form: per-site set-up (site means, first residual's site part) */

@@ -884,6 +884,10 @@ static int run_gather(rc_handle *h, const double *phi) {
 	g.one_L = h->dboxes[0].L;
 	g.one_ntx = h->dboxes[0].ntx;
 	g.one_cells = h->dboxes[0].cell_end;
+	g.nem = h->nematic ? 1 : 0;
+	g.cells = h->d_cells;
+	g.sq00 = h->d_sq00;
+	g.sq01 = h->d_sq01;
 	g.st = h->d_st;
 	TimeScope ts(h, RC_T_BOX_GATHER);
 	HIPCHK(h, launch_gather(g, h->ntiles, h->stream));
@@ -1279,26 +1283,10 @@ static int cg_solve(rc_handle *h, bool cold) {
 }
 
 // WetModel::begin_energy_computation(fields) (WetModel.cpp:194-233) up to the
-// solve: the box sums of phi Q (the gather of phi^2 has run), then the forces
-// of every cell (k_nem_force): the CG right-hand side, or the dry velocity
+// solve: the box sums of phi Q come from the gather of phi^2 (run_gather, in
+// the same pass), then the forces of every cell (k_nem_force): the CG
+// right-hand side, or the dry velocity
 static int nem_forces(rc_handle *h, const double *phi) {
-	FieldArgs f{};
-	f.boxes = h->d_boxes;
-	f.tile_box = h->d_tile_box;
-	f.tile_geo = h->d_tile_geo;
-	f.tile_count = h->d_tile_count;
-	f.tile_cells = h->d_tile_cells;
-	f.cells = h->d_cells;
-	f.phi = phi;
-	f.which = 2;
-	f.tile0 = 0;
-	f.global = 1;
-	f.fx = h->d_sq00;
-	f.fy = h->d_sq01;
-	{
-		TimeScope ts(h, RC_T_NEM_SUMS);
-		HIPCHK(h, launch_box_field(f, h->ntiles, h->stream));
-	}
 	NemArgs n{};
 	n.boxes = h->d_boxes;
 	n.cells = h->d_cells;

@@ -60,6 +60,9 @@ struct GatherArgs {
 	int build_lists;             // 1: rebuild the lists from geo, 0: reuse them
 	int nboxes;                  // 1: box 0's geometry from the fields below, no table load
 	int one_L, one_ntx, one_cells;
+	int nem;                     // also the nematic box sums of phi Q00, phi Q01 (Q of cells[])
+	const DevCell *cells;
+	double *sq00, *sq01;
 	DevStatus *st;
 };
 

@@ -200,7 +200,7 @@ __device__ inline int4 list_entry(const int4 *tile_list, int tile, int j) {
 // of each chunk of 1024 (their geometry requested at once); a ballot prefix
 // over the wave and the wave totals place the hits in cell order.
 __device__ int tile_list_build(const GatherArgs &a, int tile, const DevBox &B, int x0, int y0, int tw, int th,
-		int4 *s_list, int *s_wcnt) {
+		int4 *s_list, int *s_wcnt, int *s_cells = nullptr) {
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, L = B.L;
 	const unsigned long long lt = (1ull << lane) - 1ull;
 	int n = 0;
@@ -235,6 +235,7 @@ __device__ int tile_list_build(const GatherArgs &a, int tile, const DevBox &B, i
 				if(pos < MAXT) {
 					const int4 e = tile_entry(g[u], L, x0, y0);
 					s_list[pos] = e;
+					if(s_cells) s_cells[pos] = base + 4 * tid + u;
 					a.tile_cells[tile * MAXT + pos] = base + 4 * tid + u;
 					a.tile_geo[tile * MAXT + pos] = e;
 				}
@@ -271,19 +272,24 @@ __device__ inline DevBox gather_box(const GatherArgs &a, int tile) {
 	}
 	return a.boxes[a.tile_box[tile]];
 }
-template <bool ONE>
+// NEM: the nematic model's box sums of phi Q00 and phi Q01 in the same pass
+// (WetModel::computeGlobalSums, WetModel.cpp:423-431: phi Q added in cell
+// order, as k_box_field's which = 2), from each listed cell's Q staged in LDS.
+template <bool ONE, bool NEM>
 __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 	__shared__ int4 s_list[MAXT];
 	__shared__ int s_wcnt[4];
+	__shared__ int s_ci[NEM ? MAXT : 1];
+	__shared__ double2 s_q[NEM ? MAXT : 1];
 	const int tile = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
 	const DevBox B = gather_box<ONE>(a, tile);
 	const int t = tile - B.tile_off, L = B.L;
 	const int x0 = (t % B.ntx) * TW, y0 = (t / B.ntx) * TH;
 	const int tw = min(TW, L - x0), th = min(TH, L - y0);
 	const int sx = tid % TW, sy = tid / TW;
-	double acc = 0.0;
+	double acc = 0.0, a00 = 0.0, a01 = 0.0;
 	int cnt = 0;
-	auto add = [&](const int (&row)[GCHUNK_PHI]) {
+	auto add = [&](const int (&row)[GCHUNK_PHI], int base) {
 		double v[GCHUNK_PHI];
 #pragma unroll
 		for(int k = 0; k < GCHUNK_PHI; ++k) v[k] = row[k] >= 0 ? a.phi[row[k]] : 0.;
@@ -292,12 +298,17 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 			if(row[k] >= 0) {
 				acc += v[k] * v[k];
 				++cnt;
+				if(NEM) {
+					const double2 q = s_q[base + k];
+					a00 += q.x * v[k];
+					a01 += q.y * v[k];
+				}
 			}
 	};
 	int n;
 	if(a.build_lists) {
 		if(a.st->flags) return;
-		n = tile_list_build(a, tile, B, x0, y0, tw, th, s_list, s_wcnt);
+		n = tile_list_build(a, tile, B, x0, y0, tw, th, s_list, s_wcnt, NEM ? s_ci : nullptr);
 	}
 	else {
 		// the status word, the count and the list's first GCHUNK_LIST entries
@@ -307,18 +318,33 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 		const unsigned flags = a.st->flags;
 		n = list_count(a.tile_count, tile);
 		const int4 *src = a.tile_geo + (size_t) tile * MAXT;
+		const int *srcc = a.tile_cells + (size_t) tile * MAXT;
 		const int4 e = src[min(tid, GCHUNK_LIST - 1)];
-		// all three in hand here: the compiler issues the loads together
-		asm volatile("" :: "v"(e.x), "v"(e.y), "v"(e.z), "v"(e.w), "s"(flags), "s"(n));
-		if(tid < GCHUNK_LIST) s_list[tid] = e;
+		const int ec = NEM ? srcc[min(tid, GCHUNK_LIST - 1)] : 0;
+		// all in hand here: the compiler issues the loads together
+		asm volatile("" :: "v"(e.x), "v"(e.y), "v"(e.z), "v"(e.w), "v"(ec), "s"(flags), "s"(n));
+		if(tid < GCHUNK_LIST) {
+			s_list[tid] = e;
+			if(NEM) s_ci[tid] = ec;
+		}
 		if(flags) return;
 		const int np = min((n + 15) & ~15, MAXT);
-		for(int j = GCHUNK_LIST + tid; j < np; j += blockDim.x) s_list[j] = src[j];
+		for(int j = GCHUNK_LIST + tid; j < np; j += blockDim.x) {
+			s_list[j] = src[j];
+			if(NEM) s_ci[j] = srcc[j];
+		}
 		__syncthreads();
 	}
 	if(n > MAXT) {
 		if(tid == 0) atomicOr(&a.st->flags, ST_TILE);
 		return;
+	}
+	if(NEM) {
+		for(int j = tid; j < n; j += blockDim.x) {
+			const DevCell &C = a.cells[s_ci[j]];
+			s_q[j] = make_double2(C.Q00, C.Q01);
+		}
+		__syncthreads();
 	}
 	if(sx >= tw || sy >= th) return;
 	for(int base = 0; base < n; base += GCHUNK_PHI) {
@@ -328,19 +354,29 @@ __global__ __launch_bounds__(256) void k_box_gather(GatherArgs a) {
 			const int j = base + k;
 			row[k] = j < n ? entry_row(s_list[min(j, MAXT - 1)], sx, sy, L) : -1;
 		}
-		add(row);
+		add(row, base);
 	}
 	const int k = B.box_off + (y0 + sy) * L + x0 + sx;
 	a.phi2[k] = acc;
 	a.cover[k] = cnt;
+	if(NEM) {
+		a.sq00[k] = a00;
+		a.sq01[k] = a01;
+	}
 	if(cnt > STORE_MAX) {
 		if(atomicOr(&a.st->flags, ST_OVERLAP) == 0u) a.st->site = k;
 	}
 }
 
 hipError_t launch_gather(const GatherArgs &a, int ntiles, hipStream_t s) {
-	if(a.nboxes == 1) RC_LAUNCH(k_box_gather<true>, dim3(ntiles), dim3(256), 0, s, a);
-	else RC_LAUNCH(k_box_gather<false>, dim3(ntiles), dim3(256), 0, s, a);
+	if(a.nboxes == 1) {
+		if(a.nem) RC_LAUNCH((k_box_gather<true, true>), dim3(ntiles), dim3(256), 0, s, a);
+		else RC_LAUNCH((k_box_gather<true, false>), dim3(ntiles), dim3(256), 0, s, a);
+	}
+	else {
+		if(a.nem) RC_LAUNCH((k_box_gather<false, true>), dim3(ntiles), dim3(256), 0, s, a);
+		else RC_LAUNCH((k_box_gather<false, false>), dim3(ntiles), dim3(256), 0, s, a);
+	}
 	return hipGetLastError();
 }
 
