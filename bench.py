@@ -55,7 +55,7 @@ BYTES = {
     "cg_box_pap": (0, 56),          # read box sums, P2, diag; write N4
     "cg_row_update": (80, 24),      # read x, p, r; write x, r; read N4 and diag
     "cg_reduce": (0, 0),
-    "nem_sums": (8, 16),            # read phi; write the box sums of phi Q00, phi Q01
+    "nem_sums": (8, 16),            # (a separate sums pass; the nematic gather now writes them)
     "nem_force": (24, 24),          # read phi; write f (2 doubles); read sum phi^2, sum phi Q00, sum phi Q01
     # site form of the CG (both components, double2 fields)
     "scg_gather": (32, 32),         # read b, x0 rows; write their box sums
@@ -410,6 +410,8 @@ def main():
             if not d:
                 continue
             per_row, per_site = BYTES[name]
+            if name == "box_gather" and str(keys.get("interaction_type", "wetpolarmodel")) == "wetmodel":
+                per_site += 16   # the nematic gather also writes sum phi Q00, sum phi Q01
             b = per_row * rows + per_site * box_sites
             us = 1e3 * d["ms_total"] / d["launches"]
             field_update[_lib.T_SYMBOL.get(name, name) + " " + name] = {
