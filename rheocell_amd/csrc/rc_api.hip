@@ -99,6 +99,7 @@ struct rc_handle {
 	int cg_ntiles = 0;
 	int scg_parity = 0;           // h_scg[2 b + scg_parity]: each box's state after the last solve
 	hipEvent_t cg_e0 = nullptr, cg_e1 = nullptr;
+	bool cg_ms_pending = false;   // the last solve's time is still to be read from cg_e0 / cg_e1
 	DevStatus *h_st = nullptr;    // pinned
 
 	// state machine
@@ -1172,8 +1173,22 @@ static int cg_row_iterations(rc_handle *h, CGArgs c, bool cold) {
 	return RC_OK;
 }
 
+// The time of the last solve, when its end event was recorded without a wait
+// (the site form needs no host read after its last batch): read before the
+// events are recorded again and when the statistics are asked for.
+static void fold_cg_ms(rc_handle *h) {
+	if(!h->cg_ms_pending) return;
+	h->cg_ms_pending = false;
+	if(hipEventSynchronize(h->cg_e1) != hipSuccess) return;
+	float ms = 0.f;
+	(void) hipEventElapsedTime(&ms, h->cg_e0, h->cg_e1);
+	h->cgs.last_solve_ms = ms;
+	h->cgs.total_solve_ms += ms;
+}
+
 // Eigen ConjugateGradient solve (solve() when cold, solveWithGuess() otherwise)
 static int cg_solve(rc_handle *h, bool cold) {
+	fold_cg_ms(h);
 	CGArgs c = cg_args(h);
 	const bool sites = h->cg_form == RC_CG_SITES;
 	hipEvent_t e0 = h->cg_e0, e1 = h->cg_e1;
@@ -1238,9 +1253,16 @@ static int cg_solve(rc_handle *h, bool cold) {
 		}
 	}
 	HIPCHK(h, hipEventRecord(e1, h->stream));
-	HIPCHK(h, hipEventSynchronize(e1));
+	// the row form, the learned start and the training records read device
+	// state below; the site form's state is already on the host, so its end
+	// is not waited for here (the stream runs on into the next stage)
+	const bool wait = !sites || h->pending_learned_stat || h->rec_open;
 	float ms = 0.f;
-	(void) hipEventElapsedTime(&ms, e0, e1);
+	if(wait) {
+		HIPCHK(h, hipEventSynchronize(e1));
+		(void) hipEventElapsedTime(&ms, e0, e1);
+	}
+	else h->cg_ms_pending = true;
 	if(h->pending_learned_stat) {
 		if(!h->h_cg->reverted[0] && !h->h_cg->reverted[1]) h->cgs.learned_used += 1;
 		h->pending_learned_stat = false;
@@ -1275,8 +1297,10 @@ static int cg_solve(rc_handle *h, bool cold) {
 	h->cgs.last_iters[0] = last_it[0];
 	h->cgs.last_iters[1] = last_it[1];
 	h->cgs.last_cold = cold ? 1 : 0;
-	h->cgs.last_solve_ms = ms;
-	h->cgs.total_solve_ms += ms;
+	if(wait) {
+		h->cgs.last_solve_ms = ms;
+		h->cgs.total_solve_ms += ms;
+	}
 	h->last_iters[cold ? 1 : 0] = std::max(last_it[0], last_it[1]);
 	if(capped) h->cgs.last_cold |= 2;
 	return RC_OK;
@@ -1653,6 +1677,7 @@ static int serve_growth(rc_handle *h) {
 // Repulsion.  Runs before the first step after a layout change.
 static int autotune_stage(rc_handle *h) {
 	h->stage_tuned = true;
+	fold_cg_ms(h);   // its timing reuses the solve's events
 	int rc;
 	if((rc = run_gather(h, h->d_phiA))) return rc;
 	const bool timing = h->timing;
@@ -2125,6 +2150,7 @@ extern "C" int rc_total_rows(rc_handle *h, long long *rows) {
 
 extern "C" int rc_cg_stats(rc_handle *h, rc_cg_stats_t *out) {
 	if(!h || !out) return RC_ERR_ARG;
+	fold_cg_ms(h);
 	*out = h->cgs;
 	return RC_OK;
 }
