@@ -1252,6 +1252,13 @@ static int cg_solve(rc_handle *h, bool cold) {
 			if(rc) return rc;
 		}
 	}
+	// a halted device ran no iteration: no statistics or training records
+	// for this solve
+	if(h->h_st->flags) {
+		h->pending_learned_stat = false;
+		h->rec_open = 0;
+		return RC_OK;
+	}
 	HIPCHK(h, hipEventRecord(e1, h->stream));
 	// the row form, the learned start and the training records read device
 	// state below; the site form's state is already on the host, so its end
@@ -1358,6 +1365,8 @@ static int force_eval(rc_handle *h, double *phi, bool pure) {
 		h->size_rows.assign(nb, 0);
 		h->size_rows_old.assign(nb, 0);
 	}
+	const std::vector<int> saved_restart = h->restart_solver;
+	const std::vector<long long> saved_rows = h->size_rows, saved_rows_old = h->size_rows_old;
 	h->cold_box.assign(nb, 0);
 	bool any_cold = false;
 	for(int b = 0; b < nb; ++b) {
@@ -1376,6 +1385,14 @@ static int force_eval(rc_handle *h, double *phi, bool pure) {
 			else h->restart_solver[b] += 1;
 		}
 		if((rc = cg_solve(h, any_cold))) return rc;
+		// a halted device (a growth request of the corrector before this
+		// evaluation, or an error) ran no solve: the counters stay as they
+		// were, for the evaluation made again after the growth
+		if(h->h_st->flags) {
+			h->restart_solver = saved_restart;
+			h->size_rows = saved_rows;
+			h->size_rows_old = saved_rows_old;
+		}
 	}
 	h->cf_set = h->cur_set;
 	return RC_OK;
@@ -1485,11 +1502,11 @@ static int step_split(rc_handle *h, const double *gauss_dev, int step) {
 	a.phi_out = h->d_phiA;
 	a.gauss = nullptr;
 	if((rc = run_stage(h, MODE_FS, a))) return rc;
-	// a growth request of this corrector must be served before the force
-	// evaluation that follows it
-	HIPCHK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, h->stream));
-	HIPCHK(h, hipStreamSynchronize(h->stream));
-	if(h->h_st->flags) return -1;
+	// A growth request of this corrector must be served before the force
+	// evaluation that follows it.  That evaluation is enqueued without a wait:
+	// on a request its launches return at entry, the solve's first host read
+	// sees the request and the evaluation leaves no trace (force_eval), and
+	// the step ends here for the host to serve the growth and evaluate again.
 	if((rc = force_eval(h, h->d_phiA, false))) return rc;
 	if(h->h_st->flags) return -1;
 	return RC_OK;
