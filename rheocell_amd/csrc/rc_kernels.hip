@@ -3308,6 +3308,46 @@ __device__ inline SCGCoef scg_coef(const State &S, const double (&tot)[6], bool 
 	return k;
 }
 
+// One component's scalars of scg_coef (component c, chosen per lane): the
+// same expressions, so that one wave forms both components' at once.
+struct SCGCoef1 {
+	double rn2, gamma, alpha, beta, pi;
+	int done;
+};
+__device__ inline SCGCoef1 scg_coef1(const SCGVar &S, const double (&tot)[6], bool first, int c) {
+	const bool x = c == 0;
+	const double rho = x ? S.rho[0] : S.rho[1], rperp2 = x ? S.rperp2[0] : S.rperp2[1];
+	const double rzperp = x ? S.rzperp[0] : S.rzperp[1], thr = x ? S.thr[0] : S.thr[1];
+	const double g_prev = x ? S.gamma[0] : S.gamma[1], a_prev = x ? S.alpha[0] : S.alpha[1];
+	const double pi_prev = x ? S.pi[0] : S.pi[1];
+	const int iters = x ? S.iters[0] : S.iters[1];
+	SCGCoef1 k;
+	k.done = x ? S.done[0] : S.done[1];
+	k.rn2 = x ? S.rn2[0] : S.rn2[1];
+	k.gamma = g_prev;
+	k.alpha = 0.;
+	k.beta = 0.;
+	k.pi = pi_prev;
+	if(k.done) return k;
+	const double r2 = rho * rho;
+	const double rn2 = r2 * rperp2 + (x ? tot[0] : tot[1]);
+	k.rn2 = rn2;
+	if(rn2 < thr || iters >= S.max_iter) {
+		k.done = 1;
+		return k;
+	}
+	const double gamma = r2 * rzperp + (x ? tot[2] : tot[3]);
+	const double delta = r2 * rzperp + (x ? tot[4] : tot[5]);
+	k.gamma = gamma;
+	if(first) k.alpha = gamma / delta;
+	else {
+		k.beta = gamma / g_prev;
+		k.alpha = gamma / (delta - k.beta * gamma / a_prev);
+	}
+	k.pi = rho + k.beta * pi_prev;
+	return k;
+}
+
 // The iteration's tile with a two-site ring: slots at tile coordinates (hx, hy)
 // in -2 .. SCG_TX + 1 x -2 .. SCG_TY + 1, grid index (hy + 2) SCG_G2W + hx + 2.
 // Thread t owns slot (t mod SCG_TX, t / SCG_TX) and threads 0 .. SCG_RING2 - 1
@@ -3416,6 +3456,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	__shared__ double2 s_mz[SCG_G2W * SCG_G2H];
 	__shared__ double s_part[6 * SCG_THREADS];
 	__shared__ double s_tab[2 * SCG_NTAB];
+	__shared__ SCGCoef1 s_cf[2];
 	const SCGIterTile G = scg_iter_tile<ONE>(a);
 	const SCGTileBox &T = G.t;
 	const SCGState *Sp = a.s2 + 2 * T.box + pin;
@@ -3454,9 +3495,23 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	double tot[6];
 	scg_sums_add(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre, tot, s_red);
 	// (the table is in LDS: scg_sums_add ends with a barrier)
+	// the scalars once, lanes 0 and 1 of wave 0 (one component each), for
+	// every thread through LDS
+	if(threadIdx.x < 2) s_cf[threadIdx.x] = scg_coef1(V, tot, FIRST, threadIdx.x);
 	scg_diag(c0, s_tab);
 	if(t.hg >= 0) scg_diag(ch, s_tab);
-	const SCGCoef cf = scg_coef(V, tot, FIRST);
+	__syncthreads();
+	SCGCoef cf;
+#pragma unroll
+	for(int c = 0; c < 2; ++c) {
+		const SCGCoef1 k = s_cf[c];
+		cf.done[c] = k.done != 0;
+		cf.rn2[c] = k.rn2;
+		cf.gamma[c] = k.gamma;
+		cf.alpha[c] = k.alpha;
+		cf.beta[c] = k.beta;
+		cf.pi[c] = k.pi;
+	}
 	if(cf.done[0] && cf.done[1]) {
 		if(writer) {
 #pragma unroll
