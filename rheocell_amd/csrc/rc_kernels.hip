@@ -3537,6 +3537,19 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	};
 	const double2 q0 = pdir(c0);
 	s_mq[t.g] = scg_mz(q0, c0.m);
+	// q and s are final here: their stores go out now, overlapping the rest
+	// of the pass (every workgroup stores at once; the write-out, not the
+	// issue, is what the end of the launch waits for)
+	if(s_pass) {
+		if(V.pend[0]) s.x += V.alpha[0] * c0.q.x;
+		if(V.pend[1]) s.y += V.alpha[1] * c0.q.y;
+		if(!d0) s.x += al0 * q0.x;
+		if(!d1) s.y += al1 * q0.y;
+	}
+	if(t.in) {
+		nt_store2(a.q[pout] + t.k, q0);
+		if(s_pass) nt_store2(a.s + t.k, s);
+	}
 	double2 qh = make_double2(0., 0.);
 	if(t.hg >= 0) {
 		qh = pdir(ch);
@@ -3557,16 +3570,8 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	if(t.hin) s_mz[t.hg] = scg_mz(scg_zi(rstep(ch, qh, t.hg), ch.inv, ch.m), ch.m);
 	__syncthreads();
 	const double2 wn = scg_apply(s_mz, t.g, c0.D, z0, c0.m, eta);
-	if(s_pass) {
-		if(V.pend[0]) s.x += V.alpha[0] * c0.q.x;
-		if(V.pend[1]) s.y += V.alpha[1] * c0.q.y;
-		if(!d0) s.x += al0 * q0.x;
-		if(!d1) s.y += al1 * q0.y;
-	}
 	double part[6] = { 0., 0., 0., 0., 0., 0. };
 	if(t.in) {
-		if(s_pass) nt_store2(a.s + t.k, s);
-		nt_store2(a.q[pout] + t.k, q0);
 		nt_store2(a.y[pout] + t.k, yn0);
 		scg_part_i(part, c0.m, c0.inv, yn0, wn);
 	}
