@@ -76,7 +76,7 @@ CG_SWEEP_KINDS = ("cg_box_pap", "cg_row_update", "scg_iter")
 # way, i.e. reads x 2.0, writes x 1.0.  Recorded in profiles/.
 READ_SCALE, WRITE_SCALE = 262144.0 / 131085.0, 1.0
 TRAFFIC_KIB = {
-    ("wet512_1600", "scg_iter"): (6038.0, 10252.2),
+    ("wet512_1600", "scg_iter"): (6040.0, 10252.2),
     ("wet512_1600", "cg_row_update"): (68309.3, 84801.2),
     ("wet512_1600", "cg_gather_p"): (44435.5, 52247.6),
     ("dry512_1024", "stage_fused_pred"): (8607.8, 27347.2),
