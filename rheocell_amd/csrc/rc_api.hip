@@ -999,6 +999,8 @@ static SCGArgs scg_args(rc_handle *h, const CGArgs &c) {
 	a.one_L = h->cgb[0].L;
 	a.one_ntx = h->cgb[0].ntx;
 	a.one_eta = h->dparams[2 * 0 + h->cur_set].friction_cell;
+	a.max_box_tiles = 0;
+	for(const SCGBox &g : h->cgb) a.max_box_tiles = std::max(a.max_box_tiles, g.ntiles);
 	a.ncells = h->ncells;
 	a.ntiles = h->cg_ntiles;
 	a.nboxes = h->nboxes;

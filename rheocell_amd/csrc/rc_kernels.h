@@ -250,6 +250,7 @@ struct SCGArgs {
 	const SCGTileGeo *geo;       // [ntiles]
 	int one_L, one_ntx;          // box 0's side and tiles per row (read when nboxes == 1)
 	double one_eta;              // box 0's friction_cell in the current parameter set
+	int max_box_tiles;           // tiles of the largest box (the iteration's partial-sum preload)
 	int ncells, ntiles, nboxes;
 	DevStatus *st;
 };

@@ -2904,7 +2904,10 @@ __device__ inline void parts_sum(const double *part, int stride, int begin, int 
 // items lane, lane + 64, ... in order, then the lanes (wave_sum); the other
 // waves idle.  One value per wave keeps the cross-lane work to six wave sums
 // per workgroup.
-constexpr int SCG_PU = 8;   // items per lane requested with a launch's first loads
+// items per lane requested with a launch's first loads: 8 covers the 512
+// tiles of a box up to 1024 x 512, 16 the 1024 of a 1024^2 box (the host picks
+// by the largest box; the order of the additions is the same either way)
+constexpr int SCG_PU = 8, SCG_PU_BIG = 16;
 static_assert(SCG_THREADS >= 6 * 64, "one wave per value");
 // a tile's partials from each thread's six values (s_part: [6][SCG_THREADS]
 // doubles of LDS) -> part[j * ntiles + tile]
@@ -2922,24 +2925,30 @@ __device__ inline void scg_tile_partials(const double (&v)[6], double *s_part, d
 	}
 }
 // a box's partials: items begin .. begin + count - 1 of a [6][stride] array.
-// The first SCG_PU items of the lane are requested by scg_sums_load (all
-// addresses valid: waves 6 and 7 and items past the end read a valid item
-// that is not added), the rest by scg_sums_add, which gives every thread the
-// six sums (s_red: 6 doubles of LDS).
-__device__ inline void scg_sums_load(const double *part, int stride, int begin, int count, double (&v)[SCG_PU]) {
-	const int w = min((int) threadIdx.x >> 6, 5), lane = threadIdx.x & 63;
+// The first PU items of the lane are requested by scg_sums_load (waves 0-5
+// only; items past the end read a valid item that is not added), the rest by
+// scg_sums_add, which gives every thread the six sums (s_red: 6 doubles of
+// LDS).
+template <int PU>
+__device__ inline void scg_sums_load(const double *part, int stride, int begin, int count, double (&v)[PU]) {
+	const int w = (int) threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-	for(int i = 0; i < SCG_PU; ++i) v[i] = part[w * stride + begin + min(lane + 64 * i, count - 1)];
+	for(int i = 0; i < PU; ++i) v[i] = 0.;
+	if(w < 6) {
+#pragma unroll
+		for(int i = 0; i < PU; ++i) v[i] = part[w * stride + begin + min(lane + 64 * i, count - 1)];
+	}
 }
-__device__ inline void scg_sums_add(const double *part, int stride, int begin, int count, const double (&v)[SCG_PU],
+template <int PU>
+__device__ inline void scg_sums_add(const double *part, int stride, int begin, int count, const double (&v)[PU],
 		double (&tot)[6], double *s_red) {
 	const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 	if(w < 6) {
 		double acc = 0.;
 #pragma unroll
-		for(int i = 0; i < SCG_PU; ++i)
+		for(int i = 0; i < PU; ++i)
 			if(lane + 64 * i < count) acc += v[i];
-		for(int it = lane + 64 * SCG_PU; it < count; it += 64) acc += part[w * stride + begin + it];
+		for(int it = lane + 64 * PU; it < count; it += 64) acc += part[w * stride + begin + it];
 		acc = wave_sum(acc);
 		if(lane == 0) s_red[w] = acc;
 	}
@@ -3448,7 +3457,7 @@ __device__ inline void nt_store2(double2 *p, double2 v) {
 	const rc_dbl2 w = { v.x, v.y };
 	__builtin_nontemporal_store(w, reinterpret_cast<rc_dbl2 *>(p));
 }
-template <bool FIRST, bool ONE>
+template <bool FIRST, bool ONE, int PU>
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	const int pin = j & 1, pout = pin ^ 1;
 	__shared__ double s_red[6];
@@ -3479,8 +3488,8 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	const bool s_pass = (j & 1) != 0;
 	double2 s = make_double2(0., 0.);
 	if(s_pass) s = a.s[t.k];
-	double pre[SCG_PU];
-	scg_sums_load(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre);
+	double pre[PU];
+	scg_sums_load<PU>(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre);
 	const unsigned flags = a.st->flags;
 	SCGVar V = scg_var(Sp);
 	// an ordering point: the compiler keeps the loads above it, where they
@@ -3493,7 +3502,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	}
 	if(threadIdx.x < 2 * SCG_NTAB) s_tab[threadIdx.x] = tab;
 	double tot[6];
-	scg_sums_add(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre, tot, s_red);
+	scg_sums_add<PU>(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre, tot, s_red);
 	// (the table is in LDS: scg_sums_add ends with a barrier)
 	// the scalars once, lanes 0 and 1 of wave 0 (one component each), for
 	// every thread through LDS
@@ -3612,8 +3621,8 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_check(SCGArgs a, int j) {
 	double tot[6] = { 0., 0., 0., 0., 0., 0. };
 	if(active && j > 0) {
 		double pre[SCG_PU];
-		scg_sums_load(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre);
-		scg_sums_add(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre, tot, s_red);
+		scg_sums_load<SCG_PU>(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre);
+		scg_sums_add<SCG_PU>(a.psum[j & 1], a.ntiles, Bx.tile_off, Bx.ntiles, pre, tot, s_red);
 	}
 	if(threadIdx.x != 0) return;
 	if(!active || j == 0) {
@@ -3672,13 +3681,23 @@ hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s) {
 	case SCG_INIT_W: RC_LAUNCH(k_scg_init_w, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
 	case SCG_ROW_INIT: RC_LAUNCH(k_scg_row_init, dim3(a.ncells), dim3(256), 0, s, a); break;
 	case SCG_ITER:
-		if(a.nboxes == 1) {
-			if(j == 0) RC_LAUNCH((k_scg_iter<true, true>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
-			else RC_LAUNCH((k_scg_iter<false, true>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+		if(a.max_box_tiles > 64 * SCG_PU) {
+			if(a.nboxes == 1) {
+				if(j == 0) RC_LAUNCH((k_scg_iter<true, true, SCG_PU_BIG>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+				else RC_LAUNCH((k_scg_iter<false, true, SCG_PU_BIG>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+			}
+			else {
+				if(j == 0) RC_LAUNCH((k_scg_iter<true, false, SCG_PU_BIG>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+				else RC_LAUNCH((k_scg_iter<false, false, SCG_PU_BIG>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+			}
+		}
+		else if(a.nboxes == 1) {
+			if(j == 0) RC_LAUNCH((k_scg_iter<true, true, SCG_PU>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+			else RC_LAUNCH((k_scg_iter<false, true, SCG_PU>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
 		}
 		else {
-			if(j == 0) RC_LAUNCH((k_scg_iter<true, false>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
-			else RC_LAUNCH((k_scg_iter<false, false>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+			if(j == 0) RC_LAUNCH((k_scg_iter<true, false, SCG_PU>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+			else RC_LAUNCH((k_scg_iter<false, false, SCG_PU>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
 		}
 		break;
 	case SCG_CHECK: RC_LAUNCH(k_scg_check, dim3(a.nboxes), dim3(SCG_THREADS), 0, s, a, j); break;
