@@ -15,6 +15,9 @@ HIPCC_FLAGS = [
     # IEEE evaluation of every expression (no FMA contraction): parity with the reference source
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # leading scalar and pointer kernel arguments arrive preloaded in SGPRs
+    # (the site-form iteration puts the pointers of its first loads there)
+    "-mllvm", "-amdgpu-kernarg-preload-count=16",
     "-I" + os.path.join(ROOT, "include"),
 ]
 

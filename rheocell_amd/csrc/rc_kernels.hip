@@ -2989,19 +2989,19 @@ struct SCGIterTile {
 	double eta;
 };
 template <bool ONE>
-__device__ inline SCGIterTile scg_iter_tile(const SCGArgs &a) {
+__device__ inline SCGIterTile scg_iter_tile(const SCGArgs &a, int one_L, int one_ntx, int ntiles) {
 	SCGIterTile g;
-	const int tile = xcd_item(blockIdx.x, gridDim.x);
+	const int tile = xcd_item(blockIdx.x, ntiles);   // the grid is one workgroup per tile
 	g.t.tile = tile;
 	if(ONE) {
 		g.t.box = 0;
 		g.t.o = 0;
-		g.t.L = a.one_L;
-		g.t.x0 = (tile % a.one_ntx) * SCG_TX;
-		g.t.y0 = (tile / a.one_ntx) * SCG_TY;
+		g.t.L = one_L;
+		g.t.x0 = (tile % one_ntx) * SCG_TX;
+		g.t.y0 = (tile / one_ntx) * SCG_TY;
 		g.t.lead = tile == 0;
 		g.tile_off = 0;
-		g.ntiles = a.ntiles;
+		g.ntiles = ntiles;
 		g.eta = a.one_eta;
 	}
 	else {
@@ -3411,13 +3411,13 @@ struct SCGSite {
 	double D, inv;
 	int m, n;
 };
-__device__ inline SCGSite scg_load(const SCGArgs &a, int pin, int k, bool first) {
+__device__ inline SCGSite scg_load(const int *mnp, const double2 *yin, const double2 *qin, int k, bool first) {
 	SCGSite v;
-	const int mn = a.mn[k];
+	const int mn = mnp[k];
 	v.m = mn & 255;
 	v.n = min(mn >> 8, SCG_NTAB - 1);
-	v.y = a.y[pin][k];
-	v.q = first ? make_double2(0., 0.) : a.q[pin][k];
+	v.y = yin[k];
+	v.q = first ? make_double2(0., 0.) : qin[k];
 	return v;
 }
 __device__ inline void scg_diag(SCGSite &v, const double *s_tab) {
@@ -3457,18 +3457,24 @@ __device__ inline void nt_store2(double2 *p, double2 v) {
 	const rc_dbl2 w = { v.x, v.y };
 	__builtin_nontemporal_store(w, reinterpret_cast<rc_dbl2 *>(p));
 }
+// The pointers and sizes the first loads need lead the arguments, so that
+// they arrive preloaded in SGPRs (kernel-argument preloading) and the loads
+// go out at the first instructions, not after a read of the argument block.
 template <bool FIRST, bool ONE, int PU>
-__global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
-	const int pin = j & 1, pout = pin ^ 1;
+__global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(const double2 *yin, const double2 *qin, const int *mnp,
+		double2 *sp, const double *psin, const SCGState *Sp0, int lntx, int ntp, SCGArgs a, int j) {
+	// lntx = box 0's side | its tiles per row << 16; ntp = tiles | (j & 1) << 31
+	const int one_L = lntx & 0xFFFF, one_ntx = lntx >> 16;
+	const int ntiles = ntp & 0x7FFFFFFF, pin = (int) ((unsigned) ntp >> 31), pout = pin ^ 1;
 	__shared__ double s_red[6];
 	__shared__ double2 s_mq[SCG_G2W * SCG_G2H];
 	__shared__ double2 s_mz[SCG_G2W * SCG_G2H];
 	__shared__ double s_part[6 * SCG_THREADS];
 	__shared__ double s_tab[2 * SCG_NTAB];
 	__shared__ SCGCoef1 s_cf[2];
-	const SCGIterTile G = scg_iter_tile<ONE>(a);
+	const SCGIterTile G = scg_iter_tile<ONE>(a, one_L, one_ntx, ntiles);
 	const SCGTileBox &T = G.t;
-	const SCGState *Sp = a.s2 + 2 * T.box + pin;
+	const SCGState *Sp = Sp0 + 2 * T.box;
 	SCGState *Sn = a.s2 + 2 * T.box + pout;
 	const bool writer = T.lead && threadIdx.x == 0;
 	const double eta = G.eta;
@@ -3476,22 +3482,24 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	// Every load is requested before any is used: the site data, this lane's
 	// partials, the status word and the state's scalars (all addresses are
 	// valid whatever they hold), so the launch waits for one memory round
-	// trip, and a launch after convergence returns after that one.
-	const int tab_i = min((int) threadIdx.x, 2 * SCG_NTAB - 1);
-	const double tab = a.dtab[(2 * T.box + a.set) * 2 * SCG_NTAB + tab_i];
-	SCGSite c0 = scg_load(a, pin, t.k, FIRST);
+	// trip, and a launch after convergence returns after that one.  The
+	// state's scalars and the site data come first (their pointers arrive
+	// preloaded), the loads behind a read of the argument block last.
+	SCGVar V = scg_var(Sp);
+	SCGSite c0 = scg_load(mnp, yin, qin, t.k, FIRST);
 	SCGSite ch{};
-	if(t.hg >= 0) ch = scg_load(a, pin, t.hk, FIRST);
+	if(t.hg >= 0) ch = scg_load(mnp, yin, qin, t.hk, FIRST);
 	// s moves at odd iterations only, by the previous iteration's pending
 	// step and its own: (s + alpha_prev q_prev) + alpha q, the order of a
 	// step per iteration, with q_prev the q this iteration reads anyway
-	const bool s_pass = (j & 1) != 0;
+	const bool s_pass = pin != 0;
 	double2 s = make_double2(0., 0.);
-	if(s_pass) s = a.s[t.k];
+	if(s_pass) s = sp[t.k];
 	double pre[PU];
-	scg_sums_load<PU>(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre);
+	scg_sums_load<PU>(psin, ntiles, G.tile_off, G.ntiles, pre);
+	const int tab_i = min((int) threadIdx.x, 2 * SCG_NTAB - 1);
+	const double tab = a.dtab[(2 * T.box + a.set) * 2 * SCG_NTAB + tab_i];
 	const unsigned flags = a.st->flags;
-	SCGVar V = scg_var(Sp);
 	// an ordering point: the compiler keeps the loads above it, where they
 	// were requested, instead of moving them under the branches below
 	asm volatile("" ::: "memory");
@@ -3502,7 +3510,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	}
 	if(threadIdx.x < 2 * SCG_NTAB) s_tab[threadIdx.x] = tab;
 	double tot[6];
-	scg_sums_add<PU>(a.psum[pin], a.ntiles, G.tile_off, G.ntiles, pre, tot, s_red);
+	scg_sums_add<PU>(psin, ntiles, G.tile_off, G.ntiles, pre, tot, s_red);
 	// (the table is in LDS: scg_sums_add ends with a barrier)
 	// the scalars once, lanes 0 and 1 of wave 0 (one component each), for
 	// every thread through LDS
@@ -3557,7 +3565,7 @@ __global__ __launch_bounds__(SCG_THREADS) void k_scg_iter(SCGArgs a, int j) {
 	}
 	if(t.in) {
 		nt_store2(a.q[pout] + t.k, q0);
-		if(s_pass) nt_store2(a.s + t.k, s);
+		if(s_pass) nt_store2(sp + t.k, s);
 	}
 	double2 qh = make_double2(0., 0.);
 	if(t.hg >= 0) {
@@ -3680,26 +3688,25 @@ hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s) {
 	case SCG_INIT: RC_LAUNCH(k_scg_init, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
 	case SCG_INIT_W: RC_LAUNCH(k_scg_init_w, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
 	case SCG_ROW_INIT: RC_LAUNCH(k_scg_row_init, dim3(a.ncells), dim3(256), 0, s, a); break;
-	case SCG_ITER:
-		if(a.max_box_tiles > 64 * SCG_PU) {
-			if(a.nboxes == 1) {
-				if(j == 0) RC_LAUNCH((k_scg_iter<true, true, SCG_PU_BIG>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
-				else RC_LAUNCH((k_scg_iter<false, true, SCG_PU_BIG>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
-			}
-			else {
-				if(j == 0) RC_LAUNCH((k_scg_iter<true, false, SCG_PU_BIG>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
-				else RC_LAUNCH((k_scg_iter<false, false, SCG_PU_BIG>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
-			}
-		}
-		else if(a.nboxes == 1) {
-			if(j == 0) RC_LAUNCH((k_scg_iter<true, true, SCG_PU>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
-			else RC_LAUNCH((k_scg_iter<false, true, SCG_PU>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+	case SCG_ITER: {
+		const double2 *yin = a.y[j & 1], *qin = a.q[j & 1];
+		const double *psin = a.psum[j & 1];
+		const int lntx = a.one_L | a.one_ntx << 16, ntp = (int) ((unsigned) a.ntiles | (unsigned) (j & 1) << 31);
+		const SCGState *Sp0 = a.s2 + (j & 1);
+#define RC_SCG_ITER(F, O, U) RC_LAUNCH((k_scg_iter<F, O, U>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, yin, qin, a.mn, a.s, psin, \
+		Sp0, lntx, ntp, a, j)
+		const bool big = a.max_box_tiles > 64 * SCG_PU, one = a.nboxes == 1, first = j == 0;
+		if(big) {
+			if(one) { if(first) RC_SCG_ITER(true, true, SCG_PU_BIG); else RC_SCG_ITER(false, true, SCG_PU_BIG); }
+			else { if(first) RC_SCG_ITER(true, false, SCG_PU_BIG); else RC_SCG_ITER(false, false, SCG_PU_BIG); }
 		}
 		else {
-			if(j == 0) RC_LAUNCH((k_scg_iter<true, false, SCG_PU>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
-			else RC_LAUNCH((k_scg_iter<false, false, SCG_PU>), dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a, j);
+			if(one) { if(first) RC_SCG_ITER(true, true, SCG_PU); else RC_SCG_ITER(false, true, SCG_PU); }
+			else { if(first) RC_SCG_ITER(true, false, SCG_PU); else RC_SCG_ITER(false, false, SCG_PU); }
 		}
+#undef RC_SCG_ITER
 		break;
+	}
 	case SCG_CHECK: RC_LAUNCH(k_scg_check, dim3(a.nboxes), dim3(SCG_THREADS), 0, s, a, j); break;
 	case SCG_FINAL: RC_LAUNCH(k_scg_final, dim3(a.ncells), dim3(256), 0, s, a, j); break;
 	default: return hipErrorInvalidValue;
