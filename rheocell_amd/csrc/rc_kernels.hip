@@ -641,31 +641,23 @@ __device__ inline double v_ext(const DevParams &P, int y, int L, int index) {
 // a.walls): masked stencils, the forces' free energy in mu, no force on wall
 // sites and the forces' advection in the update.  LDS layout: stage_lds_bytes
 // (rc_kernels.h).
+// The cell records, the workgroup's cell list, the status word and the rows
+// and box sums of the first loads lead the arguments (preloaded in SGPRs).
 template <int MODE, int NPT, bool RCP, int CT, bool ST, bool WL = false>
-__global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
+__global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(const DevCell *cells_in, const int *cell_list,
+		const DevStatus *st_in, const double *phi_c_in, const double *phi2_in, StageArgs a) {
 	constexpr bool store = ST;
-	// Any status flag halts the device, except that a growth request raised
-	// by a corrector of this same step leaves the rest of that corrector stage
-	// to run (cells of other stage groups are launched after it); the host
-	// serves the growth once the stage is complete.  Correctors of later steps,
-	// already queued, halt.
-	const int step = a.step + (a.step_base ? *a.step_base : 0);
-	{
-		const unsigned f = a.st->flags;
-		if(f) {
-			const bool same_step = MODE != MODE_CF && !store && (f & ST_KIND_MASK) == ST_GROWTH &&
-					(f >> ST_STEP_SHIFT) == (unsigned) (step + 1);
-			if(!same_step) return;
-		}
-	}
+	// (the status word is requested here and tested once the first loads
+	// are under way, before anything is written)
+	const unsigned st_flags = st_in->flags;
 	constexpr int NW = CT / 64;
 	extern __shared__ double2 smem2[];
 	__shared__ double s_red[NW * 4 * 8];
 	__shared__ int s_redi[2 * NW];
 	__shared__ double s_bc;
 	const int slot = xcd_item(blockIdx.x, gridDim.x);
-	const int ci = a.cell_list ? a.cell_list[slot] : slot, tid = threadIdx.x;
-	const DevCell C = a.cells[ci];
+	const int ci = cell_list ? cell_list[slot] : slot, tid = threadIdx.x;
+	const DevCell C = cells_in[ci];
 	const DevBox B = a.boxes[C.box];
 	const DevParams P = a.params[2 * C.box + a.cf_set];
 	const int LX = C.LX, LY = C.LY, L = B.L, PW = LX + 2, WX = LX + 2, WY = LY + 2;
@@ -696,7 +688,7 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 
 	// the strip's column of phi with one row above and below (periodic), read
 	// straight from the rows; pold / dold (corrector) and v (MODE_FS) likewise
-	const double *phic = a.phi_c + C.start + qx;
+	const double *phic = phi_c_in + C.start + qx;
 	double col[NPT + 2];
 #pragma unroll
 	for(int k = 0; k < NPT + 2; ++k) {
@@ -711,7 +703,7 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 	// the box sums of phi^2 at the strip's rows, requested with the strip (rows
 	// past the strip read valid box sites and are discarded)
 	const int bcol = B.box_off + (cx + lx >= L ? cx + lx - L : cx + lx);
-	const double *phi2b = a.phi2 + bcol;
+	const double *phi2b = phi2_in + bcol;
 	double p2g[NPT];
 	unsigned wm[NPT];    // WL: wall mask bits and wall field of the strip's sites
 	double wv[NPT];
@@ -731,6 +723,18 @@ __global__ __launch_bounds__(CT, NPT <= 7 ? 4 : 1) void k_stage(StageArgs a) {
 			wm[k] = a.wmask[bcol + by * L];
 			wv[k] = a.walls[bcol + by * L];
 		}
+	}
+	// Any status flag halts the device, except that a growth request raised
+	// by a corrector of this same step leaves the rest of that corrector stage
+	// to run (cells of other stage groups are launched after it); the host
+	// serves the growth once the stage is complete.  Correctors of later steps,
+	// already queued, halt.  (The loads above stay above: an ordering point.)
+	asm volatile("" ::: "memory");
+	const int step = a.step + (a.step_base ? *a.step_base : 0);
+	if(st_flags) {
+		const bool same_step = MODE != MODE_CF && !store && (st_flags & ST_KIND_MASK) == ST_GROWTH &&
+				(st_flags >> ST_STEP_SHIFT) == (unsigned) (step + 1);
+		if(!same_step) return;
 	}
 	// row tables, the box window (adhesion only) and the phi tile with its
 	// periodic columns
@@ -1249,15 +1253,15 @@ template <int MODE, bool RCP, bool ST, bool WL>
 static void launch_mode_r(const StageArgs &a, int ncells, size_t lds, hipStream_t s) {
 	const dim3 g(ncells);
 	if(a.threads == 256) {
-		if(a.npt == 4) RC_LAUNCH((k_stage<MODE, 4, RCP, 256, ST, WL>), g, dim3(256), lds, s, a);
-		else RC_LAUNCH((k_stage<MODE, 7, RCP, 256, ST, WL>), g, dim3(256), lds, s, a);
+		if(a.npt == 4) RC_LAUNCH((k_stage<MODE, 4, RCP, 256, ST, WL>), g, dim3(256), lds, s, a.cells, a.cell_list, a.st, a.phi_c, a.phi2, a);
+		else RC_LAUNCH((k_stage<MODE, 7, RCP, 256, ST, WL>), g, dim3(256), lds, s, a.cells, a.cell_list, a.st, a.phi_c, a.phi2, a);
 		return;
 	}
 	switch(a.npt) {
-	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP, 512, ST, WL>), g, dim3(512), lds, s, a); break;
-	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP, 512, ST, WL>), g, dim3(512), lds, s, a); break;
-	case 8: RC_LAUNCH((k_stage<MODE, 8, RCP, 512, ST, WL>), g, dim3(512), lds, s, a); break;
-	default: RC_LAUNCH((k_stage<MODE, 16, RCP, 512, ST, WL>), g, dim3(512), lds, s, a); break;
+	case 4: RC_LAUNCH((k_stage<MODE, 4, RCP, 512, ST, WL>), g, dim3(512), lds, s, a.cells, a.cell_list, a.st, a.phi_c, a.phi2, a); break;
+	case 7: RC_LAUNCH((k_stage<MODE, 7, RCP, 512, ST, WL>), g, dim3(512), lds, s, a.cells, a.cell_list, a.st, a.phi_c, a.phi2, a); break;
+	case 8: RC_LAUNCH((k_stage<MODE, 8, RCP, 512, ST, WL>), g, dim3(512), lds, s, a.cells, a.cell_list, a.st, a.phi_c, a.phi2, a); break;
+	default: RC_LAUNCH((k_stage<MODE, 16, RCP, 512, ST, WL>), g, dim3(512), lds, s, a.cells, a.cell_list, a.st, a.phi_c, a.phi2, a); break;
 	}
 }
 template <int MODE, bool ST>
