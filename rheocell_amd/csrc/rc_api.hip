@@ -95,7 +95,6 @@ struct rc_handle {
 	int *d_mn = nullptr;          // [sites] rows here and at the four neighbours (site form)
 	double *d_dtab = nullptr;     // [nboxes][2][2][SCG_NTAB] D and 1 / D by neighbour rows
 	std::vector<SCGBox> cgb;      // host copy (its iteration caps follow the layout)
-	int *d_cg_tile_box = nullptr; // [cg_ntiles]
 	int cg_ntiles = 0;
 	int scg_parity = 0;           // h_scg[2 b + scg_parity]: each box's state after the last solve
 	hipEvent_t cg_e0 = nullptr, cg_e1 = nullptr;
@@ -488,7 +487,7 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		// the site-form tiles of every box (box-major) and each box's solver parameters
 		std::vector<SCGBox> &cgb = h->cgb;
 		cgb.assign(nboxes, SCGBox{});
-		std::vector<int> cg_tile_box;
+		int cg_tiles = 0;
 		std::vector<SCGTileGeo> cg_geo;
 		for(int b = 0; b < nboxes; ++b) {
 			const DevBox &B = h->dboxes[b];
@@ -496,14 +495,14 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 			g.L = B.L;
 			g.box_off = B.box_off;
 			g.ntx = (B.L + SCG_TX - 1) / SCG_TX;
-			g.tile_off = (int) cg_tile_box.size();
+			g.tile_off = cg_tiles;
 			g.ntiles = g.ntx * ((B.L + SCG_TY - 1) / SCG_TY);
 			g.cell_begin = B.cell_begin;
 			g.cell_end = B.cell_end;
 			const rc_model_params &mp = boxes[b].params;
 			g.tol = mp.CGtolerance > 0 ? mp.CGtolerance : 2.220446049250313e-16;
 			g.max_iter = mp.cg_max_iter > 0 ? mp.cg_max_iter : 1 << 30;
-			cg_tile_box.insert(cg_tile_box.end(), g.ntiles, b);
+			cg_tiles += g.ntiles;
 			for(int lt = 0; lt < g.ntiles; ++lt) {
 				SCGTileGeo G{};
 				G.box = b;
@@ -538,7 +537,7 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, dalloc(&h->d_dtab, dtab.size()));
 		HIPCHK(h, hipMemcpy(h->d_dtab, dtab.data(), sizeof(double) * dtab.size(), hipMemcpyHostToDevice));
 		HIPCHK(h, dalloc(&h->d_mn, sites));
-		h->cg_ntiles = (int) cg_tile_box.size();
+		h->cg_ntiles = cg_tiles;
 		for(double2 **f : { &h->d_sc, &h->d_ss, &h->d_sy[0], &h->d_sy[1], &h->d_sq[0], &h->d_sq[1] })
 			HIPCHK(h, dalloc(f, sites));
 		for(int i = 0; i < 2; ++i) HIPCHK(h, dalloc(&h->d_psum[i], 6 * (size_t) h->cg_ntiles));
@@ -547,8 +546,6 @@ extern "C" int rc_create(const rc_box_desc *boxes, int nboxes, rc_handle **out) 
 		HIPCHK(h, hipHostMalloc((void **) &h->h_scg, 2 * nboxes * sizeof(SCGState)));
 		HIPCHK(h, dalloc(&h->d_cgb, nboxes));
 		HIPCHK(h, hipMemcpy(h->d_cgb, cgb.data(), sizeof(SCGBox) * nboxes, hipMemcpyHostToDevice));
-		HIPCHK(h, dalloc(&h->d_cg_tile_box, h->cg_ntiles));
-		HIPCHK(h, hipMemcpy(h->d_cg_tile_box, cg_tile_box.data(), sizeof(int) * h->cg_ntiles, hipMemcpyHostToDevice));
 		HIPCHK(h, dalloc(&h->d_cg_geo, h->cg_ntiles));
 		HIPCHK(h, hipMemcpy(h->d_cg_geo, cg_geo.data(), sizeof(SCGTileGeo) * h->cg_ntiles, hipMemcpyHostToDevice));
 	}
@@ -570,7 +567,7 @@ extern "C" void rc_destroy(rc_handle *h) {
 	dfree(h->d_ggauss); dfree(h->d_step_base); dfree(h->d_bf);
 	if(h->h_step_base) (void) hipHostFree(h->h_step_base); dfree(h->d_xrec); dfree(h->d_rnrec);
 	if(h->h_cg) (void) hipHostFree(h->h_cg);
-	dfree(h->d_sc); dfree(h->d_ss); dfree(h->d_scg); dfree(h->d_cgb); dfree(h->d_cg_tile_box); dfree(h->d_cg_geo);
+	dfree(h->d_sc); dfree(h->d_ss); dfree(h->d_scg); dfree(h->d_cgb); dfree(h->d_cg_geo);
 	dfree(h->d_mn); dfree(h->d_dtab);
 	for(int i = 0; i < 2; ++i) {
 		dfree(h->d_sy[i]); dfree(h->d_sq[i]); dfree(h->d_psum[i]);
@@ -994,7 +991,6 @@ static SCGArgs scg_args(rc_handle *h, const CGArgs &c) {
 	a.part = h->d_part;
 	a.s2 = h->d_scg;
 	a.cgb = h->d_cgb;
-	a.tile_box = h->d_cg_tile_box;
 	a.geo = h->d_cg_geo;
 	a.one_L = h->cgb[0].L;
 	a.one_ntx = h->cgb[0].ntx;

@@ -206,8 +206,8 @@ hipError_t launch_zero_rows(double2 *x, long long rows, int comp_mask, hipStream
 // of b and x0 in sb and sx (k_cg_gather<CG_GATHER_BX>), D in diag, rows per
 // site in cover; the site fields are indexed like the box arrays (box_off +
 // site).  Site passes run one workgroup per SCG_TX x SCG_TY tile of a box
-// (ntiles over all boxes, box-major; tile_box[tile] = its box), one site per
-// thread.  Each box solves on its own: its own scalars (s2[2 box + parity]) and
+// (ntiles over all boxes, box-major; geo[tile] = its box and corner), one site
+// per thread.  Each box solves on its own: its own scalars (s2[2 box + parity]) and
 // partial sums (its tiles' range of psum).  Iteration j reads the fields of
 // parity j & 1 and writes those of the other parity (its neighbours' old
 // values stay readable), and the partial sums likewise.
@@ -246,7 +246,6 @@ struct SCGArgs {
 	double *psum[2];             // site partials y.m y, y.m (y / D), (y / D).m w: [6][ntiles] each
 	SCGState *s2;                // [2 * nboxes]
 	const SCGBox *cgb;           // [nboxes]
-	const int *tile_box;         // [ntiles]
 	const SCGTileGeo *geo;       // [ntiles]
 	int one_L, one_ntx;          // box 0's side and tiles per row (read when nboxes == 1)
 	double one_eta;              // box 0's friction_cell in the current parameter set

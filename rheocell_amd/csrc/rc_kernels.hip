@@ -2971,22 +2971,8 @@ struct SCGTileBox {
 	int tile, box, o, L, x0, y0;
 	bool lead;    // the box's first tile (its workgroup writes the box's state)
 };
-__device__ inline SCGTileBox scg_tile_box(const SCGArgs &a) {
-	SCGTileBox t;
-	t.tile = xcd_item(blockIdx.x, gridDim.x);
-	t.box = a.tile_box[t.tile];
-	const SCGBox Bx = a.cgb[t.box];
-	const int lt = t.tile - Bx.tile_off;
-	t.o = Bx.box_off;
-	t.L = Bx.L;
-	t.x0 = (lt % Bx.ntx) * SCG_TX;
-	t.y0 = (lt / Bx.ntx) * SCG_TY;
-	t.lead = lt == 0;
-	return t;
-}
-// the same for the iteration, with the box's tile range and viscosity: from
-// the launch arguments with one box (ONE), else one load of the tile's
-// SCGTileGeo
+// The tile with its box's tile range and viscosity: from the launch arguments
+// with one box (ONE), else one load of the tile's SCGTileGeo
 struct SCGIterTile {
 	SCGTileBox t;
 	int tile_off, ntiles;
@@ -3040,9 +3026,10 @@ __device__ inline int scg_slot_site(const SCGTileBox &T, int hx, int hy) {
 // site means b_bar = P b / m, x_bar = P x0 / m; c = b_bar - D x_bar;
 // y = (P r0) / m = c + eta N4(P x0), the site part of the first residual
 // r0 = b - A x0; s = 0.
+template <bool ONE>
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_init(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
-	const SCGTileBox T = scg_tile_box(a);
+	const SCGTileBox T = scg_iter_tile<ONE>(a, a.one_L, a.one_ntx, a.ntiles).t;
 	const int lx = threadIdx.x % SCG_TX, ly = threadIdx.x / SCG_TX;
 	if(T.x0 + lx >= T.L || T.y0 + ly >= T.L) return;
 	const int kl = (T.y0 + ly) * T.L + T.x0 + lx, k = T.o + kl;
@@ -3127,12 +3114,13 @@ __device__ inline double2 scg_mz(double2 z, int m) {
 // partials (k_scg_row_init) and writes Eigen's start (ConjugateGradient.h:
 // |b|^2 and the threshold max(tol^2 |b|^2, DBL_MIN)) into SCGState [2 box],
 // with rho = 1, pi = xi = 0; the first stop test is iteration 0's.
+template <bool ONE>
 __global__ __launch_bounds__(SCG_THREADS) void k_scg_init_w(SCGArgs a) {
 	const unsigned flags = a.st->flags;   // tested before the first store, not ahead of the loads
 	__shared__ double s_red[(SCG_THREADS / 64) * 6];
 	__shared__ double2 s_mz[SCG_HW * (SCG_TY + 2)];
 	__shared__ double s_part[6 * SCG_THREADS];
-	const SCGTileBox T = scg_tile_box(a);
+	const SCGTileBox T = scg_iter_tile<ONE>(a, a.one_L, a.one_ntx, a.ntiles).t;
 	if(T.lead) {
 		const SCGBox Bx = a.cgb[T.box];
 		double t6[6], v[6];
@@ -3689,8 +3677,14 @@ __global__ __launch_bounds__(256) void k_scg_final(SCGArgs a, int j) {
 
 hipError_t launch_scg(int which, const SCGArgs &a, int j, hipStream_t s) {
 	switch(which) {
-	case SCG_INIT: RC_LAUNCH(k_scg_init, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
-	case SCG_INIT_W: RC_LAUNCH(k_scg_init_w, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a); break;
+	case SCG_INIT:
+		if(a.nboxes == 1) RC_LAUNCH(k_scg_init<true>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a);
+		else RC_LAUNCH(k_scg_init<false>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a);
+		break;
+	case SCG_INIT_W:
+		if(a.nboxes == 1) RC_LAUNCH(k_scg_init_w<true>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a);
+		else RC_LAUNCH(k_scg_init_w<false>, dim3(a.ntiles), dim3(SCG_THREADS), 0, s, a);
+		break;
 	case SCG_ROW_INIT: RC_LAUNCH(k_scg_row_init, dim3(a.ncells), dim3(256), 0, s, a); break;
 	case SCG_ITER: {
 		const double2 *yin = a.y[j & 1], *qin = a.q[j & 1];
