@@ -3429,17 +3429,6 @@ __device__ inline double2 scg_apply(const double2 *s_mv, int g, double D, double
 	return make_double2(D * v.x - eta * (((mS.x + mW.x) + mE.x) + mN.x), D * v.y - eta * (((mS.y + mW.y) + mE.y) + mN.y));
 }
 
-// Iteration j (one launch, one workgroup per tile of every box): the box's
-// scalars (every workgroup of the box, the same fixed order of its tiles'
-// partials), then
-//   q = z + beta q (p),  A p from q over the one-site ring,  s += alpha q,
-//   y -= alpha A p,  z = y / D,  w = A z (for the sums only),
-// q over the tile and its two-site ring, the new y over the one-site ring,
-// each exchanged through LDS as m q and m z.  Only y, q (both parities) and s
-// are stored: A p and w are formed from them, so an iteration reads and
-// writes 108 B per box site.  The workgroup of the box's first tile writes
-// SCGState [2 box + ((j + 1) & 1)]: rho -= alpha pi, xi += alpha pi and the
-// recurrence scalars (nobody reads that copy during iteration j).
 // A store marked non-temporal (streaming): the fields an iteration writes are
 // read by the next launch only, which may run on another XCD, so keeping them
 // in this XCD's L2 only adds to the write-back at the end of the launch; the
@@ -3449,6 +3438,20 @@ __device__ inline void nt_store2(double2 *p, double2 v) {
 	const rc_dbl2 w = { v.x, v.y };
 	__builtin_nontemporal_store(w, reinterpret_cast<rc_dbl2 *>(p));
 }
+
+// Iteration j (one launch, one workgroup per tile of every box): the sums of
+// the box's tiles' partials (every workgroup of the box, the same fixed
+// order), the scalars of the iteration from them (two lanes, shared through
+// LDS), then
+//   q = z + beta q (p),  A p from q over the one-site ring,  s += alpha q,
+//   y -= alpha A p,  z = y / D,  w = A z (for the sums only),
+// q over the tile and its two-site ring, the new y over the one-site ring,
+// each exchanged through LDS as m q and m z.  Only y, q (both parities) and s
+// are stored: A p and w are formed from them.  s moves at odd iterations, by
+// the even iteration's step and its own, so an iteration reads and writes
+// 68 B per box site (even) or 100 B (odd).  The workgroup of the box's first
+// tile writes SCGState [2 box + ((j + 1) & 1)]: rho -= alpha pi, xi += alpha
+// pi and the recurrence scalars (nobody reads that copy during iteration j).
 // The pointers and sizes the first loads need lead the arguments, so that
 // they arrive preloaded in SGPRs (kernel-argument preloading) and the loads
 // go out at the first instructions, not after a read of the argument block.
