@@ -93,9 +93,10 @@ struct CGState {
 // products split the same way (r_perp . P^T y = 0, P^T a . P^T b = a . m b).
 // The recurrences are Chronopoulos and Gear's arrangement of the same PCG
 // (gamma = r.z and delta = z.Az reduced together, alpha = gamma / (delta -
-// beta gamma / alpha_prev), A p carried as w + beta A p_prev): one reduction,
-// one pass per iteration.  Two copies, read from [j & 1] and written to
-// [(j + 1) & 1] by iteration j.
+// beta gamma / alpha_prev); A p formed from p in the pass): one reduction, one
+// pass per iteration.  Two copies, read from [j & 1] and written to
+// [(j + 1) & 1] by iteration j; the set-up writes both, and an iteration
+// rewrites only the scalars that change.
 // One box of the site-space CG: side, first box site, tiles (SCG_TX x SCG_TY,
 // ntx per tile row, the box's first tile in the handle's tile order), cells,
 // and the solver tolerance and iteration cap of the box's parameters.
