@@ -1871,11 +1871,16 @@ hipError_t launch_stage_p(const StageArgs &a0, int ncells, int grid, hipStream_t
 // one from the box sums of phi Q, and writes f (wet) or f / friction (dry).
 // Expressions keep the reference's operand order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_nem_force(NemArgs a) {
-	if(a.st->flags) return;
+// The cell records, the status word, phi and the cell count lead the
+// arguments (preloaded in SGPRs): the cell's record is requested at the first
+// instruction, and the status word with it (tested before anything is
+// written).
+__global__ __launch_bounds__(256) void k_nem_force(const DevCell *cells_in, const DevStatus *st_in, const double *phi_in,
+		int ncells, NemArgs a) {
+	const unsigned flags = st_in->flags;
 	extern __shared__ double s_nem[];
-	const int ci = xcd_item(blockIdx.x, gridDim.x), tid = threadIdx.x;
-	const DevCell C = a.cells[ci];
+	const int ci = xcd_item(blockIdx.x, ncells), tid = threadIdx.x;
+	const DevCell C = cells_in[ci];
 	const DevBox B = a.boxes[C.box];
 	const DevParams P = a.params[2 * C.box + a.set];
 	const int LX = C.LX, LY = C.LY, S = C.S, L = B.L;
@@ -1885,7 +1890,17 @@ __global__ __launch_bounds__(256) void k_nem_force(NemArgs a) {
 	// force needs them: half the LDS, twice the workgroups per CU)
 	const bool walls = a.wmask != nullptr;
 	double *s_phi = s_nem, *s_s = s_nem + S, *s_dx = s_nem + 2 * S, *s_dy = s_nem + 3 * S;
-	for(int q = tid; q < S; q += 256) s_phi[q] = a.phi[C.start + q];
+	// the patch's phi: every load of the thread requested before the first
+	// LDS write (one memory round trip, not one per row of 256 sites)
+	constexpr int NST = 10;   // 2560 sites in one batch (41^2 = 1681; grown patches up to 49^2 = 2401)
+	double phs[NST];
+#pragma unroll
+	for(int u = 0; u < NST; ++u) phs[u] = phi_in[C.start + min(tid + 256 * u, S - 1)];
+	for(int q = tid + 256 * NST; q < S; q += 256) s_phi[q] = phi_in[C.start + q];
+#pragma unroll
+	for(int u = 0; u < NST; ++u)
+		if(tid + 256 * u < S) s_phi[tid + 256 * u] = phs[u];
+	if(flags) return;
 	__syncthreads();
 	const double *phi2 = a.phi2 + B.box_off;
 	// box site of storage site (qx, qy) and its four box neighbours
@@ -2043,7 +2058,7 @@ hipError_t launch_nem_force(const NemArgs &a, int ncells, size_t lds, hipStream_
 		if(e != hipSuccess) return e;
 		lds_set = lds;
 	}
-	RC_LAUNCH(k_nem_force, dim3(ncells), dim3(256), lds, s, a);
+	RC_LAUNCH(k_nem_force, dim3(ncells), dim3(256), lds, s, a.cells, a.st, a.phi, ncells, a);
 	return hipGetLastError();
 }
 
